@@ -1,0 +1,249 @@
+"""Python objects over the libgfx C ABI, named after their GetFEM counterparts
+(getfem.Mesh, getfem.MeshFem, getfem.MeshIm) plus the assembly Plan."""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+
+
+def _ptr(a):
+    return None if a is None else ctypes.c_void_p(a.ctypes.data)
+
+
+class Context:
+    """One device + one HIP stream (gfx_ctx)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        code = lib().gfx_ctx_create(device, ctypes.byref(h))
+        if code:
+            raise _lib.GfxError(code, "cannot open HIP device %d" % device)
+        self.h = h
+
+    def set_stream(self, stream_handle):
+        check(lib().gfx_ctx_set_stream(self.h, ctypes.c_void_p(stream_handle or 0)), self.h)
+
+    @property
+    def stream(self):
+        return lib().gfx_ctx_stream(self.h)
+
+    def synchronize(self):
+        check(lib().gfx_ctx_synchronize(self.h), self.h)
+
+    def close(self):
+        if self.h:
+            lib().gfx_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Mesh:
+    """getfem::mesh restricted to one geometric transformation."""
+
+    def __init__(self, ctx, geotrans, pts, cvx):
+        self.ctx = ctx
+        self.pts = np.ascontiguousarray(pts, dtype=np.float64)
+        self.cvx = np.ascontiguousarray(cvx, dtype=np.int32)
+        self.geotrans = geotrans
+        h = ctypes.c_void_p()
+        check(lib().gfx_mesh_create(ctx.h, geotrans.encode(), self.pts.shape[0], _ptr(self.pts),
+                                    self.cvx.shape[0], _ptr(self.cvx), ctypes.byref(h)), ctx.h)
+        self.h = h
+
+    @property
+    def nbcvx(self):
+        return self.cvx.shape[0]
+
+    def set_points(self, pts):
+        pts = np.ascontiguousarray(pts, dtype=np.float64)
+        check(lib().gfx_mesh_set_points(self.h, _ptr(pts)), self.ctx.h)
+        self.pts = pts
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().gfx_mesh_destroy(self.h)
+            self.h = None
+
+
+class MeshFem:
+    """getfem::mesh_fem with one Lagrange element and Qdim."""
+
+    def __init__(self, mesh, fem_name, qdim=1, basic_dofs=None, ndof=None):
+        self.mesh = mesh
+        self.fem_name = fem_name
+        self.qdim = qdim
+        h = ctypes.c_void_p()
+        if basic_dofs is None:
+            check(lib().gfx_mesh_fem_create(mesh.h, fem_name.encode(), qdim, ctypes.byref(h)),
+                  mesh.ctx.h)
+        else:
+            bd = np.ascontiguousarray(basic_dofs, dtype=np.int32)
+            check(lib().gfx_mesh_fem_create_from_dofs(mesh.h, fem_name.encode(), qdim, _ptr(bd),
+                                                      int(ndof), ctypes.byref(h)), mesh.ctx.h)
+        self.h = h
+
+    def nbdof(self):
+        return int(lib().gfx_mesh_fem_nb_dof(self.h))
+
+    def nb_dof_per_element(self):
+        return int(lib().gfx_mesh_fem_nb_dof_per_element(self.h))
+
+    def basic_dof_table(self):
+        out = np.zeros((self.mesh.nbcvx, self.nb_dof_per_element()), dtype=np.int32)
+        check(lib().gfx_mesh_fem_basic_dofs(self.h, _ptr(out)), self.mesh.ctx.h)
+        return out
+
+    def basic_dof_nodes(self):
+        out = np.zeros((self.nbdof() // self.qdim, 3))
+        check(lib().gfx_mesh_fem_dof_nodes(self.h, _ptr(out)), self.mesh.ctx.h)
+        return out
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().gfx_mesh_fem_destroy(self.h)
+            self.h = None
+
+
+class MeshIm:
+    """getfem::mesh_im with one integration method."""
+
+    def __init__(self, mesh, im_name):
+        self.mesh = mesh
+        self.im_name = im_name
+        h = ctypes.c_void_p()
+        check(lib().gfx_mesh_im_create(mesh.h, im_name.encode(), ctypes.byref(h)), mesh.ctx.h)
+        self.h = h
+
+    def points(self):
+        n = lib().gfx_mesh_im_nb_points(self.h)
+        p = np.zeros((n, 3))
+        w = np.zeros(n)
+        check(lib().gfx_mesh_im_points(self.h, _ptr(p), _ptr(w)), self.mesh.ctx.h)
+        return p, w
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().gfx_mesh_im_destroy(self.h)
+            self.h = None
+
+
+def _coef(v, n_cv):
+    if np.isscalar(v):
+        return _lib.GfxCoef(_lib.COEF_CONST, float(v), None), None
+    a = np.ascontiguousarray(v, dtype=np.float64)
+    if a.shape != (n_cv,):
+        raise ValueError("per-element coefficient must have one value per element")
+    return _lib.GfxCoef(_lib.COEF_PER_ELEMENT, 0.0, ctypes.c_void_p(a.ctypes.data)), a
+
+
+class Plan:
+    """One compiled order-2 term (gfx_plan): pattern + device CSR values."""
+
+    def __init__(self, mf, mim, term, coefs):
+        self.mf, self.mim, self.term = mf, mim, term
+        self.ctx = mf.mesh.ctx
+        n_cv = mf.mesh.nbcvx
+        arr = (_lib.GfxCoef * len(coefs))()
+        keep = []
+        for i, c in enumerate(coefs):
+            arr[i], a = _coef(c, n_cv)
+            keep.append(a)
+        h = ctypes.c_void_p()
+        check(lib().gfx_plan_create(mf.h, mim.h, term, arr, len(coefs), ctypes.byref(h)), self.ctx.h)
+        self.h = h
+        self.nnz = None
+
+    def set_coef(self, slot, value):
+        c, a = _coef(value, self.mf.mesh.nbcvx)
+        check(lib().gfx_plan_set_coef(self.h, slot, ctypes.byref(c)), self.ctx.h)
+
+    def set_form(self, form=_lib.FORM_FACTORIZED, precision=_lib.PREC_FP64):
+        check(lib().gfx_plan_set_form(self.h, form, precision), self.ctx.h)
+
+    def build_pattern(self):
+        nnz = ctypes.c_int64()
+        check(lib().gfx_plan_build_pattern(self.h, ctypes.byref(nnz)), self.ctx.h)
+        self.nnz = nnz.value
+        return self.nnz
+
+    def assemble(self):
+        check(lib().gfx_plan_assemble(self.h), self.ctx.h)
+
+    def dims(self):
+        n, z = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().gfx_plan_dims(self.h, ctypes.byref(n), ctypes.byref(z)), self.ctx.h)
+        return n.value, z.value
+
+    def csr(self, values=True):
+        n, nnz = self.dims()
+        indptr = np.zeros(n + 1, dtype=np.int64)
+        indices = np.zeros(nnz, dtype=np.int32)
+        vals = np.zeros(nnz) if values else None
+        check(lib().gfx_plan_export_csr(self.h, _ptr(indptr), _ptr(indices), _ptr(vals)), self.ctx.h)
+        return indptr, indices, vals
+
+    def csc(self):
+        n, nnz = self.dims()
+        colptr = np.zeros(n + 1, dtype=np.int64)
+        rows = np.zeros(nnz, dtype=np.uint32)
+        vals = np.zeros(nnz)
+        check(lib().gfx_plan_export_csc(self.h, _ptr(colptr), _ptr(rows), _ptr(vals)), self.ctx.h)
+        return colptr, rows, vals
+
+    def device_csr(self):
+        ip, ix, v = ctypes.c_void_p(), ctypes.c_void_p(), ctypes.c_void_p()
+        check(lib().gfx_plan_device_csr(self.h, ctypes.byref(ip), ctypes.byref(ix), ctypes.byref(v)),
+              self.ctx.h)
+        return ip.value, ix.value, v.value
+
+    def last_timings(self):
+        g, a = ctypes.c_float(), ctypes.c_float()
+        check(lib().gfx_plan_last_timings(self.h, ctypes.byref(g), ctypes.byref(a)), self.ctx.h)
+        return g.value, a.value
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().gfx_plan_destroy(self.h)
+            self.h = None
+
+
+# ---------------------------------------------------------------- host-only
+def integration_rule(im_name):
+    """Points (n,3) and weights (n,) of an integration method."""
+    n = ctypes.c_int()
+    check(lib().gfx_integration_rule(im_name.encode(), ctypes.byref(n), None, None))
+    p = np.zeros((n.value, 3))
+    w = np.zeros(n.value)
+    check(lib().gfx_integration_rule(im_name.encode(), ctypes.byref(n), _ptr(p), _ptr(w)))
+    return p, w
+
+
+def fem_eval(fem_name, x):
+    """Base values (nd,) and reference gradients (nd,3) at x."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nd = ctypes.c_int()
+    check(lib().gfx_fem_eval(fem_name.encode(), _ptr(x), ctypes.byref(nd), None, None))
+    v = np.zeros(nd.value)
+    g = np.zeros((nd.value, 3))
+    check(lib().gfx_fem_eval(fem_name.encode(), _ptr(x), ctypes.byref(nd), _ptr(v), _ptr(g)))
+    return v, g
+
+
+def enumerate_dofs(geotrans, fem_name, qdim, pts, cvx):
+    """First-encounter dof numbering: (basic dof table (n_cv, nd), ndof)."""
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    cvx = np.ascontiguousarray(cvx, dtype=np.int32)
+    _, g = fem_eval(fem_name, np.array([0.1, 0.1, 0.1]))
+    out = np.zeros((cvx.shape[0], g.shape[0]), dtype=np.int32)
+    ndof = ctypes.c_int64()
+    check(lib().gfx_enumerate_dofs(geotrans.encode(), fem_name.encode(), qdim, pts.shape[0],
+                                   _ptr(pts), cvx.shape[0], _ptr(cvx), _ptr(out),
+                                   ctypes.byref(ndof)))
+    return out, ndof.value

@@ -1,0 +1,607 @@
+// C ABI of libgfx: contexts, meshes, mesh_fems, integration methods and
+// order-2 assembly plans.  See include/gfx.h for the contract.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "gfx_internal.h"
+
+using namespace gfx;
+
+namespace {
+
+int fail(gfx_ctx *c, int code, const std::string &msg) {
+  if (c) c->err = msg;
+  return code;
+}
+
+int hipfail(gfx_ctx *c, hipError_t e, const char *what) {
+  std::string m = std::string(what) + ": " + hipGetErrorString(e);
+  return fail(c, e == hipErrorOutOfMemory ? GFX_E_OOM : GFX_E_HIP, m);
+}
+
+#define HIPCHK(ctx, call)                                   \
+  do {                                                      \
+    hipError_t e_ = (call);                                 \
+    if (e_ != hipSuccess) return hipfail((ctx), e_, #call); \
+  } while (0)
+
+template <typename T>
+hipError_t dalloc(T **p, size_t n) {
+  *p = nullptr;
+  if (n == 0) n = 1;
+  return hipMalloc((void **)p, n * sizeof(T));
+}
+
+template <typename T>
+void dfree(T *&p) {
+  if (p) (void)hipFree((void *)p);
+  p = nullptr;
+}
+
+int set_device(gfx_ctx *c) {
+  hipError_t e = hipSetDevice(c->device);
+  if (e != hipSuccess) return hipfail(c, e, "hipSetDevice");
+  return GFX_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *gfx_version(void) { return "libgfx 0.1 (gfx950)"; }
+
+int gfx_integration_rule(const char *im_name, int *npts, double *h_pts, double *h_w) {
+  if (!im_name || !npts) return GFX_E_INVALID;
+  IntegrationRule r;
+  if (!integration_rule(im_name, r)) return GFX_E_UNSUPPORTED;
+  *npts = r.n();
+  if (h_pts) std::copy(r.pts.begin(), r.pts.end(), h_pts);
+  if (h_w) std::copy(r.w.begin(), r.w.end(), h_w);
+  return GFX_OK;
+}
+
+int gfx_fem_eval(const char *fem_name, const double *x, int *nd, double *h_val, double *h_grad) {
+  if (!fem_name || !x) return GFX_E_INVALID;
+  FemDesc f;
+  if (!fem_descriptor(fem_name, f)) return GFX_E_UNSUPPORTED;
+  if (nd) *nd = f.nd;
+  std::vector<double> v(f.nd), g(3 * (size_t)f.nd);
+  fem_eval(f, x, v.data(), g.data());
+  if (h_val) std::copy(v.begin(), v.end(), h_val);
+  if (h_grad) std::copy(g.begin(), g.end(), h_grad);
+  return GFX_OK;
+}
+
+int gfx_enumerate_dofs(const char *geotrans, const char *fem_name, int qdim, int64_t n_pts,
+                       const double *h_pts, int64_t n_cv, const int32_t *h_cvx, int32_t *h_out,
+                       int64_t *ndof) {
+  if (!geotrans || !fem_name || !h_cvx || !h_out || !ndof || qdim < 1) return GFX_E_INVALID;
+  const int gt = geotrans_descriptor(geotrans);
+  FemDesc f;
+  if (gt < 0 || !fem_descriptor(fem_name, f)) return GFX_E_UNSUPPORTED;
+  const int nv = geotrans_nb_vertices(gt);
+  for (int64_t i = 0; i < n_cv * nv; ++i)
+    if (h_cvx[i] < 0 || h_cvx[i] >= n_pts) return GFX_E_INVALID;
+  std::vector<int32_t> dofs;
+  std::string err;
+  const int64_t n = enumerate_dofs(gt, n_pts, h_pts, n_cv, h_cvx, f, qdim, dofs, err);
+  if (n < 0) return GFX_E_UNSUPPORTED;
+  std::copy(dofs.begin(), dofs.end(), h_out);
+  *ndof = n;
+  return GFX_OK;
+}
+
+int gfx_ctx_create(int device, gfx_ctx **out) {
+  if (!out) return GFX_E_INVALID;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) return GFX_E_HIP;
+  gfx_ctx *c = new (std::nothrow) gfx_ctx();
+  if (!c) return GFX_E_OOM;
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return GFX_E_HIP;
+  }
+  c->own_stream = true;
+  *out = c;
+  return GFX_OK;
+}
+
+int gfx_ctx_destroy(gfx_ctx *c) {
+  if (!c) return GFX_E_INVALID;
+  (void)hipSetDevice(c->device);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+  return GFX_OK;
+}
+
+int gfx_ctx_set_stream(gfx_ctx *c, void *s) {
+  if (!c) return GFX_E_INVALID;
+  if (set_device(c)) return GFX_E_HIP;
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  if (s) {
+    c->stream = (hipStream_t)s;
+    c->own_stream = false;
+  } else {
+    if (hipStreamCreate(&c->stream) != hipSuccess) return fail(c, GFX_E_HIP, "hipStreamCreate");
+    c->own_stream = true;
+  }
+  return GFX_OK;
+}
+
+void *gfx_ctx_stream(gfx_ctx *c) { return c ? (void *)c->stream : nullptr; }
+
+int gfx_ctx_synchronize(gfx_ctx *c) {
+  if (!c) return GFX_E_INVALID;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFX_OK;
+}
+
+const char *gfx_ctx_last_error(gfx_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+// ----------------------------------------------------------------- mesh
+int gfx_mesh_create(gfx_ctx *c, const char *geotrans, int64_t n_pts, const double *h_pts,
+                    int64_t n_cv, const int32_t *h_cvx, gfx_mesh **out) {
+  if (!c || !out || !geotrans || n_pts < 0 || n_cv < 0 || (n_pts && !h_pts) || (n_cv && !h_cvx))
+    return fail(c, GFX_E_INVALID, "gfx_mesh_create: invalid argument");
+  *out = nullptr;
+  const int gt = geotrans_descriptor(geotrans);
+  if (gt < 0) return fail(c, GFX_E_UNSUPPORTED, std::string("unsupported geometric transformation ") + geotrans);
+  const int nv = geotrans_nb_vertices(gt);
+  if (n_cv >= (int64_t(1) << 27)) return fail(c, GFX_E_INVALID, "too many elements (limit 2^27)");
+  for (int64_t i = 0; i < n_cv * nv; ++i)
+    if (h_cvx[i] < 0 || h_cvx[i] >= n_pts)
+      return fail(c, GFX_E_INVALID, "convex references a point out of range");
+  if (set_device(c)) return GFX_E_HIP;
+  gfx_mesh *m = new (std::nothrow) gfx_mesh();
+  if (!m) return fail(c, GFX_E_OOM, "host allocation");
+  m->ctx = c; m->gt = gt; m->nv = nv; m->n_pts = n_pts; m->n_cv = n_cv;
+  m->h_pts.assign(h_pts, h_pts + 3 * n_pts);
+  m->h_cvx.assign(h_cvx, h_cvx + nv * n_cv);
+  hipError_t e = dalloc(&m->d_pts, 3 * (size_t)n_pts);
+  if (e == hipSuccess) e = dalloc(&m->d_cvx, (size_t)nv * n_cv);
+  if (e == hipSuccess && n_pts) e = hipMemcpy(m->d_pts, h_pts, 24 * (size_t)n_pts, hipMemcpyHostToDevice);
+  if (e == hipSuccess && n_cv) e = hipMemcpy(m->d_cvx, h_cvx, 4 * (size_t)nv * n_cv, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    dfree(m->d_pts); dfree(m->d_cvx); delete m;
+    return hipfail(c, e, "gfx_mesh_create upload");
+  }
+  *out = m;
+  return GFX_OK;
+}
+
+int gfx_mesh_set_points(gfx_mesh *m, const double *h_pts) {
+  if (!m || !h_pts) return GFX_E_INVALID;
+  gfx_ctx *c = m->ctx;
+  if (set_device(c)) return GFX_E_HIP;
+  std::copy(h_pts, h_pts + 3 * m->n_pts, m->h_pts.begin());
+  HIPCHK(c, hipMemcpyAsync(m->d_pts, h_pts, 24 * (size_t)m->n_pts, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFX_OK;
+}
+
+int gfx_mesh_destroy(gfx_mesh *m) {
+  if (!m) return GFX_E_INVALID;
+  (void)hipSetDevice(m->ctx->device);
+  dfree(m->d_pts); dfree(m->d_cvx);
+  delete m;
+  return GFX_OK;
+}
+
+// ------------------------------------------------------------- mesh_fem
+static int mf_finish(gfx_mesh *m, gfx_mesh_fem *mf, gfx_mesh_fem **out) {
+  gfx_ctx *c = m->ctx;
+  hipError_t e = dalloc(&mf->d_dofs, mf->h_dofs.size());
+  if (e == hipSuccess && !mf->h_dofs.empty())
+    e = hipMemcpy(mf->d_dofs, mf->h_dofs.data(), 4 * mf->h_dofs.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    dfree(mf->d_dofs); delete mf;
+    return hipfail(c, e, "gfx_mesh_fem upload");
+  }
+  *out = mf;
+  return GFX_OK;
+}
+
+int gfx_mesh_fem_create(gfx_mesh *m, const char *fem_name, int qdim, gfx_mesh_fem **out) {
+  if (!m || !fem_name || !out || qdim < 1 || qdim > 3) return fail(m ? m->ctx : nullptr, GFX_E_INVALID, "gfx_mesh_fem_create: invalid argument");
+  gfx_ctx *c = m->ctx;
+  *out = nullptr;
+  gfx_mesh_fem *mf = new (std::nothrow) gfx_mesh_fem();
+  if (!mf) return fail(c, GFX_E_OOM, "host allocation");
+  mf->mesh = m; mf->qdim = qdim;
+  if (!fem_descriptor(fem_name, mf->fem)) {
+    delete mf;
+    return fail(c, GFX_E_UNSUPPORTED, std::string("unsupported finite element ") + fem_name);
+  }
+  std::string err;
+  mf->ndof = enumerate_dofs(m->gt, m->n_pts, m->h_pts.data(), m->n_cv, m->h_cvx.data(), mf->fem,
+                            qdim, mf->h_dofs, err);
+  if (mf->ndof < 0) { delete mf; return fail(c, GFX_E_UNSUPPORTED, err); }
+  if (set_device(c)) { delete mf; return GFX_E_HIP; }
+  return mf_finish(m, mf, out);
+}
+
+int gfx_mesh_fem_create_from_dofs(gfx_mesh *m, const char *fem_name, int qdim,
+                                  const int32_t *h_basic_dofs, int64_t ndof, gfx_mesh_fem **out) {
+  if (!m || !fem_name || !out || qdim < 1 || qdim > 3 || !h_basic_dofs || ndof < 0)
+    return fail(m ? m->ctx : nullptr, GFX_E_INVALID, "gfx_mesh_fem_create_from_dofs: invalid argument");
+  gfx_ctx *c = m->ctx;
+  *out = nullptr;
+  gfx_mesh_fem *mf = new (std::nothrow) gfx_mesh_fem();
+  if (!mf) return fail(c, GFX_E_OOM, "host allocation");
+  mf->mesh = m; mf->qdim = qdim; mf->ndof = ndof;
+  if (!fem_descriptor(fem_name, mf->fem)) {
+    delete mf;
+    return fail(c, GFX_E_UNSUPPORTED, std::string("unsupported finite element ") + fem_name);
+  }
+  const size_t n = (size_t)m->n_cv * mf->fem.nd;
+  for (size_t i = 0; i < n; ++i)
+    if (h_basic_dofs[i] < 0 || h_basic_dofs[i] + qdim > ndof || h_basic_dofs[i] % qdim) {
+      delete mf;
+      return fail(c, GFX_E_INVALID, "basic dof out of range or not a multiple of qdim");
+    }
+  mf->h_dofs.assign(h_basic_dofs, h_basic_dofs + n);
+  if (set_device(c)) { delete mf; return GFX_E_HIP; }
+  return mf_finish(m, mf, out);
+}
+
+int64_t gfx_mesh_fem_nb_dof(gfx_mesh_fem *mf) { return mf ? mf->ndof : -1; }
+int gfx_mesh_fem_nb_dof_per_element(gfx_mesh_fem *mf) { return mf ? mf->fem.nd : -1; }
+
+int gfx_mesh_fem_basic_dofs(gfx_mesh_fem *mf, int32_t *h_out) {
+  if (!mf || !h_out) return GFX_E_INVALID;
+  std::copy(mf->h_dofs.begin(), mf->h_dofs.end(), h_out);
+  return GFX_OK;
+}
+
+int gfx_mesh_fem_dof_nodes(gfx_mesh_fem *mf, double *h_out) {
+  if (!mf || !h_out) return GFX_E_INVALID;
+  const gfx_mesh *m = mf->mesh;
+  const int nd = mf->fem.nd, nv = m->nv;
+  std::vector<double> N(nv);
+  std::vector<std::vector<double>> Nn(nd, std::vector<double>(nv));
+  for (int i = 0; i < nd; ++i) geotrans_eval(m->gt, &mf->fem.nodes[3 * i], Nn[i].data(), nullptr);
+  for (int64_t e = 0; e < m->n_cv; ++e)
+    for (int i = 0; i < nd; ++i) {
+      const int64_t node = mf->h_dofs[(size_t)e * nd + i] / mf->qdim;
+      double x[3] = {0, 0, 0};
+      for (int v = 0; v < nv; ++v)
+        for (int k = 0; k < 3; ++k) x[k] += Nn[i][v] * m->h_pts[3 * (size_t)m->h_cvx[(size_t)e * nv + v] + k];
+      for (int k = 0; k < 3; ++k) h_out[3 * node + k] = x[k];
+    }
+  return GFX_OK;
+}
+
+int gfx_mesh_fem_destroy(gfx_mesh_fem *mf) {
+  if (!mf) return GFX_E_INVALID;
+  (void)hipSetDevice(mf->mesh->ctx->device);
+  dfree(mf->d_dofs);
+  delete mf;
+  return GFX_OK;
+}
+
+// -------------------------------------------------------------- mesh_im
+int gfx_mesh_im_create(gfx_mesh *m, const char *im_name, gfx_mesh_im **out) {
+  if (!m || !im_name || !out) return GFX_E_INVALID;
+  *out = nullptr;
+  gfx_mesh_im *mim = new (std::nothrow) gfx_mesh_im();
+  if (!mim) return fail(m->ctx, GFX_E_OOM, "host allocation");
+  mim->mesh = m;
+  if (!integration_rule(im_name, mim->rule)) {
+    delete mim;
+    return fail(m->ctx, GFX_E_UNSUPPORTED, std::string("unsupported integration method ") + im_name);
+  }
+  if (mim->rule.gt != m->gt) {
+    delete mim;
+    return fail(m->ctx, GFX_E_INVALID, "integration method does not match the mesh element");
+  }
+  *out = mim;
+  return GFX_OK;
+}
+
+int gfx_mesh_im_nb_points(gfx_mesh_im *mim) { return mim ? mim->rule.n() : -1; }
+
+int gfx_mesh_im_points(gfx_mesh_im *mim, double *h_pts, double *h_w) {
+  if (!mim) return GFX_E_INVALID;
+  if (h_pts) std::copy(mim->rule.pts.begin(), mim->rule.pts.end(), h_pts);
+  if (h_w) std::copy(mim->rule.w.begin(), mim->rule.w.end(), h_w);
+  return GFX_OK;
+}
+
+int gfx_mesh_im_destroy(gfx_mesh_im *mim) {
+  if (!mim) return GFX_E_INVALID;
+  delete mim;
+  return GFX_OK;
+}
+
+// ---------------------------------------------------------------- plans
+static int plan_upload_coef(gfx_plan *p, int slot, const gfx_coef *cf) {
+  gfx_ctx *c = p->ctx;
+  if (cf->kind != GFX_COEF_CONST && cf->kind != GFX_COEF_PER_ELEMENT)
+    return fail(c, GFX_E_INVALID, "unknown coefficient kind");
+  p->coef[slot] = *cf;
+  p->coef[slot].h_values = nullptr;
+  dfree(p->d_coef[slot]);
+  if (cf->kind == GFX_COEF_PER_ELEMENT) {
+    if (!cf->h_values) return fail(c, GFX_E_INVALID, "per-element coefficient without values");
+    const int64_t n = p->mf->mesh->n_cv;
+    HIPCHK(c, dalloc(&p->d_coef[slot], (size_t)n));
+    HIPCHK(c, hipMemcpy(p->d_coef[slot], cf->h_values, 8 * (size_t)n, hipMemcpyHostToDevice));
+  }
+  return GFX_OK;
+}
+
+int gfx_plan_destroy(gfx_plan *p) {
+  if (!p) return GFX_E_INVALID;
+  (void)hipSetDevice(p->ctx->device);
+  dfree(p->d_coef[0]); dfree(p->d_coef[1]);
+  dfree(p->d_gref); dfree(p->d_qw); dfree(p->d_R);
+  dfree(p->d_rec); dfree(p->d_thr); dfree(p->d_keep);
+  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots);
+  dfree(p->d_nbr); dfree(p->d_mask);
+  dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
+  delete p;
+  return GFX_OK;
+}
+
+int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef *coefs,
+                    int n_coefs, gfx_plan **out) {
+  if (!mf || !mim || !out) return GFX_E_INVALID;
+  *out = nullptr;
+  gfx_ctx *c = mf->mesh->ctx;
+  if (mim->mesh != mf->mesh) return fail(c, GFX_E_INVALID, "mesh_fem and mesh_im on different meshes");
+  const gfx_mesh *m = mf->mesh;
+  if (m->gt != GT_TET_P1 || mf->fem.kind != FEM_PK)
+    return fail(c, GFX_E_UNSUPPORTED, "this plan kind needs FEM_PK(3,k) on GT_PK(3,1)");
+  const int Q = mf->qdim, nd = mf->fem.nd;
+  if (!((term == GFX_TERM_LAPLACE && (Q == 1 || Q == 3)) || (term == GFX_TERM_ISO_ELAST && Q == 3)))
+    return fail(c, GFX_E_UNSUPPORTED, "term / qdim combination not supported");
+  if (!(nd == 4 || nd == 10 || nd == 20) || (term == GFX_TERM_LAPLACE && Q == 3 && nd == 20))
+    return fail(c, GFX_E_UNSUPPORTED, "element degree not supported for this term");
+  const int need = term == GFX_TERM_LAPLACE ? 1 : 2;
+  if (n_coefs != need || !coefs) return fail(c, GFX_E_INVALID, "wrong number of coefficients");
+  if (set_device(c)) return GFX_E_HIP;
+  gfx_plan *p = new (std::nothrow) gfx_plan();
+  if (!p) return fail(c, GFX_E_OOM, "host allocation");
+  p->ctx = c; p->mf = mf; p->mim = mim; p->term = term;
+  p->nd = nd; p->qdim = Q; p->nloc = nd * Q; p->nq = mim->rule.n(); p->nv = m->nv;
+  p->kwords = (p->nloc * p->nloc + 31) / 32;
+  p->mwpi = (nd * Q * Q + 31) / 32;
+  p->ndof = mf->ndof;
+  p->n_nodes = mf->ndof / Q;
+  p->coef[1].kind = GFX_COEF_CONST;
+  p->coef[1].value = 0.0;
+  for (int s = 0; s < need; ++s) {
+    int r = plan_upload_coef(p, s, &coefs[s]);
+    if (r) { gfx_plan_destroy(p); return r; }
+  }
+  // reference tables: gradients at the points and R_ij = sum_q w_q g_i g_j^T
+  const int nq = p->nq;
+  std::vector<double> gref((size_t)nq * nd * 3), val(nd), R((size_t)nd * nd * 9, 0.0);
+  for (int q = 0; q < nq; ++q) fem_eval(mf->fem, &mim->rule.pts[3 * q], val.data(), &gref[(size_t)q * nd * 3]);
+  for (int q = 0; q < nq; ++q) {
+    const double w = mim->rule.w[q];
+    const double *g = &gref[(size_t)q * nd * 3];
+    for (int i = 0; i < nd; ++i)
+      for (int j = 0; j < nd; ++j)
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) R[((size_t)i * nd + j) * 9 + 3 * a + b] += w * g[3 * i + a] * g[3 * j + b];
+  }
+  hipError_t e = dalloc(&p->d_gref, gref.size());
+  if (e == hipSuccess) e = dalloc(&p->d_qw, (size_t)nq);
+  if (e == hipSuccess) e = dalloc(&p->d_R, R.size());
+  if (e == hipSuccess) e = dalloc(&p->d_rec, (size_t)kRecStride * m->n_cv);
+  if (e == hipSuccess) e = dalloc(&p->d_thr, (size_t)m->n_cv);
+  if (e == hipSuccess) e = dalloc(&p->d_err, 4);
+  if (e == hipSuccess) e = hipMemcpy(p->d_gref, gref.data(), 8 * gref.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->d_qw, mim->rule.w.data(), 8 * (size_t)nq, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(p->d_R, R.data(), 8 * R.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemset(p->d_err, 0, 4);
+  if (e != hipSuccess) { gfx_plan_destroy(p); return hipfail(c, e, "gfx_plan_create"); }
+  *out = p;
+  return GFX_OK;
+}
+
+int gfx_plan_set_coef(gfx_plan *p, int slot, const gfx_coef *cf) {
+  if (!p || !cf || slot < 0 || slot > 1) return GFX_E_INVALID;
+  if (set_device(p->ctx)) return GFX_E_HIP;
+  return plan_upload_coef(p, slot, cf);
+}
+
+int gfx_plan_set_form(gfx_plan *p, int form, int precision) {
+  if (!p) return GFX_E_INVALID;
+  if ((form != GFX_FORM_FACTORIZED && form != GFX_FORM_PERQP) ||
+      (precision != GFX_PREC_FP64 && precision != GFX_PREC_FP32))
+    return fail(p->ctx, GFX_E_INVALID, "unknown form or precision");
+  p->form = form;
+  p->precision = precision;
+  return GFX_OK;
+}
+
+static int plan_records(gfx_plan *p, bool with_thr) {
+  const gfx_mesh *m = p->mf->mesh;
+  gfx_ctx *c = p->ctx;
+  HIPCHK(c, launch_records_tet(c->stream, m->n_cv, m->d_pts, m->d_cvx,
+                               p->coef[0].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[0] : nullptr,
+                               p->coef[0].value,
+                               p->coef[1].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[1] : nullptr,
+                               p->coef[1].value, with_thr ? p->d_thr : nullptr, p->d_rec, p->d_err));
+  return GFX_OK;
+}
+
+static int check_err_flag(gfx_plan *p) {
+  gfx_ctx *c = p->ctx;
+  int flag = 0;
+  HIPCHK(c, hipMemcpyAsync(&flag, p->d_err, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (flag & 1) return fail(c, GFX_E_DEGENERATE_ELEMENT, "Non invertible matrix (degenerate element)");
+  if (flag & 2) return fail(c, GFX_E_PATTERN, "a node patch exceeds the kernel limits");
+  return GFX_OK;
+}
+
+int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
+  if (!p) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (set_device(c)) return GFX_E_HIP;
+  const gfx_mesh *m = p->mf->mesh;
+  const int64_t n_cv = m->n_cv, nn = p->n_nodes;
+  const int nd = p->nd, Q = p->qdim;
+  hipStream_t s = c->stream;
+  int r;
+  HIPCHK(c, hipMemsetAsync(p->d_err, 0, 4, s));
+  if ((r = plan_records(p, false))) return r;
+  // incidences
+  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots);
+  dfree(p->d_nbr); dfree(p->d_mask); dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals);
+  dfree(p->d_keep);
+  p->nnz = -1;
+  p->n_inc = n_cv * nd;
+  if (p->n_inc >= INT32_MAX) return fail(c, GFX_E_INVALID, "too many element-node incidences");
+  int32_t *tmp32 = nullptr;
+  HIPCHK(c, dalloc(&p->d_inc_ptr, (size_t)nn + 1));
+  HIPCHK(c, dalloc(&p->d_inc, (size_t)p->n_inc));
+  HIPCHK(c, dalloc(&tmp32, (size_t)(2 * nn + scan_tmp_elems(nn))));
+  hipError_t he = launch_incidences(s, n_cv, nd, Q, p->mf->d_dofs, nn, p->d_inc_ptr, p->d_inc, tmp32);
+  if (he != hipSuccess) { dfree(tmp32); return hipfail(c, he, "incidences"); }
+  // largest patch (LDS sort size) on the host
+  std::vector<int32_t> iptr((size_t)nn + 1);
+  he = hipMemcpyAsync(iptr.data(), p->d_inc_ptr, 4 * ((size_t)nn + 1), hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  dfree(tmp32);
+  if (he != hipSuccess) return hipfail(c, he, "incidence copy");
+  int maxinc = 0;
+  for (int64_t n = 0; n < nn; ++n) maxinc = std::max(maxinc, iptr[n + 1] - iptr[n]);
+  p->max_cand = maxinc * nd;
+  if (p->max_cand > 4096)
+    return fail(c, GFX_E_PATTERN, "a node belongs to too many elements for the patch kernels");
+  // thresholds and keep bits
+  HIPCHK(c, dalloc(&p->d_keep, (size_t)n_cv * p->kwords));
+  PatternArgs a{};
+  a.n_cv = n_cv; a.n_nodes = nn; a.nd = nd; a.qdim = Q; a.nloc = p->nloc;
+  a.kwords = p->kwords; a.mwpi = p->mwpi; a.dofs = p->mf->d_dofs; a.rec = p->d_rec;
+  a.gref = p->d_gref; a.qw = p->d_qw; a.nq = p->nq; a.thr = p->d_thr; a.keep = p->d_keep;
+  a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.err = p->d_err;
+  HIPCHK(c, launch_keep(s, p->term, a));
+  // patches
+  HIPCHK(c, dalloc(&p->d_slot, (size_t)p->n_inc * nd));
+  HIPCHK(c, dalloc(&p->d_nslots, (size_t)nn));
+  HIPCHK(c, dalloc(&p->d_nbr, (size_t)p->n_inc * nd));
+  HIPCHK(c, dalloc(&p->d_mask, (size_t)p->n_inc * p->mwpi));
+  int64_t *rowcnt = nullptr, *tmp64 = nullptr;
+  HIPCHK(c, dalloc(&rowcnt, (size_t)p->ndof));
+  HIPCHK(c, dalloc(&p->d_indptr, (size_t)p->ndof + 1));
+  a.slot = p->d_slot; a.nslots = p->d_nslots; a.nbr = p->d_nbr; a.mask = p->d_mask;
+  a.rowcnt = rowcnt; a.indptr = p->d_indptr;
+  he = launch_patch_build(s, a, p->max_cand);
+  if (he == hipSuccess) he = dalloc(&tmp64, (size_t)scan_tmp_elems(p->ndof));
+  if (he == hipSuccess) he = scan_exclusive_i64(s, rowcnt, p->d_indptr, p->ndof, tmp64);
+  int64_t nnz = 0;
+  if (he == hipSuccess)
+    he = hipMemcpyAsync(&nnz, p->d_indptr + p->ndof, 8, hipMemcpyDeviceToHost, s);
+  if (he == hipSuccess) he = hipStreamSynchronize(s);
+  dfree(rowcnt); dfree(tmp64);
+  if (he != hipSuccess) return hipfail(c, he, "patch build");
+  if ((r = check_err_flag(p))) return r;
+  // nslots maximum for the assembly kernel's LDS
+  std::vector<uint16_t> ns((size_t)nn);
+  HIPCHK(c, hipMemcpy(ns.data(), p->d_nslots, 2 * (size_t)nn, hipMemcpyDeviceToHost));
+  p->max_nslots = 0;
+  for (int64_t n = 0; n < nn; ++n) p->max_nslots = std::max<int>(p->max_nslots, ns[n]);
+  if ((size_t)Q * Q * p->max_nslots * 8 > 64 * 1024)
+    return fail(c, GFX_E_PATTERN, "node patch too large for the assembly kernel");
+  HIPCHK(c, dalloc(&p->d_indices, (size_t)nnz));
+  HIPCHK(c, dalloc(&p->d_vals, (size_t)nnz));
+  a.indices = p->d_indices;
+  HIPCHK(c, launch_fill_columns(s, a));
+  HIPCHK(c, hipStreamSynchronize(s));
+  dfree(p->d_nbr);
+  dfree(p->d_keep);
+  p->nnz = nnz;
+  if (nnz_out) *nnz_out = nnz;
+  return GFX_OK;
+}
+
+int gfx_plan_assemble(gfx_plan *p) {
+  if (!p) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
+  if (set_device(c)) return GFX_E_HIP;
+  int r;
+  if ((r = plan_records(p, true))) return r;
+  PatchArgs a{};
+  a.n_nodes = p->n_nodes; a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot;
+  a.nslots = p->d_nslots; a.mask = p->d_mask; a.mwpi = p->mwpi; a.indptr = p->d_indptr;
+  a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
+  a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals;
+  HIPCHK(c, launch_assemble_patch(c->stream, p->term, p->form, p->precision, a, p->max_nslots));
+  return GFX_OK;
+}
+
+int gfx_plan_dims(gfx_plan *p, int64_t *nrows, int64_t *nnz) {
+  if (!p) return GFX_E_INVALID;
+  if (nrows) *nrows = p->ndof;
+  if (nnz) *nnz = p->nnz;
+  return GFX_OK;
+}
+
+int gfx_plan_export_csr(gfx_plan *p, int64_t *h_indptr, int32_t *h_indices, double *h_vals) {
+  if (!p) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
+  if (set_device(c)) return GFX_E_HIP;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (h_indptr) HIPCHK(c, hipMemcpy(h_indptr, p->d_indptr, 8 * ((size_t)p->ndof + 1), hipMemcpyDeviceToHost));
+  if (h_indices) HIPCHK(c, hipMemcpy(h_indices, p->d_indices, 4 * (size_t)p->nnz, hipMemcpyDeviceToHost));
+  if (h_vals) HIPCHK(c, hipMemcpy(h_vals, p->d_vals, 8 * (size_t)p->nnz, hipMemcpyDeviceToHost));
+  return GFX_OK;
+}
+
+int gfx_plan_export_csc(gfx_plan *p, int64_t *h_colptr, uint32_t *h_rows, double *h_vals) {
+  if (!p || !h_colptr) return GFX_E_INVALID;
+  const int64_t n = p->ndof, nnz = p->nnz;
+  if (nnz < 0) return fail(p->ctx, GFX_E_STATE, "build the pattern first");
+  std::vector<int64_t> ip((size_t)n + 1);
+  std::vector<int32_t> ix((size_t)nnz);
+  std::vector<double> v(h_vals ? (size_t)nnz : 0);
+  int r = gfx_plan_export_csr(p, ip.data(), ix.data(), h_vals ? v.data() : nullptr);
+  if (r) return r;
+  // transpose (counting sort by column keeps rows sorted)
+  std::vector<int64_t> cnt((size_t)n + 1, 0);
+  for (int64_t k = 0; k < nnz; ++k) cnt[ix[k] + 1]++;
+  for (int64_t j = 0; j < n; ++j) cnt[j + 1] += cnt[j];
+  std::copy(cnt.begin(), cnt.end(), h_colptr);
+  for (int64_t row = 0; row < n; ++row)
+    for (int64_t k = ip[row]; k < ip[row + 1]; ++k) {
+      const int64_t d = cnt[ix[k]]++;
+      if (h_rows) h_rows[d] = (uint32_t)row;
+      if (h_vals) h_vals[d] = v[k];
+    }
+  return GFX_OK;
+}
+
+int gfx_plan_device_csr(gfx_plan *p, const int64_t **d_indptr, const int32_t **d_indices,
+                        double **d_vals) {
+  if (!p) return GFX_E_INVALID;
+  if (p->nnz < 0) return fail(p->ctx, GFX_E_STATE, "build the pattern first");
+  if (d_indptr) *d_indptr = p->d_indptr;
+  if (d_indices) *d_indices = p->d_indices;
+  if (d_vals) *d_vals = p->d_vals;
+  return GFX_OK;
+}
+
+int gfx_plan_last_timings(gfx_plan *p, float *ms_geometry, float *ms_assemble) {
+  if (!p) return GFX_E_INVALID;
+  if (ms_geometry) *ms_geometry = p->last_ms[0];
+  if (ms_assemble) *ms_assemble = p->last_ms[1];
+  return GFX_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,141 @@
+// Internal structures of libgfx: contexts, meshes, mesh_fems, integration
+// methods and assembly plans, plus the launch wrappers of the HIP kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/gfx.h"
+#include "fem_ref.h"
+
+struct gfx_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+};
+
+struct gfx_mesh {
+  gfx_ctx *ctx = nullptr;
+  int gt = 0;
+  int nv = 4;
+  int64_t n_pts = 0, n_cv = 0;
+  double *d_pts = nullptr;     // n_pts*3
+  int32_t *d_cvx = nullptr;    // n_cv*nv
+  std::vector<double> h_pts;   // host copy (dof nodes, checks)
+  std::vector<int32_t> h_cvx;
+};
+
+struct gfx_mesh_fem {
+  gfx_mesh *mesh = nullptr;
+  gfx::FemDesc fem;
+  int qdim = 1;
+  int64_t ndof = 0;
+  std::vector<int32_t> h_dofs;  // n_cv*nd basic dofs
+  int32_t *d_dofs = nullptr;
+};
+
+struct gfx_mesh_im {
+  gfx_mesh *mesh = nullptr;
+  gfx::IntegrationRule rule;
+};
+
+namespace gfx {
+
+constexpr int kRecStride = 16;  // doubles per element record
+
+// Per-element record written by the geometry kernel (affine tetrahedra):
+// [0..8] B = K^{-T} row-major, [9] |det K|, [10] coefficient a,
+// [11] coefficient b, [12] insertion threshold, [13..15] unused.
+
+struct PatchArgs {
+  int64_t n_nodes;
+  const int32_t *inc_ptr;   // n_nodes+1
+  const uint32_t *inc;      // (element << 5) | local node
+  const uint16_t *slot;     // per incidence, nd slots
+  const uint16_t *nslots;   // per node
+  const uint32_t *mask;     // presence bits, mwpi words per incidence
+  int mwpi;
+  const int64_t *indptr;    // CSR rows
+  int nd, qdim, nq;
+  const double *rec;        // element records
+  const double *gref;       // nq*nd*3 reference gradients
+  const double *qw;         // nq
+  const double *R;          // nd*nd*9 weighted reference gradient products
+  double *vals;             // CSR values
+  int xcd_chunk;            // nodes per XCD in the block swizzle
+};
+
+struct PatternArgs {
+  int64_t n_cv, n_nodes;
+  int nd, qdim, nloc, kwords, mwpi;
+  const int32_t *dofs;
+  const double *rec;
+  const double *gref;
+  const double *qw;
+  int nq;
+  double *thr;              // out: per element threshold
+  uint32_t *keep;           // out: per element keep bits (row-major nloc x nloc)
+  int32_t *inc_ptr;
+  uint32_t *inc;
+  uint16_t *slot;
+  uint16_t *nslots;
+  int32_t *nbr;
+  uint32_t *mask;
+  int64_t *rowcnt;          // per dof
+  int64_t *indptr;
+  int32_t *indices;
+  int *err;                 // device error flag
+};
+
+// kernel launch wrappers (kernels.hip)
+hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,
+                              const int32_t *cvx, const double *ca, double ca_c,
+                              const double *cb, double cb_c, const double *thr,
+                              double *rec, int *err);
+hipError_t launch_incidences(hipStream_t s, int64_t n_cv, int nd, int qdim,
+                             const int32_t *dofs, int64_t n_nodes,
+                             int32_t *inc_ptr, uint32_t *inc, int32_t *tmp);
+hipError_t launch_keep(hipStream_t s, int term, const PatternArgs &a);
+hipError_t launch_patch_build(hipStream_t s, const PatternArgs &a, int max_cand);
+hipError_t launch_fill_columns(hipStream_t s, const PatternArgs &a);
+hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec,
+                                 const PatchArgs &a, int max_nslots);
+hipError_t scan_exclusive_i64(hipStream_t s, const int64_t *in, int64_t *out,
+                              int64_t n, int64_t *tmp);
+hipError_t scan_exclusive_i32(hipStream_t s, const int32_t *in, int32_t *out,
+                              int64_t n, int32_t *tmp);
+int64_t scan_tmp_elems(int64_t n);
+
+}  // namespace gfx
+
+struct gfx_plan {
+  gfx_ctx *ctx = nullptr;
+  gfx_mesh_fem *mf = nullptr;
+  gfx_mesh_im *mim = nullptr;
+  int term = GFX_TERM_LAPLACE;
+  int nd = 0, qdim = 1, nloc = 0, nq = 0, nv = 4;
+  int kwords = 0, mwpi = 0;
+  gfx_coef coef[2];
+  double *d_coef[2] = {nullptr, nullptr};
+  double *d_gref = nullptr, *d_qw = nullptr, *d_R = nullptr;
+  double *d_rec = nullptr, *d_thr = nullptr;
+  uint32_t *d_keep = nullptr;
+  int64_t n_nodes = 0, n_inc = 0;
+  int32_t *d_inc_ptr = nullptr;
+  uint32_t *d_inc = nullptr;
+  uint16_t *d_slot = nullptr, *d_nslots = nullptr;
+  int32_t *d_nbr = nullptr;
+  uint32_t *d_mask = nullptr;
+  int max_nslots = 0, max_cand = 0;
+  int64_t ndof = 0, nnz = -1;
+  int64_t *d_indptr = nullptr;
+  int32_t *d_indices = nullptr;
+  double *d_vals = nullptr;
+  int *d_err = nullptr;
+  int form = GFX_FORM_FACTORIZED;
+  int precision = GFX_PREC_FP64;
+  float last_ms[4] = {0, 0, 0, 0};
+};

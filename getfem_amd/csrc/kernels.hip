@@ -1,0 +1,602 @@
+// HIP kernels of libgfx for gfx950 (MI355X, CDNA4).
+//
+// Data flow of one order-2 assembly (what ga_exec does element by element,
+// src/getfem_generic_assembly_compile_and_exec.cc:8750-8869, reorganised
+// for the GPU):
+//   k_records_tet    per element, FP64: K = G grad(N), det K, B = K^{-T}
+//                    (bgeot_geometric_trans.cc:270-413) and the element's
+//                    coefficients -> one 128-byte record per element.
+//   k_assemble_patch one workgroup per mesh node ("patch"): for every
+//                    element incident to the node, computes the rows of the
+//                    element matrix that belong to the node's dofs, drops
+//                    entries under the element's insertion threshold
+//                    (compile_and_exec.cc:5441-5453), accumulates them in
+//                    LDS and writes the node's CSR rows once.  Rows are owned
+//                    by exactly one workgroup: no atomics in HBM, and every
+//                    CSR value is written exactly once.
+// Pattern build (once per plan): node -> element incidences by a counting
+// sort, per-element thresholds and keep bits, then per node a segmented key
+// sort of the neighbour nodes in LDS gives the sorted CSR columns.
+#include <hip/hip_runtime.h>
+
+#include "gfx_internal.h"
+
+namespace gfx {
+
+namespace {
+
+constexpr int kWave = 64;
+constexpr int kMaxCand = 4096;  // neighbour candidates per node (LDS sort size)
+
+__device__ __forceinline__ int64_t gtid() {
+  return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+  for (int o = kWave / 2; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, kWave));
+  return v;
+}
+
+// ------------------------------------------------------------------------
+// geometry
+// ------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_records_tet(
+    int64_t n_cv, const double *__restrict__ pts, const int32_t *__restrict__ cvx,
+    const double *__restrict__ ca, double ca_c, const double *__restrict__ cb,
+    double cb_c, const double *__restrict__ thr, double *__restrict__ rec,
+    int *__restrict__ err) {
+  const int64_t e = gtid();
+  if (e >= n_cv) return;
+  const int4 v = *reinterpret_cast<const int4 *>(cvx + 4 * e);
+  const double *p0 = pts + 3 * (int64_t)v.x, *p1 = pts + 3 * (int64_t)v.y;
+  const double *p2 = pts + 3 * (int64_t)v.z, *p3 = pts + 3 * (int64_t)v.w;
+  // column-major K(r,c) = k[r + 3c] = x_{c+1}[r] - x_0[r]  (G . grad N for
+  // the P1 transformation)
+  double k[9];
+  for (int r = 0; r < 3; ++r) {
+    const double x0 = p0[r];
+    k[r] = p1[r] - x0;
+    k[r + 3] = p2[r] - x0;
+    k[r + 6] = p3[r] - x0;
+  }
+  const double a0 = k[4] * k[8] - k[5] * k[7];
+  const double a1 = k[5] * k[6] - k[3] * k[8];
+  const double a2 = k[3] * k[7] - k[4] * k[6];
+  const double J = k[0] * a0 + k[1] * a1 + k[2] * a2;
+  if (J == 0.0) { atomicOr(err, 1); }
+  double *o = rec + (int64_t)kRecStride * e;
+  // B = K^{-T} from cofactors, stored row-major B(r,c) = o[3r+c]
+  double2 *o2 = reinterpret_cast<double2 *>(o);
+  o2[0] = make_double2(a0 / J, (k[2] * k[7] - k[1] * k[8]) / J);
+  o2[1] = make_double2((k[1] * k[5] - k[2] * k[4]) / J, a1 / J);
+  o2[2] = make_double2((k[0] * k[8] - k[2] * k[6]) / J, (k[2] * k[3] - k[0] * k[5]) / J);
+  o2[3] = make_double2(a2 / J, (k[1] * k[6] - k[0] * k[7]) / J);
+  o2[4] = make_double2((k[0] * k[4] - k[1] * k[3]) / J, fabs(J));
+  o2[5] = make_double2(ca ? ca[e] : ca_c, cb ? cb[e] : cb_c);
+  o2[6] = make_double2(thr ? thr[e] : 0.0, 0.0);
+  o2[7] = make_double2(0.0, 0.0);
+}
+
+// ------------------------------------------------------------------------
+// element pair blocks: rows (i, a), columns (j, b) of the element matrix
+// ------------------------------------------------------------------------
+template <int Q, int TERM, int FORM, typename Real>
+__device__ __forceinline__ void pair_block(const double *__restrict__ rec, int i, int j,
+                                           int nd, int nq, const double *__restrict__ gref,
+                                           const double *__restrict__ qw,
+                                           const double *__restrict__ R, Real v[Q][Q]) {
+  Real B[9];
+  for (int t = 0; t < 9; ++t) B[t] = (Real)rec[t];
+  const Real aJ = (Real)rec[9], ca = (Real)rec[10], cb = (Real)rec[11];
+  if (FORM == GFX_FORM_FACTORIZED) {
+    // G = B R_ij B^T, R_ij = sum_q w_q gradref_i gradref_j^T
+    const double *r = R + ((int64_t)i * nd + j) * 9;
+    Real rr[9];
+    for (int t = 0; t < 9; ++t) rr[t] = (Real)r[t];
+    Real T[3][3], G[3][3];
+    for (int c = 0; c < 3; ++c)
+      for (int b = 0; b < 3; ++b)
+        T[c][b] = rr[3 * c] * B[3 * b] + rr[3 * c + 1] * B[3 * b + 1] + rr[3 * c + 2] * B[3 * b + 2];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        G[a][b] = B[3 * a] * T[0][b] + B[3 * a + 1] * T[1][b] + B[3 * a + 2] * T[2][b];
+    const Real tr = G[0][0] + G[1][1] + G[2][2];
+    if (TERM == GFX_TERM_ISO_ELAST) {
+      for (int a = 0; a < Q; ++a)
+        for (int b = 0; b < Q; ++b)
+          v[a][b] = aJ * (ca * G[a][b] + cb * G[b][a] + (a == b ? cb * tr : Real(0)));
+    } else {
+      for (int a = 0; a < Q; ++a)
+        for (int b = 0; b < Q; ++b) v[a][b] = (a == b) ? aJ * ca * tr : Real(0);
+    }
+  } else {
+    for (int a = 0; a < Q; ++a)
+      for (int b = 0; b < Q; ++b) v[a][b] = Real(0);
+    for (int q = 0; q < nq; ++q) {
+      const double *hi = gref + ((int64_t)q * nd + i) * 3;
+      const double *hj = gref + ((int64_t)q * nd + j) * 3;
+      const Real h0 = (Real)hi[0], h1 = (Real)hi[1], h2 = (Real)hi[2];
+      const Real k0 = (Real)hj[0], k1 = (Real)hj[1], k2 = (Real)hj[2];
+      Real gi[3], gj[3];
+      for (int a = 0; a < 3; ++a) {
+        gi[a] = B[3 * a] * h0 + B[3 * a + 1] * h1 + B[3 * a + 2] * h2;
+        gj[a] = B[3 * a] * k0 + B[3 * a + 1] * k1 + B[3 * a + 2] * k2;
+      }
+      const Real cq = aJ * (Real)qw[q];
+      const Real d = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
+      if (TERM == GFX_TERM_ISO_ELAST) {
+        for (int a = 0; a < Q; ++a)
+          for (int b = 0; b < Q; ++b)
+            v[a][b] += cq * (ca * gi[a] * gj[b] + cb * gi[b] * gj[a] + (a == b ? cb * d : Real(0)));
+      } else {
+        for (int a = 0; a < Q; ++a) v[a][a] += cq * (ca * d);
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// incidences: node -> (element, local node), counting sort
+// ------------------------------------------------------------------------
+__global__ void k_inc_count(int64_t n, int nd, int qdim, const int32_t *__restrict__ dofs,
+                            int32_t *__restrict__ cnt) {
+  const int64_t t = gtid();
+  if (t >= n) return;
+  atomicAdd(&cnt[dofs[t] / qdim], 1);
+}
+
+__global__ void k_inc_fill(int64_t n, int nd, int qdim, const int32_t *__restrict__ dofs,
+                           const int32_t *__restrict__ inc_ptr, int32_t *__restrict__ cursor,
+                           uint32_t *__restrict__ inc) {
+  const int64_t t = gtid();
+  if (t >= n) return;
+  const int node = dofs[t] / qdim;
+  const int pos = inc_ptr[node] + atomicAdd(&cursor[node], 1);
+  const int64_t e = t / nd;
+  const int i = (int)(t - e * nd);
+  inc[pos] = ((uint32_t)e << 5) | (uint32_t)i;
+}
+
+// deterministic order inside each node's list (element, then local index)
+__global__ void k_inc_sort(int64_t n_nodes, const int32_t *__restrict__ inc_ptr,
+                           uint32_t *__restrict__ inc) {
+  const int64_t n = gtid();
+  if (n >= n_nodes) return;
+  const int b = inc_ptr[n], e = inc_ptr[n + 1];
+  for (int i = b + 1; i < e; ++i) {
+    const uint32_t x = inc[i];
+    int j = i - 1;
+    while (j >= b && inc[j] > x) { inc[j + 1] = inc[j]; --j; }
+    inc[j + 1] = x;
+  }
+}
+
+// ------------------------------------------------------------------------
+// per-element threshold and keep bits (one wavefront per element)
+// ------------------------------------------------------------------------
+template <int Q, int TERM>
+__global__ __launch_bounds__(256) void k_keep(PatternArgs a) {
+  extern __shared__ uint32_t kbits[];  // 4 waves x kwords
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t e = (int64_t)blockIdx.x * 4 + w;
+  const bool live = e < a.n_cv;
+  const int nd = a.nd, nl = a.nloc, npair = nd * nd;
+  uint32_t *my = kbits + w * a.kwords;
+  for (int t = lane; t < a.kwords; t += kWave) my[t] = 0u;
+  double mx = 0.0;
+  const double *rec = a.rec + (int64_t)kRecStride * (live ? e : 0);
+  if (live) {
+    for (int p = lane; p < npair; p += kWave) {
+      double v[Q][Q];
+      pair_block<Q, TERM, GFX_FORM_PERQP, double>(rec, p / nd, p % nd, nd, a.nq, a.gref, a.qw,
+                                                  nullptr, v);
+      for (int x = 0; x < Q; ++x)
+        for (int y = 0; y < Q; ++y) mx = fmax(mx, fabs(v[x][y]));
+    }
+  }
+  mx = wave_max(mx);
+  const double thr = mx * 1e-14;
+  __syncthreads();
+  if (live && mx > 0.0) {  // an all-zero element is skipped (:5441-5442)
+    for (int p = lane; p < npair; p += kWave) {
+      const int i = p / nd, j = p % nd;
+      double v[Q][Q];
+      pair_block<Q, TERM, GFX_FORM_PERQP, double>(rec, i, j, nd, a.nq, a.gref, a.qw, nullptr, v);
+      for (int x = 0; x < Q; ++x)
+        for (int y = 0; y < Q; ++y)
+          if (fabs(v[x][y]) > thr) {
+            const int bit = (i * Q + x) * nl + (j * Q + y);
+            atomicOr(&my[bit >> 5], 1u << (bit & 31));
+          }
+    }
+  }
+  __syncthreads();
+  if (live) {
+    if (lane == 0) a.thr[e] = (mx > 0.0) ? thr : INFINITY;
+    for (int t = lane; t < a.kwords; t += kWave) a.keep[e * a.kwords + t] = my[t];
+  }
+}
+
+// ------------------------------------------------------------------------
+// node patches: sorted unique neighbour nodes, slots, presence bits
+// ------------------------------------------------------------------------
+__device__ void block_bitonic_sort(uint64_t *k, int P) {
+  for (int size = 2; size <= P; size <<= 1)
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+      for (int t = threadIdx.x; t < P / 2; t += blockDim.x) {
+        const int lo = 2 * t - (t & (stride - 1));
+        const int hi = lo + stride;
+        const bool up = ((lo & size) == 0);
+        const uint64_t x = k[lo], y = k[hi];
+        if ((x > y) == up) { k[lo] = y; k[hi] = x; }
+      }
+      __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void k_patch_build(PatternArgs a) {
+  __shared__ uint64_t keys[kMaxCand];
+  __shared__ uint16_t slotL[kMaxCand];
+  __shared__ int32_t part[256];
+  __shared__ uint32_t maskL[kMaxCand * 9 / 32 + 32];
+  __shared__ int32_t rowc[3];
+  const int64_t node = blockIdx.x;
+  if (node >= a.n_nodes) return;
+  const int nd = a.nd, Q = a.qdim, nl = a.nloc;
+  const int i0 = a.inc_ptr[node], ninc = a.inc_ptr[node + 1] - i0;
+  const int ncand = ninc * nd;
+  if (ncand > kMaxCand) { if (threadIdx.x == 0) atomicOr(a.err, 2); return; }
+  int P = 1;
+  while (P < ncand) P <<= 1;
+  for (int t = threadIdx.x; t < P; t += blockDim.x) {
+    uint64_t key = ~0ull;
+    if (t < ncand) {
+      const uint32_t inc = a.inc[i0 + t / nd];
+      const int64_t e = inc >> 5;
+      const int32_t nb = a.dofs[e * nd + (t % nd)] / Q;
+      key = ((uint64_t)(uint32_t)nb << 32) | (uint32_t)t;
+    }
+    keys[t] = key;
+  }
+  if (threadIdx.x < 3) rowc[threadIdx.x] = 0;
+  __syncthreads();
+  block_bitonic_sort(keys, P);
+  // unique flags -> slot numbers (block scan, 16 items per thread)
+  const int per = (P + blockDim.x - 1) / blockDim.x;
+  const int tb = threadIdx.x * per;
+  int s = 0;
+  for (int t = tb; t < tb + per && t < ncand; ++t)
+    s += (t == 0 || (keys[t] >> 32) != (keys[t - 1] >> 32)) ? 1 : 0;
+  part[threadIdx.x] = s;
+  __syncthreads();
+  for (int off = 1; off < (int)blockDim.x; off <<= 1) {
+    const int v = threadIdx.x >= off ? part[threadIdx.x - off] : 0;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  int run = part[threadIdx.x] - s;  // exclusive prefix
+  const int ns = part[blockDim.x - 1];
+  for (int t = tb; t < tb + per && t < ncand; ++t) {
+    const bool u = (t == 0 || (keys[t] >> 32) != (keys[t - 1] >> 32));
+    if (u) {
+      a.nbr[(int64_t)i0 * nd + run] = (int32_t)(keys[t] >> 32);
+      ++run;
+    }
+    const int orig = (int)(keys[t] & 0xffffffffu);
+    slotL[orig] = (uint16_t)(run - 1);
+  }
+  if (ns > 65535) { if (threadIdx.x == 0) atomicOr(a.err, 2); return; }
+  const int mw = (Q * Q * ns + 31) / 32;
+  for (int t = threadIdx.x; t < mw; t += blockDim.x) maskL[t] = 0u;
+  __syncthreads();
+  for (int t = threadIdx.x; t < ncand; t += blockDim.x) a.slot[(int64_t)i0 * nd + t] = slotL[t];
+  // presence: OR of the keep bits of the node's rows over its incidences
+  const int per_inc = Q * nd * Q;
+  for (int t = threadIdx.x; t < ninc * per_inc; t += blockDim.x) {
+    const int k = t / per_inc, rem = t - k * per_inc;
+    const int x = rem / (nd * Q), c = rem - x * (nd * Q);  // row component x, column c=(j,y)
+    const uint32_t inc = a.inc[i0 + k];
+    const int64_t e = inc >> 5;
+    const int i = inc & 31;
+    const int bit = (i * Q + x) * nl + c;
+    if ((a.keep[e * a.kwords + (bit >> 5)] >> (bit & 31)) & 1u) {
+      const int j = c / Q, y = c - j * Q;
+      const int pos = (x * ns + slotL[k * nd + j]) * Q + y;
+      atomicOr(&maskL[pos >> 5], 1u << (pos & 31));
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < mw; t += blockDim.x) {
+    const uint32_t m = maskL[t];
+    a.mask[(int64_t)i0 * a.mwpi + t] = m;
+    // split the word's bits by row (row x spans bits [x*ns*Q, (x+1)*ns*Q))
+    for (int x = 0; x < Q; ++x) {
+      const int lo = x * ns * Q, hi = lo + ns * Q;
+      const int wl = t * 32, wh = wl + 32;
+      const int l = max(lo, wl), h = min(hi, wh);
+      if (l < h) {
+        uint32_t mm = m >> (l - wl);
+        if (h - l < 32) mm &= (1u << (h - l)) - 1u;
+        atomicAdd(&rowc[x], __popc(mm));
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) a.nslots[node] = (uint16_t)ns;
+  if (threadIdx.x < Q) a.rowcnt[node * Q + threadIdx.x] = rowc[threadIdx.x];
+}
+
+__device__ __forceinline__ bool mask_bit(const uint32_t *m, int b) {
+  return (m[b >> 5] >> (b & 31)) & 1u;
+}
+
+__global__ __launch_bounds__(64) void k_fill_cols(PatternArgs a) {
+  const int64_t node = blockIdx.x;
+  if (node >= a.n_nodes) return;
+  const int lane = threadIdx.x, Q = a.qdim, nd = a.nd;
+  const int i0 = a.inc_ptr[node];
+  const int ns = a.nslots[node];
+  const uint32_t *m = a.mask + (int64_t)i0 * a.mwpi;
+  const int32_t *nb = a.nbr + (int64_t)i0 * nd;
+  for (int x = 0; x < Q; ++x) {
+    int64_t base = a.indptr[node * Q + x];
+    const int off = x * ns * Q, len = ns * Q;
+    for (int c0 = 0; c0 < len; c0 += kWave) {
+      const int d = c0 + lane;
+      const bool pr = d < len && mask_bit(m, off + d);
+      const uint64_t bal = __ballot(pr);
+      if (pr) {
+        const int rank = __popcll(bal & ((1ull << lane) - 1ull));
+        a.indices[base + rank] = nb[d / Q] * Q + (d % Q);
+      }
+      base += __popcll(bal);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// the assembly kernel: one workgroup (one wavefront) per node patch
+// ------------------------------------------------------------------------
+template <int ND, int Q, int TERM, int FORM, typename Real>
+__global__ __launch_bounds__(64) void k_assemble_patch(PatchArgs a) {
+  extern __shared__ double acc[];
+  // XCD-aware mapping: consecutive workgroups land on different XCDs; give
+  // each XCD a contiguous range of nodes so neighbouring patches (which share
+  // elements) share that XCD's L2
+  const int64_t wg = blockIdx.x;
+  const int64_t node = (wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3);
+  if (node >= a.n_nodes) return;
+  const int lane = threadIdx.x;
+  const int i0 = a.inc_ptr[node], ninc = a.inc_ptr[node + 1] - i0;
+  const int ns = a.nslots[node];
+  const int dense = Q * Q * ns;
+  for (int t = lane; t < dense; t += kWave) acc[t] = 0.0;
+  __syncthreads();
+  constexpr int PER = kWave / ND;  // incidences per pass
+  const int kk = lane / ND, j = lane - kk * ND;
+  for (int k0 = 0; k0 < ninc; k0 += PER) {
+    const int k = k0 + kk;
+    if (kk < PER && k < ninc) {
+      const uint32_t inc = a.inc[i0 + k];
+      const int64_t e = inc >> 5;
+      const int i = inc & 31;
+      const double *rec = a.rec + (int64_t)kRecStride * e;
+      const double thr = rec[12];
+      Real v[Q][Q];
+      pair_block<Q, TERM, FORM, Real>(rec, i, j, ND, a.nq, a.gref, a.qw, a.R, v);
+      const int s = a.slot[(int64_t)(i0 + k) * ND + j];
+      for (int x = 0; x < Q; ++x)
+        for (int y = 0; y < Q; ++y) {
+          const double val = (double)v[x][y];
+          if (fabs(val) > thr) atomicAdd(&acc[(x * ns + s) * Q + y], val);
+        }
+    }
+  }
+  __syncthreads();
+  const uint32_t *m = a.mask + (int64_t)i0 * a.mwpi;
+  for (int x = 0; x < Q; ++x) {
+    int64_t base = a.indptr[node * Q + x];
+    const int off = x * ns * Q, len = ns * Q;
+    for (int c0 = 0; c0 < len; c0 += kWave) {
+      const int d = c0 + lane;
+      const bool pr = d < len && mask_bit(m, off + d);
+      const uint64_t bal = __ballot(pr);
+      if (pr) a.vals[base + __popcll(bal & ((1ull << lane) - 1ull))] = acc[off + d];
+      base += __popcll(bal);
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// scans
+// ------------------------------------------------------------------------
+constexpr int kScanThreads = 256, kScanItems = 8, kScanBlock = kScanThreads * kScanItems;
+
+template <typename T>
+__device__ T block_exclusive_scan(T v, T *sh, T &total) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const T u = threadIdx.x >= off ? sh[threadIdx.x - off] : T(0);
+    __syncthreads();
+    sh[threadIdx.x] += u;
+    __syncthreads();
+  }
+  total = sh[kScanThreads - 1];
+  const T r = sh[threadIdx.x] - v;
+  __syncthreads();
+  return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_sums(const T *in, int64_t n, T *sums) {
+  __shared__ T sh[kScanThreads];
+  const int64_t b0 = (int64_t)blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+  T s = 0;
+  for (int t = 0; t < kScanItems; ++t)
+    if (b0 + t < n) s += in[b0 + t];
+  T total;
+  block_exclusive_scan<T>(s, sh, total);
+  if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// out[i] = offset + exclusive prefix; out[n] = total when last block
+template <typename T>
+__global__ __launch_bounds__(kScanThreads) void k_scan_apply(const T *in, int64_t n, T *out,
+                                                             const T *offs) {
+  __shared__ T sh[kScanThreads];
+  const int64_t b0 = (int64_t)blockIdx.x * kScanBlock + threadIdx.x * kScanItems;
+  T loc[kScanItems];
+  T s = 0;
+  for (int t = 0; t < kScanItems; ++t) {
+    loc[t] = (b0 + t < n) ? in[b0 + t] : T(0);
+    s += loc[t];
+  }
+  T total;
+  T run = block_exclusive_scan<T>(s, sh, total) + (offs ? offs[blockIdx.x] : T(0));
+  for (int t = 0; t < kScanItems; ++t) {
+    if (b0 + t < n) out[b0 + t] = run;
+    run += loc[t];
+  }
+  if (blockIdx.x == gridDim.x - 1 && threadIdx.x == kScanThreads - 1) out[n] = run;
+}
+
+template <typename T>
+hipError_t scan_exclusive(hipStream_t s, const T *in, T *out, int64_t n, T *tmp) {
+  const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+  if (nb <= 1) {
+    k_scan_apply<T><<<1, kScanThreads, 0, s>>>(in, n, out, nullptr);
+    return hipGetLastError();
+  }
+  T *sums = tmp, *soff = tmp + nb, *rest = tmp + 2 * nb + 1;
+  k_scan_sums<T><<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, sums);
+  hipError_t err = scan_exclusive<T>(s, sums, soff, nb, rest);
+  if (err != hipSuccess) return err;
+  k_scan_apply<T><<<(unsigned)nb, kScanThreads, 0, s>>>(in, n, out, soff);
+  return hipGetLastError();
+}
+
+__global__ void k_fill_i32(int32_t *p, int64_t n, int32_t v) {
+  const int64_t t = gtid();
+  if (t < n) p[t] = v;
+}
+
+}  // namespace
+
+int64_t scan_tmp_elems(int64_t n) {
+  int64_t tot = 0;
+  while (true) {
+    const int64_t nb = (n + kScanBlock - 1) / kScanBlock;
+    if (nb <= 1) break;
+    tot += 2 * nb + 1;
+    n = nb;
+  }
+  return tot + 16;
+}
+
+hipError_t scan_exclusive_i64(hipStream_t s, const int64_t *in, int64_t *out, int64_t n,
+                              int64_t *tmp) {
+  return scan_exclusive<int64_t>(s, in, out, n, tmp);
+}
+hipError_t scan_exclusive_i32(hipStream_t s, const int32_t *in, int32_t *out, int64_t n,
+                              int32_t *tmp) {
+  return scan_exclusive<int32_t>(s, in, out, n, tmp);
+}
+
+static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); }
+
+hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,
+                              const int32_t *cvx, const double *ca, double ca_c,
+                              const double *cb, double cb_c, const double *thr,
+                              double *rec, int *err) {
+  if (n_cv == 0) return hipSuccess;
+  k_records_tet<<<nblocks(n_cv, 256), 256, 0, s>>>(n_cv, pts, cvx, ca, ca_c, cb, cb_c, thr,
+                                                    rec, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_incidences(hipStream_t s, int64_t n_cv, int nd, int qdim,
+                             const int32_t *dofs, int64_t n_nodes, int32_t *inc_ptr,
+                             uint32_t *inc, int32_t *tmp) {
+  // tmp: n_nodes counts + n_nodes cursors + scan scratch
+  int32_t *cnt = tmp, *cursor = tmp + n_nodes, *scr = tmp + 2 * n_nodes;
+  const int64_t n = n_cv * nd;
+  k_fill_i32<<<nblocks(2 * n_nodes, 256), 256, 0, s>>>(tmp, 2 * n_nodes, 0);
+  k_inc_count<<<nblocks(n, 256), 256, 0, s>>>(n, nd, qdim, dofs, cnt);
+  hipError_t err = scan_exclusive_i32(s, cnt, inc_ptr, n_nodes, scr);
+  if (err != hipSuccess) return err;
+  k_inc_fill<<<nblocks(n, 256), 256, 0, s>>>(n, nd, qdim, dofs, inc_ptr, cursor, inc);
+  k_inc_sort<<<nblocks(n_nodes, 256), 256, 0, s>>>(n_nodes, inc_ptr, inc);
+  return hipGetLastError();
+}
+
+hipError_t launch_keep(hipStream_t s, int term, const PatternArgs &a) {
+  const unsigned nb = nblocks(a.n_cv, 4);
+  const size_t lds = 4 * (size_t)a.kwords * sizeof(uint32_t);
+  if (a.qdim == 1 && term == GFX_TERM_LAPLACE)
+    k_keep<1, GFX_TERM_LAPLACE><<<nb, 256, lds, s>>>(a);
+  else if (a.qdim == 3 && term == GFX_TERM_LAPLACE)
+    k_keep<3, GFX_TERM_LAPLACE><<<nb, 256, lds, s>>>(a);
+  else if (a.qdim == 3 && term == GFX_TERM_ISO_ELAST)
+    k_keep<3, GFX_TERM_ISO_ELAST><<<nb, 256, lds, s>>>(a);
+  else
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_patch_build(hipStream_t s, const PatternArgs &a, int max_cand) {
+  if (max_cand > kMaxCand) return hipErrorInvalidValue;
+  if (a.n_nodes == 0) return hipSuccess;
+  k_patch_build<<<(unsigned)a.n_nodes, 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_columns(hipStream_t s, const PatternArgs &a) {
+  if (a.n_nodes == 0) return hipSuccess;
+  k_fill_cols<<<(unsigned)a.n_nodes, 64, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+template <int ND, int Q, int TERM>
+static hipError_t launch_asm3(hipStream_t s, int form, int prec, const PatchArgs &a,
+                              unsigned grid, size_t lds) {
+  if (form == GFX_FORM_FACTORIZED) {
+    if (prec == GFX_PREC_FP64)
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, double><<<grid, 64, lds, s>>>(a);
+    else
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, float><<<grid, 64, lds, s>>>(a);
+  } else {
+    if (prec == GFX_PREC_FP64)
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, double><<<grid, 64, lds, s>>>(a);
+    else
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, float><<<grid, 64, lds, s>>>(a);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec,
+                                 const PatchArgs &a0, int max_nslots) {
+  if (a0.n_nodes == 0) return hipSuccess;
+  PatchArgs a = a0;
+  a.xcd_chunk = (int)((a.n_nodes + 7) / 8);
+  const unsigned grid = (unsigned)(8 * (int64_t)a.xcd_chunk);
+  const size_t lds = (size_t)a.qdim * a.qdim * max_nslots * sizeof(double);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  const int nd = a.nd, Q = a.qdim;
+  if (term == GFX_TERM_LAPLACE && Q == 1) {
+    if (nd == 4) return launch_asm3<4, 1, GFX_TERM_LAPLACE>(s, form, prec, a, grid, lds);
+    if (nd == 10) return launch_asm3<10, 1, GFX_TERM_LAPLACE>(s, form, prec, a, grid, lds);
+    if (nd == 20) return launch_asm3<20, 1, GFX_TERM_LAPLACE>(s, form, prec, a, grid, lds);
+  } else if (term == GFX_TERM_LAPLACE && Q == 3) {
+    if (nd == 4) return launch_asm3<4, 3, GFX_TERM_LAPLACE>(s, form, prec, a, grid, lds);
+    if (nd == 10) return launch_asm3<10, 3, GFX_TERM_LAPLACE>(s, form, prec, a, grid, lds);
+  } else if (term == GFX_TERM_ISO_ELAST && Q == 3) {
+    if (nd == 4) return launch_asm3<4, 3, GFX_TERM_ISO_ELAST>(s, form, prec, a, grid, lds);
+    if (nd == 10) return launch_asm3<10, 3, GFX_TERM_ISO_ELAST>(s, form, prec, a, grid, lds);
+    if (nd == 20) return launch_asm3<20, 3, GFX_TERM_ISO_ELAST>(s, form, prec, a, grid, lds);
+  }
+  return hipErrorInvalidValue;
+}
+
+}  // namespace gfx

@@ -1,0 +1,165 @@
+/*
+ * gfx.h -- C ABI of libgfx, the MI355X assembly engine behind GetFEM's
+ * generic assembly (ga_workspace::assembly) for a subset of GWFL
+ * expressions.
+ *
+ * Conventions
+ *  - Every entry point returns an int status (GFX_OK = 0); nothing throws
+ *    across the ABI.  gfx_ctx_last_error() gives the message of the last
+ *    failure on that context.  GetFEM itself reports the same conditions by
+ *    throwing gmm::gmm_error (src/gmm/gmm_except.h:55, :163).
+ *  - Pointers named h_* are host memory, d_* device memory of the context's
+ *    device; host arrays are only read during the call (GetFEM's workspace
+ *    likewise borrows its inputs, src/getfem/getfem_generic_assembly.h:271-298).
+ *  - Matrices are CSR with rows = test functions (Test_u) and sorted column
+ *    indices = trial functions (Test2_u); the same arrays are GetFEM's
+ *    col_matrix<rsvector> (CSC over trial dofs, src/gmm/gmm_vector.h:913-995)
+ *    of the transposed matrix, and gfx_plan_export_csc gives GetFEM's own
+ *    orientation.
+ *  - One gfx_ctx per host thread / stream; objects created from a context
+ *    must be used from one host thread at a time (a ga_workspace is not
+ *    thread safe either, src/getfem_models.cc:2686-2706).
+ */
+#ifndef GFX_H
+#define GFX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define GFX_OK 0
+#define GFX_E_INVALID 1            /* bad argument / size                      */
+#define GFX_E_UNSUPPORTED 2        /* expression, element or rule not supported */
+#define GFX_E_DEGENERATE_ELEMENT 3 /* det K == 0 ("Non invertible matrix",
+                                      bgeot_geometric_trans.cc:380)           */
+#define GFX_E_OOM 4                /* device allocation failed                 */
+#define GFX_E_HIP 5                /* HIP runtime error                        */
+#define GFX_E_STATE 6              /* call out of order (e.g. no pattern yet)  */
+#define GFX_E_PATTERN 7            /* a mesh patch exceeds the kernel limits   */
+
+/* ---- terms of the supported GWFL subset (after derivation, order 2) ---- */
+#define GFX_TERM_LAPLACE 0    /* a*Grad_u:Grad_Test_u (a scalar or per element)      */
+#define GFX_TERM_ISO_ELAST 1  /* lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u */
+
+/* ---- contraction forms / precisions ---- */
+#define GFX_FORM_FACTORIZED 0 /* affine element: B R_ij B^T with R_ij = sum_q w_q
+                                 grad(phi_i) grad(phi_j)^T on the reference element */
+#define GFX_FORM_PERQP 1      /* per integration point B^T D B, as ga_exec does      */
+#define GFX_PREC_FP64 0       /* element contraction in FP64                         */
+#define GFX_PREC_FP32 1       /* element contraction in FP32; geometry, threshold and
+                                 global accumulation in FP64                          */
+
+/* ---- coefficient sources ---- */
+#define GFX_COEF_CONST 0       /* one value                                 */
+#define GFX_COEF_PER_ELEMENT 1 /* one value per element (P0 field / im_data) */
+
+typedef struct {
+  int kind;            /* GFX_COEF_* */
+  double value;        /* GFX_COEF_CONST */
+  const double *h_values; /* GFX_COEF_PER_ELEMENT: n_cv values (host) */
+} gfx_coef;
+
+typedef struct gfx_ctx gfx_ctx;
+typedef struct gfx_mesh gfx_mesh;
+typedef struct gfx_mesh_fem gfx_mesh_fem;
+typedef struct gfx_mesh_im gfx_mesh_im;
+typedef struct gfx_plan gfx_plan;
+
+/* ======================= host-only utilities =============
+ * No device needed.  Reference-element data and dof numbering exactly as the
+ * plans use them. */
+/* integration rule: *npts receives the count; pts (npts*3) / w may be NULL */
+int gfx_integration_rule(const char *im_name, int *npts, double *h_pts, double *h_w);
+/* base values (nd) and reference gradients (nd*3) at x; *nd receives nd */
+int gfx_fem_eval(const char *fem_name, const double *x, int *nd, double *h_val,
+                 double *h_grad);
+/* mesh_fem::enumerate_dof (src/getfem_mesh_fem.cc:320-440) without a device:
+ * h_out receives n_cv*nd basic dofs, *ndof the number of dofs */
+int gfx_enumerate_dofs(const char *geotrans, const char *fem_name, int qdim,
+                       int64_t n_pts, const double *h_pts, int64_t n_cv,
+                       const int32_t *h_cvx, int32_t *h_out, int64_t *ndof);
+
+/* ======================= context ======================= */
+int gfx_ctx_create(int device, gfx_ctx **out);
+int gfx_ctx_destroy(gfx_ctx *ctx);
+/* use an external HIP stream (e.g. torch's current stream); NULL restores the
+ * context's own stream */
+int gfx_ctx_set_stream(gfx_ctx *ctx, void *hip_stream);
+void *gfx_ctx_stream(gfx_ctx *ctx);
+int gfx_ctx_synchronize(gfx_ctx *ctx);
+const char *gfx_ctx_last_error(gfx_ctx *ctx);
+const char *gfx_version(void);
+
+/* ======================= mesh =========================
+ * Replaces the arrays behind getfem::mesh (points, convex -> point lists),
+ * src/getfem/getfem_mesh.h; geotrans "GT_PK(3,1)" or "GT_QK(3,1)"
+ * (bgeot_geometric_trans.cc). */
+int gfx_mesh_create(gfx_ctx *ctx, const char *geotrans, int64_t n_pts,
+                    const double *h_pts, int64_t n_cv, const int32_t *h_cvx,
+                    gfx_mesh **out);
+/* move the nodes (same connectivity); h_pts is n_pts*3 host doubles */
+int gfx_mesh_set_points(gfx_mesh *m, const double *h_pts);
+int gfx_mesh_destroy(gfx_mesh *m);
+
+/* ======================= mesh_fem =====================
+ * getfem::mesh_fem with a uniform Lagrange element and Qdim
+ * (src/getfem_mesh_fem.cc:320-440, enumerate_dof): dofs are numbered in
+ * first-encounter order over elements, Qdim consecutive dofs per node. */
+int gfx_mesh_fem_create(gfx_mesh *m, const char *fem_name, int qdim,
+                        gfx_mesh_fem **out);
+/* same, with the caller's dof table: basic (component 0) dof of every element
+ * node, i.e. mesh_fem::ind_scalar_basic_dof_of_element
+ * (src/getfem/getfem_mesh_fem.h:461) for every element. */
+int gfx_mesh_fem_create_from_dofs(gfx_mesh *m, const char *fem_name, int qdim,
+                                  const int32_t *h_basic_dofs, int64_t ndof,
+                                  gfx_mesh_fem **out);
+int64_t gfx_mesh_fem_nb_dof(gfx_mesh_fem *mf);
+int gfx_mesh_fem_nb_dof_per_element(gfx_mesh_fem *mf);
+int gfx_mesh_fem_basic_dofs(gfx_mesh_fem *mf, int32_t *h_out);
+/* coordinates of the node of every basic dof (ndof/qdim x 3) */
+int gfx_mesh_fem_dof_nodes(gfx_mesh_fem *mf, double *h_out);
+int gfx_mesh_fem_destroy(gfx_mesh_fem *mf);
+
+/* ======================= mesh_im ======================
+ * getfem::mesh_im with a uniform method: "IM_TETRAHEDRON(k)",
+ * "IM_GAUSS_PARALLELEPIPED(3,k)" (src/getfem_integration.cc,
+ * src/getfem/getfem_im_list.h). */
+int gfx_mesh_im_create(gfx_mesh *m, const char *im_name, gfx_mesh_im **out);
+int gfx_mesh_im_nb_points(gfx_mesh_im *mim);
+int gfx_mesh_im_points(gfx_mesh_im *mim, double *h_pts, double *h_w);
+int gfx_mesh_im_destroy(gfx_mesh_im *mim);
+
+/* ======================= assembly plans ===============
+ * A plan is one compiled order-2 term: it replaces, for one (mesh_im,
+ * region = all elements, Test_u, Test2_u) tree, what ga_compile emits and
+ * ga_exec runs (src/getfem_generic_assembly_compile_and_exec.cc:7910-8643,
+ * :8750-9047). */
+int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term,
+                    const gfx_coef *coefs, int n_coefs, gfx_plan **out);
+int gfx_plan_set_coef(gfx_plan *p, int slot, const gfx_coef *c);
+int gfx_plan_set_form(gfx_plan *p, int form, int precision);
+/* Pattern: GetFEM's stored pattern, i.e. entries with
+ * |e| > 1e-14 * max|elem| in at least one element matrix
+ * (compile_and_exec.cc:4853-4936, :5438-5467), built on the GPU. */
+int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz);
+/* assembly(2) into the pattern's values (device resident) */
+int gfx_plan_assemble(gfx_plan *p);
+int gfx_plan_dims(gfx_plan *p, int64_t *nrows, int64_t *nnz);
+int gfx_plan_export_csr(gfx_plan *p, int64_t *h_indptr, int32_t *h_indices,
+                        double *h_vals);
+/* GetFEM orientation: CSC over trial dofs with sorted test rows, uint32 like
+ * the Python interface's gsparse export (interface/src/getfemint.cc:712-737) */
+int gfx_plan_export_csc(gfx_plan *p, int64_t *h_colptr, uint32_t *h_rows,
+                        double *h_vals);
+int gfx_plan_device_csr(gfx_plan *p, const int64_t **d_indptr,
+                        const int32_t **d_indices, double **d_vals);
+/* durations of the last gfx_plan_assemble phases (ms): geometry, contraction+scatter */
+int gfx_plan_last_timings(gfx_plan *p, float *ms_geometry, float *ms_assemble);
+int gfx_plan_destroy(gfx_plan *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,203 @@
+"""CPU oracle for GetFEM's generic-assembly path (test infrastructure only).
+
+ctypes front-end of ``oracle/gfx_oracle.c``.  Only ``tests/``,
+``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of ``bench.py`` may
+import this package; the getfem_amd product never does.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+GT_PK31, GT_QK31 = 0, 1
+FEM_PK, FEM_QK, FEM_PK_DISC0 = 0, 1, 2
+TERM_LAPLACE, TERM_ISO_ELAST, TERM_NEOHOOKEAN = 0, 1, 2
+
+_lib = None
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("oracle not built: run `make -C oracle`")
+        L = ctypes.CDLL(LIB_PATH)
+        P = ctypes.c_void_p
+        i64, i32, d = ctypes.c_int64, ctypes.c_int, ctypes.c_double
+        L.orc_im_npts.argtypes = [ctypes.c_char_p]
+        L.orc_im_rule.argtypes = [ctypes.c_char_p, P, P]
+        L.orc_fem_nb_dof.argtypes = [i32, i32]
+        L.orc_fem_nodes.argtypes = [i32, i32, P]
+        L.orc_fem_eval.argtypes = [i32, i32, P, P, P]
+        L.orc_regular_mesh.argtypes = [i32, i32, P, P]
+        L.orc_regular_mesh.restype = i64
+        L.orc_enumerate_dof.argtypes = [i32, i64, P, i64, P, i32, i32, i32, P]
+        L.orc_enumerate_dof.restype = i64
+        L.orc_element_matrix.argtypes = [P, i64, P]
+        L.orc_element_vector.argtypes = [P, i64, P]
+        L.orc_assemble_matrix.argtypes = [P, i64, i64, i32]
+        L.orc_assemble_matrix.restype = P
+        L.orc_mat_nnz.argtypes = [P]
+        L.orc_mat_nnz.restype = i64
+        L.orc_mat_ncols.argtypes = [P]
+        L.orc_mat_ncols.restype = i64
+        L.orc_mat_export_csc.argtypes = [P, P, P, P]
+        L.orc_mat_free.argtypes = [P]
+        L.orc_assemble_vector.argtypes = [P, P]
+        L.orc_compliance.argtypes = [P, P, d, d, P, P]
+        L.orc_compliance.restype = d
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class _Problem(ctypes.Structure):
+    _fields_ = [
+        ("gt", ctypes.c_int), ("n_pts", ctypes.c_int64), ("pts", ctypes.c_void_p),
+        ("n_cv", ctypes.c_int64), ("cvx", ctypes.c_void_p),
+        ("fem_kind", ctypes.c_int), ("fem_k", ctypes.c_int), ("qdim", ctypes.c_int),
+        ("dofs", ctypes.c_void_p), ("ndof", ctypes.c_int64),
+        ("nq", ctypes.c_int), ("qp", ctypes.c_void_p), ("qw", ctypes.c_void_p),
+        ("term", ctypes.c_int), ("coef_a", ctypes.c_void_p), ("coef_b", ctypes.c_void_p),
+        ("scal_a", ctypes.c_double), ("scal_b", ctypes.c_double),
+        ("U", ctypes.c_void_p),
+    ]
+
+
+def im_rule(name):
+    n = lib().orc_im_npts(name.encode())
+    if n <= 0:
+        raise ValueError("unknown integration method " + name)
+    pts = np.zeros((n, 3))
+    w = np.zeros(n)
+    lib().orc_im_rule(name.encode(), _ptr(pts), _ptr(w))
+    return pts, w
+
+
+def fem_nodes(kind, k):
+    nd = lib().orc_fem_nb_dof(kind, k)
+    out = np.zeros((nd, 3))
+    lib().orc_fem_nodes(kind, k, _ptr(out))
+    return out
+
+
+def fem_eval(kind, k, x):
+    nd = lib().orc_fem_nb_dof(kind, k)
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    val = np.zeros(nd)
+    grad = np.zeros((nd, 3))
+    lib().orc_fem_eval(kind, k, _ptr(x), _ptr(val), _ptr(grad))
+    return val, grad
+
+
+def regular_mesh(gt, n):
+    npts = (n + 1) ** 3
+    nv = 4 if gt == GT_PK31 else 8
+    ncv = 6 * n ** 3 if gt == GT_PK31 else n ** 3
+    pts = np.zeros((npts, 3))
+    cvx = np.zeros((ncv, nv), dtype=np.int32)
+    got = lib().orc_regular_mesh(gt, n, _ptr(pts), _ptr(cvx))
+    return pts[:got].copy(), cvx
+
+
+def enumerate_dof(gt, pts, cvx, fem_kind, fem_k, qdim):
+    nd = lib().orc_fem_nb_dof(fem_kind, fem_k)
+    pts = np.ascontiguousarray(pts, dtype=np.float64)
+    cvx = np.ascontiguousarray(cvx, dtype=np.int32)
+    dofs = np.zeros((cvx.shape[0], nd), dtype=np.int32)
+    ndof = lib().orc_enumerate_dof(gt, pts.shape[0], _ptr(pts), cvx.shape[0], _ptr(cvx),
+                                   fem_kind, fem_k, qdim, _ptr(dofs))
+    return dofs, int(ndof)
+
+
+class Problem:
+    """Holds the arrays of one assembly problem alive for the C side."""
+
+    def __init__(self, gt, pts, cvx, fem_kind, fem_k, qdim, dofs, ndof, im_name,
+                 term, coef_a=None, coef_b=None, scal_a=1.0, scal_b=0.0, U=None):
+        self.pts = np.ascontiguousarray(pts, dtype=np.float64)
+        self.cvx = np.ascontiguousarray(cvx, dtype=np.int32)
+        self.dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+        self.qp, self.qw = im_rule(im_name)
+        self.coef_a = None if coef_a is None else np.ascontiguousarray(coef_a, dtype=np.float64)
+        self.coef_b = None if coef_b is None else np.ascontiguousarray(coef_b, dtype=np.float64)
+        self.U = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+        self.nd = lib().orc_fem_nb_dof(fem_kind, fem_k)
+        self.nloc = self.nd * qdim
+        self.ndof = ndof
+        self.s = _Problem(gt, self.pts.shape[0], _ptr(self.pts), self.cvx.shape[0], _ptr(self.cvx),
+                          fem_kind, fem_k, qdim, _ptr(self.dofs), ndof,
+                          self.qw.shape[0], _ptr(self.qp), _ptr(self.qw), term,
+                          _ptr(self.coef_a), _ptr(self.coef_b), scal_a, scal_b, _ptr(self.U))
+
+    @property
+    def n_cv(self):
+        return self.cvx.shape[0]
+
+    def element_matrix(self, cv):
+        out = np.zeros(self.nloc * self.nloc)
+        r = lib().orc_element_matrix(ctypes.byref(self.s), cv, _ptr(out))
+        if r < 0:
+            raise ValueError("unsupported term")
+        return out.reshape(self.nloc, self.nloc).T.copy()  # rows = test
+
+    def element_vector(self, cv):
+        out = np.zeros(self.nloc)
+        if lib().orc_element_vector(ctypes.byref(self.s), cv, _ptr(out)) < 0:
+            raise ValueError("unsupported term")
+        return out
+
+    def assemble_csc(self, cv_begin=0, cv_end=None, nthreads=1):
+        """GetFEM's col_matrix<rsvector>: returns (colptr, rows, vals)."""
+        if cv_end is None:
+            cv_end = self.n_cv
+        m = lib().orc_assemble_matrix(ctypes.byref(self.s), cv_begin, cv_end, nthreads)
+        try:
+            nnz = lib().orc_mat_nnz(m)
+            colptr = np.zeros(self.ndof + 1, dtype=np.int64)
+            rows = np.zeros(nnz, dtype=np.int32)
+            vals = np.zeros(nnz)
+            lib().orc_mat_export_csc(m, _ptr(colptr), _ptr(rows), _ptr(vals))
+        finally:
+            lib().orc_mat_free(m)
+        return colptr, rows, vals
+
+    def assemble_timed(self, cv_begin, cv_end, nthreads=1):
+        """Assemble and drop the result; returns the stored nnz."""
+        m = lib().orc_assemble_matrix(ctypes.byref(self.s), cv_begin, cv_end, nthreads)
+        nnz = lib().orc_mat_nnz(m)
+        lib().orc_mat_free(m)
+        return nnz
+
+    def assemble_vector(self):
+        V = np.zeros(self.ndof)
+        if lib().orc_assemble_vector(ctypes.byref(self.s), _ptr(V)) < 0:
+            raise ValueError("unsupported term")
+        return V
+
+    def compliance(self, rho, emin=1e-3, penal=3.0, grads=True):
+        rho = np.ascontiguousarray(rho, dtype=np.float64)
+        dr = np.zeros(self.n_cv) if grads else None
+        dx = np.zeros_like(self.pts) if grads else None
+        J = lib().orc_compliance(ctypes.byref(self.s), _ptr(rho), emin, penal, _ptr(dr), _ptr(dx))
+        return J, dr, dx
+
+
+def csc_to_csr(colptr, rows, vals, n):
+    """Transpose a CSC (n x n) into CSR arrays of the same matrix."""
+    import scipy.sparse as sp
+    m = sp.csc_matrix((vals, rows, colptr), shape=(n, n))
+    r = m.tocsr()
+    r.sort_indices()
+    return r.indptr.astype(np.int64), r.indices.astype(np.int32), r.data

@@ -1,0 +1,102 @@
+/*
+ * gfx_oracle.h -- CPU restatement of GetFEM's generic-assembly path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the getfem_amd product links,
+ * loads or calls this library; it is the checker used by tests/, by the
+ * cpu_baseline leg of bench.py and by __graft_entry__.smoke().
+ *
+ * Every function follows a named piece of the reference (file:line into
+ * the GetFEM sources); the header of gfx_oracle.c lists which checks pin
+ * each function.
+ */
+#ifndef GFX_ORACLE_H
+#define GFX_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* geometric transformations */
+enum { ORC_GT_PK31 = 0, ORC_GT_QK31 = 1 };
+/* finite element kinds */
+enum { ORC_FEM_PK = 0, ORC_FEM_QK = 1, ORC_FEM_PK_DISC0 = 2 };
+/* terms (bilinear / nonlinear forms of the supported GWFL subset) */
+enum {
+  ORC_TERM_LAPLACE = 0,      /* a * Grad_u:Grad_Test_u                    */
+  ORC_TERM_ISO_ELAST = 1,    /* lambda*Div_u*Div_Test_u
+                                + 2*mu*Sym(Grad_u):Grad_Test_u             */
+  ORC_TERM_NEOHOOKEAN = 2    /* ((Id+Grad_u)*Bonet_PK2(Grad_u,[l;m])):Grad_Test_u */
+};
+
+/* ---- integration methods (getfem_integration.cc, getfem_im_list.h) ---- */
+int orc_im_npts(const char *name);
+int orc_im_rule(const char *name, double *pts /* n*3 */, double *w /* n */);
+
+/* ---- Lagrange elements (getfem_fem.cc:726-767, :795-826) ---- */
+int orc_fem_nb_dof(int kind, int k);
+int orc_fem_nodes(int kind, int k, double *nodes /* nd*3 */);
+int orc_fem_eval(int kind, int k, const double *x, double *val, double *grad);
+
+/* ---- regular_unit_mesh (getfem_regular_meshes.cc:24-78, :102-138, :237-285) ---- */
+int64_t orc_regular_mesh(int gt, int n, double *pts /* (n+1)^3*3 */,
+                         int32_t *cvx /* 6n^3*4 or n^3*8 */);
+
+/* ---- mesh_fem::enumerate_dof (getfem_mesh_fem.cc:320-440) ---- */
+int64_t orc_enumerate_dof(int gt, int64_t n_pts, const double *pts,
+                          int64_t n_cv, const int32_t *cvx,
+                          int fem_kind, int fem_k, int qdim,
+                          int32_t *basic_dofs /* n_cv * nd */);
+
+typedef struct {
+  int gt;                    /* ORC_GT_* */
+  int64_t n_pts;
+  const double *pts;         /* n_pts*3 */
+  int64_t n_cv;
+  const int32_t *cvx;        /* n_cv * nv */
+  int fem_kind, fem_k, qdim; /* the unknown u */
+  const int32_t *dofs;       /* basic dof of each element node, n_cv*nd */
+  int64_t ndof;
+  int nq;                    /* integration points on the reference element */
+  const double *qp;          /* nq*3 */
+  const double *qw;          /* nq */
+  int term;                  /* ORC_TERM_* */
+  const double *coef_a;      /* per element (a / lambda) or NULL -> scal_a */
+  const double *coef_b;      /* per element (mu) or NULL -> scal_b */
+  double scal_a, scal_b;
+  const double *U;           /* state vector (ndof), nonlinear terms */
+} orc_problem;
+
+/* element matrix (column-major, rows = Test_u), accumulated over the
+ * integration points as ga_instruction_matrix_assembly_standard_vector::exec
+ * does (compile_and_exec.cc:5426-5436). Returns n_loc. */
+int orc_element_matrix(const orc_problem *p, int64_t cv, double *elem);
+int orc_element_vector(const orc_problem *p, int64_t cv, double *elem);
+
+/* global matrix as GetFEM's col_matrix<rsvector<double>>: built by
+ * add_elem_matrix with the 1e-14*max|elem| insertion threshold
+ * (compile_and_exec.cc:4853-4936, :5438-5467). nthreads > 1 mirrors
+ * model::assembly's per-thread private matrices summed afterwards
+ * (getfem_models.cc:2686-2730). */
+void *orc_assemble_matrix(const orc_problem *p, int64_t cv_begin,
+                          int64_t cv_end, int nthreads);
+int64_t orc_mat_nnz(const void *m);
+int64_t orc_mat_ncols(const void *m);
+/* CSC over (column = trial dof, sorted rows = test dof) */
+void orc_mat_export_csc(const void *m, int64_t *colptr, int32_t *rows,
+                        double *vals);
+void orc_mat_free(void *m);
+
+/* residual vector assembly, ga_instruction_vector_assembly_mf
+ * (compile_and_exec.cc:4669-4735): no threshold */
+int orc_assemble_vector(const orc_problem *p, double *V);
+
+/* SIMP compliance J = u^T K(rho) u on P1 vector fields and its gradients
+ * with respect to rho (per element) and to the nodal coordinates. */
+double orc_compliance(const orc_problem *p, const double *rho, double emin,
+                      double penal, double *dJ_drho, double *dJ_dX);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,92 @@
+"""libgfx (HIP, gfx950) against the oracle on identical inputs: the integer
+pattern bit-exact, FP64 values within 1e-12 relative Frobenius, FP32 element
+contractions within 1e-6."""
+import numpy as np
+import pytest
+
+import getfem_amd as ga
+from tests import problems
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-12
+TOL_FP32 = 1e-6
+
+
+def gpu_plan(ctx, case):
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", case.pts, case.cvx)
+    mf = ga.MeshFem(mesh, case.fem, case.qdim)
+    mim = ga.MeshIm(mesh, case.im)
+    term = ga.TERM_LAPLACE if case.term == 0 else ga.TERM_ISO_ELAST
+    plan = ga.Plan(mf, mim, term, case.coefs())
+    plan.build_pattern()
+    return plan, (mesh, mf, mim)
+
+
+CASES = {
+    "c1_p1_poisson_n20": lambda: problems.c1(20),
+    "p1_poisson_n6_coef": lambda: problems.Case(6, 1, 1, "IM_TETRAHEDRON(1)", 0,
+                                               a=np.linspace(0.5, 2.0, 6 * 6 ** 3)),
+    "p2_poisson_n5": lambda: problems.Case(5, 2, 1, "IM_TETRAHEDRON(5)", 0),
+    "p2_elast_n4": lambda: problems.c2(4),
+    "p2_elast_n8": lambda: problems.c2(8),
+    "p2_elast_n3_unjittered": lambda: problems.Case(3, 2, 3, "IM_TETRAHEDRON(5)", 1,
+                                                   lam=np.full(162, 2.0), mu=np.full(162, 0.7),
+                                                   jitter=False),
+    "p1_elast_n5": lambda: problems.Case(5, 1, 3, "IM_TETRAHEDRON(1)", 1,
+                                        lam=np.full(750, 1.3), mu=np.full(750, 0.4)),
+    "p3_elast_n3": lambda: problems.c5(3),
+}
+
+
+@pytest.mark.parametrize("name", sorted(CASES))
+@pytest.mark.parametrize("form", [ga.FORM_FACTORIZED, ga.FORM_PERQP])
+def test_fp64_parity(ctx, name, form):
+    case = CASES[name]()
+    print("case", name, "form", form, "ndof", case.ndof, flush=True)
+    ip_ref, ix_ref, v_ref = case.oracle_csr()
+    plan, keep = gpu_plan(ctx, case)
+    plan.set_form(form, ga.PREC_FP64)
+    plan.assemble()
+    ip, ix, v = plan.csr()
+    assert np.array_equal(ip, ip_ref), "row pointers differ"
+    assert np.array_equal(ix, ix_ref), "column indices differ"
+    err = problems.rel_frob(v, v_ref)
+    print("rel frobenius error", err)
+    assert err <= TOL_FP64
+
+
+@pytest.mark.parametrize("name", ["p2_elast_n8", "c1_p1_poisson_n20", "p3_elast_n3"])
+@pytest.mark.parametrize("form", [ga.FORM_FACTORIZED, ga.FORM_PERQP])
+def test_fp32_element_error(ctx, name, form):
+    case = CASES[name]()
+    ip_ref, ix_ref, v_ref = case.oracle_csr()
+    plan, keep = gpu_plan(ctx, case)
+    plan.set_form(form, ga.PREC_FP32)
+    plan.assemble()
+    ip, ix, v = plan.csr()
+    assert np.array_equal(ix, ix_ref)
+    err = problems.rel_frob(v, v_ref)
+    print("fp32 element form: rel frobenius error vs FP64 oracle", err)
+    assert err <= TOL_FP32
+
+
+def test_csc_export_is_getfem_orientation(ctx):
+    case = CASES["p2_elast_n4"]()
+    plan, keep = gpu_plan(ctx, case)
+    plan.assemble()
+    colptr, rows, vals = plan.csc()
+    cp, r, v = case.oracle_problem().assemble_csc()
+    assert np.array_equal(colptr, cp)
+    assert np.array_equal(rows.astype(np.int64), r.astype(np.int64))
+    assert problems.rel_frob(vals, v) <= TOL_FP64
+
+
+def test_repeated_assembly_is_stable(ctx):
+    case = CASES["p2_elast_n4"]()
+    plan, keep = gpu_plan(ctx, case)
+    plan.assemble()
+    a = plan.csr()[2]
+    plan.assemble()
+    b = plan.csr()[2]
+    assert problems.rel_frob(a, b) <= 1e-14
