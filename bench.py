@@ -1,0 +1,225 @@
+#!/usr/bin/env python3
+"""Benchmark: P2-tet isotropic elasticity assembly on one MI355X.
+
+Workload: lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u, vector
+FEM_PK(3,2), IM_TETRAHEDRON(5), on the jittered unit cube split into 64^3
+cubes x 6 tetrahedra (1,572,864 tets, 6,440,067 dofs), per-element lognormal
+lambda, mu.  One step = one order-2 assembly (geometry kernel + contraction /
+scatter kernel) into the prebuilt pattern, inputs resident in HBM.
+
+Prints ONE JSON line (rank 0).  Extra keys: roofline (dominant kernel),
+cpu_baseline (the oracle port on host cores over a bounded sample), forms
+(every contraction form / precision with its error against FP64).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "P2-tet elasticity assembly elems/s + HBM GB/s; FP32 vs FP64 form at stated error"
+HBM_PEAK_GBS = 8000.0
+REFERENCE_NNZ_64 = 544886097  # GetFEM's stored nnz at 64^3x6, measured with the reference
+
+
+def bytes_per_element(n_nodes, n_geo, n_coef, n_loc, n_state=0):
+    """Algorithmic bytes per element: connectivity + geometry + coefficients +
+    an FP64 read, an FP64 write and a 4-byte CSR position per element-matrix
+    entry."""
+    return 4 * n_nodes + 24 * n_geo + 8 * n_coef + 8 * n_state + 20 * n_loc * n_loc
+
+
+def run_gpu(args):
+    import torch
+
+    import getfem_amd as ga
+    from getfem_amd import meshes
+
+    n = args.n
+    t0 = time.perf_counter()
+    pts, cvx, lam, mu = meshes.c2_inputs(n)
+    t_inputs = time.perf_counter() - t0
+    n_cv = cvx.shape[0]
+    print("inputs: n=%d, %d tets (%.1f s)" % (n, n_cv, t_inputs), file=sys.stderr, flush=True)
+
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream(dev)
+    ctx = ga.Context(0)
+    ctx.set_stream(stream.cuda_stream)
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    t0 = time.perf_counter()
+    mf = ga.MeshFem(mesh, "FEM_PK(3,2)", 3)
+    t_dofs = time.perf_counter() - t0
+    mim = ga.MeshIm(mesh, "IM_TETRAHEDRON(5)")
+    plan = ga.Plan(mf, mim, ga.TERM_ISO_ELAST, [lam, mu])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    nnz = plan.build_pattern()
+    torch.cuda.synchronize()
+    t_pattern = time.perf_counter() - t0
+    ndof = mf.nbdof()
+    print("dofs %d (%.1f s host), pattern nnz %d (%.3f s GPU)" % (ndof, t_dofs, nnz, t_pattern),
+          file=sys.stderr, flush=True)
+
+    b_elem = bytes_per_element(10, 4, 2, 30)
+    b_patch = 4 * 10 + 20 * 30 * 30  # the contraction/scatter kernel's share
+    variants = [("fp64_factorized", ga.FORM_FACTORIZED, ga.PREC_FP64),
+                ("fp64_perqp", ga.FORM_PERQP, ga.PREC_FP64),
+                ("fp32_factorized", ga.FORM_FACTORIZED, ga.PREC_FP32),
+                ("fp32_perqp", ga.FORM_PERQP, ga.PREC_FP32)]
+    if args.form != "all":
+        variants = [v for v in variants if v[0] == args.form]
+    forms = {}
+    ref_vals = None
+    for name, form, prec in variants:
+        plan.set_form(form, prec)
+        for _ in range(args.warmup):
+            plan.assemble()
+        torch.cuda.synchronize()
+        plan.set_timing(True)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        for _ in range(args.steps):
+            plan.assemble()
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_step = e0.elapsed_time(e1) / args.steps
+        calls, ms_geo, ms_asm = plan.timing_summary()
+        plan.set_timing(False)
+        ms_geo /= calls
+        ms_asm /= calls
+        _, _, vals = plan.torch_csr(dev)
+        if ref_vals is None and prec == ga.PREC_FP64:
+            ref_vals = vals.clone()
+        err = None
+        if ref_vals is not None:
+            err = float(torch.linalg.vector_norm(vals - ref_vals) / torch.linalg.vector_norm(ref_vals))
+        del vals
+        forms[name] = {
+            "ms_per_step": ms_step,
+            "elems_per_s": n_cv / (ms_step * 1e-3),
+            "gbs_algorithmic": b_elem * n_cv / (ms_step * 1e-3) / 1e9,
+            "ms_geometry_kernel": ms_geo,
+            "ms_assemble_kernel": ms_asm,
+            "rel_frobenius_err_vs_fp64": err,
+        }
+        print(name, json.dumps(forms[name]), file=sys.stderr, flush=True)
+    # integer checks of the pattern at full size
+    ip, ix, _ = plan.torch_csr(dev)
+    rowlen = ip[1:] - ip[:-1]
+    pattern = {
+        "nnz": int(nnz),
+        "nnz_equals_reference": (int(nnz) == REFERENCE_NNZ_64) if n == 64 else None,
+        "ndof": int(ndof),
+        "build_s": t_pattern,
+        "dof_enumeration_s_host": t_dofs,
+        "max_row_len": int(rowlen.max().item()),
+    }
+    del ip, ix
+    head = forms[variants[0][0]]
+    out = {
+        "metric": METRIC,
+        "value": head["elems_per_s"],
+        "unit": "elems/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64" if "fp64" in variants[0][0] else "f32",
+        "data": "synthetic (seeded jittered mesh, lognormal lambda/mu)",
+        "config": {
+            "workload": "P2-tet isotropic elasticity, %d^3x6 = %d tets, %d dofs, IM_TETRAHEDRON(5), "
+                        "per-element lambda/mu; step = geometry + contraction/scatter into the "
+                        "prebuilt CSR" % (n, n_cv, ndof),
+            "form": variants[0][0],
+            "elements": n_cv,
+            "nnz": int(nnz),
+        },
+        "hbm_gbs_algorithmic": head["gbs_algorithmic"],
+        "roofline": {
+            "bound": "hbm",
+            "achieved": b_patch * n_cv / (head["ms_assemble_kernel"] * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": b_patch * n_cv / (head["ms_assemble_kernel"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_assemble_patch",
+            "bytes_per_element": b_patch,
+            "kernel_ms": head["ms_assemble_kernel"],
+        },
+        "forms": forms,
+        "pattern": pattern,
+    }
+    return out, (pts, cvx, lam, mu)
+
+
+def cpu_baseline(inputs, threads):
+    """The oracle (a C port of GetFEM's per-element/per-point FP64 algorithm
+    with its threshold and sorted-column insertion) on a bounded sample."""
+    import oracle
+    pts, cvx, lam, mu = inputs
+    dofs, ndof = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 2, 3)
+    prob = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 2, 3, dofs, ndof,
+                          "IM_TETRAHEDRON(5)", oracle.TERM_ISO_ELAST, coef_a=lam, coef_b=mu)
+    res = {}
+    for th, m in threads:
+        m = min(m, cvx.shape[0])
+        t0 = time.perf_counter()
+        prob.assemble_timed(0, m, th)
+        dt = time.perf_counter() - t0
+        res[th] = (m / dt, m, dt)
+        print("cpu baseline: %d threads, %d elements, %.2f s, %.0f elems/s" % (th, m, dt, m / dt),
+              file=sys.stderr, flush=True)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--form", default="all")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=0, help="elements for the multi-thread CPU sample")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+    out, inputs = run_gpu(args)
+    if not args.no_cpu:
+        try:
+            import oracle
+            if not os.path.exists(oracle.LIB_PATH):
+                oracle.build()
+            cores = min(16, os.cpu_count() or 1)
+            n_cv = inputs[1].shape[0]
+            big = args.cpu_sample or min(n_cv, 12000 * cores)
+            res = cpu_baseline(inputs, [(1, min(20000, n_cv)), (cores, big)])
+            v1 = res[1][0]
+            vm, m, dt = res[cores]
+            out["cpu_baseline"] = {
+                "value": vm, "unit": "elems/s", "cores": cores, "kind": "port",
+                "sample": "first %d of %d elements (%.1f s), OpenMP element partition with "
+                          "per-thread matrices summed; 1 core: %.0f elems/s on the first %d "
+                          "elements" % (m, n_cv, dt, v1, res[1][1]),
+                "value_1core": v1,
+            }
+        except Exception as exc:  # report, never hide
+            out["cpu_baseline"] = {"value": None, "error": repr(exc)}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -64,7 +64,11 @@ SIGNATURES = {
     "gfx_plan_export_csr": (_i32, [_P, _P, _P, _P]),
     "gfx_plan_export_csc": (_i32, [_P, _P, _P, _P]),
     "gfx_plan_device_csr": (_i32, [_P, _pp, _pp, _pp]),
+    "gfx_plan_copy_csr_device": (_i32, [_P, _P, _P, _P]),
+    "gfx_plan_set_timing": (_i32, [_P, _i32]),
     "gfx_plan_last_timings": (_i32, [_P, ctypes.POINTER(_f32), ctypes.POINTER(_f32)]),
+    "gfx_plan_timing_summary": (_i32, [_P, ctypes.POINTER(_i32), ctypes.POINTER(_f64),
+                                       ctypes.POINTER(_f64)]),
     "gfx_plan_destroy": (_i32, [_P]),
 }
 

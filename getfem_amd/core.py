@@ -203,6 +203,34 @@ class Plan:
               self.ctx.h)
         return ip.value, ix.value, v.value
 
+    def copy_csr_device(self, d_indptr=None, d_indices=None, d_vals=None):
+        """Device-to-device copy into caller buffers (raw device addresses,
+        e.g. torch tensors' data_ptr())."""
+        v = lambda x: ctypes.c_void_p(x) if x else None  # noqa: E731
+        check(lib().gfx_plan_copy_csr_device(self.h, v(d_indptr), v(d_indices), v(d_vals)),
+              self.ctx.h)
+
+    def torch_csr(self, device="cuda"):
+        """The CSR arrays as torch tensors (copied on the device)."""
+        import torch
+        n, nnz = self.dims()
+        ip = torch.empty(n + 1, dtype=torch.int64, device=device)
+        ix = torch.empty(nnz, dtype=torch.int32, device=device)
+        vv = torch.empty(nnz, dtype=torch.float64, device=device)
+        self.copy_csr_device(ip.data_ptr(), ix.data_ptr(), vv.data_ptr())
+        self.ctx.synchronize()
+        return ip, ix, vv
+
+    def set_timing(self, enable=True):
+        check(lib().gfx_plan_set_timing(self.h, 1 if enable else 0), self.ctx.h)
+
+    def timing_summary(self):
+        """(calls, summed geometry ms, summed contraction+scatter ms)."""
+        n, g, a = ctypes.c_int(), ctypes.c_double(), ctypes.c_double()
+        check(lib().gfx_plan_timing_summary(self.h, ctypes.byref(n), ctypes.byref(g), ctypes.byref(a)),
+              self.ctx.h)
+        return n.value, g.value, a.value
+
     def last_timings(self):
         g, a = ctypes.c_float(), ctypes.c_float()
         check(lib().gfx_plan_last_timings(self.h, ctypes.byref(g), ctypes.byref(a)), self.ctx.h)

@@ -348,6 +348,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
+  for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   delete p;
   return GFX_OK;
 }
@@ -535,13 +536,68 @@ int gfx_plan_assemble(gfx_plan *p) {
   if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
   if (set_device(c)) return GFX_E_HIP;
   int r;
+  hipEvent_t *ev = nullptr;
+  if (p->timing) {
+    const size_t need = 3 * (size_t)(p->n_timed + 1);
+    while (p->ev.size() < need) {
+      hipEvent_t e;
+      HIPCHK(c, hipEventCreate(&e));
+      p->ev.push_back(e);
+    }
+    ev = &p->ev[3 * (size_t)p->n_timed];
+    p->n_timed++;
+    HIPCHK(c, hipEventRecord(ev[0], c->stream));
+  }
   if ((r = plan_records(p, true))) return r;
+  if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   PatchArgs a{};
   a.n_nodes = p->n_nodes; a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot;
   a.nslots = p->d_nslots; a.mask = p->d_mask; a.mwpi = p->mwpi; a.indptr = p->d_indptr;
   a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
   a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals;
   HIPCHK(c, launch_assemble_patch(c->stream, p->term, p->form, p->precision, a, p->max_nslots));
+  if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  return GFX_OK;
+}
+
+int gfx_plan_set_timing(gfx_plan *p, int enable) {
+  if (!p) return GFX_E_INVALID;
+  p->timing = enable != 0;
+  p->n_timed = 0;
+  return GFX_OK;
+}
+
+int gfx_plan_timing_summary(gfx_plan *p, int *n_calls, double *ms_geometry, double *ms_assemble) {
+  if (!p) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (set_device(c)) return GFX_E_HIP;
+  double g = 0.0, a = 0.0;
+  for (int k = 0; k < p->n_timed; ++k) {
+    hipEvent_t *ev = &p->ev[3 * (size_t)k];
+    HIPCHK(c, hipEventSynchronize(ev[2]));
+    float x = 0.f, y = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&x, ev[0], ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&y, ev[1], ev[2]));
+    g += x; a += y;
+  }
+  if (n_calls) *n_calls = p->n_timed;
+  if (ms_geometry) *ms_geometry = g;
+  if (ms_assemble) *ms_assemble = a;
+  p->n_timed = 0;
+  return GFX_OK;
+}
+
+int gfx_plan_copy_csr_device(gfx_plan *p, int64_t *d_indptr, int32_t *d_indices, double *d_vals) {
+  if (!p) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
+  if (set_device(c)) return GFX_E_HIP;
+  hipStream_t s = c->stream;
+  if (d_indptr)
+    HIPCHK(c, hipMemcpyAsync(d_indptr, p->d_indptr, 8 * ((size_t)p->ndof + 1), hipMemcpyDeviceToDevice, s));
+  if (d_indices)
+    HIPCHK(c, hipMemcpyAsync(d_indices, p->d_indices, 4 * (size_t)p->nnz, hipMemcpyDeviceToDevice, s));
+  if (d_vals) HIPCHK(c, hipMemcpyAsync(d_vals, p->d_vals, 8 * (size_t)p->nnz, hipMemcpyDeviceToDevice, s));
   return GFX_OK;
 }
 
@@ -599,8 +655,16 @@ int gfx_plan_device_csr(gfx_plan *p, const int64_t **d_indptr, const int32_t **d
 
 int gfx_plan_last_timings(gfx_plan *p, float *ms_geometry, float *ms_assemble) {
   if (!p) return GFX_E_INVALID;
-  if (ms_geometry) *ms_geometry = p->last_ms[0];
-  if (ms_assemble) *ms_assemble = p->last_ms[1];
+  gfx_ctx *c = p->ctx;
+  if (!p->timing || p->n_timed == 0) return fail(c, GFX_E_STATE, "no timed call recorded");
+  if (set_device(c)) return GFX_E_HIP;
+  hipEvent_t *ev = &p->ev[3 * (size_t)(p->n_timed - 1)];
+  HIPCHK(c, hipEventSynchronize(ev[2]));
+  float g = 0.f, a = 0.f;
+  HIPCHK(c, hipEventElapsedTime(&g, ev[0], ev[1]));
+  HIPCHK(c, hipEventElapsedTime(&a, ev[1], ev[2]));
+  if (ms_geometry) *ms_geometry = g;
+  if (ms_assemble) *ms_assemble = a;
   return GFX_OK;
 }
 

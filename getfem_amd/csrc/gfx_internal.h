@@ -137,5 +137,7 @@ struct gfx_plan {
   int *d_err = nullptr;
   int form = GFX_FORM_FACTORIZED;
   int precision = GFX_PREC_FP64;
-  float last_ms[4] = {0, 0, 0, 0};
+  bool timing = false;
+  std::vector<hipEvent_t> ev;  // 3 events per timed call
+  int n_timed = 0;
 };

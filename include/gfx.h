@@ -155,8 +155,19 @@ int gfx_plan_export_csc(gfx_plan *p, int64_t *h_colptr, uint32_t *h_rows,
                         double *h_vals);
 int gfx_plan_device_csr(gfx_plan *p, const int64_t **d_indptr,
                         const int32_t **d_indices, double **d_vals);
-/* durations of the last gfx_plan_assemble phases (ms): geometry, contraction+scatter */
+/* copy the CSR arrays into caller-owned device buffers of the same device
+ * (stream-ordered on the context's stream; any pointer may be NULL) */
+int gfx_plan_copy_csr_device(gfx_plan *p, int64_t *d_indptr, int32_t *d_indices,
+                             double *d_vals);
+/* record HIP events around the phases of every gfx_plan_assemble call (on
+ * the context's stream, no host synchronisation); enabling resets the record.
+ * gfx_plan_last_timings: durations (ms) of the last call, geometry kernel and
+ * contraction + scatter kernel; gfx_plan_timing_summary: number of recorded
+ * calls and summed durations, then resets the record. */
+int gfx_plan_set_timing(gfx_plan *p, int enable);
 int gfx_plan_last_timings(gfx_plan *p, float *ms_geometry, float *ms_assemble);
+int gfx_plan_timing_summary(gfx_plan *p, int *n_calls, double *ms_geometry,
+                            double *ms_assemble);
 int gfx_plan_destroy(gfx_plan *p);
 
 #ifdef __cplusplus

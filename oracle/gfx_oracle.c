@@ -615,19 +615,35 @@ static void nh_bonet(const double *F, double lam, double mu, double *S,
         }
 }
 
-/* real gradients at integration point q: g[i*3+k] */
-static double element_geometry(const orc_problem *p, const orc_tables *t,
-                               int64_t cv, int q, double *g) {
-  double G[24], B[9];
-  for (int v = 0; v < t->nv; ++v)
-    for (int c = 0; c < 3; ++c) G[3 * v + c] = p->pts[3 * (int64_t)p->cvx[(int64_t)t->nv * cv + v] + c];
-  double J = jacobian(G, t->nv, t->gN + q * t->nv * 3, B, NULL);
+/* real gradients at integration point q: g[i*3+k].  For the affine
+ * transformation K, J and B are computed at the first point only and reused
+ * (compile_and_exec.cc:8836-8837); per point for the trilinear one. */
+typedef struct { int64_t cv; double B[9]; double J; } geo_cache;
+
+static double element_geometry_c(const orc_problem *p, const orc_tables *t,
+                                 int64_t cv, int q, double *g, geo_cache *gc) {
+  double G[24], Bl[9];
+  double *B = gc ? gc->B : Bl;
+  double J;
+  if (gc && p->gt == ORC_GT_PK31 && gc->cv == cv && q > 0) {
+    J = gc->J;
+  } else {
+    for (int v = 0; v < t->nv; ++v)
+      for (int c = 0; c < 3; ++c) G[3 * v + c] = p->pts[3 * (int64_t)p->cvx[(int64_t)t->nv * cv + v] + c];
+    J = jacobian(G, t->nv, t->gN + q * t->nv * 3, B, NULL);
+    if (gc) { gc->cv = cv; gc->J = J; }
+  }
   const double *gr = t->grad + q * t->nd * 3;
   for (int i = 0; i < t->nd; ++i)
     for (int k = 0; k < 3; ++k)
       g[3 * i + k] = B[3 * k + 0] * gr[3 * i + 0] + B[3 * k + 1] * gr[3 * i + 1] +
                      B[3 * k + 2] * gr[3 * i + 2];
   return fabs(J);
+}
+
+static double element_geometry(const orc_problem *p, const orc_tables *t,
+                               int64_t cv, int q, double *g) {
+  return element_geometry_c(p, t, cv, q, g, NULL);
 }
 
 static double coef_of(const double *arr, double scal, int64_t cv) {
@@ -649,11 +665,10 @@ static int element_matrix_impl(const orc_problem *p, const orc_tables *t,
   int nd = t->nd, Q = t->qdim, nl = t->nloc;
   double a = coef_of(p->coef_a, p->scal_a, cv);
   double b = coef_of(p->coef_b, p->scal_b, cv);
-  /* for an affine geometry J is evaluated at the first point only
-   * (compile_and_exec.cc:8836-8837); K is constant so every point gives
-   * the same numbers */
+  geo_cache gc;
+  gc.cv = -1;
   for (int q = 0; q < t->nq; ++q) {
-    double J = element_geometry(p, t, cv, q, g);
+    double J = element_geometry_c(p, t, cv, q, g, &gc);
     double coeff = J * p->qw[q];
     if (p->term == ORC_TERM_LAPLACE) {
       for (int j = 0; j < nd; ++j)
