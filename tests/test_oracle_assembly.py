@@ -1,0 +1,315 @@
+"""Element matrices, global assembly and derived quantities of the oracle.
+
+Pins: closed-form bases + brute-force collapsed Gauss quadrature on single
+elements, rigid-body modes, symmetry, linearity in (lambda, mu), dense
+brute-force global assembly, the stored-nnz counts the survey measured with
+the reference itself, the Neo-Hookean tangent against finite differences of
+the residual, and the compliance gradients against central differences."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from getfem_amd import meshes
+from tests import problems
+
+
+# ---------------------------------------------------------------- brute force
+def collapsed_rule(m=8):
+    x, w = np.polynomial.legendre.leggauss(m)
+    x = 0.5 + 0.5 * x
+    w = 0.5 * w
+    pts, wts = [], []
+    for (i, u), (j, v), (k, t) in itertools.product(enumerate(x), enumerate(x), enumerate(x)):
+        pts.append([u, v * (1 - u), t * (1 - u) * (1 - v)])
+        wts.append(w[i] * w[j] * w[k] * (1 - u) ** 2 * (1 - v))
+    return np.array(pts), np.array(wts)
+
+
+def closed_form_bases(k, nodes_lat, lam, glam):
+    """Values and real gradients of P1/P2/P3 Lagrange bases in barycentric
+    closed form; glam[b] = gradient of lambda_b in real space."""
+    vals, grads = [], []
+    for w in nodes_lat:
+        nz = [b for b in range(4) if w[b] > 0]
+        if k == 1:
+            b = nz[0]
+            vals.append(lam[b]); grads.append(glam[b])
+        elif k == 2:
+            if len(nz) == 1:
+                b = nz[0]
+                vals.append(lam[b] * (2 * lam[b] - 1)); grads.append((4 * lam[b] - 1) * glam[b])
+            else:
+                a, b = nz
+                vals.append(4 * lam[a] * lam[b]); grads.append(4 * (lam[b] * glam[a] + lam[a] * glam[b]))
+        else:
+            if len(nz) == 1:
+                b = nz[0]
+                L = lam[b]
+                vals.append(L * (3 * L - 1) * (3 * L - 2) / 2)
+                grads.append((27 * L * L - 18 * L + 2) / 2 * glam[b])
+            elif len(nz) == 2:
+                a, b = nz if w[nz[0]] == 2 else (nz[1], nz[0])
+                La, Lb = lam[a], lam[b]
+                vals.append(4.5 * La * (3 * La - 1) * Lb)
+                grads.append(4.5 * ((6 * La - 1) * Lb * glam[a] + La * (3 * La - 1) * glam[b]))
+            else:
+                a, b, c = nz
+                vals.append(27 * lam[a] * lam[b] * lam[c])
+                grads.append(27 * (lam[b] * lam[c] * glam[a] + lam[a] * lam[c] * glam[b]
+                                   + lam[a] * lam[b] * glam[c]))
+    return np.array(vals), np.array(grads)
+
+
+def lattice(k):
+    out = []
+    for c2 in range(k + 1):
+        for c1 in range(k + 1 - c2):
+            for c0 in range(k + 1 - c1 - c2):
+                out.append((k - c0 - c1 - c2, c0, c1, c2))
+    return out
+
+
+def brute_element(X, k, lam_c, mu_c, term):
+    """Element matrix (rows = test) by collapsed quadrature in real space."""
+    K = np.stack([X[1] - X[0], X[2] - X[0], X[3] - X[0]], axis=1)
+    Kinv = np.linalg.inv(K)
+    glam = np.vstack([-Kinv.sum(0), Kinv])  # gradients of the barycentric coordinates
+    pts, wts = collapsed_rule()
+    lat = lattice(k)
+    nd = len(lat)
+    Q = 3 if term == oracle.TERM_ISO_ELAST else 1
+    E = np.zeros((nd * Q, nd * Q))
+    for p, wq in zip(pts, wts):
+        lam = np.array([1 - p.sum(), p[0], p[1], p[2]])
+        _, g = closed_form_bases(k, lat, lam, glam)
+        c = wq * abs(np.linalg.det(K))
+        if Q == 1:
+            E += c * lam_c * g @ g.T
+        else:
+            dots = g @ g.T
+            for a in range(3):
+                for b in range(3):
+                    E[a::3, b::3] += c * (lam_c * np.outer(g[:, a], g[:, b]) + mu_c * np.outer(g[:, b], g[:, a])
+                                          + (mu_c * dots if a == b else 0))
+    return E
+
+
+def single_element_problem(X, k, term, lam, mu, im):
+    cvx = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    q = 3 if term == oracle.TERM_ISO_ELAST else 1
+    dofs, ndof = oracle.enumerate_dof(oracle.GT_PK31, X, cvx, oracle.FEM_PK, k, q)
+    if term == oracle.TERM_LAPLACE:
+        return oracle.Problem(oracle.GT_PK31, X, cvx, oracle.FEM_PK, k, q, dofs, ndof, im, term, scal_a=lam)
+    return oracle.Problem(oracle.GT_PK31, X, cvx, oracle.FEM_PK, k, q, dofs, ndof, im, term,
+                          coef_a=np.array([lam]), coef_b=np.array([mu]))
+
+
+RULE = {1: "IM_TETRAHEDRON(1)", 2: "IM_TETRAHEDRON(5)", 3: "IM_TETRAHEDRON(8)"}
+X0 = np.array([[0.1, 0.0, 0.05], [1.2, 0.1, -0.1], [0.2, 0.9, 0.1], [0.05, 0.2, 1.1]])
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+@pytest.mark.parametrize("term", [oracle.TERM_LAPLACE, oracle.TERM_ISO_ELAST])
+def test_element_matrix_brute_force(k, term):
+    P = single_element_problem(X0, k, term, 1.7, 0.6, RULE[k])
+    E = P.element_matrix(0)
+    B = brute_element(X0, k, 1.7, 0.6, term)
+    assert np.abs(E - B).max() <= 1e-13 * np.abs(B).max()
+
+
+def test_p1_poisson_closed_form():
+    P = single_element_problem(X0, 1, oracle.TERM_LAPLACE, 1.0, 0.0, "IM_TETRAHEDRON(1)")
+    E = P.element_matrix(0)
+    K = np.stack([X0[c] - X0[0] for c in (1, 2, 3)], axis=1)
+    G = np.vstack([-np.linalg.inv(K).sum(0), np.linalg.inv(K)])
+    vol = abs(np.linalg.det(K)) / 6
+    assert np.allclose(E, vol * G @ G.T, rtol=0, atol=1e-14 * np.abs(E).max())
+    assert np.abs(E.sum(1)).max() < 1e-13 * np.abs(E).max()
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_elasticity_rigid_modes_symmetry_linearity(k):
+    P = single_element_problem(X0, k, oracle.TERM_ISO_ELAST, 1.3, 0.4, RULE[k])
+    E = P.element_matrix(0)
+    assert np.abs(E - E.T).max() < 1e-13 * np.abs(E).max()
+    nodes = oracle.fem_nodes(oracle.FEM_PK, k)
+    K = np.stack([X0[c] - X0[0] for c in (1, 2, 3)], axis=1)
+    xn = X0[0] + nodes @ K.T
+    modes = []
+    for a in range(3):
+        u = np.zeros_like(xn)
+        u[:, a] = 1
+        modes.append(u.ravel())
+    for w in np.eye(3):
+        modes.append(np.cross(w, xn).ravel())
+    for r in modes:
+        assert np.abs(E @ r).max() < 1e-12 * np.abs(E).max()
+    E10 = single_element_problem(X0, k, oracle.TERM_ISO_ELAST, 1.0, 0.0, RULE[k]).element_matrix(0)
+    E01 = single_element_problem(X0, k, oracle.TERM_ISO_ELAST, 0.0, 1.0, RULE[k]).element_matrix(0)
+    assert np.abs(E - (1.3 * E10 + 0.4 * E01)).max() < 1e-13 * np.abs(E).max()
+
+
+def dense_bruteforce(case):
+    """Dense global matrix: element matrices added entry by entry, entries with
+    |e| <= 1e-14 max|elem| not inserted; returns (dense, stored mask)."""
+    P = case.oracle_problem()
+    n = case.ndof
+    A = np.zeros((n, n))
+    S = np.zeros((n, n), dtype=bool)
+    q = case.qdim
+    for e in range(case.cvx.shape[0]):
+        E = P.element_matrix(e)
+        thr = 1e-14 * np.abs(E).max()
+        dofs = (case.dofs[e][:, None] + np.arange(q)[None, :]).ravel()
+        keep = np.abs(E) > thr
+        for i in range(len(dofs)):
+            for j in range(len(dofs)):
+                if keep[i, j]:
+                    A[dofs[i], dofs[j]] += E[i, j]
+                    S[dofs[i], dofs[j]] = True
+    return A, S
+
+
+@pytest.mark.parametrize("mk", ["c1_n3", "p2_n2", "p2_n2_flat"])
+def test_global_assembly_bruteforce(mk):
+    case = {"c1_n3": lambda: problems.c1(3), "p2_n2": lambda: problems.c2(2),
+            "p2_n2_flat": lambda: problems.Case(2, 2, 3, "IM_TETRAHEDRON(5)", 1, lam=np.ones(48),
+                                                mu=np.ones(48), jitter=False)}[mk]()
+    A, S = dense_bruteforce(case)
+    colptr, rows, vals = case.oracle_problem().assemble_csc()
+    n = case.ndof
+    D = np.zeros((n, n))
+    M = np.zeros((n, n), dtype=bool)
+    for j in range(n):
+        for p in range(colptr[j], colptr[j + 1]):
+            D[rows[p], j] = vals[p]
+            M[rows[p], j] = True
+        assert np.all(np.diff(rows[colptr[j]:colptr[j + 1]]) > 0)
+    assert np.array_equal(M, S)
+    assert np.abs(D - A).max() <= 1e-14 * np.abs(A).max()
+
+
+def test_reference_nnz_p1_poisson_20():
+    """GetFEM's stored nnz for P1 Poisson on the jittered 20^3x6 mesh: 128,113
+    of 128,581 topological (survey, measured with the reference)."""
+    case = problems.c1(20)
+    colptr, rows, vals = case.oracle_problem().assemble_csc()
+    assert len(vals) == 128113
+    topo = set()
+    for e in range(case.cvx.shape[0]):
+        for i in case.dofs[e]:
+            for j in case.dofs[e]:
+                topo.add((int(i), int(j)))
+    assert len(topo) == 128581
+
+
+def test_reference_nnz_p2_elasticity_8():
+    """P2 elasticity on 8^3x6: 44,760 of 1,141,065 topological entries dropped
+    (survey, measured with the reference)."""
+    case = problems.c2(8)
+    colptr, rows, vals = case.oracle_problem().assemble_csc()
+    assert len(vals) == 1141065 - 44760
+
+
+def test_threaded_assembly_equals_serial():
+    case = problems.c2(4)
+    P = case.oracle_problem()
+    a = P.assemble_csc(nthreads=1)
+    b = P.assemble_csc(nthreads=4)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert np.abs(a[2] - b[2]).max() <= 1e-14 * np.abs(a[2]).max()
+
+
+# ---------------------------------------------------------------- Neo-Hookean
+def nh_problem(n=2, amp=0.05):
+    pts, cvx = meshes.regular_unit_mesh(n, "GT_QK(3,1)")
+    dofs, ndof = oracle.enumerate_dof(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, 2, 3)
+    nodes = np.zeros((ndof // 3, 3))
+    ref = oracle.fem_nodes(oracle.FEM_QK, 2)
+    for e in range(cvx.shape[0]):
+        for i, xr in enumerate(ref):
+            N, _ = oracle.fem_eval(oracle.FEM_QK, 1, xr)
+            nodes[dofs[e, i] // 3] = N @ pts[cvx[e]]
+    U = meshes.sine_modes()(nodes, amp).ravel()
+    P = oracle.Problem(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, 2, 3, dofs, ndof,
+                       "IM_GAUSS_PARALLELEPIPED(3,5)", oracle.TERM_NEOHOOKEAN, scal_a=10.0, scal_b=1.0, U=U)
+    return P, U
+
+
+def test_neohookean_tangent_is_derivative_of_residual():
+    P, U = nh_problem()
+    e = 3
+    K = P.element_matrix(e)
+    d = P.dofs[e]
+    gl = (d[:, None] + np.arange(3)[None, :]).ravel()
+    h = 1e-6
+    for jl in [0, 5, 17, 40, 80]:
+        up = U.copy(); up[gl[jl]] += h
+        um = U.copy(); um[gl[jl]] -= h
+        Pp = oracle.Problem(P.s.gt, P.pts, P.cvx, oracle.FEM_QK, 2, 3, P.dofs, P.ndof,
+                            "IM_GAUSS_PARALLELEPIPED(3,5)", oracle.TERM_NEOHOOKEAN, scal_a=10.0, scal_b=1.0, U=up)
+        Pm = oracle.Problem(P.s.gt, P.pts, P.cvx, oracle.FEM_QK, 2, 3, P.dofs, P.ndof,
+                            "IM_GAUSS_PARALLELEPIPED(3,5)", oracle.TERM_NEOHOOKEAN, scal_a=10.0, scal_b=1.0, U=um)
+        fd = (Pp.element_vector(e) - Pm.element_vector(e)) / (2 * h)
+        assert np.abs(fd - K[:, jl]).max() < 1e-6 * np.abs(K).max()
+    assert np.abs(K - K.T).max() < 1e-12 * np.abs(K).max()
+
+
+def test_neohookean_zero_stress_at_rest():
+    P, U = nh_problem()
+    P0 = oracle.Problem(P.s.gt, P.pts, P.cvx, oracle.FEM_QK, 2, 3, P.dofs, P.ndof,
+                        "IM_GAUSS_PARALLELEPIPED(3,5)", oracle.TERM_NEOHOOKEAN, scal_a=10.0, scal_b=1.0,
+                        U=np.zeros_like(U))
+    assert np.abs(P0.assemble_vector()).max() < 1e-14
+    # at rest the tangent is linear elasticity with (lambda, mu) = (10, 1)
+    E = P0.element_matrix(0)
+    assert np.abs(E - E.T).max() < 1e-13 * np.abs(E).max()
+
+
+# ---------------------------------------------------------------- compliance
+def compliance_problem(n=2):
+    pts, cvx = meshes.regular_unit_mesh(n)
+    pts = meshes.jitter(pts, n)
+    dofs, ndof = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 1, 3)
+    u = meshes.random_state(ndof)
+    rho = meshes.simp_density(cvx.shape[0])
+    nu = 0.3
+    lh, mh = nu / ((1 + nu) * (1 - 2 * nu)), 1 / (2 * (1 + nu))
+    P = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 1, 3, dofs, ndof, "IM_TETRAHEDRON(1)",
+                       oracle.TERM_ISO_ELAST, scal_a=lh, scal_b=mh, U=u)
+    return P, u, rho, lh, mh
+
+
+def test_compliance_equals_utku():
+    P, u, rho, lh, mh = compliance_problem()
+    J, _, _ = P.compliance(rho, grads=False)
+    E = 1e-3 + rho ** 3 * (1 - 1e-3)
+    Q = oracle.Problem(oracle.GT_PK31, P.pts, P.cvx, oracle.FEM_PK, 1, 3, P.dofs, P.ndof,
+                       "IM_TETRAHEDRON(1)", oracle.TERM_ISO_ELAST, coef_a=E * lh, coef_b=E * mh)
+    colptr, rows, vals = Q.assemble_csc()
+    import scipy.sparse as sp
+    K = sp.csc_matrix((vals, rows, colptr), shape=(P.ndof, P.ndof))
+    assert abs(J - u @ (K @ u)) < 1e-12 * abs(J)
+
+
+def test_compliance_gradients_fd():
+    P, u, rho, lh, mh = compliance_problem()
+    J, dr, dx = P.compliance(rho)
+    h = 1e-6
+    rng = np.random.default_rng(7)
+    for _ in range(6):
+        d = rng.standard_normal(rho.shape)
+        Jp = P.compliance(rho + h * d, grads=False)[0]
+        Jm = P.compliance(rho - h * d, grads=False)[0]
+        fd = (Jp - Jm) / (2 * h)
+        assert abs(fd - dr @ d) < 1e-6 * max(1.0, abs(fd))
+    for _ in range(6):
+        d = rng.standard_normal(P.pts.shape)
+        Pp = oracle.Problem(oracle.GT_PK31, P.pts + h * d, P.cvx, oracle.FEM_PK, 1, 3, P.dofs, P.ndof,
+                            "IM_TETRAHEDRON(1)", oracle.TERM_ISO_ELAST, scal_a=lh, scal_b=mh, U=u)
+        Pm = oracle.Problem(oracle.GT_PK31, P.pts - h * d, P.cvx, oracle.FEM_PK, 1, 3, P.dofs, P.ndof,
+                            "IM_TETRAHEDRON(1)", oracle.TERM_ISO_ELAST, scal_a=lh, scal_b=mh, U=u)
+        fd = (Pp.compliance(rho, grads=False)[0] - Pm.compliance(rho, grads=False)[0]) / (2 * h)
+        assert abs(fd - np.sum(dx * d)) < 1e-6 * max(1.0, abs(fd))
