@@ -35,6 +35,7 @@ SIGNATURES = {
     "gfx_integration_rule": (_i32, [_cs, ctypes.POINTER(_i32), _P, _P]),
     "gfx_fem_eval": (_i32, [_cs, _P, ctypes.POINTER(_i32), _P, _P]),
     "gfx_enumerate_dofs": (_i32, [_cs, _cs, _i32, _i64, _P, _i64, _P, _P, ctypes.POINTER(_i64)]),
+    "gfx_gwfl_compile": (_i32, [_cs, _cs, ctypes.c_char_p, _i64]),
     "gfx_ctx_create": (_i32, [_i32, _pp]),
     "gfx_ctx_destroy": (_i32, [_P]),
     "gfx_ctx_set_stream": (_i32, [_P, _P]),
@@ -70,6 +71,21 @@ SIGNATURES = {
     "gfx_plan_timing_summary": (_i32, [_P, ctypes.POINTER(_i32), ctypes.POINTER(_f64),
                                        ctypes.POINTER(_f64)]),
     "gfx_plan_destroy": (_i32, [_P]),
+    "gfx_ws_create": (_i32, [_P, _pp]),
+    "gfx_ws_destroy": (_i32, [_P]),
+    "gfx_ws_add_fem_variable": (_i32, [_P, _cs, _P, _i64, _P]),
+    "gfx_ws_add_fem_constant": (_i32, [_P, _cs, _P, _P]),
+    "gfx_ws_add_fixed_size_constant": (_i32, [_P, _cs, _P, _i64]),
+    "gfx_ws_set_values": (_i32, [_P, _cs, _P]),
+    "gfx_ws_add_expression": (_i32, [_P, _cs, _P, _i32]),
+    "gfx_ws_set_form": (_i32, [_P, _i32, _i32]),
+    "gfx_ws_compile": (_i32, [_P]),
+    "gfx_ws_assembly": (_i32, [_P, _i32]),
+    "gfx_ws_plan": (_i32, [_P, _pp]),
+    "gfx_ws_describe": (_i32, [_P, ctypes.c_char_p, _i64]),
+    "gfx_compliance_create": (_i32, [_P, _P, _f64, _f64, _f64, _f64, _pp]),
+    "gfx_compliance_eval": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "gfx_compliance_destroy": (_i32, [_P]),
 }
 
 _lib = None
@@ -81,9 +97,21 @@ class GfxError(RuntimeError):
         self.code = code
 
 
+def _bind_hip_runtime():
+    """libgfx and torch both need libamdhip64.so.7 and the process keeps the
+    first one loaded.  torch only runs on the HIP runtime it ships, so load it
+    first when it is installed: libgfx then shares torch's runtime, and torch
+    streams are valid libgfx streams."""
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+
+
 def lib():
     global _lib
     if _lib is None:
+        _bind_hip_runtime()
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(
                 "libgfx is not built (%s missing): run __graft_entry__.build() or "

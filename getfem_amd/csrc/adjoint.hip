@@ -1,0 +1,167 @@
+// Adjoint of the SIMP compliance J(rho, X) = u^T K(rho, X) u for P1 vector
+// fields on affine tetrahedra, E_e = emin + rho_e^p (1 - emin):
+//   J_e        = E_e |T| W(H),  W = lambda tr(H)^2 + 2 mu eps:eps, H = grad u
+//   dJ/drho_e  = E'(rho_e) |T| W
+//   dJ/dx_v    = sum_e E_e |T| (W I - H^T P) g_v,   P = dW/dH (shape derivative)
+//   dJ/du_v    = sum_e 2 E_e |T| P g_v
+// In GWFL terms this is the order-1 assembly of the order-0 potential with
+// rho (and the pulled-back coordinates) as variables
+// (src/getfem_generic_assembly_workspace.cc:520-541, :832-847).
+// One thread per element writes per-(element, vertex) contributions; the
+// node sums are gathers over node -> element incidences (deterministic, no
+// floating-point atomics), and J is a fixed-order two-stage reduction.
+#include <hip/hip_runtime.h>
+
+#include "gfx_internal.h"
+
+namespace gfx {
+
+namespace {
+
+constexpr int kRedThreads = 256;
+
+__global__ __launch_bounds__(256) void k_compliance_elem(
+    int64_t n_cv, const double *__restrict__ X, const int32_t *__restrict__ cvx,
+    const int32_t *__restrict__ dofs, const double *__restrict__ u,
+    const double *__restrict__ rho, double lam, double mu, double emin, double penal,
+    double wsum, double *__restrict__ Jpart, double *__restrict__ drho,
+    double *__restrict__ cX, double *__restrict__ cU, int *__restrict__ err) {
+  __shared__ double red[kRedThreads];
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double Je = 0.0;
+  if (e < n_cv) {
+    const int4 v = *reinterpret_cast<const int4 *>(cvx + 4 * e);
+    const int vv[4] = {v.x, v.y, v.z, v.w};
+    double x[4][3];
+    for (int i = 0; i < 4; ++i)
+      for (int r = 0; r < 3; ++r) x[i][r] = X[3 * (int64_t)vv[i] + r];
+    double k[9];  // column-major K(r,c)
+    for (int r = 0; r < 3; ++r) {
+      k[r] = x[1][r] - x[0][r];
+      k[r + 3] = x[2][r] - x[0][r];
+      k[r + 6] = x[3][r] - x[0][r];
+    }
+    const double a0 = k[4] * k[8] - k[5] * k[7];
+    const double a1 = k[5] * k[6] - k[3] * k[8];
+    const double a2 = k[3] * k[7] - k[4] * k[6];
+    const double J = k[0] * a0 + k[1] * a1 + k[2] * a2;
+    if (J == 0.0) atomicOr(err, 1);
+    double B[3][3];  // B = K^{-T}
+    B[0][0] = a0 / J; B[1][0] = a1 / J; B[2][0] = a2 / J;
+    B[0][1] = (k[2] * k[7] - k[1] * k[8]) / J;
+    B[1][1] = (k[0] * k[8] - k[2] * k[6]) / J;
+    B[2][1] = (k[1] * k[6] - k[0] * k[7]) / J;
+    B[0][2] = (k[1] * k[5] - k[2] * k[4]) / J;
+    B[1][2] = (k[2] * k[3] - k[0] * k[5]) / J;
+    B[2][2] = (k[0] * k[4] - k[1] * k[3]) / J;
+    // P1 reference gradients: -1,-1,-1 / e1 / e2 / e3
+    double g[4][3];
+    for (int r = 0; r < 3; ++r) {
+      g[1][r] = B[r][0];
+      g[2][r] = B[r][1];
+      g[3][r] = B[r][2];
+      g[0][r] = -(B[r][0] + B[r][1] + B[r][2]);
+    }
+    double H[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    for (int i = 0; i < 4; ++i) {
+      const int64_t d = dofs[4 * e + i];
+      for (int a = 0; a < 3; ++a) {
+        const double ua = u[d + a];
+        for (int b = 0; b < 3; ++b) H[a][b] += ua * g[i][b];
+      }
+    }
+    const double tr = H[0][0] + H[1][1] + H[2][2];
+    double W = lam * tr * tr, P[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b) {
+        const double eps = 0.5 * (H[a][b] + H[b][a]);
+        W += 2.0 * mu * eps * eps;
+        P[a][b] = 4.0 * mu * eps + (a == b ? 2.0 * lam * tr : 0.0);
+      }
+    const double w = fabs(J) * wsum;
+    const double r = rho[e];
+    const double E = emin + pow(r, penal) * (1.0 - emin);
+    const double dE = penal * pow(r, penal - 1.0) * (1.0 - emin);
+    Je = E * w * W;
+    if (drho) drho[e] = dE * w * W;
+    // M = W I - H^T P
+    double M[3][3];
+    for (int a = 0; a < 3; ++a)
+      for (int b = 0; b < 3; ++b)
+        M[a][b] = (a == b ? W : 0.0) - (H[0][a] * P[0][b] + H[1][a] * P[1][b] + H[2][a] * P[2][b]);
+    const double s = E * w;
+    for (int i = 0; i < 4; ++i)
+      for (int a = 0; a < 3; ++a) {
+        if (cX) cX[(4 * e + i) * 3 + a] = s * (M[a][0] * g[i][0] + M[a][1] * g[i][1] + M[a][2] * g[i][2]);
+        if (cU)
+          cU[(4 * e + i) * 3 + a] =
+              2.0 * s * (P[a][0] * g[i][0] + P[a][1] * g[i][1] + P[a][2] * g[i][2]);
+      }
+  }
+  red[threadIdx.x] = Je;
+  __syncthreads();
+  for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) Jpart[blockIdx.x] = red[0];
+}
+
+// fixed-order sum of the block partials
+__global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double *__restrict__ part,
+                                                              int64_t n, double *__restrict__ out) {
+  __shared__ double red[kRedThreads];
+  double s = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kRedThreads) s += part[i];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int o = kRedThreads / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) *out = red[0];
+}
+
+// out[node*stride + a] = sum over incidences (e, i) of c[(4e+i)*3 + a]
+__global__ __launch_bounds__(256) void k_gather3(int64_t n_nodes, const int32_t *__restrict__ inc_ptr,
+                                                 const uint32_t *__restrict__ inc,
+                                                 const double *__restrict__ c, double *__restrict__ out) {
+  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (n >= n_nodes) return;
+  double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+  for (int k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
+    const uint32_t x = inc[k];
+    const int64_t idx = ((int64_t)(x >> 5) * 4 + (x & 31)) * 3;
+    s0 += c[idx];
+    s1 += c[idx + 1];
+    s2 += c[idx + 2];
+  }
+  out[3 * n] = s0;
+  out[3 * n + 1] = s1;
+  out[3 * n + 2] = s2;
+}
+
+}  // namespace
+
+int64_t compliance_nblocks(int64_t n_cv) { return (n_cv + 255) / 256; }
+
+hipError_t launch_compliance(hipStream_t s, int64_t n_cv, const double *X, const int32_t *cvx,
+                             const int32_t *dofs, const double *u, const double *rho, double lam,
+                             double mu, double emin, double penal, double wsum, double *Jpart,
+                             double *J, double *drho, double *cX, double *cU, int *err) {
+  const int64_t nb = compliance_nblocks(n_cv);
+  if (nb == 0) return hipErrorInvalidValue;
+  k_compliance_elem<<<(unsigned)nb, 256, 0, s>>>(n_cv, X, cvx, dofs, u, rho, lam, mu, emin, penal,
+                                                 wsum, Jpart, drho, cX, cU, err);
+  k_sum_partials<<<1, kRedThreads, 0, s>>>(Jpart, nb, J);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather3(hipStream_t s, int64_t n_nodes, const int32_t *inc_ptr,
+                          const uint32_t *inc, const double *c, double *out) {
+  if (n_nodes == 0) return hipSuccess;
+  k_gather3<<<(unsigned)((n_nodes + 255) / 256), 256, 0, s>>>(n_nodes, inc_ptr, inc, c, out);
+  return hipGetLastError();
+}
+
+}  // namespace gfx

@@ -669,3 +669,76 @@ int gfx_plan_last_timings(gfx_plan *p, float *ms_geometry, float *ms_assemble) {
 }
 
 }  // extern "C"
+
+// ----------------------------------------------------------- compliance
+extern "C" {
+
+int gfx_compliance_destroy(gfx_compliance *c) {
+  if (!c) return GFX_E_INVALID;
+  (void)hipSetDevice(c->ctx->device);
+  dfree(c->d_vinc_ptr); dfree(c->d_dinc_ptr); dfree(c->d_vinc); dfree(c->d_dinc);
+  dfree(c->d_cX); dfree(c->d_cU); dfree(c->d_Jpart); dfree(c->d_err);
+  delete c;
+  return GFX_OK;
+}
+
+int gfx_compliance_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, double lambda, double mu,
+                          double emin, double penal, gfx_compliance **out) {
+  if (!mf || !mim || !out) return GFX_E_INVALID;
+  *out = nullptr;
+  gfx_ctx *ctx = mf->mesh->ctx;
+  const gfx_mesh *m = mf->mesh;
+  if (m->gt != GT_TET_P1 || mf->fem.kind != FEM_PK || mf->fem.k != 1 || mf->qdim != 3)
+    return fail(ctx, GFX_E_UNSUPPORTED, "compliance adjoint needs FEM_PK(3,1), qdim 3, GT_PK(3,1)");
+  if (mim->mesh != m) return fail(ctx, GFX_E_INVALID, "mesh_im on another mesh");
+  if (m->n_cv == 0) return fail(ctx, GFX_E_INVALID, "empty mesh");
+  if (set_device(ctx)) return GFX_E_HIP;
+  gfx_compliance *c = new (std::nothrow) gfx_compliance();
+  if (!c) return fail(ctx, GFX_E_OOM, "host allocation");
+  c->ctx = ctx; c->mf = mf; c->lam = lambda; c->mu = mu; c->emin = emin; c->penal = penal;
+  c->n_cv = m->n_cv; c->n_pts = m->n_pts; c->ndof = mf->ndof;
+  c->wsum = 0.0;
+  for (double w : mim->rule.w) c->wsum += w;  // P1: the integrand is constant per element
+  const int64_t nnode = mf->ndof / 3;
+  int32_t *tmp = nullptr;
+  const int64_t big = std::max(m->n_pts, nnode);
+  hipError_t e = dalloc(&c->d_vinc_ptr, (size_t)m->n_pts + 1);
+  if (e == hipSuccess) e = dalloc(&c->d_dinc_ptr, (size_t)nnode + 1);
+  if (e == hipSuccess) e = dalloc(&c->d_vinc, (size_t)m->n_cv * 4);
+  if (e == hipSuccess) e = dalloc(&c->d_dinc, (size_t)m->n_cv * 4);
+  if (e == hipSuccess) e = dalloc(&c->d_cX, (size_t)m->n_cv * 12);
+  if (e == hipSuccess) e = dalloc(&c->d_cU, (size_t)m->n_cv * 12);
+  if (e == hipSuccess) e = dalloc(&c->d_Jpart, (size_t)compliance_nblocks(m->n_cv));
+  if (e == hipSuccess) e = dalloc(&c->d_err, 1);
+  if (e == hipSuccess) e = dalloc(&tmp, (size_t)(2 * big + scan_tmp_elems(big)));
+  if (e == hipSuccess)
+    e = launch_incidences(ctx->stream, m->n_cv, 4, 1, m->d_cvx, m->n_pts, c->d_vinc_ptr, c->d_vinc, tmp);
+  if (e == hipSuccess)
+    e = launch_incidences(ctx->stream, m->n_cv, 4, 3, mf->d_dofs, nnode, c->d_dinc_ptr, c->d_dinc, tmp);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  dfree(tmp);
+  if (e != hipSuccess) {
+    gfx_compliance_destroy(c);
+    return hipfail(ctx, e, "gfx_compliance_create");
+  }
+  *out = c;
+  return GFX_OK;
+}
+
+int gfx_compliance_eval(gfx_compliance *c, const double *d_X, const double *d_rho, const double *d_u,
+                        double *d_J, double *d_dJ_drho, double *d_dJ_dX, double *d_dJ_du) {
+  if (!c || !d_rho || !d_u || !d_J) return GFX_E_INVALID;
+  gfx_ctx *ctx = c->ctx;
+  if (set_device(ctx)) return GFX_E_HIP;
+  hipStream_t s = ctx->stream;
+  const gfx_mesh *m = c->mf->mesh;
+  HIPCHK(ctx, hipMemsetAsync(c->d_err, 0, 4, s));
+  HIPCHK(ctx, launch_compliance(s, c->n_cv, d_X ? d_X : m->d_pts, m->d_cvx, c->mf->d_dofs, d_u, d_rho,
+                                c->lam, c->mu, c->emin, c->penal, c->wsum, c->d_Jpart, d_J, d_dJ_drho,
+                                d_dJ_dX ? c->d_cX : nullptr, d_dJ_du ? c->d_cU : nullptr, c->d_err));
+  if (d_dJ_dX) HIPCHK(ctx, launch_gather3(s, c->n_pts, c->d_vinc_ptr, c->d_vinc, c->d_cX, d_dJ_dX));
+  if (d_dJ_du) HIPCHK(ctx, launch_gather3(s, c->ndof / 3, c->d_dinc_ptr, c->d_dinc, c->d_cU, d_dJ_du));
+  return GFX_OK;
+}
+
+}  // extern "C"

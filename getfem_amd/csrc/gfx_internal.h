@@ -109,7 +109,27 @@ hipError_t scan_exclusive_i32(hipStream_t s, const int32_t *in, int32_t *out,
                               int64_t n, int32_t *tmp);
 int64_t scan_tmp_elems(int64_t n);
 
+// adjoint.hip
+int64_t compliance_nblocks(int64_t n_cv);
+hipError_t launch_compliance(hipStream_t s, int64_t n_cv, const double *X, const int32_t *cvx,
+                             const int32_t *dofs, const double *u, const double *rho, double lam,
+                             double mu, double emin, double penal, double wsum, double *Jpart,
+                             double *J, double *drho, double *cX, double *cU, int *err);
+hipError_t launch_gather3(hipStream_t s, int64_t n_nodes, const int32_t *inc_ptr,
+                          const uint32_t *inc, const double *c, double *out);
+
 }  // namespace gfx
+
+struct gfx_compliance {
+  gfx_ctx *ctx = nullptr;
+  gfx_mesh_fem *mf = nullptr;
+  double lam = 0, mu = 0, emin = 0, penal = 3, wsum = 0;
+  int64_t n_cv = 0, n_pts = 0, ndof = 0;
+  int32_t *d_vinc_ptr = nullptr, *d_dinc_ptr = nullptr;  // point / dof-node incidences
+  uint32_t *d_vinc = nullptr, *d_dinc = nullptr;
+  double *d_cX = nullptr, *d_cU = nullptr, *d_Jpart = nullptr;
+  int *d_err = nullptr;
+};
 
 struct gfx_plan {
   gfx_ctx *ctx = nullptr;

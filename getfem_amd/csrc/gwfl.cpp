@@ -1,0 +1,638 @@
+// GWFL compile stage (subset): symbolic evaluation, derivation and
+// classification of order-2 integrands.  Grammar and priorities follow
+// GetFEM's parser (src/getfem_generic_assembly_tree.cc:36-66): '+' '-' (1),
+// '*' '/' ':' '.' '@' (2, left associative), postfix quote and unary minus
+// (3), Sym/Skew/Trace and other functions (4).
+#include "gwfl.h"
+
+#include <algorithm>
+#include <cctype>
+#include <memory>
+#include <cmath>
+#include <regex>
+#include <set>
+#include <sstream>
+
+#include "../../include/gfx.h"
+
+namespace gfx {
+
+namespace {
+
+// ----------------------------------------------------------- polynomials
+using Mono = std::vector<int>;  // sorted symbol ids
+using Poly = std::map<Mono, double>;
+
+Poly pconst(double c) {
+  Poly p;
+  if (c != 0.0) p[Mono()] = c;
+  return p;
+}
+Poly psym(int s) {
+  Poly p;
+  p[Mono{s}] = 1.0;
+  return p;
+}
+void padd_to(Poly &a, const Poly &b, double s = 1.0) {
+  for (auto &kv : b) {
+    double &v = a[kv.first];
+    v += s * kv.second;
+    if (v == 0.0) a.erase(kv.first);
+  }
+}
+Poly pmul(const Poly &a, const Poly &b) {
+  Poly r;
+  for (auto &x : a)
+    for (auto &y : b) {
+      Mono m = x.first;
+      m.insert(m.end(), y.first.begin(), y.first.end());
+      std::sort(m.begin(), m.end());
+      double &v = r[m];
+      v += x.second * y.second;
+      if (v == 0.0) r.erase(m);
+    }
+  return r;
+}
+// derivative along  sum_s  dP/ds * dir(s)
+Poly pderiv(const Poly &p, const std::map<int, int> &dir) {
+  Poly r;
+  for (auto &kv : p) {
+    const Mono &m = kv.first;
+    for (size_t i = 0; i < m.size(); ++i) {
+      auto it = dir.find(m[i]);
+      if (it == dir.end()) continue;
+      if (i > 0 && m[i] == m[i - 1]) continue;  // each distinct symbol once
+      int mult = 0;
+      for (int s : m) mult += (s == m[i]);
+      Mono n = m;
+      n.erase(n.begin() + i);
+      n.push_back(it->second);
+      std::sort(n.begin(), n.end());
+      double &v = r[n];
+      v += mult * kv.second;
+      if (v == 0.0) r.erase(n);
+    }
+  }
+  return r;
+}
+bool pequal(const Poly &a, const Poly &b) {
+  Poly d = a;
+  padd_to(d, b, -1.0);
+  double mx = 0.0, sc = 0.0;
+  for (auto &kv : d) mx = std::max(mx, std::fabs(kv.second));
+  for (auto &kv : a) sc = std::max(sc, std::fabs(kv.second));
+  for (auto &kv : b) sc = std::max(sc, std::fabs(kv.second));
+  return mx <= 1e-13 * std::max(sc, 1e-300);
+}
+
+// --------------------------------------------------------------- tensors
+struct Tensor {
+  std::vector<int> shape;  // {}, {n}, {n,m}, {n,m,p,q}
+  std::vector<Poly> v;
+  std::shared_ptr<Tensor> norm_arg;  // set by Norm(): only sqr(Norm(x)) is polynomial
+  size_t size() const { return v.size(); }
+  int order() const { return (int)shape.size(); }
+};
+
+Tensor scalar(const Poly &p) {
+  Tensor t;
+  t.v.push_back(p);
+  return t;
+}
+Tensor zeros(std::vector<int> shape) {
+  Tensor t;
+  t.shape = shape;
+  size_t n = 1;
+  for (int s : shape) n *= (size_t)s;
+  t.v.assign(n, Poly());
+  return t;
+}
+
+struct Error {
+  std::string msg;
+};
+[[noreturn]] void fail(const std::string &m) { throw Error{m}; }
+
+Tensor tadd(const Tensor &a, const Tensor &b, double s) {
+  if (a.shape != b.shape) fail("addition of tensors of different sizes");
+  Tensor r = a;
+  r.norm_arg.reset();
+  for (size_t i = 0; i < r.size(); ++i) padd_to(r.v[i], b.v[i], s);
+  return r;
+}
+Tensor tscale(const Tensor &a, const Poly &s) {
+  Tensor r = zeros(a.shape);
+  for (size_t i = 0; i < a.size(); ++i) r.v[i] = pmul(a.v[i], s);
+  return r;
+}
+// contraction of the last k indices of a with the first k of b
+Tensor tcontract(const Tensor &a, const Tensor &b, int k) {
+  if (a.order() < k || b.order() < k) fail("contraction on too many indices");
+  for (int i = 0; i < k; ++i)
+    if (a.shape[a.order() - k + i] != b.shape[i]) fail("contraction of incompatible sizes");
+  std::vector<int> sh(a.shape.begin(), a.shape.end() - k);
+  sh.insert(sh.end(), b.shape.begin() + k, b.shape.end());
+  Tensor r = zeros(sh);
+  size_t inner = 1, outer_b = 1;
+  for (int i = 0; i < k; ++i) inner *= (size_t)b.shape[i];
+  for (int i = k; i < b.order(); ++i) outer_b *= (size_t)b.shape[i];
+  const size_t outer_a = a.size() / inner;
+  for (size_t i = 0; i < outer_a; ++i)
+    for (size_t j = 0; j < outer_b; ++j) {
+      Poly s;
+      for (size_t l = 0; l < inner; ++l) padd_to(s, pmul(a.v[i * inner + l], b.v[l * outer_b + j]));
+      r.v[i * outer_b + j] = s;
+    }
+  return r;
+}
+Tensor touter(const Tensor &a, const Tensor &b) {
+  std::vector<int> sh = a.shape;
+  sh.insert(sh.end(), b.shape.begin(), b.shape.end());
+  Tensor r = zeros(sh);
+  for (size_t i = 0; i < a.size(); ++i)
+    for (size_t j = 0; j < b.size(); ++j) r.v[i * b.size() + j] = pmul(a.v[i], b.v[j]);
+  return r;
+}
+Tensor ttranspose(const Tensor &a) {
+  if (a.order() != 2) fail("transpose of a non-matrix");
+  Tensor r = zeros({a.shape[1], a.shape[0]});
+  for (int i = 0; i < a.shape[0]; ++i)
+    for (int j = 0; j < a.shape[1]; ++j) r.v[j * a.shape[0] + i] = a.v[i * a.shape[1] + j];
+  return r;
+}
+
+// --------------------------------------------------------------- parser
+struct Ctx {
+  const std::map<std::string, GwflVar> *vars;
+  std::vector<std::string> symnames;
+  std::map<std::string, int> symid;
+  int sym(const std::string &n) {
+    auto it = symid.find(n);
+    if (it != symid.end()) return it->second;
+    symid[n] = (int)symnames.size();
+    symnames.push_back(n);
+    return symid[n];
+  }
+};
+
+struct Parser {
+  const std::string &s;
+  size_t pos = 0;
+  Ctx &C;
+  Parser(const std::string &str, Ctx &c) : s(str), C(c) {}
+
+  void skip() {
+    while (pos < s.size() && std::isspace((unsigned char)s[pos])) ++pos;
+  }
+  bool eat(char c) {
+    skip();
+    if (pos < s.size() && s[pos] == c) { ++pos; return true; }
+    return false;
+  }
+  char peek() {
+    skip();
+    return pos < s.size() ? s[pos] : '\0';
+  }
+  void expect(char c) {
+    if (!eat(c)) fail(std::string("expected '") + c + "' at position " + std::to_string(pos));
+  }
+
+  Tensor expr() {
+    Tensor a = term();
+    for (;;) {
+      if (eat('+')) a = tadd(a, term(), 1.0);
+      else if (peek() == '-') { ++pos; a = tadd(a, term(), -1.0); }
+      else return a;
+    }
+  }
+  Tensor term() {
+    Tensor a = unary();
+    for (;;) {
+      char c = peek();
+      if (c == '*') { ++pos; a = mul(a, unary()); }
+      else if (c == '/') {
+        ++pos;
+        Tensor b = unary();
+        if (b.order() != 0 || b.v[0].size() > 1 || (b.v[0].size() == 1 && !b.v[0].begin()->first.empty()))
+          fail("division by a non-constant");
+        if (b.v[0].empty()) fail("division by zero");
+        a = tscale(a, pconst(1.0 / b.v[0].begin()->second));
+      } else if (c == ':') { ++pos; a = colon(a, unary()); }
+      else if (c == '.' && !(pos + 1 < s.size() && (s[pos + 1] == '*' || s[pos + 1] == '/'))) {
+        ++pos;
+        a = tcontract(a, unary(), 1);
+      } else if (c == '@') { ++pos; a = touter(a, unary()); }
+      else return a;
+    }
+  }
+  static Tensor mul(const Tensor &a, const Tensor &b) {
+    if (a.order() == 0) return tscale(b, a.v[0]);
+    if (b.order() == 0) return tscale(a, b.v[0]);
+    if (a.order() == 2 && (b.order() == 2 || b.order() == 1)) return tcontract(a, b, 1);
+    if (a.order() == 4 && b.order() == 2) return tcontract(a, b, 2);
+    fail("unsupported '*' between tensors of orders " + std::to_string(a.order()) + " and " +
+         std::to_string(b.order()));
+  }
+  static Tensor colon(const Tensor &a, const Tensor &b) {
+    if (a.order() >= 2 && b.order() >= 2) return tcontract(a, b, 2);
+    if (a.order() == 1 && b.order() == 1) return tcontract(a, b, 1);
+    fail("unsupported ':' between tensors of orders " + std::to_string(a.order()) + " and " +
+         std::to_string(b.order()));
+  }
+  Tensor unary() {
+    if (eat('-')) {
+      Tensor a = unary();
+      return tscale(a, pconst(-1.0));
+    }
+    if (eat('+')) return unary();
+    return postfix();
+  }
+  Tensor postfix() {
+    Tensor a = primary();
+    for (;;) {
+      if (peek() == '\'') { ++pos; a = ttranspose(a); }
+      else if (peek() == '(') a = index(a);
+      else return a;
+    }
+  }
+  Tensor index(const Tensor &a) {
+    expect('(');
+    std::vector<int> idx;  // -1 = ':'
+    for (;;) {
+      skip();
+      if (peek() == ':') { ++pos; idx.push_back(-1); }
+      else {
+        Tensor i = expr();
+        if (i.order() != 0 || i.v[0].size() > 1 || (i.v[0].size() == 1 && !i.v[0].begin()->first.empty()))
+          fail("index must be a number");
+        const double v = i.v[0].empty() ? 0.0 : i.v[0].begin()->second;
+        idx.push_back((int)std::lround(v) - 1);
+      }
+      if (eat(')')) break;
+      expect(',');
+    }
+    if ((int)idx.size() != a.order()) fail("wrong number of indices");
+    std::vector<int> sh;
+    for (size_t k = 0; k < idx.size(); ++k) {
+      if (idx[k] == -1) sh.push_back(a.shape[k]);
+      else if (idx[k] < 0 || idx[k] >= a.shape[k]) fail("index out of range");
+    }
+    Tensor r = zeros(sh);
+    // enumerate the kept indices
+    const size_t n = r.size();
+    for (size_t f = 0; f < n; ++f) {
+      size_t rem = f, off = 0, stride = 1;
+      std::vector<int> full(idx.size());
+      std::vector<int> kept;
+      for (size_t k = idx.size(); k-- > 0;)
+        if (idx[k] == -1) {
+          full[k] = (int)(rem % (size_t)a.shape[k]);
+          rem /= (size_t)a.shape[k];
+        } else full[k] = idx[k];
+      for (size_t k = idx.size(); k-- > 0;) { off += full[k] * stride; stride *= (size_t)a.shape[k]; }
+      r.v[f] = a.v[off];
+    }
+    return r;
+  }
+  std::string name() {
+    skip();
+    size_t b = pos;
+    while (pos < s.size() && (std::isalnum((unsigned char)s[pos]) || s[pos] == '_')) ++pos;
+    return s.substr(b, pos - b);
+  }
+  Tensor primary() {
+    skip();
+    if (pos >= s.size()) fail("unexpected end of expression");
+    const char c = s[pos];
+    if (std::isdigit((unsigned char)c) || (c == '.' && pos + 1 < s.size() && std::isdigit((unsigned char)s[pos + 1]))) {
+      size_t used = 0;
+      double v = std::stod(s.substr(pos), &used);
+      pos += used;
+      return scalar(pconst(v));
+    }
+    if (c == '(') {
+      ++pos;
+      Tensor a = expr();
+      expect(')');
+      return a;
+    }
+    if (c == '[') return bracket();
+    if (std::isalpha((unsigned char)c)) {
+      const std::string n = name();
+      if (peek() == '(' && is_function(n)) return function(n);
+      return identifier(n);
+    }
+    fail(std::string("unexpected character '") + c + "'");
+  }
+  Tensor bracket() {
+    expect('[');
+    std::vector<std::vector<Tensor>> rows(1);
+    for (;;) {
+      rows.back().push_back(expr());
+      if (eat(',')) continue;
+      if (eat(';')) { rows.emplace_back(); continue; }
+      expect(']');
+      break;
+    }
+    for (auto &r : rows)
+      for (auto &t : r)
+        if (t.order() != 0) fail("only scalar entries in [...]");
+    if (rows[0].size() == 1) {  // column vector
+      Tensor r = zeros({(int)rows.size()});
+      for (size_t i = 0; i < rows.size(); ++i) {
+        if (rows[i].size() != 1) fail("ragged [...]");
+        r.v[i] = rows[i][0].v[0];
+      }
+      return r;
+    }
+    const int m = (int)rows.size(), n = (int)rows[0].size();
+    Tensor r = zeros({m, n});
+    for (int i = 0; i < m; ++i) {
+      if ((int)rows[i].size() != n) fail("ragged [...]");
+      for (int j = 0; j < n; ++j) r.v[i * n + j] = rows[i][j].v[0];
+    }
+    return r;
+  }
+  static bool is_function(const std::string &n) {
+    static const std::set<std::string> f = {"Sym", "Skew", "Trace", "Id", "sqr", "Norm", "Norm_sqr",
+                                            "qdim", "Deviator"};
+    return f.count(n) > 0;
+  }
+  Tensor function(const std::string &n) {
+    expect('(');
+    if (n == "qdim") {
+      const std::string v = name();
+      expect(')');
+      auto it = C.vars->find(v);
+      if (it == C.vars->end()) fail("unknown variable " + v);
+      return scalar(pconst(it->second.qdim));
+    }
+    Tensor a = expr();
+    expect(')');
+    if (n == "Id") {
+      if (a.order() != 0 || a.v[0].size() != 1 || !a.v[0].begin()->first.empty()) fail("Id(n) needs a number");
+      const int d = (int)std::lround(a.v[0].begin()->second);
+      Tensor r = zeros({d, d});
+      for (int i = 0; i < d; ++i) r.v[i * d + i] = pconst(1.0);
+      return r;
+    }
+    if (n == "Trace") {
+      if (a.order() != 2 || a.shape[0] != a.shape[1]) fail("Trace of a non-square matrix");
+      Poly s;
+      for (int i = 0; i < a.shape[0]; ++i) padd_to(s, a.v[i * a.shape[0] + i]);
+      return scalar(s);
+    }
+    if (n == "Sym" || n == "Skew") {
+      Tensor t = ttranspose(a);
+      Tensor r = tadd(a, t, n == "Sym" ? 1.0 : -1.0);
+      return tscale(r, pconst(0.5));
+    }
+    if (n == "Deviator") {
+      if (a.order() != 2) fail("Deviator of a non-matrix");
+      Poly tr;
+      const int d = a.shape[0];
+      for (int i = 0; i < d; ++i) padd_to(tr, a.v[i * d + i]);
+      Tensor r = a;
+      for (int i = 0; i < d; ++i) padd_to(r.v[i * d + i], tr, -1.0 / d);
+      return r;
+    }
+    if (n == "Norm_sqr") return tcontract(a, a, a.order());
+    if (n == "Norm") {
+      Tensor r = scalar(Poly());
+      r.norm_arg = std::make_shared<Tensor>(a);
+      return r;
+    }
+    if (n == "sqr") {
+      if (a.norm_arg) return tcontract(*a.norm_arg, *a.norm_arg, a.norm_arg->order());
+      if (a.order() != 0) fail("sqr of a non-scalar");
+      return scalar(pmul(a.v[0], a.v[0]));
+    }
+    fail("unsupported function " + n);
+  }
+
+  // symbol tags: V value, G gradient, T / GT test value / gradient, S / GS
+  // second test function value / gradient; components after '_'
+  static std::string tag_of(const std::string &kind) {
+    if (kind == "Val") return "V";
+    if (kind == "Grad" || kind == "Div") return "G";
+    if (kind == "Test") return "T";
+    if (kind == "Grad_Test" || kind == "Div_Test") return "GT";
+    if (kind == "Test2") return "S";
+    return "GS";  // Grad_Test2, Div_Test2
+  }
+
+  Tensor field(const std::string &var, const std::string &kind) {
+    auto it = C.vars->find(var);
+    if (it == C.vars->end()) fail("unknown variable " + var);
+    const GwflVar &v = it->second;
+    if (!v.is_fem) fail(kind + " of a non-fem quantity " + var);
+    const int q = v.qdim;
+    const std::string base = var + ":" + tag_of(kind) + "_";
+    if (kind == "Grad" || kind == "Grad_Test" || kind == "Grad_Test2") {
+      if (q == 1) {
+        Tensor t = zeros({3});
+        for (int b = 0; b < 3; ++b) t.v[b] = psym(C.sym(base + std::to_string(b)));
+        return t;
+      }
+      Tensor t = zeros({q, 3});
+      for (int a = 0; a < q; ++a)
+        for (int b = 0; b < 3; ++b) t.v[a * 3 + b] = psym(C.sym(base + std::to_string(a) + std::to_string(b)));
+      return t;
+    }
+    if (kind == "Div" || kind == "Div_Test" || kind == "Div_Test2") {
+      if (q != 3) fail("Div of a non-vector field " + var);
+      Poly s;
+      for (int a = 0; a < 3; ++a) padd_to(s, psym(C.sym(base + std::to_string(a) + std::to_string(a))));
+      return scalar(s);
+    }
+    if (q == 1) return scalar(psym(C.sym(base)));
+    Tensor t = zeros({q});
+    for (int a = 0; a < q; ++a) t.v[a] = psym(C.sym(base + std::to_string(a)));
+    return t;
+  }
+
+  Tensor identifier(const std::string &n) {
+    if (n == "meshdim") return scalar(pconst(3.0));
+    if (n == "pi") return scalar(pconst(M_PI));
+    static const char *pre[] = {"Grad_Test2_", "Grad_Test_", "Div_Test2_", "Div_Test_", "Test2_",
+                                "Test_", "Grad_", "Div_"};
+    for (const char *p : pre) {
+      const std::string ps(p);
+      if (n.compare(0, ps.size(), ps) == 0 && n.size() > ps.size()) {
+        const std::string var = n.substr(ps.size());
+        const std::string kind = ps.substr(0, ps.size() - 1);
+        if (C.vars->count(var)) return field(var, kind == "Test2" ? "Test2" : kind);
+      }
+    }
+    auto it = C.vars->find(n);
+    if (it == C.vars->end()) fail("unknown name " + n);
+    const GwflVar &v = it->second;
+    if (v.is_fem && !v.is_constant) return field(n, "Val");
+    if (v.is_fem && !v.per_element) fail("fem constant " + n + " is not constant per element (only P0 supported)");
+    if (v.qdim != 1) fail("constant " + n + " is not a scalar");
+    return scalar(psym(C.sym("c:" + n)));
+  }
+};
+
+}  // namespace
+
+bool gwfl_compile_order2(const std::vector<std::string> &exprs,
+                         const std::map<std::string, GwflVar> &vars, GwflKernelParams &out,
+                         std::string &err) {
+  out = GwflKernelParams();
+  // the finite strain brick's expression (getfem_nonlinear_elasticity.cc:2319-2320)
+  static const std::regex nh(
+      R"(^\(\(Id\(meshdim\)\+Grad_(\w+)\)\*\(Compressible_Neo_Hookean_Bonet_PK2\(Grad_\1,\[(\w+);(\w+)\]\)\)\):Grad_Test_\1$)");
+  if (exprs.size() == 1) {
+    std::string e;
+    for (char c : exprs[0]) if (!std::isspace((unsigned char)c)) e += c;
+    std::smatch m;
+    if (std::regex_match(e, m, nh)) {
+      out.term = GFX_TERM_NEOHOOKEAN;
+      out.unknown = m[1];
+      out.coef[0].terms.push_back({1.0, {m[2]}});
+      out.coef[1].terms.push_back({1.0, {m[3]}});
+      out.describe = "neo-hookean(bonet) lambda=" + std::string(m[2]) + " mu=" + std::string(m[3]);
+      return true;
+    }
+  }
+  Ctx C;
+  C.vars = &vars;
+  Poly total;
+  std::string unknown;
+  try {
+    for (const std::string &ex : exprs) {
+      Parser P(ex, C);
+      Tensor t = P.expr();
+      P.skip();
+      if (P.pos != ex.size()) fail("trailing characters in expression");
+      if (t.order() != 0) fail("the expression is not a scalar");
+      Poly p = t.v[0];
+      // the variables it depends on and its test-function order
+      std::set<std::string> dvars;
+      int order = 0;
+      for (auto &kv : p)
+        for (int s : kv.first) {
+          const std::string &nm = C.symnames[s];
+          const size_t c = nm.find(':');
+          if (c == std::string::npos) continue;
+          const std::string var = nm.substr(0, c);
+          if (var == "c") continue;
+          const std::string tag = nm.substr(c + 1, nm.find('_', c) - c - 1);
+          if (tag == "S" || tag == "GS") order = std::max(order, 2);
+          else if (tag == "T" || tag == "GT") order = std::max(order, 1);
+          else dvars.insert(var);
+        }
+      for (const std::string &v : dvars) {
+        if (!unknown.empty() && unknown != v) fail("several unknowns in one term are not supported");
+        unknown = v;
+      }
+      // differentiate up to order 2 (ga_workspace::add_tree)
+      while (order < 2) {
+        if (unknown.empty()) fail("no unknown to differentiate with respect to");
+        const GwflVar &v = vars.at(unknown);
+        std::map<int, int> dir;
+        const std::string tv = order == 0 ? "T" : "S", tg = order == 0 ? "GT" : "GS";
+        for (int a = 0; a < v.qdim; ++a) {
+          const std::string sa = v.qdim == 1 ? "" : std::to_string(a);
+          dir[C.sym(unknown + ":V_" + sa)] = C.sym(unknown + ":" + tv + "_" + sa);
+          for (int b = 0; b < 3; ++b)
+            dir[C.sym(unknown + ":G_" + sa + std::to_string(b))] =
+                C.sym(unknown + ":" + tg + "_" + sa + std::to_string(b));
+        }
+        p = pderiv(p, dir);
+        ++order;
+      }
+      padd_to(total, p);
+    }
+    // classify: every monomial = coefficients x one Grad_Test x one Grad_Test2
+    std::string tvar;
+    std::map<std::pair<std::string, std::string>, Poly> D;
+    for (auto &kv : total) {
+      std::string t1, t2;
+      Mono coef;
+      for (int s : kv.first) {
+        const std::string &nm = C.symnames[s];
+        const size_t c = nm.find(':');
+        const std::string var = nm.substr(0, c);
+        if (var == "c") { coef.push_back(s); continue; }
+        const size_t us = nm.find('_', c);
+        const std::string tag = nm.substr(c + 1, us - c - 1), comp = nm.substr(us + 1);
+        if (tag == "GS") {
+          if (!t2.empty()) fail("term not bilinear");
+          t2 = comp;
+        } else if (tag == "GT") {
+          if (!t1.empty()) fail("term not bilinear");
+          t1 = comp;
+        } else if (tag == "T" || tag == "S") {
+          fail("value terms (Test_u, mass matrices) are not supported by the kernels");
+        } else {
+          fail("the order-2 term depends on the unknown (nonlinear form outside the supported subset)");
+        }
+        if (tvar.empty()) tvar = var;
+        else if (tvar != var) fail("coupling between different variables is not supported");
+      }
+      if (t1.empty() || t2.empty()) fail("term not bilinear in the test functions");
+      padd_to(D[{t1, t2}], Poly{{coef, kv.second}});
+    }
+    if (tvar.empty()) fail("empty order-2 term");
+    const int q = vars.at(tvar).qdim;
+    auto get = [&](const std::string &a, const std::string &b) {
+      auto it = D.find({a, b});
+      return it == D.end() ? Poly() : it->second;
+    };
+    auto idx = [&](int a, int b) { return q == 1 ? std::to_string(b) : std::to_string(a) + std::to_string(b); };
+    auto coef_of = [&](const Poly &p, GwflCoef &c) {
+      for (auto &kv : p) {
+        std::vector<std::string> nm;
+        for (int s : kv.first) nm.push_back(C.symnames[s].substr(2));
+        c.terms.push_back({kv.second, nm});
+      }
+    };
+    out.unknown = tvar;
+    // Laplace: D_{ab,cd} = alpha delta_ac delta_bd
+    const Poly alpha = get(idx(0, 0), idx(0, 0));
+    bool lap = true;
+    for (int a = 0; a < q && lap; ++a)
+      for (int b = 0; b < 3 && lap; ++b)
+        for (int c = 0; c < q && lap; ++c)
+          for (int d = 0; d < 3 && lap; ++d)
+            lap = pequal(get(idx(a, b), idx(c, d)), (a == c && b == d) ? alpha : Poly());
+    if (lap) {
+      out.term = GFX_TERM_LAPLACE;
+      coef_of(alpha, out.coef[0]);
+      out.describe = "laplace";
+      return true;
+    }
+    if (q == 3) {
+      const Poly lam = get("00", "11"), mu = get("01", "01");
+      bool iso = true;
+      for (int a = 0; a < 3 && iso; ++a)
+        for (int b = 0; b < 3 && iso; ++b)
+          for (int c = 0; c < 3 && iso; ++c)
+            for (int d = 0; d < 3 && iso; ++d) {
+              Poly e;
+              if (a == b && c == d) padd_to(e, lam);
+              if (a == c && b == d) padd_to(e, mu);
+              if (a == d && b == c) padd_to(e, mu);
+              iso = pequal(get(idx(a, b), idx(c, d)), e);
+            }
+      if (iso) {
+        out.term = GFX_TERM_ISO_ELAST;
+        coef_of(lam, out.coef[0]);
+        coef_of(mu, out.coef[1]);
+        out.describe = "isotropic elasticity";
+        return true;
+      }
+    }
+    fail("the order-2 term is not one of the supported forms (Laplacian, isotropic elasticity)");
+  } catch (const Error &e) {
+    err = e.msg;
+    return false;
+  } catch (const std::exception &e) {
+    err = std::string("parse error: ") + e.what();
+    return false;
+  }
+}
+
+}  // namespace gfx

@@ -1,0 +1,43 @@
+// GWFL compile stage for the supported subset: parse an expression, evaluate
+// it symbolically (every tensor entry a polynomial in the components of
+// Grad_u, Grad_Test_u, Grad_Test2_u and the coefficient names), differentiate
+// it with respect to the unknown up to order 2 (the role of ga_derivative,
+// src/getfem_generic_assembly_semantic.cc:3965, called from
+// ga_workspace::add_tree, src/getfem_generic_assembly_workspace.cc:520-541),
+// and classify the order-2 integrand into the parameters of an assembly
+// kernel.  This replaces the instruction lists that ga_compile emits
+// (src/getfem_generic_assembly_compile_and_exec.cc:7910-8643).
+#pragma once
+#include <map>
+#include <string>
+#include <vector>
+
+namespace gfx {
+
+struct GwflVar {
+  std::string name;
+  bool is_fem = false;       // fem variable / constant (otherwise fixed size)
+  bool is_constant = false;  // not differentiated
+  int qdim = 1;              // components of the field (fem) / size (fixed)
+  bool per_element = false;  // fem constant on a P0 element (one value per element)
+};
+
+// coefficient of the kernel: value = sum_k scale_k * prod(names_k)
+struct GwflCoef {
+  std::vector<std::pair<double, std::vector<std::string>>> terms;
+};
+
+struct GwflKernelParams {
+  int term = -1;           // GFX_TERM_*
+  std::string unknown;     // name of the variable
+  GwflCoef coef[2];        // LAPLACE: a ; ISO_ELAST: lambda, mu
+  std::string describe;    // human-readable normal form
+};
+
+// Compile the sum of expressions (each with its derivation order request) for
+// an order-2 assembly.  Returns false with a message on unsupported input.
+bool gwfl_compile_order2(const std::vector<std::string> &exprs,
+                         const std::map<std::string, GwflVar> &vars, GwflKernelParams &out,
+                         std::string &err);
+
+}  // namespace gfx

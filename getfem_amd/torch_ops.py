@@ -1,0 +1,70 @@
+"""torch surface of libgfx: the SIMP compliance adjoint as a
+torch.autograd.Function.
+
+J(rho, X, u) = u^T K(rho, X) u for isotropic elasticity on a P1 vector
+mesh_fem with E_e = emin + rho_e^penal (1 - emin).  One libgfx call computes
+J and the three gradients on the device (stream = torch's current stream);
+backward scales them by the incoming gradient.  GetFEM's route to the same
+derivatives is the order-1 assembly of the order-0 potential with rho and the
+pulled-back coordinates as variables (src/getfem_generic_assembly_workspace.cc:
+520-541, :832-847).
+"""
+import ctypes
+
+import torch
+
+from ._lib import check, lib
+
+
+class Compliance:
+    """gfx_compliance bound to a MeshFem (FEM_PK(3,1), qdim 3) and MeshIm."""
+
+    def __init__(self, mf, mim, lam, mu, emin=1e-3, penal=3.0):
+        self.mf, self.mim = mf, mim
+        self.ctx = mf.mesh.ctx
+        h = ctypes.c_void_p()
+        check(lib().gfx_compliance_create(mf.h, mim.h, float(lam), float(mu), float(emin),
+                                          float(penal), ctypes.byref(h)), self.ctx.h)
+        self.h = h
+
+    def eval(self, rho, X, u, want_grads=True):
+        """All tensors float64 on the device; X may be None (mesh points)."""
+        dev = rho.device
+        stream = torch.cuda.current_stream(dev)
+        self.ctx.set_stream(stream.cuda_stream)
+        rho = rho.contiguous()
+        u = u.contiguous()
+        Xc = X.contiguous() if X is not None else None
+        J = torch.empty((), dtype=torch.float64, device=dev)
+        dr = torch.empty_like(rho) if want_grads else None
+        n_pts = self.mf.mesh.pts.shape[0]
+        dx = torch.empty((n_pts, 3), dtype=torch.float64, device=dev) if want_grads else None
+        du = torch.empty_like(u) if want_grads else None
+        p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+        check(lib().gfx_compliance_eval(self.h, p(Xc), p(rho), p(u), p(J), p(dr), p(dx), p(du)),
+              self.ctx.h)
+        return J, dr, dx, du
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().gfx_compliance_destroy(self.h)
+            self.h = None
+
+
+class ComplianceFunction(torch.autograd.Function):
+    """J = ComplianceFunction.apply(rho, X, u, op)."""
+
+    @staticmethod
+    def forward(ctx, rho, X, u, op):
+        J, dr, dx, du = op.eval(rho, X, u, want_grads=True)
+        ctx.save_for_backward(dr, dx, du)
+        return J
+
+    @staticmethod
+    def backward(ctx, gJ):
+        dr, dx, du = ctx.saved_tensors
+        return gJ * dr, gJ * dx, gJ * du, None
+
+
+def compliance(rho, X, u, op):
+    return ComplianceFunction.apply(rho, X, u, op)

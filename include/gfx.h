@@ -42,6 +42,8 @@ extern "C" {
 /* ---- terms of the supported GWFL subset (after derivation, order 2) ---- */
 #define GFX_TERM_LAPLACE 0    /* a*Grad_u:Grad_Test_u (a scalar or per element)      */
 #define GFX_TERM_ISO_ELAST 1  /* lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u */
+#define GFX_TERM_NEOHOOKEAN 2 /* ((Id(meshdim)+Grad_u)*(Compressible_Neo_Hookean_Bonet_PK2(
+                                 Grad_u,[lambda;mu]))):Grad_Test_u (tangent + residual)  */
 
 /* ---- contraction forms / precisions ---- */
 #define GFX_FORM_FACTORIZED 0 /* affine element: B R_ij B^T with R_ij = sum_q w_q
@@ -80,6 +82,12 @@ int gfx_fem_eval(const char *fem_name, const double *x, int *nd, double *h_val,
 int gfx_enumerate_dofs(const char *geotrans, const char *fem_name, int qdim,
                        int64_t n_pts, const double *h_pts, int64_t n_cv,
                        const int32_t *h_cvx, int32_t *h_out, int64_t *ndof);
+
+/* the GWFL compile stage alone: decls is "name:kind:n;..." with kind
+ * var (fem variable, n components), const (fixed-size constant, n values) or
+ * p0 (per-element fem constant); exprs are separated by '\n' and summed;
+ * out receives the normal form of the order-2 term (as gfx_ws_describe) */
+int gfx_gwfl_compile(const char *exprs, const char *decls, char *out, int64_t len);
 
 /* ======================= context ======================= */
 int gfx_ctx_create(int device, gfx_ctx **out);
@@ -169,6 +177,58 @@ int gfx_plan_last_timings(gfx_plan *p, float *ms_geometry, float *ms_assemble);
 int gfx_plan_timing_summary(gfx_plan *p, int *n_calls, double *ms_geometry,
                             double *ms_assemble);
 int gfx_plan_destroy(gfx_plan *p);
+
+/* ======================= workspace (ga_workspace) =======
+ * The drop-in for getfem::ga_workspace (src/getfem/getfem_generic_assembly.h:
+ * 262-597) on the supported GWFL subset.  Names follow GWFL rules; a fem
+ * variable is differentiated, constants are not.  Expressions of derivation
+ * order 2 are summed (ga_workspace::add_tree merges trees with the same test
+ * functions, src/getfem_generic_assembly_workspace.cc:474-495), compiled by
+ * gfx_ws_compile into kernel parameters and assembled by gfx_ws_assembly(2)
+ * into the plan returned by gfx_ws_plan (CSR on the device; export with the
+ * gfx_plan_* functions).  Values of variables/constants are copied. */
+typedef struct gfx_workspace gfx_workspace;
+int gfx_ws_create(gfx_ctx *ctx, gfx_workspace **out);
+int gfx_ws_destroy(gfx_workspace *w);
+/* ga_workspace::add_fem_variable(name, mf, I, V) (.h:453-455): V has
+ * mf's nb_dof values (NULL = zeros); first = I.first() */
+int gfx_ws_add_fem_variable(gfx_workspace *w, const char *name, gfx_mesh_fem *mf,
+                            int64_t first, const double *h_V);
+/* ga_workspace::add_fem_constant(name, mf, V) (.h:465-466) */
+int gfx_ws_add_fem_constant(gfx_workspace *w, const char *name, gfx_mesh_fem *mf,
+                            const double *h_V);
+/* ga_workspace::add_fixed_size_constant(name, V) (.h:467-468) */
+int gfx_ws_add_fixed_size_constant(gfx_workspace *w, const char *name, const double *h_V,
+                                   int64_t n);
+/* new values for a registered name (same size); the pattern is kept */
+int gfx_ws_set_values(gfx_workspace *w, const char *name, const double *h_V);
+/* ga_workspace::add_expression(expr, mim, region = all, der_order) (.h:425-428) */
+int gfx_ws_add_expression(gfx_workspace *w, const char *expr, gfx_mesh_im *mim, int der_order);
+int gfx_ws_set_form(gfx_workspace *w, int form, int precision);
+/* the compile stage alone (GFX_E_UNSUPPORTED with a message outside the subset) */
+int gfx_ws_compile(gfx_workspace *w);
+/* ga_workspace::assembly(order) (.h:564): order 2 */
+int gfx_ws_assembly(gfx_workspace *w, int order);
+int gfx_ws_plan(gfx_workspace *w, gfx_plan **out);
+/* normal form of the compiled term, e.g. "isotropic elasticity unknown=u ..." */
+int gfx_ws_describe(gfx_workspace *w, char *buf, int64_t len);
+
+/* ======================= adjoint (SIMP compliance) =====
+ * J = u^T K(rho, X) u with K the isotropic elasticity matrix of a P1 vector
+ * mesh_fem, E_e = emin + rho_e^penal (1 - emin) scaling (lambda, mu).  The
+ * gradients are what GetFEM's order-1 assembly of the order-0 potential
+ * gives when rho and the (pulled-back) coordinates are variables
+ * (src/getfem_generic_assembly_workspace.cc:520-541, :832-847).
+ * All arrays are device pointers on the context's device, stream-ordered on
+ * its stream; d_X (n_pts*3) may be NULL for the mesh's own points; any
+ * output but d_J may be NULL. */
+typedef struct gfx_compliance gfx_compliance;
+int gfx_compliance_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, double lambda, double mu,
+                          double emin, double penal, gfx_compliance **out);
+int gfx_compliance_eval(gfx_compliance *c, const double *d_X, const double *d_rho,
+                        const double *d_u, double *d_J, double *d_dJ_drho,
+                        double *d_dJ_dX, double *d_dJ_du);
+int gfx_compliance_destroy(gfx_compliance *c);
 
 #ifdef __cplusplus
 }
