@@ -3,7 +3,7 @@
 //   J_e        = E_e |T| W(H),  W = lambda tr(H)^2 + 2 mu eps:eps, H = grad u
 //   dJ/drho_e  = E'(rho_e) |T| W
 //   dJ/dx_v    = sum_e E_e |T| (W I - H^T P) g_v,   P = dW/dH (shape derivative)
-//   dJ/du_v    = sum_e 2 E_e |T| P g_v
+//   dJ/du_v    = sum_e E_e |T| P g_v   (= 2 K u)
 // In GWFL terms this is the order-1 assembly of the order-0 potential with
 // rho (and the pulled-back coordinates) as variables
 // (src/getfem_generic_assembly_workspace.cc:520-541, :832-847).
@@ -20,13 +20,22 @@ namespace {
 
 constexpr int kRedThreads = 256;
 
+// (s, c) += (b, bc) with Knuth's TwoSum on the leading parts
+__device__ __forceinline__ void two_sum_into(double &s, double &c, double b, double bc) {
+  const double t = s + b;
+  const double bb = t - s;
+  const double err = (s - (t - bb)) + (b - bb);
+  s = t;
+  c += err + bc;
+}
+
 __global__ __launch_bounds__(256) void k_compliance_elem(
     int64_t n_cv, const double *__restrict__ X, const int32_t *__restrict__ cvx,
     const int32_t *__restrict__ dofs, const double *__restrict__ u,
     const double *__restrict__ rho, double lam, double mu, double emin, double penal,
     double wsum, double *__restrict__ Jpart, double *__restrict__ drho,
     double *__restrict__ cX, double *__restrict__ cU, int *__restrict__ err) {
-  __shared__ double red[kRedThreads];
+  __shared__ double red[kRedThreads], redc[kRedThreads];
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   double Je = 0.0;
   if (e < n_cv) {
@@ -93,33 +102,41 @@ __global__ __launch_bounds__(256) void k_compliance_elem(
     for (int i = 0; i < 4; ++i)
       for (int a = 0; a < 3; ++a) {
         if (cX) cX[(4 * e + i) * 3 + a] = s * (M[a][0] * g[i][0] + M[a][1] * g[i][1] + M[a][2] * g[i][2]);
-        if (cU)
-          cU[(4 * e + i) * 3 + a] =
-              2.0 * s * (P[a][0] * g[i][0] + P[a][1] * g[i][1] + P[a][2] * g[i][2]);
+        if (cU)  // P = dW/dH already carries the factor 2 of the quadratic form
+          cU[(4 * e + i) * 3 + a] = s * (P[a][0] * g[i][0] + P[a][1] * g[i][1] + P[a][2] * g[i][2]);
       }
   }
+  // compensated (TwoSum) tree reduction: J is accurate to a few ulps so that
+  // finite differences of J resolve its derivatives
   red[threadIdx.x] = Je;
+  redc[threadIdx.x] = 0.0;
   __syncthreads();
   for (int o = kRedThreads / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    if ((int)threadIdx.x < o) two_sum_into(red[threadIdx.x], redc[threadIdx.x], red[threadIdx.x + o],
+                                           redc[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) Jpart[blockIdx.x] = red[0];
+  if (threadIdx.x == 0) {
+    Jpart[2 * blockIdx.x] = red[0];
+    Jpart[2 * blockIdx.x + 1] = redc[0];
+  }
 }
 
-// fixed-order sum of the block partials
+// fixed-order compensated sum of the block partials (value, compensation)
 __global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double *__restrict__ part,
                                                               int64_t n, double *__restrict__ out) {
-  __shared__ double red[kRedThreads];
-  double s = 0.0;
-  for (int64_t i = threadIdx.x; i < n; i += kRedThreads) s += part[i];
+  __shared__ double red[kRedThreads], redc[kRedThreads];
+  double s = 0.0, c = 0.0;
+  for (int64_t i = threadIdx.x; i < n; i += kRedThreads) two_sum_into(s, c, part[2 * i], part[2 * i + 1]);
   red[threadIdx.x] = s;
+  redc[threadIdx.x] = c;
   __syncthreads();
   for (int o = kRedThreads / 2; o > 0; o >>= 1) {
-    if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
+    if ((int)threadIdx.x < o) two_sum_into(red[threadIdx.x], redc[threadIdx.x], red[threadIdx.x + o],
+                                           redc[threadIdx.x + o]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) *out = red[0];
+  if (threadIdx.x == 0) *out = red[0] + redc[0];
 }
 
 // out[node*stride + a] = sum over incidences (e, i) of c[(4e+i)*3 + a]

@@ -708,7 +708,7 @@ int gfx_compliance_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, double lambda, dou
   if (e == hipSuccess) e = dalloc(&c->d_dinc, (size_t)m->n_cv * 4);
   if (e == hipSuccess) e = dalloc(&c->d_cX, (size_t)m->n_cv * 12);
   if (e == hipSuccess) e = dalloc(&c->d_cU, (size_t)m->n_cv * 12);
-  if (e == hipSuccess) e = dalloc(&c->d_Jpart, (size_t)compliance_nblocks(m->n_cv));
+  if (e == hipSuccess) e = dalloc(&c->d_Jpart, 2 * (size_t)compliance_nblocks(m->n_cv));
   if (e == hipSuccess) e = dalloc(&c->d_err, 1);
   if (e == hipSuccess) e = dalloc(&tmp, (size_t)(2 * big + scan_tmp_elems(big)));
   if (e == hipSuccess)
