@@ -24,7 +24,6 @@ sys.path.insert(0, ROOT)
 
 METRIC = "P2-tet elasticity assembly elems/s + HBM GB/s; FP32 vs FP64 form at stated error"
 HBM_PEAK_GBS = 8000.0
-REFERENCE_NNZ_64 = 544886097  # GetFEM's stored nnz at 64^3x6, measured with the reference
 
 
 def bytes_per_element(n_nodes, n_geo, n_coef, n_loc, n_state=0):
@@ -34,15 +33,39 @@ def bytes_per_element(n_nodes, n_geo, n_coef, n_loc, n_state=0):
     return 4 * n_nodes + 24 * n_geo + 8 * n_coef + 8 * n_state + 20 * n_loc * n_loc
 
 
+CONFIGS = {
+    # name: (n, fem, qdim, im, term, n_nodes, n_coef, reference nnz at the default n)
+    "c1": (20, "FEM_PK(3,1)", 1, "IM_TETRAHEDRON(1)", "laplace", 4, 0, 128113),
+    "c2": (64, "FEM_PK(3,2)", 3, "IM_TETRAHEDRON(5)", "elast", 10, 2, 544886097),
+    "c5": (32, "FEM_PK(3,3)", 3, "IM_TETRAHEDRON(8)", "elast", 20, 2, 386702445),
+    "c3": (48, "FEM_QK(3,2)", 3, "IM_GAUSS_PARALLELEPIPED(3,5)", "neohookean", 27, 2, 513599625),
+}
+FP64_PEAK_TFLOPS = 78.6
+WORKLOAD = {
+    "c1": "P1-tet Poisson Grad_u.Grad_Test_u",
+    "c2": "P2-tet isotropic elasticity lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u",
+    "c5": "P3-tet isotropic elasticity lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u",
+    "c3": "Q2-hex compressible Neo-Hookean (Bonet) tangent + residual, lambda=10, mu=1, "
+          "u = sum of 8 sine modes (max |u| = 0.05)",
+}
+
+
 def run_gpu(args):
     import torch
 
     import getfem_amd as ga
     from getfem_amd import meshes
 
-    n = args.n
+    n0, fem, q, im, term, n_nodes, n_coef, ref_nnz = CONFIGS[args.config]
+    n = args.n or n0
     t0 = time.perf_counter()
-    pts, cvx, lam, mu = meshes.c2_inputs(n)
+    hexes = term == "neohookean"
+    pts, cvx = meshes.regular_unit_mesh(n, "GT_QK(3,1)" if hexes else "GT_PK(3,1)")
+    if hexes:
+        lam, mu = 10.0, 1.0
+    else:
+        pts = meshes.jitter(pts, n)
+        lam, mu = meshes.lognormal_lame(cvx.shape[0])
     t_inputs = time.perf_counter() - t0
     n_cv = cvx.shape[0]
     print("inputs: n=%d, %d tets (%.1f s)" % (n, n_cv, t_inputs), file=sys.stderr, flush=True)
@@ -52,12 +75,20 @@ def run_gpu(args):
     stream = torch.cuda.current_stream(dev)
     ctx = ga.Context(0)
     ctx.set_stream(stream.cuda_stream)
-    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    mesh = ga.Mesh(ctx, "GT_QK(3,1)" if hexes else "GT_PK(3,1)", pts, cvx)
     t0 = time.perf_counter()
-    mf = ga.MeshFem(mesh, "FEM_PK(3,2)", 3)
+    mf = ga.MeshFem(mesh, fem, q)
     t_dofs = time.perf_counter() - t0
-    mim = ga.MeshIm(mesh, "IM_TETRAHEDRON(5)")
-    plan = ga.Plan(mf, mim, ga.TERM_ISO_ELAST, [lam, mu])
+    mim = ga.MeshIm(mesh, im)
+    state = None
+    if term == "laplace":
+        plan = ga.Plan(mf, mim, ga.TERM_LAPLACE, [1.0])
+    elif hexes:
+        plan = ga.Plan(mf, mim, ga.TERM_NEOHOOKEAN, [lam, mu])
+        state = meshes.sine_modes()(mf.basic_dof_nodes(), 0.05).ravel()
+        plan.set_state(state)
+    else:
+        plan = ga.Plan(mf, mim, ga.TERM_ISO_ELAST, [lam, mu])
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     nnz = plan.build_pattern()
@@ -67,13 +98,19 @@ def run_gpu(args):
     print("dofs %d (%.1f s host), pattern nnz %d (%.3f s GPU)" % (ndof, t_dofs, nnz, t_pattern),
           file=sys.stderr, flush=True)
 
-    b_elem = bytes_per_element(10, 4, 2, 30)
-    b_patch = 4 * 10 + 20 * 30 * 30  # the contraction/scatter kernel's share
+    n_loc = n_nodes * q
+    if hexes:  # 8 geometric nodes, state gathered, residual assembled
+        b_elem = bytes_per_element(n_nodes, 8, n_coef, n_loc, n_state=n_loc) + 16 * n_loc
+    else:
+        b_elem = bytes_per_element(n_nodes, 4, n_coef, n_loc)
+    b_patch = 4 * n_nodes + 20 * n_loc * n_loc  # the contraction/scatter kernel's share
     variants = [("fp64_factorized", ga.FORM_FACTORIZED, ga.PREC_FP64),
                 ("fp64_perqp", ga.FORM_PERQP, ga.PREC_FP64),
                 ("fp32_factorized", ga.FORM_FACTORIZED, ga.PREC_FP32),
                 ("fp32_perqp", ga.FORM_PERQP, ga.PREC_FP32)]
-    if args.form != "all":
+    if hexes:
+        variants = [("fp64_element_batched", ga.FORM_PERQP, ga.PREC_FP64)]
+    if args.form != "all" and not hexes:
         variants = [v for v in variants if v[0] == args.form]
     forms = {}
     ref_vals = None
@@ -116,7 +153,7 @@ def run_gpu(args):
     rowlen = ip[1:] - ip[:-1]
     pattern = {
         "nnz": int(nnz),
-        "nnz_equals_reference": (int(nnz) == REFERENCE_NNZ_64) if n == 64 else None,
+        "nnz_equals_reference": (int(nnz) == ref_nnz) if n == n0 else None,
         "ndof": int(ndof),
         "build_s": t_pattern,
         "dof_enumeration_s_host": t_dofs,
@@ -138,9 +175,10 @@ def run_gpu(args):
         "dtype": "f64" if "fp64" in variants[0][0] else "f32",
         "data": "synthetic (seeded jittered mesh, lognormal lambda/mu)",
         "config": {
-            "workload": "P2-tet isotropic elasticity, %d^3x6 = %d tets, %d dofs, IM_TETRAHEDRON(5), "
-                        "per-element lambda/mu; step = geometry + contraction/scatter into the "
-                        "prebuilt CSR" % (n, n_cv, ndof),
+            "workload": "%s, %d^3x6 = %d tets, %d dofs, %s, per-element lambda/mu; step = geometry + "
+                        "contraction/scatter into the prebuilt CSR" % (WORKLOAD[args.config], n, n_cv,
+                                                                       ndof, im),
+            "name": args.config,
             "form": variants[0][0],
             "elements": n_cv,
             "nnz": int(nnz),
@@ -160,17 +198,42 @@ def run_gpu(args):
         "forms": forms,
         "pattern": pattern,
     }
-    return out, (pts, cvx, lam, mu)
+    if hexes:
+        # the element kernel (per-point law + 81x81 contraction) is FP64-compute bound;
+        # FLOPs per element of the per-point B^T D B model: 2 nq (n_loc^2 d + n_loc d^2), d = 9
+        flop = 2 * 27 * (n_loc * n_loc * 9 + n_loc * 81)
+        t_el = head["ms_geometry_kernel"] * 1e-3
+        out["roofline"] = {
+            "bound": "mfma", "achieved": flop * n_cv / t_el / 1e12, "peak": FP64_PEAK_TFLOPS,
+            "unit": "TFLOP/s", "frac": flop * n_cv / t_el / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+            "kernel": "k_nh_element", "flop_per_element": flop, "kernel_ms": head["ms_geometry_kernel"],
+            "gather_kernel": {"kernel": "k_gather_emat", "ms": head["ms_assemble_kernel"],
+                              "achieved_gbs": b_patch * n_cv / (head["ms_assemble_kernel"] * 1e-3) / 1e9},
+        }
+        out["dtype"] = "f64"
+    return out, (pts, cvx, lam, mu, fem, q, im, term, state)
 
 
 def cpu_baseline(inputs, threads):
     """The oracle (a C port of GetFEM's per-element/per-point FP64 algorithm
     with its threshold and sorted-column insertion) on a bounded sample."""
     import oracle
-    pts, cvx, lam, mu = inputs
-    dofs, ndof = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 2, 3)
-    prob = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 2, 3, dofs, ndof,
-                          "IM_TETRAHEDRON(5)", oracle.TERM_ISO_ELAST, coef_a=lam, coef_b=mu)
+    pts, cvx, lam, mu, fem, q, im, term, state = inputs
+    k = int(fem[-2])
+    if term == "neohookean":
+        dofs, ndof = oracle.enumerate_dof(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, k, q)
+        prob = oracle.Problem(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, k, q, dofs, ndof, im,
+                              oracle.TERM_NEOHOOKEAN, scal_a=lam, scal_b=mu, U=state)
+    else:
+        dofs, ndof = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, k, q)
+    if term == "neohookean":
+        pass
+    elif term == "laplace":
+        prob = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, k, q, dofs, ndof, im,
+                              oracle.TERM_LAPLACE, scal_a=1.0)
+    else:
+        prob = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, k, q, dofs, ndof, im,
+                              oracle.TERM_ISO_ELAST, coef_a=lam, coef_b=mu)
     res = {}
     for th, m in threads:
         m = min(m, cvx.shape[0])
@@ -188,11 +251,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--n", type=int, default=64)
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--n", type=int, default=0, help="mesh subdivisions (default: the config's)")
     ap.add_argument("--form", default="all")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=0, help="elements for the multi-thread CPU sample")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 1 GiB device copy 5 times (calibrates HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+        src = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_per_launch": 8 << 27}))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
@@ -205,8 +280,9 @@ def main():
                 oracle.build()
             cores = min(16, os.cpu_count() or 1)
             n_cv = inputs[1].shape[0]
-            big = args.cpu_sample or min(n_cv, 12000 * cores)
-            res = cpu_baseline(inputs, [(1, min(20000, n_cv)), (cores, big)])
+            per_core = {"c1": 400000, "c2": 12000, "c5": 3000, "c3": 1000}[args.config]
+            big = args.cpu_sample or min(n_cv, per_core * cores)
+            res = cpu_baseline(inputs, [(1, min(per_core, n_cv)), (cores, big)])
             v1 = res[1][0]
             vm, m, dt = res[cores]
             out["cpu_baseline"] = {

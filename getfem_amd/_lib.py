@@ -14,7 +14,7 @@ ERRORS = {
     1: "GFX_E_INVALID", 2: "GFX_E_UNSUPPORTED", 3: "GFX_E_DEGENERATE_ELEMENT",
     4: "GFX_E_OOM", 5: "GFX_E_HIP", 6: "GFX_E_STATE", 7: "GFX_E_PATTERN",
 }
-TERM_LAPLACE, TERM_ISO_ELAST = 0, 1
+TERM_LAPLACE, TERM_ISO_ELAST, TERM_NEOHOOKEAN = 0, 1, 2
 FORM_FACTORIZED, FORM_PERQP = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 COEF_CONST, COEF_PER_ELEMENT = 0, 1
@@ -61,6 +61,9 @@ SIGNATURES = {
     "gfx_plan_set_form": (_i32, [_P, _i32, _i32]),
     "gfx_plan_build_pattern": (_i32, [_P, ctypes.POINTER(_i64)]),
     "gfx_plan_assemble": (_i32, [_P]),
+    "gfx_plan_set_state": (_i32, [_P, _P]),
+    "gfx_plan_residual": (_i32, [_P, _P]),
+    "gfx_plan_copy_residual_device": (_i32, [_P, _P]),
     "gfx_plan_dims": (_i32, [_P, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "gfx_plan_export_csr": (_i32, [_P, _P, _P, _P]),
     "gfx_plan_export_csc": (_i32, [_P, _P, _P, _P]),

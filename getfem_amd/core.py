@@ -176,6 +176,17 @@ class Plan:
     def assemble(self):
         check(lib().gfx_plan_assemble(self.h), self.ctx.h)
 
+    def set_state(self, U):
+        """Current displacement of a nonlinear (Neo-Hookean) plan."""
+        U = np.ascontiguousarray(U, dtype=np.float64)
+        check(lib().gfx_plan_set_state(self.h, _ptr(U)), self.ctx.h)
+
+    def residual(self):
+        n, _ = self.dims()
+        V = np.zeros(n)
+        check(lib().gfx_plan_residual(self.h, _ptr(V)), self.ctx.h)
+        return V
+
     def dims(self):
         n, z = ctypes.c_int64(), ctypes.c_int64()
         check(lib().gfx_plan_dims(self.h, ctypes.byref(n), ctypes.byref(z)), self.ctx.h)

@@ -348,6 +348,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
+  dfree(p->d_gN); dfree(p->d_E); dfree(p->d_res); dfree(p->d_U); dfree(p->d_rhs);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   delete p;
   return GFX_OK;
@@ -360,13 +361,19 @@ int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef
   gfx_ctx *c = mf->mesh->ctx;
   if (mim->mesh != mf->mesh) return fail(c, GFX_E_INVALID, "mesh_fem and mesh_im on different meshes");
   const gfx_mesh *m = mf->mesh;
-  if (m->gt != GT_TET_P1 || mf->fem.kind != FEM_PK)
-    return fail(c, GFX_E_UNSUPPORTED, "this plan kind needs FEM_PK(3,k) on GT_PK(3,1)");
   const int Q = mf->qdim, nd = mf->fem.nd;
-  if (!((term == GFX_TERM_LAPLACE && (Q == 1 || Q == 3)) || (term == GFX_TERM_ISO_ELAST && Q == 3)))
-    return fail(c, GFX_E_UNSUPPORTED, "term / qdim combination not supported");
-  if (!(nd == 4 || nd == 10 || nd == 20) || (term == GFX_TERM_LAPLACE && Q == 3 && nd == 20))
-    return fail(c, GFX_E_UNSUPPORTED, "element degree not supported for this term");
+  const bool eb = term == GFX_TERM_NEOHOOKEAN;
+  if (eb) {
+    if (m->gt != GT_HEX_Q1 || mf->fem.kind != FEM_QK || mf->fem.k != 2 || Q != 3)
+      return fail(c, GFX_E_UNSUPPORTED, "the Neo-Hookean plan needs FEM_QK(3,2), qdim 3, on GT_QK(3,1)");
+  } else {
+    if (m->gt != GT_TET_P1 || mf->fem.kind != FEM_PK)
+      return fail(c, GFX_E_UNSUPPORTED, "this plan kind needs FEM_PK(3,k) on GT_PK(3,1)");
+    if (!((term == GFX_TERM_LAPLACE && (Q == 1 || Q == 3)) || (term == GFX_TERM_ISO_ELAST && Q == 3)))
+      return fail(c, GFX_E_UNSUPPORTED, "term / qdim combination not supported");
+    if (!(nd == 4 || nd == 10 || nd == 20) || (term == GFX_TERM_LAPLACE && Q == 3 && nd == 20))
+      return fail(c, GFX_E_UNSUPPORTED, "element degree not supported for this term");
+  }
   const int need = term == GFX_TERM_LAPLACE ? 1 : 2;
   if (n_coefs != need || !coefs) return fail(c, GFX_E_INVALID, "wrong number of coefficients");
   if (set_device(c)) return GFX_E_HIP;
@@ -396,8 +403,18 @@ int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef
         for (int a = 0; a < 3; ++a)
           for (int b = 0; b < 3; ++b) R[((size_t)i * nd + j) * 9 + 3 * a + b] += w * g[3 * i + a] * g[3 * j + b];
   }
+  p->eb = eb;
+  std::vector<double> gN((size_t)nq * m->nv * 3), Nv(m->nv);
+  for (int q = 0; q < nq; ++q) geotrans_eval(m->gt, &mim->rule.pts[3 * q], Nv.data(), &gN[(size_t)q * m->nv * 3]);
   hipError_t e = dalloc(&p->d_gref, gref.size());
   if (e == hipSuccess) e = dalloc(&p->d_qw, (size_t)nq);
+  if (e == hipSuccess && eb) e = dalloc(&p->d_gN, gN.size());
+  if (e == hipSuccess && eb) e = hipMemcpy(p->d_gN, gN.data(), 8 * gN.size(), hipMemcpyHostToDevice);
+  if (e == hipSuccess && eb) e = dalloc(&p->d_E, (size_t)m->n_cv * p->nloc * p->nloc);
+  if (e == hipSuccess && eb) e = dalloc(&p->d_res, (size_t)m->n_cv * p->nloc);
+  if (e == hipSuccess && eb) e = dalloc(&p->d_U, (size_t)p->ndof);
+  if (e == hipSuccess && eb) e = dalloc(&p->d_rhs, (size_t)p->ndof);
+  if (e == hipSuccess && eb) e = hipMemset(p->d_U, 0, 8 * (size_t)p->ndof);
   if (e == hipSuccess) e = dalloc(&p->d_R, R.size());
   if (e == hipSuccess) e = dalloc(&p->d_rec, (size_t)kRecStride * m->n_cv);
   if (e == hipSuccess) e = dalloc(&p->d_thr, (size_t)m->n_cv);
@@ -438,6 +455,20 @@ static int plan_records(gfx_plan *p, bool with_thr) {
   return GFX_OK;
 }
 
+static int plan_eb_elements(gfx_plan *p) {
+  const gfx_mesh *m = p->mf->mesh;
+  gfx_ctx *c = p->ctx;
+  EbArgs a{};
+  a.n_cv = m->n_cv; a.pts = m->d_pts; a.cvx = m->d_cvx; a.dofs = p->mf->d_dofs; a.U = p->d_U;
+  a.ca = p->coef[0].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[0] : nullptr;
+  a.cb = p->coef[1].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[1] : nullptr;
+  a.ca_c = p->coef[0].value; a.cb_c = p->coef[1].value;
+  a.gN = p->d_gN; a.gref = p->d_gref; a.qw = p->d_qw; a.nq = p->nq;
+  a.E = p->d_E; a.res = p->d_res; a.thr = p->d_thr; a.err = p->d_err;
+  HIPCHK(c, launch_nh_element(c->stream, a));
+  return GFX_OK;
+}
+
 static int check_err_flag(gfx_plan *p) {
   gfx_ctx *c = p->ctx;
   int flag = 0;
@@ -458,7 +489,7 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   hipStream_t s = c->stream;
   int r;
   HIPCHK(c, hipMemsetAsync(p->d_err, 0, 4, s));
-  if ((r = plan_records(p, false))) return r;
+  if ((r = p->eb ? plan_eb_elements(p) : plan_records(p, false))) return r;
   // incidences
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots);
   dfree(p->d_nbr); dfree(p->d_mask); dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals);
@@ -490,7 +521,8 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   a.kwords = p->kwords; a.mwpi = p->mwpi; a.dofs = p->mf->d_dofs; a.rec = p->d_rec;
   a.gref = p->d_gref; a.qw = p->d_qw; a.nq = p->nq; a.thr = p->d_thr; a.keep = p->d_keep;
   a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.err = p->d_err;
-  HIPCHK(c, launch_keep(s, p->term, a));
+  if (p->eb) HIPCHK(c, launch_keep_emat(s, n_cv, p->nloc, p->kwords, p->d_E, p->d_thr, p->d_keep));
+  else HIPCHK(c, launch_keep(s, p->term, a));
   // patches
   HIPCHK(c, dalloc(&p->d_slot, (size_t)p->n_inc * nd));
   HIPCHK(c, dalloc(&p->d_nslots, (size_t)nn));
@@ -548,15 +580,50 @@ int gfx_plan_assemble(gfx_plan *p) {
     p->n_timed++;
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
   }
-  if ((r = plan_records(p, true))) return r;
+  if ((r = p->eb ? plan_eb_elements(p) : plan_records(p, true))) return r;
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   PatchArgs a{};
   a.n_nodes = p->n_nodes; a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot;
   a.nslots = p->d_nslots; a.mask = p->d_mask; a.mwpi = p->mwpi; a.indptr = p->d_indptr;
   a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
   a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals;
-  HIPCHK(c, launch_assemble_patch(c->stream, p->term, p->form, p->precision, a, p->max_nslots));
+  if (p->eb) {
+    HIPCHK(c, launch_gather_emat(c->stream, a, p->d_E, p->d_thr, p->max_nslots));
+    HIPCHK(c, launch_gather_vec(c->stream, p->n_nodes, p->nd, p->qdim, p->d_inc_ptr, p->d_inc, p->d_res,
+                                p->d_rhs));
+  } else {
+    HIPCHK(c, launch_assemble_patch(c->stream, p->term, p->form, p->precision, a, p->max_nslots));
+  }
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
+  return GFX_OK;
+}
+
+int gfx_plan_set_state(gfx_plan *p, const double *h_U) {
+  if (!p || !h_U) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (!p->eb) return fail(c, GFX_E_STATE, "this plan has no state");
+  if (set_device(c)) return GFX_E_HIP;
+  HIPCHK(c, hipMemcpyAsync(p->d_U, h_U, 8 * (size_t)p->ndof, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFX_OK;
+}
+
+int gfx_plan_residual(gfx_plan *p, double *h_V) {
+  if (!p || !h_V) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (!p->eb) return fail(c, GFX_E_STATE, "this plan has no residual");
+  if (set_device(c)) return GFX_E_HIP;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(h_V, p->d_rhs, 8 * (size_t)p->ndof, hipMemcpyDeviceToHost));
+  return GFX_OK;
+}
+
+int gfx_plan_copy_residual_device(gfx_plan *p, double *d_V) {
+  if (!p || !d_V) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (!p->eb) return fail(c, GFX_E_STATE, "this plan has no residual");
+  if (set_device(c)) return GFX_E_HIP;
+  HIPCHK(c, hipMemcpyAsync(d_V, p->d_rhs, 8 * (size_t)p->ndof, hipMemcpyDeviceToDevice, c->stream));
   return GFX_OK;
 }
 

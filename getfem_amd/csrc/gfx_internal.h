@@ -90,6 +90,24 @@ struct PatternArgs {
   int *err;                 // device error flag
 };
 
+struct EbArgs {
+  int64_t n_cv;
+  const double *pts;
+  const int32_t *cvx;
+  const int32_t *dofs;
+  const double *U;          // state (ndof)
+  const double *ca, *cb;    // per-element coefficients or null
+  double ca_c, cb_c;
+  const double *gN;         // nq*8*3 transformation gradients
+  const double *gref;       // nq*nd*3 base gradients
+  const double *qw;
+  int nq;
+  double *E;                // n_cv * nl * nl element matrices (row-major)
+  double *res;              // n_cv * nl element residuals
+  double *thr;              // n_cv thresholds
+  int *err;
+};
+
 // kernel launch wrappers (kernels.hip)
 hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,
                               const int32_t *cvx, const double *ca, double ca_c,
@@ -108,6 +126,15 @@ hipError_t scan_exclusive_i64(hipStream_t s, const int64_t *in, int64_t *out,
 hipError_t scan_exclusive_i32(hipStream_t s, const int32_t *in, int32_t *out,
                               int64_t n, int32_t *tmp);
 int64_t scan_tmp_elems(int64_t n);
+
+// kernels_eb.hip
+hipError_t launch_nh_element(hipStream_t s, const EbArgs &a);
+hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, const double *E,
+                            const double *thr, uint32_t *keep);
+hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a, const double *E, const double *thr,
+                              int max_nslots);
+hipError_t launch_gather_vec(hipStream_t s, int64_t n_nodes, int nd, int Q, const int32_t *inc_ptr,
+                             const uint32_t *inc, const double *res, double *V);
 
 // adjoint.hip
 int64_t compliance_nblocks(int64_t n_cv);
@@ -157,6 +184,9 @@ struct gfx_plan {
   int *d_err = nullptr;
   int form = GFX_FORM_FACTORIZED;
   int precision = GFX_PREC_FP64;
+  // element-batched plans (non-affine / nonlinear): buffered element matrices
+  bool eb = false;
+  double *d_gN = nullptr, *d_E = nullptr, *d_res = nullptr, *d_U = nullptr, *d_rhs = nullptr;
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 3 events per timed call
   int n_timed = 0;

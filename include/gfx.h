@@ -152,8 +152,15 @@ int gfx_plan_set_form(gfx_plan *p, int form, int precision);
  * |e| > 1e-14 * max|elem| in at least one element matrix
  * (compile_and_exec.cc:4853-4936, :5438-5467), built on the GPU. */
 int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz);
-/* assembly(2) into the pattern's values (device resident) */
+/* assembly(2) into the pattern's values (device resident); for a
+ * GFX_TERM_NEOHOOKEAN plan also the residual (order-1 assembly of the same
+ * expression, ga_instruction_vector_assembly_mf, compile_and_exec.cc:4669-4735) */
 int gfx_plan_assemble(gfx_plan *p);
+/* nonlinear plans: the current displacement (ndof host doubles) and the
+ * assembled residual vector */
+int gfx_plan_set_state(gfx_plan *p, const double *h_U);
+int gfx_plan_residual(gfx_plan *p, double *h_V);
+int gfx_plan_copy_residual_device(gfx_plan *p, double *d_V);
 int gfx_plan_dims(gfx_plan *p, int64_t *nrows, int64_t *nnz);
 int gfx_plan_export_csr(gfx_plan *p, int64_t *h_indptr, int32_t *h_indices,
                         double *h_vals);
