@@ -109,8 +109,9 @@ def run_gpu(args):
                 ("fp32_factorized", ga.FORM_FACTORIZED, ga.PREC_FP32),
                 ("fp32_perqp", ga.FORM_PERQP, ga.PREC_FP32)]
     if hexes:
-        variants = [("fp64_element_batched", ga.FORM_PERQP, ga.PREC_FP64)]
-    if args.form != "all" and not hexes:
+        variants = [("fp64_element_batched", ga.FORM_PERQP, ga.PREC_FP64),
+                    ("fp32_element_batched", ga.FORM_PERQP, ga.PREC_FP32)]
+    if args.form != "all":
         variants = [v for v in variants if v[0] == args.form]
     forms = {}
     ref_vals = None

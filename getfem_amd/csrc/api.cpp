@@ -455,7 +455,7 @@ static int plan_records(gfx_plan *p, bool with_thr) {
   return GFX_OK;
 }
 
-static int plan_eb_elements(gfx_plan *p) {
+static int plan_eb_elements(gfx_plan *p, int precision) {
   const gfx_mesh *m = p->mf->mesh;
   gfx_ctx *c = p->ctx;
   EbArgs a{};
@@ -465,7 +465,7 @@ static int plan_eb_elements(gfx_plan *p) {
   a.ca_c = p->coef[0].value; a.cb_c = p->coef[1].value;
   a.gN = p->d_gN; a.gref = p->d_gref; a.qw = p->d_qw; a.nq = p->nq;
   a.E = p->d_E; a.res = p->d_res; a.thr = p->d_thr; a.err = p->d_err;
-  HIPCHK(c, launch_nh_element(c->stream, a));
+  HIPCHK(c, launch_nh_element(c->stream, a, precision));
   return GFX_OK;
 }
 
@@ -489,7 +489,7 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   hipStream_t s = c->stream;
   int r;
   HIPCHK(c, hipMemsetAsync(p->d_err, 0, 4, s));
-  if ((r = p->eb ? plan_eb_elements(p) : plan_records(p, false))) return r;
+  if ((r = p->eb ? plan_eb_elements(p, GFX_PREC_FP64) : plan_records(p, false))) return r;
   // incidences
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots);
   dfree(p->d_nbr); dfree(p->d_mask); dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals);
@@ -512,6 +512,7 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   int maxinc = 0;
   for (int64_t n = 0; n < nn; ++n) maxinc = std::max(maxinc, iptr[n + 1] - iptr[n]);
   p->max_cand = maxinc * nd;
+  p->max_inc = maxinc;
   if (p->max_cand > 4096)
     return fail(c, GFX_E_PATTERN, "a node belongs to too many elements for the patch kernels");
   // thresholds and keep bits
@@ -580,13 +581,13 @@ int gfx_plan_assemble(gfx_plan *p) {
     p->n_timed++;
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
   }
-  if ((r = p->eb ? plan_eb_elements(p) : plan_records(p, true))) return r;
+  if ((r = p->eb ? plan_eb_elements(p, p->precision) : plan_records(p, true))) return r;
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   PatchArgs a{};
   a.n_nodes = p->n_nodes; a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot;
   a.nslots = p->d_nslots; a.mask = p->d_mask; a.mwpi = p->mwpi; a.indptr = p->d_indptr;
   a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
-  a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals;
+  a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals; a.max_inc = p->max_inc;
   if (p->eb) {
     HIPCHK(c, launch_gather_emat(c->stream, a, p->d_E, p->d_thr, p->max_nslots));
     HIPCHK(c, launch_gather_vec(c->stream, p->n_nodes, p->nd, p->qdim, p->d_inc_ptr, p->d_inc, p->d_res,

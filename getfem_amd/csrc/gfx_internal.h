@@ -66,6 +66,8 @@ struct PatchArgs {
   const double *R;          // nd*nd*9 weighted reference gradient products
   double *vals;             // CSR values
   int xcd_chunk;            // nodes per XCD in the block swizzle
+  int max_inc;              // largest number of incidences of a node
+  int lds_acc, lds_mask, lds_inc, lds_wave;  // dynamic LDS layout (set by the launcher)
 };
 
 struct PatternArgs {
@@ -128,7 +130,7 @@ hipError_t scan_exclusive_i32(hipStream_t s, const int32_t *in, int32_t *out,
 int64_t scan_tmp_elems(int64_t n);
 
 // kernels_eb.hip
-hipError_t launch_nh_element(hipStream_t s, const EbArgs &a);
+hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision);
 hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, const double *E,
                             const double *thr, uint32_t *keep);
 hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a, const double *E, const double *thr,
@@ -176,7 +178,7 @@ struct gfx_plan {
   uint16_t *d_slot = nullptr, *d_nslots = nullptr;
   int32_t *d_nbr = nullptr;
   uint32_t *d_mask = nullptr;
-  int max_nslots = 0, max_cand = 0;
+  int max_nslots = 0, max_cand = 0, max_inc = 0;
   int64_t ndof = 0, nnz = -1;
   int64_t *d_indptr = nullptr;
   int32_t *d_indices = nullptr;

@@ -358,34 +358,49 @@ __global__ __launch_bounds__(64) void k_fill_cols(PatternArgs a) {
 // ------------------------------------------------------------------------
 // the assembly kernel: one workgroup (one wavefront) per node patch
 // ------------------------------------------------------------------------
+// Dynamic LDS: [acc: Q*Q*max_ns doubles][mask: mwmax words][inc: max_inc]
+// [slots: max_inc*ND u16].  The patch's incidences, slot maps and presence
+// words are staged in one round of independent loads, so a wavefront waits
+// on two memory latencies (staging, element records) before it computes.
+constexpr int kPatchWaves = 1;  // node patches per workgroup, one per wavefront (1: LDS sized per patch)
+
 template <int ND, int Q, int TERM, int FORM, typename Real>
-__global__ __launch_bounds__(64) void k_assemble_patch(PatchArgs a) {
-  extern __shared__ double acc[];
-  // XCD-aware mapping: consecutive workgroups land on different XCDs; give
-  // each XCD a contiguous range of nodes so neighbouring patches (which share
-  // elements) share that XCD's L2
+__global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArgs a) {
+  extern __shared__ double lds_all[];
   const int64_t wg = blockIdx.x;
-  const int64_t node = (wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3);
-  if (node >= a.n_nodes) return;
-  const int lane = threadIdx.x;
-  const int i0 = a.inc_ptr[node], ninc = a.inc_ptr[node + 1] - i0;
-  const int ns = a.nslots[node];
-  const int dense = Q * Q * ns;
+  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const int64_t node = ((wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3)) * kPatchWaves + w;
+  const bool live = node < a.n_nodes;
+  const int i0 = live ? a.inc_ptr[node] : 0, ninc = live ? a.inc_ptr[node + 1] - i0 : 0;
+  const int ns = live ? a.nslots[node] : 0;
+  int64_t rowbase[Q];
+  for (int x = 0; x < Q; ++x) rowbase[x] = live ? a.indptr[node * Q + x] : 0;
+  double *lds = lds_all + (size_t)w * a.lds_wave;  // this wavefront's region
+  double *acc = lds;
+  uint32_t *smask = reinterpret_cast<uint32_t *>(lds + a.lds_acc);
+  uint32_t *sinc = smask + a.lds_mask;
+  uint16_t *sslot = reinterpret_cast<uint16_t *>(sinc + a.lds_inc);
+  const int dense = Q * Q * ns, mw = (dense + 31) / 32;
   for (int t = lane; t < dense; t += kWave) acc[t] = 0.0;
+  const uint32_t *gm = a.mask + (int64_t)i0 * a.mwpi;  // ninc = 0 when not live: no loads
+  for (int t = lane; t < mw; t += kWave) smask[t] = gm[t];
+  for (int t = lane; t < ninc; t += kWave) sinc[t] = a.inc[i0 + t];
+  const uint16_t *gs = a.slot + (int64_t)i0 * ND;
+  for (int t = lane; t < ninc * ND; t += kWave) sslot[t] = gs[t];
   __syncthreads();
   constexpr int PER = kWave / ND;  // incidences per pass
   const int kk = lane / ND, j = lane - kk * ND;
   for (int k0 = 0; k0 < ninc; k0 += PER) {
     const int k = k0 + kk;
     if (kk < PER && k < ninc) {
-      const uint32_t inc = a.inc[i0 + k];
+      const uint32_t inc = sinc[k];
       const int64_t e = inc >> 5;
       const int i = inc & 31;
       const double *rec = a.rec + (int64_t)kRecStride * e;
       const double thr = rec[12];
       Real v[Q][Q];
       pair_block<Q, TERM, FORM, Real>(rec, i, j, ND, a.nq, a.gref, a.qw, a.R, v);
-      const int s = a.slot[(int64_t)(i0 + k) * ND + j];
+      const int s = sslot[k * ND + j];
       for (int x = 0; x < Q; ++x)
         for (int y = 0; y < Q; ++y) {
           const double val = (double)v[x][y];
@@ -394,13 +409,12 @@ __global__ __launch_bounds__(64) void k_assemble_patch(PatchArgs a) {
     }
   }
   __syncthreads();
-  const uint32_t *m = a.mask + (int64_t)i0 * a.mwpi;
   for (int x = 0; x < Q; ++x) {
-    int64_t base = a.indptr[node * Q + x];
-    const int off = x * ns * Q, len = ns * Q;
+    int64_t base = rowbase[x];
+    const int off = x * ns * Q, len = ns * Q;  // len = 0 when not live
     for (int c0 = 0; c0 < len; c0 += kWave) {
       const int d = c0 + lane;
-      const bool pr = d < len && mask_bit(m, off + d);
+      const bool pr = d < len && mask_bit(smask, off + d);
       const uint64_t bal = __ballot(pr);
       if (pr) a.vals[base + __popcll(bal & ((1ull << lane) - 1ull))] = acc[off + d];
       base += __popcll(bal);
@@ -563,14 +577,14 @@ static hipError_t launch_asm3(hipStream_t s, int form, int prec, const PatchArgs
                               unsigned grid, size_t lds) {
   if (form == GFX_FORM_FACTORIZED) {
     if (prec == GFX_PREC_FP64)
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, double><<<grid, 64, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, double><<<grid, 64 * kPatchWaves, lds, s>>>(a);
     else
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, float><<<grid, 64, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, float><<<grid, 64 * kPatchWaves, lds, s>>>(a);
   } else {
     if (prec == GFX_PREC_FP64)
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, double><<<grid, 64, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, double><<<grid, 64 * kPatchWaves, lds, s>>>(a);
     else
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, float><<<grid, 64, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, float><<<grid, 64 * kPatchWaves, lds, s>>>(a);
   }
   return hipGetLastError();
 }
@@ -579,9 +593,17 @@ hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec,
                                  const PatchArgs &a0, int max_nslots) {
   if (a0.n_nodes == 0) return hipSuccess;
   PatchArgs a = a0;
-  a.xcd_chunk = (int)((a.n_nodes + 7) / 8);
+  const int64_t groups = (a.n_nodes + kPatchWaves - 1) / kPatchWaves;
+  a.xcd_chunk = (int)((groups + 7) / 8);
   const unsigned grid = (unsigned)(8 * (int64_t)a.xcd_chunk);
-  const size_t lds = (size_t)a.qdim * a.qdim * max_nslots * sizeof(double);
+  const int dense = a.qdim * a.qdim * max_nslots;
+  a.lds_acc = dense;                                   // doubles
+  a.lds_mask = ((dense + 31) / 32 + 1) & ~1;           // u32 words, keeps 8-byte alignment
+  a.lds_inc = (a.max_inc + 1) & ~1;                    // u32 words
+  const size_t per_wave = 8 * (size_t)a.lds_acc + 4 * (size_t)(a.lds_mask + a.lds_inc) +
+                          2 * (size_t)a.max_inc * a.nd;
+  a.lds_wave = (int)((per_wave + 7) / 8);              // doubles per wavefront region
+  const size_t lds = 8 * (size_t)a.lds_wave * kPatchWaves;
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   const int nd = a.nd, Q = a.qdim;
   if (term == GFX_TERM_LAPLACE && Q == 1) {

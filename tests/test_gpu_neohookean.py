@@ -72,3 +72,20 @@ def test_neohookean_full_size_pattern(ctx):
     assert nnz == 513599625
     R = plan.residual()
     assert np.all(np.isfinite(R))
+
+
+def test_neohookean_fp32_element_form(ctx):
+    """FP32 contraction, FP64 geometry/law/threshold/accumulation: error
+    against the FP64 oracle stated and bounded."""
+    pts, cvx, mf, plan, U, keep = nh_setup(ctx, 3)
+    plan.build_pattern()
+    plan.set_form(ga.FORM_PERQP, ga.PREC_FP32)
+    plan.assemble()
+    ip, ix, v = plan.csr()
+    P = oracle.Problem(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, 2, 3, mf.basic_dof_table(), mf.nbdof(),
+                       "IM_GAUSS_PARALLELEPIPED(3,5)", oracle.TERM_NEOHOOKEAN, scal_a=10.0, scal_b=1.0, U=U)
+    ipr, ixr, vr = oracle.csc_to_csr(*P.assemble_csc(), mf.nbdof())
+    assert np.array_equal(ix, ixr)
+    err = problems.rel_frob(v, vr)
+    print("fp32 element form: rel frobenius error", err)
+    assert err <= 1e-6
