@@ -7,7 +7,8 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "_build", "libgfx.so")
+# GFX_LIBRARY names another build of the same library (same ABI)
+LIB_PATH = os.environ.get("GFX_LIBRARY") or os.path.join(_HERE, "_build", "libgfx.so")
 
 GFX_OK = 0
 ERRORS = {
@@ -62,7 +63,9 @@ SIGNATURES = {
     "gfx_plan_build_pattern": (_i32, [_P, ctypes.POINTER(_i64)]),
     "gfx_plan_assemble": (_i32, [_P]),
     "gfx_plan_set_state": (_i32, [_P, _P]),
+    "gfx_plan_assemble_residual": (_i32, [_P]),
     "gfx_plan_residual": (_i32, [_P, _P]),
+    "gfx_plan_spmv": (_i32, [_P, _P, _P]),
     "gfx_plan_copy_residual_device": (_i32, [_P, _P]),
     "gfx_plan_dims": (_i32, [_P, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "gfx_plan_export_csr": (_i32, [_P, _P, _P, _P]),
@@ -85,6 +88,7 @@ SIGNATURES = {
     "gfx_ws_compile": (_i32, [_P]),
     "gfx_ws_assembly": (_i32, [_P, _i32]),
     "gfx_ws_plan": (_i32, [_P, _pp]),
+    "gfx_ws_assembled_vector": (_i32, [_P, _P]),
     "gfx_ws_describe": (_i32, [_P, ctypes.c_char_p, _i64]),
     "gfx_compliance_create": (_i32, [_P, _P, _f64, _f64, _f64, _f64, _pp]),
     "gfx_compliance_eval": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),

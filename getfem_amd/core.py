@@ -181,6 +181,14 @@ class Plan:
         U = np.ascontiguousarray(U, dtype=np.float64)
         check(lib().gfx_plan_set_state(self.h, _ptr(U)), self.ctx.h)
 
+    def assemble_residual(self):
+        """Order-1 assembly alone (nonlinear plans, after build_pattern)."""
+        check(lib().gfx_plan_assemble_residual(self.h), self.ctx.h)
+
+    def spmv(self, d_x, d_y):
+        """d_y = K d_x on the device (raw device addresses, e.g. data_ptr())."""
+        check(lib().gfx_plan_spmv(self.h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y)), self.ctx.h)
+
     def residual(self):
         n, _ = self.dims()
         V = np.zeros(n)

@@ -345,7 +345,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_coef[0]); dfree(p->d_coef[1]);
   dfree(p->d_gref); dfree(p->d_qw); dfree(p->d_R);
   dfree(p->d_rec); dfree(p->d_thr); dfree(p->d_keep);
-  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots);
+  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
   dfree(p->d_gN); dfree(p->d_E); dfree(p->d_res); dfree(p->d_U); dfree(p->d_rhs);
@@ -479,6 +479,58 @@ static int check_err_flag(gfx_plan *p) {
   return GFX_OK;
 }
 
+// Group the nodes by patch size (number of neighbour slots).  The assembly
+// kernels reserve LDS for the largest patch of a launch, so one launch per
+// group lets the many small patches run at the occupancy their own size
+// allows.  Size classes with few nodes join the next class up.
+static int plan_bins(gfx_plan *p, const std::vector<uint16_t> &ns) {
+  gfx_ctx *c = p->ctx;
+  const int64_t nn = p->n_nodes;
+  static const int kCaps[] = {8,   16,  24,  32,   48,   64,   96,   128,  192,   256,   384,
+                              512, 768, 1024, 1536, 2048, 3072, 4096, 8192, 16384, 32767};
+  constexpr int kClasses = sizeof(kCaps) / sizeof(kCaps[0]);
+  constexpr int64_t kMinBin = 16384;  // nodes; below this a launch is not worth its overhead
+  std::vector<int> cls((size_t)nn);
+  std::vector<int64_t> cnt(kClasses, 0);
+  std::vector<int> cmax(kClasses, 0);
+  for (int64_t n = 0; n < nn; ++n) {
+    const int v = ns[n] & 0x7fff;
+    int k = 0;
+    while (kCaps[k] < v) ++k;
+    cls[n] = k;
+    cnt[k]++;
+    cmax[k] = std::max(cmax[k], v);
+  }
+  int last = -1;
+  for (int k = 0; k < kClasses; ++k)
+    if (cnt[k]) last = k;
+  std::vector<int> group(kClasses, 0);
+  p->bins.clear();
+  int64_t pending = 0, off = 0;
+  int gmax = 0;
+  for (int k = 0; k <= last; ++k) {
+    group[k] = (int)p->bins.size();
+    pending += cnt[k];
+    gmax = std::max(gmax, cmax[k]);
+    if (pending && (pending >= kMinBin || k == last)) {
+      p->bins.push_back({off, pending, std::max(gmax, 1)});
+      off += pending;
+      pending = 0;
+      gmax = 0;
+    }
+  }
+  std::vector<int64_t> cur(p->bins.size());
+  for (size_t b = 0; b < p->bins.size(); ++b) cur[b] = p->bins[b].off;
+  std::vector<int32_t> order((size_t)nn);
+  for (int64_t n = 0; n < nn; ++n) order[cur[group[cls[n]]]++] = (int32_t)n;
+  dfree(p->d_order);
+  if (nn) {
+    HIPCHK(c, dalloc(&p->d_order, (size_t)nn));
+    HIPCHK(c, hipMemcpy(p->d_order, order.data(), 4 * (size_t)nn, hipMemcpyHostToDevice));
+  }
+  return GFX_OK;
+}
+
 int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   if (!p) return GFX_E_INVALID;
   gfx_ctx *c = p->ctx;
@@ -491,7 +543,7 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   HIPCHK(c, hipMemsetAsync(p->d_err, 0, 4, s));
   if ((r = p->eb ? plan_eb_elements(p, GFX_PREC_FP64) : plan_records(p, false))) return r;
   // incidences
-  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots);
+  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order);
   dfree(p->d_nbr); dfree(p->d_mask); dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals);
   dfree(p->d_keep);
   p->nnz = -1;
@@ -548,9 +600,10 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   std::vector<uint16_t> ns((size_t)nn);
   HIPCHK(c, hipMemcpy(ns.data(), p->d_nslots, 2 * (size_t)nn, hipMemcpyDeviceToHost));
   p->max_nslots = 0;
-  for (int64_t n = 0; n < nn; ++n) p->max_nslots = std::max<int>(p->max_nslots, ns[n]);
+  for (int64_t n = 0; n < nn; ++n) p->max_nslots = std::max<int>(p->max_nslots, ns[n] & 0x7fff);
   if ((size_t)Q * Q * p->max_nslots * 8 > 64 * 1024)
     return fail(c, GFX_E_PATTERN, "node patch too large for the assembly kernel");
+  if ((r = plan_bins(p, ns))) return r;
   HIPCHK(c, dalloc(&p->d_indices, (size_t)nnz));
   HIPCHK(c, dalloc(&p->d_vals, (size_t)nnz));
   a.indices = p->d_indices;
@@ -584,17 +637,19 @@ int gfx_plan_assemble(gfx_plan *p) {
   if ((r = p->eb ? plan_eb_elements(p, p->precision) : plan_records(p, true))) return r;
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   PatchArgs a{};
-  a.n_nodes = p->n_nodes; a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot;
+  a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot;
   a.nslots = p->d_nslots; a.mask = p->d_mask; a.mwpi = p->mwpi; a.indptr = p->d_indptr;
   a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
   a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals; a.max_inc = p->max_inc;
-  if (p->eb) {
-    HIPCHK(c, launch_gather_emat(c->stream, a, p->d_E, p->d_thr, p->max_nslots));
+  for (const gfx_plan::Bin &b : p->bins) {  // one launch per patch-size bin
+    a.n_nodes = b.count;
+    a.order = p->d_order + b.off;
+    if (p->eb) HIPCHK(c, launch_gather_emat(c->stream, a, p->d_E, p->d_thr, b.cap));
+    else HIPCHK(c, launch_assemble_patch(c->stream, p->term, p->form, p->precision, a, b.cap));
+  }
+  if (p->eb)
     HIPCHK(c, launch_gather_vec(c->stream, p->n_nodes, p->nd, p->qdim, p->d_inc_ptr, p->d_inc, p->d_res,
                                 p->d_rhs));
-  } else {
-    HIPCHK(c, launch_assemble_patch(c->stream, p->term, p->form, p->precision, a, p->max_nslots));
-  }
   if (ev) HIPCHK(c, hipEventRecord(ev[2], c->stream));
   return GFX_OK;
 }
@@ -606,6 +661,28 @@ int gfx_plan_set_state(gfx_plan *p, const double *h_U) {
   if (set_device(c)) return GFX_E_HIP;
   HIPCHK(c, hipMemcpyAsync(p->d_U, h_U, 8 * (size_t)p->ndof, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  return GFX_OK;
+}
+
+int gfx_plan_assemble_residual(gfx_plan *p) {
+  if (!p) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (!p->eb) return fail(c, GFX_E_STATE, "this plan has no residual");
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
+  if (set_device(c)) return GFX_E_HIP;
+  int r;
+  if ((r = plan_eb_elements(p, p->precision))) return r;
+  HIPCHK(c, launch_gather_vec(c->stream, p->n_nodes, p->nd, p->qdim, p->d_inc_ptr, p->d_inc, p->d_res,
+                              p->d_rhs));
+  return GFX_OK;
+}
+
+int gfx_plan_spmv(gfx_plan *p, const double *d_x, double *d_y) {
+  if (!p || !d_x || !d_y) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
+  if (set_device(c)) return GFX_E_HIP;
+  HIPCHK(c, launch_spmv(c->stream, p->ndof, p->nnz, p->d_indptr, p->d_indices, p->d_vals, d_x, d_y));
   return GFX_OK;
 }
 

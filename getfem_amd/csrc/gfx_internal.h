@@ -52,6 +52,7 @@ constexpr int kRecStride = 16;  // doubles per element record
 
 struct PatchArgs {
   int64_t n_nodes;
+  const int32_t *order;     // node ids of this launch's bin (null: nodes 0..n_nodes-1)
   const int32_t *inc_ptr;   // n_nodes+1
   const uint32_t *inc;      // (element << 5) | local node
   const uint16_t *slot;     // per incidence, nd slots
@@ -138,6 +139,10 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a, const double *E
 hipError_t launch_gather_vec(hipStream_t s, int64_t n_nodes, int nd, int Q, const int32_t *inc_ptr,
                              const uint32_t *inc, const double *res, double *V);
 
+// spmv.hip
+hipError_t launch_spmv(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr,
+                       const int32_t *indices, const double *vals, const double *x, double *y);
+
 // adjoint.hip
 int64_t compliance_nblocks(int64_t n_cv);
 hipError_t launch_compliance(hipStream_t s, int64_t n_cv, const double *X, const int32_t *cvx,
@@ -179,6 +184,11 @@ struct gfx_plan {
   int32_t *d_nbr = nullptr;
   uint32_t *d_mask = nullptr;
   int max_nslots = 0, max_cand = 0, max_inc = 0;
+  // nodes grouped by patch size so each launch reserves LDS for its own
+  // largest patch: d_order holds the bins back to back
+  int32_t *d_order = nullptr;
+  struct Bin { int64_t off, count; int cap; };
+  std::vector<Bin> bins;
   int64_t ndof = 0, nnz = -1;
   int64_t *d_indptr = nullptr;
   int32_t *d_indices = nullptr;

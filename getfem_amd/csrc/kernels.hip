@@ -27,6 +27,7 @@ namespace {
 
 constexpr int kWave = 64;
 constexpr int kMaxCand = 4096;  // neighbour candidates per node (LDS sort size)
+constexpr int kFullRows = 0x8000;  // flag in nslots: all of the node's rows are full
 
 __device__ __forceinline__ int64_t gtid() {
   return (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,7 +287,7 @@ __global__ __launch_bounds__(256) void k_patch_build(PatternArgs a) {
     const int orig = (int)(keys[t] & 0xffffffffu);
     slotL[orig] = (uint16_t)(run - 1);
   }
-  if (ns > 65535) { if (threadIdx.x == 0) atomicOr(a.err, 2); return; }
+  if (ns >= kFullRows) { if (threadIdx.x == 0) atomicOr(a.err, 2); return; }
   const int mw = (Q * Q * ns + 31) / 32;
   for (int t = threadIdx.x; t < mw; t += blockDim.x) maskL[t] = 0u;
   __syncthreads();
@@ -323,7 +324,13 @@ __global__ __launch_bounds__(256) void k_patch_build(PatternArgs a) {
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0) a.nslots[node] = (uint16_t)ns;
+  // bit 15 of nslots: every (row component, slot, column component) is
+  // present, so the node's rows are a plain copy of its dense accumulator
+  if (threadIdx.x == 0) {
+    int tot = 0;
+    for (int x = 0; x < Q; ++x) tot += rowc[x];
+    a.nslots[node] = (uint16_t)(ns | (tot == Q * Q * ns ? kFullRows : 0));
+  }
   if (threadIdx.x < Q) a.rowcnt[node * Q + threadIdx.x] = rowc[threadIdx.x];
 }
 
@@ -336,7 +343,7 @@ __global__ __launch_bounds__(64) void k_fill_cols(PatternArgs a) {
   if (node >= a.n_nodes) return;
   const int lane = threadIdx.x, Q = a.qdim, nd = a.nd;
   const int i0 = a.inc_ptr[node];
-  const int ns = a.nslots[node];
+  const int ns = a.nslots[node] & (kFullRows - 1);
   const uint32_t *m = a.mask + (int64_t)i0 * a.mwpi;
   const int32_t *nb = a.nbr + (int64_t)i0 * nd;
   for (int x = 0; x < Q; ++x) {
@@ -369,10 +376,13 @@ __global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArg
   extern __shared__ double lds_all[];
   const int64_t wg = blockIdx.x;
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int64_t node = ((wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3)) * kPatchWaves + w;
-  const bool live = node < a.n_nodes;
+  const int64_t pos = ((wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3)) * kPatchWaves + w;
+  const bool live = pos < a.n_nodes;
+  const int64_t node = (a.order && live) ? a.order[pos] : pos;
   const int i0 = live ? a.inc_ptr[node] : 0, ninc = live ? a.inc_ptr[node + 1] - i0 : 0;
-  const int ns = live ? a.nslots[node] : 0;
+  const int nsraw = live ? a.nslots[node] : 0;
+  const int ns = nsraw & (kFullRows - 1);
+  const bool full = (nsraw & kFullRows) != 0;
   int64_t rowbase[Q];
   for (int x = 0; x < Q; ++x) rowbase[x] = live ? a.indptr[node * Q + x] : 0;
   double *lds = lds_all + (size_t)w * a.lds_wave;  // this wavefront's region
@@ -383,7 +393,8 @@ __global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArg
   const int dense = Q * Q * ns, mw = (dense + 31) / 32;
   for (int t = lane; t < dense; t += kWave) acc[t] = 0.0;
   const uint32_t *gm = a.mask + (int64_t)i0 * a.mwpi;  // ninc = 0 when not live: no loads
-  for (int t = lane; t < mw; t += kWave) smask[t] = gm[t];
+  if (!full)
+    for (int t = lane; t < mw; t += kWave) smask[t] = gm[t];
   for (int t = lane; t < ninc; t += kWave) sinc[t] = a.inc[i0 + t];
   const uint16_t *gs = a.slot + (int64_t)i0 * ND;
   for (int t = lane; t < ninc * ND; t += kWave) sslot[t] = gs[t];
@@ -409,6 +420,14 @@ __global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArg
     }
   }
   __syncthreads();
+  if (full) {  // every entry present: rows are contiguous copies of the accumulator
+    for (int x = 0; x < Q; ++x) {
+      double *dst = a.vals + rowbase[x];
+      const double *src = acc + x * ns * Q;
+      for (int d = lane; d < ns * Q; d += kWave) dst[d] = src[d];
+    }
+    return;
+  }
   for (int x = 0; x < Q; ++x) {
     int64_t base = rowbase[x];
     const int off = x * ns * Q, len = ns * Q;  // len = 0 when not live

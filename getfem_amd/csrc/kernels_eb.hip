@@ -223,12 +223,15 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   __shared__ double sthr[kMaxInc];
   __shared__ uint16_t sslot[kMaxInc * ND];
   const int64_t wg = blockIdx.x;
-  const int64_t node = (wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3);
-  if (node >= a.n_nodes) return;
+  const int64_t pos = (wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3);
+  if (pos >= a.n_nodes) return;
+  const int64_t node = a.order ? a.order[pos] : pos;
   constexpr int NL = ND * Q, RUN = Q * NL;
   const int t = threadIdx.x;
   const int i0 = a.inc_ptr[node], ninc = a.inc_ptr[node + 1] - i0;
-  const int ns = a.nslots[node];
+  const int nsraw = a.nslots[node];
+  const int ns = nsraw & 0x7fff;
+  const bool full = (nsraw & 0x8000) != 0;  // all of the node's rows are full
   if (ninc > kMaxInc) return;  // excluded by the launcher (max_inc check)
   for (int x = t; x < Q * Q * ns; x += kGatherThreads) acc[x] = 0.0;
   if (t < ninc) {
@@ -247,6 +250,14 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
     if (fabs(v) > sthr[k]) atomicAdd(&acc[(x * ns + sslot[k * ND + j]) * Q + y], v);
   }
   __syncthreads();
+  if (full) {  // plain copies of the accumulator rows
+    for (int x = 0; x < Q; ++x) {
+      double *dst = a.vals + a.indptr[node * Q + x];
+      const double *src = acc + x * ns * Q;
+      for (int d = t; d < ns * Q; d += kGatherThreads) dst[d] = src[d];
+    }
+    return;
+  }
   // write: wavefront x handles row x
   const uint32_t *m = a.mask + (int64_t)i0 * a.mwpi;
   const int w = t / 64, lane = t % 64;

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <algorithm>
 #include <map>
 #include <new>
 #include <string>
@@ -44,8 +45,15 @@ struct gfx_workspace {
   std::vector<WsExpr> exprs;
   gfx_plan *plan = nullptr;
   bool dirty = true, coefs_dirty = true;
+  bool state_dirty = true;  // the unknown's values changed since the last upload
+  bool matrix_valid = false;  // plan values = K at the current coefficients
+  bool pattern_stale = false;  // Neo-Hookean: state or coefficients changed since the pattern
   GwflKernelParams params;
   int form = GFX_FORM_FACTORIZED, precision = GFX_PREC_FP64;
+  // order-1 result: the unknown's values and the assembled vector (device)
+  double *d_u = nullptr, *d_V = nullptr;
+  const double *V_out = nullptr;  // d_V or the plan's residual
+  int64_t ndof = 0;
 };
 
 namespace {
@@ -142,6 +150,9 @@ int gfx_ws_create(gfx_ctx *ctx, gfx_workspace **out) {
 int gfx_ws_destroy(gfx_workspace *w) {
   if (!w) return GFX_E_INVALID;
   if (w->plan) gfx_plan_destroy(w->plan);
+  (void)hipSetDevice(w->ctx->device);
+  if (w->d_u) (void)hipFree(w->d_u);
+  if (w->d_V) (void)hipFree(w->d_V);
   delete w;
   return GFX_OK;
 }
@@ -181,13 +192,15 @@ int gfx_ws_set_values(gfx_workspace *w, const char *name, const double *h_V) {
   auto it = w->vars.find(name);
   if (it == w->vars.end()) return wfail(w, GFX_E_INVALID, std::string("unknown name ") + name);
   std::memcpy(it->second.V.data(), h_V, 8 * it->second.V.size());
-  w->coefs_dirty = true;
+  if (it->second.kind == VAR_FEM) w->state_dirty = true;
+  else w->coefs_dirty = true;
   return GFX_OK;
 }
 
 int gfx_ws_add_expression(gfx_workspace *w, const char *expr, gfx_mesh_im *mim, int der_order) {
   if (!w || !expr || !mim) return GFX_E_INVALID;
-  if (der_order != 2) return wfail(w, GFX_E_UNSUPPORTED, "only derivation order 2 is supported");
+  if (der_order != 1 && der_order != 2)
+    return wfail(w, GFX_E_UNSUPPORTED, "derivation order must be 1 or 2");
   if (!w->exprs.empty() && w->exprs[0].mim != mim)
     return wfail(w, GFX_E_UNSUPPORTED, "all expressions must use the same mesh_im");
   w->exprs.push_back({expr, mim, der_order});
@@ -223,26 +236,51 @@ int gfx_ws_compile(gfx_workspace *w) {
   return GFX_OK;
 }
 
+// ga_workspace::assembly(order) (src/getfem_generic_assembly_workspace.cc:
+// 791-936).  order 2: the tangent matrix into the plan's CSR.  order 1: the
+// vector of the first derivative at the unknown's current value; for the
+// Neo-Hookean term the element residual gathered per node, for the linear
+// terms (whose first derivative is bilinear in u and the test function) the
+// product K u of the assembled matrix.
 int gfx_ws_assembly(gfx_workspace *w, int order) {
   if (!w) return GFX_E_INVALID;
-  if (order != 2) return wfail(w, GFX_E_UNSUPPORTED, "only order-2 (matrix) assembly is supported here");
+  if (order != 1 && order != 2)
+    return wfail(w, GFX_E_UNSUPPORTED, "only order-1 (vector) and order-2 (matrix) assembly are supported");
   int r;
   if (w->dirty) {
     if ((r = gfx_ws_compile(w))) return r;
-    if (w->params.term != GFX_TERM_LAPLACE && w->params.term != GFX_TERM_ISO_ELAST)
-      return wfail(w, GFX_E_UNSUPPORTED, "term compiled (" + w->params.describe + ") has no matrix plan on this mesh");
+    const int t = w->params.term;
+    if (t != GFX_TERM_LAPLACE && t != GFX_TERM_ISO_ELAST && t != GFX_TERM_NEOHOOKEAN)
+      return wfail(w, GFX_E_UNSUPPORTED, "term compiled (" + w->params.describe + ") has no plan");
+    for (auto &e : w->exprs)
+      if (e.der_order < order)
+        return wfail(w, GFX_E_STATE, "an expression was added with a lower derivation order");
     const WsVar &u = w->vars.at(w->params.unknown);
     if (u.kind != VAR_FEM) return wfail(w, GFX_E_INVALID, "the unknown must be a fem variable");
     std::vector<gfx_coef> cf;
     std::vector<std::vector<double>> store;
     if ((r = bind_coefs(w, u.mf->mesh->n_cv, cf, store))) return r;
     if (w->plan) { gfx_plan_destroy(w->plan); w->plan = nullptr; }
-    if ((r = gfx_plan_create(u.mf, w->exprs[0].mim, w->params.term, cf.data(), (int)cf.size(), &w->plan)))
+    if ((r = gfx_plan_create(u.mf, w->exprs[0].mim, t, cf.data(), (int)cf.size(), &w->plan)))
       return r;
     if ((r = gfx_plan_set_form(w->plan, w->form, w->precision))) return r;
+    if (t == GFX_TERM_NEOHOOKEAN && (r = gfx_plan_set_state(w->plan, u.V.data()))) return r;
     if ((r = gfx_plan_build_pattern(w->plan, nullptr))) return r;
+    if (w->ndof != u.mf->ndof) {
+      if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
+      if (w->d_u) (void)hipFree(w->d_u);
+      if (w->d_V) (void)hipFree(w->d_V);
+      w->d_u = w->d_V = nullptr;
+      const size_t bytes = 8 * (size_t)std::max<int64_t>(u.mf->ndof, 1);
+      if (hipMalloc((void **)&w->d_u, bytes) != hipSuccess || hipMalloc((void **)&w->d_V, bytes) != hipSuccess)
+        return wfail(w, GFX_E_OOM, "workspace vectors");
+      w->ndof = u.mf->ndof;
+    }
     w->dirty = false;
     w->coefs_dirty = false;
+    w->state_dirty = t != GFX_TERM_NEOHOOKEAN;
+    w->pattern_stale = false;
+    w->matrix_valid = false;
   } else if (w->coefs_dirty) {
     const WsVar &u = w->vars.at(w->params.unknown);
     std::vector<gfx_coef> cf;
@@ -251,8 +289,52 @@ int gfx_ws_assembly(gfx_workspace *w, int order) {
     for (size_t s = 0; s < cf.size(); ++s)
       if ((r = gfx_plan_set_coef(w->plan, (int)s, &cf[s]))) return r;
     w->coefs_dirty = false;
+    w->matrix_valid = false;
+    w->pattern_stale = true;
   }
-  return gfx_plan_assemble(w->plan);
+  const WsVar &u = w->vars.at(w->params.unknown);
+  if (w->params.term == GFX_TERM_NEOHOOKEAN) {
+    if (w->state_dirty) {
+      if ((r = gfx_plan_set_state(w->plan, u.V.data()))) return r;
+      w->state_dirty = false;
+      w->pattern_stale = true;
+    }
+    w->V_out = w->plan->d_rhs;
+    if (order == 1) return gfx_plan_assemble_residual(w->plan);
+    // the stored pattern depends on the values (threshold on each element
+    // matrix), as GetFEM's matrix is rebuilt by every assembly
+    if (w->pattern_stale) {
+      if ((r = gfx_plan_build_pattern(w->plan, nullptr))) return r;
+      w->pattern_stale = false;
+    }
+    return gfx_plan_assemble(w->plan);
+  }
+  if (order == 2 || !w->matrix_valid) {
+    if ((r = gfx_plan_assemble(w->plan))) return r;
+    w->matrix_valid = true;
+  }
+  if (order == 1) {
+    if (w->state_dirty) {
+      if (hipMemcpyAsync(w->d_u, u.V.data(), 8 * (size_t)w->ndof, hipMemcpyHostToDevice,
+                         w->ctx->stream) != hipSuccess ||
+          hipStreamSynchronize(w->ctx->stream) != hipSuccess)
+        return wfail(w, GFX_E_HIP, "upload of the unknown");
+      w->state_dirty = false;
+    }
+    if ((r = gfx_plan_spmv(w->plan, w->d_u, w->d_V))) return r;
+    w->V_out = w->d_V;
+  }
+  return GFX_OK;
+}
+
+int gfx_ws_assembled_vector(gfx_workspace *w, double *h_V) {
+  if (!w || !h_V) return GFX_E_INVALID;
+  if (!w->V_out) return wfail(w, GFX_E_STATE, "no order-1 assembly done yet");
+  if (hipSetDevice(w->ctx->device) != hipSuccess ||
+      hipStreamSynchronize(w->ctx->stream) != hipSuccess ||
+      hipMemcpy(h_V, w->V_out, 8 * (size_t)w->ndof, hipMemcpyDeviceToHost) != hipSuccess)
+    return wfail(w, GFX_E_HIP, "copy of the assembled vector");
+  return GFX_OK;
 }
 
 static void describe_params(const GwflKernelParams &p, std::string &d) {

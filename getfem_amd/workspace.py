@@ -6,7 +6,7 @@ add_fem_constant, add_fixed_size_constant, add_expression, assembly) and
 expression, region, name, is_variable, mf, value, ...)``
 (interface/src/gf_asm.cc:421-548, :762-821): order-2 results are returned as
 scipy CSC matrices, like the interface's gsparse export
-(interface/src/getfemint.cc:712-737).
+(interface/src/getfemint.cc:712-737), order-1 results as numpy vectors.
 """
 import ctypes
 
@@ -78,6 +78,15 @@ class GaWorkspace:
     def assembly(self, order=2):
         check(lib().gfx_ws_assembly(self.h, order), self.ctx.h)
 
+    def assembled_vector(self):
+        """The order-1 result (nb_dof values of the unknown's mesh_fem)."""
+        p = self._plan_handle()
+        n, z = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().gfx_plan_dims(p, ctypes.byref(n), ctypes.byref(z)), self.ctx.h)
+        V = np.zeros(n.value)
+        check(lib().gfx_ws_assembled_vector(self.h, _ptr(V)), self.ctx.h)
+        return V
+
     def _plan_handle(self):
         h = ctypes.c_void_p()
         check(lib().gfx_ws_plan(self.h, ctypes.byref(h)), self.ctx.h)
@@ -138,4 +147,6 @@ def asm_generic(mim, order, expression, region, *args, ctx=None):
     ws.assembly(order)
     if order == 2:
         return ws.assembled_matrix()
-    raise ValueError("only order 2 is supported")
+    if order == 1:
+        return ws.assembled_vector()
+    raise ValueError("only orders 1 and 2 are supported")

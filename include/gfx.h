@@ -159,6 +159,11 @@ int gfx_plan_assemble(gfx_plan *p);
 /* nonlinear plans: the current displacement (ndof host doubles) and the
  * assembled residual vector */
 int gfx_plan_set_state(gfx_plan *p, const double *h_U);
+/* order-1 assembly alone (the residual; ga_workspace::assembly(1),
+ * src/getfem_generic_assembly_workspace.cc:832-847 with
+ * ga_instruction_vector_assembly_mf, compile_and_exec.cc:4669-4735), after
+ * the pattern is built; the matrix values are left as they are */
+int gfx_plan_assemble_residual(gfx_plan *p);
 int gfx_plan_residual(gfx_plan *p, double *h_V);
 int gfx_plan_copy_residual_device(gfx_plan *p, double *d_V);
 int gfx_plan_dims(gfx_plan *p, int64_t *nrows, int64_t *nnz);
@@ -174,6 +179,11 @@ int gfx_plan_device_csr(gfx_plan *p, const int64_t **d_indptr,
  * (stream-ordered on the context's stream; any pointer may be NULL) */
 int gfx_plan_copy_csr_device(gfx_plan *p, int64_t *d_indptr, int32_t *d_indices,
                              double *d_vals);
+/* d_y = K d_x with the assembled values (device vectors of nb_dof doubles,
+ * stream-ordered): the product of a linear brick's matrix with the current
+ * variable value that model::assembly adds to the residual
+ * (gmm::mult_add(matj, vec_linear2, ...), src/getfem_models.cc:2560-2575) */
+int gfx_plan_spmv(gfx_plan *p, const double *d_x, double *d_y);
 /* record HIP events around the phases of every gfx_plan_assemble call (on
  * the context's stream, no host synchronisation); enabling resets the record.
  * gfx_plan_last_timings: durations (ms) of the last call, geometry kernel and
@@ -214,8 +224,16 @@ int gfx_ws_add_expression(gfx_workspace *w, const char *expr, gfx_mesh_im *mim, 
 int gfx_ws_set_form(gfx_workspace *w, int form, int precision);
 /* the compile stage alone (GFX_E_UNSUPPORTED with a message outside the subset) */
 int gfx_ws_compile(gfx_workspace *w);
-/* ga_workspace::assembly(order) (.h:564): order 2 */
+/* ga_workspace::assembly(order) (.h:564).  order 2: the matrix (for the
+ * Neo-Hookean term the pattern is rebuilt when the state or the coefficients
+ * changed, since GetFEM's stored pattern depends on the values).  order 1: the
+ * vector of the expression's first derivative at the unknown's current value
+ * (Neo-Hookean: the residual; linear terms: K u); read it with
+ * gfx_ws_assembled_vector.  Expressions must have been added with a
+ * derivation order >= order. */
 int gfx_ws_assembly(gfx_workspace *w, int order);
+/* ga_workspace::assembled_vector() (.h:389-390): nb_dof doubles to the host */
+int gfx_ws_assembled_vector(gfx_workspace *w, double *h_V);
 int gfx_ws_plan(gfx_workspace *w, gfx_plan **out);
 /* normal form of the compiled term, e.g. "isotropic elasticity unknown=u ..." */
 int gfx_ws_describe(gfx_workspace *w, char *buf, int64_t len);

@@ -181,6 +181,19 @@ class ga_workspace {
     check(gfx_plan_export_csc(p, m.colptr.data(), m.rows.data(), m.values.data()), ctx_);
     return m;
   }
+  // ga_workspace::assembled_vector() (.h:389-390), after assembly(1)
+  std::vector<double> assembled_vector() const {
+    gfx_plan *p = nullptr;
+    check(gfx_ws_plan(h_, &p), ctx_);
+    int64_t n = 0, nnz = 0;
+    check(gfx_plan_dims(p, &n, &nnz), ctx_);
+    std::vector<double> V((size_t)n);
+    check(gfx_ws_assembled_vector(h_, V.data()), ctx_);
+    return V;
+  }
+  void set_values(const std::string &name, const std::vector<double> &V) {
+    check(gfx_ws_set_values(h_, name.c_str(), V.data()), ctx_);
+  }
 
  private:
   gfx_ctx *ctx_;

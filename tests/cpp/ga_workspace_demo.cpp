@@ -2,7 +2,9 @@
 // (compare tests/test_assembly.cc:812-818 of the reference): reads a
 // tetrahedral mesh (points.f64, convexes.i32) from a directory, assembles
 // linear elasticity with fixed-size lambda = 3, mu = 2 on FEM_PK(3,2) x 3,
-// and writes GetFEM's CSC orientation (colptr.i64, rows.u32, values.f64).
+// and writes GetFEM's CSC orientation (colptr.i64, rows.u32, values.f64),
+// then the order-1 vector at u_i = sin(i) (V.f64).
+#include <cmath>
 #include <cstdio>
 #include <fstream>
 #include <iostream>
@@ -54,6 +56,11 @@ int main(int argc, char **argv) {
     save(dir + "/rows.u32", K.rows);
     save(dir + "/values.f64", K.values);
     std::cout << workspace.describe() << "\nnnz " << K.values.size() << "\n";
+    // order 1 at u_i = sin(i): the vector K u
+    for (size_t i = 0; i < U.size(); ++i) U[i] = std::sin((double)i);
+    workspace.set_values("u", U);
+    workspace.assembly(1);
+    save(dir + "/V.f64", workspace.assembled_vector());
     // an unsupported term is reported as an exception, like gmm::gmm_error
     try {
       getfem_amd::ga_workspace w2(ctx);

@@ -37,3 +37,7 @@ def test_cpp_workspace_matches_oracle(tmp_path):
     cpr, rr, vr = P.assemble_csc()
     assert np.array_equal(cp, cpr) and np.array_equal(rows.astype(np.int64), rr.astype(np.int64))
     assert problems.rel_frob(vals, vr) <= 1e-12
+    import scipy.sparse as sp
+    K = sp.csc_matrix((vr, rr, cpr), shape=(ndof, ndof))
+    V = np.fromfile(tmp_path / "V.f64")
+    assert problems.rel_frob(V, K @ np.sin(np.arange(ndof, dtype=np.float64))) <= 1e-12

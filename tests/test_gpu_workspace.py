@@ -79,3 +79,72 @@ def test_asm_generic_is_getfem_csc(ctx):
     cp, r, v = case.oracle_problem().assemble_csc()
     assert np.array_equal(K.indptr, cp) and np.array_equal(K.indices, r)
     assert problems.rel_frob(K.data, v) <= 1e-12
+
+
+def test_order1_linear_is_K_u(ctx):
+    """assembly(1) of a bilinear expression: the vector of its first
+    derivative at the current u, i.e. K u (oracle CSC times u)."""
+    case = problems.c2(3)
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", case.pts, case.cvx)
+    mf = ga.MeshFem(mesh, "FEM_PK(3,2)", 3)
+    mim = ga.MeshIm(mesh, "IM_TETRAHEDRON(5)")
+    rng = np.random.default_rng(7)
+    u = rng.standard_normal(mf.nbdof())
+    ws = GaWorkspace(ctx)
+    ws.add_fem_variable("u", mf, u)
+    ws.add_fixed_size_constant("lambda", 3.0)
+    ws.add_fixed_size_constant("mu", 2.0)
+    ws.add_expression("lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u", mim)
+    ws.assembly(1)
+    V = ws.assembled_vector()
+    P = oracle.Problem(oracle.GT_PK31, case.pts, case.cvx, oracle.FEM_PK, 2, 3, mf.basic_dof_table(),
+                       mf.nbdof(), "IM_TETRAHEDRON(5)", oracle.TERM_ISO_ELAST, scal_a=3.0, scal_b=2.0)
+    import scipy.sparse as sp
+    cp, r, v = P.assemble_csc()
+    K = sp.csc_matrix((v, r, cp), shape=(mf.nbdof(), mf.nbdof()))
+    assert problems.rel_frob(V, K @ u) <= 1e-12
+    # new values of the unknown: same workspace, new vector
+    ws.set_values("u", 2 * u)
+    ws.assembly(1)
+    assert problems.rel_frob(ws.assembled_vector(), 2 * (K @ u)) <= 1e-12
+    # asm_generic order 1 gives the same vector
+    V1 = asm_generic(mim, 1, "lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u", -1,
+                     "u", 1, mf, u, "lambda", 0, 3.0, "mu", 0, 2.0)
+    assert problems.rel_frob(V1, K @ u) <= 1e-12
+
+
+NH_EXPR = "((Id(meshdim)+Grad_u)*(Compressible_Neo_Hookean_Bonet_PK2(Grad_u,[lambda;mu]))):Grad_Test_u"
+
+
+def test_workspace_neohookean_tangent_and_residual(ctx):
+    """The finite strain brick's expression through the workspace: order 2 is
+    the tangent (pattern rebuilt for each new state, as GetFEM's stored
+    pattern depends on the values) and order 1 the residual."""
+    n = 2
+    pts, cvx = meshes.regular_unit_mesh(n, "GT_QK(3,1)")
+    mesh = ga.Mesh(ctx, "GT_QK(3,1)", pts, cvx)
+    mf = ga.MeshFem(mesh, "FEM_QK(3,2)", 3)
+    mim = ga.MeshIm(mesh, "IM_GAUSS_PARALLELEPIPED(3,5)")
+    ws = GaWorkspace(ctx)
+    ws.add_fem_variable("u", mf, np.zeros(mf.nbdof()))
+    ws.add_fixed_size_constant("lambda", 10.0)
+    ws.add_fixed_size_constant("mu", 1.0)
+    ws.add_expression(NH_EXPR, mim)
+    assert "neo-hookean" in ws.compile()
+    for scale in (0.0, 0.05, 0.02):
+        U = meshes.sine_modes()(mf.basic_dof_nodes(), 0.05).ravel() * (scale / 0.05)
+        ws.set_values("u", U)
+        ws.assembly(1)
+        R = ws.assembled_vector()
+        ws.assembly(2)
+        ip, ix, v = ws.assembled_matrix_csr()
+        P = oracle.Problem(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, 2, 3, mf.basic_dof_table(), mf.nbdof(),
+                           "IM_GAUSS_PARALLELEPIPED(3,5)", oracle.TERM_NEOHOOKEAN, scal_a=10.0, scal_b=1.0, U=U)
+        ipr, ixr, vr = oracle.csc_to_csr(*P.assemble_csc(), mf.nbdof())
+        assert np.array_equal(ip, ipr) and np.array_equal(ix, ixr), scale
+        assert problems.rel_frob(v, vr) <= 1e-12
+        Rr = P.assemble_vector()
+        if scale == 0.0:
+            assert np.abs(R).max() == 0.0 and np.abs(Rr).max() == 0.0
+        else:
+            assert problems.rel_frob(R, Rr) <= 1e-12
