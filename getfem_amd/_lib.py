@@ -19,6 +19,7 @@ TERM_LAPLACE, TERM_ISO_ELAST, TERM_NEOHOOKEAN = 0, 1, 2
 FORM_FACTORIZED, FORM_PERQP = 0, 1
 PREC_FP64, PREC_FP32 = 0, 1
 COEF_CONST, COEF_PER_ELEMENT = 0, 1
+BUILD_MATRIX, BUILD_RHS, MODEL_LINEAR, MODEL_SYMMETRIC = 1, 2, 4, 8
 
 # every symbol include/gfx.h declares, with (restype, argtypes)
 _P = ctypes.c_void_p
@@ -66,6 +67,8 @@ SIGNATURES = {
     "gfx_plan_assemble_residual": (_i32, [_P]),
     "gfx_plan_residual": (_i32, [_P, _P]),
     "gfx_plan_spmv": (_i32, [_P, _P, _P]),
+    "gfx_plan_dof_constraints": (_i32, [_P, _i64, _P, _P, _P, _P, _i32]),
+    "gfx_plan_solve_cg": (_i32, [_P, _P, _P, _f64, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_f64)]),
     "gfx_plan_copy_residual_device": (_i32, [_P, _P]),
     "gfx_plan_dims": (_i32, [_P, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
     "gfx_plan_export_csr": (_i32, [_P, _P, _P, _P]),

@@ -189,6 +189,15 @@ class Plan:
         """d_y = K d_x on the device (raw device addresses, e.g. data_ptr())."""
         check(lib().gfx_plan_spmv(self.h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y)), self.ctx.h)
 
+    def solve_cg(self, d_b, d_x, rtol=1e-10, maxit=10000):
+        """Jacobi-preconditioned CG on the device: d_x (initial guess in,
+        solution out) and d_b are raw device addresses.  Returns (iterations,
+        relative residual)."""
+        it, res = ctypes.c_int(), ctypes.c_double()
+        check(lib().gfx_plan_solve_cg(self.h, ctypes.c_void_p(d_b), ctypes.c_void_p(d_x), rtol, maxit,
+                                      ctypes.byref(it), ctypes.byref(res)), self.ctx.h)
+        return it.value, res.value
+
     def residual(self):
         n, _ = self.dims()
         V = np.zeros(n)

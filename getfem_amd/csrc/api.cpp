@@ -348,7 +348,8 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
-  dfree(p->d_gN); dfree(p->d_E); dfree(p->d_res); dfree(p->d_U); dfree(p->d_rhs);
+  dfree(p->d_gN); dfree(p->d_E); dfree(p->d_res); dfree(p->d_U); dfree(p->d_rhs); dfree(p->d_cg);
+  dfree(p->d_fixed); dfree(p->d_go);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   delete p;
   return GFX_OK;
@@ -683,6 +684,46 @@ int gfx_plan_spmv(gfx_plan *p, const double *d_x, double *d_y) {
   if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
   if (set_device(c)) return GFX_E_HIP;
   HIPCHK(c, launch_spmv(c->stream, p->ndof, p->nnz, p->d_indptr, p->d_indices, p->d_vals, d_x, d_y));
+  return GFX_OK;
+}
+
+int gfx_plan_solve_cg(gfx_plan *p, const double *d_b, double *d_x, double rtol, int maxit, int *iters,
+                      double *rel_res) {
+  if (!p || !d_b || !d_x || !iters || !rel_res || !(rtol > 0) || maxit < 0) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern and assemble first");
+  if (set_device(c)) return GFX_E_HIP;
+  if (!p->d_cg) HIPCHK(c, dalloc(&p->d_cg, cg_workspace_doubles(p->ndof)));
+  HIPCHK(c, cg_solve(c->stream, p->ndof, p->nnz, p->d_indptr, p->d_indices, p->d_vals, d_b, d_x, rtol, maxit,
+                     p->d_cg, iters, rel_res));
+  return GFX_OK;
+}
+
+int gfx_plan_dof_constraints(gfx_plan *p, int64_t n_fixed, const int32_t *h_dofs, const double *h_go,
+                             const double *d_pr, double *d_rhs, int flags) {
+  if (!p || n_fixed < 0 || (n_fixed && (!h_dofs || !h_go))) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern and assemble first");
+  if (set_device(c)) return GFX_E_HIP;
+  const int64_t n = p->ndof;
+  std::vector<uint8_t> fixed((size_t)n, 0);
+  std::vector<double> go((size_t)n, 0.0);
+  for (int64_t k = 0; k < n_fixed; ++k) {
+    if (h_dofs[k] < 0 || h_dofs[k] >= n) return fail(c, GFX_E_INVALID, "constrained dof out of range");
+    fixed[h_dofs[k]] = 1;
+    go[h_dofs[k]] = h_go[k];
+  }
+  if (!p->d_fixed) HIPCHK(c, dalloc(&p->d_fixed, (size_t)n));
+  if (!p->d_go) HIPCHK(c, dalloc(&p->d_go, (size_t)n));
+  HIPCHK(c, hipMemcpyAsync(p->d_fixed, fixed.data(), (size_t)n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(p->d_go, go.data(), 8 * (size_t)n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemsetAsync(p->d_err, 0, 4, c->stream));
+  HIPCHK(c, launch_dof_constraints(c->stream, n, p->d_indptr, p->d_indices, p->d_vals, p->d_fixed, p->d_go,
+                                   d_pr, d_rhs, flags, p->d_err));
+  int flag = 0;
+  HIPCHK(c, hipMemcpyAsync(&flag, p->d_err, 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  if (flag & 4) return fail(c, GFX_E_PATTERN, "a constrained dof has no diagonal entry in the pattern");
   return GFX_OK;
 }
 

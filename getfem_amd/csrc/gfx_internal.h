@@ -142,6 +142,13 @@ hipError_t launch_gather_vec(hipStream_t s, int64_t n_nodes, int nd, int Q, cons
 // spmv.hip
 hipError_t launch_spmv(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr,
                        const int32_t *indices, const double *vals, const double *x, double *y);
+hipError_t launch_dof_constraints(hipStream_t s, int64_t n, const int64_t *indptr, const int32_t *indices,
+                                  double *vals, const uint8_t *fixed, const double *go, const double *pr,
+                                  double *rhs, int flags, int *err);
+size_t cg_workspace_doubles(int64_t n);
+hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
+                    const double *vals, const double *b, double *x, double rtol, int maxit, double *work,
+                    int *iters, double *rel_res);
 
 // adjoint.hip
 int64_t compliance_nblocks(int64_t n_cv);
@@ -199,6 +206,9 @@ struct gfx_plan {
   // element-batched plans (non-affine / nonlinear): buffered element matrices
   bool eb = false;
   double *d_gN = nullptr, *d_E = nullptr, *d_res = nullptr, *d_U = nullptr, *d_rhs = nullptr;
+  double *d_cg = nullptr;  // conjugate gradient work vectors (allocated on first solve)
+  uint8_t *d_fixed = nullptr;  // dof constraints: mask and prescribed values
+  double *d_go = nullptr;
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 3 events per timed call
   int n_timed = 0;

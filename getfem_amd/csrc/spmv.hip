@@ -1,7 +1,16 @@
-// y = K x over the assembled CSR (rows = test dofs, sorted columns).  G lanes
-// of a wavefront share one row: each lane walks the row with stride G, then
-// the G partial sums are combined by shuffles.  G is picked from the mean row
-// length so that short rows (P1) do not leave most lanes idle.
+// Sparse operations on the assembled CSR (rows = test dofs, sorted columns):
+// the product y = K x, dof constraints (Dirichlet by simplification) and a
+// Jacobi-preconditioned conjugate gradient solve.
+//
+// SpMV: G lanes of a wavefront share one row; each lane walks the row with
+// stride G, then the G partial sums are combined by shuffles.  G is picked
+// from the mean row length so that short rows (P1) do not leave most lanes
+// idle.
+//
+// CG: every scalar of the recurrence (alpha, beta, r.z, r.r) lives in device
+// memory and is produced by fixed-order block reductions, so an iteration
+// issues kernels only and the result does not depend on scheduling.  The host
+// reads r.r once per iteration for the stopping test.
 #include <hip/hip_runtime.h>
 
 #include "gfx_internal.h"
@@ -10,6 +19,8 @@ namespace gfx {
 namespace {
 
 constexpr int kSpmvThreads = 256;
+constexpr int kRedThreads = 256;   // threads of a reduction block
+constexpr int kRedBlocks = 1024;   // partial sums per dot product (fixed: deterministic order)
 
 template <int G>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int64_t n, const int64_t *__restrict__ indptr,
@@ -37,7 +48,169 @@ hipError_t spmv_g(hipStream_t s, int64_t n, const int64_t *indptr, const int32_t
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------- CG
+struct CgScal {
+  double rz, rr, bb, pq, alpha, beta;
+};
+
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  sh[threadIdx.x] = v;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+// inverse diagonal (0 where the diagonal is absent or zero: that dof is left alone)
+__global__ void k_inv_diag(int64_t n, const int64_t *__restrict__ indptr,
+                           const int32_t *__restrict__ indices, const double *__restrict__ vals,
+                           double *__restrict__ dinv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int64_t lo = indptr[i], hi = indptr[i + 1];
+  double d = 0.0;
+  while (lo < hi) {  // sorted columns: binary search for column i
+    const int64_t mid = (lo + hi) / 2;
+    const int32_t c = indices[mid];
+    if (c == i) { d = vals[mid]; break; }
+    if (c < i) lo = mid + 1;
+    else hi = mid;
+  }
+  dinv[i] = d != 0.0 ? 1.0 / d : 0.0;
+}
+
+// r = b - r (r holds K x on entry), z = D^-1 r, p = z; partials of r.z, r.r, b.b
+__global__ __launch_bounds__(kRedThreads) void k_cg_start(int64_t n, const double *__restrict__ b,
+                                                          const double *__restrict__ dinv,
+                                                          double *__restrict__ r, double *__restrict__ z,
+                                                          double *__restrict__ p, double *__restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  double rz = 0, rr = 0, bb = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (int64_t)kRedBlocks * kRedThreads) {
+    const double ri = b[i] - r[i], zi = dinv[i] * ri;
+    r[i] = ri; z[i] = zi; p[i] = zi;
+    rz += ri * zi; rr += ri * ri; bb += b[i] * b[i];
+  }
+  rz = block_sum(rz, sh); rr = block_sum(rr, sh); bb = block_sum(bb, sh);
+  if (threadIdx.x == 0) {
+    part[blockIdx.x] = rz; part[kRedBlocks + blockIdx.x] = rr; part[2 * kRedBlocks + blockIdx.x] = bb;
+  }
+}
+
+__global__ __launch_bounds__(kRedThreads) void k_dot(int64_t n, const double *__restrict__ x,
+                                                     const double *__restrict__ y, double *__restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  double s = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (int64_t)kRedBlocks * kRedThreads)
+    s += x[i] * y[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+
+// sum `count` partial arrays of kRedBlocks entries in a fixed order;
+// mode 0: start (rz, rr, bb), 1: pq -> alpha, 2: (rz_new, rr) -> beta
+__global__ __launch_bounds__(kRedThreads) void k_cg_scalars(const double *__restrict__ part, int mode,
+                                                            CgScal *__restrict__ sc) {
+  __shared__ double sh[kRedThreads];
+  double v[3] = {0, 0, 0};
+  const int count = mode == 1 ? 1 : (mode == 0 ? 3 : 2);
+  for (int k = 0; k < count; ++k) {
+    double s = 0;
+    for (int i = threadIdx.x; i < kRedBlocks; i += kRedThreads) s += part[k * kRedBlocks + i];
+    v[k] = block_sum(s, sh);
+  }
+  if (threadIdx.x != 0) return;
+  if (mode == 0) {
+    sc->rz = v[0]; sc->rr = v[1]; sc->bb = v[2];
+  } else if (mode == 1) {
+    sc->pq = v[0];
+    sc->alpha = v[0] != 0.0 ? sc->rz / v[0] : 0.0;
+  } else {
+    sc->beta = sc->rz != 0.0 ? v[0] / sc->rz : 0.0;
+    sc->rz = v[0]; sc->rr = v[1];
+  }
+}
+
+// x += alpha p, r -= alpha q, z = D^-1 r; partials of r.z and r.r
+__global__ __launch_bounds__(kRedThreads) void k_cg_update(int64_t n, const CgScal *__restrict__ sc,
+                                                           const double *__restrict__ p,
+                                                           const double *__restrict__ q,
+                                                           const double *__restrict__ dinv,
+                                                           double *__restrict__ x, double *__restrict__ r,
+                                                           double *__restrict__ z, double *__restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  const double alpha = sc->alpha;
+  double rz = 0, rr = 0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (int64_t)kRedBlocks * kRedThreads) {
+    x[i] += alpha * p[i];
+    const double ri = r[i] - alpha * q[i], zi = dinv[i] * ri;
+    r[i] = ri; z[i] = zi;
+    rz += ri * zi; rr += ri * ri;
+  }
+  rz = block_sum(rz, sh); rr = block_sum(rr, sh);
+  if (threadIdx.x == 0) { part[blockIdx.x] = rz; part[kRedBlocks + blockIdx.x] = rr; }
+}
+
+// p = z + beta p
+__global__ void k_cg_direction(int64_t n, const CgScal *__restrict__ sc, const double *__restrict__ z,
+                               double *__restrict__ p) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = z[i] + sc->beta * p[i];
+}
+
+unsigned nb256(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+// ---------------------------------------------------------- dof constraints
+// fixed[i] = 1 for a constrained dof, go[i] its prescribed value.  One
+// thread per row; the row's entries are read before any is cleared, so the
+// symmetric linear right-hand side correction uses the assembled K.
+__global__ void k_dof_constraints(int64_t n, const int64_t *__restrict__ indptr,
+                                  const int32_t *__restrict__ indices, double *__restrict__ vals,
+                                  const uint8_t *__restrict__ fixed, const double *__restrict__ go,
+                                  const double *__restrict__ pr, double *__restrict__ rhs, int flags,
+                                  int *__restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const bool mat = flags & GFX_BUILD_MATRIX, vec = (flags & GFX_BUILD_RHS) && rhs;
+  const bool lin = flags & GFX_MODEL_LINEAR, sym = flags & GFX_MODEL_SYMMETRIC;
+  const int64_t b = indptr[i], e = indptr[i + 1];
+  if (fixed[i]) {
+    if (vec) rhs[i] = lin ? go[i] : go[i] - (pr ? pr[i] : 0.0);
+    if (mat) {
+      bool diag = false;
+      for (int64_t k = b; k < e; ++k) {
+        const bool d = indices[k] == i;
+        vals[k] = d ? 1.0 : 0.0;
+        diag |= d;
+      }
+      if (!diag) atomicOr(err, 4);
+    }
+    return;
+  }
+  if (vec && lin && sym) {  // rhs -= K(:, fixed) go
+    double s = 0.0;
+    for (int64_t k = b; k < e; ++k)
+      if (fixed[indices[k]]) s += vals[k] * go[indices[k]];
+    rhs[i] -= s;
+  }
+  if (mat && sym)
+    for (int64_t k = b; k < e; ++k)
+      if (fixed[indices[k]]) vals[k] = 0.0;
+}
+
 }  // namespace
+
+hipError_t launch_dof_constraints(hipStream_t s, int64_t n, const int64_t *indptr, const int32_t *indices,
+                                  double *vals, const uint8_t *fixed, const double *go, const double *pr,
+                                  double *rhs, int flags, int *err) {
+  if (n == 0) return hipSuccess;
+  k_dof_constraints<<<nb256(n), 256, 0, s>>>(n, indptr, indices, vals, fixed, go, pr, rhs, flags, err);
+  return hipGetLastError();
+}
 
 hipError_t launch_spmv(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr,
                        const int32_t *indices, const double *vals, const double *x, double *y) {
@@ -47,6 +220,51 @@ hipError_t launch_spmv(hipStream_t s, int64_t n, int64_t nnz, const int64_t *ind
   if (mean <= 24) return spmv_g<8>(s, n, indptr, indices, vals, x, y);
   if (mean <= 64) return spmv_g<16>(s, n, indptr, indices, vals, x, y);
   return spmv_g<32>(s, n, indptr, indices, vals, x, y);
+}
+
+size_t cg_workspace_doubles(int64_t n) { return 5 * (size_t)n + 3 * kRedBlocks + 8; }
+
+hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
+                    const double *vals, const double *b, double *x, double rtol, int maxit, double *work,
+                    int *iters, double *rel_res) {
+  *iters = 0;
+  *rel_res = 0.0;
+  if (n == 0) return hipSuccess;
+  double *dinv = work, *r = work + n, *z = work + 2 * n, *p = work + 3 * n, *q = work + 4 * n;
+  double *part = work + 5 * n;
+  CgScal *sc = reinterpret_cast<CgScal *>(part + 3 * kRedBlocks);
+  hipError_t e;
+  k_inv_diag<<<nb256(n), 256, 0, s>>>(n, indptr, indices, vals, dinv);
+  if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, x, r)) != hipSuccess) return e;
+  k_cg_start<<<kRedBlocks, kRedThreads, 0, s>>>(n, b, dinv, r, z, p, part);
+  k_cg_scalars<<<1, kRedThreads, 0, s>>>(part, 0, sc);
+  CgScal h{};
+  if ((e = hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  const double bn = sqrt(h.bb);
+  if (bn == 0.0) {  // b = 0: x = 0 is the solution
+    if ((e = hipMemsetAsync(x, 0, 8 * (size_t)n, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+  double res = sqrt(h.rr) / bn;
+  int it = 0;
+  while (res > rtol && it < maxit) {
+    if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, p, q)) != hipSuccess) return e;
+    k_dot<<<kRedBlocks, kRedThreads, 0, s>>>(n, p, q, part);
+    k_cg_scalars<<<1, kRedThreads, 0, s>>>(part, 1, sc);
+    k_cg_update<<<kRedBlocks, kRedThreads, 0, s>>>(n, sc, p, q, dinv, x, r, z, part);
+    k_cg_scalars<<<1, kRedThreads, 0, s>>>(part, 2, sc);
+    k_cg_direction<<<nb256(n), 256, 0, s>>>(n, sc, z, p);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    ++it;
+    res = sqrt(h.rr) / bn;
+    if (h.pq <= 0.0) break;  // K not positive definite along p
+  }
+  *iters = it;
+  *rel_res = res;
+  return hipSuccess;
 }
 
 }  // namespace gfx

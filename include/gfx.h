@@ -184,6 +184,34 @@ int gfx_plan_copy_csr_device(gfx_plan *p, int64_t *d_indptr, int32_t *d_indices,
  * variable value that model::assembly adds to the residual
  * (gmm::mult_add(matj, vec_linear2, ...), src/getfem_models.cc:2560-2575) */
 int gfx_plan_spmv(gfx_plan *p, const double *d_x, double *d_y);
+/* Solve K x = b on the device by the conjugate gradient method with a Jacobi
+ * (diagonal) preconditioner, for the coercive (symmetric positive definite)
+ * systems GetFEM sends to gmm::cg (default_linear_solver picks
+ * linear_solver_cg_preconditioned_ildlt, src/getfem/getfem_model_solvers.h:
+ * 90-98, :652-655; the preconditioner here is the diagonal, not ILDLT).
+ * Stopping test as gmm::cg (src/gmm/gmm_solver_cg.h:85-120): ||b - K x|| <=
+ * rtol ||b||; b = 0 gives x = 0.  d_x: initial guess in, solution out (device,
+ * nb_dof doubles).  iters and rel_res receive the iteration count and the
+ * final relative residual; reaching maxit is not an error (compare rel_res
+ * with rtol, as gmm::iteration::converged). */
+int gfx_plan_solve_cg(gfx_plan *p, const double *d_b, double *d_x, double rtol, int maxit,
+                      int *iters, double *rel_res);
+/* Dof constraints as model::assembly applies them after the bricks
+ * ("Post simplification for dof constraints", src/getfem_models.cc:
+ * 2806-2870; added by add_Dirichlet_condition_with_simplification, :4864-4940
+ * and :5019-5030).  h_dofs[k] (k < n_fixed) gets the prescribed value h_go[k].
+ * GFX_BUILD_RHS with d_rhs: linear model -> if GFX_MODEL_SYMMETRIC, d_rhs -=
+ * K(:, fixed) go with the assembled K, then d_rhs[fixed] = go; nonlinear
+ * model -> d_rhs[fixed] = go - d_pr[fixed] (d_pr: the current values, device,
+ * NULL = 0).  GFX_BUILD_MATRIX: rows of the fixed dofs cleared (and their
+ * columns when GFX_MODEL_SYMMETRIC), diagonal set to 1.  Cleared entries stay
+ * in the pattern as explicit zeros (gmm's rsvector erases them). */
+#define GFX_BUILD_MATRIX 1
+#define GFX_BUILD_RHS 2
+#define GFX_MODEL_LINEAR 4
+#define GFX_MODEL_SYMMETRIC 8
+int gfx_plan_dof_constraints(gfx_plan *p, int64_t n_fixed, const int32_t *h_dofs,
+                             const double *h_go, const double *d_pr, double *d_rhs, int flags);
 /* record HIP events around the phases of every gfx_plan_assemble call (on
  * the context's stream, no host synchronisation); enabling resets the record.
  * gfx_plan_last_timings: durations (ms) of the last call, geometry kernel and
