@@ -41,6 +41,25 @@ CONFIGS = {
     "c3": (48, "FEM_QK(3,2)", 3, "IM_GAUSS_PARALLELEPIPED(3,5)", "neohookean", 27, 2, 513599625),
 }
 FP64_PEAK_TFLOPS = 78.6
+# HBM bytes per assembly of the dominant kernel(s), measured with rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, summed over one assembly's
+# launches) at each config's default size; FETCH_SIZE is scaled by 2 and
+# WRITE_SIZE by 1, the factors a plain 1 GiB device copy gives
+# (bench.py --copy-calibration).  Every form of a config has the same memory
+# pattern (FP64 values and accumulation), so one figure serves all forms.
+PMC_TRAFFIC_KIB = {  # config: (kernel, FETCH_SIZE KiB, WRITE_SIZE KiB)
+    "c2": ("k_assemble_patch", 940249, 4291198),
+    "c5": ("k_assemble_patch", 266893, 3036596),
+    "c3": ("k_nh_element", 84667, 5747330),
+}
+FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
+
+
+def pmc_traffic_bytes(config, n, n0):
+    if config not in PMC_TRAFFIC_KIB or n != n0:
+        return None
+    _, f, w = PMC_TRAFFIC_KIB[config]
+    return (FETCH_FACTOR * f + WRITE_FACTOR * w) * 1024.0
 WORKLOAD = {
     "c1": "P1-tet Poisson Grad_u.Grad_Test_u",
     "c2": "P2-tet isotropic elasticity lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u",
@@ -162,6 +181,10 @@ def run_gpu(args):
     }
     del ip, ix
     head = forms[variants[0][0]]
+    # compulsory bytes of one assembly: CSR values + column indices + row
+    # pointers + points + connectivity + per-element coefficients
+    compulsory = (12 * int(nnz) + 8 * (int(ndof) + 1) + 24 * pts.shape[0] + 4 * cvx.size
+                  + 8 * n_coef * n_cv)
     out = {
         "metric": METRIC,
         "value": head["elems_per_s"],
@@ -191,10 +214,13 @@ def run_gpu(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": b_patch * n_cv / (head["ms_assemble_kernel"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": pmc_traffic_bytes(args.config, n, n0),
             "kernel": "k_assemble_patch",
             "bytes_per_element": b_patch,
             "kernel_ms": head["ms_assemble_kernel"],
+            "launches_per_assembly": "one per patch-size bin of nodes; kernel_ms is their sum",
+            "compulsory_bytes": compulsory,
+            "compulsory_frac": compulsory / (head["ms_assemble_kernel"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
         "forms": forms,
         "pattern": pattern,
@@ -206,7 +232,8 @@ def run_gpu(args):
         t_el = head["ms_geometry_kernel"] * 1e-3
         out["roofline"] = {
             "bound": "mfma", "achieved": flop * n_cv / t_el / 1e12, "peak": FP64_PEAK_TFLOPS,
-            "unit": "TFLOP/s", "frac": flop * n_cv / t_el / 1e12 / FP64_PEAK_TFLOPS, "traffic": None,
+            "unit": "TFLOP/s", "frac": flop * n_cv / t_el / 1e12 / FP64_PEAK_TFLOPS,
+            "traffic": pmc_traffic_bytes(args.config, n, n0),
             "kernel": "k_nh_element", "flop_per_element": flop, "kernel_ms": head["ms_geometry_kernel"],
             "gather_kernel": {"kernel": "k_gather_emat", "ms": head["ms_assemble_kernel"],
                               "achieved_gbs": b_patch * n_cv / (head["ms_assemble_kernel"] * 1e-3) / 1e9},
@@ -281,9 +308,12 @@ def main():
                 oracle.build()
             cores = min(16, os.cpu_count() or 1)
             n_cv = inputs[1].shape[0]
-            per_core = {"c1": 400000, "c2": 12000, "c5": 3000, "c3": 1000}[args.config]
-            big = args.cpu_sample or min(n_cv, per_core * cores)
-            res = cpu_baseline(inputs, [(1, min(per_core, n_cv)), (cores, big)])
+            # samples sized for about 5 s on one core and 10 s on all cores
+            # (measured rates: P2 43k/73k, P3 5.2k/19k, Q2 1.3k/7.4k elems/s)
+            one, many = {"c1": (48000, 48000), "c2": (240000, 786432), "c5": (25000, 196608),
+                         "c3": (6000, 73728)}[args.config]
+            big = args.cpu_sample or min(n_cv, many)
+            res = cpu_baseline(inputs, [(1, min(one, n_cv)), (cores, big)])
             v1 = res[1][0]
             vm, m, dt = res[cores]
             out["cpu_baseline"] = {

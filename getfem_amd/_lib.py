@@ -92,6 +92,7 @@ SIGNATURES = {
     "gfx_ws_assembly": (_i32, [_P, _i32]),
     "gfx_ws_plan": (_i32, [_P, _pp]),
     "gfx_ws_assembled_vector": (_i32, [_P, _P]),
+    "gfx_ws_copy_assembled_vector_device": (_i32, [_P, _P]),
     "gfx_ws_describe": (_i32, [_P, ctypes.c_char_p, _i64]),
     "gfx_compliance_create": (_i32, [_P, _P, _f64, _f64, _f64, _f64, _pp]),
     "gfx_compliance_eval": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),

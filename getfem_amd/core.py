@@ -189,6 +189,20 @@ class Plan:
         """d_y = K d_x on the device (raw device addresses, e.g. data_ptr())."""
         check(lib().gfx_plan_spmv(self.h, ctypes.c_void_p(d_x), ctypes.c_void_p(d_y)), self.ctx.h)
 
+    def copy_residual_device(self, d_V):
+        """The assembled residual into a device buffer (raw address)."""
+        check(lib().gfx_plan_copy_residual_device(self.h, ctypes.c_void_p(d_V)), self.ctx.h)
+
+    def dof_constraints(self, dofs, values, d_pr=None, d_rhs=None, flags=0):
+        """Dirichlet condition by simplification on the device CSR (and on the
+        device right-hand side d_rhs); flags from BUILD_MATRIX, BUILD_RHS,
+        MODEL_LINEAR, MODEL_SYMMETRIC."""
+        dofs = np.ascontiguousarray(dofs, dtype=np.int32)
+        values = np.ascontiguousarray(np.broadcast_to(values, dofs.shape), dtype=np.float64)
+        v = lambda x: ctypes.c_void_p(x) if x else None  # noqa: E731
+        check(lib().gfx_plan_dof_constraints(self.h, dofs.size, _ptr(dofs), _ptr(values), v(d_pr),
+                                             v(d_rhs), flags), self.ctx.h)
+
     def solve_cg(self, d_b, d_x, rtol=1e-10, maxit=10000):
         """Jacobi-preconditioned CG on the device: d_x (initial guess in,
         solution out) and d_b are raw device addresses.  Returns (iterations,

@@ -327,6 +327,16 @@ int gfx_ws_assembly(gfx_workspace *w, int order) {
   return GFX_OK;
 }
 
+int gfx_ws_copy_assembled_vector_device(gfx_workspace *w, double *d_V) {
+  if (!w || !d_V) return GFX_E_INVALID;
+  if (!w->V_out) return wfail(w, GFX_E_STATE, "no order-1 assembly done yet");
+  if (hipSetDevice(w->ctx->device) != hipSuccess ||
+      hipMemcpyAsync(d_V, w->V_out, 8 * (size_t)w->ndof, hipMemcpyDeviceToDevice, w->ctx->stream) !=
+          hipSuccess)
+    return wfail(w, GFX_E_HIP, "copy of the assembled vector");
+  return GFX_OK;
+}
+
 int gfx_ws_assembled_vector(gfx_workspace *w, double *h_V) {
   if (!w || !h_V) return GFX_E_INVALID;
   if (!w->V_out) return wfail(w, GFX_E_STATE, "no order-1 assembly done yet");

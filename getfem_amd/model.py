@@ -1,0 +1,235 @@
+"""A model over the device assembly, after getfem::model (src/getfem_models.cc)
+and the Python interface's gf.Model('real'), for one fem variable, generic
+assembly terms and Dirichlet conditions by simplification.
+
+What runs where: the tangent matrix and the residual are assembled on the
+device by the workspace (ga_workspace inside model::assembly,
+src/getfem_models.cc:2646-2736, one device call instead of one per OpenMP
+partition); the dof constraints are applied to the device CSR
+(src/getfem_models.cc:2806-2870); the linear systems are solved on the device
+by Jacobi-preconditioned CG; the Newton loop follows classical_Newton
+(src/getfem/getfem_model_solvers.h:548-605) with basic_newton_line_search
+(:336-360).  The state vector stays on the device between iterations; the
+host sees scalars only (and one copy of the state to the workspace per
+assembly).
+
+Sign convention as model::assembly: rhs = -(order-1 assembly of the
+nonlinear terms); a linear model's rhs holds only the constraint data.
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import check, lib
+from .workspace import GaWorkspace
+
+
+class Model:
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.ws = GaWorkspace(ctx)
+        self.var = None
+        self.mf = None
+        self.data = {}
+        self.terms = []  # (expr, mim, linear, symmetric, coercive)
+        self.dirichlet = []  # (dofs, dataname or None)
+        self.U = None  # device state (torch)
+        self._rhs = None
+        self._built = False
+
+    # -------------------------------------------------------------- variables
+    def add_fem_variable(self, name, mf):
+        """model::add_fem_variable (one fem variable per model here)."""
+        import torch
+        if self.var is not None:
+            raise ValueError("this model holds one fem variable")
+        self.var, self.mf = name, mf
+        self.U = torch.zeros(mf.nbdof(), dtype=torch.float64, device="cuda")
+        self.ws.add_fem_variable(name, mf, np.zeros(mf.nbdof()))
+
+    def add_initialized_data(self, name, value):
+        """model::add_initialized_fixed_size_data."""
+        v = np.atleast_1d(np.asarray(value, dtype=np.float64))
+        self.data[name] = v
+        self.ws.add_fixed_size_constant(name, v)
+        self._built = False
+
+    def add_initialized_fem_data(self, name, mf, V):
+        """model::add_initialized_fem_data (P0 data on the variable's mesh)."""
+        V = np.asarray(V, dtype=np.float64)
+        self.data[name] = V
+        self.ws.add_fem_constant(name, mf, V)
+
+    def set_variable(self, name, V):
+        import torch
+        if name == self.var:
+            self.U.copy_(torch.as_tensor(np.asarray(V, dtype=np.float64)))
+        else:
+            V = np.atleast_1d(np.asarray(V, dtype=np.float64))
+            self.data[name] = V
+            self.ws.set_values(name, V)
+
+    def variable(self, name):
+        if name == self.var:
+            return self.U.cpu().numpy()
+        return self.data[name]
+
+    # ------------------------------------------------------------------ bricks
+    def add_linear_term(self, mim, expr, region=-1, is_symmetric=False, is_coercive=False):
+        """getfem::add_linear_term (src/getfem_models.cc, generic linear
+        assembly brick)."""
+        self._add_term(mim, expr, region, True, is_symmetric, is_coercive)
+
+    def add_nonlinear_term(self, mim, expr, region=-1, is_symmetric=False, is_coercive=False):
+        """getfem::add_nonlinear_term (generic nonlinear assembly brick)."""
+        self._add_term(mim, expr, region, False, is_symmetric, is_coercive)
+
+    def _add_term(self, mim, expr, region, linear, sym, coercive):
+        if region not in (-1, None):
+            raise ValueError("only region -1 (all elements) is supported")
+        self.terms.append((expr, mim, linear, sym, coercive))
+        self.ws.add_expression(expr, mim, 2)
+
+    def add_Dirichlet_condition_with_simplification(self, varname, dofs, dataname=None):
+        """getfem::add_Dirichlet_condition_with_simplification
+        (src/getfem_models.cc:4864-4940, :5019-5030).  `dofs` are the
+        constrained dofs (GetFEM takes a mesh region and uses
+        mf.dof_on_region(rg)).  dataname: fixed-size data of qdim values
+        (value of dof i = data[i % qdim]) or fem data on the same mesh_fem;
+        None = homogeneous."""
+        if varname != self.var:
+            raise ValueError("unknown variable " + varname)
+        self.dirichlet.append((np.asarray(dofs, dtype=np.int64), dataname))
+
+    def is_linear(self):
+        return all(t[2] for t in self.terms)
+
+    def is_symmetric(self):
+        return all(t[3] for t in self.terms)
+
+    def is_coercive(self):
+        return all(t[4] for t in self.terms)
+
+    # ---------------------------------------------------------------- assembly
+    def _constraints(self):
+        dofs, vals = [], []
+        q = self.mf.qdim
+        for d, name in self.dirichlet:
+            if name is None:
+                v = np.zeros(d.size)
+            else:
+                A = self.data[name]
+                v = A[d % q] if A.size == q else A[d]
+            dofs.append(d)
+            vals.append(v)
+        if not dofs:
+            return np.zeros(0, np.int32), np.zeros(0)
+        d = np.concatenate(dofs)
+        v = np.concatenate(vals)
+        order = np.argsort(d, kind="stable")  # a dof constrained twice keeps its last value
+        d, v = d[order], v[order]
+        last = np.r_[d[1:] != d[:-1], True]
+        return d[last].astype(np.int32), v[last]
+
+    def _plan(self):
+        h = ctypes.c_void_p()
+        check(lib().gfx_ws_plan(self.ws.h, ctypes.byref(h)), self.ctx.h)
+        return h
+
+    def approx_external_load(self):
+        """model::approx_external_load: here the 1-norm of the Dirichlet data."""
+        return float(np.abs(self._constraints()[1]).sum())
+
+    def assembly(self, option="build_all"):
+        """model::assembly(BUILD_ALL | BUILD_MATRIX | BUILD_RHS): tangent
+        matrix into the workspace plan's device CSR, rhs into a device vector."""
+        import torch
+        build_matrix = option in ("build_all", "build_matrix")
+        build_rhs = option in ("build_all", "build_rhs")
+        self.ws.set_values(self.var, self.U.cpu().numpy())
+        n = self.mf.nbdof()
+        rhs = torch.zeros(n, dtype=torch.float64, device="cuda")
+        if build_matrix or not self._built:
+            self.ws.assembly(2)
+            self._built = True
+        if build_rhs and not self.is_linear():
+            self.ws.assembly(1)
+            check(lib().gfx_ws_copy_assembled_vector_device(self.ws.h, ctypes.c_void_p(rhs.data_ptr())),
+                  self.ctx.h)
+            check(lib().gfx_ctx_synchronize(self.ctx.h), self.ctx.h)
+            rhs.neg_()
+        dofs, vals = self._constraints()
+        flags = (_lib.BUILD_MATRIX if build_matrix else 0) | (_lib.BUILD_RHS if build_rhs else 0)
+        flags |= _lib.MODEL_LINEAR if self.is_linear() else 0
+        flags |= _lib.MODEL_SYMMETRIC if self.is_symmetric() else 0
+        if dofs.size:
+            check(lib().gfx_plan_dof_constraints(self._plan(), dofs.size, ctypes.c_void_p(dofs.ctypes.data),
+                                                 ctypes.c_void_p(vals.ctypes.data),
+                                                 ctypes.c_void_p(self.U.data_ptr()),
+                                                 ctypes.c_void_p(rhs.data_ptr()), flags), self.ctx.h)
+        check(lib().gfx_ctx_synchronize(self.ctx.h), self.ctx.h)
+        self._rhs = rhs
+        return rhs
+
+    def tangent_matrix(self):
+        return self.ws.assembled_matrix()
+
+    def rhs(self):
+        return self._rhs.cpu().numpy()
+
+    def _cg(self, rhs, dx, rtol, maxit):
+        it, res = ctypes.c_int(), ctypes.c_double()
+        check(lib().gfx_plan_solve_cg(self._plan(), ctypes.c_void_p(rhs.data_ptr()),
+                                      ctypes.c_void_p(dx.data_ptr()), rtol, maxit, ctypes.byref(it),
+                                      ctypes.byref(res)), self.ctx.h)
+        return it.value, res.value
+
+    # ------------------------------------------------------------------- solve
+    def solve(self, max_iter=100, max_res=1e-6, cg_maxit=20000, noisy=False):
+        """getfem::standard_solve for this model: a linear model is one CG solve
+        of K U = rhs; otherwise classical_Newton with basic_newton_line_search.
+        The systems must be symmetric positive definite (CG).  Returns
+        (newton iterations, converged, list of (cg iterations, cg residual))."""
+        import torch
+        if not self.is_symmetric():
+            raise ValueError("the device solver is CG: declare the terms symmetric (is_symmetric=True)")
+        cg_rtol = max_res / 20.0  # iter_linsolv0.set_resmax(iter.get_resmax()/20)
+        log = []
+        if self.is_linear():
+            rhs = self.assembly("build_all")
+            it, res = self._cg(rhs, self.U, cg_rtol, cg_maxit)
+            log.append((it, res))
+            return 1, res <= cg_rtol, log
+        eln = self.approx_external_load() or 1.0
+        rhs = self.assembly("build_rhs")
+        res = float(rhs.abs().sum())
+        crit = res / eln
+        k = 0
+        while crit > max_res and k < max_iter:
+            rhs = self.assembly("build_all")
+            dU = torch.zeros_like(self.U)
+            it, cres = self._cg(rhs, dU, cg_rtol, cg_maxit)
+            log.append((it, cres))
+            if cres > cg_rtol:
+                break
+            # basic_newton_line_search: alpha = 1, 0.6, 0.36, ...
+            U0 = self.U.clone()
+            first = prev = res
+            alpha, tries = 1.0, 0
+            while True:
+                conv_alpha = alpha
+                alpha *= 3.0 / 5.0
+                tries += 1
+                self.U.copy_(U0 + conv_alpha * dU)
+                r = float(self.assembly("build_rhs").abs().sum())
+                if (k == 0 or r < first / 2 or (conv_alpha <= 1e-3 and r < 2 * first)
+                        or (tries > 1 and r > prev and prev < 5.0 / 3.0 * first)):
+                    break
+                prev = r
+            res = r
+            k += 1
+            crit = min(res / eln, float(dU.abs().sum()) / max(1e-25, float(self.U.abs().sum())))
+            if noisy:
+                print("newton %d: alpha %g residual %.3e crit %.3e (cg %d its)" % (k, conv_alpha, res, crit, it))
+        return k, crit <= max_res, log

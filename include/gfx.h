@@ -262,6 +262,8 @@ int gfx_ws_compile(gfx_workspace *w);
 int gfx_ws_assembly(gfx_workspace *w, int order);
 /* ga_workspace::assembled_vector() (.h:389-390): nb_dof doubles to the host */
 int gfx_ws_assembled_vector(gfx_workspace *w, double *h_V);
+/* the same vector into a caller-owned device buffer (stream-ordered) */
+int gfx_ws_copy_assembled_vector_device(gfx_workspace *w, double *d_V);
 int gfx_ws_plan(gfx_workspace *w, gfx_plan **out);
 /* normal form of the compiled term, e.g. "isotropic elasticity unknown=u ..." */
 int gfx_ws_describe(gfx_workspace *w, char *buf, int64_t len);
