@@ -420,12 +420,12 @@ __global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArg
     }
   }
   __syncthreads();
-  if (full) {  // every entry present: rows are contiguous copies of the accumulator
-    for (int x = 0; x < Q; ++x) {
-      double *dst = a.vals + rowbase[x];
-      const double *src = acc + x * ns * Q;
-      for (int d = lane; d < ns * Q; d += kWave) dst[d] = src[d];
-    }
+  // The values are written once and not read again by this kernel: stream
+  // them past the caches (non-temporal stores), which keeps L2 for the
+  // element records and incidence data the neighbouring patches reuse.
+  if (full) {  // every entry present: the node's Q rows are one contiguous run
+    double *dst = a.vals + rowbase[0];
+    for (int d = lane; d < Q * ns * Q; d += kWave) __builtin_nontemporal_store(acc[d], dst + d);
     return;
   }
   for (int x = 0; x < Q; ++x) {
@@ -435,7 +435,7 @@ __global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArg
       const int d = c0 + lane;
       const bool pr = d < len && mask_bit(smask, off + d);
       const uint64_t bal = __ballot(pr);
-      if (pr) a.vals[base + __popcll(bal & ((1ull << lane) - 1ull))] = acc[off + d];
+      if (pr) __builtin_nontemporal_store(acc[off + d], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
       base += __popcll(bal);
     }
   }

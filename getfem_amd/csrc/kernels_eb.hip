@@ -182,7 +182,8 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   }
   if (owner) {
     double *E = a.E + (int64_t)e * kNl * kNl;
-    for (int i = 0; i < kNd; ++i) E[(3 * i + ar) * kNl + jb] = (double)K[i];
+    // streamed to HBM: the buffer (GBs) is read back once by the gather
+    for (int i = 0; i < kNd; ++i) __builtin_nontemporal_store((double)K[i], E + (3 * i + ar) * kNl + jb);
   }
   if (t == 0) a.thr[e] = red[0] > 0.0 ? 1e-14 * red[0] : INFINITY;
 }
@@ -245,17 +246,15 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   const int total = ninc * RUN;
   for (int f = t; f < total; f += kGatherThreads) {
     const int k = f / RUN, o = f - k * RUN;
-    const double v = E[sbase[k] + o];
+    const double v = __builtin_nontemporal_load(E + sbase[k] + o);  // read once
     const int x = o / NL, c = o - x * NL, j = c / Q, y = c - j * Q;
     if (fabs(v) > sthr[k]) atomicAdd(&acc[(x * ns + sslot[k * ND + j]) * Q + y], v);
   }
   __syncthreads();
-  if (full) {  // plain copies of the accumulator rows
-    for (int x = 0; x < Q; ++x) {
-      double *dst = a.vals + a.indptr[node * Q + x];
-      const double *src = acc + x * ns * Q;
-      for (int d = t; d < ns * Q; d += kGatherThreads) dst[d] = src[d];
-    }
+  // values are streamed past the caches (written once, never re-read here)
+  if (full) {  // the node's Q full rows are one contiguous run of the CSR
+    double *dst = a.vals + a.indptr[node * Q];
+    for (int d = t; d < Q * ns * Q; d += kGatherThreads) __builtin_nontemporal_store(acc[d], dst + d);
     return;
   }
   // write: wavefront x handles row x
@@ -269,7 +268,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
       const int d = c0 + lane;
       const bool pr = d < len && mbit(m, off + d);
       const uint64_t bal = __ballot(pr);
-      if (pr) a.vals[base + __popcll(bal & ((1ull << lane) - 1ull))] = acc[off + d];
+      if (pr) __builtin_nontemporal_store(acc[off + d], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
       base += __popcll(bal);
     }
   }
