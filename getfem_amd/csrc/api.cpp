@@ -345,7 +345,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_coef[0]); dfree(p->d_coef[1]);
   dfree(p->d_gref); dfree(p->d_qw); dfree(p->d_R);
   dfree(p->d_rec); dfree(p->d_thr); dfree(p->d_keep);
-  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order);
+  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order); dfree(p->d_desc);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
   dfree(p->d_gN); dfree(p->d_E); dfree(p->d_res); dfree(p->d_U); dfree(p->d_rhs); dfree(p->d_cg);
@@ -544,7 +544,7 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   HIPCHK(c, hipMemsetAsync(p->d_err, 0, 4, s));
   if ((r = p->eb ? plan_eb_elements(p, GFX_PREC_FP64) : plan_records(p, false))) return r;
   // incidences
-  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order);
+  dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order); dfree(p->d_desc);
   dfree(p->d_nbr); dfree(p->d_mask); dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals);
   dfree(p->d_keep);
   p->nnz = -1;
@@ -605,6 +605,9 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   if ((size_t)Q * Q * p->max_nslots * 8 > 64 * 1024)
     return fail(c, GFX_E_PATTERN, "node patch too large for the assembly kernel");
   if ((r = plan_bins(p, ns))) return r;
+  dfree(p->d_desc);
+  HIPCHK(c, dalloc(&p->d_desc, (size_t)nn));
+  HIPCHK(c, launch_fill_desc(s, nn, Q, p->d_order, p->d_inc_ptr, p->d_nslots, p->d_indptr, p->d_desc));
   HIPCHK(c, dalloc(&p->d_indices, (size_t)nnz));
   HIPCHK(c, dalloc(&p->d_vals, (size_t)nnz));
   a.indices = p->d_indices;
@@ -645,6 +648,7 @@ int gfx_plan_assemble(gfx_plan *p) {
   for (const gfx_plan::Bin &b : p->bins) {  // one launch per patch-size bin
     a.n_nodes = b.count;
     a.order = p->d_order + b.off;
+    a.desc = p->d_desc + b.off;
     if (p->eb) HIPCHK(c, launch_gather_emat(c->stream, a, p->d_E, p->d_thr, b.cap));
     else HIPCHK(c, launch_assemble_patch(c->stream, p->term, p->form, p->precision, a, b.cap));
   }

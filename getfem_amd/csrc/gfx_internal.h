@@ -50,9 +50,19 @@ constexpr int kRecStride = 16;  // doubles per element record
 // [0..8] B = K^{-T} row-major, [9] |det K|, [10] coefficient a,
 // [11] coefficient b, [12] insertion threshold, [13..15] unused.
 
+// What a patch kernel needs first about its node, in one 16-byte load:
+// the node's first CSR value, its incidence range and its slot count (bit 15:
+// all rows full).
+struct PatchDesc {
+  int64_t rowbase;          // indptr[node * qdim]
+  int32_t i0;               // inc_ptr[node]
+  uint32_t ninc_ns;         // incidences (low 16 bits) | nslots (high 16 bits)
+};
+
 struct PatchArgs {
   int64_t n_nodes;
-  const int32_t *order;     // node ids of this launch's bin (null: nodes 0..n_nodes-1)
+  const int32_t *order;     // node ids of this launch's bin
+  const PatchDesc *desc;    // descriptors of the same nodes
   const int32_t *inc_ptr;   // n_nodes+1
   const uint32_t *inc;      // (element << 5) | local node
   const uint16_t *slot;     // per incidence, nd slots
@@ -122,6 +132,8 @@ hipError_t launch_incidences(hipStream_t s, int64_t n_cv, int nd, int qdim,
 hipError_t launch_keep(hipStream_t s, int term, const PatternArgs &a);
 hipError_t launch_patch_build(hipStream_t s, const PatternArgs &a, int max_cand);
 hipError_t launch_fill_columns(hipStream_t s, const PatternArgs &a);
+hipError_t launch_fill_desc(hipStream_t s, int64_t n, int qdim, const int32_t *order, const int32_t *inc_ptr,
+                            const uint16_t *nslots, const int64_t *indptr, PatchDesc *desc);
 hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec,
                                  const PatchArgs &a, int max_nslots);
 hipError_t scan_exclusive_i64(hipStream_t s, const int64_t *in, int64_t *out,
@@ -194,6 +206,7 @@ struct gfx_plan {
   // nodes grouped by patch size so each launch reserves LDS for its own
   // largest patch: d_order holds the bins back to back
   int32_t *d_order = nullptr;
+  gfx::PatchDesc *d_desc = nullptr;  // per position of d_order
   struct Bin { int64_t off, count; int cap; };
   std::vector<Bin> bins;
   int64_t ndof = 0, nnz = -1;

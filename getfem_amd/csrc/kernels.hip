@@ -362,6 +362,19 @@ __global__ __launch_bounds__(64) void k_fill_cols(PatternArgs a) {
   }
 }
 
+__global__ void k_fill_desc(int64_t n, int qdim, const int32_t *__restrict__ order,
+                            const int32_t *__restrict__ inc_ptr, const uint16_t *__restrict__ nslots,
+                            const int64_t *__restrict__ indptr, PatchDesc *__restrict__ desc) {
+  const int64_t p = gtid();
+  if (p >= n) return;
+  const int64_t node = order[p];
+  PatchDesc d;
+  d.rowbase = indptr[node * qdim];
+  d.i0 = inc_ptr[node];
+  d.ninc_ns = (uint32_t)(inc_ptr[node + 1] - d.i0) | ((uint32_t)nslots[node] << 16);
+  desc[p] = d;
+}
+
 // ------------------------------------------------------------------------
 // the assembly kernel: one workgroup (one wavefront) per node patch
 // ------------------------------------------------------------------------
@@ -378,13 +391,12 @@ __global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArg
   const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const int64_t pos = ((wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3)) * kPatchWaves + w;
   const bool live = pos < a.n_nodes;
-  const int64_t node = (a.order && live) ? a.order[pos] : pos;
-  const int i0 = live ? a.inc_ptr[node] : 0, ninc = live ? a.inc_ptr[node + 1] - i0 : 0;
-  const int nsraw = live ? a.nslots[node] : 0;
+  PatchDesc dsc{0, 0, 0u};
+  if (live) dsc = a.desc[pos];
+  const int i0 = dsc.i0, ninc = (int)(dsc.ninc_ns & 0xffffu);
+  const int nsraw = (int)(dsc.ninc_ns >> 16);
   const int ns = nsraw & (kFullRows - 1);
   const bool full = (nsraw & kFullRows) != 0;
-  int64_t rowbase[Q];
-  for (int x = 0; x < Q; ++x) rowbase[x] = live ? a.indptr[node * Q + x] : 0;
   double *lds = lds_all + (size_t)w * a.lds_wave;  // this wavefront's region
   double *acc = lds;
   uint32_t *smask = reinterpret_cast<uint32_t *>(lds + a.lds_acc);
@@ -424,12 +436,13 @@ __global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArg
   // them past the caches (non-temporal stores), which keeps L2 for the
   // element records and incidence data the neighbouring patches reuse.
   if (full) {  // every entry present: the node's Q rows are one contiguous run
-    double *dst = a.vals + rowbase[0];
+    double *dst = a.vals + dsc.rowbase;
     for (int d = lane; d < Q * ns * Q; d += kWave) __builtin_nontemporal_store(acc[d], dst + d);
     return;
   }
+  const int64_t node = live ? a.order[pos] : 0;
   for (int x = 0; x < Q; ++x) {
-    int64_t base = rowbase[x];
+    int64_t base = x == 0 ? dsc.rowbase : (live ? a.indptr[node * Q + x] : 0);
     const int off = x * ns * Q, len = ns * Q;  // len = 0 when not live
     for (int c0 = 0; c0 < len; c0 += kWave) {
       const int d = c0 + lane;
@@ -546,6 +559,13 @@ hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,
   if (n_cv == 0) return hipSuccess;
   k_records_tet<<<nblocks(n_cv, 256), 256, 0, s>>>(n_cv, pts, cvx, ca, ca_c, cb, cb_c, thr,
                                                     rec, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_desc(hipStream_t s, int64_t n, int qdim, const int32_t *order, const int32_t *inc_ptr,
+                            const uint16_t *nslots, const int64_t *indptr, PatchDesc *desc) {
+  if (n == 0) return hipSuccess;
+  k_fill_desc<<<nblocks(n, 256), 256, 0, s>>>(n, qdim, order, inc_ptr, nslots, indptr, desc);
   return hipGetLastError();
 }
 

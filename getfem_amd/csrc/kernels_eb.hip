@@ -226,11 +226,11 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   const int64_t wg = blockIdx.x;
   const int64_t pos = (wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3);
   if (pos >= a.n_nodes) return;
-  const int64_t node = a.order ? a.order[pos] : pos;
+  const PatchDesc dsc = a.desc[pos];
   constexpr int NL = ND * Q, RUN = Q * NL;
   const int t = threadIdx.x;
-  const int i0 = a.inc_ptr[node], ninc = a.inc_ptr[node + 1] - i0;
-  const int nsraw = a.nslots[node];
+  const int i0 = dsc.i0, ninc = (int)(dsc.ninc_ns & 0xffffu);
+  const int nsraw = (int)(dsc.ninc_ns >> 16);
   const int ns = nsraw & 0x7fff;
   const bool full = (nsraw & 0x8000) != 0;  // all of the node's rows are full
   if (ninc > kMaxInc) return;  // excluded by the launcher (max_inc check)
@@ -253,7 +253,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   __syncthreads();
   // values are streamed past the caches (written once, never re-read here)
   if (full) {  // the node's Q full rows are one contiguous run of the CSR
-    double *dst = a.vals + a.indptr[node * Q];
+    double *dst = a.vals + dsc.rowbase;
     for (int d = t; d < Q * ns * Q; d += kGatherThreads) __builtin_nontemporal_store(acc[d], dst + d);
     return;
   }
@@ -262,6 +262,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   const int w = t / 64, lane = t % 64;
   if (w < Q) {
     const int x = w;
+    const int64_t node = a.order[pos];
     int64_t base = a.indptr[node * Q + x];
     const int off = x * ns * Q, len = ns * Q;
     for (int c0 = 0; c0 < len; c0 += 64) {
