@@ -484,7 +484,7 @@ static int check_err_flag(gfx_plan *p) {
 // kernels reserve LDS for the largest patch of a launch, so one launch per
 // group lets the many small patches run at the occupancy their own size
 // allows.  Size classes with few nodes join the next class up.
-static int plan_bins(gfx_plan *p, const std::vector<uint16_t> &ns) {
+static int plan_bins(gfx_plan *p, const std::vector<uint16_t> &ns, const std::vector<int32_t> &iptr) {
   gfx_ctx *c = p->ctx;
   const int64_t nn = p->n_nodes;
   static const int kCaps[] = {8,   16,  24,  32,   48,   64,   96,   128,  192,   256,   384,
@@ -514,11 +514,23 @@ static int plan_bins(gfx_plan *p, const std::vector<uint16_t> &ns) {
     pending += cnt[k];
     gmax = std::max(gmax, cmax[k]);
     if (pending && (pending >= kMinBin || k == last)) {
-      p->bins.push_back({off, pending, std::max(gmax, 1)});
+      p->bins.push_back({off, pending, std::max(gmax, 1), 1});
       off += pending;
       pending = 0;
       gmax = 0;
     }
+  }
+  // patches per wavefront: small patches are packed so that one wave's
+  // incidences fill about two passes of its lanes
+  std::vector<int> bmax(p->bins.size(), 0);
+  for (int64_t n = 0; n < nn; ++n) {
+    const int b = group[cls[n]];
+    bmax[b] = std::max(bmax[b], iptr[n + 1] - iptr[n]);
+  }
+  const int lanes = (64 / p->nd) * p->nd;
+  for (size_t b = 0; b < p->bins.size(); ++b) {
+    const int per = std::max(1, bmax[b]) * p->nd;
+    p->bins[b].npw = p->eb ? 1 : std::max(1, std::min(4, 2 * lanes / per));
   }
   std::vector<int64_t> cur(p->bins.size());
   for (size_t b = 0; b < p->bins.size(); ++b) cur[b] = p->bins[b].off;
@@ -604,7 +616,7 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   for (int64_t n = 0; n < nn; ++n) p->max_nslots = std::max<int>(p->max_nslots, ns[n] & 0x7fff);
   if ((size_t)Q * Q * p->max_nslots * 8 > 64 * 1024)
     return fail(c, GFX_E_PATTERN, "node patch too large for the assembly kernel");
-  if ((r = plan_bins(p, ns))) return r;
+  if ((r = plan_bins(p, ns, iptr))) return r;
   dfree(p->d_desc);
   HIPCHK(c, dalloc(&p->d_desc, (size_t)nn));
   HIPCHK(c, launch_fill_desc(s, nn, Q, p->d_order, p->d_inc_ptr, p->d_nslots, p->d_indptr, p->d_desc));
@@ -649,6 +661,7 @@ int gfx_plan_assemble(gfx_plan *p) {
     a.n_nodes = b.count;
     a.order = p->d_order + b.off;
     a.desc = p->d_desc + b.off;
+    a.npw = b.npw;
     if (p->eb) HIPCHK(c, launch_gather_emat(c->stream, a, p->d_E, p->d_thr, b.cap));
     else HIPCHK(c, launch_assemble_patch(c->stream, p->term, p->form, p->precision, a, b.cap));
   }

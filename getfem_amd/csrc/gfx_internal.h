@@ -78,7 +78,8 @@ struct PatchArgs {
   double *vals;             // CSR values
   int xcd_chunk;            // nodes per XCD in the block swizzle
   int max_inc;              // largest number of incidences of a node
-  int lds_acc, lds_mask, lds_inc, lds_wave;  // dynamic LDS layout (set by the launcher)
+  int npw;                  // node patches per wavefront (1..4)
+  int lds_acc, lds_mask, lds_inc;  // dynamic LDS layout (set by the launcher)
 };
 
 struct PatternArgs {
@@ -207,7 +208,7 @@ struct gfx_plan {
   // largest patch: d_order holds the bins back to back
   int32_t *d_order = nullptr;
   gfx::PatchDesc *d_desc = nullptr;  // per position of d_order
-  struct Bin { int64_t off, count; int cap; };
+  struct Bin { int64_t off, count; int cap, npw; };
   std::vector<Bin> bins;
   int64_t ndof = 0, nnz = -1;
   int64_t *d_indptr = nullptr;

@@ -378,56 +378,91 @@ __global__ void k_fill_desc(int64_t n, int qdim, const int32_t *__restrict__ ord
 // ------------------------------------------------------------------------
 // the assembly kernel: one workgroup (one wavefront) per node patch
 // ------------------------------------------------------------------------
-// Dynamic LDS: [acc: Q*Q*max_ns doubles][mask: mwmax words][inc: max_inc]
-// [slots: max_inc*ND u16].  The patch's incidences, slot maps and presence
-// words are staged in one round of independent loads, so a wavefront waits
-// on two memory latencies (staging, element records) before it computes.
-constexpr int kPatchWaves = 1;  // node patches per workgroup, one per wavefront (1: LDS sized per patch)
+// A wavefront assembles npw node patches of one bin at once (npw = 1 for
+// large patches, up to kMaxNpw for small ones): their incidence lists are
+// concatenated, so the lanes of one pass may serve different patches and the
+// chain of dependent loads (descriptor -> incidences -> element records) is
+// paid once for all of them.
+// Dynamic LDS: [acc: npw x lds_acc doubles][mask: npw x lds_mask words]
+// [inc: lds_inc words][slots: lds_inc x ND u16].  The incidences, slot maps
+// and presence words are staged in one round of independent loads.
+constexpr int kMaxNpw = 4;
+
+template <typename T>
+__device__ __forceinline__ T pick4(const T (&v)[kMaxNpw], int k) {
+  return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+}
 
 template <int ND, int Q, int TERM, int FORM, typename Real>
-__global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArgs a) {
-  extern __shared__ double lds_all[];
+__global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
+  extern __shared__ double lds[];
   const int64_t wg = blockIdx.x;
-  const int w = threadIdx.x / kWave, lane = threadIdx.x % kWave;
-  const int64_t pos = ((wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3)) * kPatchWaves + w;
-  const bool live = pos < a.n_nodes;
-  PatchDesc dsc{0, 0, 0u};
-  if (live) dsc = a.desc[pos];
-  const int i0 = dsc.i0, ninc = (int)(dsc.ninc_ns & 0xffffu);
-  const int nsraw = (int)(dsc.ninc_ns >> 16);
-  const int ns = nsraw & (kFullRows - 1);
-  const bool full = (nsraw & kFullRows) != 0;
-  double *lds = lds_all + (size_t)w * a.lds_wave;  // this wavefront's region
+  const int lane = threadIdx.x;
+  const int npw = a.npw;
+  const int64_t pos0 = ((wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3)) * npw;
+  int i0[kMaxNpw], ns[kMaxNpw], st[kMaxNpw], pre[kMaxNpw + 1];
+  bool full[kMaxNpw];
+  int64_t rb[kMaxNpw];
+  pre[0] = 0;
+#pragma unroll
+  for (int k = 0; k < kMaxNpw; ++k) {
+    PatchDesc d{0, 0, 0u};
+    if (k < npw && pos0 + k < a.n_nodes) d = a.desc[pos0 + k];
+    const int nsraw = (int)(d.ninc_ns >> 16);
+    i0[k] = d.i0;
+    ns[k] = nsraw & (kFullRows - 1);
+    full[k] = (nsraw & kFullRows) != 0;
+    rb[k] = d.rowbase;
+    st[k] = pre[k];
+    pre[k + 1] = pre[k] + (int)(d.ninc_ns & 0xffffu);
+  }
+  const int total = pre[kMaxNpw];  // incidences of the wave's patches
   double *acc = lds;
-  uint32_t *smask = reinterpret_cast<uint32_t *>(lds + a.lds_acc);
-  uint32_t *sinc = smask + a.lds_mask;
+  uint32_t *smask = reinterpret_cast<uint32_t *>(lds + (size_t)npw * a.lds_acc);
+  uint32_t *sinc = smask + (size_t)npw * a.lds_mask;
   uint16_t *sslot = reinterpret_cast<uint16_t *>(sinc + a.lds_inc);
-  const int dense = Q * Q * ns, mw = (dense + 31) / 32;
-  for (int t = lane; t < dense; t += kWave) acc[t] = 0.0;
-  const uint32_t *gm = a.mask + (int64_t)i0 * a.mwpi;  // ninc = 0 when not live: no loads
-  if (!full)
-    for (int t = lane; t < mw; t += kWave) smask[t] = gm[t];
-  for (int t = lane; t < ninc; t += kWave) sinc[t] = a.inc[i0 + t];
-  const uint16_t *gs = a.slot + (int64_t)i0 * ND;
-  for (int t = lane; t < ninc * ND; t += kWave) sslot[t] = gs[t];
+#pragma unroll
+  for (int k = 0; k < kMaxNpw; ++k) {
+    if (k >= npw) break;
+    const int dense = Q * Q * ns[k];
+    double *ak = acc + (size_t)k * a.lds_acc;
+    for (int t = lane; t < dense; t += kWave) ak[t] = 0.0;
+    if (!full[k]) {
+      const uint32_t *gm = a.mask + (int64_t)i0[k] * a.mwpi;  // no loads for an empty patch
+      uint32_t *sm = smask + (size_t)k * a.lds_mask;
+      for (int t = lane; t < (dense + 31) / 32; t += kWave) sm[t] = gm[t];
+    }
+  }
+  // patch of combined incidence c: k = number of patch starts at or below c
+  auto patch_of = [&](int c) { return (int)(c >= pre[1]) + (int)(c >= pre[2]) + (int)(c >= pre[3]); };
+  for (int t = lane; t < total; t += kWave) {
+    const int k = patch_of(t);
+    sinc[t] = a.inc[pick4(i0, k) + (t - pick4(st, k))];
+  }
+  for (int t = lane; t < total * ND; t += kWave) {
+    const int c = t / ND, k = patch_of(c);
+    sslot[t] = a.slot[((int64_t)pick4(i0, k) + (c - pick4(st, k))) * ND + (t - c * ND)];
+  }
   __syncthreads();
   constexpr int PER = kWave / ND;  // incidences per pass
   const int kk = lane / ND, j = lane - kk * ND;
-  for (int k0 = 0; k0 < ninc; k0 += PER) {
-    const int k = k0 + kk;
-    if (kk < PER && k < ninc) {
-      const uint32_t inc = sinc[k];
+  for (int c0 = 0; c0 < total; c0 += PER) {
+    const int c = c0 + kk;
+    if (kk < PER && c < total) {
+      const int k = patch_of(c), nsk = pick4(ns, k);
+      double *ak = acc + (size_t)k * a.lds_acc;
+      const uint32_t inc = sinc[c];
       const int64_t e = inc >> 5;
       const int i = inc & 31;
       const double *rec = a.rec + (int64_t)kRecStride * e;
       const double thr = rec[12];
       Real v[Q][Q];
       pair_block<Q, TERM, FORM, Real>(rec, i, j, ND, a.nq, a.gref, a.qw, a.R, v);
-      const int s = sslot[k * ND + j];
+      const int s = sslot[c * ND + j];
       for (int x = 0; x < Q; ++x)
         for (int y = 0; y < Q; ++y) {
           const double val = (double)v[x][y];
-          if (fabs(val) > thr) atomicAdd(&acc[(x * ns + s) * Q + y], val);
+          if (fabs(val) > thr) atomicAdd(&ak[(x * nsk + s) * Q + y], val);
         }
     }
   }
@@ -435,21 +470,28 @@ __global__ __launch_bounds__(kWave * kPatchWaves) void k_assemble_patch(PatchArg
   // The values are written once and not read again by this kernel: stream
   // them past the caches (non-temporal stores), which keeps L2 for the
   // element records and incidence data the neighbouring patches reuse.
-  if (full) {  // every entry present: the node's Q rows are one contiguous run
-    double *dst = a.vals + dsc.rowbase;
-    for (int d = lane; d < Q * ns * Q; d += kWave) __builtin_nontemporal_store(acc[d], dst + d);
-    return;
-  }
-  const int64_t node = live ? a.order[pos] : 0;
-  for (int x = 0; x < Q; ++x) {
-    int64_t base = x == 0 ? dsc.rowbase : (live ? a.indptr[node * Q + x] : 0);
-    const int off = x * ns * Q, len = ns * Q;  // len = 0 when not live
-    for (int c0 = 0; c0 < len; c0 += kWave) {
-      const int d = c0 + lane;
-      const bool pr = d < len && mask_bit(smask, off + d);
-      const uint64_t bal = __ballot(pr);
-      if (pr) __builtin_nontemporal_store(acc[off + d], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
-      base += __popcll(bal);
+#pragma unroll
+  for (int k = 0; k < kMaxNpw; ++k) {
+    if (k >= npw || pos0 + k >= a.n_nodes) break;
+    const double *ak = acc + (size_t)k * a.lds_acc;
+    const int nsk = ns[k];
+    if (full[k]) {  // every entry present: the node's Q rows are one contiguous run
+      double *dst = a.vals + rb[k];
+      for (int d = lane; d < Q * nsk * Q; d += kWave) __builtin_nontemporal_store(ak[d], dst + d);
+      continue;
+    }
+    const uint32_t *sm = smask + (size_t)k * a.lds_mask;
+    const int64_t node = a.order[pos0 + k];
+    for (int x = 0; x < Q; ++x) {
+      int64_t base = x == 0 ? rb[k] : a.indptr[node * Q + x];
+      const int off = x * nsk * Q, len = nsk * Q;
+      for (int c0 = 0; c0 < len; c0 += kWave) {
+        const int d = c0 + lane;
+        const bool pr = d < len && mask_bit(sm, off + d);
+        const uint64_t bal = __ballot(pr);
+        if (pr) __builtin_nontemporal_store(ak[off + d], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
+        base += __popcll(bal);
+      }
     }
   }
 }
@@ -616,14 +658,14 @@ static hipError_t launch_asm3(hipStream_t s, int form, int prec, const PatchArgs
                               unsigned grid, size_t lds) {
   if (form == GFX_FORM_FACTORIZED) {
     if (prec == GFX_PREC_FP64)
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, double><<<grid, 64 * kPatchWaves, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, double><<<grid, kWave, lds, s>>>(a);
     else
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, float><<<grid, 64 * kPatchWaves, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, float><<<grid, kWave, lds, s>>>(a);
   } else {
     if (prec == GFX_PREC_FP64)
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, double><<<grid, 64 * kPatchWaves, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, double><<<grid, kWave, lds, s>>>(a);
     else
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, float><<<grid, 64 * kPatchWaves, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, float><<<grid, kWave, lds, s>>>(a);
   }
   return hipGetLastError();
 }
@@ -632,17 +674,16 @@ hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec,
                                  const PatchArgs &a0, int max_nslots) {
   if (a0.n_nodes == 0) return hipSuccess;
   PatchArgs a = a0;
-  const int64_t groups = (a.n_nodes + kPatchWaves - 1) / kPatchWaves;
-  a.xcd_chunk = (int)((groups + 7) / 8);
+  if (a.npw < 1 || a.npw > kMaxNpw) return hipErrorInvalidValue;
+  const int64_t waves = (a.n_nodes + a.npw - 1) / a.npw;
+  a.xcd_chunk = (int)((waves + 7) / 8);
   const unsigned grid = (unsigned)(8 * (int64_t)a.xcd_chunk);
   const int dense = a.qdim * a.qdim * max_nslots;
-  a.lds_acc = dense;                                   // doubles
-  a.lds_mask = ((dense + 31) / 32 + 1) & ~1;           // u32 words, keeps 8-byte alignment
-  a.lds_inc = (a.max_inc + 1) & ~1;                    // u32 words
-  const size_t per_wave = 8 * (size_t)a.lds_acc + 4 * (size_t)(a.lds_mask + a.lds_inc) +
-                          2 * (size_t)a.max_inc * a.nd;
-  a.lds_wave = (int)((per_wave + 7) / 8);              // doubles per wavefront region
-  const size_t lds = 8 * (size_t)a.lds_wave * kPatchWaves;
+  a.lds_acc = dense;                                   // doubles per patch
+  a.lds_mask = ((dense + 31) / 32 + 1) & ~1;           // u32 words per patch, keeps 8-byte alignment
+  a.lds_inc = (a.npw * a.max_inc + 1) & ~1;            // u32 words for the wave's incidences
+  const size_t lds = 8 * (size_t)a.npw * a.lds_acc + 4 * (size_t)a.npw * a.lds_mask +
+                     4 * (size_t)a.lds_inc + 2 * (size_t)a.lds_inc * a.nd;
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   const int nd = a.nd, Q = a.qdim;
   if (term == GFX_TERM_LAPLACE && Q == 1) {
