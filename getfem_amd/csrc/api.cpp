@@ -521,7 +521,7 @@ static int plan_bins(gfx_plan *p, const std::vector<uint16_t> &ns, const std::ve
     }
   }
   // patches per wavefront: small patches are packed so that one wave's
-  // incidences fill about two passes of its lanes
+  // incidences fill about three passes of its lanes
   std::vector<int> bmax(p->bins.size(), 0);
   for (int64_t n = 0; n < nn; ++n) {
     const int b = group[cls[n]];
@@ -530,7 +530,7 @@ static int plan_bins(gfx_plan *p, const std::vector<uint16_t> &ns, const std::ve
   const int lanes = (64 / p->nd) * p->nd;
   for (size_t b = 0; b < p->bins.size(); ++b) {
     const int per = std::max(1, bmax[b]) * p->nd;
-    p->bins[b].npw = p->eb ? 1 : std::max(1, std::min(4, 2 * lanes / per));
+    p->bins[b].npw = p->eb ? 1 : std::max(1, std::min(4, 3 * lanes / per));
   }
   std::vector<int64_t> cur(p->bins.size());
   for (size_t b = 0; b < p->bins.size(); ++b) cur[b] = p->bins[b].off;
