@@ -20,17 +20,20 @@ namespace {
 constexpr int kEbThreads = 256;
 constexpr int kNd = 27, kNl = 81, kNv = 8;
 
-// One workgroup per element, two phases.
+// One workgroup per element.
 // Phase A, every integration point at once (FP64):
 //   K_q = G grad(N)(q), J_q, B_q = K_q^{-T}, wJ_q = |J_q| w_q     (trilinear map)
 //   g[q][i] = B_q grad(phi_i)(q);  H_q = sum_i U_i (x) g[q][i];  F_q = I + H_q
 //   S_q = mu (I - C^-1) + lambda/2 ln(I3) C^-1,  A = dS/dE  (Bonet law)
-//   D_q[(a,c),(b,n)] = wJ_q (delta_ab S[n][c] + sum_km F[a][k] A[k c m n] F[b][m])
 //   re[(i,a)] = sum_q wJ_q (F_q S_q g[q][i])_a
-// Phase B, no barriers: thread (a, j, b) owns column (j,b) of the rows (i,a),
-//   K[(i,a),(j,b)] = sum_q sum_c g[q][i][c] T_q[c],  T_q[c] = sum_n D_q[(a,c),(b,n)] g[q][j][n]
-// with the g[q][i] reads uniform across the wavefront (LDS broadcast).
+// Phase B, over chunks of kQc points:
+//   B0  D_q[(a,c),(b,n)] = wJ_q (delta_ab S[n][c] + sum_km F[a][k] A[k c m n] F[b][m])
+//   B1  T_q[(a,c),(j,b)] = sum_n D_q[(a,c),(b,n)] g[q][j][n]
+//   B2  K[(i,a),(j,b)] += sum_c g[q][i][c] T_q[(a,c),(j,b)]
+// B2 is a register-blocked product: thread (a, i-block of 3, jb-block of 9)
+// keeps 27 sums, and each (q, c) step reads 3 + 9 values for 27 FMAs.
 constexpr int kNq = 27;
+constexpr int kQc = 3;  // points per phase-B chunk
 
 template <typename Real>
 __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
@@ -40,7 +43,9 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   constexpr bool kLow = !std::is_same<Real, double>::value;
   __shared__ Real grl[kLow ? kNq : 1][kNd][3];  // low-precision copy (FP32 form only)
   Real (*gr)[kNd][3] = kLow ? grl : reinterpret_cast<Real (*)[kNd][3]>(gq);
-  __shared__ Real Dq[kNq][81];
+  __shared__ double Ac[kQc][81];
+  __shared__ Real Dc[kQc][81];
+  __shared__ Real Tc[kQc][9][kNl];
   __shared__ double red[kEbThreads];
   const int64_t e = blockIdx.x;
   if (e >= a.n_cv || a.nq != kNq) return;
@@ -122,24 +127,7 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     }
   }
   __syncthreads();
-  // A5: material tangent per point (weights folded in) and the residual
-  for (int x = t; x < kNq * 81; x += kEbThreads) {
-    const int q = x / 81, ac = (x % 81) / 9, bn = x % 9;
-    const int aa = ac / 3, c = ac % 3, bb = bn / 3, n = bn % 3;
-    const double *F = Fq[q], *Ci = Cq[q];
-    const double ln3 = l3[q];
-    double s = (aa == bb) ? Sq[q][3 * n + c] : 0.0;
-    for (int k = 0; k < 3; ++k) {
-      double in = 0.0;
-      for (int m = 0; m < 3; ++m) {
-        const double dCi = -0.5 * (Ci[3 * k + m] * Ci[3 * n + c] + Ci[3 * k + n] * Ci[3 * m + c]);
-        const double A = 2.0 * (-mu * dCi + 0.5 * lam * (Ci[3 * m + n] * Ci[3 * k + c] + ln3 * dCi));
-        in += A * F[3 * bb + m];
-      }
-      s += F[3 * aa + k] * in;
-    }
-    Dq[q][x % 81] = (Real)(wJ[q] * s);
-  }
+  // A5: the residual
   if (t < kNl) {
     const int i = t / 3, ar = t % 3;
     double re = 0.0;
@@ -154,26 +142,73 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     }
     a.res[(int64_t)e * kNl + t] = re;
   }
-  __syncthreads();
-  // Phase B
-  Real K[kNd];
-  for (int i = 0; i < kNd; ++i) K[i] = Real(0);
+  // Phase B: thread (a, ib, jbb) for t < 243
   const bool owner = t < 3 * kNl;
-  const int ar = owner ? t / kNl : 0, jb = owner ? t % kNl : 0, j = jb / 3, bb = jb % 3;
-  if (owner) {
-    for (int q = 0; q < kNq; ++q) {
-      const Real gj0 = gr[q][j][0], gj1 = gr[q][j][1], gj2 = gr[q][j][2];
-      Real T[3];
-      for (int c = 0; c < 3; ++c) {
-        const Real *d = &Dq[q][9 * (3 * ar + c) + 3 * bb];
-        T[c] = d[0] * gj0 + d[1] * gj1 + d[2] * gj2;
-      }
-      for (int i = 0; i < kNd; ++i) K[i] += gr[q][i][0] * T[0] + gr[q][i][1] * T[1] + gr[q][i][2] * T[2];
+  const int ar = owner ? t / kNl : 0, rr = owner ? t % kNl : 0;
+  const int ib = rr / 9, jbb = rr % 9;  // rows i = 3 ib + r (r < 3), columns jb = 9 jbb + m (m < 9)
+  // the sums stay FP64 in both forms (FP32 accumulators measured slower here)
+  double K[3][9];
+  for (int r = 0; r < 3; ++r)
+    for (int m = 0; m < 9; ++m) K[r][m] = 0.0;
+  for (int q0 = 0; q0 < kNq; q0 += kQc) {
+    // B0a: material tangent A[k c m n] = dS_kc/dE_mn of the chunk's points
+    //   A = 2 (-mu dCi + lambda/2 (Ci_mn Ci_kc + ln(I3) dCi)),
+    //   dCi = dCi_kc/dC_mn = -(Ci_km Ci_nc + Ci_kn Ci_mc) / 2
+    for (int x = t; x < kQc * 81; x += kEbThreads) {
+      const int qq = x / 81, q = q0 + qq, r = x % 81;
+      const int k = r / 27, c = (r / 9) % 3, m = (r / 3) % 3, n = r % 3;
+      const double *Ci = Cq[q];
+      const double dCi = -0.5 * (Ci[3 * k + m] * Ci[3 * n + c] + Ci[3 * k + n] * Ci[3 * m + c]);
+      Ac[qq][r] = 2.0 * (-mu * dCi + 0.5 * lam * (Ci[3 * m + n] * Ci[3 * k + c] + l3[q] * dCi));
     }
+    __syncthreads();
+    // B0b: D_q[(a,c),(b,n)] = wJ_q (delta_ab S_nc + sum_km F_ak A[k c m n] F_bm)
+    for (int x = t; x < kQc * 81; x += kEbThreads) {
+      const int qq = x / 81, q = q0 + qq, ac = (x % 81) / 9, bn = x % 9;
+      const int aa = ac / 3, c = ac % 3, bb = bn / 3, n = bn % 3;
+      const double *F = Fq[q], *A = Ac[qq];
+      double s = (aa == bb) ? Sq[q][3 * n + c] : 0.0;
+      for (int k = 0; k < 3; ++k) {
+        const double *Ak = A + 27 * k + 9 * c + n;  // A[k c m n] at Ak[3 m]
+        s += F[3 * aa + k] * (Ak[0] * F[3 * bb] + Ak[3] * F[3 * bb + 1] + Ak[6] * F[3 * bb + 2]);
+      }
+      Dc[qq][x % 81] = (Real)(wJ[q] * s);
+    }
+    __syncthreads();
+    // B1: T_q[(a,c),(j,b)] = sum_n D_q[(a,c),(b,n)] g[q][j][n]; thread (a, jb)
+    if (owner) {
+      const int jb = rr, j = jb / 3, bb = jb % 3;
+      for (int qq = 0; qq < kQc; ++qq) {
+        const Real *g = gr[q0 + qq][j];
+        const Real g0 = g[0], g1 = g[1], g2 = g[2];
+        for (int c = 0; c < 3; ++c) {
+          const Real *d = &Dc[qq][9 * (3 * ar + c) + 3 * bb];
+          Tc[qq][3 * ar + c][jb] = d[0] * g0 + d[1] * g1 + d[2] * g2;
+        }
+      }
+    }
+    __syncthreads();
+    // B2: K += G^T T over the chunk's (q, c)
+    if (owner) {
+      for (int qq = 0; qq < kQc; ++qq)
+        for (int c = 0; c < 3; ++c) {
+          const double g0 = gr[q0 + qq][3 * ib][c], g1 = gr[q0 + qq][3 * ib + 1][c],
+                       g2 = gr[q0 + qq][3 * ib + 2][c];
+          const Real *tv = &Tc[qq][3 * ar + c][9 * jbb];
+          for (int m = 0; m < 9; ++m) {
+            const double v = tv[m];
+            K[0][m] += g0 * v;
+            K[1][m] += g1 * v;
+            K[2][m] += g2 * v;
+          }
+        }
+    }
+    __syncthreads();
   }
   // threshold 1e-14 max|K| (all-zero element skipped)
   double mx = 0.0;
-  for (int i = 0; i < kNd; ++i) mx = fmax(mx, fabs((double)K[i]));
+  for (int r = 0; r < 3; ++r)
+    for (int m = 0; m < 9; ++m) mx = fmax(mx, fabs((double)K[r][m]));
   red[t] = owner ? mx : 0.0;
   __syncthreads();
   for (int o = kEbThreads / 2; o > 0; o >>= 1) {
@@ -181,9 +216,12 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     __syncthreads();
   }
   if (owner) {
-    double *E = a.E + (int64_t)e * kNl * kNl;
     // streamed to HBM: the buffer (GBs) is read back once by the gather
-    for (int i = 0; i < kNd; ++i) __builtin_nontemporal_store((double)K[i], E + (3 * i + ar) * kNl + jb);
+    double *E = a.E + (int64_t)e * kNl * kNl;
+    for (int r = 0; r < 3; ++r) {
+      double *row = E + (int64_t)(3 * (3 * ib + r) + ar) * kNl + 9 * jbb;
+      for (int m = 0; m < 9; ++m) __builtin_nontemporal_store((double)K[r][m], row + m);
+    }
   }
   if (t == 0) a.thr[e] = red[0] > 0.0 ? 1e-14 * red[0] : INFINITY;
 }
