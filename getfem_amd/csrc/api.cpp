@@ -530,7 +530,10 @@ static int plan_bins(gfx_plan *p, const std::vector<uint16_t> &ns, const std::ve
   const int lanes = (64 / p->nd) * p->nd;
   for (size_t b = 0; b < p->bins.size(); ++b) {
     const int per = std::max(1, bmax[b]) * p->nd;
-    p->bins[b].npw = p->eb ? 1 : std::max(1, std::min(4, 3 * lanes / per));
+    if (p->eb)  // gather workgroups stream ~4 incidences each
+      p->bins[b].npw = std::max(1, std::min({4, 4 / std::max(1, bmax[b]), 64 / std::max(1, p->max_inc)}));
+    else
+      p->bins[b].npw = std::max(1, std::min(4, 3 * lanes / per));
   }
   std::vector<int64_t> cur(p->bins.size());
   for (size_t b = 0; b < p->bins.size(); ++b) cur[b] = p->bins[b].off;

@@ -253,62 +253,91 @@ __device__ __forceinline__ bool mbit(const uint32_t *m, int b) { return (m[b >> 
 // loads, LDS atomics), then the node's CSR rows are written once
 constexpr int kGatherThreads = 256;
 
+// npw (1..4) small patches of one bin share a workgroup: their incidence
+// lists are concatenated and streamed together.
+constexpr int kGatherNpw = 4;
+
 template <int ND, int Q>
 __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, const double *__restrict__ E,
                                                                 const double *__restrict__ thr) {
-  extern __shared__ double acc[];
-  constexpr int kMaxInc = 64;
+  extern __shared__ double acc[];  // npw accumulators of lds_acc doubles
+  constexpr int kMaxInc = 64;      // incidences per workgroup (checked by the launcher)
   __shared__ int64_t sbase[kMaxInc];   // element-matrix run of each incidence
   __shared__ double sthr[kMaxInc];
   __shared__ uint16_t sslot[kMaxInc * ND];
-  const int64_t wg = blockIdx.x;
-  const int64_t pos = (wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3);
-  if (pos >= a.n_nodes) return;
-  const PatchDesc dsc = a.desc[pos];
+  __shared__ int sacc[kMaxInc];        // accumulator offset and slot count of each incidence's patch
+  __shared__ int sns[kMaxInc];
   constexpr int NL = ND * Q, RUN = Q * NL;
+  const int64_t wg = blockIdx.x;
+  const int npw = a.npw;
+  const int64_t pos0 = ((wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3)) * npw;
+  if (pos0 >= a.n_nodes) return;
   const int t = threadIdx.x;
-  const int i0 = dsc.i0, ninc = (int)(dsc.ninc_ns & 0xffffu);
-  const int nsraw = (int)(dsc.ninc_ns >> 16);
-  const int ns = nsraw & 0x7fff;
-  const bool full = (nsraw & 0x8000) != 0;  // all of the node's rows are full
-  if (ninc > kMaxInc) return;  // excluded by the launcher (max_inc check)
-  for (int x = t; x < Q * Q * ns; x += kGatherThreads) acc[x] = 0.0;
-  if (t < ninc) {
-    const uint32_t inc = a.inc[i0 + t];
-    const int64_t e = inc >> 5;
-    sbase[t] = e * (int64_t)NL * NL + (int64_t)(inc & 31) * RUN;
-    sthr[t] = thr[e];
+  PatchDesc dsc[kGatherNpw];
+  int pre[kGatherNpw + 1];
+  pre[0] = 0;
+#pragma unroll
+  for (int k = 0; k < kGatherNpw; ++k) {
+    dsc[k] = PatchDesc{0, 0, 0u};
+    if (k < npw && pos0 + k < a.n_nodes) dsc[k] = a.desc[pos0 + k];
+    pre[k + 1] = pre[k] + (int)(dsc[k].ninc_ns & 0xffffu);
   }
-  for (int x = t; x < ninc * ND; x += kGatherThreads) sslot[x] = a.slot[(int64_t)i0 * ND + x];
+  const int total_inc = pre[kGatherNpw];
+  if (total_inc > kMaxInc) return;  // excluded by the launcher
+#pragma unroll
+  for (int k = 0; k < kGatherNpw; ++k) {
+    if (k >= npw) break;
+    const int ns = (int)(dsc[k].ninc_ns >> 16) & 0x7fff;
+    double *ak = acc + (size_t)k * a.lds_acc;
+    for (int x = t; x < Q * Q * ns; x += kGatherThreads) ak[x] = 0.0;
+    const int ninc = (int)(dsc[k].ninc_ns & 0xffffu);
+    if (t < ninc) {
+      const uint32_t inc = a.inc[dsc[k].i0 + t];
+      const int64_t e = inc >> 5;
+      sbase[pre[k] + t] = e * (int64_t)NL * NL + (int64_t)(inc & 31) * RUN;
+      sthr[pre[k] + t] = thr[e];
+      sacc[pre[k] + t] = k * a.lds_acc;
+      sns[pre[k] + t] = ns;
+    }
+    for (int x = t; x < ninc * ND; x += kGatherThreads)
+      sslot[pre[k] * ND + x] = a.slot[(int64_t)dsc[k].i0 * ND + x];
+  }
   __syncthreads();
-  const int total = ninc * RUN;
+  const int total = total_inc * RUN;
   for (int f = t; f < total; f += kGatherThreads) {
     const int k = f / RUN, o = f - k * RUN;
     const double v = __builtin_nontemporal_load(E + sbase[k] + o);  // read once
     const int x = o / NL, c = o - x * NL, j = c / Q, y = c - j * Q;
-    if (fabs(v) > sthr[k]) atomicAdd(&acc[(x * ns + sslot[k * ND + j]) * Q + y], v);
+    const int ns = sns[k];
+    if (fabs(v) > sthr[k]) atomicAdd(&acc[sacc[k] + (x * ns + sslot[k * ND + j]) * Q + y], v);
   }
   __syncthreads();
   // values are streamed past the caches (written once, never re-read here)
-  if (full) {  // the node's Q full rows are one contiguous run of the CSR
-    double *dst = a.vals + dsc.rowbase;
-    for (int d = t; d < Q * ns * Q; d += kGatherThreads) __builtin_nontemporal_store(acc[d], dst + d);
-    return;
-  }
-  // write: wavefront x handles row x
-  const uint32_t *m = a.mask + (int64_t)i0 * a.mwpi;
   const int w = t / 64, lane = t % 64;
-  if (w < Q) {
-    const int x = w;
-    const int64_t node = a.order[pos];
-    int64_t base = a.indptr[node * Q + x];
-    const int off = x * ns * Q, len = ns * Q;
-    for (int c0 = 0; c0 < len; c0 += 64) {
-      const int d = c0 + lane;
-      const bool pr = d < len && mbit(m, off + d);
-      const uint64_t bal = __ballot(pr);
-      if (pr) __builtin_nontemporal_store(acc[off + d], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
-      base += __popcll(bal);
+#pragma unroll
+  for (int k = 0; k < kGatherNpw; ++k) {
+    if (k >= npw || pos0 + k >= a.n_nodes) break;
+    const int nsraw = (int)(dsc[k].ninc_ns >> 16), ns = nsraw & 0x7fff;
+    const double *ak = acc + (size_t)k * a.lds_acc;
+    if (nsraw & 0x8000) {  // all rows full: the node's Q rows are one contiguous run of the CSR
+      double *dst = a.vals + dsc[k].rowbase;
+      for (int d = t; d < Q * ns * Q; d += kGatherThreads) __builtin_nontemporal_store(ak[d], dst + d);
+      continue;
+    }
+    // wavefront x writes row x from the presence bits
+    const uint32_t *m = a.mask + (int64_t)dsc[k].i0 * a.mwpi;
+    if (w < Q) {
+      const int x = w;
+      const int64_t node = a.order[pos0 + k];
+      int64_t base = a.indptr[node * Q + x];
+      const int off = x * ns * Q, len = ns * Q;
+      for (int c0 = 0; c0 < len; c0 += 64) {
+        const int d = c0 + lane;
+        const bool pr = d < len && mbit(m, off + d);
+        const uint64_t bal = __ballot(pr);
+        if (pr) __builtin_nontemporal_store(ak[off + d], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
+        base += __popcll(bal);
+      }
     }
   }
 }
@@ -355,10 +384,13 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const double *
                               int max_nslots) {
   if (a0.n_nodes == 0) return hipSuccess;
   PatchArgs a = a0;
-  a.xcd_chunk = (int)((a.n_nodes + 7) / 8);
+  if (a.npw < 1 || a.npw > kGatherNpw) return hipErrorInvalidValue;
+  const int64_t groups = (a.n_nodes + a.npw - 1) / a.npw;
+  a.xcd_chunk = (int)((groups + 7) / 8);
   const unsigned grid = (unsigned)(8 * (int64_t)a.xcd_chunk);
-  const size_t lds = (size_t)a.qdim * a.qdim * max_nslots * sizeof(double);
-  if (lds > 48 * 1024 || a.max_inc > 64) return hipErrorInvalidValue;
+  a.lds_acc = a.qdim * a.qdim * max_nslots;
+  const size_t lds = (size_t)a.npw * a.lds_acc * sizeof(double);
+  if (lds > 48 * 1024 || a.npw * a.max_inc > 64) return hipErrorInvalidValue;
   if (a.nd == 27 && a.qdim == 3) k_gather_emat<27, 3><<<grid, kGatherThreads, lds, s>>>(a, E, thr);
   else if (a.nd == 8 && a.qdim == 3) k_gather_emat<8, 3><<<grid, kGatherThreads, lds, s>>>(a, E, thr);
   else return hipErrorInvalidValue;
