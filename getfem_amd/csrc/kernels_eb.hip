@@ -45,7 +45,12 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   Real (*gr)[kNd][3] = kLow ? grl : reinterpret_cast<Real (*)[kNd][3]>(gq);
   __shared__ double Ac[kQc][81];
   __shared__ Real Dc[kQc][81];
-  __shared__ Real Tc[kQc][9][kNl];
+  // T of a chunk and, at the end, a 27-row slab of E.  The FP32 form reads the
+  // gradients from its own copy in phase B, so its buffer reuses gq.
+  __shared__ double Tbuf_d[kLow ? 1 : kQc * 9 * kNl];
+  double *Tbuf = kLow ? &gq[0][0][0] : Tbuf_d;
+  static_assert(sizeof(gq) >= kQc * 9 * kNl * sizeof(double), "the FP32 form's buffer fits in gq");
+  Real (*Tc)[9][kNl] = reinterpret_cast<Real (*)[9][kNl]>(Tbuf);
   __shared__ double red[kEbThreads];
   const int64_t e = blockIdx.x;
   if (e >= a.n_cv || a.nq != kNq) return;
@@ -215,15 +220,25 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     if (t < o) red[t] = fmax(red[t], red[t + o]);
     __syncthreads();
   }
-  if (owner) {
-    // streamed to HBM: the buffer (GBs) is read back once by the gather
-    double *E = a.E + (int64_t)e * kNl * kNl;
-    for (int r = 0; r < 3; ++r) {
-      double *row = E + (int64_t)(3 * (3 * ib + r) + ar) * kNl + 9 * jbb;
-      for (int m = 0; m < 9; ++m) __builtin_nontemporal_store((double)K[r][m], row + m);
-    }
-  }
   if (t == 0) a.thr[e] = red[0] > 0.0 ? 1e-14 * red[0] : INFINITY;
+  // E is streamed to HBM (GBs, read back once by the gather) in three slabs of
+  // 27 contiguous rows (rows 27p..27p+26 = i in 9p..9p+8, owned by ib in
+  // 3p..3p+2), staged through LDS so that each store instruction writes
+  // consecutive addresses across the lanes
+  double *slab = Tbuf;
+  static_assert(kQc * 9 >= 27, "a 27-row slab fits in the T buffer");
+  double *E = a.E + (int64_t)e * kNl * kNl;
+  for (int p = 0; p < 3; ++p) {
+    if (owner && ib / 3 == p)
+      for (int r = 0; r < 3; ++r) {
+        double *row = slab + (3 * (3 * (ib - 3 * p) + r) + ar) * kNl + 9 * jbb;
+        for (int m = 0; m < 9; ++m) row[m] = K[r][m];
+      }
+    __syncthreads();
+    for (int x = t; x < 27 * kNl; x += kEbThreads)
+      __builtin_nontemporal_store(slab[x], E + (int64_t)p * 27 * kNl + x);
+    __syncthreads();
+  }
 }
 
 // keep bits of element matrices held in the buffer: bit r*nl + c
