@@ -342,6 +342,10 @@ static int plan_upload_coef(gfx_plan *p, int slot, const gfx_coef *cf) {
 int gfx_plan_destroy(gfx_plan *p) {
   if (!p) return GFX_E_INVALID;
   (void)hipSetDevice(p->ctx->device);
+  for (hipStream_t st : p->aux) (void)hipStreamSynchronize(st);
+  for (hipStream_t st : p->aux) (void)hipStreamDestroy(st);
+  for (hipEvent_t e : p->join) (void)hipEventDestroy(e);
+  if (p->fork) (void)hipEventDestroy(p->fork);
   dfree(p->d_coef[0]); dfree(p->d_coef[1]);
   dfree(p->d_gref); dfree(p->d_qw); dfree(p->d_R);
   dfree(p->d_rec); dfree(p->d_thr); dfree(p->d_keep);
@@ -660,13 +664,36 @@ int gfx_plan_assemble(gfx_plan *p) {
   a.nslots = p->d_nslots; a.mask = p->d_mask; a.mwpi = p->mwpi; a.indptr = p->d_indptr;
   a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
   a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals; a.max_inc = p->max_inc;
-  for (const gfx_plan::Bin &b : p->bins) {  // one launch per patch-size bin
+  // The bins write disjoint rows: they run concurrently on up to three
+  // auxiliary streams forked from and joined back into the context's stream
+  // (events only, no host synchronisation), which fills the tail of one
+  // launch with the next.
+  const int nb = (int)p->bins.size();
+  const int naux = std::min(nb - 1, 3);
+  while ((int)p->aux.size() < naux) {
+    hipStream_t st; hipEvent_t ej;
+    HIPCHK(c, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+    HIPCHK(c, hipEventCreateWithFlags(&ej, hipEventDisableTiming));
+    p->aux.push_back(st); p->join.push_back(ej);
+  }
+  if (naux > 0) {
+    if (!p->fork) HIPCHK(c, hipEventCreateWithFlags(&p->fork, hipEventDisableTiming));
+    HIPCHK(c, hipEventRecord(p->fork, c->stream));
+    for (int i = 0; i < naux; ++i) HIPCHK(c, hipStreamWaitEvent(p->aux[i], p->fork, 0));
+  }
+  for (int k = 0; k < nb; ++k) {  // one launch per patch-size bin, spread over streams
+    const gfx_plan::Bin &b = p->bins[k];
+    hipStream_t st = k == 0 ? c->stream : p->aux[(k - 1) % naux];
     a.n_nodes = b.count;
     a.order = p->d_order + b.off;
     a.desc = p->d_desc + b.off;
     a.npw = b.npw;
-    if (p->eb) HIPCHK(c, launch_gather_emat(c->stream, a, p->d_E, p->d_thr, b.cap));
-    else HIPCHK(c, launch_assemble_patch(c->stream, p->term, p->form, p->precision, a, b.cap));
+    if (p->eb) HIPCHK(c, launch_gather_emat(st, a, p->d_E, p->d_thr, b.cap));
+    else HIPCHK(c, launch_assemble_patch(st, p->term, p->form, p->precision, a, b.cap));
+  }
+  for (int i = 0; i < naux; ++i) {
+    HIPCHK(c, hipEventRecord(p->join[i], p->aux[i]));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, p->join[i], 0));
   }
   if (p->eb)
     HIPCHK(c, launch_gather_vec(c->stream, p->n_nodes, p->nd, p->qdim, p->d_inc_ptr, p->d_inc, p->d_res,

@@ -223,6 +223,9 @@ struct gfx_plan {
   double *d_cg = nullptr;  // conjugate gradient work vectors (allocated on first solve)
   uint8_t *d_fixed = nullptr;  // dof constraints: mask and prescribed values
   double *d_go = nullptr;
+  std::vector<hipStream_t> aux;  // streams the patch-size bins run on beside ctx->stream
+  std::vector<hipEvent_t> join;
+  hipEvent_t fork = nullptr;
   bool timing = false;
   std::vector<hipEvent_t> ev;  // 3 events per timed call
   int n_timed = 0;
