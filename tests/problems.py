@@ -59,3 +59,12 @@ def c5(n=32):
 
 def rel_frob(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def golden(name):
+    """Known answers recorded from the reference (tests/golden/reference_counts.json)."""
+    import json
+    import os
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_counts.json")
+    with open(path) as f:
+        return json.load(f)[name]

@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import getfem_amd as ga
+from getfem_amd import meshes
 from tests import problems
 
 pytestmark = pytest.mark.gpu
@@ -90,3 +91,32 @@ def test_repeated_assembly_is_stable(ctx):
     plan.assemble()
     b = plan.csr()[2]
     assert problems.rel_frob(a, b) <= 1e-14
+
+
+@pytest.mark.parametrize("name,n,fem,im,nd", [
+    ("p2_elasticity_64cube_tet5", 64, "FEM_PK(3,2)", "IM_TETRAHEDRON(5)", 10),
+    ("p3_elasticity_32cube_tet8", 32, "FEM_PK(3,3)", "IM_TETRAHEDRON(8)", 20),
+])
+def test_full_size_stored_nnz_equals_reference(ctx, name, n, fem, im, nd):
+    """At full size the stored pattern has exactly the entry count GetFEM
+    itself produced on the same inputs, and the columns of every row are
+    strictly increasing."""
+    import torch
+    kat = problems.golden(name)
+    pts, cvx = meshes.regular_unit_mesh(n)
+    pts = meshes.jitter(pts, n)
+    lam, mu = meshes.lognormal_lame(cvx.shape[0])
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    mf = ga.MeshFem(mesh, fem, 3)
+    mim = ga.MeshIm(mesh, im)
+    plan = ga.Plan(mf, mim, ga.TERM_ISO_ELAST, [lam, mu])
+    nnz = plan.build_pattern()
+    plan.assemble()
+    print(name, "nnz", nnz, "ndof", mf.nbdof(), flush=True)
+    assert nnz == kat["stored_nnz"] and mf.nbdof() == kat["ndof"]
+    ip, ix, v = plan.torch_csr()
+    d = ix[1:].to(torch.int64) - ix[:-1].to(torch.int64)
+    row_start = torch.zeros(ix.numel(), dtype=torch.bool, device=ix.device)
+    row_start[ip[:-1][ip[:-1] < ix.numel()]] = True
+    assert bool(((d > 0) | row_start[1:]).all())  # strictly increasing inside each row
+    assert bool(torch.isfinite(v).all())

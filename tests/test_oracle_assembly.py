@@ -194,23 +194,25 @@ def test_global_assembly_bruteforce(mk):
 def test_reference_nnz_p1_poisson_20():
     """GetFEM's stored nnz for P1 Poisson on the jittered 20^3x6 mesh: 128,113
     of 128,581 topological (survey, measured with the reference)."""
+    kat = problems.golden("p1_poisson_20cube_tet1")
     case = problems.c1(20)
     colptr, rows, vals = case.oracle_problem().assemble_csc()
-    assert len(vals) == 128113
+    assert len(vals) == kat["stored_nnz"] == 128113
     topo = set()
     for e in range(case.cvx.shape[0]):
         for i in case.dofs[e]:
             for j in case.dofs[e]:
                 topo.add((int(i), int(j)))
-    assert len(topo) == 128581
+    assert len(topo) == kat["topological_nnz"]
 
 
 def test_reference_nnz_p2_elasticity_8():
     """P2 elasticity on 8^3x6: 44,760 of 1,141,065 topological entries dropped
     (survey, measured with the reference)."""
+    kat = problems.golden("p2_elasticity_8cube_tet5")
     case = problems.c2(8)
     colptr, rows, vals = case.oracle_problem().assemble_csc()
-    assert len(vals) == 1141065 - 44760
+    assert len(vals) == kat["topological_nnz"] - kat["dropped_by_threshold"]
 
 
 def test_threaded_assembly_equals_serial():
