@@ -1,6 +1,7 @@
 """Python objects over the libgfx C ABI, named after their GetFEM counterparts
 (getfem.Mesh, getfem.MeshFem, getfem.MeshIm) plus the assembly Plan."""
 import ctypes
+import sys
 
 import numpy as np
 
@@ -33,11 +34,15 @@ class Context:
         check(lib().gfx_ctx_synchronize(self.h), self.h)
 
     def close(self):
+        """Release the context; objects created from it stay valid (the
+        library frees the context after the last of them)."""
         if self.h:
             lib().gfx_ctx_destroy(self.h)
             self.h = None
 
     def __del__(self):
+        if sys.is_finalizing():
+            return
         try:
             self.close()
         except Exception:
@@ -67,9 +72,9 @@ class Mesh:
         self.pts = pts
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not sys.is_finalizing():
             lib().gfx_mesh_destroy(self.h)
-            self.h = None
+        self.h = None
 
 
 class MeshFem:
@@ -106,9 +111,9 @@ class MeshFem:
         return out
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not sys.is_finalizing():
             lib().gfx_mesh_fem_destroy(self.h)
-            self.h = None
+        self.h = None
 
 
 class MeshIm:
@@ -129,9 +134,9 @@ class MeshIm:
         return p, w
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not sys.is_finalizing():
             lib().gfx_mesh_im_destroy(self.h)
-            self.h = None
+        self.h = None
 
 
 def _coef(v, n_cv):
@@ -279,9 +284,9 @@ class Plan:
         return g.value, a.value
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not sys.is_finalizing():
             lib().gfx_plan_destroy(self.h)
-            self.h = None
+        self.h = None
 
 
 # ---------------------------------------------------------------- host-only

@@ -51,7 +51,50 @@ int set_device(gfx_ctx *c) {
   return GFX_OK;
 }
 
+void release(gfx_ctx *c) {
+  (void)hipSetDevice(c->device);
+  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+void release(gfx_mesh *m) {
+  gfx_ctx *c = m->ctx;
+  (void)hipSetDevice(c->device);
+  dfree(m->d_pts); dfree(m->d_cvx);
+  delete m;
+  unref(c);
+}
+
+void release(gfx_mesh_fem *mf) {
+  gfx_mesh *m = mf->mesh;
+  (void)hipSetDevice(m->ctx->device);
+  dfree(mf->d_dofs);
+  delete mf;
+  unref(m);
+}
+
+void release(gfx_mesh_im *mim) {
+  gfx_mesh *m = mim->mesh;
+  delete mim;
+  unref(m);
+}
+
+template <typename T>
+void close_or_mark(T *o) {
+  o->closing = true;
+  if (o->refs == 0) release(o);
+}
+
 }  // namespace
+
+void gfx::ref(gfx_ctx *c) { ++c->refs; }
+void gfx::unref(gfx_ctx *c) { if (--c->refs == 0 && c->closing) release(c); }
+void gfx::ref(gfx_mesh *m) { ++m->refs; }
+void gfx::unref(gfx_mesh *m) { if (--m->refs == 0 && m->closing) release(m); }
+void gfx::ref(gfx_mesh_fem *mf) { ++mf->refs; }
+void gfx::unref(gfx_mesh_fem *mf) { if (--mf->refs == 0 && mf->closing) release(mf); }
+void gfx::ref(gfx_mesh_im *mim) { ++mim->refs; }
+void gfx::unref(gfx_mesh_im *mim) { if (--mim->refs == 0 && mim->closing) release(mim); }
 
 extern "C" {
 
@@ -116,10 +159,8 @@ int gfx_ctx_create(int device, gfx_ctx **out) {
 }
 
 int gfx_ctx_destroy(gfx_ctx *c) {
-  if (!c) return GFX_E_INVALID;
-  (void)hipSetDevice(c->device);
-  if (c->own_stream && c->stream) (void)hipStreamDestroy(c->stream);
-  delete c;
+  if (!c || c->closing) return GFX_E_INVALID;
+  close_or_mark(c);
   return GFX_OK;
 }
 
@@ -174,6 +215,7 @@ int gfx_mesh_create(gfx_ctx *c, const char *geotrans, int64_t n_pts, const doubl
     dfree(m->d_pts); dfree(m->d_cvx); delete m;
     return hipfail(c, e, "gfx_mesh_create upload");
   }
+  ref(c);
   *out = m;
   return GFX_OK;
 }
@@ -189,10 +231,8 @@ int gfx_mesh_set_points(gfx_mesh *m, const double *h_pts) {
 }
 
 int gfx_mesh_destroy(gfx_mesh *m) {
-  if (!m) return GFX_E_INVALID;
-  (void)hipSetDevice(m->ctx->device);
-  dfree(m->d_pts); dfree(m->d_cvx);
-  delete m;
+  if (!m || m->closing) return GFX_E_INVALID;
+  close_or_mark(m);
   return GFX_OK;
 }
 
@@ -206,6 +246,7 @@ static int mf_finish(gfx_mesh *m, gfx_mesh_fem *mf, gfx_mesh_fem **out) {
     dfree(mf->d_dofs); delete mf;
     return hipfail(c, e, "gfx_mesh_fem upload");
   }
+  ref(m);
   *out = mf;
   return GFX_OK;
 }
@@ -281,10 +322,8 @@ int gfx_mesh_fem_dof_nodes(gfx_mesh_fem *mf, double *h_out) {
 }
 
 int gfx_mesh_fem_destroy(gfx_mesh_fem *mf) {
-  if (!mf) return GFX_E_INVALID;
-  (void)hipSetDevice(mf->mesh->ctx->device);
-  dfree(mf->d_dofs);
-  delete mf;
+  if (!mf || mf->closing) return GFX_E_INVALID;
+  close_or_mark(mf);
   return GFX_OK;
 }
 
@@ -303,6 +342,7 @@ int gfx_mesh_im_create(gfx_mesh *m, const char *im_name, gfx_mesh_im **out) {
     delete mim;
     return fail(m->ctx, GFX_E_INVALID, "integration method does not match the mesh element");
   }
+  ref(m);
   *out = mim;
   return GFX_OK;
 }
@@ -317,8 +357,8 @@ int gfx_mesh_im_points(gfx_mesh_im *mim, double *h_pts, double *h_w) {
 }
 
 int gfx_mesh_im_destroy(gfx_mesh_im *mim) {
-  if (!mim) return GFX_E_INVALID;
-  delete mim;
+  if (!mim || mim->closing) return GFX_E_INVALID;
+  close_or_mark(mim);
   return GFX_OK;
 }
 
@@ -355,13 +395,17 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_gN); dfree(p->d_E); dfree(p->d_res); dfree(p->d_U); dfree(p->d_rhs); dfree(p->d_cg);
   dfree(p->d_fixed); dfree(p->d_go);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
+  gfx_ctx *c = p->ctx;
+  gfx_mesh_fem *mf = p->mf;
+  gfx_mesh_im *mim = p->mim;
   delete p;
+  unref(mim); unref(mf); unref(c);
   return GFX_OK;
 }
 
 int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef *coefs,
                     int n_coefs, gfx_plan **out) {
-  if (!mf || !mim || !out) return GFX_E_INVALID;
+  if (!mf || !mim || !out || mf->closing || mim->closing) return GFX_E_INVALID;
   *out = nullptr;
   gfx_ctx *c = mf->mesh->ctx;
   if (mim->mesh != mf->mesh) return fail(c, GFX_E_INVALID, "mesh_fem and mesh_im on different meshes");
@@ -385,6 +429,7 @@ int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef
   gfx_plan *p = new (std::nothrow) gfx_plan();
   if (!p) return fail(c, GFX_E_OOM, "host allocation");
   p->ctx = c; p->mf = mf; p->mim = mim; p->term = term;
+  ref(c); ref(mf); ref(mim);  // released by gfx_plan_destroy (also on the failure paths below)
   p->nd = nd; p->qdim = Q; p->nloc = nd * Q; p->nq = mim->rule.n(); p->nv = m->nv;
   p->kwords = (p->nloc * p->nloc + 31) / 32;
   p->mwpi = (nd * Q * Q + 31) / 32;
@@ -567,8 +612,20 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   dfree(p->d_nbr); dfree(p->d_mask); dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals);
   dfree(p->d_keep);
   p->nnz = -1;
+  p->bins.clear();
   p->n_inc = n_cv * nd;
   if (p->n_inc >= INT32_MAX) return fail(c, GFX_E_INVALID, "too many element-node incidences");
+  if (n_cv == 0 || nn == 0) {  // no element: an empty matrix (all rows empty)
+    HIPCHK(c, dalloc(&p->d_indptr, (size_t)p->ndof + 1));
+    HIPCHK(c, hipMemsetAsync(p->d_indptr, 0, 8 * ((size_t)p->ndof + 1), s));
+    HIPCHK(c, dalloc(&p->d_indices, 1));
+    HIPCHK(c, dalloc(&p->d_vals, 1));
+    if (p->eb && p->ndof) HIPCHK(c, hipMemsetAsync(p->d_rhs, 0, 8 * (size_t)p->ndof, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    p->nnz = 0;
+    if (nnz_out) *nnz_out = 0;
+    return GFX_OK;
+  }
   int32_t *tmp32 = nullptr;
   HIPCHK(c, dalloc(&p->d_inc_ptr, (size_t)nn + 1));
   HIPCHK(c, dalloc(&p->d_inc, (size_t)p->n_inc));
@@ -911,7 +968,10 @@ int gfx_compliance_destroy(gfx_compliance *c) {
   (void)hipSetDevice(c->ctx->device);
   dfree(c->d_vinc_ptr); dfree(c->d_dinc_ptr); dfree(c->d_vinc); dfree(c->d_dinc);
   dfree(c->d_cX); dfree(c->d_cU); dfree(c->d_Jpart); dfree(c->d_err);
+  gfx_ctx *ctx = c->ctx;
+  gfx_mesh_fem *mf = c->mf;
   delete c;
+  unref(mf); unref(ctx);
   return GFX_OK;
 }
 
@@ -929,6 +989,7 @@ int gfx_compliance_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, double lambda, dou
   gfx_compliance *c = new (std::nothrow) gfx_compliance();
   if (!c) return fail(ctx, GFX_E_OOM, "host allocation");
   c->ctx = ctx; c->mf = mf; c->lam = lambda; c->mu = mu; c->emin = emin; c->penal = penal;
+  ref(ctx); ref(mf);  // released by gfx_compliance_destroy
   c->n_cv = m->n_cv; c->n_pts = m->n_pts; c->ndof = mf->ndof;
   c->wsum = 0.0;
   for (double w : mim->rule.w) c->wsum += w;  // P1: the integrand is constant per element

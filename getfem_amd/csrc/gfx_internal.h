@@ -10,14 +10,23 @@
 #include "../../include/gfx.h"
 #include "fem_ref.h"
 
-struct gfx_ctx {
+// Lifetimes: every object counts the objects that point at it (a mesh its
+// mesh_fems/mesh_ims, a context its meshes, plans, workspaces).  Destroying
+// an object that is still referenced only marks it; it is released when the
+// last referring object goes, so any destruction order is safe.
+struct gfx_refcount {
+  int refs = 0;
+  bool closing = false;
+};
+
+struct gfx_ctx : gfx_refcount {
   int device = 0;
   hipStream_t stream = nullptr;
   bool own_stream = false;
   std::string err;
 };
 
-struct gfx_mesh {
+struct gfx_mesh : gfx_refcount {
   gfx_ctx *ctx = nullptr;
   int gt = 0;
   int nv = 4;
@@ -28,7 +37,7 @@ struct gfx_mesh {
   std::vector<int32_t> h_cvx;
 };
 
-struct gfx_mesh_fem {
+struct gfx_mesh_fem : gfx_refcount {
   gfx_mesh *mesh = nullptr;
   gfx::FemDesc fem;
   int qdim = 1;
@@ -37,7 +46,7 @@ struct gfx_mesh_fem {
   int32_t *d_dofs = nullptr;
 };
 
-struct gfx_mesh_im {
+struct gfx_mesh_im : gfx_refcount {
   gfx_mesh *mesh = nullptr;
   gfx::IntegrationRule rule;
 };
@@ -121,6 +130,16 @@ struct EbArgs {
   double *thr;              // n_cv thresholds
   int *err;
 };
+
+// reference counting of the objects above (api.cpp)
+void ref(gfx_ctx *c);
+void unref(gfx_ctx *c);
+void ref(gfx_mesh *m);
+void unref(gfx_mesh *m);
+void ref(gfx_mesh_fem *mf);
+void unref(gfx_mesh_fem *mf);
+void ref(gfx_mesh_im *mim);
+void unref(gfx_mesh_im *mim);
 
 // kernel launch wrappers (kernels.hip)
 hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,

@@ -143,6 +143,7 @@ int gfx_ws_create(gfx_ctx *ctx, gfx_workspace **out) {
   gfx_workspace *w = new (std::nothrow) gfx_workspace();
   if (!w) return GFX_E_OOM;
   w->ctx = ctx;
+  ref(ctx);
   *out = w;
   return GFX_OK;
 }
@@ -153,7 +154,11 @@ int gfx_ws_destroy(gfx_workspace *w) {
   (void)hipSetDevice(w->ctx->device);
   if (w->d_u) (void)hipFree(w->d_u);
   if (w->d_V) (void)hipFree(w->d_V);
+  for (auto &kv : w->vars)
+    if (kv.second.mf) unref(kv.second.mf);
+  gfx_ctx *c = w->ctx;
   delete w;
+  unref(c);
   return GFX_OK;
 }
 
@@ -168,6 +173,9 @@ static int add_var(gfx_workspace *w, const char *name, int kind, gfx_mesh_fem *m
   if (size < 0) return wfail(w, GFX_E_INVALID, "negative size");
   v.V.assign((size_t)size, 0.0);
   if (V) std::memcpy(v.V.data(), V, 8 * (size_t)size);
+  if (mf) ref(mf);  // the workspace keeps the mesh_fem alive while it names it
+  auto old = w->vars.find(name);
+  if (old != w->vars.end() && old->second.mf) unref(old->second.mf);
   w->vars[name] = v;
   w->dirty = true;
   return GFX_OK;

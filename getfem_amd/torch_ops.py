@@ -10,6 +10,7 @@ pulled-back coordinates as variables (src/getfem_generic_assembly_workspace.cc:
 520-541, :832-847).
 """
 import ctypes
+import sys
 
 import torch
 
@@ -46,9 +47,9 @@ class Compliance:
         return J, dr, dx, du
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not sys.is_finalizing():
             lib().gfx_compliance_destroy(self.h)
-            self.h = None
+        self.h = None
 
 
 class ComplianceFunction(torch.autograd.Function):

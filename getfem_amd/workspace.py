@@ -9,6 +9,7 @@ scipy CSC matrices, like the interface's gsparse export
 (interface/src/getfemint.cc:712-737), order-1 results as numpy vectors.
 """
 import ctypes
+import sys
 
 import numpy as np
 
@@ -116,9 +117,9 @@ class GaWorkspace:
         return sp.csc_matrix((v, r, cp), shape=(n.value, n.value))
 
     def __del__(self):
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and not sys.is_finalizing():
             lib().gfx_ws_destroy(self.h)
-            self.h = None
+        self.h = None
 
 
 def asm_generic(mim, order, expression, region, *args, ctx=None):
