@@ -235,18 +235,21 @@ __device__ void block_bitonic_sort(uint64_t *k, int P) {
     }
 }
 
-__global__ __launch_bounds__(256) void k_patch_build(PatternArgs a) {
-  __shared__ uint64_t keys[kMaxCand];
-  __shared__ uint16_t slotL[kMaxCand];
+// Dynamic LDS, sized by the launcher for the largest candidate list pmax
+// (a power of two): [keys: pmax u64][slots: pmax u16][presence: mwords u32].
+__global__ __launch_bounds__(256) void k_patch_build(PatternArgs a, int pmax) {
+  extern __shared__ uint64_t lds_pb[];
+  uint64_t *keys = lds_pb;
+  uint16_t *slotL = reinterpret_cast<uint16_t *>(keys + pmax);
+  uint32_t *maskL = reinterpret_cast<uint32_t *>(slotL + pmax + (pmax & 1));
   __shared__ int32_t part[256];
-  __shared__ uint32_t maskL[kMaxCand * 9 / 32 + 32];
   __shared__ int32_t rowc[3];
   const int64_t node = blockIdx.x;
   if (node >= a.n_nodes) return;
   const int nd = a.nd, Q = a.qdim, nl = a.nloc;
   const int i0 = a.inc_ptr[node], ninc = a.inc_ptr[node + 1] - i0;
   const int ncand = ninc * nd;
-  if (ncand > kMaxCand) { if (threadIdx.x == 0) atomicOr(a.err, 2); return; }
+  if (ncand > pmax) { if (threadIdx.x == 0) atomicOr(a.err, 2); return; }
   int P = 1;
   while (P < ncand) P <<= 1;
   for (int t = threadIdx.x; t < P; t += blockDim.x) {
@@ -643,7 +646,15 @@ hipError_t launch_keep(hipStream_t s, int term, const PatternArgs &a) {
 hipError_t launch_patch_build(hipStream_t s, const PatternArgs &a, int max_cand) {
   if (max_cand > kMaxCand) return hipErrorInvalidValue;
   if (a.n_nodes == 0) return hipSuccess;
-  k_patch_build<<<(unsigned)a.n_nodes, 256, 0, s>>>(a);
+  int pmax = 1;
+  while (pmax < max_cand) pmax <<= 1;
+  pmax = pmax < 64 ? 64 : pmax;
+  // presence bits: Q*Q*ns <= Q*Q*pmax
+  const size_t mwords = ((size_t)a.qdim * a.qdim * pmax + 31) / 32 + 1;
+  const size_t lds = 8 * (size_t)pmax + 2 * (size_t)(pmax + (pmax & 1)) + 4 * mwords;
+  // small lists: smaller workgroups, so more nodes run at once
+  const int threads = pmax >= 1024 ? 256 : (pmax >= 256 ? 128 : 64);
+  k_patch_build<<<(unsigned)a.n_nodes, threads, lds, s>>>(a, pmax);
   return hipGetLastError();
 }
 
