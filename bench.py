@@ -242,6 +242,83 @@ def run_gpu(args):
     return out, (pts, cvx, lam, mu, fem, q, im, term, state)
 
 
+C4_NU = 0.3  # Poisson ratio of the SIMP problem (lambda, mu of E = 1)
+
+
+def run_c4(args):
+    """SIMP compliance J = u^T K(rho) u on P1 tets (32^3x6) with its three
+    gradients (dJ/drho, dJ/dX, dJ/du) in one device call per step."""
+    import torch
+
+    import getfem_amd as ga
+    from getfem_amd import meshes
+    from getfem_amd.torch_ops import Compliance
+
+    n = args.n or 32
+    lam = C4_NU / ((1 + C4_NU) * (1 - 2 * C4_NU))
+    mu = 1 / (2 * (1 + C4_NU))
+    pts, cvx = meshes.regular_unit_mesh(n)
+    pts = meshes.jitter(pts, n)
+    dev = torch.device("cuda:0")
+    ctx = ga.Context(0)
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    mf = ga.MeshFem(mesh, "FEM_PK(3,1)", 3)
+    mim = ga.MeshIm(mesh, "IM_TETRAHEDRON(1)")
+    op = Compliance(mf, mim, lam, mu, 1e-3, 3.0)
+    rho = torch.tensor(meshes.simp_density(cvx.shape[0]), device=dev)
+    u = torch.tensor(meshes.random_state(mf.nbdof()), device=dev)
+    X = torch.tensor(pts, device=dev)
+    n_cv = cvx.shape[0]
+    for _ in range(args.warmup):
+        op.eval(rho, X, u)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(args.steps):
+        J, dr, dx, du = op.eval(rho, X, u)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / args.steps
+    b_elem = 416  # 4*4 + 24*4 + 8 + 8*12 + 8 + 16*12 bytes per element (gradient pass)
+    out = {
+        "metric": METRIC,
+        "value": n_cv / (ms * 1e-3),
+        "unit": "elems/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded jittered mesh, rho ~ U(0.1,1), u ~ N(0,1))",
+        "config": {"workload": "P1-tet SIMP compliance J = u^T K(rho) u with dJ/drho, dJ/dX, dJ/du, "
+                               "%d^3x6 = %d tets, E = 1e-3 + rho^3 (1 - 1e-3), nu = 0.3; step = one device "
+                               "call (element kernel + two node gathers)" % (n, n_cv),
+                   "name": "c4", "elements": n_cv},
+        "J": float(J),
+        "roofline": {"bound": "hbm", "achieved": b_elem * n_cv / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": b_elem * n_cv / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "k_compliance_elem + k_gather3 x2", "bytes_per_element": b_elem,
+                     "kernel_ms": ms,
+                     "note": "82 MB per step: cache-resident on MI355X (256 MB last-level cache); the step "
+                             "is launch/latency bound"},
+    }
+    return out, (pts, cvx, mf.basic_dof_table(), mf.nbdof(), lam, mu, rho.cpu().numpy(), u.cpu().numpy())
+
+
+def cpu_baseline_c4(inputs):
+    """The oracle's compliance with its analytic gradients, repeated over the
+    whole mesh for about 5 s (one thread)."""
+    import oracle
+    pts, cvx, dofs, ndof, lam, mu, rho, u = inputs
+    prob = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 1, 3, dofs, ndof, "IM_TETRAHEDRON(1)",
+                          oracle.TERM_ISO_ELAST, scal_a=lam, scal_b=mu, U=u)
+    t0 = time.perf_counter()
+    reps = 0
+    while time.perf_counter() - t0 < 5.0:
+        prob.compliance(rho, 1e-3, 3.0, grads=True)
+        reps += 1
+    dt = time.perf_counter() - t0
+    return {"value": reps * cvx.shape[0] / dt, "unit": "elems/s", "cores": 1, "kind": "port",
+            "sample": "%d full evaluations (J, dJ/drho, dJ/dX) of the %d-element mesh, %.1f s" % (reps, cvx.shape[0], dt)}
+
+
 def cpu_baseline(inputs, threads):
     """The oracle (a C port of GetFEM's per-element/per-point FP64 algorithm
     with its threshold and sorted-column insertion) on a bounded sample."""
@@ -279,7 +356,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS))
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4"])
     ap.add_argument("--n", type=int, default=0, help="mesh subdivisions (default: the config's)")
     ap.add_argument("--form", default="all")
     ap.add_argument("--no-cpu", action="store_true")
@@ -300,6 +377,18 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c4":
+        out, inputs = run_c4(args)
+        if not args.no_cpu:
+            try:
+                import oracle
+                if not os.path.exists(oracle.LIB_PATH):
+                    oracle.build()
+                out["cpu_baseline"] = cpu_baseline_c4(inputs)
+            except Exception as exc:  # report, never hide
+                out["cpu_baseline"] = {"value": None, "error": repr(exc)}
+        print(json.dumps(out), flush=True)
+        return
     out, inputs = run_gpu(args)
     if not args.no_cpu:
         try:
