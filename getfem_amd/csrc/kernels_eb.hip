@@ -33,7 +33,182 @@ constexpr int kNd = 27, kNl = 81, kNv = 8;
 // B2 is a register-blocked product: thread (a, i-block of 3, jb-block of 9)
 // keeps 27 sums, and each (q, c) step reads 3 + 9 values for 27 FMAs.
 constexpr int kNq = 27;
-constexpr int kQc = 3;  // points per phase-B chunk
+constexpr int kQc = 3;  // points per phase-B chunk (FP64 form)
+
+typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
+typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
+
+// x = hi + mid + lo with three bf16 numbers (24 of FP32's bits; the rest is
+// the rounding of lo)
+__device__ __forceinline__ void split3(float x, __bf16 &hi, __bf16 &mid, __bf16 &lo) {
+  hi = (__bf16)x;
+  const float r1 = x - (float)hi;
+  mid = (__bf16)r1;
+  lo = (__bf16)(r1 - (float)mid);
+}
+
+// acc += A B in about FP32 accuracy from bf16 MFMAs: the six products of the
+// three-way splits whose orders of magnitude reach 2^-24 of the leading one.
+__device__ __forceinline__ f32x4 mfma_bf16x3(const bf16x8 &ah, const bf16x8 &am, const bf16x8 &al,
+                                             const bf16x8 &bh, const bf16x8 &bm, const bf16x8 &bl,
+                                             f32x4 acc) {
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
+  return acc;
+}
+
+// Phase B of the FP32 form on the matrix cores.  For each a:
+//   K_a[i][(j,b)] = sum_k G[i][k] T_a[k][(j,b)],  k = (q, c),
+//   G[i][k] = g[q][i][c],  T_a[k][(j,b)] = sum_n D_q[(a,c),(b,n)] g[q][j][n],
+// as 16x16 output tiles (i: 2 tiles of 27 rows, (j,b): 6 tiles of 81
+// columns, 3 values of a: 36 tiles; wavefront w owns the (a, column tile)
+// pairs w, w+4, ...).  The k axis is k = 4 q + c with c = 3 a zero column,
+// so a 32-wide k-step covers 8 points and a lane's 8 k-values are 2 points
+// x 4 components.  G and T are split into bf16 triples in registers; T is
+// computed straight into the B fragments (no LDS round trip).  MFMA
+// 16x16x32 bf16 lane layout: A[i][k]: lane i + 16 (k / 8), element k % 8;
+// B[k][j]: lane j + 16 (k / 8); D[i][j]: lane j + 16 (i / 4), element i % 4.
+constexpr int kQm = 8;  // points per k-step
+
+__device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, double lam, double mu,
+                                                const double (*Fq)[9], const double (*Sq)[9],
+                                                const double (*Cq)[9], const double *l3, const double *wJ,
+                                                const float4 (*g4)[kNd], double (*Ac)[81], float4 (*Dc)[27],
+                                                double *slab, double *red, const EbArgs &a) {
+  const int w = t / 64, l = t % 64;
+  const int lr = l % 16, lq = 2 * (l / 16);  // the lane's first point within a k-step
+  f32x4 acc[5][2];
+#pragma unroll
+  for (int r = 0; r < 5; ++r)
+    for (int m = 0; m < 2; ++m) acc[r][m] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int q0 = 0; q0 < kNq; q0 += kQm) {
+    const int nq = min(kQm, kNq - q0);
+    // B0a: material tangent A[k c m n] of the step's points
+    for (int x = t; x < nq * 81; x += kEbThreads) {
+      const int qq = x / 81, q = q0 + qq, r = x % 81;
+      const int k = r / 27, c = (r / 9) % 3, m = (r / 3) % 3, n = r % 3;
+      const double *Ci = Cq[q];
+      const double dCi = -0.5 * (Ci[3 * k + m] * Ci[3 * n + c] + Ci[3 * k + n] * Ci[3 * m + c]);
+      Ac[qq][r] = 2.0 * (-mu * dCi + 0.5 * lam * (Ci[3 * m + n] * Ci[3 * k + c] + l3[q] * dCi));
+    }
+    __syncthreads();
+    // B0b: D_q[(a,c),(b,n)] = wJ_q (delta_ab S_nc + sum_km F_ak A[k c m n] F_bm), stored as
+    // float4 over n: Dc[qq][(a c) * 3 + b]
+    for (int x = t; x < nq * 27; x += kEbThreads) {
+      const int qq = x / 27, q = q0 + qq, acb = x % 27, ac = acb / 3, bb = acb % 3;
+      const int aa = ac / 3, c = ac % 3;
+      const double *F = Fq[q], *A = Ac[qq];
+      double d[3];
+      for (int n = 0; n < 3; ++n) {
+        double sv = (aa == bb) ? Sq[q][3 * n + c] : 0.0;
+        for (int k = 0; k < 3; ++k) {
+          const double *Ak = A + 27 * k + 9 * c + n;
+          sv += F[3 * aa + k] * (Ak[0] * F[3 * bb] + Ak[3] * F[3 * bb + 1] + Ak[6] * F[3 * bb + 2]);
+        }
+        d[n] = wJ[q] * sv;
+      }
+      Dc[qq][acb] = make_float4((float)d[0], (float)d[1], (float)d[2], 0.f);
+    }
+    __syncthreads();
+    // the lane's two points (zero beyond the last point)
+    const int qa = q0 + lq, qb = q0 + lq + 1;
+    const bool va = qa < kNq, vb = qb < kNq;
+    // A fragments of the two row tiles: element x = 4 (point) + c
+    bf16x8 ah[2], am[2], al[2];
+#pragma unroll
+    for (int mt = 0; mt < 2; ++mt) {
+      const int i = 16 * mt + lr;
+      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+      const float4 ga = (va && i < kNd) ? g4[qa][i] : z, gb = (vb && i < kNd) ? g4[qb][i] : z;
+      const float v[8] = {ga.x, ga.y, ga.z, 0.f, gb.x, gb.y, gb.z, 0.f};
+#pragma unroll
+      for (int x = 0; x < 8; ++x) {
+        __bf16 h, m_, lo;
+        split3(v[x], h, m_, lo);
+        ah[mt][x] = h; am[mt][x] = m_; al[mt][x] = lo;
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+      const int p = w + 4 * r;  // (a, column tile) pair, uniform across the wavefront
+      if (p < 18) {
+        const int aa = p / 6, jb = 16 * (p % 6) + lr, j = jb / 3, bb = jb % 3;
+        const bool vj = jb < kNl;
+        float v[8];
+#pragma unroll
+        for (int h2 = 0; h2 < 2; ++h2) {
+          const int q = h2 ? qb : qa;
+          const bool ok = (h2 ? vb : va) && vj;
+          const float4 gj = ok ? g4[q][j] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const float4 d = ok ? Dc[q - q0][(3 * aa + c) * 3 + bb] : make_float4(0.f, 0.f, 0.f, 0.f);
+            v[4 * h2 + c] = d.x * gj.x + d.y * gj.y + d.z * gj.z;
+          }
+          v[4 * h2 + 3] = 0.f;
+        }
+        bf16x8 bh, bm, bl;
+#pragma unroll
+        for (int x = 0; x < 8; ++x) {
+          __bf16 h, m_, lo;
+          split3(v[x], h, m_, lo);
+          bh[x] = h; bm[x] = m_; bl[x] = lo;
+        }
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt) acc[r][mt] = mfma_bf16x3(ah[mt], am[mt], al[mt], bh, bm, bl, acc[r][mt]);
+      }
+    }
+    __syncthreads();  // the next step overwrites Ac and Dc
+  }
+  // threshold 1e-14 max|K| (all-zero element skipped)
+  double mx = 0.0;
+#pragma unroll
+  for (int r = 0; r < 5; ++r) {
+    const int p = w + 4 * r;
+    if (p < 18) {
+      const int jb = 16 * (p % 6) + lr;
+#pragma unroll
+      for (int mt = 0; mt < 2; ++mt)
+        for (int x = 0; x < 4; ++x) {
+          const int i = 16 * mt + 4 * (l / 16) + x;
+          if (i < kNd && jb < kNl) mx = fmax(mx, fabs((double)acc[r][mt][x]));
+        }
+    }
+  }
+  red[t] = mx;
+  __syncthreads();
+  for (int o = kEbThreads / 2; o > 0; o >>= 1) {
+    if (t < o) red[t] = fmax(red[t], red[t + o]);
+    __syncthreads();
+  }
+  if (t == 0) a.thr[e] = red[0] > 0.0 ? 1e-14 * red[0] : INFINITY;
+  // E in three slabs of 27 contiguous rows (row 3 i + a, i in 9 p..9 p+8),
+  // staged through LDS and streamed with consecutive addresses across lanes
+  double *E = a.E + e * kNl * kNl;
+  for (int ps = 0; ps < 3; ++ps) {
+#pragma unroll
+    for (int r = 0; r < 5; ++r) {
+      const int p = w + 4 * r;
+      if (p < 18) {
+        const int aa = p / 6, jb = 16 * (p % 6) + lr;
+#pragma unroll
+        for (int mt = 0; mt < 2; ++mt)
+          for (int x = 0; x < 4; ++x) {
+            const int i = 16 * mt + 4 * (l / 16) + x;
+            if (i < kNd && jb < kNl && i / 9 == ps) slab[(3 * (i - 9 * ps) + aa) * kNl + jb] = acc[r][mt][x];
+          }
+      }
+    }
+    __syncthreads();
+    for (int x = t; x < 27 * kNl; x += kEbThreads)
+      __builtin_nontemporal_store(slab[x], E + (int64_t)ps * 27 * kNl + x);
+    __syncthreads();
+  }
+}
 
 template <typename Real>
 __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
@@ -41,10 +216,12 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   __shared__ double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], l3[kNq];
   __shared__ double gq[kNq][kNd][3];
   constexpr bool kLow = !std::is_same<Real, double>::value;
-  __shared__ Real grl[kLow ? kNq : 1][kNd][3];  // low-precision copy (FP32 form only)
-  Real (*gr)[kNd][3] = kLow ? grl : reinterpret_cast<Real (*)[kNd][3]>(gq);
-  __shared__ double Ac[kQc][81];
-  __shared__ Real Dc[kQc][81];
+  __shared__ float4 g4[kLow ? kNq : 1][kNd];  // FP32 form: the gradients as float4 (w = 0)
+  Real (*gr)[kNd][3] = reinterpret_cast<Real (*)[kNd][3]>(gq);  // FP64 form
+  __shared__ double Ac[kLow ? 1 : kQc][81];
+  __shared__ Real Dc[kLow ? 1 : kQc][81];
+  __shared__ double Ac32[kLow ? kQm : 1][81];  // FP32 form: a k-step's material tangents
+  __shared__ float4 Dc32[kLow ? kQm : 1][27];
   // T of a chunk and, at the end, a 27-row slab of E.  The FP32 form reads the
   // gradients from its own copy in phase B, so its buffer reuses gq.
   __shared__ double Tbuf_d[kLow ? 1 : kQc * 9 * kNl];
@@ -92,7 +269,11 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     const double *B = Bq[q];
     const double v = B[3 * c] * h[0] + B[3 * c + 1] * h[1] + B[3 * c + 2] * h[2];
     gq[q][i][c] = v;
-    if (kLow) grl[q][i][c] = (Real)v;
+    if (kLow) {
+      float *g4f = reinterpret_cast<float *>(&g4[q][i]);
+      g4f[c] = (float)v;
+      if (c == 2) g4f[3] = 0.f;
+    }
   }
   __syncthreads();
   // A3: deformation gradient
@@ -147,97 +328,101 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     }
     a.res[(int64_t)e * kNl + t] = re;
   }
-  // Phase B: thread (a, ib, jbb) for t < 243
-  const bool owner = t < 3 * kNl;
-  const int ar = owner ? t / kNl : 0, rr = owner ? t % kNl : 0;
-  const int ib = rr / 9, jbb = rr % 9;  // rows i = 3 ib + r (r < 3), columns jb = 9 jbb + m (m < 9)
-  // the sums stay FP64 in both forms (FP32 accumulators measured slower here)
-  double K[3][9];
-  for (int r = 0; r < 3; ++r)
-    for (int m = 0; m < 9; ++m) K[r][m] = 0.0;
-  for (int q0 = 0; q0 < kNq; q0 += kQc) {
-    // B0a: material tangent A[k c m n] = dS_kc/dE_mn of the chunk's points
-    //   A = 2 (-mu dCi + lambda/2 (Ci_mn Ci_kc + ln(I3) dCi)),
-    //   dCi = dCi_kc/dC_mn = -(Ci_km Ci_nc + Ci_kn Ci_mc) / 2
-    for (int x = t; x < kQc * 81; x += kEbThreads) {
-      const int qq = x / 81, q = q0 + qq, r = x % 81;
-      const int k = r / 27, c = (r / 9) % 3, m = (r / 3) % 3, n = r % 3;
-      const double *Ci = Cq[q];
-      const double dCi = -0.5 * (Ci[3 * k + m] * Ci[3 * n + c] + Ci[3 * k + n] * Ci[3 * m + c]);
-      Ac[qq][r] = 2.0 * (-mu * dCi + 0.5 * lam * (Ci[3 * m + n] * Ci[3 * k + c] + l3[q] * dCi));
-    }
-    __syncthreads();
-    // B0b: D_q[(a,c),(b,n)] = wJ_q (delta_ab S_nc + sum_km F_ak A[k c m n] F_bm)
-    for (int x = t; x < kQc * 81; x += kEbThreads) {
-      const int qq = x / 81, q = q0 + qq, ac = (x % 81) / 9, bn = x % 9;
-      const int aa = ac / 3, c = ac % 3, bb = bn / 3, n = bn % 3;
-      const double *F = Fq[q], *A = Ac[qq];
-      double s = (aa == bb) ? Sq[q][3 * n + c] : 0.0;
-      for (int k = 0; k < 3; ++k) {
-        const double *Ak = A + 27 * k + 9 * c + n;  // A[k c m n] at Ak[3 m]
-        s += F[3 * aa + k] * (Ak[0] * F[3 * bb] + Ak[3] * F[3 * bb + 1] + Ak[6] * F[3 * bb + 2]);
+  if constexpr (!kLow) {
+    // Phase B: thread (a, ib, jbb) for t < 243
+    const bool owner = t < 3 * kNl;
+    const int ar = owner ? t / kNl : 0, rr = owner ? t % kNl : 0;
+    const int ib = rr / 9, jbb = rr % 9;  // rows i = 3 ib + r (r < 3), columns jb = 9 jbb + m (m < 9)
+    // the sums stay FP64 in both forms (FP32 accumulators measured slower here)
+    double K[3][9];
+    for (int r = 0; r < 3; ++r)
+      for (int m = 0; m < 9; ++m) K[r][m] = 0.0;
+    for (int q0 = 0; q0 < kNq; q0 += kQc) {
+      // B0a: material tangent A[k c m n] = dS_kc/dE_mn of the chunk's points
+      //   A = 2 (-mu dCi + lambda/2 (Ci_mn Ci_kc + ln(I3) dCi)),
+      //   dCi = dCi_kc/dC_mn = -(Ci_km Ci_nc + Ci_kn Ci_mc) / 2
+      for (int x = t; x < kQc * 81; x += kEbThreads) {
+        const int qq = x / 81, q = q0 + qq, r = x % 81;
+        const int k = r / 27, c = (r / 9) % 3, m = (r / 3) % 3, n = r % 3;
+        const double *Ci = Cq[q];
+        const double dCi = -0.5 * (Ci[3 * k + m] * Ci[3 * n + c] + Ci[3 * k + n] * Ci[3 * m + c]);
+        Ac[qq][r] = 2.0 * (-mu * dCi + 0.5 * lam * (Ci[3 * m + n] * Ci[3 * k + c] + l3[q] * dCi));
       }
-      Dc[qq][x % 81] = (Real)(wJ[q] * s);
-    }
-    __syncthreads();
-    // B1: T_q[(a,c),(j,b)] = sum_n D_q[(a,c),(b,n)] g[q][j][n]; thread (a, jb)
-    if (owner) {
-      const int jb = rr, j = jb / 3, bb = jb % 3;
-      for (int qq = 0; qq < kQc; ++qq) {
-        const Real *g = gr[q0 + qq][j];
-        const Real g0 = g[0], g1 = g[1], g2 = g[2];
-        for (int c = 0; c < 3; ++c) {
-          const Real *d = &Dc[qq][9 * (3 * ar + c) + 3 * bb];
-          Tc[qq][3 * ar + c][jb] = d[0] * g0 + d[1] * g1 + d[2] * g2;
+      __syncthreads();
+      // B0b: D_q[(a,c),(b,n)] = wJ_q (delta_ab S_nc + sum_km F_ak A[k c m n] F_bm)
+      for (int x = t; x < kQc * 81; x += kEbThreads) {
+        const int qq = x / 81, q = q0 + qq, ac = (x % 81) / 9, bn = x % 9;
+        const int aa = ac / 3, c = ac % 3, bb = bn / 3, n = bn % 3;
+        const double *F = Fq[q], *A = Ac[qq];
+        double s = (aa == bb) ? Sq[q][3 * n + c] : 0.0;
+        for (int k = 0; k < 3; ++k) {
+          const double *Ak = A + 27 * k + 9 * c + n;  // A[k c m n] at Ak[3 m]
+          s += F[3 * aa + k] * (Ak[0] * F[3 * bb] + Ak[3] * F[3 * bb + 1] + Ak[6] * F[3 * bb + 2]);
         }
+        Dc[qq][x % 81] = (Real)(wJ[q] * s);
       }
-    }
-    __syncthreads();
-    // B2: K += G^T T over the chunk's (q, c)
-    if (owner) {
-      for (int qq = 0; qq < kQc; ++qq)
-        for (int c = 0; c < 3; ++c) {
-          const double g0 = gr[q0 + qq][3 * ib][c], g1 = gr[q0 + qq][3 * ib + 1][c],
-                       g2 = gr[q0 + qq][3 * ib + 2][c];
-          const Real *tv = &Tc[qq][3 * ar + c][9 * jbb];
-          for (int m = 0; m < 9; ++m) {
-            const double v = tv[m];
-            K[0][m] += g0 * v;
-            K[1][m] += g1 * v;
-            K[2][m] += g2 * v;
+      __syncthreads();
+      // B1: T_q[(a,c),(j,b)] = sum_n D_q[(a,c),(b,n)] g[q][j][n]; thread (a, jb)
+      if (owner) {
+        const int jb = rr, j = jb / 3, bb = jb % 3;
+        for (int qq = 0; qq < kQc; ++qq) {
+          const Real *g = gr[q0 + qq][j];
+          const Real g0 = g[0], g1 = g[1], g2 = g[2];
+          for (int c = 0; c < 3; ++c) {
+            const Real *d = &Dc[qq][9 * (3 * ar + c) + 3 * bb];
+            Tc[qq][3 * ar + c][jb] = d[0] * g0 + d[1] * g1 + d[2] * g2;
           }
         }
-    }
-    __syncthreads();
-  }
-  // threshold 1e-14 max|K| (all-zero element skipped)
-  double mx = 0.0;
-  for (int r = 0; r < 3; ++r)
-    for (int m = 0; m < 9; ++m) mx = fmax(mx, fabs((double)K[r][m]));
-  red[t] = owner ? mx : 0.0;
-  __syncthreads();
-  for (int o = kEbThreads / 2; o > 0; o >>= 1) {
-    if (t < o) red[t] = fmax(red[t], red[t + o]);
-    __syncthreads();
-  }
-  if (t == 0) a.thr[e] = red[0] > 0.0 ? 1e-14 * red[0] : INFINITY;
-  // E is streamed to HBM (GBs, read back once by the gather) in three slabs of
-  // 27 contiguous rows (rows 27p..27p+26 = i in 9p..9p+8, owned by ib in
-  // 3p..3p+2), staged through LDS so that each store instruction writes
-  // consecutive addresses across the lanes
-  double *slab = Tbuf;
-  static_assert(kQc * 9 >= 27, "a 27-row slab fits in the T buffer");
-  double *E = a.E + (int64_t)e * kNl * kNl;
-  for (int p = 0; p < 3; ++p) {
-    if (owner && ib / 3 == p)
-      for (int r = 0; r < 3; ++r) {
-        double *row = slab + (3 * (3 * (ib - 3 * p) + r) + ar) * kNl + 9 * jbb;
-        for (int m = 0; m < 9; ++m) row[m] = K[r][m];
       }
+      __syncthreads();
+      // B2: K += G^T T over the chunk's (q, c)
+      if (owner) {
+        for (int qq = 0; qq < kQc; ++qq)
+          for (int c = 0; c < 3; ++c) {
+            const double g0 = gr[q0 + qq][3 * ib][c], g1 = gr[q0 + qq][3 * ib + 1][c],
+                         g2 = gr[q0 + qq][3 * ib + 2][c];
+            const Real *tv = &Tc[qq][3 * ar + c][9 * jbb];
+            for (int m = 0; m < 9; ++m) {
+              const double v = tv[m];
+              K[0][m] += g0 * v;
+              K[1][m] += g1 * v;
+              K[2][m] += g2 * v;
+            }
+          }
+      }
+      __syncthreads();
+    }
+    // threshold 1e-14 max|K| (all-zero element skipped)
+    double mx = 0.0;
+    for (int r = 0; r < 3; ++r)
+      for (int m = 0; m < 9; ++m) mx = fmax(mx, fabs((double)K[r][m]));
+    red[t] = owner ? mx : 0.0;
     __syncthreads();
-    for (int x = t; x < 27 * kNl; x += kEbThreads)
-      __builtin_nontemporal_store(slab[x], E + (int64_t)p * 27 * kNl + x);
-    __syncthreads();
+    for (int o = kEbThreads / 2; o > 0; o >>= 1) {
+      if (t < o) red[t] = fmax(red[t], red[t + o]);
+      __syncthreads();
+    }
+    if (t == 0) a.thr[e] = red[0] > 0.0 ? 1e-14 * red[0] : INFINITY;
+    // E is streamed to HBM (GBs, read back once by the gather) in three slabs of
+    // 27 contiguous rows (rows 27p..27p+26 = i in 9p..9p+8, owned by ib in
+    // 3p..3p+2), staged through LDS so that each store instruction writes
+    // consecutive addresses across the lanes
+    double *slab = Tbuf;
+    static_assert(kQc * 9 >= 27, "a 27-row slab fits in the T buffer");
+    double *E = a.E + (int64_t)e * kNl * kNl;
+    for (int p = 0; p < 3; ++p) {
+      if (owner && ib / 3 == p)
+        for (int r = 0; r < 3; ++r) {
+          double *row = slab + (3 * (3 * (ib - 3 * p) + r) + ar) * kNl + 9 * jbb;
+          for (int m = 0; m < 9; ++m) row[m] = K[r][m];
+        }
+      __syncthreads();
+      for (int x = t; x < 27 * kNl; x += kEbThreads)
+        __builtin_nontemporal_store(slab[x], E + (int64_t)p * 27 * kNl + x);
+      __syncthreads();
+    }
+  } else {
+    nh_phase_b_mfma(t, e, lam, mu, Fq, Sq, Cq, l3, wJ, g4, Ac32, Dc32, Tbuf, red, a);
   }
 }
 
