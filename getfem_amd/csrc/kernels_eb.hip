@@ -313,21 +313,24 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     }
   }
   __syncthreads();
-  // A5: the residual
-  if (t < kNl) {
-    const int i = t / 3, ar = t % 3;
-    double re = 0.0;
-    for (int q = 0; q < kNq; ++q) {
-      const double *F = Fq[q], *S = Sq[q];
-      double pg = 0.0;
-      for (int c = 0; c < 3; ++c) {
-        const double P = F[3 * ar] * S[c] + F[3 * ar + 1] * S[3 + c] + F[3 * ar + 2] * S[6 + c];
-        pg += P * gq[q][i][c];
-      }
-      re += wJ[q] * pg;
-    }
-    a.res[(int64_t)e * kNl + t] = re;
+  // A5: the residual re[(i,a)] = sum_q sum_c P_q[a][c] g[q][i][c] with
+  // P_q = wJ_q F_q S_q (first Piola stress times the weight, into Bq, which
+  // is free after A2), as three partial sums over 9 points each
+  if (t < kNq * 9) {
+    const int q = t / 9, ar = (t % 9) / 3, c = t % 3;
+    const double *F = Fq[q], *S = Sq[q];
+    Bq[q][t % 9] = wJ[q] * (F[3 * ar] * S[c] + F[3 * ar + 1] * S[3 + c] + F[3 * ar + 2] * S[6 + c]);
   }
+  __syncthreads();
+  if (t < 3 * kNl) {
+    const int grp = t / kNl, o = t % kNl, i = o / 3, ar = o % 3;
+    double re = 0.0;
+    for (int q = 9 * grp; q < 9 * grp + 9; ++q)
+      re += Bq[q][3 * ar] * gq[q][i][0] + Bq[q][3 * ar + 1] * gq[q][i][1] + Bq[q][3 * ar + 2] * gq[q][i][2];
+    red[t] = re;
+  }
+  __syncthreads();
+  if (t < kNl) a.res[(int64_t)e * kNl + t] = red[t] + red[kNl + t] + red[2 * kNl + t];
   if constexpr (!kLow) {
     // Phase B: thread (a, ib, jbb) for t < 243
     const bool owner = t < 3 * kNl;
