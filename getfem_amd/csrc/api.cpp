@@ -745,7 +745,8 @@ int gfx_plan_assemble(gfx_plan *p) {
     a.order = p->d_order + b.off;
     a.desc = p->d_desc + b.off;
     a.npw = b.npw;
-    if (p->eb) HIPCHK(c, launch_gather_emat(st, a, p->d_E, p->d_thr, b.cap));
+    if (p->eb)  // the element kernel wrote E in the form's precision
+      HIPCHK(c, launch_gather_emat(st, a, p->d_E, p->precision == GFX_PREC_FP32, p->d_thr, b.cap));
     else HIPCHK(c, launch_assemble_patch(st, p->term, p->form, p->precision, a, b.cap));
   }
   for (int i = 0; i < naux; ++i) {

@@ -125,7 +125,7 @@ struct EbArgs {
   const double *gref;       // nq*nd*3 base gradients
   const double *qw;
   int nq;
-  double *E;                // n_cv * nl * nl element matrices (row-major)
+  double *E;                // n_cv * nl * nl element matrices (row-major; FP32 values in the FP32 form)
   double *res;              // n_cv * nl element residuals
   double *thr;              // n_cv thresholds
   int *err;
@@ -166,7 +166,7 @@ int64_t scan_tmp_elems(int64_t n);
 hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision);
 hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, const double *E,
                             const double *thr, uint32_t *keep);
-hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a, const double *E, const double *thr,
+hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a, const void *E, bool e_fp32, const double *thr,
                               int max_nslots);
 hipError_t launch_gather_vec(hipStream_t s, int64_t n_nodes, int nd, int Q, const int32_t *inc_ptr,
                              const uint32_t *inc, const double *res, double *V);

@@ -186,9 +186,11 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, double lam, do
     __syncthreads();
   }
   if (t == 0) a.thr[e] = red[0] > 0.0 ? 1e-14 * red[0] : INFINITY;
-  // E in three slabs of 27 contiguous rows (row 3 i + a, i in 9 p..9 p+8),
-  // staged through LDS and streamed with consecutive addresses across lanes
-  double *E = a.E + e * kNl * kNl;
+  // E (FP32: the form's values are FP32 already) in three slabs of 27
+  // contiguous rows (row 3 i + a, i in 9 p..9 p+8), staged through LDS and
+  // streamed with consecutive addresses across lanes
+  float *E = reinterpret_cast<float *>(a.E) + e * kNl * kNl;
+  float *slab32 = reinterpret_cast<float *>(slab);
   for (int ps = 0; ps < 3; ++ps) {
 #pragma unroll
     for (int r = 0; r < 5; ++r) {
@@ -199,13 +201,13 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, double lam, do
         for (int mt = 0; mt < 2; ++mt)
           for (int x = 0; x < 4; ++x) {
             const int i = 16 * mt + 4 * (l / 16) + x;
-            if (i < kNd && jb < kNl && i / 9 == ps) slab[(3 * (i - 9 * ps) + aa) * kNl + jb] = acc[r][mt][x];
+            if (i < kNd && jb < kNl && i / 9 == ps) slab32[(3 * (i - 9 * ps) + aa) * kNl + jb] = acc[r][mt][x];
           }
       }
     }
     __syncthreads();
     for (int x = t; x < 27 * kNl; x += kEbThreads)
-      __builtin_nontemporal_store(slab[x], E + (int64_t)ps * 27 * kNl + x);
+      __builtin_nontemporal_store(slab32[x], E + (int64_t)ps * 27 * kNl + x);
     __syncthreads();
   }
 }
@@ -460,8 +462,8 @@ constexpr int kGatherThreads = 256;
 // lists are concatenated and streamed together.
 constexpr int kGatherNpw = 4;
 
-template <int ND, int Q>
-__global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, const double *__restrict__ E,
+template <int ND, int Q, typename TE>
+__global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, const TE *__restrict__ E,
                                                                 const double *__restrict__ thr) {
   extern __shared__ double acc[];  // npw accumulators of lds_acc doubles
   constexpr int kMaxInc = 64;      // incidences per workgroup (checked by the launcher)
@@ -509,7 +511,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   const int total = total_inc * RUN;
   for (int f = t; f < total; f += kGatherThreads) {
     const int k = f / RUN, o = f - k * RUN;
-    const double v = __builtin_nontemporal_load(E + sbase[k] + o);  // read once
+    const double v = (double)__builtin_nontemporal_load(E + sbase[k] + o);  // read once
     const int x = o / NL, c = o - x * NL, j = c / Q, y = c - j * Q;
     const int ns = sns[k];
     if (fabs(v) > sthr[k]) atomicAdd(&acc[sacc[k] + (x * ns + sslot[k * ND + j]) * Q + y], v);
@@ -583,7 +585,7 @@ hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, con
   return hipGetLastError();
 }
 
-hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const double *E, const double *thr,
+hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const void *E, bool e_fp32, const double *thr,
                               int max_nslots) {
   if (a0.n_nodes == 0) return hipSuccess;
   PatchArgs a = a0;
@@ -594,8 +596,15 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const double *
   a.lds_acc = a.qdim * a.qdim * max_nslots;
   const size_t lds = (size_t)a.npw * a.lds_acc * sizeof(double);
   if (lds > 48 * 1024 || a.npw * a.max_inc > 64) return hipErrorInvalidValue;
-  if (a.nd == 27 && a.qdim == 3) k_gather_emat<27, 3><<<grid, kGatherThreads, lds, s>>>(a, E, thr);
-  else if (a.nd == 8 && a.qdim == 3) k_gather_emat<8, 3><<<grid, kGatherThreads, lds, s>>>(a, E, thr);
+  const double *E64 = static_cast<const double *>(E);
+  const float *E32 = static_cast<const float *>(E);
+  if (a.nd == 27 && a.qdim == 3) {
+    if (e_fp32) k_gather_emat<27, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
+    else k_gather_emat<27, 3, double><<<grid, kGatherThreads, lds, s>>>(a, E64, thr);
+  } else if (a.nd == 8 && a.qdim == 3) {
+    if (e_fp32) k_gather_emat<8, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
+    else k_gather_emat<8, 3, double><<<grid, kGatherThreads, lds, s>>>(a, E64, thr);
+  }
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
