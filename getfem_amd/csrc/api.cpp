@@ -1005,6 +1005,7 @@ int gfx_compliance_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, double lambda, dou
   if (e == hipSuccess) e = dalloc(&c->d_cU, (size_t)m->n_cv * 12);
   if (e == hipSuccess) e = dalloc(&c->d_Jpart, 2 * (size_t)compliance_nblocks(m->n_cv));
   if (e == hipSuccess) e = dalloc(&c->d_err, 1);
+  if (e == hipSuccess) e = hipMemset(c->d_err, 0, 4);  // sticky: set by any evaluation that meets J = 0
   if (e == hipSuccess) e = dalloc(&tmp, (size_t)(2 * big + scan_tmp_elems(big)));
   if (e == hipSuccess)
     e = launch_incidences(ctx->stream, m->n_cv, 4, 1, m->d_cvx, m->n_pts, c->d_vinc_ptr, c->d_vinc, tmp);
@@ -1027,7 +1028,7 @@ int gfx_compliance_eval(gfx_compliance *c, const double *d_X, const double *d_rh
   if (set_device(ctx)) return GFX_E_HIP;
   hipStream_t s = ctx->stream;
   const gfx_mesh *m = c->mf->mesh;
-  HIPCHK(ctx, hipMemsetAsync(c->d_err, 0, 4, s));
+  // kernels only (no memset): the call can be captured into a graph and replayed
   HIPCHK(ctx, launch_compliance(s, c->n_cv, d_X ? d_X : m->d_pts, m->d_cvx, c->mf->d_dofs, d_u, d_rho,
                                 c->lam, c->mu, c->emin, c->penal, c->wsum, c->d_Jpart, d_J, d_dJ_drho,
                                 d_dJ_dX ? c->d_cX : nullptr, d_dJ_du ? c->d_cU : nullptr, c->d_err));

@@ -28,19 +28,24 @@ class Compliance:
                                           float(penal), ctypes.byref(h)), self.ctx.h)
         self.h = h
 
-    def eval(self, rho, X, u, want_grads=True):
-        """All tensors float64 on the device; X may be None (mesh points)."""
+    def eval(self, rho, X, u, want_grads=True, out=None):
+        """All tensors float64 on the device; X may be None (mesh points).
+        out: optional (J, drho, dX, du) device tensors to write into."""
         dev = rho.device
-        stream = torch.cuda.current_stream(dev)
-        self.ctx.set_stream(stream.cuda_stream)
+        stream = torch.cuda.current_stream(dev).cuda_stream
+        if stream != (self.ctx.stream or 0):  # the context may be shared: compare with its stream
+            self.ctx.set_stream(stream)
         rho = rho.contiguous()
         u = u.contiguous()
         Xc = X.contiguous() if X is not None else None
-        J = torch.empty((), dtype=torch.float64, device=dev)
-        dr = torch.empty_like(rho) if want_grads else None
-        n_pts = self.mf.mesh.pts.shape[0]
-        dx = torch.empty((n_pts, 3), dtype=torch.float64, device=dev) if want_grads else None
-        du = torch.empty_like(u) if want_grads else None
+        if out is not None:
+            J, dr, dx, du = out
+        else:
+            J = torch.empty((), dtype=torch.float64, device=dev)
+            dr = torch.empty_like(rho) if want_grads else None
+            n_pts = self.mf.mesh.pts.shape[0]
+            dx = torch.empty((n_pts, 3), dtype=torch.float64, device=dev) if want_grads else None
+            du = torch.empty_like(u) if want_grads else None
         p = lambda t: ctypes.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
         check(lib().gfx_compliance_eval(self.h, p(Xc), p(rho), p(u), p(J), p(dr), p(dx), p(du)),
               self.ctx.h)

@@ -280,6 +280,8 @@ int gfx_ws_describe(gfx_workspace *w, char *buf, int64_t len);
 typedef struct gfx_compliance gfx_compliance;
 int gfx_compliance_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, double lambda, double mu,
                           double emin, double penal, gfx_compliance **out);
+/* kernels only on the context's stream (no allocation, copy, memset or host
+ * synchronisation): a call may be captured into a HIP graph and replayed */
 int gfx_compliance_eval(gfx_compliance *c, const double *d_X, const double *d_rho,
                         const double *d_u, double *d_J, double *d_dJ_drho,
                         double *d_dJ_dX, double *d_dJ_du);

@@ -87,3 +87,33 @@ def test_adjoint_finite_differences_64_directions(ctx):
             worst = max(worst, err)
     print("worst relative FD error over 64 directions", worst)
     assert worst <= 1e-6
+
+
+def test_compliance_graph_replays_match_eager(ctx):
+    """gfx_compliance_eval is kernels only: captured once into a graph, every
+    replay (with eager evaluations and synchronisations in between, on
+    changing inputs) gives the eager result."""
+    import torch
+    op, rho, X, u, prob, keep = setup(3, ctx)
+    outs = (torch.empty((), dtype=torch.float64, device="cuda"), torch.empty_like(rho),
+            torch.empty_like(X), torch.empty_like(u))
+    side = torch.cuda.Stream()
+    side.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(side):
+        op.eval(rho, X, u, out=outs)
+    torch.cuda.current_stream().wait_stream(side)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        op.eval(rho, X, u, out=outs)
+    torch.cuda.synchronize()
+    for k in range(4):
+        with torch.no_grad():
+            rho.mul_(0.9).add_(0.05)
+            u.mul_(1.1)
+        J, dr, dx, du = op.eval(rho.clone(), X, u.clone())  # eager, fresh outputs
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        for a, b in zip((J, dr, dx, du), outs):
+            assert torch.equal(a, b), k
