@@ -721,12 +721,12 @@ int gfx_plan_assemble(gfx_plan *p) {
   a.nslots = p->d_nslots; a.mask = p->d_mask; a.mwpi = p->mwpi; a.indptr = p->d_indptr;
   a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
   a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals; a.max_inc = p->max_inc;
-  // The bins write disjoint rows: they run concurrently on up to three
-  // auxiliary streams forked from and joined back into the context's stream
-  // (events only, no host synchronisation), which fills the tail of one
-  // launch with the next.
+  // The bins write disjoint rows: they run concurrently, each on its own
+  // stream (up to seven auxiliary streams forked from and joined back into
+  // the context's stream by events, no host synchronisation), which fills the
+  // tail of one launch with the next.
   const int nb = (int)p->bins.size();
-  const int naux = std::min(nb - 1, 3);
+  const int naux = std::min(nb - 1, 7);
   while ((int)p->aux.size() < naux) {
     hipStream_t st; hipEvent_t ej;
     HIPCHK(c, hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
