@@ -259,6 +259,48 @@ def test_neohookean_tangent_is_derivative_of_residual():
     assert np.abs(K - K.T).max() < 1e-12 * np.abs(K).max()
 
 
+def bonet_energy(P, U, lam=10.0, mu=1.0):
+    """Total stored energy sum_e sum_q w_q |J_q| W(C_q) with GetFEM's Bonet
+    energy W = mu/2 (I1 - 3 - ln I3) + lambda/8 (ln I3)^2
+    (Neo_Hookean_hyperelastic_law::strain_energy, src/getfem_nonlinear_elasticity.cc:612-633),
+    evaluated here in numpy from the reference bases and the quadrature."""
+    qp, qw = oracle.im_rule("IM_GAUSS_PARALLELEPIPED(3,5)")
+    tot, vol = 0.0, 0.0
+    for e in range(P.cvx.shape[0]):
+        X = P.pts[P.cvx[e]]
+        Ue = np.array([U[P.dofs[e, i]:P.dofs[e, i] + 3] for i in range(27)])
+        for x, w in zip(qp, qw):
+            _, gN = oracle.fem_eval(oracle.FEM_QK, 1, x)
+            K = X.T @ gN
+            J = np.linalg.det(K)
+            B = np.linalg.inv(K).T
+            _, gr = oracle.fem_eval(oracle.FEM_QK, 2, x)
+            g = gr @ B.T
+            F = np.eye(3) + Ue.T @ g
+            C = F.T @ F
+            l3 = np.log(np.linalg.det(C))
+            tot += w * abs(J) * (mu / 2 * (np.trace(C) - 3 - l3) + lam / 8 * l3 * l3)
+            vol += w * abs(J)
+    return tot, vol
+
+
+def test_neohookean_residual_is_gradient_of_bonet_energy():
+    """The oracle's residual (its PK2 law) against central differences of the
+    reference's stored-energy formula: pins S to GetFEM's Bonet energy, which
+    the tangent-vs-residual check alone cannot."""
+    P, U = nh_problem()
+    R = P.assemble_vector()
+    _, vol = bonet_energy(P, U)
+    assert abs(vol - 1.0) < 1e-12  # the quadrature and the geometry cover the unit cube
+    h = 1e-6
+    rng = np.random.default_rng(2)
+    for d in rng.choice(P.ndof, 8, replace=False):
+        up = U.copy(); up[d] += h
+        um = U.copy(); um[d] -= h
+        fd = (bonet_energy(P, up)[0] - bonet_energy(P, um)[0]) / (2 * h)
+        assert abs(fd - R[d]) <= 1e-6 * np.abs(R).max(), (d, fd, R[d])
+
+
 def test_neohookean_zero_stress_at_rest():
     P, U = nh_problem()
     P0 = oracle.Problem(P.s.gt, P.pts, P.cvx, oracle.FEM_QK, 2, 3, P.dofs, P.ndof,
