@@ -596,6 +596,94 @@ static int plan_bins(gfx_plan *p, const std::vector<uint16_t> &ns, const std::ve
   return GFX_OK;
 }
 
+// Patch plans: nodes in node order (which is spatial order: first-encounter
+// numbering over the element order) are packed into waves of up to 4 node
+// patches while their incidences fill about three passes of the lanes and
+// their LDS stays within kWaveLds bytes, so that LDS does not bound the
+// occupancy below what the kernel's registers allow.  One launch then sweeps
+// the mesh in order and the element records of a region are read from HBM
+// about once.  Nodes whose patch alone exceeds the budget get one wave each,
+// in launches by size class.  d_order receives 4 node ids per wave (-1: an
+// empty slot): the large classes first, then the packed waves, one bin each.
+static int plan_waves(gfx_plan *p, const std::vector<uint16_t> &ns, const std::vector<int32_t> &iptr) {
+  gfx_ctx *c = p->ctx;
+  const int64_t nn = p->n_nodes;
+  const int Q = p->qdim, half = p->nd / 2;
+  constexpr int kWaveLds = 6144;
+  const int inc_cap = 3 * (64 / p->nd);
+  auto need = [&](int64_t n, int &acc, int &flat, int &inc) {
+    const int s = ns[n] & 0x7fff;
+    const bool full = (ns[n] & 0x8000) != 0;
+    inc = iptr[n + 1] - iptr[n];
+    acc = Q * Q * s;
+    flat = (full ? 0 : (Q * Q * s + 31) / 32) + inc * (1 + half);
+  };
+  // large patches: size classes by LDS bytes (each launch reserves LDS for
+  // its own largest patch), one node per wave
+  static const int kLargeCaps[] = {8192, 12288, 16384, 24576, 32768, 49152, 65536};
+  constexpr int kLargeClasses = sizeof(kLargeCaps) / sizeof(kLargeCaps[0]);
+  std::vector<int32_t> packed, large[kLargeClasses];
+  gfx_plan::Bin bm{0, 0, 0, 4}, bl[kLargeClasses];
+  for (auto &b : bl) b = gfx_plan::Bin{0, 0, 0, 1};
+  int cnt = 0, sinc = 0, sacc = 0, sflat = 0;
+  auto close = [&]() {
+    if (!cnt) return;
+    for (; cnt < 4; ++cnt) packed.push_back(-1);
+    bm.acc_cap = std::max(bm.acc_cap, sacc);
+    bm.flat_cap = std::max(bm.flat_cap, sflat);
+    bm.count++;
+    cnt = sinc = sacc = sflat = 0;
+  };
+  for (int64_t n = 0; n < nn; ++n) {
+    int acc, flat, inc;
+    need(n, acc, flat, inc);
+    const int bytes = 8 * acc + 4 * flat;
+    if (bytes > kWaveLds) {
+      int k = 0;
+      while (k < kLargeClasses - 1 && kLargeCaps[k] < bytes) ++k;
+      if (bytes > kLargeCaps[k]) return fail(c, GFX_E_PATTERN, "node patch too large for the assembly kernel");
+      large[k].push_back((int32_t)n);
+      for (int t = 1; t < 4; ++t) large[k].push_back(-1);
+      bl[k].acc_cap = std::max(bl[k].acc_cap, acc);
+      bl[k].flat_cap = std::max(bl[k].flat_cap, flat);
+      bl[k].count++;
+      continue;
+    }
+    if (cnt && (cnt == 4 || sinc + inc > inc_cap || 8 * (sacc + acc) + 4 * (sflat + flat) > kWaveLds)) close();
+    packed.push_back((int32_t)n);
+    cnt++;
+    sinc += inc;
+    sacc += acc;
+    sflat += flat;
+  }
+  close();
+  // the large patches are launched first, largest first (their waves are the
+  // longest); the packed launch fills the GPU around them
+  p->bins.clear();
+  std::vector<int32_t> all;
+  int64_t off = 0;
+  for (int k = kLargeClasses - 1; k >= 0; --k) {
+    if (!bl[k].count) continue;
+    bl[k].off = off;
+    off += bl[k].count;
+    p->bins.push_back(bl[k]);
+    all.insert(all.end(), large[k].begin(), large[k].end());
+  }
+  if (bm.count) {
+    bm.off = off;
+    p->bins.push_back(bm);
+    all.insert(all.end(), packed.begin(), packed.end());
+  }
+  packed.swap(all);
+  p->n_order = (int64_t)packed.size();
+  dfree(p->d_order);
+  if (!packed.empty()) {
+    HIPCHK(c, dalloc(&p->d_order, packed.size()));
+    HIPCHK(c, hipMemcpy(p->d_order, packed.data(), 4 * packed.size(), hipMemcpyHostToDevice));
+  }
+  return GFX_OK;
+}
+
 int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   if (!p) return GFX_E_INVALID;
   gfx_ctx *c = p->ctx;
@@ -653,7 +741,10 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.err = p->d_err;
   if (p->eb) HIPCHK(c, launch_keep_emat(s, n_cv, p->nloc, p->kwords, p->d_E, p->d_thr, p->d_keep));
   else HIPCHK(c, launch_keep(s, p->term, a));
-  // patches
+  // patches (the patch kernel addresses presence words and slot pairs with
+  // 32-bit word offsets)
+  if ((int64_t)p->n_inc * std::max(p->mwpi, nd / 2) >= (int64_t(1) << 32))
+    return fail(c, GFX_E_UNSUPPORTED, "mesh too large for one plan (presence/slot maps beyond 2^32 words)");
   HIPCHK(c, dalloc(&p->d_slot, (size_t)p->n_inc * nd));
   HIPCHK(c, dalloc(&p->d_nslots, (size_t)nn));
   HIPCHK(c, dalloc(&p->d_nbr, (size_t)p->n_inc * nd));
@@ -680,10 +771,11 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   for (int64_t n = 0; n < nn; ++n) p->max_nslots = std::max<int>(p->max_nslots, ns[n] & 0x7fff);
   if ((size_t)Q * Q * p->max_nslots * 8 > 64 * 1024)
     return fail(c, GFX_E_PATTERN, "node patch too large for the assembly kernel");
-  if ((r = plan_bins(p, ns, iptr))) return r;
+  if ((r = p->eb ? plan_bins(p, ns, iptr) : plan_waves(p, ns, iptr))) return r;
+  if (p->eb) p->n_order = nn;
   dfree(p->d_desc);
-  HIPCHK(c, dalloc(&p->d_desc, (size_t)nn));
-  HIPCHK(c, launch_fill_desc(s, nn, Q, p->d_order, p->d_inc_ptr, p->d_nslots, p->d_indptr, p->d_desc));
+  HIPCHK(c, dalloc(&p->d_desc, (size_t)p->n_order));
+  HIPCHK(c, launch_fill_desc(s, p->n_order, Q, p->d_order, p->d_inc_ptr, p->d_nslots, p->d_indptr, p->d_desc));
   HIPCHK(c, dalloc(&p->d_indices, (size_t)nnz));
   HIPCHK(c, dalloc(&p->d_vals, (size_t)nnz));
   a.indices = p->d_indices;
@@ -741,13 +833,19 @@ int gfx_plan_assemble(gfx_plan *p) {
   for (int k = 0; k < nb; ++k) {  // one launch per patch-size bin, spread over streams
     const gfx_plan::Bin &b = p->bins[k];
     hipStream_t st = k == 0 ? c->stream : p->aux[(k - 1) % naux];
-    a.n_nodes = b.count;
-    a.order = p->d_order + b.off;
-    a.desc = p->d_desc + b.off;
-    a.npw = b.npw;
-    if (p->eb)  // the element kernel wrote E in the form's precision
+    if (p->eb) {  // the element kernel wrote E in the form's precision
+      a.n_nodes = b.count;
+      a.order = p->d_order + b.off;
+      a.desc = p->d_desc + b.off;
+      a.npw = b.npw;
       HIPCHK(c, launch_gather_emat(st, a, p->d_E, p->precision == GFX_PREC_FP32, p->d_thr, b.cap));
-    else HIPCHK(c, launch_assemble_patch(st, p->term, p->form, p->precision, a, b.cap));
+    } else {
+      a.n_waves = b.count;
+      a.desc = p->d_desc + 4 * b.off;
+      a.lds_acc = b.acc_cap;
+      a.lds_flat = b.flat_cap;
+      HIPCHK(c, launch_assemble_patch(st, p->term, p->form, p->precision, a));
+    }
   }
   for (int i = 0; i < naux; ++i) {
     HIPCHK(c, hipEventRecord(p->join[i], p->aux[i]));

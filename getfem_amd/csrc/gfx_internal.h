@@ -85,10 +85,15 @@ struct PatchArgs {
   const double *qw;         // nq
   const double *R;          // nd*nd*9 weighted reference gradient products
   double *vals;             // CSR values
-  int xcd_chunk;            // nodes per XCD in the block swizzle
+  int xcd_chunk;            // waves (patch kernel) or nodes (gather) per XCD in the block swizzle
   int max_inc;              // largest number of incidences of a node
-  int npw;                  // node patches per wavefront (1..4)
-  int lds_acc, lds_mask, lds_inc;  // dynamic LDS layout (set by the launcher)
+  int npw;                  // gather kernel: node patches per workgroup (1..4)
+  int lds_acc, lds_mask, lds_inc;  // gather kernel: dynamic LDS layout (set by its launcher)
+  // patch kernel: desc holds n_waves groups of 4 descriptors (empty slots
+  // zero); lds_acc and lds_flat are the largest accumulator (doubles) and
+  // staging run (4-byte words) of any of the launch's waves
+  int64_t n_waves;
+  int lds_flat;
 };
 
 struct PatternArgs {
@@ -154,8 +159,7 @@ hipError_t launch_patch_build(hipStream_t s, const PatternArgs &a, int max_cand)
 hipError_t launch_fill_columns(hipStream_t s, const PatternArgs &a);
 hipError_t launch_fill_desc(hipStream_t s, int64_t n, int qdim, const int32_t *order, const int32_t *inc_ptr,
                             const uint16_t *nslots, const int64_t *indptr, PatchDesc *desc);
-hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec,
-                                 const PatchArgs &a, int max_nslots);
+hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec, const PatchArgs &a);
 hipError_t scan_exclusive_i64(hipStream_t s, const int64_t *in, int64_t *out,
                               int64_t n, int64_t *tmp);
 hipError_t scan_exclusive_i32(hipStream_t s, const int32_t *in, int32_t *out,
@@ -223,12 +227,17 @@ struct gfx_plan {
   int32_t *d_nbr = nullptr;
   uint32_t *d_mask = nullptr;
   int max_nslots = 0, max_cand = 0, max_inc = 0;
-  // nodes grouped by patch size so each launch reserves LDS for its own
-  // largest patch: d_order holds the bins back to back
+  // element-batched plans: nodes grouped by patch size so each gather launch
+  // reserves LDS for its own largest patch (d_order holds the bins back to
+  // back); patch plans: 4 node ids per wave (-1: empty slot)
   int32_t *d_order = nullptr;
   gfx::PatchDesc *d_desc = nullptr;  // per position of d_order
-  struct Bin { int64_t off, count; int cap, npw; };
+  // gather bins: node positions [off, off+count), largest nslots cap, npw
+  // patches per workgroup; patch launches: wave groups [off, off+count) of
+  // d_desc / d_order (4 entries each), LDS caps acc_cap doubles + flat_cap words
+  struct Bin { int64_t off, count; int cap, npw; int acc_cap = 0, flat_cap = 0; };
   std::vector<Bin> bins;
+  int64_t n_order = 0;       // entries of d_order / d_desc
   int64_t ndof = 0, nnz = -1;
   int64_t *d_indptr = nullptr;
   int32_t *d_indices = nullptr;

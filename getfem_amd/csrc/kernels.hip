@@ -371,25 +371,34 @@ __global__ void k_fill_desc(int64_t n, int qdim, const int32_t *__restrict__ ord
   const int64_t p = gtid();
   if (p >= n) return;
   const int64_t node = order[p];
-  PatchDesc d;
-  d.rowbase = indptr[node * qdim];
-  d.i0 = inc_ptr[node];
-  d.ninc_ns = (uint32_t)(inc_ptr[node + 1] - d.i0) | ((uint32_t)nslots[node] << 16);
+  PatchDesc d{0, 0, 0u};  // node -1: an empty slot of a wave's descriptor group
+  if (node >= 0) {
+    d.rowbase = indptr[node * qdim];
+    d.i0 = inc_ptr[node];
+    d.ninc_ns = (uint32_t)(inc_ptr[node + 1] - d.i0) | ((uint32_t)nslots[node] << 16);
+  }
   desc[p] = d;
 }
 
 // ------------------------------------------------------------------------
 // the assembly kernel: one workgroup (one wavefront) per node patch
 // ------------------------------------------------------------------------
-// A wavefront assembles npw node patches of one bin at once (npw = 1 for
-// large patches, up to kMaxNpw for small ones): their incidence lists are
-// concatenated, so the lanes of one pass may serve different patches and the
-// chain of dependent loads (descriptor -> incidences -> element records) is
-// paid once for all of them.
-// Dynamic LDS: [acc: npw x lds_acc doubles][mask: npw x lds_mask words]
-// [inc: lds_inc words][slots: lds_inc x ND u16].  The incidences, slot maps
-// and presence words are staged in one round of independent loads.
+// A wavefront assembles up to kMaxNpw node patches at once, given by its
+// group of kMaxNpw descriptors (empty slots are zero).  The host packs
+// consecutive nodes (node order is spatial order) into a wave while their
+// incidences fill about three passes of the lanes and their LDS fits the
+// launch's budget, so one launch sweeps the mesh in order and the element
+// records of a region are read from HBM about once.  The incidence lists of
+// the wave's patches are concatenated: the lanes of one pass may serve
+// different patches, and the chain of dependent loads (descriptors ->
+// incidences -> element records) is paid once for all of them.
+// Dynamic LDS: [accumulators: lds_acc doubles, patch k at the prefix sum of
+// Q*Q*ns] then one flat run of lds_flat 4-byte words: the presence words of
+// the patches that are not full, the incidences, and the slot maps as u16
+// pairs (ND is even).  The run is filled by one round of independent loads
+// (kStage words per lane in flight), so staging costs one memory round trip.
 constexpr int kMaxNpw = 4;
+constexpr int kStage = 6;
 
 template <typename T>
 __device__ __forceinline__ T pick4(const T (&v)[kMaxNpw], int k) {
@@ -398,19 +407,23 @@ __device__ __forceinline__ T pick4(const T (&v)[kMaxNpw], int k) {
 
 template <int ND, int Q, int TERM, int FORM, typename Real>
 __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
+  static_assert(ND % 2 == 0, "slot maps are staged as u16 pairs");
   extern __shared__ double lds[];
   const int64_t wg = blockIdx.x;
   const int lane = threadIdx.x;
-  const int npw = a.npw;
-  const int64_t pos0 = ((wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3)) * npw;
-  int i0[kMaxNpw], ns[kMaxNpw], st[kMaxNpw], pre[kMaxNpw + 1];
+  const int64_t w = (wg & 7) * (int64_t)a.xcd_chunk + (wg >> 3);  // XCD x: a contiguous run of waves
+  if (w >= a.n_waves) return;
+  const PatchDesc *wd = a.desc + w * kMaxNpw;
+  int i0[kMaxNpw], ns[kMaxNpw], st[kMaxNpw], pre[kMaxNpw + 1], mst[kMaxNpw], mpre[kMaxNpw + 1];
+  int aoff[kMaxNpw];
   bool full[kMaxNpw];
   int64_t rb[kMaxNpw];
   pre[0] = 0;
+  mpre[0] = 0;
+  int apre = 0;
 #pragma unroll
   for (int k = 0; k < kMaxNpw; ++k) {
-    PatchDesc d{0, 0, 0u};
-    if (k < npw && pos0 + k < a.n_nodes) d = a.desc[pos0 + k];
+    const PatchDesc d = wd[k];
     const int nsraw = (int)(d.ninc_ns >> 16);
     i0[k] = d.i0;
     ns[k] = nsraw & (kFullRows - 1);
@@ -418,42 +431,85 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
     rb[k] = d.rowbase;
     st[k] = pre[k];
     pre[k + 1] = pre[k] + (int)(d.ninc_ns & 0xffffu);
+    mst[k] = mpre[k];
+    mpre[k + 1] = mpre[k] + (full[k] ? 0 : (Q * Q * ns[k] + 31) / 32);
+    aoff[k] = apre;
+    apre += Q * Q * ns[k];
   }
   const int total = pre[kMaxNpw];  // incidences of the wave's patches
+  const int W = mpre[kMaxNpw];     // presence words of the wave's patches
+  constexpr int HALF = ND / 2;
+  const int nwords = W + total + total * HALF;
+  constexpr int PER = kWave / ND;  // incidences per pass
   double *acc = lds;
-  uint32_t *smask = reinterpret_cast<uint32_t *>(lds + (size_t)npw * a.lds_acc);
-  uint32_t *sinc = smask + (size_t)npw * a.lds_mask;
-  uint16_t *sslot = reinterpret_cast<uint16_t *>(sinc + a.lds_inc);
-#pragma unroll
-  for (int k = 0; k < kMaxNpw; ++k) {
-    if (k >= npw) break;
-    const int dense = Q * Q * ns[k];
-    double *ak = acc + (size_t)k * a.lds_acc;
-    for (int t = lane; t < dense; t += kWave) ak[t] = 0.0;
-    if (!full[k]) {
-      const uint32_t *gm = a.mask + (int64_t)i0[k] * a.mwpi;  // no loads for an empty patch
-      uint32_t *sm = smask + (size_t)k * a.lds_mask;
-      for (int t = lane; t < (dense + 31) / 32; t += kWave) sm[t] = gm[t];
-    }
-  }
+  uint32_t *smask = reinterpret_cast<uint32_t *>(lds + a.lds_acc);
+  const uint32_t *sinc = smask + W;
+  const uint16_t *sslot = reinterpret_cast<const uint16_t *>(sinc + total);
+  const uint32_t *slot32 = reinterpret_cast<const uint32_t *>(a.slot);
   // patch of combined incidence c: k = number of patch starts at or below c
   auto patch_of = [&](int c) { return (int)(c >= pre[1]) + (int)(c >= pre[2]) + (int)(c >= pre[3]); };
-  for (int t = lane; t < total; t += kWave) {
-    const int k = patch_of(t);
-    sinc[t] = a.inc[pick4(i0, k) + (t - pick4(st, k))];
+  // The staged words form 12 runs (presence words, incidences, slot pairs of
+  // patches 0..3), each a contiguous range of its source array: run q starts
+  // at flat index rs[q] and maps t to source word ro[q] + t (mod 2^32).  The
+  // source offset of t is ro[0] + t plus the steps ro[q] - ro[q-1] of every
+  // run start at or below t (a sum of selects: no indexed array, so the
+  // table stays in scalar registers; empty runs add their step and the next
+  // run's step together, which leaves the next run's offset).
+  int rs[3 * kMaxNpw];
+  uint32_t ro[3 * kMaxNpw];
+#pragma unroll
+  for (int k = 0; k < kMaxNpw; ++k) {
+    rs[k] = mst[k];
+    ro[k] = (uint32_t)i0[k] * (uint32_t)a.mwpi - (uint32_t)rs[k];
+    rs[kMaxNpw + k] = W + st[k];
+    ro[kMaxNpw + k] = (uint32_t)i0[k] - (uint32_t)rs[kMaxNpw + k];
+    rs[2 * kMaxNpw + k] = W + total + st[k] * HALF;
+    ro[2 * kMaxNpw + k] = (uint32_t)i0[k] * HALF - (uint32_t)rs[2 * kMaxNpw + k];
   }
-  for (int t = lane; t < total * ND; t += kWave) {
-    const int c = t / ND, k = patch_of(c);
-    sslot[t] = a.slot[((int64_t)pick4(i0, k) + (c - pick4(st, k))) * ND + (t - c * ND)];
+  uint32_t step[3 * kMaxNpw];
+#pragma unroll
+  for (int q = 1; q < 3 * kMaxNpw; ++q) step[q] = ro[q] - ro[q - 1];
+  auto source = [&](int t) -> const uint32_t * {
+    uint32_t off = ro[0] + (uint32_t)t;
+#pragma unroll
+    for (int q = 1; q < 3 * kMaxNpw; ++q) off += t >= rs[q] ? step[q] : 0u;
+    const uint32_t *base = t < W ? a.mask : (t < W + total ? a.inc : slot32);
+    return base + off;
+  };
+  // Loads are unconditional (the index is clamped to the last word, so every
+  // address is valid) and only the LDS stores are masked: no branches between
+  // the loads, so all kStage of them are in flight before the first wait.
+  const int last = nwords > 0 ? nwords - 1 : 0;
+  uint32_t buf[kStage];
+  if (nwords > 0) {
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) buf[r] = *source(min(r * kWave + lane, last));
+  }
+  // clear the accumulators while the staging loads are in flight
+  for (int t = lane; t < apre; t += kWave) acc[t] = 0.0;
+  if (nwords > 0) {
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) {
+      const int t = r * kWave + lane;
+      if (t < nwords) smask[t] = buf[r];
+    }
+  }
+  for (int t0 = kStage * kWave; t0 < nwords; t0 += kStage * kWave) {  // large waves
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) buf[r] = *source(min(t0 + r * kWave + lane, last));
+#pragma unroll
+    for (int r = 0; r < kStage; ++r) {
+      const int t = t0 + r * kWave + lane;
+      if (t < nwords) smask[t] = buf[r];
+    }
   }
   __syncthreads();
-  constexpr int PER = kWave / ND;  // incidences per pass
   const int kk = lane / ND, j = lane - kk * ND;
   for (int c0 = 0; c0 < total; c0 += PER) {
     const int c = c0 + kk;
     if (kk < PER && c < total) {
       const int k = patch_of(c), nsk = pick4(ns, k);
-      double *ak = acc + (size_t)k * a.lds_acc;
+      double *ak = acc + pick4(aoff, k);
       const uint32_t inc = sinc[c];
       const int64_t e = inc >> 5;
       const int i = inc & 31;
@@ -473,26 +529,56 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
   // The values are written once and not read again by this kernel: stream
   // them past the caches (non-temporal stores), which keeps L2 for the
   // element records and incidence data the neighbouring patches reuse.
+  // A node's Q rows are consecutive CSR rows and consecutive runs of its
+  // dense accumulator, so the node's values are one compaction of the
+  // Q*ns*Q dense entries by the presence bits, starting at the node's first
+  // CSR value.  The LDS reads of kOut chunks are issued together.
+  constexpr int kOut = 4;
 #pragma unroll
   for (int k = 0; k < kMaxNpw; ++k) {
-    if (k >= npw || pos0 + k >= a.n_nodes) break;
-    const double *ak = acc + (size_t)k * a.lds_acc;
-    const int nsk = ns[k];
-    if (full[k]) {  // every entry present: the node's Q rows are one contiguous run
+    const double *ak = acc + aoff[k];
+    const int n_d = Q * ns[k] * Q;  // 0 for an empty slot
+    if (full[k]) {  // every entry present: one contiguous run, 16-byte stores
       double *dst = a.vals + rb[k];
-      for (int d = lane; d < Q * nsk * Q; d += kWave) __builtin_nontemporal_store(ak[d], dst + d);
+      const int head = (int)(rb[k] & 1);  // one 8-byte value up to 16-byte alignment
+      if (head && lane == 0 && n_d > 0) __builtin_nontemporal_store(ak[0], dst);
+      const int npair = (n_d - head) >> 1;
+      typedef double double2v __attribute__((ext_vector_type(2)));
+      double2v *dst2 = reinterpret_cast<double2v *>(dst + head);
+      for (int t0 = 0; t0 < npair; t0 += kOut * kWave) {
+        double2v v2[kOut];
+#pragma unroll
+        for (int u = 0; u < kOut; ++u) {  // clamped: the reads are unconditional
+          const int t = min(t0 + u * kWave + lane, max(npair - 1, 0));
+          v2[u].x = ak[head + 2 * t];
+          v2[u].y = ak[head + 2 * t + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < kOut; ++u) {
+          const int t = t0 + u * kWave + lane;
+          if (t < npair) __builtin_nontemporal_store(v2[u], dst2 + t);
+        }
+      }
+      if (((n_d - head) & 1) && lane == 0) __builtin_nontemporal_store(ak[n_d - 1], dst + n_d - 1);
       continue;
     }
-    const uint32_t *sm = smask + (size_t)k * a.lds_mask;
-    const int64_t node = a.order[pos0 + k];
-    for (int x = 0; x < Q; ++x) {
-      int64_t base = x == 0 ? rb[k] : a.indptr[node * Q + x];
-      const int off = x * nsk * Q, len = nsk * Q;
-      for (int c0 = 0; c0 < len; c0 += kWave) {
-        const int d = c0 + lane;
-        const bool pr = d < len && mask_bit(sm, off + d);
+    const uint32_t *sm = smask + mst[k];
+    int64_t base = rb[k];
+    for (int c0 = 0; c0 < n_d; c0 += kOut * kWave) {
+      uint32_t word[kOut];
+      double val[kOut];
+#pragma unroll
+      for (int u = 0; u < kOut; ++u) {  // clamped indices: the reads are unconditional
+        const int dc = min(c0 + u * kWave + lane, n_d - 1);
+        word[u] = sm[dc >> 5];
+        val[u] = ak[dc];
+      }
+#pragma unroll
+      for (int u = 0; u < kOut; ++u) {
+        const int d = c0 + u * kWave + lane;
+        const bool pr = (d < n_d) & (bool)((word[u] >> (d & 31)) & 1u);
         const uint64_t bal = __ballot(pr);
-        if (pr) __builtin_nontemporal_store(ak[off + d], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
+        if (pr) __builtin_nontemporal_store(val[u], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
         base += __popcll(bal);
       }
     }
@@ -681,21 +767,15 @@ static hipError_t launch_asm3(hipStream_t s, int form, int prec, const PatchArgs
   return hipGetLastError();
 }
 
-hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec,
-                                 const PatchArgs &a0, int max_nslots) {
-  if (a0.n_nodes == 0) return hipSuccess;
+hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec, const PatchArgs &a0) {
+  // a0.n_waves wave descriptor groups; a0.lds_acc / a0.lds_flat: the largest
+  // accumulator (doubles) and staging run (4-byte words) of any of its waves
+  if (a0.n_waves == 0) return hipSuccess;
   PatchArgs a = a0;
-  if (a.npw < 1 || a.npw > kMaxNpw) return hipErrorInvalidValue;
-  const int64_t waves = (a.n_nodes + a.npw - 1) / a.npw;
-  a.xcd_chunk = (int)((waves + 7) / 8);
+  a.xcd_chunk = (int)((a.n_waves + 7) / 8);
   const unsigned grid = (unsigned)(8 * (int64_t)a.xcd_chunk);
-  const int dense = a.qdim * a.qdim * max_nslots;
-  a.lds_acc = dense;                                   // doubles per patch
-  a.lds_mask = ((dense + 31) / 32 + 1) & ~1;           // u32 words per patch, keeps 8-byte alignment
-  a.lds_inc = (a.npw * a.max_inc + 1) & ~1;            // u32 words for the wave's incidences
-  const size_t lds = 8 * (size_t)a.npw * a.lds_acc + 4 * (size_t)a.npw * a.lds_mask +
-                     4 * (size_t)a.lds_inc + 2 * (size_t)a.lds_inc * a.nd;
-  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  const size_t lds = 8 * (size_t)a.lds_acc + 4 * (size_t)a.lds_flat;
+  if (a.lds_acc < 0 || a.lds_flat < 0 || lds > 64 * 1024) return hipErrorInvalidValue;
   const int nd = a.nd, Q = a.qdim;
   if (term == GFX_TERM_LAPLACE && Q == 1) {
     if (nd == 4) return launch_asm3<4, 1, GFX_TERM_LAPLACE>(s, form, prec, a, grid, lds);
