@@ -610,7 +610,7 @@ static int plan_waves(gfx_plan *p, const std::vector<uint16_t> &ns, const std::v
   const int64_t nn = p->n_nodes;
   const int Q = p->qdim, half = p->nd / 2;
   constexpr int kWaveLds = 6144;
-  const int inc_cap = 3 * (64 / p->nd);
+  const int inc_cap = 3 * (64 / (p->nd >= 20 ? p->nd / 2 : p->nd));  // three passes (kernel: lanes per incidence)
   auto need = [&](int64_t n, int &acc, int &flat, int &inc) {
     const int s = ns[n] & 0x7fff;
     const bool full = (ns[n] & 0x8000) != 0;

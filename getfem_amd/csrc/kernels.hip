@@ -440,7 +440,11 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
   const int W = mpre[kMaxNpw];     // presence words of the wave's patches
   constexpr int HALF = ND / 2;
   const int nwords = W + total + total * HALF;
-  constexpr int PER = kWave / ND;  // incidences per pass
+  // pair blocks per lane: 2 for large elements (P3: 6 incidences per pass
+  // instead of 3, half the passes of a vertex patch), else 1
+  constexpr int JB = ND >= 20 ? 2 : 1;
+  constexpr int LPI = ND / JB;      // lanes per incidence
+  constexpr int PER = kWave / LPI;  // incidences per pass
   double *acc = lds;
   uint32_t *smask = reinterpret_cast<uint32_t *>(lds + a.lds_acc);
   const uint32_t *sinc = smask + W;
@@ -504,7 +508,9 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
     }
   }
   __syncthreads();
-  const int kk = lane / ND, j = lane - kk * ND;
+  // Lanes: LPI per incidence, each computing the JB pair blocks (j, j + LPI,
+  // ...) of its incidence, so the element record is loaded once for JB blocks.
+  const int kk = lane / LPI, jl = lane - kk * LPI;
   for (int c0 = 0; c0 < total; c0 += PER) {
     const int c = c0 + kk;
     if (kk < PER && c < total) {
@@ -515,14 +521,18 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
       const int i = inc & 31;
       const double *rec = a.rec + (int64_t)kRecStride * e;
       const double thr = rec[12];
-      Real v[Q][Q];
-      pair_block<Q, TERM, FORM, Real>(rec, i, j, ND, a.nq, a.gref, a.qw, a.R, v);
-      const int s = sslot[c * ND + j];
-      for (int x = 0; x < Q; ++x)
-        for (int y = 0; y < Q; ++y) {
-          const double val = (double)v[x][y];
-          if (fabs(val) > thr) atomicAdd(&ak[(x * nsk + s) * Q + y], val);
-        }
+#pragma unroll 1
+      for (int jb = 0; jb < JB; ++jb) {
+        const int j = jl + jb * LPI;
+        Real v[Q][Q];
+        pair_block<Q, TERM, FORM, Real>(rec, i, j, ND, a.nq, a.gref, a.qw, a.R, v);
+        const int s = sslot[c * ND + j];
+        for (int x = 0; x < Q; ++x)
+          for (int y = 0; y < Q; ++y) {
+            const double val = (double)v[x][y];
+            if (fabs(val) > thr) atomicAdd(&ak[(x * nsk + s) * Q + y], val);
+          }
+      }
     }
   }
   __syncthreads();
