@@ -218,7 +218,7 @@ def run_gpu(args):
             "kernel": "k_assemble_patch",
             "bytes_per_element": b_patch,
             "kernel_ms": head["ms_assemble_kernel"],
-            "launches_per_assembly": "one per patch-size bin of nodes; kernel_ms is their sum",
+            "launches_per_assembly": "one packed launch in node order plus one per size class of large patches, concurrent; kernel_ms is their span",
             "compulsory_bytes": compulsory,
             "compulsory_frac": compulsory / (head["ms_assemble_kernel"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },

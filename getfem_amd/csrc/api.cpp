@@ -813,10 +813,11 @@ int gfx_plan_assemble(gfx_plan *p) {
   a.nslots = p->d_nslots; a.mask = p->d_mask; a.mwpi = p->mwpi; a.indptr = p->d_indptr;
   a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
   a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals; a.max_inc = p->max_inc;
-  // The bins write disjoint rows: they run concurrently, each on its own
-  // stream (up to seven auxiliary streams forked from and joined back into
-  // the context's stream by events, no host synchronisation), which fills the
-  // tail of one launch with the next.
+  // The launches (patch plans: large-patch size classes and the packed
+  // waves; element-batched plans: gather bins) write disjoint rows: they run
+  // concurrently, each on its own stream (up to seven auxiliary streams forked
+  // from and joined back into the context's stream by events, no host
+  // synchronisation), which fills the tail of one launch with the next.
   const int nb = (int)p->bins.size();
   const int naux = std::min(nb - 1, 7);
   while ((int)p->aux.size() < naux) {
@@ -830,7 +831,7 @@ int gfx_plan_assemble(gfx_plan *p) {
     HIPCHK(c, hipEventRecord(p->fork, c->stream));
     for (int i = 0; i < naux; ++i) HIPCHK(c, hipStreamWaitEvent(p->aux[i], p->fork, 0));
   }
-  for (int k = 0; k < nb; ++k) {  // one launch per patch-size bin, spread over streams
+  for (int k = 0; k < nb; ++k) {  // one launch per bin, spread over streams
     const gfx_plan::Bin &b = p->bins[k];
     hipStream_t st = k == 0 ? c->stream : p->aux[(k - 1) % naux];
     if (p->eb) {  // the element kernel wrote E in the form's precision

@@ -6,9 +6,10 @@
 //   k_records_tet    per element, FP64: K = G grad(N), det K, B = K^{-T}
 //                    (bgeot_geometric_trans.cc:270-413) and the element's
 //                    coefficients -> one 128-byte record per element.
-//   k_assemble_patch one workgroup per mesh node ("patch"): for every
-//                    element incident to the node, computes the rows of the
-//                    element matrix that belong to the node's dofs, drops
+//   k_assemble_patch one wavefront per group of up to 4 mesh nodes
+//                    ("patches"): for every element incident to a node,
+//                    computes the rows of the element matrix that belong to
+//                    the node's dofs, drops
 //                    entries under the element's insertion threshold
 //                    (compile_and_exec.cc:5441-5453), accumulates them in
 //                    LDS and writes the node's CSR rows once.  Rows are owned
@@ -381,7 +382,7 @@ __global__ void k_fill_desc(int64_t n, int qdim, const int32_t *__restrict__ ord
 }
 
 // ------------------------------------------------------------------------
-// the assembly kernel: one workgroup (one wavefront) per node patch
+// the assembly kernel: one workgroup (one wavefront) per group of node patches
 // ------------------------------------------------------------------------
 // A wavefront assembles up to kMaxNpw node patches at once, given by its
 // group of kMaxNpw descriptors (empty slots are zero).  The host packs
