@@ -508,13 +508,27 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
       sslot[pre[k] * ND + x] = a.slot[(int64_t)dsc[k].i0 * ND + x];
   }
   __syncthreads();
+  // kB entries per thread in flight: the loads use clamped indices (always a
+  // valid entry) and are issued before any of them is used
+  constexpr int kB = 8;
   const int total = total_inc * RUN;
-  for (int f = t; f < total; f += kGatherThreads) {
-    const int k = f / RUN, o = f - k * RUN;
-    const double v = (double)__builtin_nontemporal_load(E + sbase[k] + o);  // read once
-    const int x = o / NL, c = o - x * NL, j = c / Q, y = c - j * Q;
-    const int ns = sns[k];
-    if (fabs(v) > sthr[k]) atomicAdd(&acc[sacc[k] + (x * ns + sslot[k * ND + j]) * Q + y], v);
+  for (int f0 = 0; f0 < total; f0 += kB * kGatherThreads) {
+    double v[kB];
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int f = min(f0 + u * kGatherThreads + t, total - 1);
+      const int k = f / RUN, o = f - k * RUN;
+      v[u] = (double)__builtin_nontemporal_load(E + sbase[k] + o);  // read once
+    }
+#pragma unroll
+    for (int u = 0; u < kB; ++u) {
+      const int f = f0 + u * kGatherThreads + t;
+      if (f >= total) break;
+      const int k = f / RUN, o = f - k * RUN;
+      const int x = o / NL, c = o - x * NL, j = c / Q, y = c - j * Q;
+      const int ns = sns[k];
+      if (fabs(v[u]) > sthr[k]) atomicAdd(&acc[sacc[k] + (x * ns + sslot[k * ND + j]) * Q + y], v[u]);
+    }
   }
   __syncthreads();
   // values are streamed past the caches (written once, never re-read here)
@@ -529,13 +543,23 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
       for (int d = t; d < Q * ns * Q; d += kGatherThreads) __builtin_nontemporal_store(ak[d], dst + d);
       continue;
     }
-    // wavefront x writes row x from the presence bits
+    // wavefront x writes row x from the presence bits; the node's rows are
+    // consecutive CSR rows, so row x starts after the present entries of
+    // rows 0..x-1 (bits [0, x*ns*Q) of the node's presence words)
     const uint32_t *m = a.mask + (int64_t)dsc[k].i0 * a.mwpi;
     if (w < Q) {
       const int x = w;
-      const int64_t node = a.order[pos0 + k];
-      int64_t base = a.indptr[node * Q + x];
       const int off = x * ns * Q, len = ns * Q;
+      int cnt = 0;
+      for (int b0 = 0; b0 < off; b0 += 64 * 32) {
+        const int b = b0 + 32 * lane;  // this lane's word of the prefix
+        if (b < off) {
+          const uint32_t word = m[b >> 5];
+          cnt += __popc(off - b >= 32 ? word : (word & ((1u << (off - b)) - 1u)));
+        }
+      }
+      for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
+      int64_t base = dsc[k].rowbase + cnt;
       for (int c0 = 0; c0 < len; c0 += 64) {
         const int d = c0 + lane;
         const bool pr = d < len && mbit(m, off + d);
