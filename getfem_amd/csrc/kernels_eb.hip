@@ -489,23 +489,43 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   }
   const int total_inc = pre[kGatherNpw];
   if (total_inc > kMaxInc) return;  // excluded by the launcher
+  // staging, flattened over the workgroup's incidences (patch k of combined
+  // incidence c: the number of patch starts at or below c): the incidence
+  // words and slot maps are loaded together, then the thresholds
+  auto patch_of = [&](int c) { return (int)(c >= pre[1]) + (int)(c >= pre[2]) + (int)(c >= pre[3]); };
+  int gi0[kGatherNpw], gns[kGatherNpw], pre4[kGatherNpw];
+#pragma unroll
+  for (int k = 0; k < kGatherNpw; ++k) {
+    gi0[k] = dsc[k].i0;
+    gns[k] = (int)(dsc[k].ninc_ns >> 16) & 0x7fff;
+    pre4[k] = pre[k];
+  }
+  auto sel = [](const int (&v)[kGatherNpw], int k) {  // by value: no indexed array
+    return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
+  };
+  uint32_t inc = 0;
+  if (t < total_inc) {
+    const int k = patch_of(t);
+    inc = a.inc[sel(gi0, k) + (t - sel(pre4, k))];
+  }
+  for (int x = t; x < total_inc * ND; x += kGatherThreads) {
+    const int c = x / ND, k = patch_of(c);
+    sslot[x] = a.slot[((int64_t)sel(gi0, k) + (c - sel(pre4, k))) * ND + (x - c * ND)];
+  }
 #pragma unroll
   for (int k = 0; k < kGatherNpw; ++k) {
     if (k >= npw) break;
     const int ns = (int)(dsc[k].ninc_ns >> 16) & 0x7fff;
     double *ak = acc + (size_t)k * a.lds_acc;
     for (int x = t; x < Q * Q * ns; x += kGatherThreads) ak[x] = 0.0;
-    const int ninc = (int)(dsc[k].ninc_ns & 0xffffu);
-    if (t < ninc) {
-      const uint32_t inc = a.inc[dsc[k].i0 + t];
-      const int64_t e = inc >> 5;
-      sbase[pre[k] + t] = e * (int64_t)NL * NL + (int64_t)(inc & 31) * RUN;
-      sthr[pre[k] + t] = thr[e];
-      sacc[pre[k] + t] = k * a.lds_acc;
-      sns[pre[k] + t] = ns;
-    }
-    for (int x = t; x < ninc * ND; x += kGatherThreads)
-      sslot[pre[k] * ND + x] = a.slot[(int64_t)dsc[k].i0 * ND + x];
+  }
+  if (t < total_inc) {
+    const int k = patch_of(t);
+    const int64_t e = inc >> 5;
+    sbase[t] = e * (int64_t)NL * NL + (int64_t)(inc & 31) * RUN;
+    sthr[t] = thr[e];
+    sacc[t] = k * a.lds_acc;
+    sns[t] = sel(gns, k);
   }
   __syncthreads();
   // kB entries per thread in flight: the loads use clamped indices (always a
