@@ -580,12 +580,26 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
       }
       for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o, 64);
       int64_t base = dsc[k].rowbase + cnt;
-      for (int c0 = 0; c0 < len; c0 += 64) {
-        const int d = c0 + lane;
-        const bool pr = d < len && mbit(m, off + d);
-        const uint64_t bal = __ballot(pr);
-        if (pr) __builtin_nontemporal_store(ak[off + d], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
-        base += __popcll(bal);
+      // kOut chunks of 64 per round: presence words and values are read with
+      // clamped indices before any ballot, so their latencies overlap
+      constexpr int kOut = 8;
+      for (int c0 = 0; c0 < len; c0 += kOut * 64) {
+        uint32_t word[kOut];
+        double val[kOut];
+#pragma unroll
+        for (int u = 0; u < kOut; ++u) {
+          const int dc = off + min(c0 + u * 64 + lane, len - 1);
+          word[u] = m[dc >> 5];
+          val[u] = ak[dc];
+        }
+#pragma unroll
+        for (int u = 0; u < kOut; ++u) {
+          const int d = c0 + u * 64 + lane;
+          const bool pr = (d < len) & (bool)((word[u] >> ((off + d) & 31)) & 1u);
+          const uint64_t bal = __ballot(pr);
+          if (pr) __builtin_nontemporal_store(val[u], a.vals + base + __popcll(bal & ((1ull << lane) - 1ull)));
+          base += __popcll(bal);
+        }
       }
     }
   }
