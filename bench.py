@@ -48,9 +48,9 @@ FP64_PEAK_TFLOPS = 78.6
 # (bench.py --copy-calibration).  Every form of a config has the same memory
 # pattern (FP64 values and accumulation), so one figure serves all forms.
 PMC_TRAFFIC_KIB = {  # config: (kernel, FETCH_SIZE KiB, WRITE_SIZE KiB)
-    "c2": ("k_assemble_patch", 892718, 4562144),
-    "c5": ("k_assemble_patch", 234576, 3276065),
-    "c3": ("k_nh_element", 55209, 6028335),
+    "c2": ("k_assemble_patch", 455118, 4533318),
+    "c5": ("k_assemble_patch", 180195, 3198760),
+    "c3": ("k_nh_element", 55208, 6025668),
 }
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
 
