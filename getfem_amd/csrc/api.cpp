@@ -389,6 +389,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_coef[0]); dfree(p->d_coef[1]);
   dfree(p->d_gref); dfree(p->d_qw); dfree(p->d_R);
   dfree(p->d_rec); dfree(p->d_thr); dfree(p->d_keep);
+  dfree(p->d_gref32); dfree(p->d_qw32); dfree(p->d_R32); dfree(p->d_rec32);
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order); dfree(p->d_desc);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
@@ -473,6 +474,16 @@ int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef
   if (e == hipSuccess) e = hipMemcpy(p->d_qw, mim->rule.w.data(), 8 * (size_t)nq, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_R, R.data(), 8 * R.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(p->d_err, 0, 4);
+  if (!eb) {  // FP32 copies of the tables for the FP32 contraction form
+    std::vector<float> gref32(gref.begin(), gref.end()), R32(R.begin(), R.end());
+    std::vector<float> qw32(mim->rule.w.begin(), mim->rule.w.end());
+    if (e == hipSuccess) e = dalloc(&p->d_gref32, gref32.size());
+    if (e == hipSuccess) e = dalloc(&p->d_qw32, qw32.size());
+    if (e == hipSuccess) e = dalloc(&p->d_R32, R32.size());
+    if (e == hipSuccess) e = hipMemcpy(p->d_gref32, gref32.data(), 4 * gref32.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_qw32, qw32.data(), 4 * qw32.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->d_R32, R32.data(), 4 * R32.size(), hipMemcpyHostToDevice);
+  }
   if (e != hipSuccess) { gfx_plan_destroy(p); return hipfail(c, e, "gfx_plan_create"); }
   *out = p;
   return GFX_OK;
@@ -494,14 +505,19 @@ int gfx_plan_set_form(gfx_plan *p, int form, int precision) {
   return GFX_OK;
 }
 
+// Element records for the pattern build (FP64, no thresholds yet) or for an
+// assembly (with thresholds; FP32 records for the FP32 contraction form).
 static int plan_records(gfx_plan *p, bool with_thr) {
   const gfx_mesh *m = p->mf->mesh;
   gfx_ctx *c = p->ctx;
+  const bool fp32 = with_thr && p->precision == GFX_PREC_FP32;
+  if (fp32 && !p->d_rec32) HIPCHK(c, dalloc(&p->d_rec32, (size_t)kRecStride * m->n_cv));
   HIPCHK(c, launch_records_tet(c->stream, m->n_cv, m->d_pts, m->d_cvx,
                                p->coef[0].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[0] : nullptr,
                                p->coef[0].value,
                                p->coef[1].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[1] : nullptr,
-                               p->coef[1].value, with_thr ? p->d_thr : nullptr, p->d_rec, p->d_err));
+                               p->coef[1].value, with_thr ? p->d_thr : nullptr, p->d_rec,
+                               fp32 ? p->d_rec32 : nullptr, p->d_err));
   return GFX_OK;
 }
 
@@ -813,6 +829,7 @@ int gfx_plan_assemble(gfx_plan *p) {
   a.nslots = p->d_nslots; a.mask = p->d_mask; a.mwpi = p->mwpi; a.indptr = p->d_indptr;
   a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
   a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals; a.max_inc = p->max_inc;
+  a.rec32 = p->d_rec32; a.gref32 = p->d_gref32; a.qw32 = p->d_qw32; a.R32 = p->d_R32;
   // The launches (patch plans: large-patch size classes and the packed
   // waves; element-batched plans: gather bins) write disjoint rows: they run
   // concurrently, each on its own stream (up to seven auxiliary streams forked

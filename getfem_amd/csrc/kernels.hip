@@ -42,11 +42,12 @@ __device__ __forceinline__ double wave_max(double v) {
 // ------------------------------------------------------------------------
 // geometry
 // ------------------------------------------------------------------------
+// rec (FP64) or rec32 (FP32 contraction form, 64 bytes per element) is written
 __global__ __launch_bounds__(256) void k_records_tet(
     int64_t n_cv, const double *__restrict__ pts, const int32_t *__restrict__ cvx,
     const double *__restrict__ ca, double ca_c, const double *__restrict__ cb,
     double cb_c, const double *__restrict__ thr, double *__restrict__ rec,
-    int *__restrict__ err) {
+    float *__restrict__ rec32, int *__restrict__ err) {
   const int64_t e = gtid();
   if (e >= n_cv) return;
   const int4 v = *reinterpret_cast<const int4 *>(cvx + 4 * e);
@@ -66,33 +67,40 @@ __global__ __launch_bounds__(256) void k_records_tet(
   const double a2 = k[3] * k[7] - k[4] * k[6];
   const double J = k[0] * a0 + k[1] * a1 + k[2] * a2;
   if (J == 0.0) { atomicOr(err, 1); }
-  double *o = rec + (int64_t)kRecStride * e;
-  // B = K^{-T} from cofactors, stored row-major B(r,c) = o[3r+c]
-  double2 *o2 = reinterpret_cast<double2 *>(o);
-  o2[0] = make_double2(a0 / J, (k[2] * k[7] - k[1] * k[8]) / J);
-  o2[1] = make_double2((k[1] * k[5] - k[2] * k[4]) / J, a1 / J);
-  o2[2] = make_double2((k[0] * k[8] - k[2] * k[6]) / J, (k[2] * k[3] - k[0] * k[5]) / J);
-  o2[3] = make_double2(a2 / J, (k[1] * k[6] - k[0] * k[7]) / J);
-  o2[4] = make_double2((k[0] * k[4] - k[1] * k[3]) / J, fabs(J));
-  o2[5] = make_double2(ca ? ca[e] : ca_c, cb ? cb[e] : cb_c);
-  o2[6] = make_double2(thr ? thr[e] : 0.0, 0.0);
-  o2[7] = make_double2(0.0, 0.0);
+  // B = K^{-T} from cofactors, stored row-major B(r,c) = o[3r+c], then |J|,
+  // the two coefficients and the insertion threshold
+  double o[16];
+  o[0] = a0 / J; o[1] = (k[2] * k[7] - k[1] * k[8]) / J;
+  o[2] = (k[1] * k[5] - k[2] * k[4]) / J; o[3] = a1 / J;
+  o[4] = (k[0] * k[8] - k[2] * k[6]) / J; o[5] = (k[2] * k[3] - k[0] * k[5]) / J;
+  o[6] = a2 / J; o[7] = (k[1] * k[6] - k[0] * k[7]) / J;
+  o[8] = (k[0] * k[4] - k[1] * k[3]) / J; o[9] = fabs(J);
+  o[10] = ca ? ca[e] : ca_c; o[11] = cb ? cb[e] : cb_c;
+  o[12] = thr ? thr[e] : 0.0; o[13] = o[14] = o[15] = 0.0;
+  if (rec32) {
+    float4 *o4 = reinterpret_cast<float4 *>(rec32 + (int64_t)kRecStride * e);
+    for (int t = 0; t < 4; ++t)
+      o4[t] = make_float4((float)o[4 * t], (float)o[4 * t + 1], (float)o[4 * t + 2], (float)o[4 * t + 3]);
+  } else {
+    double2 *o2 = reinterpret_cast<double2 *>(rec + (int64_t)kRecStride * e);
+    for (int t = 0; t < 8; ++t) o2[t] = make_double2(o[2 * t], o[2 * t + 1]);
+  }
 }
 
 // ------------------------------------------------------------------------
 // element pair blocks: rows (i, a), columns (j, b) of the element matrix
 // ------------------------------------------------------------------------
 template <int Q, int TERM, int FORM, typename Real>
-__device__ __forceinline__ void pair_block(const double *__restrict__ rec, int i, int j,
-                                           int nd, int nq, const double *__restrict__ gref,
-                                           const double *__restrict__ qw,
-                                           const double *__restrict__ R, Real v[Q][Q]) {
+__device__ __forceinline__ void pair_block(const Real *__restrict__ rec, int i, int j,
+                                           int nd, int nq, const Real *__restrict__ gref,
+                                           const Real *__restrict__ qw,
+                                           const Real *__restrict__ R, Real v[Q][Q]) {
   Real B[9];
   for (int t = 0; t < 9; ++t) B[t] = (Real)rec[t];
   const Real aJ = (Real)rec[9], ca = (Real)rec[10], cb = (Real)rec[11];
   if (FORM == GFX_FORM_FACTORIZED) {
     // G = B R_ij B^T, R_ij = sum_q w_q gradref_i gradref_j^T
-    const double *r = R + ((int64_t)i * nd + j) * 9;
+    const Real *r = R + ((int64_t)i * nd + j) * 9;
     Real rr[9];
     for (int t = 0; t < 9; ++t) rr[t] = (Real)r[t];
     Real T[3][3], G[3][3];
@@ -115,8 +123,8 @@ __device__ __forceinline__ void pair_block(const double *__restrict__ rec, int i
     for (int a = 0; a < Q; ++a)
       for (int b = 0; b < Q; ++b) v[a][b] = Real(0);
     for (int q = 0; q < nq; ++q) {
-      const double *hi = gref + ((int64_t)q * nd + i) * 3;
-      const double *hj = gref + ((int64_t)q * nd + j) * 3;
+      const Real *hi = gref + ((int64_t)q * nd + i) * 3;
+      const Real *hj = gref + ((int64_t)q * nd + j) * 3;
       const Real h0 = (Real)hi[0], h1 = (Real)hi[1], h2 = (Real)hi[2];
       const Real k0 = (Real)hj[0], k1 = (Real)hj[1], k2 = (Real)hj[2];
       Real gi[3], gj[3];
@@ -512,6 +520,14 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
   // Lanes: LPI per incidence, each computing the JB pair blocks (j, j + LPI,
   // ...) of its incidence, so the element record is loaded once for JB blocks.
   const int kk = lane / LPI, jl = lane - kk * LPI;
+  // tables in the contraction's precision (the FP32 form reads FP32 records:
+  // half the bytes per lane)
+  const Real *tab_rec, *tab_gref, *tab_qw, *tab_R;
+  if constexpr (sizeof(Real) == 4) {
+    tab_rec = a.rec32; tab_gref = a.gref32; tab_qw = a.qw32; tab_R = a.R32;
+  } else {
+    tab_rec = a.rec; tab_gref = a.gref; tab_qw = a.qw; tab_R = a.R;
+  }
   for (int c0 = 0; c0 < total; c0 += PER) {
     const int c = c0 + kk;
     if (kk < PER && c < total) {
@@ -520,13 +536,13 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
       const uint32_t inc = sinc[c];
       const int64_t e = inc >> 5;
       const int i = inc & 31;
-      const double *rec = a.rec + (int64_t)kRecStride * e;
-      const double thr = rec[12];
+      const Real *rec = tab_rec + (int64_t)kRecStride * e;
+      const double thr = (double)rec[12];
 #pragma unroll 1
       for (int jb = 0; jb < JB; ++jb) {
         const int j = jl + jb * LPI;
         Real v[Q][Q];
-        pair_block<Q, TERM, FORM, Real>(rec, i, j, ND, a.nq, a.gref, a.qw, a.R, v);
+        pair_block<Q, TERM, FORM, Real>(rec, i, j, ND, a.nq, tab_gref, tab_qw, tab_R, v);
         const int s = sslot[c * ND + j];
         for (int x = 0; x < Q; ++x)
           for (int y = 0; y < Q; ++y) {
@@ -697,10 +713,10 @@ static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); 
 hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,
                               const int32_t *cvx, const double *ca, double ca_c,
                               const double *cb, double cb_c, const double *thr,
-                              double *rec, int *err) {
+                              double *rec, float *rec32, int *err) {
   if (n_cv == 0) return hipSuccess;
   k_records_tet<<<nblocks(n_cv, 256), 256, 0, s>>>(n_cv, pts, cvx, ca, ca_c, cb, cb_c, thr,
-                                                    rec, err);
+                                                    rec, rec32, err);
   return hipGetLastError();
 }
 
