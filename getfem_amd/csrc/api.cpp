@@ -389,7 +389,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_coef[0]); dfree(p->d_coef[1]);
   dfree(p->d_gref); dfree(p->d_qw); dfree(p->d_R);
   dfree(p->d_rec); dfree(p->d_thr); dfree(p->d_keep);
-  dfree(p->d_gref32); dfree(p->d_qw32); dfree(p->d_R32); dfree(p->d_rec32);
+  dfree(p->d_gref32); dfree(p->d_qw32); dfree(p->d_R32); dfree(p->d_rec32); dfree(p->d_Rq);
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order); dfree(p->d_desc);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
@@ -402,6 +402,51 @@ int gfx_plan_destroy(gfx_plan *p) {
   delete p;
   unref(mim); unref(mf); unref(c);
   return GFX_OK;
+}
+
+// The reference tensor R_ij = sum_q w_q gradref_i gradref_j^T of an exact
+// rule is rational: R = N / L with small integers N (P1/IM_TETRAHEDRON(1):
+// L = 6, |N| <= 1; P2/(5): L = 30, |N| <= 8; P3/(8): |N| < 2^15).  Stored as
+// int8 (16-byte blocks) or int16 (32-byte blocks) it costs the patch kernel's
+// lanes 16 or 32 bytes per pair block instead of 72; the integers convert to
+// floating point exactly and 1/L is folded into the element's scale, so the
+// result differs from the FP64 table only by rounding.  rk = 0 when R has no
+// such form (a rule that does not integrate the products exactly).
+static void detect_int_reference(const std::vector<double> &R, int nblk, int &rk, double &L,
+                                 std::vector<uint8_t> &out) {
+  static const int kL[] = {1,    2,    3,    4,    5,    6,    8,    9,     10,    12,    15,
+                           16,   18,   20,   24,   30,   36,   40,   45,    48,    60,    72,
+                           80,   90,   96,   120,  144,  180,  240,  280,   315,   360,   420,
+                           480,  504,  560,  630,  720,  840,  1008, 1120,  1260,  1440,  1680,
+                           2016, 2240, 2520, 2880, 3360, 4032, 5040, 6720,  7560,  10080, 15120,
+                           20160, 30240, 40320, 60480, 80640, 120960, 181440, 362880};
+  rk = 0;
+  L = 1.0;
+  for (int Lc : kL) {
+    double nmax = 0.0;
+    bool ok = true;
+    for (double r : R) {
+      const double n = r * Lc, rn = std::nearbyint(n);
+      if (std::fabs(n - rn) > 1e-9 * std::max(1.0, std::fabs(n))) { ok = false; break; }
+      nmax = std::max(nmax, std::fabs(rn));
+    }
+    if (!ok) continue;
+    if (nmax > 32767.0) return;
+    rk = nmax <= 127.0 ? 1 : 2;
+    L = Lc;
+    const int bs = rk == 1 ? 16 : 32;
+    out.assign((size_t)nblk * bs, 0);
+    for (int b = 0; b < nblk; ++b)
+      for (int t = 0; t < 9; ++t) {
+        const int n = (int)std::nearbyint(R[(size_t)b * 9 + t] * Lc);
+        if (rk == 1) out[(size_t)b * 16 + t] = (uint8_t)(int8_t)n;
+        else {
+          const uint16_t u = (uint16_t)(int16_t)n;
+          std::memcpy(&out[(size_t)b * 32 + 2 * t], &u, 2);
+        }
+      }
+    return;
+  }
 }
 
 int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef *coefs,
@@ -474,6 +519,12 @@ int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef
   if (e == hipSuccess) e = hipMemcpy(p->d_qw, mim->rule.w.data(), 8 * (size_t)nq, hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemcpy(p->d_R, R.data(), 8 * R.size(), hipMemcpyHostToDevice);
   if (e == hipSuccess) e = hipMemset(p->d_err, 0, 4);
+  if (!eb) {  // integer reference tensor, if R has one
+    std::vector<uint8_t> rq;
+    detect_int_reference(R, nd * nd, p->rk, p->r_L, rq);
+    if (p->rk && e == hipSuccess) e = dalloc(&p->d_Rq, rq.size());
+    if (p->rk && e == hipSuccess) e = hipMemcpy(p->d_Rq, rq.data(), rq.size(), hipMemcpyHostToDevice);
+  }
   if (!eb) {  // FP32 copies of the tables for the FP32 contraction form
     std::vector<float> gref32(gref.begin(), gref.end()), R32(R.begin(), R.end());
     std::vector<float> qw32(mim->rule.w.begin(), mim->rule.w.end());
@@ -516,8 +567,9 @@ static int plan_records(gfx_plan *p, bool with_thr) {
                                p->coef[0].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[0] : nullptr,
                                p->coef[0].value,
                                p->coef[1].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[1] : nullptr,
-                               p->coef[1].value, with_thr ? p->d_thr : nullptr, p->d_rec,
-                               fp32 ? p->d_rec32 : nullptr, p->d_err));
+                               p->coef[1].value, with_thr ? p->d_thr : nullptr,
+                               p->rk ? 1.0 / p->r_L : 1.0, p->d_rec, fp32 ? p->d_rec32 : nullptr,
+                               p->d_err));
   return GFX_OK;
 }
 
@@ -830,6 +882,7 @@ int gfx_plan_assemble(gfx_plan *p) {
   a.nd = p->nd; a.qdim = p->qdim; a.nq = p->nq; a.rec = p->d_rec; a.gref = p->d_gref;
   a.qw = p->d_qw; a.R = p->d_R; a.vals = p->d_vals; a.max_inc = p->max_inc;
   a.rec32 = p->d_rec32; a.gref32 = p->d_gref32; a.qw32 = p->d_qw32; a.R32 = p->d_R32;
+  a.Rq = p->d_Rq; a.rk = p->rk;
   // The launches (patch plans: large-patch size classes and the packed
   // waves; element-batched plans: gather bins) write disjoint rows: they run
   // concurrently, each on its own stream (up to seven auxiliary streams forked

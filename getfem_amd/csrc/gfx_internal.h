@@ -86,6 +86,10 @@ struct PatchArgs {
   const double *R;          // nd*nd*9 weighted reference gradient products
   // the same in FP32, read by the FP32 contraction form
   const float *rec32, *gref32, *qw32, *R32;
+  // the reference tensor as integers N with R = N / L (rk 1: int8 in 16-byte
+  // blocks, 2: int16 in 32-byte blocks; 0: not available, R / R32 are used)
+  const void *Rq;
+  int rk;
   double *vals;             // CSR values
   int xcd_chunk;            // waves (patch kernel) or nodes (gather) per XCD in the block swizzle
   int max_inc;              // largest number of incidences of a node
@@ -152,7 +156,7 @@ void unref(gfx_mesh_im *mim);
 hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,
                               const int32_t *cvx, const double *ca, double ca_c,
                               const double *cb, double cb_c, const double *thr,
-                              double *rec, float *rec32, int *err);
+                              double r_scale, double *rec, float *rec32, int *err);
 hipError_t launch_incidences(hipStream_t s, int64_t n_cv, int nd, int qdim,
                              const int32_t *dofs, int64_t n_nodes,
                              int32_t *inc_ptr, uint32_t *inc, int32_t *tmp);
@@ -223,6 +227,11 @@ struct gfx_plan {
   double *d_rec = nullptr, *d_thr = nullptr;
   // FP32 contraction form: the tables and the element records in FP32
   float *d_gref32 = nullptr, *d_qw32 = nullptr, *d_R32 = nullptr, *d_rec32 = nullptr;
+  // R = N / r_L with small integers N (exact quadrature of polynomials): rk 1
+  // int8 / 2 int16 blocks in d_Rq, 0 if R has no such form
+  int rk = 0;
+  double r_L = 1.0;
+  uint8_t *d_Rq = nullptr;
   uint32_t *d_keep = nullptr;
   int64_t n_nodes = 0, n_inc = 0;
   int32_t *d_inc_ptr = nullptr;

@@ -46,7 +46,7 @@ __device__ __forceinline__ double wave_max(double v) {
 __global__ __launch_bounds__(256) void k_records_tet(
     int64_t n_cv, const double *__restrict__ pts, const int32_t *__restrict__ cvx,
     const double *__restrict__ ca, double ca_c, const double *__restrict__ cb,
-    double cb_c, const double *__restrict__ thr, double *__restrict__ rec,
+    double cb_c, const double *__restrict__ thr, double r_scale, double *__restrict__ rec,
     float *__restrict__ rec32, int *__restrict__ err) {
   const int64_t e = gtid();
   if (e >= n_cv) return;
@@ -67,16 +67,20 @@ __global__ __launch_bounds__(256) void k_records_tet(
   const double a2 = k[3] * k[7] - k[4] * k[6];
   const double J = k[0] * a0 + k[1] * a1 + k[2] * a2;
   if (J == 0.0) { atomicOr(err, 1); }
-  // B = K^{-T} from cofactors, stored row-major B(r,c) = o[3r+c], then |J|,
-  // the two coefficients and the insertion threshold
+  // Record: B = K^{-T} from cofactors, row-major B(r,c) = o[3r+c]; then the
+  // factorized form's scales |J| a s and |J| b s (s: the scale of the stored
+  // reference tensor, 1/L for an integer table R = N/L), the insertion
+  // threshold, and |J|, a, b for the per-point form
   double o[16];
   o[0] = a0 / J; o[1] = (k[2] * k[7] - k[1] * k[8]) / J;
   o[2] = (k[1] * k[5] - k[2] * k[4]) / J; o[3] = a1 / J;
   o[4] = (k[0] * k[8] - k[2] * k[6]) / J; o[5] = (k[2] * k[3] - k[0] * k[5]) / J;
   o[6] = a2 / J; o[7] = (k[1] * k[6] - k[0] * k[7]) / J;
-  o[8] = (k[0] * k[4] - k[1] * k[3]) / J; o[9] = fabs(J);
-  o[10] = ca ? ca[e] : ca_c; o[11] = cb ? cb[e] : cb_c;
-  o[12] = thr ? thr[e] : 0.0; o[13] = o[14] = o[15] = 0.0;
+  o[8] = (k[0] * k[4] - k[1] * k[3]) / J;
+  const double aJ = fabs(J), va = ca ? ca[e] : ca_c, vb = cb ? cb[e] : cb_c;
+  o[9] = aJ * va * r_scale; o[10] = aJ * vb * r_scale;
+  o[11] = thr ? thr[e] : 0.0;
+  o[12] = aJ; o[13] = va; o[14] = vb; o[15] = 0.0;
   if (rec32) {
     float4 *o4 = reinterpret_cast<float4 *>(rec32 + (int64_t)kRecStride * e);
     for (int t = 0; t < 4; ++t)
@@ -90,19 +94,43 @@ __global__ __launch_bounds__(256) void k_records_tet(
 // ------------------------------------------------------------------------
 // element pair blocks: rows (i, a), columns (j, b) of the element matrix
 // ------------------------------------------------------------------------
-template <int Q, int TERM, int FORM, typename Real>
+// RK: format of the reference tensor R for the factorized form: 0 = Real
+// (9 per block), 1 = int8 N with R = N/L (16-byte blocks), 2 = int16 N
+// (32-byte blocks); the scale 1/L is folded into the record's |J| a s, |J| b s.
+template <int RK>
+__device__ __forceinline__ void ref_block_int(const void *__restrict__ R, int64_t blk, int (&n)[9]) {
+  if constexpr (RK == 1) {
+    const int4 w = reinterpret_cast<const int4 *>(R)[blk];
+    const int ww[3] = {w.x, w.y, w.z};
+    for (int t = 0; t < 9; ++t) n[t] = (int)(int8_t)(ww[t >> 2] >> (8 * (t & 3)));
+  } else {
+    const int4 w0 = reinterpret_cast<const int4 *>(R)[2 * blk];
+    const int w1 = reinterpret_cast<const int *>(R)[8 * blk + 4];
+    const int ww[5] = {w0.x, w0.y, w0.z, w0.w, w1};
+    for (int t = 0; t < 9; ++t) n[t] = (int)(int16_t)(ww[t >> 1] >> (16 * (t & 1)));
+  }
+}
+
+template <int Q, int TERM, int FORM, typename Real, int RK = 0>
 __device__ __forceinline__ void pair_block(const Real *__restrict__ rec, int i, int j,
                                            int nd, int nq, const Real *__restrict__ gref,
                                            const Real *__restrict__ qw,
-                                           const Real *__restrict__ R, Real v[Q][Q]) {
+                                           const void *__restrict__ R, Real v[Q][Q]) {
   Real B[9];
   for (int t = 0; t < 9; ++t) B[t] = (Real)rec[t];
-  const Real aJ = (Real)rec[9], ca = (Real)rec[10], cb = (Real)rec[11];
   if (FORM == GFX_FORM_FACTORIZED) {
     // G = B R_ij B^T, R_ij = sum_q w_q gradref_i gradref_j^T
-    const Real *r = R + ((int64_t)i * nd + j) * 9;
+    const int64_t blk = (int64_t)i * nd + j;
     Real rr[9];
-    for (int t = 0; t < 9; ++t) rr[t] = (Real)r[t];
+    if constexpr (RK == 0) {
+      const Real *r = reinterpret_cast<const Real *>(R) + blk * 9;
+      for (int t = 0; t < 9; ++t) rr[t] = r[t];
+    } else {
+      int n[9];
+      ref_block_int<RK>(R, blk, n);
+      for (int t = 0; t < 9; ++t) rr[t] = (Real)n[t];
+    }
+    const Real sa = rec[9], sb = rec[10];  // |J| a s, |J| b s
     Real T[3][3], G[3][3];
     for (int c = 0; c < 3; ++c)
       for (int b = 0; b < 3; ++b)
@@ -114,12 +142,13 @@ __device__ __forceinline__ void pair_block(const Real *__restrict__ rec, int i, 
     if (TERM == GFX_TERM_ISO_ELAST) {
       for (int a = 0; a < Q; ++a)
         for (int b = 0; b < Q; ++b)
-          v[a][b] = aJ * (ca * G[a][b] + cb * G[b][a] + (a == b ? cb * tr : Real(0)));
+          v[a][b] = sa * G[a][b] + sb * (G[b][a] + (a == b ? tr : Real(0)));
     } else {
       for (int a = 0; a < Q; ++a)
-        for (int b = 0; b < Q; ++b) v[a][b] = (a == b) ? aJ * ca * tr : Real(0);
+        for (int b = 0; b < Q; ++b) v[a][b] = (a == b) ? sa * tr : Real(0);
     }
   } else {
+    const Real aJ = rec[12], ca = rec[13], cb = rec[14];
     for (int a = 0; a < Q; ++a)
       for (int b = 0; b < Q; ++b) v[a][b] = Real(0);
     for (int q = 0; q < nq; ++q) {
@@ -414,7 +443,7 @@ __device__ __forceinline__ T pick4(const T (&v)[kMaxNpw], int k) {
   return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
 }
 
-template <int ND, int Q, int TERM, int FORM, typename Real>
+template <int ND, int Q, int TERM, int FORM, typename Real, int RK>
 __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
   static_assert(ND % 2 == 0, "slot maps are staged as u16 pairs");
   extern __shared__ double lds[];
@@ -522,12 +551,14 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
   const int kk = lane / LPI, jl = lane - kk * LPI;
   // tables in the contraction's precision (the FP32 form reads FP32 records:
   // half the bytes per lane)
-  const Real *tab_rec, *tab_gref, *tab_qw, *tab_R;
+  const Real *tab_rec, *tab_gref, *tab_qw;
+  const void *tab_R;
   if constexpr (sizeof(Real) == 4) {
     tab_rec = a.rec32; tab_gref = a.gref32; tab_qw = a.qw32; tab_R = a.R32;
   } else {
     tab_rec = a.rec; tab_gref = a.gref; tab_qw = a.qw; tab_R = a.R;
   }
+  if constexpr (RK != 0) tab_R = a.Rq;  // integer reference tensor (both precisions)
   for (int c0 = 0; c0 < total; c0 += PER) {
     const int c = c0 + kk;
     if (kk < PER && c < total) {
@@ -537,12 +568,12 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
       const int64_t e = inc >> 5;
       const int i = inc & 31;
       const Real *rec = tab_rec + (int64_t)kRecStride * e;
-      const double thr = (double)rec[12];
+      const double thr = (double)rec[11];
 #pragma unroll 1
       for (int jb = 0; jb < JB; ++jb) {
         const int j = jl + jb * LPI;
         Real v[Q][Q];
-        pair_block<Q, TERM, FORM, Real>(rec, i, j, ND, a.nq, tab_gref, tab_qw, tab_R, v);
+        pair_block<Q, TERM, FORM, Real, RK>(rec, i, j, ND, a.nq, tab_gref, tab_qw, tab_R, v);
         const int s = sslot[c * ND + j];
         for (int x = 0; x < Q; ++x)
           for (int y = 0; y < Q; ++y) {
@@ -713,10 +744,10 @@ static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); 
 hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,
                               const int32_t *cvx, const double *ca, double ca_c,
                               const double *cb, double cb_c, const double *thr,
-                              double *rec, float *rec32, int *err) {
+                              double r_scale, double *rec, float *rec32, int *err) {
   if (n_cv == 0) return hipSuccess;
   k_records_tet<<<nblocks(n_cv, 256), 256, 0, s>>>(n_cv, pts, cvx, ca, ca_c, cb, cb_c, thr,
-                                                    rec, rec32, err);
+                                                    r_scale, rec, rec32, err);
   return hipGetLastError();
 }
 
@@ -780,16 +811,24 @@ hipError_t launch_fill_columns(hipStream_t s, const PatternArgs &a) {
 template <int ND, int Q, int TERM>
 static hipError_t launch_asm3(hipStream_t s, int form, int prec, const PatchArgs &a,
                               unsigned grid, size_t lds) {
-  if (form == GFX_FORM_FACTORIZED) {
-    if (prec == GFX_PREC_FP64)
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, double><<<grid, kWave, lds, s>>>(a);
-    else
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_FACTORIZED, float><<<grid, kWave, lds, s>>>(a);
+  constexpr int F = GFX_FORM_FACTORIZED, P = GFX_FORM_PERQP;
+  if (form == F) {
+    const bool d = prec == GFX_PREC_FP64;
+    if (a.rk == 1) {
+      if (d) k_assemble_patch<ND, Q, TERM, F, double, 1><<<grid, kWave, lds, s>>>(a);
+      else k_assemble_patch<ND, Q, TERM, F, float, 1><<<grid, kWave, lds, s>>>(a);
+    } else if (a.rk == 2) {
+      if (d) k_assemble_patch<ND, Q, TERM, F, double, 2><<<grid, kWave, lds, s>>>(a);
+      else k_assemble_patch<ND, Q, TERM, F, float, 2><<<grid, kWave, lds, s>>>(a);
+    } else {
+      if (d) k_assemble_patch<ND, Q, TERM, F, double, 0><<<grid, kWave, lds, s>>>(a);
+      else k_assemble_patch<ND, Q, TERM, F, float, 0><<<grid, kWave, lds, s>>>(a);
+    }
   } else {
     if (prec == GFX_PREC_FP64)
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, double><<<grid, kWave, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, P, double, 0><<<grid, kWave, lds, s>>>(a);
     else
-      k_assemble_patch<ND, Q, TERM, GFX_FORM_PERQP, float><<<grid, kWave, lds, s>>>(a);
+      k_assemble_patch<ND, Q, TERM, P, float, 0><<<grid, kWave, lds, s>>>(a);
   }
   return hipGetLastError();
 }
