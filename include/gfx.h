@@ -50,8 +50,9 @@ extern "C" {
                                  grad(phi_i) grad(phi_j)^T on the reference element */
 #define GFX_FORM_PERQP 1      /* per integration point B^T D B, as ga_exec does      */
 #define GFX_PREC_FP64 0       /* element contraction in FP64                         */
-#define GFX_PREC_FP32 1       /* element contraction in FP32; geometry, threshold and
-                                 global accumulation in FP64                          */
+#define GFX_PREC_FP32 1       /* element contraction in FP32 from FP32 element records
+                                 (B, scales, threshold rounded once per element);
+                                 geometry and global accumulation in FP64           */
 
 /* ---- coefficient sources ---- */
 #define GFX_COEF_CONST 0       /* one value                                 */
