@@ -41,6 +41,7 @@ CONFIGS = {
     "c3": (48, "FEM_QK(3,2)", 3, "IM_GAUSS_PARALLELEPIPED(3,5)", "neohookean", 27, 2, 513599625),
 }
 FP64_PEAK_TFLOPS = 78.6
+BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix peak
 # HBM bytes per assembly of the dominant kernel(s), measured with rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE (KiB per dispatch, summed over one assembly's
 # launches) at each config's default size; FETCH_SIZE is scaled by 2 and
@@ -225,6 +226,9 @@ def run_gpu(args):
         "forms": forms,
         "pattern": pattern,
     }
+    if not hexes:
+        out["mfma"] = {"used": False,
+                       "note": "3x3 pair blocks on the vector ALUs (FP64 matrix rate = vector rate on MI355X)"}
     if hexes:
         # the element kernel (per-point law + 81x81 contraction) is FP64-compute bound;
         # FLOPs per element of the per-point B^T D B model: 2 nq (n_loc^2 d + n_loc d^2), d = 9
@@ -239,6 +243,22 @@ def run_gpu(args):
                               "achieved_gbs": b_patch * n_cv / (head["ms_assemble_kernel"] * 1e-3) / 1e9},
         }
         out["dtype"] = "f64"
+        f32 = forms.get("fp32_element_batched")
+        if f32:
+            # the FP32 form's G^T T product on the matrix cores: per element 4 k-steps x 18
+            # (a, column tile) pairs x 2 row tiles x 6 bf16 products (bf16x3 split) of
+            # v_mfma_f32_16x16x32_bf16 (16384 FLOP each), padding included
+            n_mfma = 4 * 18 * 2 * 6
+            issued = n_mfma * 2 * 16 * 16 * 32
+            t32 = f32["ms_geometry_kernel"] * 1e-3
+            out["mfma"] = {
+                "form": "fp32_element_batched", "kernel": "k_nh_element",
+                "instruction": "v_mfma_f32_16x16x32_bf16", "mfma_per_element": n_mfma,
+                "issued_tflops": issued * n_cv / t32 / 1e12, "peak_tflops": BF16_PEAK_TFLOPS,
+                "frac": issued * n_cv / t32 / 1e12 / BF16_PEAK_TFLOPS,
+                "useful_fp32_flop_per_element": 2 * 3 * 27 * 81 * 81,
+                "kernel_ms": f32["ms_geometry_kernel"],
+            }
     return out, (pts, cvx, lam, mu, fem, q, im, term, state)
 
 
