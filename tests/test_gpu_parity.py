@@ -37,6 +37,10 @@ CASES = {
     "p1_elast_n5": lambda: problems.Case(5, 1, 3, "IM_TETRAHEDRON(1)", 1,
                                         lam=np.full(750, 1.3), mu=np.full(750, 0.4)),
     "p3_elast_n3": lambda: problems.c5(3),
+    # no integer form of the reference tensor (irrational points, inexact
+    # rule): the floating-point table path of the factorized form
+    "p3_elast_n2_im2": lambda: problems.Case(2, 3, 3, "IM_TETRAHEDRON(2)", 1,
+                                             lam=np.linspace(0.5, 2.0, 48), mu=np.linspace(1.5, 0.3, 48)),
 }
 
 
@@ -57,7 +61,7 @@ def test_fp64_parity(ctx, name, form):
     assert err <= TOL_FP64
 
 
-@pytest.mark.parametrize("name", ["p2_elast_n8", "c1_p1_poisson_n20", "p3_elast_n3"])
+@pytest.mark.parametrize("name", ["p2_elast_n8", "c1_p1_poisson_n20", "p3_elast_n3", "p3_elast_n2_im2"])
 @pytest.mark.parametrize("form", [ga.FORM_FACTORIZED, ga.FORM_PERQP])
 def test_fp32_element_error(ctx, name, form):
     case = CASES[name]()

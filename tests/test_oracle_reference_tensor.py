@@ -71,3 +71,16 @@ def test_p3_reference_tensor_has_an_int16_form():
             return
     pytest.fail("no common denominator found for the P3 reference tensor")
 
+
+
+def test_an_inexact_rule_with_irrational_points_has_no_integer_form():
+    """P3 under IM_TETRAHEDRON(2) (degree 2 < 4, points (5 -+ sqrt 5)/20):
+    R is not N/L for any candidate L, so the plan keeps the floating-point
+    table -- the case the GPU parity suite runs to cover that path."""
+    R, nd = reference_tensor(3, "IM_TETRAHEDRON(2)")
+    for L in (1, 2, 3, 4, 5, 6, 8, 9, 10, 12, 15, 16, 18, 20, 24, 30, 36, 40, 45, 48, 60, 72, 80, 90,
+              96, 120, 144, 180, 240, 280, 315, 360, 420, 480, 504, 560, 630, 720, 840, 1008, 1120,
+              1260, 1440, 1680, 2016, 2240, 2520, 2880, 3360, 4032, 5040, 6720, 7560, 10080, 15120,
+              20160, 30240, 40320, 60480, 80640, 120960, 181440, 362880):
+        N = R * L
+        assert not np.all(np.abs(N - np.rint(N)) <= 1e-9 * np.maximum(1.0, np.abs(N)))
