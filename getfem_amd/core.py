@@ -2,6 +2,7 @@
 (getfem.Mesh, getfem.MeshFem, getfem.MeshIm) plus the assembly Plan."""
 import ctypes
 import sys
+import weakref
 
 import numpy as np
 
@@ -148,6 +149,18 @@ def _coef(v, n_cv):
     return _lib.GfxCoef(_lib.COEF_PER_ELEMENT, 0.0, ctypes.c_void_p(a.ctypes.data)), a
 
 
+class _DeviceBuffer:
+    """A plan-owned device array exported through __cuda_array_interface__
+    (version 2: the consumer orders its work after the context's stream).
+    Holding the plan keeps the buffer allocated for as long as a tensor made
+    from it lives."""
+
+    def __init__(self, plan, ptr, n, typestr):
+        self.plan = plan
+        self.__cuda_array_interface__ = {"shape": (n,), "typestr": typestr, "data": (ptr, False),
+                                         "strides": None, "version": 2}
+
+
 class Plan:
     """One compiled order-2 term (gfx_plan): pattern + device CSR values."""
 
@@ -164,6 +177,7 @@ class Plan:
         check(lib().gfx_plan_create(mf.h, mim.h, term, arr, len(coefs), ctypes.byref(h)), self.ctx.h)
         self.h = h
         self.nnz = None
+        self._views = weakref.WeakSet()
 
     def set_coef(self, slot, value):
         c, a = _coef(value, self.mf.mesh.nbcvx)
@@ -173,6 +187,9 @@ class Plan:
         check(lib().gfx_plan_set_form(self.h, form, precision), self.ctx.h)
 
     def build_pattern(self):
+        if len(self._views):
+            raise RuntimeError("torch views of this plan's CSR are alive: a new pattern "
+                               "reallocates the arrays they point to")
         nnz = ctypes.c_int64()
         check(lib().gfx_plan_build_pattern(self.h, ctypes.byref(nnz)), self.ctx.h)
         self.nnz = nnz.value
@@ -267,6 +284,21 @@ class Plan:
         self.copy_csr_device(ip.data_ptr(), ix.data_ptr(), vv.data_ptr())
         self.ctx.synchronize()
         return ip, ix, vv
+
+    def torch_csr_view(self):
+        """The plan's own device CSR arrays as torch tensors, without a copy
+        (indptr int64, indices int32, values float64).  The values follow every
+        later assemble(); the tensors keep the plan alive, and build_pattern()
+        refuses to run while any of them does."""
+        import torch
+        n, nnz = self.dims()
+        ip, ix, vv = self.device_csr()
+        out = []
+        for ptr, cnt, ts in ((ip, n + 1, "<i8"), (ix, nnz, "<i4"), (vv, nnz, "<f8")):
+            buf = _DeviceBuffer(self, ptr, cnt, ts)
+            self._views.add(buf)
+            out.append(torch.as_tensor(buf, device="cuda"))
+        return tuple(out)
 
     def set_timing(self, enable=True):
         check(lib().gfx_plan_set_timing(self.h, 1 if enable else 0), self.ctx.h)
