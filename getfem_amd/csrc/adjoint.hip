@@ -123,8 +123,8 @@ __global__ __launch_bounds__(256) void k_compliance_elem(
 }
 
 // fixed-order compensated sum of the block partials (value, compensation)
-__global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double *__restrict__ part,
-                                                              int64_t n, double *__restrict__ out) {
+__device__ __forceinline__ void sum_partials_block(const double *__restrict__ part, int64_t n,
+                                                   double *__restrict__ out) {
   __shared__ double red[kRedThreads], redc[kRedThreads];
   double s = 0.0, c = 0.0;
   for (int64_t i = threadIdx.x; i < n; i += kRedThreads) two_sum_into(s, c, part[2 * i], part[2 * i + 1]);
@@ -139,11 +139,10 @@ __global__ __launch_bounds__(kRedThreads) void k_sum_partials(const double *__re
   if (threadIdx.x == 0) *out = red[0] + redc[0];
 }
 
-// out[node*stride + a] = sum over incidences (e, i) of c[(4e+i)*3 + a]
-__global__ __launch_bounds__(256) void k_gather3(int64_t n_nodes, const int32_t *__restrict__ inc_ptr,
-                                                 const uint32_t *__restrict__ inc,
-                                                 const double *__restrict__ c, double *__restrict__ out) {
-  const int64_t n = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+// out[node*3 + a] = sum over incidences (e, i) of c[(4e+i)*3 + a]
+__device__ __forceinline__ void gather3_node(int64_t n, int64_t n_nodes, const int32_t *__restrict__ inc_ptr,
+                                             const uint32_t *__restrict__ inc,
+                                             const double *__restrict__ c, double *__restrict__ out) {
   if (n >= n_nodes) return;
   double s0 = 0.0, s1 = 0.0, s2 = 0.0;
   for (int k = inc_ptr[n]; k < inc_ptr[n + 1]; ++k) {
@@ -158,6 +157,27 @@ __global__ __launch_bounds__(256) void k_gather3(int64_t n_nodes, const int32_t 
   out[3 * n + 2] = s2;
 }
 
+// The three reductions after the element kernel are independent, so they run
+// as one launch: blocks [0, nbx) gather dJ/dX per mesh point, the next nbu
+// blocks gather dJ/du per dof node, and the last block sums J.
+__global__ __launch_bounds__(kRedThreads) void k_compliance_finish(
+    int64_t nbx, int64_t nx, const int32_t *__restrict__ xptr, const uint32_t *__restrict__ xinc,
+    const double *__restrict__ cX, double *__restrict__ outX, int64_t nbu, int64_t nu,
+    const int32_t *__restrict__ uptr, const uint32_t *__restrict__ uinc, const double *__restrict__ cU,
+    double *__restrict__ outU, const double *__restrict__ Jpart, int64_t nbJ, double *__restrict__ J) {
+  int64_t b = blockIdx.x;
+  if (b < nbx) {
+    gather3_node(b * kRedThreads + threadIdx.x, nx, xptr, xinc, cX, outX);
+    return;
+  }
+  b -= nbx;
+  if (b < nbu) {
+    gather3_node(b * kRedThreads + threadIdx.x, nu, uptr, uinc, cU, outU);
+    return;
+  }
+  sum_partials_block(Jpart, nbJ, J);
+}
+
 }  // namespace
 
 int64_t compliance_nblocks(int64_t n_cv) { return (n_cv + 255) / 256; }
@@ -170,14 +190,18 @@ hipError_t launch_compliance(hipStream_t s, int64_t n_cv, const double *X, const
   if (nb == 0) return hipErrorInvalidValue;
   k_compliance_elem<<<(unsigned)nb, 256, 0, s>>>(n_cv, X, cvx, dofs, u, rho, lam, mu, emin, penal,
                                                  wsum, Jpart, drho, cX, cU, err);
-  k_sum_partials<<<1, kRedThreads, 0, s>>>(Jpart, nb, J);
   return hipGetLastError();
 }
 
-hipError_t launch_gather3(hipStream_t s, int64_t n_nodes, const int32_t *inc_ptr,
-                          const uint32_t *inc, const double *c, double *out) {
-  if (n_nodes == 0) return hipSuccess;
-  k_gather3<<<(unsigned)((n_nodes + 255) / 256), 256, 0, s>>>(n_nodes, inc_ptr, inc, c, out);
+hipError_t launch_compliance_finish(hipStream_t s, int64_t n_cv, int64_t nx, const int32_t *xptr,
+                                    const uint32_t *xinc, const double *cX, double *outX, int64_t nu,
+                                    const int32_t *uptr, const uint32_t *uinc, const double *cU,
+                                    double *outU, const double *Jpart, double *J) {
+  // a gather runs only when its output is requested (outX / outU non-null)
+  const int64_t nbx = outX ? (nx + kRedThreads - 1) / kRedThreads : 0;
+  const int64_t nbu = outU ? (nu + kRedThreads - 1) / kRedThreads : 0;
+  k_compliance_finish<<<(unsigned)(nbx + nbu + 1), kRedThreads, 0, s>>>(
+      nbx, nx, xptr, xinc, cX, outX, nbu, nu, uptr, uinc, cU, outU, Jpart, compliance_nblocks(n_cv), J);
   return hipGetLastError();
 }
 

@@ -1201,8 +1201,9 @@ int gfx_compliance_eval(gfx_compliance *c, const double *d_X, const double *d_rh
   HIPCHK(ctx, launch_compliance(s, c->n_cv, d_X ? d_X : m->d_pts, m->d_cvx, c->mf->d_dofs, d_u, d_rho,
                                 c->lam, c->mu, c->emin, c->penal, c->wsum, c->d_Jpart, d_J, d_dJ_drho,
                                 d_dJ_dX ? c->d_cX : nullptr, d_dJ_du ? c->d_cU : nullptr, c->d_err));
-  if (d_dJ_dX) HIPCHK(ctx, launch_gather3(s, c->n_pts, c->d_vinc_ptr, c->d_vinc, c->d_cX, d_dJ_dX));
-  if (d_dJ_du) HIPCHK(ctx, launch_gather3(s, c->ndof / 3, c->d_dinc_ptr, c->d_dinc, c->d_cU, d_dJ_du));
+  HIPCHK(ctx, launch_compliance_finish(s, c->n_cv, c->n_pts, c->d_vinc_ptr, c->d_vinc, c->d_cX, d_dJ_dX,
+                                       c->ndof / 3, c->d_dinc_ptr, c->d_dinc, c->d_cU, d_dJ_du,
+                                       c->d_Jpart, d_J));
   return GFX_OK;
 }
 

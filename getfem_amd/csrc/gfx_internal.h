@@ -198,8 +198,10 @@ hipError_t launch_compliance(hipStream_t s, int64_t n_cv, const double *X, const
                              const int32_t *dofs, const double *u, const double *rho, double lam,
                              double mu, double emin, double penal, double wsum, double *Jpart,
                              double *J, double *drho, double *cX, double *cU, int *err);
-hipError_t launch_gather3(hipStream_t s, int64_t n_nodes, const int32_t *inc_ptr,
-                          const uint32_t *inc, const double *c, double *out);
+hipError_t launch_compliance_finish(hipStream_t s, int64_t n_cv, int64_t nx, const int32_t *xptr,
+                                    const uint32_t *xinc, const double *cX, double *outX, int64_t nu,
+                                    const int32_t *uptr, const uint32_t *uinc, const double *cU,
+                                    double *outU, const double *Jpart, double *J);
 
 }  // namespace gfx
 
