@@ -22,7 +22,8 @@
  *    elasticity matrix, brute-force high-order quadrature on single
  *    elements, linearity in (lambda, mu).
  *  - orc_assemble_matrix: dense brute force (numpy) on small meshes, the
- *    stored-nnz counts the survey measured with the reference itself
+ *    stored-nnz counts GetFEM itself produced on the same inputs
+ *    (tests/golden/reference_counts.json)
  *    (P1 Poisson 20^3x6: 128,113 stored of 128,581 topological;
  *    P2 elasticity 8^3x6: 1,096,305 of 1,141,065).
  *  - orc_assemble_vector (Neo-Hookean residual): finite differences of the

@@ -2,8 +2,8 @@
 
 Pins: closed-form bases + brute-force collapsed Gauss quadrature on single
 elements, rigid-body modes, symmetry, linearity in (lambda, mu), dense
-brute-force global assembly, the stored-nnz counts the survey measured with
-the reference itself, the Neo-Hookean tangent against finite differences of
+brute-force global assembly, the stored-nnz counts GetFEM itself produced on
+the same inputs (tests/golden/reference_counts.json), the Neo-Hookean tangent against finite differences of
 the residual, and the compliance gradients against central differences."""
 import itertools
 
@@ -193,7 +193,7 @@ def test_global_assembly_bruteforce(mk):
 
 def test_reference_nnz_p1_poisson_20():
     """GetFEM's stored nnz for P1 Poisson on the jittered 20^3x6 mesh: 128,113
-    of 128,581 topological (survey, measured with the reference)."""
+    of 128,581 topological (measured with GetFEM on the same inputs)."""
     kat = problems.golden("p1_poisson_20cube_tet1")
     case = problems.c1(20)
     colptr, rows, vals = case.oracle_problem().assemble_csc()
@@ -208,7 +208,7 @@ def test_reference_nnz_p1_poisson_20():
 
 def test_reference_nnz_p2_elasticity_8():
     """P2 elasticity on 8^3x6: 44,760 of 1,141,065 topological entries dropped
-    (survey, measured with the reference)."""
+    (measured with GetFEM on the same inputs)."""
     kat = problems.golden("p2_elasticity_8cube_tet5")
     case = problems.c2(8)
     colptr, rows, vals = case.oracle_problem().assemble_csc()
