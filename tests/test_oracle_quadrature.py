@@ -1,7 +1,7 @@
 """Integration rules: the oracle and libgfx against the mathematics.
 
 Point counts are the reference's own (IM_TETRAHEDRON(1)=1, (5)=15, (8)=43,
-IM_GAUSS_PARALLELEPIPED(3,5)=27, measured with GetFEM in the survey); every
+IM_GAUSS_PARALLELEPIPED(3,5)=27, as GetFEM's tables in getfem_im_list.h give them); every
 monomial up to the rule degree must integrate exactly on the reference
 tetrahedron (a! b! c! / (a+b+c+3)!) or cube (1/((a+1)(b+1)(c+1)))."""
 from math import factorial
