@@ -48,8 +48,13 @@ __global__ __launch_bounds__(256) void k_records_tet(
     const double *__restrict__ ca, double ca_c, const double *__restrict__ cb,
     double cb_c, const double *__restrict__ thr, double r_scale, double *__restrict__ rec,
     float *__restrict__ rec32, int *__restrict__ err) {
-  const int64_t e = gtid();
-  if (e >= n_cv) return;
+  // The block's records are staged in LDS (rows padded by one 16-byte piece
+  // against bank conflicts) and written back as one contiguous run: 16-byte
+  // pieces of consecutive records from consecutive threads.
+  __shared__ double2 stage[256 * 9];
+  const int64_t e0 = (int64_t)blockIdx.x * 256, e = e0 + threadIdx.x;
+  const int nblk = (int)min<int64_t>(256, n_cv - e0);
+  if (e < n_cv) {
   const int4 v = *reinterpret_cast<const int4 *>(cvx + 4 * e);
   const double *p0 = pts + 3 * (int64_t)v.x, *p1 = pts + 3 * (int64_t)v.y;
   const double *p2 = pts + 3 * (int64_t)v.z, *p3 = pts + 3 * (int64_t)v.w;
@@ -82,12 +87,21 @@ __global__ __launch_bounds__(256) void k_records_tet(
   o[11] = thr ? thr[e] : 0.0;
   o[12] = aJ; o[13] = va; o[14] = vb; o[15] = 0.0;
   if (rec32) {
-    float4 *o4 = reinterpret_cast<float4 *>(rec32 + (int64_t)kRecStride * e);
+    float4 *st4 = reinterpret_cast<float4 *>(stage + 9 * threadIdx.x);
     for (int t = 0; t < 4; ++t)
-      o4[t] = make_float4((float)o[4 * t], (float)o[4 * t + 1], (float)o[4 * t + 2], (float)o[4 * t + 3]);
+      st4[t] = make_float4((float)o[4 * t], (float)o[4 * t + 1], (float)o[4 * t + 2], (float)o[4 * t + 3]);
   } else {
-    double2 *o2 = reinterpret_cast<double2 *>(rec + (int64_t)kRecStride * e);
-    for (int t = 0; t < 8; ++t) o2[t] = make_double2(o[2 * t], o[2 * t + 1]);
+    for (int t = 0; t < 8; ++t) stage[9 * threadIdx.x + t] = make_double2(o[2 * t], o[2 * t + 1]);
+  }
+  }
+  __syncthreads();
+  // 16-byte pieces: 8 per FP64 record, 4 per FP32 record
+  const int per = rec32 ? 4 : 8;
+  double2 *dst = rec32 ? reinterpret_cast<double2 *>(rec32 + (int64_t)kRecStride * e0)
+                       : reinterpret_cast<double2 *>(rec + (int64_t)kRecStride * e0);
+  for (int x = threadIdx.x; x < nblk * per; x += 256) {
+    const int r = x / per, piece = x - r * per;
+    dst[x] = stage[9 * r + piece];
   }
 }
 
