@@ -226,7 +226,8 @@ class Plan:
                                              v(d_rhs), flags), self.ctx.h)
 
     def solve_cg(self, d_b, d_x, rtol=1e-10, maxit=10000):
-        """Jacobi-preconditioned CG on the device: d_x (initial guess in,
+        """Block-Jacobi-preconditioned CG on the device (3x3 node blocks for a
+        vector field, the diagonal otherwise): d_x (initial guess in,
         solution out) and d_b are raw device addresses.  Returns (iterations,
         relative residual)."""
         it, res = ctypes.c_int(), ctypes.c_double()

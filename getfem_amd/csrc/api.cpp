@@ -968,8 +968,10 @@ int gfx_plan_solve_cg(gfx_plan *p, const double *d_b, double *d_x, double rtol, 
   if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern and assemble first");
   if (set_device(c)) return GFX_E_HIP;
   if (!p->d_cg) HIPCHK(c, dalloc(&p->d_cg, cg_workspace_doubles(p->ndof)));
+  // vector fields: 3x3 block Jacobi (a node's coupled components), else the diagonal
+  const int qb = (p->qdim == 3 && p->ndof % 3 == 0) ? 3 : 1;
   HIPCHK(c, cg_solve(c->stream, p->ndof, p->nnz, p->d_indptr, p->d_indices, p->d_vals, d_b, d_x, rtol, maxit,
-                     p->d_cg, iters, rel_res));
+                     qb, p->d_cg, iters, rel_res));
   return GFX_OK;
 }
 

@@ -83,17 +83,74 @@ __global__ void k_inv_diag(int64_t n, const int64_t *__restrict__ indptr,
   dinv[i] = d != 0.0 ? 1.0 / d : 0.0;
 }
 
-// r = b - r (r holds K x on entry), z = D^-1 r, p = z; partials of r.z, r.r, b.b
-__global__ __launch_bounds__(kRedThreads) void k_cg_start(int64_t n, const double *__restrict__ b,
-                                                          const double *__restrict__ dinv,
+// 3x3 block Jacobi for vector fields (dof = 3 node + component): the inverse
+// of each node's diagonal block; a singular block (e.g. rows cleared by a
+// Dirichlet simplification) falls back to the inverse diagonal of that node
+__global__ void k_inv_block3(int64_t nn, const int64_t *__restrict__ indptr,
+                             const int32_t *__restrict__ indices, const double *__restrict__ vals,
+                             double *__restrict__ minv) {
+  const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= nn) return;
+  double M[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+  for (int a = 0; a < 3; ++a) {
+    const int64_t row = 3 * u + a;
+    int64_t lo = indptr[row], hi = indptr[row + 1];
+    while (lo < hi) {  // first column >= 3u (sorted columns)
+      const int64_t mid = (lo + hi) / 2;
+      if (indices[mid] < 3 * u) lo = mid + 1;
+      else hi = mid;
+    }
+    for (int64_t k = lo; k < indptr[row + 1] && indices[k] < 3 * u + 3; ++k) M[a][indices[k] - 3 * u] = vals[k];
+  }
+  const double c00 = M[1][1] * M[2][2] - M[1][2] * M[2][1], c01 = M[1][2] * M[2][0] - M[1][0] * M[2][2],
+               c02 = M[1][0] * M[2][1] - M[1][1] * M[2][0];
+  const double det = M[0][0] * c00 + M[0][1] * c01 + M[0][2] * c02;
+  double *o = minv + 9 * u;
+  const double scale = fabs(M[0][0]) + fabs(M[1][1]) + fabs(M[2][2]);
+  if (det != 0.0 && fabs(det) > 1e-300 && fabs(det) > 1e-14 * scale * scale * scale) {
+    const double id = 1.0 / det;
+    o[0] = c00 * id; o[1] = (M[0][2] * M[2][1] - M[0][1] * M[2][2]) * id;
+    o[2] = (M[0][1] * M[1][2] - M[0][2] * M[1][1]) * id;
+    o[3] = c01 * id; o[4] = (M[0][0] * M[2][2] - M[0][2] * M[2][0]) * id;
+    o[5] = (M[0][2] * M[1][0] - M[0][0] * M[1][2]) * id;
+    o[6] = c02 * id; o[7] = (M[0][1] * M[2][0] - M[0][0] * M[2][1]) * id;
+    o[8] = (M[0][0] * M[1][1] - M[0][1] * M[1][0]) * id;
+  } else {
+    for (int t = 0; t < 9; ++t) o[t] = 0.0;
+    for (int a = 0; a < 3; ++a) o[4 * a] = M[a][a] != 0.0 ? 1.0 / M[a][a] : 0.0;
+  }
+}
+
+// z = M^-1 r on one unit: a dof (QB = 1, inverse diagonal) or a node's three
+// dofs (QB = 3, inverse 3x3 block)
+template <int QB>
+__device__ __forceinline__ void precond(const double *__restrict__ minv, int64_t u, const double (&r)[QB],
+                                        double (&z)[QB]) {
+  if constexpr (QB == 1) {
+    z[0] = minv[u] * r[0];
+  } else {
+    const double *M = minv + 9 * u;
+    for (int a = 0; a < 3; ++a) z[a] = M[3 * a] * r[0] + M[3 * a + 1] * r[1] + M[3 * a + 2] * r[2];
+  }
+}
+
+// r = b - r (r holds K x on entry), z = M^-1 r, p = z; partials of r.z, r.r, b.b
+template <int QB>
+__global__ __launch_bounds__(kRedThreads) void k_cg_start(int64_t nu, const double *__restrict__ b,
+                                                          const double *__restrict__ minv,
                                                           double *__restrict__ r, double *__restrict__ z,
                                                           double *__restrict__ p, double *__restrict__ part) {
   __shared__ double sh[kRedThreads];
   double rz = 0, rr = 0, bb = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (int64_t)kRedBlocks * kRedThreads) {
-    const double ri = b[i] - r[i], zi = dinv[i] * ri;
-    r[i] = ri; z[i] = zi; p[i] = zi;
-    rz += ri * zi; rr += ri * ri; bb += b[i] * b[i];
+  for (int64_t u = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; u < nu; u += (int64_t)kRedBlocks * kRedThreads) {
+    double ri[QB], zi[QB];
+    for (int a = 0; a < QB; ++a) ri[a] = b[QB * u + a] - r[QB * u + a];
+    precond<QB>(minv, u, ri, zi);
+    for (int a = 0; a < QB; ++a) {
+      const int64_t i = QB * u + a;
+      r[i] = ri[a]; z[i] = zi[a]; p[i] = zi[a];
+      rz += ri[a] * zi[a]; rr += ri[a] * ri[a]; bb += b[i] * b[i];
+    }
   }
   rz = block_sum(rz, sh); rr = block_sum(rr, sh); bb = block_sum(bb, sh);
   if (threadIdx.x == 0) {
@@ -135,21 +192,30 @@ __global__ __launch_bounds__(kRedThreads) void k_cg_scalars(const double *__rest
   }
 }
 
-// x += alpha p, r -= alpha q, z = D^-1 r; partials of r.z and r.r
-__global__ __launch_bounds__(kRedThreads) void k_cg_update(int64_t n, const CgScal *__restrict__ sc,
+// x += alpha p, r -= alpha q, z = M^-1 r; partials of r.z and r.r
+template <int QB>
+__global__ __launch_bounds__(kRedThreads) void k_cg_update(int64_t nu, const CgScal *__restrict__ sc,
                                                            const double *__restrict__ p,
                                                            const double *__restrict__ q,
-                                                           const double *__restrict__ dinv,
+                                                           const double *__restrict__ minv,
                                                            double *__restrict__ x, double *__restrict__ r,
                                                            double *__restrict__ z, double *__restrict__ part) {
   __shared__ double sh[kRedThreads];
   const double alpha = sc->alpha;
   double rz = 0, rr = 0;
-  for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (int64_t)kRedBlocks * kRedThreads) {
-    x[i] += alpha * p[i];
-    const double ri = r[i] - alpha * q[i], zi = dinv[i] * ri;
-    r[i] = ri; z[i] = zi;
-    rz += ri * zi; rr += ri * ri;
+  for (int64_t u = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; u < nu; u += (int64_t)kRedBlocks * kRedThreads) {
+    double ri[QB], zi[QB];
+    for (int a = 0; a < QB; ++a) {
+      const int64_t i = QB * u + a;
+      x[i] += alpha * p[i];
+      ri[a] = r[i] - alpha * q[i];
+    }
+    precond<QB>(minv, u, ri, zi);
+    for (int a = 0; a < QB; ++a) {
+      const int64_t i = QB * u + a;
+      r[i] = ri[a]; z[i] = zi[a];
+      rz += ri[a] * zi[a]; rr += ri[a] * ri[a];
+    }
   }
   rz = block_sum(rz, sh); rr = block_sum(rr, sh);
   if (threadIdx.x == 0) { part[blockIdx.x] = rz; part[kRedBlocks + blockIdx.x] = rr; }
@@ -222,21 +288,26 @@ hipError_t launch_spmv(hipStream_t s, int64_t n, int64_t nnz, const int64_t *ind
   return spmv_g<32>(s, n, indptr, indices, vals, x, y);
 }
 
-size_t cg_workspace_doubles(int64_t n) { return 5 * (size_t)n + 3 * kRedBlocks + 8; }
+// preconditioner storage (3 n: room for the 3x3 blocks) + r, z, p, q + partials
+size_t cg_workspace_doubles(int64_t n) { return 7 * (size_t)n + 3 * kRedBlocks + 8; }
 
 hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
-                    const double *vals, const double *b, double *x, double rtol, int maxit, double *work,
-                    int *iters, double *rel_res) {
+                    const double *vals, const double *b, double *x, double rtol, int maxit, int qb,
+                    double *work, int *iters, double *rel_res) {
   *iters = 0;
   *rel_res = 0.0;
   if (n == 0) return hipSuccess;
-  double *dinv = work, *r = work + n, *z = work + 2 * n, *p = work + 3 * n, *q = work + 4 * n;
-  double *part = work + 5 * n;
+  if (qb != 1 && (qb != 3 || n % 3 != 0)) return hipErrorInvalidValue;
+  double *minv = work, *r = work + 3 * n, *z = work + 4 * n, *p = work + 5 * n, *q = work + 6 * n;
+  double *part = work + 7 * n;
   CgScal *sc = reinterpret_cast<CgScal *>(part + 3 * kRedBlocks);
+  const int64_t nu = n / qb;
   hipError_t e;
-  k_inv_diag<<<nb256(n), 256, 0, s>>>(n, indptr, indices, vals, dinv);
+  if (qb == 3) k_inv_block3<<<nb256(nu), 256, 0, s>>>(nu, indptr, indices, vals, minv);
+  else k_inv_diag<<<nb256(n), 256, 0, s>>>(n, indptr, indices, vals, minv);
   if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, x, r)) != hipSuccess) return e;
-  k_cg_start<<<kRedBlocks, kRedThreads, 0, s>>>(n, b, dinv, r, z, p, part);
+  if (qb == 3) k_cg_start<3><<<kRedBlocks, kRedThreads, 0, s>>>(nu, b, minv, r, z, p, part);
+  else k_cg_start<1><<<kRedBlocks, kRedThreads, 0, s>>>(nu, b, minv, r, z, p, part);
   k_cg_scalars<<<1, kRedThreads, 0, s>>>(part, 0, sc);
   CgScal h{};
   if ((e = hipMemcpyAsync(&h, sc, sizeof h, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
@@ -252,7 +323,8 @@ hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr
     if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, p, q)) != hipSuccess) return e;
     k_dot<<<kRedBlocks, kRedThreads, 0, s>>>(n, p, q, part);
     k_cg_scalars<<<1, kRedThreads, 0, s>>>(part, 1, sc);
-    k_cg_update<<<kRedBlocks, kRedThreads, 0, s>>>(n, sc, p, q, dinv, x, r, z, part);
+    if (qb == 3) k_cg_update<3><<<kRedBlocks, kRedThreads, 0, s>>>(nu, sc, p, q, minv, x, r, z, part);
+    else k_cg_update<1><<<kRedBlocks, kRedThreads, 0, s>>>(nu, sc, p, q, minv, x, r, z, part);
     k_cg_scalars<<<1, kRedThreads, 0, s>>>(part, 2, sc);
     k_cg_direction<<<nb256(n), 256, 0, s>>>(n, sc, z, p);
     if ((e = hipGetLastError()) != hipSuccess) return e;

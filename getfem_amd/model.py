@@ -7,7 +7,7 @@ device by the workspace (ga_workspace inside model::assembly,
 src/getfem_models.cc:2646-2736, one device call instead of one per OpenMP
 partition); the dof constraints are applied to the device CSR
 (src/getfem_models.cc:2806-2870); the linear systems are solved on the device
-by Jacobi-preconditioned CG; the Newton loop follows classical_Newton
+by block-Jacobi-preconditioned CG; the Newton loop follows classical_Newton
 (src/getfem/getfem_model_solvers.h:548-605) with basic_newton_line_search
 (:336-360).  The state vector stays on the device between iterations; the
 host sees scalars only (and one copy of the state to the workspace per

@@ -185,11 +185,12 @@ int gfx_plan_copy_csr_device(gfx_plan *p, int64_t *d_indptr, int32_t *d_indices,
  * variable value that model::assembly adds to the residual
  * (gmm::mult_add(matj, vec_linear2, ...), src/getfem_models.cc:2560-2575) */
 int gfx_plan_spmv(gfx_plan *p, const double *d_x, double *d_y);
-/* Solve K x = b on the device by the conjugate gradient method with a Jacobi
- * (diagonal) preconditioner, for the coercive (symmetric positive definite)
- * systems GetFEM sends to gmm::cg (default_linear_solver picks
- * linear_solver_cg_preconditioned_ildlt, src/getfem/getfem_model_solvers.h:
- * 90-98, :652-655; the preconditioner here is the diagonal, not ILDLT).
+/* Solve K x = b on the device by the preconditioned conjugate gradient
+ * method, for the coercive (symmetric positive definite) systems GetFEM sends
+ * to gmm::cg (default_linear_solver picks linear_solver_cg_preconditioned_ildlt,
+ * src/getfem/getfem_model_solvers.h:90-98, :652-655; the preconditioner here is
+ * block Jacobi, not ILDLT: the inverse 3x3 diagonal block of each node for a
+ * qdim-3 field, the inverse diagonal otherwise).
  * Stopping test as gmm::cg (src/gmm/gmm_solver_cg.h:85-120): ||b - K x|| <=
  * rtol ||b||; b = 0 gives x = 0.  d_x: initial guess in, solution out (device,
  * nb_dof doubles).  iters and rel_res receive the iteration count and the

@@ -1,5 +1,5 @@
 """Sparse operations on the assembled device CSR: the product K x against the
-oracle's CSC, and the Jacobi-preconditioned CG solve (gmm::cg's stopping test
+oracle's CSC, and the block-Jacobi-preconditioned CG solve (gmm::cg's stopping test
 ||b - K x|| <= rtol ||b||, checked on the host with the oracle's matrix)."""
 import numpy as np
 import pytest
