@@ -55,44 +55,44 @@ __global__ __launch_bounds__(256) void k_records_tet(
   const int64_t e0 = (int64_t)blockIdx.x * 256, e = e0 + threadIdx.x;
   const int nblk = (int)min<int64_t>(256, n_cv - e0);
   if (e < n_cv) {
-  const int4 v = *reinterpret_cast<const int4 *>(cvx + 4 * e);
-  const double *p0 = pts + 3 * (int64_t)v.x, *p1 = pts + 3 * (int64_t)v.y;
-  const double *p2 = pts + 3 * (int64_t)v.z, *p3 = pts + 3 * (int64_t)v.w;
-  // column-major K(r,c) = k[r + 3c] = x_{c+1}[r] - x_0[r]  (G . grad N for
-  // the P1 transformation)
-  double k[9];
-  for (int r = 0; r < 3; ++r) {
-    const double x0 = p0[r];
-    k[r] = p1[r] - x0;
-    k[r + 3] = p2[r] - x0;
-    k[r + 6] = p3[r] - x0;
-  }
-  const double a0 = k[4] * k[8] - k[5] * k[7];
-  const double a1 = k[5] * k[6] - k[3] * k[8];
-  const double a2 = k[3] * k[7] - k[4] * k[6];
-  const double J = k[0] * a0 + k[1] * a1 + k[2] * a2;
-  if (J == 0.0) { atomicOr(err, 1); }
-  // Record: B = K^{-T} from cofactors, row-major B(r,c) = o[3r+c]; then the
-  // factorized form's scales |J| a s and |J| b s (s: the scale of the stored
-  // reference tensor, 1/L for an integer table R = N/L), the insertion
-  // threshold, and |J|, a, b for the per-point form
-  double o[16];
-  o[0] = a0 / J; o[1] = (k[2] * k[7] - k[1] * k[8]) / J;
-  o[2] = (k[1] * k[5] - k[2] * k[4]) / J; o[3] = a1 / J;
-  o[4] = (k[0] * k[8] - k[2] * k[6]) / J; o[5] = (k[2] * k[3] - k[0] * k[5]) / J;
-  o[6] = a2 / J; o[7] = (k[1] * k[6] - k[0] * k[7]) / J;
-  o[8] = (k[0] * k[4] - k[1] * k[3]) / J;
-  const double aJ = fabs(J), va = ca ? ca[e] : ca_c, vb = cb ? cb[e] : cb_c;
-  o[9] = aJ * va * r_scale; o[10] = aJ * vb * r_scale;
-  o[11] = thr ? thr[e] : 0.0;
-  o[12] = aJ; o[13] = va; o[14] = vb; o[15] = 0.0;
-  if (rec32) {
-    float4 *st4 = reinterpret_cast<float4 *>(stage + 9 * threadIdx.x);
-    for (int t = 0; t < 4; ++t)
-      st4[t] = make_float4((float)o[4 * t], (float)o[4 * t + 1], (float)o[4 * t + 2], (float)o[4 * t + 3]);
-  } else {
-    for (int t = 0; t < 8; ++t) stage[9 * threadIdx.x + t] = make_double2(o[2 * t], o[2 * t + 1]);
-  }
+    const int4 v = *reinterpret_cast<const int4 *>(cvx + 4 * e);
+    const double *p0 = pts + 3 * (int64_t)v.x, *p1 = pts + 3 * (int64_t)v.y;
+    const double *p2 = pts + 3 * (int64_t)v.z, *p3 = pts + 3 * (int64_t)v.w;
+    // column-major K(r,c) = k[r + 3c] = x_{c+1}[r] - x_0[r]  (G . grad N for
+    // the P1 transformation)
+    double k[9];
+    for (int r = 0; r < 3; ++r) {
+      const double x0 = p0[r];
+      k[r] = p1[r] - x0;
+      k[r + 3] = p2[r] - x0;
+      k[r + 6] = p3[r] - x0;
+    }
+    const double a0 = k[4] * k[8] - k[5] * k[7];
+    const double a1 = k[5] * k[6] - k[3] * k[8];
+    const double a2 = k[3] * k[7] - k[4] * k[6];
+    const double J = k[0] * a0 + k[1] * a1 + k[2] * a2;
+    if (J == 0.0) { atomicOr(err, 1); }
+    // Record: B = K^{-T} from cofactors, row-major B(r,c) = o[3r+c]; then the
+    // factorized form's scales |J| a s and |J| b s (s: the scale of the stored
+    // reference tensor, 1/L for an integer table R = N/L), the insertion
+    // threshold, and |J|, a, b for the per-point form
+    double o[16];
+    o[0] = a0 / J; o[1] = (k[2] * k[7] - k[1] * k[8]) / J;
+    o[2] = (k[1] * k[5] - k[2] * k[4]) / J; o[3] = a1 / J;
+    o[4] = (k[0] * k[8] - k[2] * k[6]) / J; o[5] = (k[2] * k[3] - k[0] * k[5]) / J;
+    o[6] = a2 / J; o[7] = (k[1] * k[6] - k[0] * k[7]) / J;
+    o[8] = (k[0] * k[4] - k[1] * k[3]) / J;
+    const double aJ = fabs(J), va = ca ? ca[e] : ca_c, vb = cb ? cb[e] : cb_c;
+    o[9] = aJ * va * r_scale; o[10] = aJ * vb * r_scale;
+    o[11] = thr ? thr[e] : 0.0;
+    o[12] = aJ; o[13] = va; o[14] = vb; o[15] = 0.0;
+    if (rec32) {
+      float4 *st4 = reinterpret_cast<float4 *>(stage + 9 * threadIdx.x);
+      for (int t = 0; t < 4; ++t)
+        st4[t] = make_float4((float)o[4 * t], (float)o[4 * t + 1], (float)o[4 * t + 2], (float)o[4 * t + 3]);
+    } else {
+      for (int t = 0; t < 8; ++t) stage[9 * threadIdx.x + t] = make_double2(o[2 * t], o[2 * t + 1]);
+    }
   }
   __syncthreads();
   // 16-byte pieces: 8 per FP64 record, 4 per FP32 record
