@@ -5,6 +5,7 @@ import numpy as np
 import pytest
 
 import getfem_amd as ga
+import oracle
 from getfem_amd import meshes
 from tests import problems
 
@@ -103,8 +104,9 @@ def test_repeated_assembly_is_stable(ctx):
 ])
 def test_full_size_stored_nnz_equals_reference(ctx, name, n, fem, im, nd):
     """At full size the stored pattern has exactly the entry count GetFEM
-    itself produced on the same inputs, and the columns of every row are
-    strictly increasing."""
+    itself produced on the same inputs, the columns of every row are strictly
+    increasing, and 48 sampled rows equal the rows rebuilt from the oracle's
+    element matrices (columns bit-exact, values within the FP64 tolerance)."""
     import torch
     kat = problems.golden(name)
     pts, cvx = meshes.regular_unit_mesh(n)
@@ -118,9 +120,43 @@ def test_full_size_stored_nnz_equals_reference(ctx, name, n, fem, im, nd):
     plan.assemble()
     print(name, "nnz", nnz, "ndof", mf.nbdof(), flush=True)
     assert nnz == kat["stored_nnz"] and mf.nbdof() == kat["ndof"]
-    ip, ix, v = plan.torch_csr()
+    ip, ix, v = plan.torch_csr_view()
     d = ix[1:].to(torch.int64) - ix[:-1].to(torch.int64)
     row_start = torch.zeros(ix.numel(), dtype=torch.bool, device=ix.device)
     row_start[ip[:-1][ip[:-1] < ix.numel()]] = True
     assert bool(((d > 0) | row_start[1:]).all())  # strictly increasing inside each row
     assert bool(torch.isfinite(v).all())
+    # Values at full size: sampled rows rebuilt from the oracle's element
+    # matrices of the elements around the row's node, each element's entries
+    # kept above its own 1e-14 max|K_e| (compile_and_exec.cc:5441-5453).
+    k = {"FEM_PK(3,2)": 2, "FEM_PK(3,3)": 3}[fem]
+    dofs = mf.basic_dof_table()
+    D = np.asarray(dofs).reshape(cvx.shape[0], nd)
+    P = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, k, 3, dofs, mf.nbdof(), im,
+                       oracle.TERM_ISO_ELAST, coef_a=lam, coef_b=mu)
+    rows = np.random.default_rng(7).choice(mf.nbdof(), 48, replace=False)
+    ip_h = ip.cpu().numpy()
+    worst = 0.0
+    for row in rows:
+        node_dof, comp = row - row % 3, row % 3
+        acc = {}
+        for e in np.nonzero((D == node_dof).any(axis=1))[0]:
+            Ke = P.element_matrix(int(e))
+            thr = 1e-14 * np.abs(Ke).max()
+            li = int(np.nonzero(D[e] == node_dof)[0][0])
+            krow = Ke[3 * li + comp]
+            for j in range(nd):
+                for b in range(3):
+                    val = krow[3 * j + b]
+                    if abs(val) > thr:
+                        col = int(D[e, j]) + b
+                        acc[col] = acc.get(col, 0.0) + val
+        cols_ref = np.array(sorted(acc))
+        vals_ref = np.array([acc[c] for c in cols_ref])
+        s0, s1 = int(ip_h[row]), int(ip_h[row + 1])
+        cols = ix[s0:s1].cpu().numpy().astype(np.int64)
+        vals = v[s0:s1].cpu().numpy()
+        assert np.array_equal(cols, cols_ref), "row %d: columns differ" % row
+        worst = max(worst, np.linalg.norm(vals - vals_ref) / np.linalg.norm(vals_ref))
+    print(name, "sampled rows", len(rows), "worst rel error", worst, flush=True)
+    assert worst <= TOL_FP64
