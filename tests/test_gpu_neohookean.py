@@ -64,7 +64,10 @@ def test_neohookean_per_element_coefficients(ctx):
 
 
 def test_neohookean_full_size_pattern(ctx):
-    import torch
+    """48^3 hexes: the reference's stored nnz, and 32 sampled tangent rows and
+    residual entries equal to those rebuilt from the oracle's element
+    matrices and vectors of the elements around each row's node (each
+    element's matrix entries kept above its own 1e-14 max|K_e|)."""
     pts, cvx, mf, plan, U, keep = nh_setup(ctx, 48)
     nnz = plan.build_pattern()
     plan.assemble()
@@ -73,6 +76,36 @@ def test_neohookean_full_size_pattern(ctx):
     assert nnz == kat["stored_nnz"] and mf.nbdof() == kat["ndof"]
     R = plan.residual()
     assert np.all(np.isfinite(R))
+    dofs = mf.basic_dof_table()
+    D = np.asarray(dofs)
+    nd = D.shape[1]
+    P = oracle.Problem(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, 2, 3, dofs, mf.nbdof(),
+                       "IM_GAUSS_PARALLELEPIPED(3,5)", oracle.TERM_NEOHOOKEAN, scal_a=10.0, scal_b=1.0, U=U)
+    ip, ix, v = plan.torch_csr_view()
+    ip_h = ip.cpu().numpy()
+    worst = worst_r = 0.0
+    for row in np.random.default_rng(3).choice(mf.nbdof(), 32, replace=False):
+        node_dof, comp = row - row % 3, row % 3
+        acc, res = {}, 0.0
+        for e in np.nonzero((D == node_dof).any(axis=1))[0]:
+            Ke = P.element_matrix(int(e))
+            thr = 1e-14 * np.abs(Ke).max()
+            li = int(np.nonzero(D[e] == node_dof)[0][0])
+            res += P.element_vector(int(e))[3 * li + comp]
+            krow = Ke[3 * li + comp]
+            for j in range(nd):
+                for b in range(3):
+                    if abs(krow[3 * j + b]) > thr:
+                        col = int(D[e, j]) + b
+                        acc[col] = acc.get(col, 0.0) + krow[3 * j + b]
+        cols_ref = np.array(sorted(acc))
+        vals_ref = np.array([acc[c] for c in cols_ref])
+        s0, s1 = int(ip_h[row]), int(ip_h[row + 1])
+        assert np.array_equal(ix[s0:s1].cpu().numpy().astype(np.int64), cols_ref), "row %d" % row
+        worst = max(worst, np.linalg.norm(v[s0:s1].cpu().numpy() - vals_ref) / np.linalg.norm(vals_ref))
+        worst_r = max(worst_r, abs(R[row] - res) / max(abs(res), 1e-300 + np.abs(R).max() * 1e-12))
+    print("C3 sampled rows: worst tangent rel error", worst, "residual", worst_r, flush=True)
+    assert worst <= 1e-12 and worst_r <= 1e-12
 
 
 def test_neohookean_fp32_element_form(ctx):
