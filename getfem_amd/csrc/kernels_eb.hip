@@ -61,6 +61,18 @@ __device__ __forceinline__ f32x4 mfma_bf16x3(const bf16x8 &ah, const bf16x8 &am,
   return acc;
 }
 
+// maximum over the workgroup's threads: shuffles within each wavefront, then
+// the wavefronts' maxima through LDS (one barrier instead of a tree)
+__device__ __forceinline__ double block_max(double v, double *red) {
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  const int t = threadIdx.x;
+  if ((t & 63) == 0) red[t >> 6] = v;
+  __syncthreads();
+  double m = red[0];
+  for (int w = 1; w < kEbThreads / 64; ++w) m = fmax(m, red[w]);
+  return m;
+}
+
 // Phase B of the FP32 form on the matrix cores.  For each a:
 //   K_a[i][(j,b)] = sum_k G[i][k] T_a[k][(j,b)],  k = (q, c),
 //   G[i][k] = g[q][i][c],  T_a[k][(j,b)] = sum_n D_q[(a,c),(b,n)] g[q][j][n],
@@ -179,13 +191,8 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, double lam, do
         }
     }
   }
-  red[t] = mx;
-  __syncthreads();
-  for (int o = kEbThreads / 2; o > 0; o >>= 1) {
-    if (t < o) red[t] = fmax(red[t], red[t + o]);
-    __syncthreads();
-  }
-  if (t == 0) a.thr[e] = red[0] > 0.0 ? 1e-14 * red[0] : INFINITY;
+  const double mxe = block_max(mx, red);
+  if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
   // E (FP32: the form's values are FP32 already) in three slabs of 27
   // contiguous rows (row 3 i + a, i in 9 p..9 p+8), staged through LDS and
   // streamed with consecutive addresses across lanes
@@ -220,7 +227,6 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   constexpr bool kLow = !std::is_same<Real, double>::value;
   __shared__ float4 g4[kLow ? kNq : 1][kNd];  // FP32 form: the gradients as float4 (w = 0)
   Real (*gr)[kNd][3] = reinterpret_cast<Real (*)[kNd][3]>(gq);  // FP64 form
-  __shared__ double Ac[kLow ? 1 : kQc][81];
   __shared__ Real Dc[kLow ? 1 : kQc][81];
   __shared__ double Ac32[kLow ? kQm : 1][81];  // FP32 form: a k-step's material tangents
   __shared__ float4 Dc32[kLow ? kQm : 1][27];
@@ -343,26 +349,24 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     for (int r = 0; r < 3; ++r)
       for (int m = 0; m < 9; ++m) K[r][m] = 0.0;
     for (int q0 = 0; q0 < kNq; q0 += kQc) {
-      // B0a: material tangent A[k c m n] = dS_kc/dE_mn of the chunk's points
+      // B0: D_q[(a,c),(b,n)] = wJ_q (delta_ab S_nc + sum_km F_ak A[k c m n] F_bm), with the
+      // material tangent A[k c m n] = dS_kc/dE_mn evaluated in place (no LDS round trip):
       //   A = 2 (-mu dCi + lambda/2 (Ci_mn Ci_kc + ln(I3) dCi)),
       //   dCi = dCi_kc/dC_mn = -(Ci_km Ci_nc + Ci_kn Ci_mc) / 2
       for (int x = t; x < kQc * 81; x += kEbThreads) {
-        const int qq = x / 81, q = q0 + qq, r = x % 81;
-        const int k = r / 27, c = (r / 9) % 3, m = (r / 3) % 3, n = r % 3;
-        const double *Ci = Cq[q];
-        const double dCi = -0.5 * (Ci[3 * k + m] * Ci[3 * n + c] + Ci[3 * k + n] * Ci[3 * m + c]);
-        Ac[qq][r] = 2.0 * (-mu * dCi + 0.5 * lam * (Ci[3 * m + n] * Ci[3 * k + c] + l3[q] * dCi));
-      }
-      __syncthreads();
-      // B0b: D_q[(a,c),(b,n)] = wJ_q (delta_ab S_nc + sum_km F_ak A[k c m n] F_bm)
-      for (int x = t; x < kQc * 81; x += kEbThreads) {
         const int qq = x / 81, q = q0 + qq, ac = (x % 81) / 9, bn = x % 9;
         const int aa = ac / 3, c = ac % 3, bb = bn / 3, n = bn % 3;
-        const double *F = Fq[q], *A = Ac[qq];
+        const double *F = Fq[q], *Ci = Cq[q];
+        const double lq = l3[q];
         double s = (aa == bb) ? Sq[q][3 * n + c] : 0.0;
         for (int k = 0; k < 3; ++k) {
-          const double *Ak = A + 27 * k + 9 * c + n;  // A[k c m n] at Ak[3 m]
-          s += F[3 * aa + k] * (Ak[0] * F[3 * bb] + Ak[3] * F[3 * bb + 1] + Ak[6] * F[3 * bb + 2]);
+          double sk = 0.0;
+          for (int m = 0; m < 3; ++m) {
+            const double dCi = -0.5 * (Ci[3 * k + m] * Ci[3 * n + c] + Ci[3 * k + n] * Ci[3 * m + c]);
+            const double A = 2.0 * (-mu * dCi + 0.5 * lam * (Ci[3 * m + n] * Ci[3 * k + c] + lq * dCi));
+            sk += A * F[3 * bb + m];
+          }
+          s += F[3 * aa + k] * sk;
         }
         Dc[qq][x % 81] = (Real)(wJ[q] * s);
       }
@@ -401,13 +405,8 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     double mx = 0.0;
     for (int r = 0; r < 3; ++r)
       for (int m = 0; m < 9; ++m) mx = fmax(mx, fabs((double)K[r][m]));
-    red[t] = owner ? mx : 0.0;
-    __syncthreads();
-    for (int o = kEbThreads / 2; o > 0; o >>= 1) {
-      if (t < o) red[t] = fmax(red[t], red[t + o]);
-      __syncthreads();
-    }
-    if (t == 0) a.thr[e] = red[0] > 0.0 ? 1e-14 * red[0] : INFINITY;
+    const double mxe = block_max(owner ? mx : 0.0, red);
+    if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
     // E is streamed to HBM (GBs, read back once by the gather) in three slabs of
     // 27 contiguous rows (rows 27p..27p+26 = i in 9p..9p+8, owned by ib in
     // 3p..3p+2), staged through LDS so that each store instruction writes
