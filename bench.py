@@ -128,6 +128,8 @@ def run_gpu(args):
                 ("fp64_perqp", ga.FORM_PERQP, ga.PREC_FP64),
                 ("fp32_factorized", ga.FORM_FACTORIZED, ga.PREC_FP32),
                 ("fp32_perqp", ga.FORM_PERQP, ga.PREC_FP32)]
+    if term != "laplace" and not hexes and n_nodes in (10, 20):
+        variants.append(("bf16x2_element_batched", ga.FORM_FACTORIZED, ga.PREC_BF16X2))
     if hexes:
         variants = [("fp64_element_batched", ga.FORM_PERQP, ga.PREC_FP64),
                     ("fp32_element_batched", ga.FORM_PERQP, ga.PREC_FP32)]
@@ -229,6 +231,27 @@ def run_gpu(args):
     if not hexes:
         out["mfma"] = {"used": False,
                        "note": "3x3 pair blocks on the vector ALUs (FP64 matrix rate = vector rate on MI355X)"}
+        bdb = forms.get("bf16x2_element_batched")
+        if bdb:
+            # element matrices B^T D B on the matrix cores: ceil(nq/4) k-steps of 32
+            # (4 points x 8 Voigt rows, 2 of them zero) x (n_loc/16 rounded up)^2
+            # output tiles x 3 bf16 products of v_mfma_f32_16x16x32_bf16 (16384 FLOP each)
+            nq = mim.points()[1].size
+            nt = -(-n_loc // 16)
+            n_mfma = -(-nq // 4) * nt * nt * 3
+            issued = n_mfma * 2 * 16 * 16 * 32
+            t_el = bdb["ms_geometry_kernel"] * 1e-3
+            out["mfma"] = {
+                "form": "bf16x2_element_batched", "kernel": "k_elast_bdb_mfma",
+                "instruction": "v_mfma_f32_16x16x32_bf16", "mfma_per_element": n_mfma,
+                "issued_tflops": issued * n_cv / t_el / 1e12, "peak_tflops": BF16_PEAK_TFLOPS,
+                "frac": issued * n_cv / t_el / 1e12 / BF16_PEAK_TFLOPS,
+                "useful_flop_per_element": 2 * nq * (6 * n_loc * n_loc + 36 * n_loc),
+                "ms_records_plus_element_kernel": bdb["ms_geometry_kernel"],
+                "ms_gather": bdb["ms_assemble_kernel"],
+                "note": "the FP64 node-patch form (36 kFLOP-class factorized pair blocks) stays the "
+                        "headline; this form buffers n_loc^2 FP32 per element and gathers them",
+            }
     if hexes:
         # the element kernel (per-point law + 81x81 contraction) is FP64-compute bound;
         # FLOPs per element of the per-point B^T D B model: 2 nq (n_loc^2 d + n_loc d^2), d = 9

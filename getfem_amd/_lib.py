@@ -17,7 +17,7 @@ ERRORS = {
 }
 TERM_LAPLACE, TERM_ISO_ELAST, TERM_NEOHOOKEAN = 0, 1, 2
 FORM_FACTORIZED, FORM_PERQP = 0, 1
-PREC_FP64, PREC_FP32 = 0, 1
+PREC_FP64, PREC_FP32, PREC_BF16X2 = 0, 1, 2
 COEF_CONST, COEF_PER_ELEMENT = 0, 1
 BUILD_MATRIX, BUILD_RHS, MODEL_LINEAR, MODEL_SYMMETRIC = 1, 2, 4, 8
 

@@ -389,7 +389,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_coef[0]); dfree(p->d_coef[1]);
   dfree(p->d_gref); dfree(p->d_qw); dfree(p->d_R);
   dfree(p->d_rec); dfree(p->d_thr); dfree(p->d_keep);
-  dfree(p->d_gref32); dfree(p->d_qw32); dfree(p->d_R32); dfree(p->d_rec32); dfree(p->d_Rq);
+  dfree(p->d_gref32); dfree(p->d_qw32); dfree(p->d_R32); dfree(p->d_rec32); dfree(p->d_Rq); dfree(p->d_E32);
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order); dfree(p->d_desc);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
@@ -549,8 +549,12 @@ int gfx_plan_set_coef(gfx_plan *p, int slot, const gfx_coef *cf) {
 int gfx_plan_set_form(gfx_plan *p, int form, int precision) {
   if (!p) return GFX_E_INVALID;
   if ((form != GFX_FORM_FACTORIZED && form != GFX_FORM_PERQP) ||
-      (precision != GFX_PREC_FP64 && precision != GFX_PREC_FP32))
+      (precision != GFX_PREC_FP64 && precision != GFX_PREC_FP32 && precision != GFX_PREC_BF16X2))
     return fail(p->ctx, GFX_E_INVALID, "unknown form or precision");
+  if (precision == GFX_PREC_BF16X2 &&
+      (p->eb || p->term != GFX_TERM_ISO_ELAST || p->qdim != 3 || (p->nd != 10 && p->nd != 20)))
+    return fail(p->ctx, GFX_E_UNSUPPORTED,
+                "the bf16x2 form needs isotropic elasticity on a P2 or P3 vector field of an affine tet mesh");
   p->form = form;
   p->precision = precision;
   return GFX_OK;
@@ -699,6 +703,7 @@ static int plan_waves(gfx_plan *p, const std::vector<uint16_t> &ns, const std::v
     for (; cnt < 4; ++cnt) packed.push_back(-1);
     bm.acc_cap = std::max(bm.acc_cap, sacc);
     bm.flat_cap = std::max(bm.flat_cap, sflat);
+    bm.inc_max = std::max(bm.inc_max, sinc);
     bm.count++;
     cnt = sinc = sacc = sflat = 0;
   };
@@ -714,6 +719,7 @@ static int plan_waves(gfx_plan *p, const std::vector<uint16_t> &ns, const std::v
       for (int t = 1; t < 4; ++t) large[k].push_back(-1);
       bl[k].acc_cap = std::max(bl[k].acc_cap, acc);
       bl[k].flat_cap = std::max(bl[k].flat_cap, flat);
+      bl[k].inc_max = std::max(bl[k].inc_max, inc);
       bl[k].count++;
       continue;
     }
@@ -875,6 +881,12 @@ int gfx_plan_assemble(gfx_plan *p) {
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
   }
   if ((r = p->eb ? plan_eb_elements(p, p->precision) : plan_records(p, true))) return r;
+  const bool bdb = !p->eb && p->precision == GFX_PREC_BF16X2;
+  if (bdb) {  // element matrices on the matrix cores, then the node-patch gather below
+    const int64_t n_cv = p->mf->mesh->n_cv;
+    if (!p->d_E32) HIPCHK(c, dalloc(&p->d_E32, (size_t)n_cv * p->nloc * p->nloc));
+    HIPCHK(c, launch_elast_bdb_mfma(c->stream, n_cv, p->nd, p->nq, p->d_rec, p->d_gref, p->d_qw, p->d_E32));
+  }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   PatchArgs a{};
   a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot;
@@ -909,7 +921,13 @@ int gfx_plan_assemble(gfx_plan *p) {
       a.order = p->d_order + b.off;
       a.desc = p->d_desc + b.off;
       a.npw = b.npw;
-      HIPCHK(c, launch_gather_emat(st, a, p->d_E, p->precision == GFX_PREC_FP32, p->d_thr, b.cap));
+      HIPCHK(c, launch_gather_emat(st, a, p->d_E, p->precision == GFX_PREC_FP32, p->d_thr,
+                                   b.npw * p->qdim * p->qdim * b.cap, b.npw * p->max_inc));
+    } else if (bdb) {  // the packed waves' descriptor groups (4 slots, empty ones zero)
+      a.n_nodes = 4 * b.count;
+      a.desc = p->d_desc + 4 * b.off;
+      a.npw = 4;
+      HIPCHK(c, launch_gather_emat(st, a, p->d_E32, true, p->d_thr, b.acc_cap, b.inc_max));
     } else {
       a.n_waves = b.count;
       a.desc = p->d_desc + 4 * b.off;

@@ -176,8 +176,10 @@ int64_t scan_tmp_elems(int64_t n);
 hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision);
 hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, const double *E,
                             const double *thr, uint32_t *keep);
+hipError_t launch_elast_bdb_mfma(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,
+                                 const double *gref, const double *qw, float *E);
 hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a, const void *E, bool e_fp32, const double *thr,
-                              int max_nslots);
+                              int acc_cap, int group_inc);
 hipError_t launch_gather_vec(hipStream_t s, int64_t n_nodes, int nd, int Q, const int32_t *inc_ptr,
                              const uint32_t *inc, const double *res, double *V);
 
@@ -229,6 +231,7 @@ struct gfx_plan {
   double *d_rec = nullptr, *d_thr = nullptr;
   // FP32 contraction form: the tables and the element records in FP32
   float *d_gref32 = nullptr, *d_qw32 = nullptr, *d_R32 = nullptr, *d_rec32 = nullptr;
+  float *d_E32 = nullptr;  // element matrices of the GFX_PREC_BF16X2 form (allocated on first use)
   // R = N / r_L with small integers N (exact quadrature of polynomials): rk 1
   // int8 / 2 int16 blocks in d_Rq, 0 if R has no such form
   int rk = 0;
@@ -250,7 +253,7 @@ struct gfx_plan {
   // gather bins: node positions [off, off+count), largest nslots cap, npw
   // patches per workgroup; patch launches: wave groups [off, off+count) of
   // d_desc / d_order (4 entries each), LDS caps acc_cap doubles + flat_cap words
-  struct Bin { int64_t off, count; int cap, npw; int acc_cap = 0, flat_cap = 0; };
+  struct Bin { int64_t off, count; int cap, npw; int acc_cap = 0, flat_cap = 0, inc_max = 0; };
   std::vector<Bin> bins;
   int64_t n_order = 0;       // entries of d_order / d_desc
   int64_t ndof = 0, nnz = -1;

@@ -449,8 +449,6 @@ __global__ __launch_bounds__(256) void k_keep_emat(int64_t n_cv, int nl, int kwo
   keep[w] = m;
 }
 
-__device__ __forceinline__ bool mbit(const uint32_t *m, int b) { return (m[b >> 5] >> (b & 31)) & 1u; }
-
 // node-patch gather of the rows of buffered element matrices: the Q rows of
 // an incidence are one contiguous run of Q*NL doubles; the runs of all the
 // node's incidences are streamed by the whole workgroup at once (independent
@@ -492,12 +490,15 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   // incidence c: the number of patch starts at or below c): the incidence
   // words and slot maps are loaded together, then the thresholds
   auto patch_of = [&](int c) { return (int)(c >= pre[1]) + (int)(c >= pre[2]) + (int)(c >= pre[3]); };
-  int gi0[kGatherNpw], gns[kGatherNpw], pre4[kGatherNpw];
+  int gi0[kGatherNpw], gns[kGatherNpw], pre4[kGatherNpw], aoff[kGatherNpw];
+  int apre = 0;  // accumulator of patch k at the prefix sum of Q*Q*ns
 #pragma unroll
   for (int k = 0; k < kGatherNpw; ++k) {
     gi0[k] = dsc[k].i0;
     gns[k] = (int)(dsc[k].ninc_ns >> 16) & 0x7fff;
     pre4[k] = pre[k];
+    aoff[k] = apre;
+    apre += Q * Q * gns[k];
   }
   auto sel = [](const int (&v)[kGatherNpw], int k) {  // by value: no indexed array
     return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
@@ -511,19 +512,13 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
     const int c = x / ND, k = patch_of(c);
     sslot[x] = a.slot[((int64_t)sel(gi0, k) + (c - sel(pre4, k))) * ND + (x - c * ND)];
   }
-#pragma unroll
-  for (int k = 0; k < kGatherNpw; ++k) {
-    if (k >= npw) break;
-    const int ns = (int)(dsc[k].ninc_ns >> 16) & 0x7fff;
-    double *ak = acc + (size_t)k * a.lds_acc;
-    for (int x = t; x < Q * Q * ns; x += kGatherThreads) ak[x] = 0.0;
-  }
+  for (int x = t; x < apre; x += kGatherThreads) acc[x] = 0.0;
   if (t < total_inc) {
     const int k = patch_of(t);
     const int64_t e = inc >> 5;
     sbase[t] = e * (int64_t)NL * NL + (int64_t)(inc & 31) * RUN;
     sthr[t] = thr[e];
-    sacc[t] = k * a.lds_acc;
+    sacc[t] = sel(aoff, k);
     sns[t] = sel(gns, k);
   }
   __syncthreads();
@@ -556,7 +551,7 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   for (int k = 0; k < kGatherNpw; ++k) {
     if (k >= npw || pos0 + k >= a.n_nodes) break;
     const int nsraw = (int)(dsc[k].ninc_ns >> 16), ns = nsraw & 0x7fff;
-    const double *ak = acc + (size_t)k * a.lds_acc;
+    const double *ak = acc + aoff[k];
     if (nsraw & 0x8000) {  // all rows full: the node's Q rows are one contiguous run of the CSR
       double *dst = a.vals + dsc[k].rowbase;
       for (int d = t; d < Q * ns * Q; d += kGatherThreads) __builtin_nontemporal_store(ak[d], dst + d);
@@ -604,6 +599,130 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   }
 }
 
+// Element-batched isotropic elasticity on the matrix cores (affine tets, the
+// bf16 split form): E = sum_q w_q |J| B_q^T D B_q with B_q the 6 x NL Voigt
+// strain-displacement matrix at point q (engineering shears; rows xx, yy, zz,
+// yz, xz, xy) and D the isotropic Voigt matrix (lambda + 2 mu on the normal
+// diagonal, lambda off it, mu on the shears).  As one product
+// E[m][n] = sum_k A[m][k] C[k][n] with k = 8 q + s (s < 6; 6, 7 zero),
+// A = B^T and C = w |J| D B, one 32-wide k-step is 4 points.  The NL x NL
+// output is cut into 16 x 16 tiles; wavefront w owns tiles w, w + 4, ...
+// MFMA 16x16x32 bf16 lane layout: A[i][k] in lane i + 16 (k / 8), element
+// k % 8; C[k][j] in lane j + 16 (k / 8), element k % 8; D[i][j] in lane
+// j + 16 (i / 4), element i % 4.  So a lane's fragment is the 8 Voigt rows of
+// one point (q = 4 step + lane / 16) for one dof: it is built in registers
+// from that dof's gradient, read from LDS (all points' gradients are computed
+// once per element), split into bf16 hi + lo, and the three products hi hi,
+// hi lo, lo hi are accumulated in FP32 (lo lo is below 2^-16 of the leading
+// term).  E is written row-major in FP32, rows = test dofs (i, a).
+__device__ __forceinline__ void split2(float x, __bf16 &hi, __bf16 &lo) {
+  hi = (__bf16)x;
+  lo = (__bf16)(x - (float)hi);
+}
+
+// Voigt rows of dof (node, axis ax) with gradient g: B[s] (s < 6)
+__device__ __forceinline__ void voigt_b(const float4 &g, int ax, float b[6]) {
+  b[0] = ax == 0 ? g.x : 0.f;
+  b[1] = ax == 1 ? g.y : 0.f;
+  b[2] = ax == 2 ? g.z : 0.f;
+  b[3] = ax == 1 ? g.z : (ax == 2 ? g.y : 0.f);  // yz
+  b[4] = ax == 0 ? g.z : (ax == 2 ? g.x : 0.f);  // xz
+  b[5] = ax == 0 ? g.y : (ax == 1 ? g.x : 0.f);  // xy
+}
+
+// the fragment of dof m (m >= NL: padding, zero) at the point of g; with
+// lam/mu: the rows of w |J| D B (g.w = w |J|), else the rows of B
+template <int NL, bool DB>
+__device__ __forceinline__ void bdb_fragment(const float4 *gp, int m, float lam, float mu, bf16x8 &hi,
+                                             bf16x8 &lo) {
+  float v[8];
+  const float4 g = gp[m < NL ? m / 3 : 0];
+  const int ax = m % 3;
+  float b[6];
+  voigt_b(g, ax, b);
+  if (DB) {
+    const float ga = ax == 0 ? g.x : (ax == 1 ? g.y : g.z), lg = lam * ga * g.w, mw = mu * g.w;
+    for (int s = 0; s < 3; ++s) v[s] = lg + 2.f * mw * b[s];
+    for (int s = 3; s < 6; ++s) v[s] = mw * b[s];
+  } else {
+    for (int s = 0; s < 6; ++s) v[s] = b[s];
+  }
+  v[6] = v[7] = 0.f;
+#pragma unroll
+  for (int x = 0; x < 8; ++x) {
+    const float y = m < NL ? v[x] : 0.f;
+    __bf16 h, l;
+    split2(y, h, l);
+    hi[x] = h;
+    lo[x] = l;
+  }
+}
+
+template <int ND>
+__global__ __launch_bounds__(256) void k_elast_bdb_mfma(int64_t n_cv, int nq, const double *__restrict__ rec,
+                                                        const double *__restrict__ gref,
+                                                        const double *__restrict__ qw, float *__restrict__ E) {
+  constexpr int NL = 3 * ND, NT = (NL + 15) / 16, NU = (NT * NT + 3) / 4;
+  extern __shared__ float4 gs[];  // [4 * steps][ND]: (grad phi_i at point q, w_q |J|)
+  const int64_t e = blockIdx.x;
+  if (e >= n_cv) return;
+  const int t = threadIdx.x, w = t / 64, l = t % 64;
+  const int steps = (nq + 3) / 4;
+  const double *r = rec + (int64_t)kRecStride * e;  // B row-major, |J| at 12, lambda 13, mu 14
+  double Bm[9];
+#pragma unroll
+  for (int x = 0; x < 9; ++x) Bm[x] = r[x];
+  const double aJ = r[12];
+  const float lam = (float)r[13], mu = (float)r[14];
+  for (int x = t; x < 4 * steps * ND; x += 256) {
+    const int q = x / ND, i = x - q * ND;
+    float4 g{0.f, 0.f, 0.f, 0.f};
+    if (q < nq) {
+      const double *h = gref + ((int64_t)q * ND + i) * 3;
+      const double h0 = h[0], h1 = h[1], h2 = h[2];
+      g.x = (float)(Bm[0] * h0 + Bm[1] * h1 + Bm[2] * h2);
+      g.y = (float)(Bm[3] * h0 + Bm[4] * h1 + Bm[5] * h2);
+      g.z = (float)(Bm[6] * h0 + Bm[7] * h1 + Bm[8] * h2);
+      g.w = (float)(qw[q] * aJ);
+    }
+    gs[x] = g;
+  }
+  __syncthreads();
+  f32x4 acc[NU];
+#pragma unroll
+  for (int u = 0; u < NU; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const int lr = l % 16, qq = l / 16;
+  for (int k = 0; k < steps; ++k) {
+    const float4 *gp = gs + (4 * k + qq) * ND;
+#pragma unroll
+    for (int u = 0; u < NU; ++u) {
+      const int p = w + 4 * u;
+      if (p < NT * NT) {  // uniform across the wavefront
+        const int rt = p / NT, ct = p % NT;
+        bf16x8 ah, al, ch, cl;
+        bdb_fragment<NL, false>(gp, 16 * rt + lr, lam, mu, ah, al);
+        bdb_fragment<NL, true>(gp, 16 * ct + lr, lam, mu, ch, cl);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ch, acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, cl, acc[u], 0, 0, 0);
+        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ch, acc[u], 0, 0, 0);
+      }
+    }
+  }
+  float *Ee = E + (int64_t)e * NL * NL;
+#pragma unroll
+  for (int u = 0; u < NU; ++u) {
+    const int p = w + 4 * u;
+    if (p < NT * NT) {
+      const int rt = p / NT, ct = p % NT, col = 16 * ct + lr;
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const int row = 16 * rt + 4 * qq + x;
+        if (row < NL && col < NL) __builtin_nontemporal_store(acc[u][x], Ee + row * NL + col);
+      }
+    }
+  }
+}
+
 // V[node*Q + x] = sum over incidences (e, i) of res[(e*nd + i)*Q + x]
 __global__ __launch_bounds__(256) void k_gather_vec(int64_t n_nodes, int nd, int Q,
                                                     const int32_t *__restrict__ inc_ptr,
@@ -634,6 +753,17 @@ hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision) {
   return hipGetLastError();
 }
 
+hipError_t launch_elast_bdb_mfma(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,
+                                 const double *gref, const double *qw, float *E) {
+  if (n_cv == 0) return hipSuccess;
+  const size_t lds = 16 * (size_t)(4 * ((nq + 3) / 4)) * nd;  // one float4 per (point, node)
+  if (nq < 1 || lds > 64 * 1024) return hipErrorInvalidValue;
+  if (nd == 10) k_elast_bdb_mfma<10><<<(unsigned)n_cv, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
+  else if (nd == 20) k_elast_bdb_mfma<20><<<(unsigned)n_cv, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
 hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, const double *E,
                             const double *thr, uint32_t *keep) {
   const int64_t n = n_cv * kwords;
@@ -643,16 +773,17 @@ hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, con
 }
 
 hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const void *E, bool e_fp32, const double *thr,
-                              int max_nslots) {
+                              int acc_cap, int group_inc) {
+  // acc_cap: the largest sum of Q*Q*nslots over the patches of one workgroup
+  // (doubles of LDS); group_inc: the largest number of incidences of one workgroup
   if (a0.n_nodes == 0) return hipSuccess;
   PatchArgs a = a0;
   if (a.npw < 1 || a.npw > kGatherNpw) return hipErrorInvalidValue;
   const int64_t groups = (a.n_nodes + a.npw - 1) / a.npw;
   a.xcd_chunk = (int)((groups + 7) / 8);
   const unsigned grid = (unsigned)(8 * (int64_t)a.xcd_chunk);
-  a.lds_acc = a.qdim * a.qdim * max_nslots;
-  const size_t lds = (size_t)a.npw * a.lds_acc * sizeof(double);
-  if (lds > 48 * 1024 || a.npw * a.max_inc > 64) return hipErrorInvalidValue;
+  const size_t lds = (size_t)acc_cap * sizeof(double);
+  if (acc_cap < 0 || lds > 48 * 1024 || group_inc > 64) return hipErrorInvalidValue;
   const double *E64 = static_cast<const double *>(E);
   const float *E32 = static_cast<const float *>(E);
   if (a.nd == 27 && a.qdim == 3) {
@@ -661,6 +792,10 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const void *E,
   } else if (a.nd == 8 && a.qdim == 3) {
     if (e_fp32) k_gather_emat<8, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
     else k_gather_emat<8, 3, double><<<grid, kGatherThreads, lds, s>>>(a, E64, thr);
+  } else if (a.nd == 10 && a.qdim == 3 && e_fp32) {  // element-batched affine P2 (bf16 split form)
+    k_gather_emat<10, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
+  } else if (a.nd == 20 && a.qdim == 3 && e_fp32) {  // element-batched affine P3
+    k_gather_emat<20, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
   }
   else return hipErrorInvalidValue;
   return hipGetLastError();

@@ -53,6 +53,11 @@ extern "C" {
 #define GFX_PREC_FP32 1       /* element contraction in FP32 from FP32 element records
                                  (B, scales, threshold rounded once per element);
                                  geometry and global accumulation in FP64           */
+#define GFX_PREC_BF16X2 2     /* isotropic elasticity, affine P2/P3 (vector) plans:
+                                 element matrices B^T D B on the matrix cores (bf16
+                                 hi + lo split, three products, FP32 accumulation),
+                                 buffered in FP32 and gathered into the CSR in FP64;
+                                 relative error about 1e-5 of the element norm      */
 
 /* ---- coefficient sources ---- */
 #define GFX_COEF_CONST 0       /* one value                                 */

@@ -1,6 +1,6 @@
 """libgfx (HIP, gfx950) against the oracle on identical inputs: the integer
 pattern bit-exact, FP64 values within 1e-12 relative Frobenius, FP32 element
-contractions within 1e-6."""
+contractions within 1e-6, bf16 split element matrices within 1e-4."""
 import numpy as np
 import pytest
 
@@ -75,6 +75,36 @@ def test_fp32_element_error(ctx, name, form):
     err = problems.rel_frob(v, v_ref)
     print("fp32 element form: rel frobenius error vs FP64 oracle", err)
     assert err <= TOL_FP32
+
+
+TOL_BF16X2 = 1e-4  # relative Frobenius: bf16 hi + lo products, FP32 element accumulation
+
+
+@pytest.mark.parametrize("name", ["p2_elast_n8", "p2_elast_n3_unjittered", "p3_elast_n3"])
+def test_bf16x2_element_batched(ctx, name):
+    """Element matrices B^T D B on the matrix cores, gathered over the packed
+    waves and the large-patch launches into GetFEM's pattern."""
+    case = CASES[name]()
+    print("case", name, "ndof", case.ndof, flush=True)
+    ip_ref, ix_ref, v_ref = case.oracle_csr()
+    plan, keep = gpu_plan(ctx, case)
+    plan.set_form(ga.FORM_FACTORIZED, ga.PREC_BF16X2)
+    plan.assemble()
+    ip, ix, v = plan.csr()
+    assert np.array_equal(ip, ip_ref) and np.array_equal(ix, ix_ref)
+    err = problems.rel_frob(v, v_ref)
+    print("bf16x2 element-batched form: rel frobenius error vs FP64 oracle", err)
+    assert err <= TOL_BF16X2
+    # the same plan returns to the FP64 form
+    plan.set_form(ga.FORM_FACTORIZED, ga.PREC_FP64)
+    plan.assemble()
+    assert problems.rel_frob(plan.csr()[2], v_ref) <= TOL_FP64
+
+
+def test_bf16x2_rejects_other_terms(ctx):
+    plan, keep = gpu_plan(ctx, CASES["p2_poisson_n5"]())
+    with pytest.raises(ga.GfxError):
+        plan.set_form(ga.FORM_FACTORIZED, ga.PREC_BF16X2)
 
 
 def test_csc_export_is_getfem_orientation(ctx):
