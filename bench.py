@@ -233,12 +233,14 @@ def run_gpu(args):
                        "note": "3x3 pair blocks on the vector ALUs (FP64 matrix rate = vector rate on MI355X)"}
         bdb = forms.get("bf16x2_element_batched")
         if bdb:
-            # element matrices B^T D B on the matrix cores: ceil(nq/4) k-steps of 32
-            # (4 points x 8 Voigt rows, 2 of them zero) x (n_loc/16 rounded up)^2
-            # output tiles x 3 bf16 products of v_mfma_f32_16x16x32_bf16 (16384 FLOP each)
+            # element matrices sum_q sign(w_q) F_q^T F_q (F = sqrt(|w_q||J|) S B_q, S S = D)
+            # on the matrix cores: ceil(nq/4) k-steps of 32 (4 points x 8 Voigt rows, 2 of
+            # them zero) x nt (nt + 1) / 2 upper 16 x 16 tiles (nt = n_loc/16 rounded up;
+            # the lower ones are mirrored) x 3 bf16 products of v_mfma_f32_16x16x32_bf16
+            # (16384 FLOP each)
             nq = mim.points()[1].size
             nt = -(-n_loc // 16)
-            n_mfma = -(-nq // 4) * nt * nt * 3
+            n_mfma = -(-nq // 4) * (nt * (nt + 1) // 2) * 3
             issued = n_mfma * 2 * 16 * 16 * 32
             t_el = bdb["ms_geometry_kernel"] * 1e-3
             out["mfma"] = {
@@ -247,6 +249,7 @@ def run_gpu(args):
                 "issued_tflops": issued * n_cv / t_el / 1e12, "peak_tflops": BF16_PEAK_TFLOPS,
                 "frac": issued * n_cv / t_el / 1e12 / BF16_PEAK_TFLOPS,
                 "useful_flop_per_element": 2 * nq * (6 * n_loc * n_loc + 36 * n_loc),
+                "useful_tflops_equivalent": 2 * nq * (6 * n_loc * n_loc + 36 * n_loc) * n_cv / t_el / 1e12,
                 "ms_records_plus_element_kernel": bdb["ms_geometry_kernel"],
                 "ms_gather": bdb["ms_assemble_kernel"],
                 "note": "the FP64 node-patch form (36 kFLOP-class factorized pair blocks) stays the "

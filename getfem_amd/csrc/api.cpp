@@ -369,10 +369,12 @@ static int plan_upload_coef(gfx_plan *p, int slot, const gfx_coef *cf) {
     return fail(c, GFX_E_INVALID, "unknown coefficient kind");
   p->coef[slot] = *cf;
   p->coef[slot].h_values = nullptr;
+  p->coef_min[slot] = cf->value;
   dfree(p->d_coef[slot]);
   if (cf->kind == GFX_COEF_PER_ELEMENT) {
     if (!cf->h_values) return fail(c, GFX_E_INVALID, "per-element coefficient without values");
     const int64_t n = p->mf->mesh->n_cv;
+    p->coef_min[slot] = n ? *std::min_element(cf->h_values, cf->h_values + n) : 0.0;
     HIPCHK(c, dalloc(&p->d_coef[slot], (size_t)n));
     HIPCHK(c, hipMemcpy(p->d_coef[slot], cf->h_values, 8 * (size_t)n, hipMemcpyHostToDevice));
   }
@@ -884,6 +886,11 @@ int gfx_plan_assemble(gfx_plan *p) {
   const bool bdb = !p->eb && p->precision == GFX_PREC_BF16X2;
   if (bdb) {  // element matrices on the matrix cores, then the node-patch gather below
     const int64_t n_cv = p->mf->mesh->n_cv;
+    // the kernel factors D through its square root: mu > 0 and 3 lambda + 2 mu > 0
+    // for every element (sufficient: from the smallest lambda and mu)
+    const double lmin = p->coef_min[0], mmin = p->coef_min[1];
+    if (!(mmin > 0.0 && 3.0 * std::min(lmin, 0.0) + 2.0 * mmin > 0.0))
+      return fail(c, GFX_E_UNSUPPORTED, "the bf16x2 form needs mu > 0 and 3*lambda + 2*mu > 0 on every element");
     if (!p->d_E32) HIPCHK(c, dalloc(&p->d_E32, (size_t)n_cv * p->nloc * p->nloc));
     HIPCHK(c, launch_elast_bdb_mfma(c->stream, n_cv, p->nd, p->nq, p->d_rec, p->d_gref, p->d_qw, p->d_E32));
   }

@@ -226,6 +226,7 @@ struct gfx_plan {
   int nd = 0, qdim = 1, nloc = 0, nq = 0, nv = 4;
   int kwords = 0, mwpi = 0;
   gfx_coef coef[2];
+  double coef_min[2] = {0.0, 0.0};  // smallest value of each coefficient (host copy)
   double *d_coef[2] = {nullptr, nullptr};
   double *d_gref = nullptr, *d_qw = nullptr, *d_R = nullptr;
   double *d_rec = nullptr, *d_thr = nullptr;

@@ -459,8 +459,8 @@ constexpr int kGatherThreads = 256;
 // lists are concatenated and streamed together.
 constexpr int kGatherNpw = 4;
 
-template <int ND, int Q, typename TE>
-__global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, const TE *__restrict__ E,
+template <int ND, int Q, typename TE, int TH = kGatherThreads>
+__global__ __launch_bounds__(TH) void k_gather_emat(PatchArgs a, const TE *__restrict__ E,
                                                                 const double *__restrict__ thr) {
   extern __shared__ double acc[];  // npw accumulators of lds_acc doubles
   constexpr int kMaxInc = 64;      // incidences per workgroup (checked by the launcher)
@@ -508,11 +508,11 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
     const int k = patch_of(t);
     inc = a.inc[sel(gi0, k) + (t - sel(pre4, k))];
   }
-  for (int x = t; x < total_inc * ND; x += kGatherThreads) {
+  for (int x = t; x < total_inc * ND; x += TH) {
     const int c = x / ND, k = patch_of(c);
     sslot[x] = a.slot[((int64_t)sel(gi0, k) + (c - sel(pre4, k))) * ND + (x - c * ND)];
   }
-  for (int x = t; x < apre; x += kGatherThreads) acc[x] = 0.0;
+  for (int x = t; x < apre; x += TH) acc[x] = 0.0;
   if (t < total_inc) {
     const int k = patch_of(t);
     const int64_t e = inc >> 5;
@@ -524,19 +524,19 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
   __syncthreads();
   // kB entries per thread in flight: the loads use clamped indices (always a
   // valid entry) and are issued before any of them is used
-  constexpr int kB = 8;
+  constexpr int kB = TH == 64 ? 16 : 8;
   const int total = total_inc * RUN;
-  for (int f0 = 0; f0 < total; f0 += kB * kGatherThreads) {
+  for (int f0 = 0; f0 < total; f0 += kB * TH) {
     double v[kB];
 #pragma unroll
     for (int u = 0; u < kB; ++u) {
-      const int f = min(f0 + u * kGatherThreads + t, total - 1);
+      const int f = min(f0 + u * TH + t, total - 1);
       const int k = f / RUN, o = f - k * RUN;
       v[u] = (double)__builtin_nontemporal_load(E + sbase[k] + o);  // read once
     }
 #pragma unroll
     for (int u = 0; u < kB; ++u) {
-      const int f = f0 + u * kGatherThreads + t;
+      const int f = f0 + u * TH + t;
       if (f >= total) break;
       const int k = f / RUN, o = f - k * RUN;
       const int x = o / NL, c = o - x * NL, j = c / Q, y = c - j * Q;
@@ -554,15 +554,14 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
     const double *ak = acc + aoff[k];
     if (nsraw & 0x8000) {  // all rows full: the node's Q rows are one contiguous run of the CSR
       double *dst = a.vals + dsc[k].rowbase;
-      for (int d = t; d < Q * ns * Q; d += kGatherThreads) __builtin_nontemporal_store(ak[d], dst + d);
+      for (int d = t; d < Q * ns * Q; d += TH) __builtin_nontemporal_store(ak[d], dst + d);
       continue;
     }
     // wavefront x writes row x from the presence bits; the node's rows are
     // consecutive CSR rows, so row x starts after the present entries of
     // rows 0..x-1 (bits [0, x*ns*Q) of the node's presence words)
     const uint32_t *m = a.mask + (int64_t)dsc[k].i0 * a.mwpi;
-    if (w < Q) {
-      const int x = w;
+    for (int x = w; x < Q; x += TH / 64) {
       const int off = x * ns * Q, len = ns * Q;
       int cnt = 0;
       for (int b0 = 0; b0 < off; b0 += 64 * 32) {
@@ -603,123 +602,164 @@ __global__ __launch_bounds__(kGatherThreads) void k_gather_emat(PatchArgs a, con
 // bf16 split form): E = sum_q w_q |J| B_q^T D B_q with B_q the 6 x NL Voigt
 // strain-displacement matrix at point q (engineering shears; rows xx, yy, zz,
 // yz, xz, xy) and D the isotropic Voigt matrix (lambda + 2 mu on the normal
-// diagonal, lambda off it, mu on the shears).  As one product
-// E[m][n] = sum_k A[m][k] C[k][n] with k = 8 q + s (s < 6; 6, 7 zero),
-// A = B^T and C = w |J| D B, one 32-wide k-step is 4 points.  The NL x NL
-// output is cut into 16 x 16 tiles; wavefront w owns tiles w, w + 4, ...
+// diagonal, lambda off it, mu on the shears).  D = S S with the symmetric root
+// S = sqrt(2 mu) I + c 1 1^T on the normal block, c = (sqrt(2 mu + 3 lambda)
+// - sqrt(2 mu)) / 3, and sqrt(mu) on the shears (D positive definite: mu > 0,
+// 3 lambda + 2 mu > 0).  With F_q = sqrt(|w_q| |J|) S B_q,
+// E = sum_q sign(w_q) F_q^T F_q: one product E[m][n] = sum_k A[m][k] C[k][n]
+// with A[m][k] = F[k][m], C[k][n] = sign F[k][n] and k = 8 q + s (s < 6; 6, 7
+// zero), so one 32-wide k-step is 4 points.  E is symmetric: one wavefront
+// computes one element's upper NT x NT tiles (16 x 16) and mirrors them.
 // MFMA 16x16x32 bf16 lane layout: A[i][k] in lane i + 16 (k / 8), element
 // k % 8; C[k][j] in lane j + 16 (k / 8), element k % 8; D[i][j] in lane
-// j + 16 (i / 4), element i % 4.  So a lane's fragment is the 8 Voigt rows of
-// one point (q = 4 step + lane / 16) for one dof: it is built in registers
-// from that dof's gradient, read from LDS (all points' gradients are computed
-// once per element), split into bf16 hi + lo, and the three products hi hi,
-// hi lo, lo hi are accumulated in FP32 (lo lo is below 2^-16 of the leading
-// term).  E is written row-major in FP32, rows = test dofs (i, a).
+// j + 16 (i / 4), element i % 4.  So the A fragment of row tile a and the C
+// fragment of column tile a are the same registers (up to the sign of w_q):
+// the 8 rows of F at one point (q = 4 step + lane / 16) for the dof
+// 16 a + lane % 16, built from the reference gradient of its shape function
+// (the same for every element: staged in LDS once per workgroup) and the
+// element's inverse Jacobian, then split into bf16 hi + lo; the products
+// hi hi, hi lo and lo hi are accumulated in FP32 (lo lo is below 2^-16 of
+// the leading term).  The workgroup's four wavefronts sweep the elements with
+// a grid stride, the next element's record loaded while the current one is
+// computed.  E is written row-major in FP32 (rows = test dofs (i, a)) through
+// the wavefront's LDS block as whole 16-byte pieces.
 __device__ __forceinline__ void split2(float x, __bf16 &hi, __bf16 &lo) {
   hi = (__bf16)x;
   lo = (__bf16)(x - (float)hi);
 }
 
-// Voigt rows of dof (node, axis ax) with gradient g: B[s] (s < 6)
-__device__ __forceinline__ void voigt_b(const float4 &g, int ax, float b[6]) {
-  b[0] = ax == 0 ? g.x : 0.f;
-  b[1] = ax == 1 ? g.y : 0.f;
-  b[2] = ax == 2 ? g.z : 0.f;
-  b[3] = ax == 1 ? g.z : (ax == 2 ? g.y : 0.f);  // yz
-  b[4] = ax == 0 ? g.z : (ax == 2 ? g.x : 0.f);  // xz
-  b[5] = ax == 0 ? g.y : (ax == 1 ? g.x : 0.f);  // xy
+typedef __attribute__((__vector_size__(4 * sizeof(uint32_t)))) uint32_t u32x4;
+
+__device__ __forceinline__ bf16x8 bf16_xor(const bf16x8 &x, uint32_t m) {
+  u32x4 u = __builtin_bit_cast(u32x4, x);
+  u ^= m;
+  return __builtin_bit_cast(bf16x8, u);
 }
 
-// the fragment of dof m (m >= NL: padding, zero) at the point of g; with
-// lam/mu: the rows of w |J| D B (g.w = w |J|), else the rows of B
-template <int NL, bool DB>
-__device__ __forceinline__ void bdb_fragment(const float4 *gp, int m, float lam, float mu, bf16x8 &hi,
-                                             bf16x8 &lo) {
-  float v[8];
-  const float4 g = gp[m < NL ? m / 3 : 0];
+// rows of F at the point of the staged reference gradient h (h.w = the signed
+// sqrt(|w_q|)) for dof m (m >= NL: padding, zero); on / off / sh: sqrt(|J|)
+// times the normal-diagonal, normal-off-diagonal and shear factors of S
+template <int NL>
+__device__ __forceinline__ void bdb_factor(const float4 *hp, int m, const float (&Bm)[9], float on, float off,
+                                           float sh, bf16x8 &hi, bf16x8 &lo) {
+  const float4 h = hp[m < NL ? m / 3 : 0];
+  const float gx = Bm[0] * h.x + Bm[1] * h.y + Bm[2] * h.z;
+  const float gy = Bm[3] * h.x + Bm[4] * h.y + Bm[5] * h.z;
+  const float gz = Bm[6] * h.x + Bm[7] * h.y + Bm[8] * h.z;
   const int ax = m % 3;
-  float b[6];
-  voigt_b(g, ax, b);
-  if (DB) {
-    const float ga = ax == 0 ? g.x : (ax == 1 ? g.y : g.z), lg = lam * ga * g.w, mw = mu * g.w;
-    for (int s = 0; s < 3; ++s) v[s] = lg + 2.f * mw * b[s];
-    for (int s = 3; s < 6; ++s) v[s] = mw * b[s];
-  } else {
-    for (int s = 0; s < 6; ++s) v[s] = b[s];
-  }
+  const float r = m < NL ? fabsf(h.w) : 0.f;  // padding rows are zero
+  const float ga = r * (ax == 0 ? gx : (ax == 1 ? gy : gz)), vn = on * ga, vo = off * ga;
+  const float rs = sh * r;
+  float v[8];
+  v[0] = ax == 0 ? vn : vo;
+  v[1] = ax == 1 ? vn : vo;
+  v[2] = ax == 2 ? vn : vo;
+  v[3] = rs * (ax == 1 ? gz : (ax == 2 ? gy : 0.f));  // yz
+  v[4] = rs * (ax == 0 ? gz : (ax == 2 ? gx : 0.f));  // xz
+  v[5] = rs * (ax == 0 ? gy : (ax == 1 ? gx : 0.f));  // xy
   v[6] = v[7] = 0.f;
 #pragma unroll
   for (int x = 0; x < 8; ++x) {
-    const float y = m < NL ? v[x] : 0.f;
-    __bf16 h, l;
-    split2(y, h, l);
-    hi[x] = h;
-    lo[x] = l;
+    __bf16 a, b;
+    split2(v[x], a, b);
+    hi[x] = a;
+    lo[x] = b;
   }
 }
+
+constexpr int kBdbMaxGroups = 8192;
 
 template <int ND>
 __global__ __launch_bounds__(256) void k_elast_bdb_mfma(int64_t n_cv, int nq, const double *__restrict__ rec,
                                                         const double *__restrict__ gref,
                                                         const double *__restrict__ qw, float *__restrict__ E) {
-  constexpr int NL = 3 * ND, NT = (NL + 15) / 16, NU = (NT * NT + 3) / 4;
-  extern __shared__ float4 gs[];  // [4 * steps][ND]: (grad phi_i at point q, w_q |J|)
-  const int64_t e = blockIdx.x;
-  if (e >= n_cv) return;
+  constexpr int NL = 3 * ND, NT = (NL + 15) / 16, NP = NT * (NT + 1) / 2;
+  static_assert(NL * NL % 4 == 0, "element matrices are whole float4 runs");
+  extern __shared__ float4 hs[];  // [4 * steps][ND]: (reference gradient of phi_i at point q,
+                                  // signed sqrt(|w_q|)), then one NL x NL float block per wavefront
   const int t = threadIdx.x, w = t / 64, l = t % 64;
   const int steps = (nq + 3) / 4;
-  const double *r = rec + (int64_t)kRecStride * e;  // B row-major, |J| at 12, lambda 13, mu 14
-  double Bm[9];
-#pragma unroll
-  for (int x = 0; x < 9; ++x) Bm[x] = r[x];
-  const double aJ = r[12];
-  const float lam = (float)r[13], mu = (float)r[14];
+  float *es = reinterpret_cast<float *>(hs + 4 * steps * ND) + w * NL * NL;
+  int neg = 0;
   for (int x = t; x < 4 * steps * ND; x += 256) {
-    const int q = x / ND, i = x - q * ND;
-    float4 g{0.f, 0.f, 0.f, 0.f};
+    const int q = x / ND;
+    float4 h{0.f, 0.f, 0.f, 0.f};
     if (q < nq) {
-      const double *h = gref + ((int64_t)q * ND + i) * 3;
-      const double h0 = h[0], h1 = h[1], h2 = h[2];
-      g.x = (float)(Bm[0] * h0 + Bm[1] * h1 + Bm[2] * h2);
-      g.y = (float)(Bm[3] * h0 + Bm[4] * h1 + Bm[5] * h2);
-      g.z = (float)(Bm[6] * h0 + Bm[7] * h1 + Bm[8] * h2);
-      g.w = (float)(qw[q] * aJ);
+      const double *g = gref + (int64_t)x * 3;  // [q][i][3]
+      const double wq = qw[q];
+      h = float4{(float)g[0], (float)g[1], (float)g[2], (float)copysign(sqrt(fabs(wq)), wq)};
+      neg |= wq < 0.0;
     }
-    gs[x] = g;
+    hs[x] = h;
   }
-  __syncthreads();
-  f32x4 acc[NU];
-#pragma unroll
-  for (int u = 0; u < NU; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const bool any_neg = __syncthreads_or(neg);  // a rule with negative weights
   const int lr = l % 16, qq = l / 16;
-  for (int k = 0; k < steps; ++k) {
-    const float4 *gp = gs + (4 * k + qq) * ND;
+  const int64_t stride = 4 * (int64_t)gridDim.x;
+  int64_t e = 4 * (int64_t)blockIdx.x + __builtin_amdgcn_readfirstlane(w);  // uniform: scalar record loads
+  if (e >= n_cv) return;  // no barrier follows
+  double nx[12];  // the record of the element in flight: B row-major, |J|, lambda, mu
 #pragma unroll
-    for (int u = 0; u < NU; ++u) {
-      const int p = w + 4 * u;
-      if (p < NT * NT) {  // uniform across the wavefront
-        const int rt = p / NT, ct = p % NT;
-        bf16x8 ah, al, ch, cl;
-        bdb_fragment<NL, false>(gp, 16 * rt + lr, lam, mu, ah, al);
-        bdb_fragment<NL, true>(gp, 16 * ct + lr, lam, mu, ch, cl);
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, ch, acc[u], 0, 0, 0);
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, cl, acc[u], 0, 0, 0);
-        acc[u] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, ch, acc[u], 0, 0, 0);
+  for (int x = 0; x < 9; ++x) nx[x] = rec[kRecStride * e + x];
+#pragma unroll
+  for (int x = 0; x < 3; ++x) nx[9 + x] = rec[kRecStride * e + 12 + x];
+  for (; e < n_cv; e += stride) {
+    float Bm[9];
+#pragma unroll
+    for (int x = 0; x < 9; ++x) Bm[x] = (float)nx[x];
+    const double sj = sqrt(nx[9]), lam = nx[10], mu = nx[11];
+    const double s2m = sqrt(2.0 * mu), c = (sqrt(2.0 * mu + 3.0 * lam) - s2m) / 3.0;
+    const float on = (float)(sj * (s2m + c)), off = (float)(sj * c), sh = (float)(sj * sqrt(mu));
+    const int64_t en = min(e + stride, n_cv - 1);
+#pragma unroll
+    for (int x = 0; x < 9; ++x) nx[x] = rec[kRecStride * en + x];
+#pragma unroll
+    for (int x = 0; x < 3; ++x) nx[9 + x] = rec[kRecStride * en + 12 + x];
+    f32x4 acc[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) acc[p] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int k = 0; k < steps; ++k) {
+      const float4 *hp = hs + (4 * k + qq) * ND;
+      bf16x8 fh[NT], fl[NT];
+#pragma unroll
+      for (int a = 0; a < NT; ++a) bdb_factor<NL>(hp, 16 * a + lr, Bm, on, off, sh, fh[a], fl[a]);
+      const uint32_t sgn = any_neg && hp[0].w < 0.f ? 0x80008000u : 0u;  // C = sign(w_q) F
+      int p = 0;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        const bf16x8 ch = any_neg ? bf16_xor(fh[b], sgn) : fh[b];
+        const bf16x8 cl = any_neg ? bf16_xor(fl[b], sgn) : fl[b];
+#pragma unroll
+        for (int a = 0; a <= b; ++a, ++p) {
+          acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fl[a], ch, acc[p], 0, 0, 0);
+          acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], cl, acc[p], 0, 0, 0);
+          acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], ch, acc[p], 0, 0, 0);
+        }
       }
     }
-  }
-  float *Ee = E + (int64_t)e * NL * NL;
+    int p = 0;
 #pragma unroll
-  for (int u = 0; u < NU; ++u) {
-    const int p = w + 4 * u;
-    if (p < NT * NT) {
-      const int rt = p / NT, ct = p % NT, col = 16 * ct + lr;
+    for (int b = 0; b < NT; ++b)
 #pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        const int row = 16 * rt + 4 * qq + x;
-        if (row < NL && col < NL) __builtin_nontemporal_store(acc[u][x], Ee + row * NL + col);
+      for (int a = 0; a <= b; ++a, ++p) {
+        const int col = 16 * b + lr;
+#pragma unroll
+        for (int x = 0; x < 4; ++x) {
+          const int row = 16 * a + 4 * qq + x;
+          if (row < NL && col < NL) {
+            es[row * NL + col] = acc[p][x];
+            if (a != b) es[col * NL + row] = acc[p][x];
+          }
+        }
       }
-    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    const f32x4 *es4 = reinterpret_cast<const f32x4 *>(es);
+    f32x4 *E4 = reinterpret_cast<f32x4 *>(E + e * NL * NL);
+    for (int x = l; x < NL * NL / 4; x += 64) __builtin_nontemporal_store(es4[x], E4 + x);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // reads done before the next element's writes
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
 }
 
@@ -756,10 +796,12 @@ hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision) {
 hipError_t launch_elast_bdb_mfma(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,
                                  const double *gref, const double *qw, float *E) {
   if (n_cv == 0) return hipSuccess;
-  const size_t lds = 16 * (size_t)(4 * ((nq + 3) / 4)) * nd;  // one float4 per (point, node)
-  if (nq < 1 || lds > 64 * 1024) return hipErrorInvalidValue;
-  if (nd == 10) k_elast_bdb_mfma<10><<<(unsigned)n_cv, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
-  else if (nd == 20) k_elast_bdb_mfma<20><<<(unsigned)n_cv, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
+  // one float4 per (point, node), then an element matrix per wavefront
+  const size_t lds = 16 * (size_t)(4 * ((nq + 3) / 4)) * nd + 4 * 4 * (size_t)(9 * nd * nd);
+  if (nq < 1 || lds > 160 * 1024) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)std::min<int64_t>((n_cv + 3) / 4, kBdbMaxGroups);  // 4 wavefronts each
+  if (nd == 10) k_elast_bdb_mfma<10><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
+  else if (nd == 20) k_elast_bdb_mfma<20><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
   else return hipErrorInvalidValue;
   return hipGetLastError();
 }
@@ -793,9 +835,9 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const void *E,
     if (e_fp32) k_gather_emat<8, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
     else k_gather_emat<8, 3, double><<<grid, kGatherThreads, lds, s>>>(a, E64, thr);
   } else if (a.nd == 10 && a.qdim == 3 && e_fp32) {  // element-batched affine P2 (bf16 split form)
-    k_gather_emat<10, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
+    k_gather_emat<10, 3, float, 64><<<grid, 64, lds, s>>>(a, E32, thr);
   } else if (a.nd == 20 && a.qdim == 3 && e_fp32) {  // element-batched affine P3
-    k_gather_emat<20, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
+    k_gather_emat<20, 3, float, 64><<<grid, 64, lds, s>>>(a, E32, thr);
   }
   else return hipErrorInvalidValue;
   return hipGetLastError();

@@ -80,11 +80,18 @@ def test_fp32_element_error(ctx, name, form):
 TOL_BF16X2 = 1e-4  # relative Frobenius: bf16 hi + lo products, FP32 element accumulation
 
 
-@pytest.mark.parametrize("name", ["p2_elast_n8", "p2_elast_n3_unjittered", "p3_elast_n3"])
+BF16X2_CASES = dict(CASES)
+# a rule with a negative weight (IM_TETRAHEDRON(3): -2/15 at the centroid)
+BF16X2_CASES["p2_elast_n4_im3"] = lambda: problems.Case(4, 2, 3, "IM_TETRAHEDRON(3)", 1,
+                                                        lam=np.linspace(0.5, 2.0, 384),
+                                                        mu=np.linspace(1.5, 0.3, 384))
+
+
+@pytest.mark.parametrize("name", ["p2_elast_n8", "p2_elast_n3_unjittered", "p3_elast_n3", "p2_elast_n4_im3"])
 def test_bf16x2_element_batched(ctx, name):
     """Element matrices B^T D B on the matrix cores, gathered over the packed
     waves and the large-patch launches into GetFEM's pattern."""
-    case = CASES[name]()
+    case = BF16X2_CASES[name]()
     print("case", name, "ndof", case.ndof, flush=True)
     ip_ref, ix_ref, v_ref = case.oracle_csr()
     plan, keep = gpu_plan(ctx, case)
@@ -105,6 +112,17 @@ def test_bf16x2_rejects_other_terms(ctx):
     plan, keep = gpu_plan(ctx, CASES["p2_poisson_n5"]())
     with pytest.raises(ga.GfxError):
         plan.set_form(ga.FORM_FACTORIZED, ga.PREC_BF16X2)
+
+
+def test_bf16x2_rejects_indefinite_material(ctx):
+    """3 lambda + 2 mu <= 0 on one element: D has no real square root."""
+    lam = np.full(162, 1.0)
+    lam[7] = -0.5  # with mu = 0.7: 3 lambda + 2 mu = -0.1
+    case = problems.Case(3, 2, 3, "IM_TETRAHEDRON(5)", 1, lam=lam, mu=np.full(162, 0.7))
+    plan, keep = gpu_plan(ctx, case)
+    plan.set_form(ga.FORM_FACTORIZED, ga.PREC_BF16X2)
+    with pytest.raises(ga.GfxError):
+        plan.assemble()
 
 
 def test_csc_export_is_getfem_orientation(ctx):
