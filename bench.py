@@ -202,9 +202,12 @@ def run_gpu(args):
         "dtype": "f64" if "fp64" in variants[0][0] else "f32",
         "data": "synthetic (seeded jittered mesh, lognormal lambda/mu)",
         "config": {
-            "workload": "%s, %d^3x6 = %d tets, %d dofs, %s, per-element lambda/mu; step = geometry + "
-                        "contraction/scatter into the prebuilt CSR" % (WORKLOAD[args.config], n, n_cv,
-                                                                       ndof, im),
+            "workload": ("%s, %d^3 = %d hexes, %d dofs, %s; step = element matrices and residuals + "
+                         "node gathers into the prebuilt CSR" % (WORKLOAD[args.config], n, n_cv, ndof, im))
+                        if hexes else
+                        ("%s, %d^3x6 = %d tets, %d dofs, %s, per-element lambda/mu; step = geometry + "
+                         "contraction/scatter into the prebuilt CSR" % (WORKLOAD[args.config], n, n_cv,
+                                                                        ndof, im)),
             "name": args.config,
             "form": variants[0][0],
             "elements": n_cv,
