@@ -669,17 +669,22 @@ __device__ __forceinline__ void bdb_factor(const float4 *hp, int m, const float 
 
 constexpr int kBdbMaxGroups = 8192;
 
+// rows of the element matrix staged at once: half of them when that half is a
+// whole number of 16-byte pieces (P3: 30 x 60 floats), else all (P2)
+__host__ __device__ constexpr int bdb_stage_rows(int nl) { return (nl / 2) * nl % 4 == 0 && nl % 2 == 0 ? nl / 2 : nl; }
+
 template <int ND>
 __global__ __launch_bounds__(256) void k_elast_bdb_mfma(int64_t n_cv, int nq, const double *__restrict__ rec,
                                                         const double *__restrict__ gref,
                                                         const double *__restrict__ qw, float *__restrict__ E) {
   constexpr int NL = 3 * ND, NT = (NL + 15) / 16, NP = NT * (NT + 1) / 2;
+  constexpr int RC = bdb_stage_rows(NL), KC = NL / RC;
   static_assert(NL * NL % 4 == 0, "element matrices are whole float4 runs");
   extern __shared__ float4 hs[];  // [4 * steps][ND]: (reference gradient of phi_i at point q,
-                                  // signed sqrt(|w_q|)), then one NL x NL float block per wavefront
+                                  // signed sqrt(|w_q|)), then one RC x NL float block per wavefront
   const int t = threadIdx.x, w = t / 64, l = t % 64;
   const int steps = (nq + 3) / 4;
-  float *es = reinterpret_cast<float *>(hs + 4 * steps * ND) + w * NL * NL;
+  float *es = reinterpret_cast<float *>(hs + 4 * steps * ND) + w * RC * NL;
   int neg = 0;
   for (int x = t; x < 4 * steps * ND; x += 256) {
     const int q = x / ND;
@@ -736,30 +741,34 @@ __global__ __launch_bounds__(256) void k_elast_bdb_mfma(int64_t n_cv, int nq, co
         }
       }
     }
-    int p = 0;
+    // KC row chunks of RC rows, each staged whole and written as 16-byte pieces
 #pragma unroll
-    for (int b = 0; b < NT; ++b)
+    for (int h = 0; h < KC; ++h) {
+      int p = 0;
 #pragma unroll
-      for (int a = 0; a <= b; ++a, ++p) {
-        const int col = 16 * b + lr;
+      for (int b = 0; b < NT; ++b)
 #pragma unroll
-        for (int x = 0; x < 4; ++x) {
-          const int row = 16 * a + 4 * qq + x;
-          if (row < NL && col < NL) {
-            es[row * NL + col] = acc[p][x];
-            if (a != b) es[col * NL + row] = acc[p][x];
+        for (int a = 0; a <= b; ++a, ++p) {
+          const int col = 16 * b + lr;
+#pragma unroll
+          for (int x = 0; x < 4; ++x) {
+            const int row = 16 * a + 4 * qq + x;
+            if (row < NL && col < NL) {
+              if (row / RC == h) es[(row - RC * h) * NL + col] = acc[p][x];
+              if (a != b && col / RC == h) es[(col - RC * h) * NL + row] = acc[p][x];
+            }
           }
         }
-      }
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    const f32x4 *es4 = reinterpret_cast<const f32x4 *>(es);
-    f32x4 *E4 = reinterpret_cast<f32x4 *>(E + e * NL * NL);
-    for (int x = l; x < NL * NL / 4; x += 64) __builtin_nontemporal_store(es4[x], E4 + x);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // reads done before the next element's writes
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const f32x4 *es4 = reinterpret_cast<const f32x4 *>(es);
+      f32x4 *E4 = reinterpret_cast<f32x4 *>(E + e * NL * NL + RC * h * NL);
+      for (int x = l; x < RC * NL / 4; x += 64) __builtin_nontemporal_store(es4[x], E4 + x);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");  // reads done before the next writes
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
   }
 }
 
@@ -797,7 +806,7 @@ hipError_t launch_elast_bdb_mfma(hipStream_t s, int64_t n_cv, int nd, int nq, co
                                  const double *gref, const double *qw, float *E) {
   if (n_cv == 0) return hipSuccess;
   // one float4 per (point, node), then an element matrix per wavefront
-  const size_t lds = 16 * (size_t)(4 * ((nq + 3) / 4)) * nd + 4 * 4 * (size_t)(9 * nd * nd);
+  const size_t lds = 16 * (size_t)(4 * ((nq + 3) / 4)) * nd + 4 * 4 * (size_t)(bdb_stage_rows(3 * nd) * 3 * nd);
   if (nq < 1 || lds > 160 * 1024) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)std::min<int64_t>((n_cv + 3) / 4, kBdbMaxGroups);  // 4 wavefronts each
   if (nd == 10) k_elast_bdb_mfma<10><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);

@@ -208,3 +208,29 @@ def test_full_size_stored_nnz_equals_reference(ctx, name, n, fem, im, nd):
         worst = max(worst, np.linalg.norm(vals - vals_ref) / np.linalg.norm(vals_ref))
     print(name, "sampled rows", len(rows), "worst rel error", worst, flush=True)
     assert worst <= TOL_FP64
+
+
+def test_full_size_bf16x2_against_fp64(ctx):
+    """P3 elasticity 32^3x6 (196,608 tets, IM_TETRAHEDRON(8)): the bf16x2
+    element-batched form on the full mesh against the FP64 factorized form of
+    the same plan (itself pinned to the oracle above), relative Frobenius
+    within TOL_BF16X2 and no entry off by more than 1e-4 of the largest."""
+    import torch
+    pts, cvx = meshes.regular_unit_mesh(32)
+    pts = meshes.jitter(pts, 32)
+    lam, mu = meshes.lognormal_lame(cvx.shape[0])
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    mf = ga.MeshFem(mesh, "FEM_PK(3,3)", 3)
+    mim = ga.MeshIm(mesh, "IM_TETRAHEDRON(8)")
+    plan = ga.Plan(mf, mim, ga.TERM_ISO_ELAST, [lam, mu])
+    plan.build_pattern()
+    plan.set_form(ga.FORM_FACTORIZED, ga.PREC_FP64)
+    plan.assemble()
+    v64 = plan.torch_csr_view()[2].clone()
+    plan.set_form(ga.FORM_FACTORIZED, ga.PREC_BF16X2)
+    plan.assemble()
+    v = plan.torch_csr_view()[2]
+    err = float(torch.linalg.vector_norm(v - v64) / torch.linalg.vector_norm(v64))
+    worst = float((v - v64).abs().max() / v64.abs().max())
+    print("bf16x2 vs fp64 at full size: rel frobenius", err, "max entry", worst, flush=True)
+    assert err <= TOL_BF16X2 and worst <= 1e-4
