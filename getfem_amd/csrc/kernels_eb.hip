@@ -450,19 +450,23 @@ __global__ __launch_bounds__(256) void k_keep_emat(int64_t n_cv, int nl, int kwo
 }
 
 // node-patch gather of the rows of buffered element matrices: the Q rows of
-// an incidence are one contiguous run of Q*NL doubles; the runs of all the
-// node's incidences are streamed by the whole workgroup at once (independent
-// loads, LDS atomics), then the node's CSR rows are written once
+// an incidence are one contiguous run of Q*NL values (FP64 or FP32); the runs
+// of all the node's incidences are streamed by the whole workgroup at once
+// (independent loads, LDS atomics), then the node's CSR rows are written once.
+// TH threads per workgroup: 256 for the Q2 hexahedra (one node, long runs),
+// 64 for the affine tets' packed wave groups (short runs, more groups in
+// flight); wavefront w writes rows w, w + TH/64, ... of each node.
 constexpr int kGatherThreads = 256;
 
 // npw (1..4) small patches of one bin share a workgroup: their incidence
-// lists are concatenated and streamed together.
+// lists are concatenated and streamed together; patch k's accumulator starts
+// at the prefix sum of Q*Q*ns over the patches before it.
 constexpr int kGatherNpw = 4;
 
 template <int ND, int Q, typename TE, int TH = kGatherThreads>
 __global__ __launch_bounds__(TH) void k_gather_emat(PatchArgs a, const TE *__restrict__ E,
-                                                                const double *__restrict__ thr) {
-  extern __shared__ double acc[];  // npw accumulators of lds_acc doubles
+                                                    const double *__restrict__ thr) {
+  extern __shared__ double acc[];  // the npw accumulators, acc_cap doubles in all
   constexpr int kMaxInc = 64;      // incidences per workgroup (checked by the launcher)
   __shared__ int64_t sbase[kMaxInc];   // element-matrix run of each incidence
   __shared__ double sthr[kMaxInc];
@@ -805,9 +809,9 @@ hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision) {
 hipError_t launch_elast_bdb_mfma(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,
                                  const double *gref, const double *qw, float *E) {
   if (n_cv == 0) return hipSuccess;
-  // one float4 per (point, node), then an element matrix per wavefront
+  // one float4 per (point, node), then the staged rows of one element matrix per wavefront
   const size_t lds = 16 * (size_t)(4 * ((nq + 3) / 4)) * nd + 4 * 4 * (size_t)(bdb_stage_rows(3 * nd) * 3 * nd);
-  if (nq < 1 || lds > 160 * 1024) return hipErrorInvalidValue;
+  if (nq < 1 || lds > 64 * 1024) return hipErrorInvalidValue;
   const unsigned grid = (unsigned)std::min<int64_t>((n_cv + 3) / 4, kBdbMaxGroups);  // 4 wavefronts each
   if (nd == 10) k_elast_bdb_mfma<10><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
   else if (nd == 20) k_elast_bdb_mfma<20><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
