@@ -507,6 +507,17 @@ __global__ __launch_bounds__(TH) void k_gather_emat(PatchArgs a, const TE *__res
   auto sel = [](const int (&v)[kGatherNpw], int k) {  // by value: no indexed array
     return k == 0 ? v[0] : (k == 1 ? v[1] : (k == 2 ? v[2] : v[3]));
   };
+  // presence words of the patches that are not full (patch k's at the prefix
+  // sum wpre[k]), staged in LDS behind the accumulators with the other staged
+  // data, so that the write-out makes no global loads
+  int wpre[kGatherNpw];
+  int wtot = 0;
+#pragma unroll
+  for (int k = 0; k < kGatherNpw; ++k) {
+    wpre[k] = wtot;
+    wtot += (dsc[k].ninc_ns >> 31) ? 0 : (Q * Q * gns[k] + 31) / 32;
+  }
+  uint32_t *swords = reinterpret_cast<uint32_t *>(acc + a.lds_acc);
   uint32_t inc = 0;
   if (t < total_inc) {
     const int k = patch_of(t);
@@ -515,6 +526,10 @@ __global__ __launch_bounds__(TH) void k_gather_emat(PatchArgs a, const TE *__res
   for (int x = t; x < total_inc * ND; x += TH) {
     const int c = x / ND, k = patch_of(c);
     sslot[x] = a.slot[((int64_t)sel(gi0, k) + (c - sel(pre4, k))) * ND + (x - c * ND)];
+  }
+  for (int x = t; x < wtot; x += TH) {
+    const int k = (int)(x >= wpre[1]) + (int)(x >= wpre[2]) + (int)(x >= wpre[3]);
+    swords[x] = a.mask[(int64_t)sel(gi0, k) * a.mwpi + (x - sel(wpre, k))];
   }
   for (int x = t; x < apre; x += TH) acc[x] = 0.0;
   if (t < total_inc) {
@@ -564,7 +579,7 @@ __global__ __launch_bounds__(TH) void k_gather_emat(PatchArgs a, const TE *__res
     // wavefront x writes row x from the presence bits; the node's rows are
     // consecutive CSR rows, so row x starts after the present entries of
     // rows 0..x-1 (bits [0, x*ns*Q) of the node's presence words)
-    const uint32_t *m = a.mask + (int64_t)dsc[k].i0 * a.mwpi;
+    const uint32_t *m = swords + wpre[k];
     for (int x = w; x < Q; x += TH / 64) {
       const int off = x * ns * Q, len = ns * Q;
       int cnt = 0;
@@ -837,8 +852,11 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const void *E,
   const int64_t groups = (a.n_nodes + a.npw - 1) / a.npw;
   a.xcd_chunk = (int)((groups + 7) / 8);
   const unsigned grid = (unsigned)(8 * (int64_t)a.xcd_chunk);
-  const size_t lds = (size_t)acc_cap * sizeof(double);
-  if (acc_cap < 0 || lds > 48 * 1024 || group_inc > 64) return hipErrorInvalidValue;
+  // accumulators, then the presence words: at most ceil(Q*Q*ns/32) per patch,
+  // so acc_cap/32 + kGatherNpw in all
+  if (acc_cap < 0 || 8 * (size_t)acc_cap > 48 * 1024 || group_inc > 64) return hipErrorInvalidValue;
+  a.lds_acc = acc_cap;
+  const size_t lds = (size_t)acc_cap * sizeof(double) + 4 * (size_t)(acc_cap / 32 + kGatherNpw);
   const double *E64 = static_cast<const double *>(E);
   const float *E32 = static_cast<const float *>(E);
   if (a.nd == 27 && a.qdim == 3) {
