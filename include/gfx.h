@@ -57,7 +57,9 @@ extern "C" {
                                  element matrices B^T D B on the matrix cores (bf16
                                  hi + lo split, three products, FP32 accumulation),
                                  buffered in FP32 and gathered into the CSR in FP64;
-                                 relative error about 1e-5 of the element norm      */
+                                 relative error about 1e-5 of the element norm; the
+                                 form argument is ignored; needs mu > 0 and
+                                 3 lambda + 2 mu > 0 (checked at assembly)          */
 
 /* ---- coefficient sources ---- */
 #define GFX_COEF_CONST 0       /* one value                                 */
