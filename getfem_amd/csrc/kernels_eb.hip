@@ -541,69 +541,48 @@ __global__ __launch_bounds__(TH) void k_gather_emat(PatchArgs a, const TE *__res
     sns[t] = sel(gns, k);
   }
   __syncthreads();
-  if constexpr (TH == 64) {
-    // one wavefront (the affine tets' packed groups, runs of 90 or 180)
-    // the stream: incidence by incidence (uniform across the workgroup), each
-    // thread owning the offsets t, t + TH, ... of the incidence's run, whose
-    // (row, node, component) split is fixed; kI incidences' loads are issued
-    // before any is used (offsets past the run load entry 0 and are dropped)
-    constexpr int NV = (RUN + TH - 1) / TH;
-    constexpr int kI = NV >= 16 ? 1 : 16 / NV;
-    int ov[NV], xv[NV], jv[NV], yv[NV];
-    bool okv[NV];
+  // the stream: incidence by incidence (uniform across the workgroup), each
+  // thread owning the offsets t, t + TH, ... of the incidence's run, whose
+  // (row, node, component) split is fixed; kI incidences' loads are issued
+  // before any is used (offsets past the run load entry 0 and are dropped).
+  // One wavefront (affine tets, runs of 90 or 180, 12-24 incidences per
+  // group) loads kI incidences with clamped indices; 256 threads (Q2
+  // hexahedra, runs of 243, often fewer incidences than kI) skip the loads
+  // past the group's last incidence.
+  constexpr int NV = (RUN + TH - 1) / TH;
+  constexpr int kI = TH == 64 ? (NV >= 16 ? 1 : 16 / NV) : 8;
+  int ov[NV], xv[NV], jv[NV], yv[NV];
+  bool okv[NV];
 #pragma unroll
-    for (int v = 0; v < NV; ++v) {
-      const int o = t + v * TH;
-      okv[v] = o < RUN;
-      ov[v] = okv[v] ? o : 0;
-      xv[v] = ov[v] / NL;
-      const int c = ov[v] - xv[v] * NL;
-      jv[v] = c / Q;
-      yv[v] = c - jv[v] * Q;
-    }
-    for (int c0 = 0; c0 < total_inc; c0 += kI) {
-      TE val[kI][NV];
+  for (int v = 0; v < NV; ++v) {
+    const int o = t + v * TH;
+    okv[v] = o < RUN;
+    ov[v] = okv[v] ? o : 0;
+    xv[v] = ov[v] / NL;
+    const int c = ov[v] - xv[v] * NL;
+    jv[v] = c / Q;
+    yv[v] = c - jv[v] * Q;
+  }
+  for (int c0 = 0; c0 < total_inc; c0 += kI) {
+    TE val[kI][NV];
 #pragma unroll
-      for (int i = 0; i < kI; ++i) {
+    for (int i = 0; i < kI; ++i) {
+      if (TH == 64 || c0 + i < total_inc) {  // uniform
         const int64_t base = sbase[min(c0 + i, total_inc - 1)];
 #pragma unroll
         for (int v = 0; v < NV; ++v) val[i][v] = __builtin_nontemporal_load(E + base + ov[v]);  // read once
       }
-#pragma unroll
-      for (int i = 0; i < kI; ++i) {
-        const int k = c0 + i;
-        if (k >= total_inc) break;
-        const double th = sthr[k];
-        const int ns = sns[k], ak = sacc[k];
-#pragma unroll
-        for (int v = 0; v < NV; ++v) {
-          const double x = (double)val[i][v];
-          if (okv[v] && fabs(x) > th) atomicAdd(&acc[ak + (xv[v] * ns + sslot[k * ND + jv[v]]) * Q + yv[v]], x);
-        }
-      }
     }
-  } else {
-    // 256 threads (Q2 hexahedra, runs of 243, few incidences per group): the
-    // stream flattened over the group's incidences, kB entries per thread in flight: the loads use clamped indices (always a
-    // valid entry) and are issued before any of them is used
-    constexpr int kB = 8;
-    const int total = total_inc * RUN;
-    for (int f0 = 0; f0 < total; f0 += kB * TH) {
-      double v[kB];
 #pragma unroll
-      for (int u = 0; u < kB; ++u) {
-        const int f = min(f0 + u * TH + t, total - 1);
-        const int k = f / RUN, o = f - k * RUN;
-        v[u] = (double)__builtin_nontemporal_load(E + sbase[k] + o);  // read once
-      }
+    for (int i = 0; i < kI; ++i) {
+      const int k = c0 + i;
+      if (k >= total_inc) break;
+      const double th = sthr[k];
+      const int ns = sns[k], ak = sacc[k];
 #pragma unroll
-      for (int u = 0; u < kB; ++u) {
-        const int f = f0 + u * TH + t;
-        if (f >= total) break;
-        const int k = f / RUN, o = f - k * RUN;
-        const int x = o / NL, c = o - x * NL, j = c / Q, y = c - j * Q;
-        const int ns = sns[k];
-        if (fabs(v[u]) > sthr[k]) atomicAdd(&acc[sacc[k] + (x * ns + sslot[k * ND + j]) * Q + y], v[u]);
+      for (int v = 0; v < NV; ++v) {
+        const double x = (double)val[i][v];
+        if (okv[v] && fabs(x) > th) atomicAdd(&acc[ak + (xv[v] * ns + sslot[k * ND + jv[v]]) * Q + yv[v]], x);
       }
     }
   }
