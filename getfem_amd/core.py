@@ -184,6 +184,13 @@ class Plan:
         check(lib().gfx_plan_set_coef(self.h, slot, ctypes.byref(c)), self.ctx.h)
 
     def set_form(self, form=_lib.FORM_FACTORIZED, precision=_lib.PREC_FP64):
+        """Contraction form and precision of the next assemblies: FORM_FACTORIZED
+        (reference tensor, affine elements) or FORM_PERQP (per integration
+        point); PREC_FP64, PREC_FP32 (FP32 element contraction, FP64
+        accumulation) or PREC_BF16X2 (isotropic elasticity on P2/P3 vector
+        fields only: element matrices B^T D B on the matrix cores with a bf16
+        hi + lo split, gathered into the same CSR; the form is ignored).  An
+        unsupported combination raises GfxError."""
         check(lib().gfx_plan_set_form(self.h, form, precision), self.ctx.h)
 
     def build_pattern(self):
