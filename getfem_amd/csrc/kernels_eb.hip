@@ -753,9 +753,11 @@ __global__ __launch_bounds__(256) void k_elast_bdb_mfma(int64_t n_cv, int nq, co
     float Bm[9];
 #pragma unroll
     for (int x = 0; x < 9; ++x) Bm[x] = (float)nx[x];
-    const double sj = sqrt(nx[9]), lam = nx[10], mu = nx[11];
-    const double s2m = sqrt(2.0 * mu), c = (sqrt(2.0 * mu + 3.0 * lam) - s2m) / 3.0;
-    const float on = (float)(sj * (s2m + c)), off = (float)(sj * c), sh = (float)(sj * sqrt(mu));
+    // S's factors in FP32 (the products are rounded to bf16 pairs anyway)
+    const float sj = sqrtf((float)nx[9]), lam = (float)nx[10], mu = (float)nx[11];
+    // c = (sqrt(2 mu + 3 lambda) - sqrt(2 mu)) / 3 without the cancellation
+    const float s2m = sqrtf(2.f * mu), c = lam / (sqrtf(2.f * mu + 3.f * lam) + s2m);
+    const float on = sj * (s2m + c), off = sj * c, sh = sj * sqrtf(mu);
     const int64_t en = min(e + stride, n_cv - 1);
 #pragma unroll
     for (int x = 0; x < 9; ++x) nx[x] = rec[kRecStride * en + x];

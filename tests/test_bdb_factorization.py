@@ -57,6 +57,20 @@ def test_symmetric_root_squares_to_d(lam, mu):
     assert np.allclose(s @ s, voigt_d(lam, mu), rtol=1e-13, atol=1e-13 * (lam + mu))
 
 
+@pytest.mark.parametrize("lam,mu", [(1.0, 0.5), (1e-6, 1.0), (-0.5, 1.0), (3.7, 0.01)])
+def test_cancellation_free_c_in_fp32(lam, mu):
+    """The kernel evaluates c as lambda / (sqrt(2 mu + 3 lambda) + sqrt(2 mu))
+    in FP32: the same number as the difference form, and S^2 = D to FP32
+    accuracy of the material scale even for lambda << mu."""
+    f = np.float32
+    c32 = f(lam) / (np.sqrt(f(2) * f(mu) + f(3) * f(lam)) + np.sqrt(f(2) * f(mu)))
+    c64 = (np.sqrt(2 * mu + 3 * lam) - np.sqrt(2 * mu)) / 3
+    assert abs(float(c32) - c64) <= 4e-7 * abs(c64) + 1e-30
+    s = sym_root(lam, mu)
+    s[:3, :3] += float(c32) - c64  # the FP32 c in place of the exact one
+    assert np.abs(s @ s - voigt_d(lam, mu)).max() <= 2e-6 * (abs(lam) + mu)
+
+
 def test_voigt_rows_give_the_isotropic_pair_block():
     rng = np.random.default_rng(3)
     g = rng.normal(size=(10, 3))
