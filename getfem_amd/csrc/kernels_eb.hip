@@ -711,6 +711,34 @@ __device__ __forceinline__ void bdb_factor(const float4 *hp, int m, const float 
 
 constexpr int kBdbMaxGroups = 8192;
 
+// the k-steps of one element: fragments of the NT dof tiles at 4 points, then
+// the upper tiles' three bf16 products; NEG: C = sign(w_q) F (one XOR)
+template <int ND, bool NEG>
+__device__ __forceinline__ void bdb_steps(const float4 *hs, int steps, int qq, int lr, const float (&Bm)[9],
+                                          float on, float off, float sh,
+                                          f32x4 (&acc)[((3 * ND + 15) / 16) * ((3 * ND + 15) / 16 + 1) / 2]) {
+  constexpr int NL = 3 * ND, NT = (NL + 15) / 16;
+  for (int k = 0; k < steps; ++k) {
+    const float4 *hp = hs + (4 * k + qq) * ND;
+    bf16x8 fh[NT], fl[NT];
+#pragma unroll
+    for (int a = 0; a < NT; ++a) bdb_factor<NL>(hp, 16 * a + lr, Bm, on, off, sh, fh[a], fl[a]);
+    const uint32_t sgn = NEG && hp[0].w < 0.f ? 0x80008000u : 0u;
+    int p = 0;
+#pragma unroll
+    for (int b = 0; b < NT; ++b) {
+      const bf16x8 ch = NEG ? bf16_xor(fh[b], sgn) : fh[b];
+      const bf16x8 cl = NEG ? bf16_xor(fl[b], sgn) : fl[b];
+#pragma unroll
+      for (int a = 0; a <= b; ++a, ++p) {
+        acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fl[a], ch, acc[p], 0, 0, 0);
+        acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], cl, acc[p], 0, 0, 0);
+        acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], ch, acc[p], 0, 0, 0);
+      }
+    }
+  }
+}
+
 // rows of the element matrix staged at once: half of them when that half is a
 // whole number of 16-byte pieces (P3: 30 x 60 floats), else all (P2)
 __host__ __device__ constexpr int bdb_stage_rows(int nl) { return (nl / 2) * nl % 4 == 0 && nl % 2 == 0 ? nl / 2 : nl; }
@@ -766,25 +794,10 @@ __global__ __launch_bounds__(256) void k_elast_bdb_mfma(int64_t n_cv, int nq, co
     f32x4 acc[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) acc[p] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k = 0; k < steps; ++k) {
-      const float4 *hp = hs + (4 * k + qq) * ND;
-      bf16x8 fh[NT], fl[NT];
-#pragma unroll
-      for (int a = 0; a < NT; ++a) bdb_factor<NL>(hp, 16 * a + lr, Bm, on, off, sh, fh[a], fl[a]);
-      const uint32_t sgn = any_neg && hp[0].w < 0.f ? 0x80008000u : 0u;  // C = sign(w_q) F
-      int p = 0;
-#pragma unroll
-      for (int b = 0; b < NT; ++b) {
-        const bf16x8 ch = any_neg ? bf16_xor(fh[b], sgn) : fh[b];
-        const bf16x8 cl = any_neg ? bf16_xor(fl[b], sgn) : fl[b];
-#pragma unroll
-        for (int a = 0; a <= b; ++a, ++p) {
-          acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fl[a], ch, acc[p], 0, 0, 0);
-          acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], cl, acc[p], 0, 0, 0);
-          acc[p] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(fh[a], ch, acc[p], 0, 0, 0);
-        }
-      }
-    }
+    if (any_neg)  // uniform: only rules with a negative weight pay for the signs
+      bdb_steps<ND, true>(hs, steps, qq, lr, Bm, on, off, sh, acc);
+    else
+      bdb_steps<ND, false>(hs, steps, qq, lr, Bm, on, off, sh, acc);
     // KC row chunks of RC rows, each staged whole and written as 16-byte pieces
 #pragma unroll
     for (int h = 0; h < KC; ++h) {
