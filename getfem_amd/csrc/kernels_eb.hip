@@ -463,11 +463,11 @@ constexpr int kGatherThreads = 256;
 // at the prefix sum of Q*Q*ns over the patches before it.
 constexpr int kGatherNpw = 4;
 
-template <int ND, int Q, typename TE, int TH = kGatherThreads>
+template <int ND, int Q, typename TE, int TH = kGatherThreads, int MI = 64>
 __global__ __launch_bounds__(TH) void k_gather_emat(PatchArgs a, const TE *__restrict__ E,
                                                     const double *__restrict__ thr) {
   extern __shared__ double acc[];  // the npw accumulators, acc_cap doubles in all
-  constexpr int kMaxInc = 64;      // incidences per workgroup (checked by the launcher)
+  constexpr int kMaxInc = MI;      // incidences per workgroup (checked by the launcher)
   __shared__ int64_t sbase[kMaxInc];   // element-matrix run of each incidence
   __shared__ double sthr[kMaxInc];
   __shared__ uint16_t sslot[kMaxInc * ND];
@@ -904,9 +904,13 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const void *E,
     if (e_fp32) k_gather_emat<8, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
     else k_gather_emat<8, 3, double><<<grid, kGatherThreads, lds, s>>>(a, E64, thr);
   } else if (a.nd == 10 && a.qdim == 3 && e_fp32) {  // element-batched affine P2 (bf16 split form)
-    k_gather_emat<10, 3, float, 64><<<grid, 64, lds, s>>>(a, E32, thr);
+    // groups of at most 32 incidences (the packed waves): half the static
+    // LDS per group, so more groups fit on a CU
+    if (group_inc <= 32) k_gather_emat<10, 3, float, 64, 32><<<grid, 64, lds, s>>>(a, E32, thr);
+    else k_gather_emat<10, 3, float, 64><<<grid, 64, lds, s>>>(a, E32, thr);
   } else if (a.nd == 20 && a.qdim == 3 && e_fp32) {  // element-batched affine P3
-    k_gather_emat<20, 3, float, 64><<<grid, 64, lds, s>>>(a, E32, thr);
+    if (group_inc <= 32) k_gather_emat<20, 3, float, 64, 32><<<grid, 64, lds, s>>>(a, E32, thr);
+    else k_gather_emat<20, 3, float, 64><<<grid, 64, lds, s>>>(a, E32, thr);
   }
   else return hipErrorInvalidValue;
   return hipGetLastError();
