@@ -49,9 +49,9 @@ BF16_PEAK_TFLOPS = 2500.0  # MI355X dense bf16 matrix peak
 # (bench.py --copy-calibration).  Every form of a config has the same memory
 # pattern (FP64 values and accumulation), so one figure serves all forms.
 PMC_TRAFFIC_KIB = {  # config: (kernel, FETCH_SIZE KiB, WRITE_SIZE KiB)
-    "c2": ("k_assemble_patch", 455098, 4520967),
-    "c5": ("k_assemble_patch", 180084, 3187788),
-    "c3": ("k_nh_element", 55194, 6041018),
+    "c2": ("k_assemble_patch", 455101, 4521372),
+    "c5": ("k_assemble_patch", 180081, 3182486),
+    "c3": ("k_nh_element", 55194, 6042268),
 }
 FETCH_FACTOR, WRITE_FACTOR = 2.0, 1.0
 
