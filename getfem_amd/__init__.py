@@ -4,9 +4,9 @@
 791-936) on a subset of GWFL expressions, executed by hand-written HIP kernels
 for gfx950 behind the C ABI of ``include/gfx.h`` (libgfx).
 """
-from ._lib import (COEF_CONST, COEF_PER_ELEMENT, FORM_FACTORIZED, FORM_PERQP,  # noqa: F401
-                   PREC_BF16X2, PREC_FP32, PREC_FP64, TERM_ISO_ELAST, TERM_LAPLACE, TERM_NEOHOOKEAN,
-                   GfxError)
+from ._lib import (COEF_CONST, COEF_PER_ELEMENT, CSC, CSR, FORM_FACTORIZED, FORM_PERQP,  # noqa: F401
+                   PREC_BF16X2, PREC_FP32, PREC_FP64, TERM_ISO_ELAST, TERM_ISOTROPIC, TERM_LAPLACE,
+                   TERM_NEOHOOKEAN, GfxError)
 from .core import Context, Mesh, MeshFem, MeshIm, Plan  # noqa: F401
 from . import meshes  # noqa: F401
 

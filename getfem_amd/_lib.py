@@ -15,7 +15,8 @@ ERRORS = {
     1: "GFX_E_INVALID", 2: "GFX_E_UNSUPPORTED", 3: "GFX_E_DEGENERATE_ELEMENT",
     4: "GFX_E_OOM", 5: "GFX_E_HIP", 6: "GFX_E_STATE", 7: "GFX_E_PATTERN",
 }
-TERM_LAPLACE, TERM_ISO_ELAST, TERM_NEOHOOKEAN = 0, 1, 2
+TERM_LAPLACE, TERM_ISO_ELAST, TERM_NEOHOOKEAN, TERM_ISOTROPIC = 0, 1, 2, 3
+CSR, CSC = 0, 1
 FORM_FACTORIZED, FORM_PERQP = 0, 1
 PREC_FP64, PREC_FP32, PREC_BF16X2 = 0, 1, 2
 COEF_CONST, COEF_PER_ELEMENT = 0, 1
@@ -38,6 +39,7 @@ SIGNATURES = {
     "gfx_fem_eval": (_i32, [_cs, _P, ctypes.POINTER(_i32), _P, _P]),
     "gfx_enumerate_dofs": (_i32, [_cs, _cs, _i32, _i64, _P, _i64, _P, _P, ctypes.POINTER(_i64)]),
     "gfx_gwfl_compile": (_i32, [_cs, _cs, ctypes.c_char_p, _i64]),
+    "gfx_gwfl_order": (_i32, [_cs, _cs, ctypes.POINTER(_i32)]),
     "gfx_ctx_create": (_i32, [_i32, _pp]),
     "gfx_ctx_destroy": (_i32, [_P]),
     "gfx_ctx_set_stream": (_i32, [_P, _P]),
@@ -64,6 +66,11 @@ SIGNATURES = {
     "gfx_plan_build_pattern": (_i32, [_P, ctypes.POINTER(_i64)]),
     "gfx_plan_assemble": (_i32, [_P]),
     "gfx_plan_set_state": (_i32, [_P, _P]),
+    "gfx_plan_set_state_device": (_i32, [_P, _P]),
+    "gfx_plan_pin_pattern": (_i32, [_P, _i32]),
+    "gfx_plan_pattern_id": (_i32, [_P, ctypes.POINTER(_i64)]),
+    "gfx_plan_spmv_add": (_i32, [_P, _P, _P]),
+    "gfx_plan_add_to_matrix": (_i32, [_P, _i32, _i64, _P, _P, _P]),
     "gfx_plan_assemble_residual": (_i32, [_P]),
     "gfx_plan_residual": (_i32, [_P, _P]),
     "gfx_plan_spmv": (_i32, [_P, _P, _P]),
@@ -86,6 +93,13 @@ SIGNATURES = {
     "gfx_ws_add_fem_constant": (_i32, [_P, _cs, _P, _P]),
     "gfx_ws_add_fixed_size_constant": (_i32, [_P, _cs, _P, _i64]),
     "gfx_ws_set_values": (_i32, [_P, _cs, _P]),
+    "gfx_ws_set_values_device": (_i32, [_P, _cs, _P]),
+    "gfx_ws_set_assembled_matrix": (_i32, [_P, _i32, _i64, _P, _P, _P]),
+    "gfx_ws_device_csr": (_i32, [_P, ctypes.POINTER(_i64), ctypes.POINTER(_i64), _pp, _pp, _pp]),
+    "gfx_ws_matrix_dims": (_i32, [_P, ctypes.POINTER(_i64), ctypes.POINTER(_i64)]),
+    "gfx_ws_export_csr": (_i32, [_P, _P, _P, _P]),
+    "gfx_ws_export_csc": (_i32, [_P, _P, _P, _P]),
+    "gfx_ws_nb_trees": (_i32, [_P]),
     "gfx_ws_add_expression": (_i32, [_P, _cs, _P, _i32]),
     "gfx_ws_set_form": (_i32, [_P, _i32, _i32]),
     "gfx_ws_compile": (_i32, [_P]),
@@ -96,6 +110,7 @@ SIGNATURES = {
     "gfx_ws_describe": (_i32, [_P, ctypes.c_char_p, _i64]),
     "gfx_compliance_create": (_i32, [_P, _P, _f64, _f64, _f64, _f64, _pp]),
     "gfx_compliance_eval": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "gfx_compliance_status": (_i32, [_P]),
     "gfx_compliance_destroy": (_i32, [_P]),
 }
 

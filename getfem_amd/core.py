@@ -203,7 +203,35 @@ class Plan:
         return self.nnz
 
     def assemble(self):
+        """Order-2 assembly.  After new coefficients, moved nodes or a new
+        state the thresholds are re-derived and the pattern is rebuilt if
+        GetFEM's would differ; while torch views of the CSR are alive the
+        pattern is pinned and such a change raises GfxError instead."""
+        self._sync_pin()
         check(lib().gfx_plan_assemble(self.h), self.ctx.h)
+
+    def _sync_pin(self):
+        check(lib().gfx_plan_pin_pattern(self.h, 1 if len(self._views) else 0), self.ctx.h)
+
+    def pattern_id(self):
+        """Counter of pattern (re)builds: device pointers taken from
+        device_csr() are valid while it does not change."""
+        i = ctypes.c_int64()
+        check(lib().gfx_plan_pattern_id(self.h, ctypes.byref(i)), self.ctx.h)
+        return i.value
+
+    def set_state_device(self, d_U):
+        """State of a nonlinear plan from a device vector (raw address)."""
+        check(lib().gfx_plan_set_state_device(self.h, ctypes.c_void_p(d_U)), self.ctx.h)
+
+    def add_to_matrix(self, indptr, indices, values, orientation=_lib.CSR):
+        """Add the assembled values into a caller-owned device matrix given
+        by torch tensors (CSR rows = test dofs, or CSC in GetFEM's orientation)
+        whose pattern holds this plan's."""
+        n = indptr.numel() - 1
+        check(lib().gfx_plan_add_to_matrix(self.h, orientation, n, ctypes.c_void_p(indptr.data_ptr()),
+                                           ctypes.c_void_p(indices.data_ptr()),
+                                           ctypes.c_void_p(values.data_ptr())), self.ctx.h)
 
     def set_state(self, U):
         """Current displacement of a nonlinear (Neo-Hookean) plan."""
@@ -306,6 +334,7 @@ class Plan:
             buf = _DeviceBuffer(self, ptr, cnt, ts)
             self._views.add(buf)
             out.append(torch.as_tensor(buf, device="cuda"))
+        self._sync_pin()
         return tuple(out)
 
     def set_timing(self, enable=True):

@@ -227,6 +227,7 @@ int gfx_mesh_set_points(gfx_mesh *m, const double *h_pts) {
   std::copy(h_pts, h_pts + 3 * m->n_pts, m->h_pts.begin());
   HIPCHK(c, hipMemcpyAsync(m->d_pts, h_pts, 24 * (size_t)m->n_pts, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  m->version++;  // plans on this mesh re-derive their thresholds and pattern
   return GFX_OK;
 }
 
@@ -378,7 +379,13 @@ static int plan_upload_coef(gfx_plan *p, int slot, const gfx_coef *cf) {
     HIPCHK(c, dalloc(&p->d_coef[slot], (size_t)n));
     HIPCHK(c, hipMemcpy(p->d_coef[slot], cf->h_values, 8 * (size_t)n, hipMemcpyHostToDevice));
   }
+  p->values_changed = true;  // thresholds and pattern are re-derived at the next assembly
   return GFX_OK;
+}
+
+// number of coefficients of a term
+static int term_coefs(int term) {
+  return term == GFX_TERM_LAPLACE ? 1 : (term == GFX_TERM_ISOTROPIC ? 3 : 2);
 }
 
 int gfx_plan_destroy(gfx_plan *p) {
@@ -388,7 +395,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   for (hipStream_t st : p->aux) (void)hipStreamDestroy(st);
   for (hipEvent_t e : p->join) (void)hipEventDestroy(e);
   if (p->fork) (void)hipEventDestroy(p->fork);
-  dfree(p->d_coef[0]); dfree(p->d_coef[1]);
+  dfree(p->d_coef[0]); dfree(p->d_coef[1]); dfree(p->d_coef[2]);
   dfree(p->d_gref); dfree(p->d_qw); dfree(p->d_R);
   dfree(p->d_rec); dfree(p->d_thr); dfree(p->d_keep);
   dfree(p->d_gref32); dfree(p->d_qw32); dfree(p->d_R32); dfree(p->d_rec32); dfree(p->d_Rq); dfree(p->d_E32);
@@ -466,12 +473,15 @@ int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef
   } else {
     if (m->gt != GT_TET_P1 || mf->fem.kind != FEM_PK)
       return fail(c, GFX_E_UNSUPPORTED, "this plan kind needs FEM_PK(3,k) on GT_PK(3,1)");
-    if (!((term == GFX_TERM_LAPLACE && (Q == 1 || Q == 3)) || (term == GFX_TERM_ISO_ELAST && Q == 3)))
+    if (!((term == GFX_TERM_LAPLACE && (Q == 1 || Q == 3)) ||
+          ((term == GFX_TERM_ISO_ELAST || term == GFX_TERM_ISOTROPIC) && Q == 3)))
       return fail(c, GFX_E_UNSUPPORTED, "term / qdim combination not supported");
     if (!(nd == 4 || nd == 10 || nd == 20) || (term == GFX_TERM_LAPLACE && Q == 3 && nd == 20))
       return fail(c, GFX_E_UNSUPPORTED, "element degree not supported for this term");
   }
-  const int need = term == GFX_TERM_LAPLACE ? 1 : 2;
+  if (term < GFX_TERM_LAPLACE || term > GFX_TERM_ISOTROPIC)
+    return fail(c, GFX_E_UNSUPPORTED, "unknown term");
+  const int need = term_coefs(term);
   if (n_coefs != need || !coefs) return fail(c, GFX_E_INVALID, "wrong number of coefficients");
   if (set_device(c)) return GFX_E_HIP;
   gfx_plan *p = new (std::nothrow) gfx_plan();
@@ -483,8 +493,10 @@ int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef
   p->mwpi = (nd * Q * Q + 31) / 32;
   p->ndof = mf->ndof;
   p->n_nodes = mf->ndof / Q;
-  p->coef[1].kind = GFX_COEF_CONST;
-  p->coef[1].value = 0.0;
+  for (int s = 0; s < 3; ++s) {
+    p->coef[s].kind = GFX_COEF_CONST;
+    p->coef[s].value = 0.0;
+  }
   for (int s = 0; s < need; ++s) {
     int r = plan_upload_coef(p, s, &coefs[s]);
     if (r) { gfx_plan_destroy(p); return r; }
@@ -543,7 +555,7 @@ int gfx_plan_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, int term, const gfx_coef
 }
 
 int gfx_plan_set_coef(gfx_plan *p, int slot, const gfx_coef *cf) {
-  if (!p || !cf || slot < 0 || slot > 1) return GFX_E_INVALID;
+  if (!p || !cf || slot < 0 || slot >= term_coefs(p->term)) return GFX_E_INVALID;
   if (set_device(p->ctx)) return GFX_E_HIP;
   return plan_upload_coef(p, slot, cf);
 }
@@ -569,13 +581,16 @@ static int plan_records(gfx_plan *p, bool with_thr) {
   gfx_ctx *c = p->ctx;
   const bool fp32 = with_thr && p->precision == GFX_PREC_FP32;
   if (fp32 && !p->d_rec32) HIPCHK(c, dalloc(&p->d_rec32, (size_t)kRecStride * m->n_cv));
-  HIPCHK(c, launch_records_tet(c->stream, m->n_cv, m->d_pts, m->d_cvx,
-                               p->coef[0].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[0] : nullptr,
-                               p->coef[0].value,
-                               p->coef[1].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[1] : nullptr,
-                               p->coef[1].value, with_thr ? p->d_thr : nullptr,
-                               p->rk ? 1.0 / p->r_L : 1.0, p->d_rec, fp32 ? p->d_rec32 : nullptr,
-                               p->d_err));
+  // coefficient c of the record: mu for elasticity (its Laplacian part), the
+  // third coefficient of the isotropic term, none for the Laplacian
+  const int sc = p->term == GFX_TERM_ISOTROPIC ? 2 : (p->term == GFX_TERM_ISO_ELAST ? 1 : -1);
+  auto dptr = [&](int k) -> const double * {
+    return k >= 0 && p->coef[k].kind == GFX_COEF_PER_ELEMENT ? p->d_coef[k] : nullptr;
+  };
+  HIPCHK(c, launch_records_tet(c->stream, m->n_cv, m->d_pts, m->d_cvx, dptr(0), p->coef[0].value,
+                               dptr(1), p->coef[1].value, dptr(sc), sc >= 0 ? p->coef[sc].value : 0.0,
+                               with_thr ? p->d_thr : nullptr, p->rk ? 1.0 / p->r_L : 1.0, p->d_rec,
+                               fp32 ? p->d_rec32 : nullptr, p->d_err));
   return GFX_OK;
 }
 
@@ -763,6 +778,7 @@ static int plan_waves(gfx_plan *p, const std::vector<uint16_t> &ns, const std::v
 int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   if (!p) return GFX_E_INVALID;
   gfx_ctx *c = p->ctx;
+  if (p->pinned) return fail(c, GFX_E_STATE, "the pattern is pinned (callers hold its CSR arrays)");
   if (set_device(c)) return GFX_E_HIP;
   const gfx_mesh *m = p->mf->mesh;
   const int64_t n_cv = m->n_cv, nn = p->n_nodes;
@@ -787,6 +803,9 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
     if (p->eb && p->ndof) HIPCHK(c, hipMemsetAsync(p->d_rhs, 0, 8 * (size_t)p->ndof, s));
     HIPCHK(c, hipStreamSynchronize(s));
     p->nnz = 0;
+    p->pts_version = m->version;
+    p->values_changed = false;
+    p->pattern_id++;
     if (nnz_out) *nnz_out = 0;
     return GFX_OK;
   }
@@ -858,9 +877,67 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   HIPCHK(c, launch_fill_columns(s, a));
   HIPCHK(c, hipStreamSynchronize(s));
   dfree(p->d_nbr);
-  dfree(p->d_keep);
+  // the keep bits stay: re-validation after a change of values compares
+  // the pattern they give with this one
   p->nnz = nnz;
+  p->pts_version = m->version;
+  p->values_changed = false;
+  p->pattern_id++;
   if (nnz_out) *nnz_out = nnz;
+  return GFX_OK;
+}
+
+// The values behind the stored pattern changed (coefficients, node positions
+// or state): re-derive the per-element thresholds and keep bits at the new
+// values, as GetFEM's per-assembly threshold does, and compare the presence
+// bits they give with the stored ones.  Equal: the pattern stands and the
+// new thresholds are used.  Different: the pattern is rebuilt (refused while
+// pinned).  A degenerate element is reported here, like GetFEM's "Non
+// invertible matrix" at every assembly.  For element-batched plans the FP64
+// element matrices computed here serve the assembly that follows (*eb_fresh).
+static int plan_validate(gfx_plan *p, bool *eb_fresh) {
+  gfx_ctx *c = p->ctx;
+  const gfx_mesh *m = p->mf->mesh;
+  hipStream_t s = c->stream;
+  *eb_fresh = false;
+  if (p->nnz == 0) {  // no element: nothing to re-derive
+    p->pts_version = m->version;
+    p->values_changed = false;
+    return GFX_OK;
+  }
+  int r;
+  HIPCHK(c, hipMemsetAsync(p->d_err, 0, 4, s));
+  if (!p->d_keep) HIPCHK(c, dalloc(&p->d_keep, (size_t)m->n_cv * p->kwords));
+  PatternArgs a{};
+  a.n_cv = m->n_cv; a.n_nodes = p->n_nodes; a.nd = p->nd; a.qdim = p->qdim; a.nloc = p->nloc;
+  a.kwords = p->kwords; a.mwpi = p->mwpi; a.dofs = p->mf->d_dofs; a.rec = p->d_rec;
+  a.gref = p->d_gref; a.qw = p->d_qw; a.nq = p->nq; a.thr = p->d_thr; a.keep = p->d_keep;
+  a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot; a.nslots = p->d_nslots;
+  a.mask = p->d_mask; a.err = p->d_err;
+  if (p->eb) {
+    if ((r = plan_eb_elements(p, GFX_PREC_FP64))) return r;
+    HIPCHK(c, launch_keep_emat(s, m->n_cv, p->nloc, p->kwords, p->d_E, p->d_thr, p->d_keep));
+  } else {
+    if ((r = plan_records(p, false))) return r;
+    HIPCHK(c, launch_keep(s, p->term, a));
+  }
+  HIPCHK(c, launch_mask_check(s, a, p->max_nslots));
+  int flag = 0;
+  HIPCHK(c, hipMemcpyAsync(&flag, p->d_err, 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (flag & 1) return fail(c, GFX_E_DEGENERATE_ELEMENT, "Non invertible matrix (degenerate element)");
+  if (flag & 4) {
+    if (p->pinned)
+      return fail(c, GFX_E_STATE, "the new values change the stored pattern, which is pinned "
+                                  "(callers hold its CSR arrays): unpin it and assemble again");
+    int64_t nnz = 0;
+    if ((r = gfx_plan_build_pattern(p, &nnz))) return r;
+    *eb_fresh = p->eb;  // the rebuild ran the FP64 element kernel at the current values
+    return GFX_OK;
+  }
+  p->pts_version = m->version;
+  p->values_changed = false;
+  *eb_fresh = p->eb;
   return GFX_OK;
 }
 
@@ -870,6 +947,9 @@ int gfx_plan_assemble(gfx_plan *p) {
   if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
   if (set_device(c)) return GFX_E_HIP;
   int r;
+  bool eb_fresh = false;  // element-batched: E and thresholds already computed in FP64
+  if (p->values_changed || p->pts_version != p->mf->mesh->version)
+    if ((r = plan_validate(p, &eb_fresh))) return r;
   hipEvent_t *ev = nullptr;
   if (p->timing) {
     const size_t need = 3 * (size_t)(p->n_timed + 1);
@@ -882,7 +962,8 @@ int gfx_plan_assemble(gfx_plan *p) {
     p->n_timed++;
     HIPCHK(c, hipEventRecord(ev[0], c->stream));
   }
-  if ((r = p->eb ? plan_eb_elements(p, p->precision) : plan_records(p, true))) return r;
+  if (!(eb_fresh && p->precision == GFX_PREC_FP64))
+    if ((r = p->eb ? plan_eb_elements(p, p->precision) : plan_records(p, true))) return r;
   const bool bdb = !p->eb && p->precision == GFX_PREC_BF16X2;
   if (bdb) {  // element matrices on the matrix cores, then the node-patch gather below
     const int64_t n_cv = p->mf->mesh->n_cv;
@@ -961,6 +1042,29 @@ int gfx_plan_set_state(gfx_plan *p, const double *h_U) {
   if (set_device(c)) return GFX_E_HIP;
   HIPCHK(c, hipMemcpyAsync(p->d_U, h_U, 8 * (size_t)p->ndof, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  p->values_changed = true;  // the element matrices, thresholds and pattern follow the state
+  return GFX_OK;
+}
+
+int gfx_plan_set_state_device(gfx_plan *p, const double *d_U) {
+  if (!p || !d_U) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (!p->eb) return fail(c, GFX_E_STATE, "this plan has no state");
+  if (set_device(c)) return GFX_E_HIP;
+  HIPCHK(c, hipMemcpyAsync(p->d_U, d_U, 8 * (size_t)p->ndof, hipMemcpyDeviceToDevice, c->stream));
+  p->values_changed = true;
+  return GFX_OK;
+}
+
+int gfx_plan_pin_pattern(gfx_plan *p, int pin) {
+  if (!p) return GFX_E_INVALID;
+  p->pinned = pin != 0;
+  return GFX_OK;
+}
+
+int gfx_plan_pattern_id(gfx_plan *p, int64_t *id) {
+  if (!p || !id) return GFX_E_INVALID;
+  *id = p->pattern_id;
   return GFX_OK;
 }
 
@@ -974,6 +1078,34 @@ int gfx_plan_assemble_residual(gfx_plan *p) {
   if ((r = plan_eb_elements(p, p->precision))) return r;
   HIPCHK(c, launch_gather_vec(c->stream, p->n_nodes, p->nd, p->qdim, p->d_inc_ptr, p->d_inc, p->d_res,
                               p->d_rhs));
+  return GFX_OK;
+}
+
+int gfx_plan_add_to_matrix(gfx_plan *p, int orientation, int64_t n, const int64_t *d_ptr,
+                           const int32_t *d_ind, double *d_vals) {
+  if (!p || !d_ptr || !d_ind || !d_vals || (orientation != GFX_CSR && orientation != GFX_CSC))
+    return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern and assemble first");
+  if (n != p->ndof) return fail(c, GFX_E_INVALID, "target matrix of another size");
+  if (set_device(c)) return GFX_E_HIP;
+  hipStream_t s = c->stream;
+  HIPCHK(c, hipMemsetAsync(p->d_err, 0, 4, s));
+  HIPCHK(c, launch_add_into(s, orientation == GFX_CSC, p->ndof, p->d_indptr, p->d_indices, p->d_vals, d_ptr,
+                            d_ind, d_vals, p->d_err));
+  int flag = 0;
+  HIPCHK(c, hipMemcpyAsync(&flag, p->d_err, 4, hipMemcpyDeviceToHost, s));
+  HIPCHK(c, hipStreamSynchronize(s));
+  if (flag & 8) return fail(c, GFX_E_PATTERN, "the target matrix lacks entries of the assembled pattern");
+  return GFX_OK;
+}
+
+int gfx_plan_spmv_add(gfx_plan *p, const double *d_x, double *d_y) {
+  if (!p || !d_x || !d_y) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
+  if (set_device(c)) return GFX_E_HIP;
+  HIPCHK(c, launch_spmv(c->stream, p->ndof, p->nnz, p->d_indptr, p->d_indices, p->d_vals, d_x, d_y, true));
   return GFX_OK;
 }
 
@@ -1214,6 +1346,21 @@ int gfx_compliance_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, double lambda, dou
     return hipfail(ctx, e, "gfx_compliance_create");
   }
   *out = c;
+  return GFX_OK;
+}
+
+int gfx_compliance_status(gfx_compliance *c) {
+  if (!c) return GFX_E_INVALID;
+  gfx_ctx *ctx = c->ctx;
+  if (set_device(ctx)) return GFX_E_HIP;
+  int flag = 0;
+  HIPCHK(ctx, hipMemcpyAsync(&flag, c->d_err, 4, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  if (flag) {
+    HIPCHK(ctx, hipMemsetAsync(c->d_err, 0, 4, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    return fail(ctx, GFX_E_DEGENERATE_ELEMENT, "Non invertible matrix (degenerate element)");
+  }
   return GFX_OK;
 }
 

@@ -28,6 +28,7 @@ struct gfx_ctx : gfx_refcount {
 
 struct gfx_mesh : gfx_refcount {
   gfx_ctx *ctx = nullptr;
+  uint64_t version = 0;        // incremented by gfx_mesh_set_points
   int gt = 0;
   int nv = 4;
   int64_t n_pts = 0, n_cv = 0;
@@ -56,8 +57,10 @@ namespace gfx {
 constexpr int kRecStride = 16;  // doubles per element record
 
 // Per-element record written by the geometry kernel (affine tetrahedra):
-// [0..8] B = K^{-T} row-major, [9] |det K|, [10] coefficient a,
-// [11] coefficient b, [12] insertion threshold, [13..15] unused.
+// [0..8] B = K^{-T} row-major, [9] |J| a s, [10] |J| b s (s: scale of the
+// stored reference tensor), [11] insertion threshold, [12] |J| a, [13] |J| b,
+// [14] |J| c, [15] |J| c s; a, b, c are the term's coefficients (Laplace: a;
+// elasticity: lambda, mu, mu; isotropic: alpha, beta, gamma).
 
 // What a patch kernel needs first about its node, in one 16-byte load:
 // the node's first CSR value, its incidence range and its slot count (bit 15:
@@ -155,14 +158,16 @@ void unref(gfx_mesh_im *mim);
 // kernel launch wrappers (kernels.hip)
 hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,
                               const int32_t *cvx, const double *ca, double ca_c,
-                              const double *cb, double cb_c, const double *thr,
-                              double r_scale, double *rec, float *rec32, int *err);
+                              const double *cb, double cb_c, const double *cc, double cc_c,
+                              const double *thr, double r_scale, double *rec, float *rec32,
+                              int *err);
 hipError_t launch_incidences(hipStream_t s, int64_t n_cv, int nd, int qdim,
                              const int32_t *dofs, int64_t n_nodes,
                              int32_t *inc_ptr, uint32_t *inc, int32_t *tmp);
 hipError_t launch_keep(hipStream_t s, int term, const PatternArgs &a);
 hipError_t launch_patch_build(hipStream_t s, const PatternArgs &a, int max_cand);
 hipError_t launch_fill_columns(hipStream_t s, const PatternArgs &a);
+hipError_t launch_mask_check(hipStream_t s, const PatternArgs &a, int max_nslots);
 hipError_t launch_fill_desc(hipStream_t s, int64_t n, int qdim, const int32_t *order, const int32_t *inc_ptr,
                             const uint16_t *nslots, const int64_t *indptr, PatchDesc *desc);
 hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec, const PatchArgs &a);
@@ -185,7 +190,18 @@ hipError_t launch_gather_vec(hipStream_t s, int64_t n_nodes, int nd, int Q, cons
 
 // spmv.hip
 hipError_t launch_spmv(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr,
-                       const int32_t *indices, const double *vals, const double *x, double *y);
+                       const int32_t *indices, const double *vals, const double *x, double *y,
+                       bool acc = false);
+hipError_t launch_add_into(hipStream_t s, bool csc, int64_t n, const int64_t *ip, const int32_t *ix,
+                           const double *v, const int64_t *tp, const int32_t *ti, double *tv, int *err);
+hipError_t launch_union_count(hipStream_t s, int64_t n, const int64_t *ipa, const int32_t *ixa,
+                              const int64_t *ipb, const int32_t *ixb, int64_t *cnt);
+hipError_t launch_union_fill(hipStream_t s, int64_t n, const int64_t *ipa, const int32_t *ixa,
+                             const int64_t *ipb, const int32_t *ixb, const int64_t *ipu, int32_t *ixu,
+                             int64_t *mapa, int64_t *mapb);
+hipError_t launch_scatter_add(hipStream_t s, int64_t nnz, const int64_t *map, const double *v, double *out);
+hipError_t launch_gather_i64(hipStream_t s, int64_t n, const int64_t *map, const int64_t *src, int64_t *dst);
+hipError_t launch_fill_f64(hipStream_t s, int64_t n, double *p, double v);
 hipError_t launch_dof_constraints(hipStream_t s, int64_t n, const int64_t *indptr, const int32_t *indices,
                                   double *vals, const uint8_t *fixed, const double *go, const double *pr,
                                   double *rhs, int flags, int *err);
@@ -225,9 +241,18 @@ struct gfx_plan {
   int term = GFX_TERM_LAPLACE;
   int nd = 0, qdim = 1, nloc = 0, nq = 0, nv = 4;
   int kwords = 0, mwpi = 0;
-  gfx_coef coef[2];
-  double coef_min[2] = {0.0, 0.0};  // smallest value of each coefficient (host copy)
-  double *d_coef[2] = {nullptr, nullptr};
+  gfx_coef coef[3];
+  double coef_min[3] = {0.0, 0.0, 0.0};  // smallest value of each coefficient (host copy)
+  double *d_coef[3] = {nullptr, nullptr, nullptr};
+  // The stored pattern is GetFEM's for the values it was built from.  After
+  // the coefficients, the node positions (mesh version) or the state change,
+  // the next assembly re-derives the thresholds and keep bits and rebuilds
+  // the pattern if its presence bits differ (GetFEM thresholds every element
+  // matrix at every assembly, compile_and_exec.cc:5441-5453).
+  uint64_t pts_version = 0;  // mesh version the thresholds were derived from
+  bool values_changed = false;
+  bool pinned = false;       // callers hold the CSR arrays: a rebuild is refused
+  int64_t pattern_id = 0;    // incremented by every pattern (re)build
   double *d_gref = nullptr, *d_qw = nullptr, *d_R = nullptr;
   double *d_rec = nullptr, *d_thr = nullptr;
   // FP32 contraction form: the tables and the element records in FP32

@@ -476,18 +476,64 @@ struct Parser {
 
 }  // namespace
 
+namespace {
+
+// the finite strain brick's expression (getfem_nonlinear_elasticity.cc:2319-2320)
+bool match_neohookean(const std::string &expr, std::smatch &m, std::string &squeezed) {
+  static const std::regex nh(
+      R"(^\(\(Id\(meshdim\)\+Grad_(\w+)\)\*\(Compressible_Neo_Hookean_Bonet_PK2\(Grad_\1,\[(\w+);(\w+)\]\)\)\):Grad_Test_\1$)");
+  squeezed.clear();
+  for (char c : expr) if (!std::isspace((unsigned char)c)) squeezed += c;
+  return std::regex_match(squeezed, m, nh);
+}
+
+// highest test-function order among the monomials of a polynomial
+int test_order(const Poly &p, const Ctx &C) {
+  int order = 0;
+  for (auto &kv : p)
+    for (int s : kv.first) {
+      const std::string &nm = C.symnames[s];
+      const size_t c = nm.find(':');
+      if (c == std::string::npos || nm.substr(0, c) == "c") continue;
+      const std::string tag = nm.substr(c + 1, nm.find('_', c) - c - 1);
+      if (tag == "S" || tag == "GS") order = std::max(order, 2);
+      else if (tag == "T" || tag == "GT") order = std::max(order, 1);
+    }
+  return order;
+}
+
+}  // namespace
+
+int gwfl_expression_order(const std::string &expr, const std::map<std::string, GwflVar> &vars,
+                          std::string &err) {
+  std::smatch m;
+  std::string sq;
+  if (match_neohookean(expr, m, sq)) return 1;
+  Ctx C;
+  C.vars = &vars;
+  try {
+    Parser P(expr, C);
+    Tensor t = P.expr();
+    P.skip();
+    if (P.pos != expr.size()) fail("trailing characters in expression");
+    if (t.order() != 0) fail("the expression is not a scalar");
+    return test_order(t.v[0], C);
+  } catch (const Error &e) {
+    err = e.msg;
+  } catch (const std::exception &e) {
+    err = std::string("parse error: ") + e.what();
+  }
+  return -1;
+}
+
 bool gwfl_compile_order2(const std::vector<std::string> &exprs,
                          const std::map<std::string, GwflVar> &vars, GwflKernelParams &out,
                          std::string &err) {
   out = GwflKernelParams();
-  // the finite strain brick's expression (getfem_nonlinear_elasticity.cc:2319-2320)
-  static const std::regex nh(
-      R"(^\(\(Id\(meshdim\)\+Grad_(\w+)\)\*\(Compressible_Neo_Hookean_Bonet_PK2\(Grad_\1,\[(\w+);(\w+)\]\)\)\):Grad_Test_\1$)");
   if (exprs.size() == 1) {
     std::string e;
-    for (char c : exprs[0]) if (!std::isspace((unsigned char)c)) e += c;
     std::smatch m;
-    if (std::regex_match(e, m, nh)) {
+    if (match_neohookean(exprs[0], m, e)) {
       out.term = GFX_TERM_NEOHOOKEAN;
       out.unknown = m[1];
       out.coef[0].terms.push_back({1.0, {m[2]}});
@@ -605,27 +651,38 @@ bool gwfl_compile_order2(const std::vector<std::string> &exprs,
       return true;
     }
     if (q == 3) {
-      const Poly lam = get("00", "11"), mu = get("01", "01");
+      // D_{ab,cd} = alpha d_ab d_cd + beta d_ad d_bc + gamma d_ac d_bd: the
+      // div-div, transposed-gradient and gradient:gradient contractions
+      const Poly alpha3 = get("00", "11"), beta = get("01", "10"), gamma = get("01", "01");
       bool iso = true;
       for (int a = 0; a < 3 && iso; ++a)
         for (int b = 0; b < 3 && iso; ++b)
           for (int c = 0; c < 3 && iso; ++c)
             for (int d = 0; d < 3 && iso; ++d) {
               Poly e;
-              if (a == b && c == d) padd_to(e, lam);
-              if (a == c && b == d) padd_to(e, mu);
-              if (a == d && b == c) padd_to(e, mu);
+              if (a == b && c == d) padd_to(e, alpha3);
+              if (a == d && b == c) padd_to(e, beta);
+              if (a == c && b == d) padd_to(e, gamma);
               iso = pequal(get(idx(a, b), idx(c, d)), e);
             }
-      if (iso) {
+      if (iso && pequal(beta, gamma)) {
         out.term = GFX_TERM_ISO_ELAST;
-        coef_of(lam, out.coef[0]);
-        coef_of(mu, out.coef[1]);
+        coef_of(alpha3, out.coef[0]);
+        coef_of(beta, out.coef[1]);
         out.describe = "isotropic elasticity";
         return true;
       }
+      if (iso) {
+        out.term = GFX_TERM_ISOTROPIC;
+        coef_of(alpha3, out.coef[0]);
+        coef_of(beta, out.coef[1]);
+        coef_of(gamma, out.coef[2]);
+        out.describe = "isotropic";
+        return true;
+      }
     }
-    fail("the order-2 term is not one of the supported forms (Laplacian, isotropic elasticity)");
+    fail("the order-2 term is not one of the supported forms (Laplacian, isotropic elasticity, "
+         "isotropic div-div + transposed-gradient + gradient terms)");
   } catch (const Error &e) {
     err = e.msg;
     return false;

@@ -30,9 +30,17 @@ struct GwflCoef {
 struct GwflKernelParams {
   int term = -1;           // GFX_TERM_*
   std::string unknown;     // name of the variable
-  GwflCoef coef[2];        // LAPLACE: a ; ISO_ELAST: lambda, mu
+  GwflCoef coef[3];        // LAPLACE: a ; ISO_ELAST: lambda, mu ; ISOTROPIC: alpha, beta, gamma
   std::string describe;    // human-readable normal form
 };
+
+// The test-function order of one expression (0: potential, 1: residual form
+// with Test_u, 2: bilinear form with Test_u and Test2_u), the number of
+// test-function kinds it contains, as ga_workspace::add_expression computes
+// it before deriving the higher orders (src/getfem_generic_assembly_
+// workspace.cc:545-600).  Returns -1 with a message on a parse error.
+int gwfl_expression_order(const std::string &expr, const std::map<std::string, GwflVar> &vars,
+                          std::string &err);
 
 // Compile the sum of expressions (each with its derivation order request) for
 // an order-2 assembly.  Returns false with a message on unsupported input.

@@ -46,8 +46,8 @@ __device__ __forceinline__ double wave_max(double v) {
 __global__ __launch_bounds__(256) void k_records_tet(
     int64_t n_cv, const double *__restrict__ pts, const int32_t *__restrict__ cvx,
     const double *__restrict__ ca, double ca_c, const double *__restrict__ cb,
-    double cb_c, const double *__restrict__ thr, double r_scale, double *__restrict__ rec,
-    float *__restrict__ rec32, int *__restrict__ err) {
+    double cb_c, const double *__restrict__ cc, double cc_c, const double *__restrict__ thr,
+    double r_scale, double *__restrict__ rec, float *__restrict__ rec32, int *__restrict__ err) {
   // The block's records are staged in LDS (rows padded by one 16-byte piece
   // against bank conflicts) and written back as one contiguous run: 16-byte
   // pieces of consecutive records from consecutive threads.
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(256) void k_records_tet(
     // Record: B = K^{-T} from cofactors, row-major B(r,c) = o[3r+c]; then the
     // factorized form's scales |J| a s and |J| b s (s: the scale of the stored
     // reference tensor, 1/L for an integer table R = N/L), the insertion
-    // threshold, and |J|, a, b for the per-point form
+    // threshold, the per-point form's |J| a, |J| b, |J| c, and |J| c s
     double o[16];
     o[0] = a0 / J; o[1] = (k[2] * k[7] - k[1] * k[8]) / J;
     o[2] = (k[1] * k[5] - k[2] * k[4]) / J; o[3] = a1 / J;
@@ -83,9 +83,10 @@ __global__ __launch_bounds__(256) void k_records_tet(
     o[6] = a2 / J; o[7] = (k[1] * k[6] - k[0] * k[7]) / J;
     o[8] = (k[0] * k[4] - k[1] * k[3]) / J;
     const double aJ = fabs(J), va = ca ? ca[e] : ca_c, vb = cb ? cb[e] : cb_c;
+    const double vc = cc ? cc[e] : cc_c;
     o[9] = aJ * va * r_scale; o[10] = aJ * vb * r_scale;
     o[11] = thr ? thr[e] : 0.0;
-    o[12] = aJ; o[13] = va; o[14] = vb; o[15] = 0.0;
+    o[12] = aJ * va; o[13] = aJ * vb; o[14] = aJ * vc; o[15] = aJ * vc * r_scale;
     if (rec32) {
       float4 *st4 = reinterpret_cast<float4 *>(stage + 9 * threadIdx.x);
       for (int t = 0; t < 4; ++t)
@@ -130,6 +131,10 @@ __device__ __forceinline__ void pair_block(const Real *__restrict__ rec, int i, 
                                            int nd, int nq, const Real *__restrict__ gref,
                                            const Real *__restrict__ qw,
                                            const void *__restrict__ R, Real v[Q][Q]) {
+  // Terms (rows (i, a), columns (j, b), g = B grad(phi) at a point):
+  //   LAPLACE     a g_i.g_j delta_ab
+  //   ISO_ELAST   lambda g_ia g_jb + mu g_ib g_ja + mu delta_ab g_i.g_j
+  //   ISOTROPIC   alpha g_ia g_jb + beta g_ib g_ja + gamma delta_ab g_i.g_j
   Real B[9];
   for (int t = 0; t < 9; ++t) B[t] = (Real)rec[t];
   if (FORM == GFX_FORM_FACTORIZED) {
@@ -157,12 +162,18 @@ __device__ __forceinline__ void pair_block(const Real *__restrict__ rec, int i, 
       for (int a = 0; a < Q; ++a)
         for (int b = 0; b < Q; ++b)
           v[a][b] = sa * G[a][b] + sb * (G[b][a] + (a == b ? tr : Real(0)));
+    } else if (TERM == GFX_TERM_ISOTROPIC) {
+      const Real sc = rec[15];  // |J| c s
+      for (int a = 0; a < Q; ++a)
+        for (int b = 0; b < Q; ++b)
+          v[a][b] = sa * G[a][b] + sb * G[b][a] + (a == b ? sc * tr : Real(0));
     } else {
       for (int a = 0; a < Q; ++a)
         for (int b = 0; b < Q; ++b) v[a][b] = (a == b) ? sa * tr : Real(0);
     }
   } else {
-    const Real aJ = rec[12], ca = rec[13], cb = rec[14];
+    const Real ca = rec[12], cb = rec[13];  // |J| a, |J| b
+    const Real cc = TERM == GFX_TERM_ISOTROPIC ? rec[14] : cb;
     for (int a = 0; a < Q; ++a)
       for (int b = 0; b < Q; ++b) v[a][b] = Real(0);
     for (int q = 0; q < nq; ++q) {
@@ -175,14 +186,14 @@ __device__ __forceinline__ void pair_block(const Real *__restrict__ rec, int i, 
         gi[a] = B[3 * a] * h0 + B[3 * a + 1] * h1 + B[3 * a + 2] * h2;
         gj[a] = B[3 * a] * k0 + B[3 * a + 1] * k1 + B[3 * a + 2] * k2;
       }
-      const Real cq = aJ * (Real)qw[q];
+      const Real wq = (Real)qw[q];
       const Real d = gi[0] * gj[0] + gi[1] * gj[1] + gi[2] * gj[2];
-      if (TERM == GFX_TERM_ISO_ELAST) {
+      if (TERM == GFX_TERM_ISO_ELAST || TERM == GFX_TERM_ISOTROPIC) {
         for (int a = 0; a < Q; ++a)
           for (int b = 0; b < Q; ++b)
-            v[a][b] += cq * (ca * gi[a] * gj[b] + cb * gi[b] * gj[a] + (a == b ? cb * d : Real(0)));
+            v[a][b] += wq * (ca * gi[a] * gj[b] + cb * gi[b] * gj[a] + (a == b ? cc * d : Real(0)));
       } else {
-        for (int a = 0; a < Q; ++a) v[a][a] += cq * (ca * d);
+        for (int a = 0; a < Q; ++a) v[a][a] += wq * (ca * d);
       }
     }
   }
@@ -387,6 +398,42 @@ __global__ __launch_bounds__(256) void k_patch_build(PatternArgs a, int pmax) {
     a.nslots[node] = (uint16_t)(ns | (tot == Q * Q * ns ? kFullRows : 0));
   }
   if (threadIdx.x < Q) a.rowcnt[node * Q + threadIdx.x] = rowc[threadIdx.x];
+}
+
+// Re-validation of a built pattern after the values it was built from changed
+// (coefficients, node positions, the state of a nonlinear plan): the
+// presence words of every node are recomputed from the new keep bits through
+// the stored slot maps (the slots depend on the topology only) and compared
+// with the stored ones.  Any difference sets bit 4 of the error flag: the
+// stored pattern is no longer GetFEM's and must be rebuilt.  Dynamic LDS:
+// mwords u32 (the largest node's presence words).
+__global__ __launch_bounds__(128) void k_mask_check(PatternArgs a) {
+  extern __shared__ uint32_t maskC[];
+  const int64_t node = blockIdx.x;
+  if (node >= a.n_nodes) return;
+  const int nd = a.nd, Q = a.qdim, nl = a.nloc;
+  const int i0 = a.inc_ptr[node], ninc = a.inc_ptr[node + 1] - i0;
+  const int ns = a.nslots[node] & (kFullRows - 1);
+  const int mw = (Q * Q * ns + 31) / 32;
+  for (int t = threadIdx.x; t < mw; t += blockDim.x) maskC[t] = 0u;
+  __syncthreads();
+  const int per_inc = Q * nd * Q;
+  for (int t = threadIdx.x; t < ninc * per_inc; t += blockDim.x) {
+    const int k = t / per_inc, rem = t - k * per_inc;
+    const int x = rem / (nd * Q), c = rem - x * (nd * Q);
+    const uint32_t inc = a.inc[i0 + k];
+    const int64_t e = inc >> 5;
+    const int i = inc & 31;
+    const int bit = (i * Q + x) * nl + c;
+    if ((a.keep[e * a.kwords + (bit >> 5)] >> (bit & 31)) & 1u) {
+      const int j = c / Q, y = c - j * Q;
+      const int pos = (x * ns + (int)a.slot[(int64_t)(i0 + k) * nd + j]) * Q + y;
+      atomicOr(&maskC[pos >> 5], 1u << (pos & 31));
+    }
+  }
+  __syncthreads();
+  for (int t = threadIdx.x; t < mw; t += blockDim.x)
+    if (maskC[t] != a.mask[(int64_t)i0 * a.mwpi + t]) atomicOr(a.err, 4);
 }
 
 __device__ __forceinline__ bool mask_bit(const uint32_t *m, int b) {
@@ -757,11 +804,12 @@ static unsigned nblocks(int64_t n, int b) { return (unsigned)((n + b - 1) / b); 
 
 hipError_t launch_records_tet(hipStream_t s, int64_t n_cv, const double *pts,
                               const int32_t *cvx, const double *ca, double ca_c,
-                              const double *cb, double cb_c, const double *thr,
-                              double r_scale, double *rec, float *rec32, int *err) {
+                              const double *cb, double cb_c, const double *cc, double cc_c,
+                              const double *thr, double r_scale, double *rec, float *rec32,
+                              int *err) {
   if (n_cv == 0) return hipSuccess;
-  k_records_tet<<<nblocks(n_cv, 256), 256, 0, s>>>(n_cv, pts, cvx, ca, ca_c, cb, cb_c, thr,
-                                                    r_scale, rec, rec32, err);
+  k_records_tet<<<nblocks(n_cv, 256), 256, 0, s>>>(n_cv, pts, cvx, ca, ca_c, cb, cb_c, cc, cc_c,
+                                                    thr, r_scale, rec, rec32, err);
   return hipGetLastError();
 }
 
@@ -796,6 +844,8 @@ hipError_t launch_keep(hipStream_t s, int term, const PatternArgs &a) {
     k_keep<3, GFX_TERM_LAPLACE><<<nb, 256, lds, s>>>(a);
   else if (a.qdim == 3 && term == GFX_TERM_ISO_ELAST)
     k_keep<3, GFX_TERM_ISO_ELAST><<<nb, 256, lds, s>>>(a);
+  else if (a.qdim == 3 && term == GFX_TERM_ISOTROPIC)
+    k_keep<3, GFX_TERM_ISOTROPIC><<<nb, 256, lds, s>>>(a);
   else
     return hipErrorInvalidValue;
   return hipGetLastError();
@@ -813,6 +863,15 @@ hipError_t launch_patch_build(hipStream_t s, const PatternArgs &a, int max_cand)
   // small lists: smaller workgroups, so more nodes run at once
   const int threads = pmax >= 1024 ? 256 : (pmax >= 256 ? 128 : 64);
   k_patch_build<<<(unsigned)a.n_nodes, threads, lds, s>>>(a, pmax);
+  return hipGetLastError();
+}
+
+hipError_t launch_mask_check(hipStream_t s, const PatternArgs &a, int max_nslots) {
+  if (a.n_nodes == 0) return hipSuccess;
+  const size_t mwords = ((size_t)a.qdim * a.qdim * max_nslots + 31) / 32;
+  const size_t lds = 4 * (mwords > 0 ? mwords : 1);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  k_mask_check<<<(unsigned)a.n_nodes, 128, lds, s>>>(a);
   return hipGetLastError();
 }
 
@@ -868,6 +927,10 @@ hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec, co
     if (nd == 4) return launch_asm3<4, 3, GFX_TERM_ISO_ELAST>(s, form, prec, a, grid, lds);
     if (nd == 10) return launch_asm3<10, 3, GFX_TERM_ISO_ELAST>(s, form, prec, a, grid, lds);
     if (nd == 20) return launch_asm3<20, 3, GFX_TERM_ISO_ELAST>(s, form, prec, a, grid, lds);
+  } else if (term == GFX_TERM_ISOTROPIC && Q == 3) {
+    if (nd == 4) return launch_asm3<4, 3, GFX_TERM_ISOTROPIC>(s, form, prec, a, grid, lds);
+    if (nd == 10) return launch_asm3<10, 3, GFX_TERM_ISOTROPIC>(s, form, prec, a, grid, lds);
+    if (nd == 20) return launch_asm3<20, 3, GFX_TERM_ISOTROPIC>(s, form, prec, a, grid, lds);
   }
   return hipErrorInvalidValue;
 }

@@ -772,25 +772,26 @@ __global__ __launch_bounds__(256) void k_elast_bdb_mfma(int64_t n_cv, int nq, co
   const int64_t stride = 4 * (int64_t)gridDim.x;
   int64_t e = 4 * (int64_t)blockIdx.x + __builtin_amdgcn_readfirstlane(w);  // uniform: scalar record loads
   if (e >= n_cv) return;  // no barrier follows
-  double nx[12];  // the record of the element in flight: B row-major, |J|, lambda, mu
+  double nx[11];  // the record of the element in flight: B row-major, |J| lambda, |J| mu
 #pragma unroll
   for (int x = 0; x < 9; ++x) nx[x] = rec[kRecStride * e + x];
 #pragma unroll
-  for (int x = 0; x < 3; ++x) nx[9 + x] = rec[kRecStride * e + 12 + x];
+  for (int x = 0; x < 2; ++x) nx[9 + x] = rec[kRecStride * e + 12 + x];
   for (; e < n_cv; e += stride) {
     float Bm[9];
 #pragma unroll
     for (int x = 0; x < 9; ++x) Bm[x] = (float)nx[x];
-    // S's factors in FP32 (the products are rounded to bf16 pairs anyway)
-    const float sj = sqrtf((float)nx[9]), lam = (float)nx[10], mu = (float)nx[11];
+    // S's factors in FP32 (the products are rounded to bf16 pairs anyway); the
+    // record's coefficients carry |J|, and S(|J| lambda, |J| mu) = sqrt(|J|) S(lambda, mu)
+    const float lam = (float)nx[9], mu = (float)nx[10];
     // c = (sqrt(2 mu + 3 lambda) - sqrt(2 mu)) / 3 without the cancellation
     const float s2m = sqrtf(2.f * mu), c = lam / (sqrtf(2.f * mu + 3.f * lam) + s2m);
-    const float on = sj * (s2m + c), off = sj * c, sh = sj * sqrtf(mu);
+    const float on = s2m + c, off = c, sh = sqrtf(mu);
     const int64_t en = min(e + stride, n_cv - 1);
 #pragma unroll
     for (int x = 0; x < 9; ++x) nx[x] = rec[kRecStride * en + x];
 #pragma unroll
-    for (int x = 0; x < 3; ++x) nx[9 + x] = rec[kRecStride * en + 12 + x];
+    for (int x = 0; x < 2; ++x) nx[9 + x] = rec[kRecStride * en + 12 + x];
     f32x4 acc[NP];
 #pragma unroll
     for (int p = 0; p < NP; ++p) acc[p] = f32x4{0.f, 0.f, 0.f, 0.f};

@@ -22,7 +22,7 @@ constexpr int kSpmvThreads = 256;
 constexpr int kRedThreads = 256;   // threads of a reduction block
 constexpr int kRedBlocks = 1024;   // partial sums per dot product (fixed: deterministic order)
 
-template <int G>
+template <int G, bool ACC>
 __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int64_t n, const int64_t *__restrict__ indptr,
                                                         const int32_t *__restrict__ indices,
                                                         const double *__restrict__ vals,
@@ -36,16 +36,111 @@ __global__ __launch_bounds__(kSpmvThreads) void k_spmv(int64_t n, const int64_t 
     for (int64_t k = b + lane; k < e; k += G) s += vals[k] * x[indices[k]];
   }
   for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, G);
-  if (row < n && lane == 0) y[row] = s;
+  if (row < n && lane == 0) y[row] = ACC ? y[row] + s : s;
 }
 
 template <int G>
 hipError_t spmv_g(hipStream_t s, int64_t n, const int64_t *indptr, const int32_t *indices,
-                  const double *vals, const double *x, double *y) {
+                  const double *vals, const double *x, double *y, bool acc) {
   const int64_t threads = n * G;
   const unsigned grid = (unsigned)((threads + kSpmvThreads - 1) / kSpmvThreads);
-  k_spmv<G><<<grid, kSpmvThreads, 0, s>>>(n, indptr, indices, vals, x, y);
+  if (acc) k_spmv<G, true><<<grid, kSpmvThreads, 0, s>>>(n, indptr, indices, vals, x, y);
+  else k_spmv<G, false><<<grid, kSpmvThreads, 0, s>>>(n, indptr, indices, vals, x, y);
   return hipGetLastError();
+}
+
+// ------------------------------------------- adding into another matrix
+// The plan's CSR (rows = test dofs) added into a caller matrix whose pattern
+// holds it.  CSR target: one thread per row walks both sorted rows (the
+// target's row is a superset).  CSC target (GetFEM's col_matrix orientation:
+// column = trial dof, sorted rows = test dofs): each entry (r, c) is found by
+// a binary search in the target's column c.  An entry the target lacks sets
+// bit 8 of the error flag (the value is dropped).
+__global__ void k_add_into_csr(int64_t n, const int64_t *__restrict__ ip, const int32_t *__restrict__ ix,
+                               const double *__restrict__ v, const int64_t *__restrict__ tip,
+                               const int32_t *__restrict__ tix, double *__restrict__ tv, int *__restrict__ err) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  int64_t t = tip[row];
+  const int64_t te = tip[row + 1];
+  for (int64_t k = ip[row]; k < ip[row + 1]; ++k) {
+    const int32_t c = ix[k];
+    while (t < te && tix[t] < c) ++t;
+    if (t < te && tix[t] == c) tv[t] += v[k];
+    else atomicOr(err, 8);
+  }
+}
+
+__global__ void k_add_into_csc(int64_t n, const int64_t *__restrict__ ip, const int32_t *__restrict__ ix,
+                               const double *__restrict__ v, const int64_t *__restrict__ tcp,
+                               const int32_t *__restrict__ tri, double *__restrict__ tv, int *__restrict__ err) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  for (int64_t k = ip[row]; k < ip[row + 1]; ++k) {
+    const int32_t c = ix[k];
+    int64_t lo = tcp[c], hi = tcp[c + 1];
+    while (lo < hi) {
+      const int64_t mid = (lo + hi) / 2;
+      if (tri[mid] < row) lo = mid + 1;
+      else hi = mid;
+    }
+    if (lo < tcp[c + 1] && tri[lo] == row) tv[lo] += v[k];
+    else atomicOr(err, 8);
+  }
+}
+
+// union of two CSR patterns with the same rows: per-row merge counts, then
+// the merged columns and, for every entry of either input, its position in
+// the union
+__global__ void k_union_count(int64_t n, const int64_t *__restrict__ ipa, const int32_t *__restrict__ ixa,
+                              const int64_t *__restrict__ ipb, const int32_t *__restrict__ ixb,
+                              int64_t *__restrict__ cnt) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  int64_t a = ipa[row], ae = ipa[row + 1], b = ipb[row], be = ipb[row + 1], m = 0;
+  while (a < ae || b < be) {
+    const int32_t ca = a < ae ? ixa[a] : INT32_MAX, cb = b < be ? ixb[b] : INT32_MAX;
+    a += ca <= cb;
+    b += cb <= ca;
+    ++m;
+  }
+  cnt[row] = m;
+}
+
+__global__ void k_union_fill(int64_t n, const int64_t *__restrict__ ipa, const int32_t *__restrict__ ixa,
+                             const int64_t *__restrict__ ipb, const int32_t *__restrict__ ixb,
+                             const int64_t *__restrict__ ipu, int32_t *__restrict__ ixu,
+                             int64_t *__restrict__ mapa, int64_t *__restrict__ mapb) {
+  const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (row >= n) return;
+  int64_t a = ipa[row], ae = ipa[row + 1], b = ipb[row], be = ipb[row + 1], u = ipu[row];
+  while (a < ae || b < be) {
+    const int32_t ca = a < ae ? ixa[a] : INT32_MAX, cb = b < be ? ixb[b] : INT32_MAX;
+    const int32_t c = ca < cb ? ca : cb;
+    ixu[u] = c;
+    if (ca == c) mapa[a++] = u;
+    if (cb == c) mapb[b++] = u;
+    ++u;
+  }
+}
+
+// out[map[k]] += v[k] (map injective: no two entries share a target)
+__global__ void k_scatter_add(int64_t nnz, const int64_t *__restrict__ map, const double *__restrict__ v,
+                              double *__restrict__ out) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < nnz) out[map[k]] += v[k];
+}
+
+// dst[k] = src[map[k]] (composition of position maps)
+__global__ void k_gather_i64(int64_t n, const int64_t *__restrict__ map, const int64_t *__restrict__ src,
+                             int64_t *__restrict__ dst) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) dst[k] = src[map[k]];
+}
+
+__global__ void k_fill_f64(int64_t n, double *__restrict__ p, double v) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n) p[k] = v;
 }
 
 // ---------------------------------------------------------------- CG
@@ -279,13 +374,54 @@ hipError_t launch_dof_constraints(hipStream_t s, int64_t n, const int64_t *indpt
 }
 
 hipError_t launch_spmv(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr,
-                       const int32_t *indices, const double *vals, const double *x, double *y) {
+                       const int32_t *indices, const double *vals, const double *x, double *y, bool acc) {
   if (n == 0) return hipSuccess;
   const double mean = (double)nnz / (double)n;
-  if (mean <= 12) return spmv_g<4>(s, n, indptr, indices, vals, x, y);
-  if (mean <= 24) return spmv_g<8>(s, n, indptr, indices, vals, x, y);
-  if (mean <= 64) return spmv_g<16>(s, n, indptr, indices, vals, x, y);
-  return spmv_g<32>(s, n, indptr, indices, vals, x, y);
+  if (mean <= 12) return spmv_g<4>(s, n, indptr, indices, vals, x, y, acc);
+  if (mean <= 24) return spmv_g<8>(s, n, indptr, indices, vals, x, y, acc);
+  if (mean <= 64) return spmv_g<16>(s, n, indptr, indices, vals, x, y, acc);
+  return spmv_g<32>(s, n, indptr, indices, vals, x, y, acc);
+}
+
+hipError_t launch_add_into(hipStream_t s, bool csc, int64_t n, const int64_t *ip, const int32_t *ix,
+                           const double *v, const int64_t *tp, const int32_t *ti, double *tv, int *err) {
+  if (n == 0) return hipSuccess;
+  if (csc) k_add_into_csc<<<nb256(n), 256, 0, s>>>(n, ip, ix, v, tp, ti, tv, err);
+  else k_add_into_csr<<<nb256(n), 256, 0, s>>>(n, ip, ix, v, tp, ti, tv, err);
+  return hipGetLastError();
+}
+
+hipError_t launch_union_count(hipStream_t s, int64_t n, const int64_t *ipa, const int32_t *ixa,
+                              const int64_t *ipb, const int32_t *ixb, int64_t *cnt) {
+  if (n == 0) return hipSuccess;
+  k_union_count<<<nb256(n), 256, 0, s>>>(n, ipa, ixa, ipb, ixb, cnt);
+  return hipGetLastError();
+}
+
+hipError_t launch_union_fill(hipStream_t s, int64_t n, const int64_t *ipa, const int32_t *ixa,
+                             const int64_t *ipb, const int32_t *ixb, const int64_t *ipu, int32_t *ixu,
+                             int64_t *mapa, int64_t *mapb) {
+  if (n == 0) return hipSuccess;
+  k_union_fill<<<nb256(n), 256, 0, s>>>(n, ipa, ixa, ipb, ixb, ipu, ixu, mapa, mapb);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_add(hipStream_t s, int64_t nnz, const int64_t *map, const double *v, double *out) {
+  if (nnz == 0) return hipSuccess;
+  k_scatter_add<<<nb256(nnz), 256, 0, s>>>(nnz, map, v, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_gather_i64(hipStream_t s, int64_t n, const int64_t *map, const int64_t *src, int64_t *dst) {
+  if (n == 0) return hipSuccess;
+  k_gather_i64<<<nb256(n), 256, 0, s>>>(n, map, src, dst);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_f64(hipStream_t s, int64_t n, double *p, double v) {
+  if (n == 0) return hipSuccess;
+  k_fill_f64<<<nb256(n), 256, 0, s>>>(n, p, v);
+  return hipGetLastError();
 }
 
 // preconditioner storage (3 n: room for the 3x3 blocks) + r, z, p, q + partials
@@ -305,7 +441,7 @@ hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr
   hipError_t e;
   if (qb == 3) k_inv_block3<<<nb256(nu), 256, 0, s>>>(nu, indptr, indices, vals, minv);
   else k_inv_diag<<<nb256(n), 256, 0, s>>>(n, indptr, indices, vals, minv);
-  if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, x, r)) != hipSuccess) return e;
+  if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, x, r, false)) != hipSuccess) return e;
   if (qb == 3) k_cg_start<3><<<kRedBlocks, kRedThreads, 0, s>>>(nu, b, minv, r, z, p, part);
   else k_cg_start<1><<<kRedBlocks, kRedThreads, 0, s>>>(nu, b, minv, r, z, p, part);
   k_cg_scalars<<<1, kRedThreads, 0, s>>>(part, 0, sc);
@@ -320,7 +456,7 @@ hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr
   double res = sqrt(h.rr) / bn;
   int it = 0;
   while (res > rtol && it < maxit) {
-    if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, p, q)) != hipSuccess) return e;
+    if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, p, q, false)) != hipSuccess) return e;
     k_dot<<<kRedBlocks, kRedThreads, 0, s>>>(n, p, q, part);
     k_cg_scalars<<<1, kRedThreads, 0, s>>>(part, 1, sc);
     if (qb == 3) k_cg_update<3><<<kRedBlocks, kRedThreads, 0, s>>>(nu, sc, p, q, minv, x, r, z, part);

@@ -10,8 +10,10 @@ partition); the dof constraints are applied to the device CSR
 by block-Jacobi-preconditioned CG; the Newton loop follows classical_Newton
 (src/getfem/getfem_model_solvers.h:548-605) with basic_newton_line_search
 (:336-360).  The state vector stays on the device between iterations; the
-host sees scalars only (and one copy of the state to the workspace per
-assembly).
+host sees scalars only.  A Newton iteration's tangent and residual come from
+one element pass (the workspace reuses the residual of the tangent assembly),
+and the pattern is rebuilt only when the new state changes GetFEM's
+threshold-filtered pattern.
 
 Sign convention as model::assembly: rhs = -(order-1 assembly of the
 nonlinear terms); a linear model's rhs holds only the constraint data.
@@ -147,7 +149,10 @@ class Model:
         import torch
         build_matrix = option in ("build_all", "build_matrix")
         build_rhs = option in ("build_all", "build_rhs")
-        self.ws.set_values(self.var, self.U.cpu().numpy())
+        # the state goes to the workspace device to device (no host copy);
+        # torch's writes to U are complete before the context's stream reads it
+        torch.cuda.current_stream().synchronize()
+        self.ws.set_values_device(self.var, self.U.data_ptr())
         n = self.mf.nbdof()
         rhs = torch.zeros(n, dtype=torch.float64, device="cuda")
         if build_matrix or not self._built:

@@ -51,6 +51,11 @@ class Compliance:
               self.ctx.h)
         return J, dr, dx, du
 
+    def check(self):
+        """Raise GfxError (GFX_E_DEGENERATE_ELEMENT) if an evaluation since the
+        last check met an element with det K = 0; synchronises the stream."""
+        check(lib().gfx_compliance_status(self.h), self.ctx.h)
+
     def __del__(self):
         if getattr(self, "h", None) and not sys.is_finalizing():
             lib().gfx_compliance_destroy(self.h)
@@ -63,13 +68,16 @@ class ComplianceFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, rho, X, u, op):
         J, dr, dx, du = op.eval(rho, X, u, want_grads=True)
+        op.check()  # a degenerate element raises, as GetFEM's assembly does
         ctx.save_for_backward(dr, dx, du)
         return J
 
     @staticmethod
     def backward(ctx, gJ):
         dr, dx, du = ctx.saved_tensors
-        return gJ * dr, gJ * dx, gJ * du, None
+        need = ctx.needs_input_grad  # X may be None (the mesh points): no gradient for it
+        return (gJ * dr if need[0] else None, gJ * dx if need[1] else None,
+                gJ * du if need[2] else None, None)
 
 
 def compliance(rho, X, u, op):

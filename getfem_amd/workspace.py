@@ -13,7 +13,7 @@ import sys
 
 import numpy as np
 
-from ._lib import FORM_FACTORIZED, PREC_FP64, check, lib
+from ._lib import CSR, FORM_FACTORIZED, PREC_FP64, check, lib
 from .core import MeshFem
 
 
@@ -34,6 +34,14 @@ def gwfl_compile(exprs, decls):
     return buf.value.decode()
 
 
+def gwfl_order(expr, decls):
+    """Test-function order of one expression (0, 1 or 2), no device."""
+    d = ";".join("%s:%s:%d" % (k, v[0], v[1]) for k, v in decls.items())
+    o = ctypes.c_int()
+    check(lib().gfx_gwfl_order(expr.encode(), d.encode(), ctypes.byref(o)))
+    return o.value
+
+
 class GaWorkspace:
     def __init__(self, ctx):
         self.ctx = ctx
@@ -41,6 +49,7 @@ class GaWorkspace:
         check(lib().gfx_ws_create(ctx.h, ctypes.byref(h)), ctx.h)
         self.h = h
         self._keep = []
+        self._ext = None
 
     def add_fem_variable(self, name, mf, V=None, first=0):
         V = None if V is None else np.ascontiguousarray(V, dtype=np.float64)
@@ -81,40 +90,72 @@ class GaWorkspace:
 
     def assembled_vector(self):
         """The order-1 result (nb_dof values of the unknown's mesh_fem)."""
-        p = self._plan_handle()
-        n, z = ctypes.c_int64(), ctypes.c_int64()
-        check(lib().gfx_plan_dims(p, ctypes.byref(n), ctypes.byref(z)), self.ctx.h)
-        V = np.zeros(n.value)
+        n = self._nbdof()
+        V = np.zeros(n)
         check(lib().gfx_ws_assembled_vector(self.h, _ptr(V)), self.ctx.h)
         return V
+
+    def set_values_device(self, name, d_V):
+        """New values of the fem variable from a device vector (raw address,
+        e.g. a torch tensor's data_ptr()); stream-ordered, no host copy."""
+        check(lib().gfx_ws_set_values_device(self.h, name.encode(), ctypes.c_void_p(d_V)), self.ctx.h)
+
+    def set_assembled_matrix(self, indptr=None, indices=None, values=None, orientation=CSR):
+        """ga_workspace::set_assembled_matrix: assembly(2) then adds into this
+        caller-owned device matrix (torch tensors; CSR rows = test dofs, or CSC
+        in GetFEM's orientation) without clearing it.  No arguments: back to
+        the workspace's own matrix."""
+        if indptr is None:
+            check(lib().gfx_ws_set_assembled_matrix(self.h, orientation, 0, None, None, None), self.ctx.h)
+            self._ext = None
+            return
+        self._ext = (indptr, indices, values)  # keep the caller's storage alive
+        check(lib().gfx_ws_set_assembled_matrix(self.h, orientation, indptr.numel() - 1,
+                                                ctypes.c_void_p(indptr.data_ptr()),
+                                                ctypes.c_void_p(indices.data_ptr()),
+                                                ctypes.c_void_p(values.data_ptr())), self.ctx.h)
+
+    def nb_trees(self):
+        return int(lib().gfx_ws_nb_trees(self.h))
 
     def _plan_handle(self):
         h = ctypes.c_void_p()
         check(lib().gfx_ws_plan(self.h, ctypes.byref(h)), self.ctx.h)
         return h
 
-    def assembled_matrix_csr(self):
-        """(indptr int64, indices int32, values float64), rows = Test_u."""
-        p = self._plan_handle()
+    def _nbdof(self):
         n, z = ctypes.c_int64(), ctypes.c_int64()
+        code = lib().gfx_ws_matrix_dims(self.h, ctypes.byref(n), ctypes.byref(z))
+        if code == 0:
+            return n.value
+        p = self._plan_handle()
         check(lib().gfx_plan_dims(p, ctypes.byref(n), ctypes.byref(z)), self.ctx.h)
-        ip = np.zeros(n.value + 1, dtype=np.int64)
-        ix = np.zeros(z.value, dtype=np.int32)
-        v = np.zeros(z.value)
-        check(lib().gfx_plan_export_csr(p, _ptr(ip), _ptr(ix), _ptr(v)), self.ctx.h)
+        return n.value
+
+    def _dims(self):
+        n, z = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().gfx_ws_matrix_dims(self.h, ctypes.byref(n), ctypes.byref(z)), self.ctx.h)
+        return n.value, z.value
+
+    def assembled_matrix_csr(self):
+        """(indptr int64, indices int32, values float64), rows = Test_u: the
+        sum of the workspace's trees."""
+        n, z = self._dims()
+        ip = np.zeros(n + 1, dtype=np.int64)
+        ix = np.zeros(z, dtype=np.int32)
+        v = np.zeros(z)
+        check(lib().gfx_ws_export_csr(self.h, _ptr(ip), _ptr(ix), _ptr(v)), self.ctx.h)
         return ip, ix, v
 
     def assembled_matrix(self):
         """scipy.sparse.csc_matrix in GetFEM's orientation (columns = trial dofs)."""
         import scipy.sparse as sp
-        p = self._plan_handle()
-        n, z = ctypes.c_int64(), ctypes.c_int64()
-        check(lib().gfx_plan_dims(p, ctypes.byref(n), ctypes.byref(z)), self.ctx.h)
-        cp = np.zeros(n.value + 1, dtype=np.int64)
-        r = np.zeros(z.value, dtype=np.uint32)
-        v = np.zeros(z.value)
-        check(lib().gfx_plan_export_csc(p, _ptr(cp), _ptr(r), _ptr(v)), self.ctx.h)
-        return sp.csc_matrix((v, r, cp), shape=(n.value, n.value))
+        n, z = self._dims()
+        cp = np.zeros(n + 1, dtype=np.int64)
+        r = np.zeros(z, dtype=np.uint32)
+        v = np.zeros(z)
+        check(lib().gfx_ws_export_csc(self.h, _ptr(cp), _ptr(r), _ptr(v)), self.ctx.h)
+        return sp.csc_matrix((v, r, cp), shape=(n, n))
 
     def __del__(self):
         if getattr(self, "h", None) and not sys.is_finalizing():

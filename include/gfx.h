@@ -44,6 +44,11 @@ extern "C" {
 #define GFX_TERM_ISO_ELAST 1  /* lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u */
 #define GFX_TERM_NEOHOOKEAN 2 /* ((Id(meshdim)+Grad_u)*(Compressible_Neo_Hookean_Bonet_PK2(
                                  Grad_u,[lambda;mu]))):Grad_Test_u (tangent + residual)  */
+#define GFX_TERM_ISOTROPIC 3  /* alpha*Div_u*Div_Test_u + beta*(Grad_u)':Grad_Test_u
+                                 + gamma*Grad_u:Grad_Test_u on a vector field: every sum
+                                 of the two terms above on one mesh_im, which GetFEM
+                                 merges into one tree (src/getfem_generic_assembly_
+                                 workspace.cc:474-495); coefficients alpha, beta, gamma */
 
 /* ---- contraction forms / precisions ---- */
 #define GFX_FORM_FACTORIZED 0 /* affine element: B R_ij B^T with R_ij = sum_q w_q
@@ -96,6 +101,12 @@ int gfx_enumerate_dofs(const char *geotrans, const char *fem_name, int qdim,
  * p0 (per-element fem constant); exprs are separated by '\n' and summed;
  * out receives the normal form of the order-2 term (as gfx_ws_describe) */
 int gfx_gwfl_compile(const char *exprs, const char *decls, char *out, int64_t len);
+/* test-function order of one expression (0: potential, 1: with Test_u, 2:
+ * with Test_u and Test2_u), as ga_workspace::add_expression computes it
+ * before deriving the higher orders (src/getfem_generic_assembly_workspace.cc:
+ * 545-600); the workspace runs an expression in assembly(k) for order <= k <=
+ * its derivation order */
+int gfx_gwfl_order(const char *expr, const char *decls, int *order);
 
 /* ======================= context ======================= */
 int gfx_ctx_create(int device, gfx_ctx **out);
@@ -167,6 +178,20 @@ int gfx_plan_assemble(gfx_plan *p);
 /* nonlinear plans: the current displacement (ndof host doubles) and the
  * assembled residual vector */
 int gfx_plan_set_state(gfx_plan *p, const double *h_U);
+/* the same from a device vector (nb_dof doubles, stream-ordered, no host
+ * synchronisation): a Newton loop keeps its state on the device */
+int gfx_plan_set_state_device(gfx_plan *p, const double *d_U);
+/* Re-thresholding.  GetFEM thresholds every element matrix at every assembly
+ * (compile_and_exec.cc:5441-5453), so its stored pattern follows the values.
+ * After gfx_plan_set_coef, gfx_mesh_set_points (on the plan's mesh) or a new
+ * state, the next gfx_plan_assemble re-derives every element's threshold and
+ * keep bits at the new values and compares the pattern they give with the
+ * stored one: the same pattern is kept, a different one is rebuilt (new CSR
+ * arrays; gfx_plan_pattern_id changes).  A pinned pattern is never rebuilt:
+ * the assembly returns GFX_E_STATE instead, so that device pointers a caller
+ * holds (gfx_plan_device_csr) stay valid. */
+int gfx_plan_pin_pattern(gfx_plan *p, int pin);
+int gfx_plan_pattern_id(gfx_plan *p, int64_t *id);
 /* order-1 assembly alone (the residual; ga_workspace::assembly(1),
  * src/getfem_generic_assembly_workspace.cc:832-847 with
  * ga_instruction_vector_assembly_mf, compile_and_exec.cc:4669-4735), after
@@ -192,6 +217,20 @@ int gfx_plan_copy_csr_device(gfx_plan *p, int64_t *d_indptr, int32_t *d_indices,
  * variable value that model::assembly adds to the residual
  * (gmm::mult_add(matj, vec_linear2, ...), src/getfem_models.cc:2560-2575) */
 int gfx_plan_spmv(gfx_plan *p, const double *d_x, double *d_y);
+/* d_y += K d_x */
+int gfx_plan_spmv_add(gfx_plan *p, const double *d_x, double *d_y);
+/* Add the assembled values into a caller-owned device matrix, as GetFEM's
+ * assembly adds into the matrix given by ga_workspace::set_assembled_matrix
+ * (src/getfem/getfem_generic_assembly.h:378-381; the workspace clears K only
+ * when it owns it, src/getfem_generic_assembly_workspace.cc:804-807).  The
+ * target's pattern must hold the plan's (GFX_E_PATTERN otherwise; entries it
+ * lacks are dropped).  GFX_CSR: rows = test dofs, sorted columns; GFX_CSC:
+ * GetFEM's col_matrix orientation, columns = trial dofs, sorted rows.  n is
+ * the number of dofs; synchronises the context's stream (error check). */
+#define GFX_CSR 0
+#define GFX_CSC 1
+int gfx_plan_add_to_matrix(gfx_plan *p, int orientation, int64_t n, const int64_t *d_ptr,
+                           const int32_t *d_ind, double *d_vals);
 /* Solve K x = b on the device by the preconditioned conjugate gradient
  * method, for the coercive (symmetric positive definite) systems GetFEM sends
  * to gmm::cg (default_linear_solver picks linear_solver_cg_preconditioned_ildlt,
@@ -254,8 +293,13 @@ int gfx_ws_add_fem_constant(gfx_workspace *w, const char *name, gfx_mesh_fem *mf
 /* ga_workspace::add_fixed_size_constant(name, V) (.h:467-468) */
 int gfx_ws_add_fixed_size_constant(gfx_workspace *w, const char *name, const double *h_V,
                                    int64_t n);
-/* new values for a registered name (same size); the pattern is kept */
+/* new values for a registered name (same size).  The plans keep their
+ * patterns unless the new values change GetFEM's threshold-filtered pattern
+ * (re-derived at the next assembly, see gfx_plan_pin_pattern). */
 int gfx_ws_set_values(gfx_workspace *w, const char *name, const double *h_V);
+/* new values of the fem variable from a device vector (stream-ordered, no
+ * host synchronisation; compiles the workspace first if needed) */
+int gfx_ws_set_values_device(gfx_workspace *w, const char *name, const double *d_V);
 /* ga_workspace::add_expression(expr, mim, region = all, der_order) (.h:425-428) */
 int gfx_ws_add_expression(gfx_workspace *w, const char *expr, gfx_mesh_im *mim, int der_order);
 int gfx_ws_set_form(gfx_workspace *w, int form, int precision);
@@ -273,7 +317,31 @@ int gfx_ws_assembly(gfx_workspace *w, int order);
 int gfx_ws_assembled_vector(gfx_workspace *w, double *h_V);
 /* the same vector into a caller-owned device buffer (stream-ordered) */
 int gfx_ws_copy_assembled_vector_device(gfx_workspace *w, double *d_V);
+/* the plan of a workspace with one tree (GFX_E_STATE with several) */
 int gfx_ws_plan(gfx_workspace *w, gfx_plan **out);
+/* number of order-2 trees: one per mesh_im with order-2 expressions (GetFEM
+ * merges the expressions of one (mesh_im, region, test functions) into one
+ * tree, src/getfem_generic_assembly_workspace.cc:474-495) */
+int gfx_ws_nb_trees(gfx_workspace *w);
+/* The workspace's assembled matrix (CSR, rows = Test_u): the single tree's
+ * plan CSR, or with several trees the workspace's own matrix, whose pattern
+ * is the union of the trees' patterns and whose values are their sum (ga_exec
+ * adds every tree into K).  Device pointers stay valid until the next
+ * assembly that changes a pattern. */
+int gfx_ws_device_csr(gfx_workspace *w, int64_t *nrows, int64_t *nnz, const int64_t **d_indptr,
+                      const int32_t **d_indices, double **d_vals);
+int gfx_ws_matrix_dims(gfx_workspace *w, int64_t *nrows, int64_t *nnz);
+int gfx_ws_export_csr(gfx_workspace *w, int64_t *h_indptr, int32_t *h_indices, double *h_vals);
+/* GetFEM orientation (CSC over trial dofs, uint32 rows) */
+int gfx_ws_export_csc(gfx_workspace *w, int64_t *h_colptr, uint32_t *h_rows, double *h_vals);
+/* ga_workspace::set_assembled_matrix (src/getfem/getfem_generic_assembly.h:
+ * 378-381): assembly(2) then adds every tree into this caller-owned device
+ * matrix (GFX_CSR or GFX_CSC, its pattern holding the trees' patterns) and
+ * does not clear it, as GetFEM leaves a matrix it does not own uncleared
+ * (src/getfem_generic_assembly_workspace.cc:804-807).  d_ptr NULL: back to
+ * the workspace's own matrix. */
+int gfx_ws_set_assembled_matrix(gfx_workspace *w, int orientation, int64_t n, const int64_t *d_ptr,
+                                const int32_t *d_ind, double *d_vals);
 /* normal form of the compiled term, e.g. "isotropic elasticity unknown=u ..." */
 int gfx_ws_describe(gfx_workspace *w, char *buf, int64_t len);
 
@@ -294,6 +362,11 @@ int gfx_compliance_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, double lambda, dou
 int gfx_compliance_eval(gfx_compliance *c, const double *d_X, const double *d_rho,
                         const double *d_u, double *d_J, double *d_dJ_drho,
                         double *d_dJ_dX, double *d_dJ_du);
+/* GFX_E_DEGENERATE_ELEMENT if an evaluation since the last call met an
+ * element with det K = 0 (GetFEM: "Non invertible matrix",
+ * bgeot_geometric_trans.cc:380); synchronises the context's stream and clears
+ * the flag.  gfx_compliance_eval itself never synchronises. */
+int gfx_compliance_status(gfx_compliance *c);
 int gfx_compliance_destroy(gfx_compliance *c);
 
 #ifdef __cplusplus

@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 
 GT_PK31, GT_QK31 = 0, 1
 FEM_PK, FEM_QK, FEM_PK_DISC0 = 0, 1, 2
-TERM_LAPLACE, TERM_ISO_ELAST, TERM_NEOHOOKEAN = 0, 1, 2
+TERM_LAPLACE, TERM_ISO_ELAST, TERM_NEOHOOKEAN, TERM_ISOTROPIC = 0, 1, 2, 3
 
 _lib = None
 
@@ -71,7 +71,7 @@ class _Problem(ctypes.Structure):
         ("nq", ctypes.c_int), ("qp", ctypes.c_void_p), ("qw", ctypes.c_void_p),
         ("term", ctypes.c_int), ("coef_a", ctypes.c_void_p), ("coef_b", ctypes.c_void_p),
         ("scal_a", ctypes.c_double), ("scal_b", ctypes.c_double),
-        ("U", ctypes.c_void_p),
+        ("U", ctypes.c_void_p), ("coef_c", ctypes.c_void_p), ("scal_c", ctypes.c_double),
     ]
 
 
@@ -125,7 +125,8 @@ class Problem:
     """Holds the arrays of one assembly problem alive for the C side."""
 
     def __init__(self, gt, pts, cvx, fem_kind, fem_k, qdim, dofs, ndof, im_name,
-                 term, coef_a=None, coef_b=None, scal_a=1.0, scal_b=0.0, U=None):
+                 term, coef_a=None, coef_b=None, scal_a=1.0, scal_b=0.0, U=None,
+                 coef_c=None, scal_c=0.0):
         self.pts = np.ascontiguousarray(pts, dtype=np.float64)
         self.cvx = np.ascontiguousarray(cvx, dtype=np.int32)
         self.dofs = np.ascontiguousarray(dofs, dtype=np.int32)
@@ -133,13 +134,15 @@ class Problem:
         self.coef_a = None if coef_a is None else np.ascontiguousarray(coef_a, dtype=np.float64)
         self.coef_b = None if coef_b is None else np.ascontiguousarray(coef_b, dtype=np.float64)
         self.U = None if U is None else np.ascontiguousarray(U, dtype=np.float64)
+        self.coef_c = None if coef_c is None else np.ascontiguousarray(coef_c, dtype=np.float64)
         self.nd = lib().orc_fem_nb_dof(fem_kind, fem_k)
         self.nloc = self.nd * qdim
         self.ndof = ndof
         self.s = _Problem(gt, self.pts.shape[0], _ptr(self.pts), self.cvx.shape[0], _ptr(self.cvx),
                           fem_kind, fem_k, qdim, _ptr(self.dofs), ndof,
                           self.qw.shape[0], _ptr(self.qp), _ptr(self.qw), term,
-                          _ptr(self.coef_a), _ptr(self.coef_b), scal_a, scal_b, _ptr(self.U))
+                          _ptr(self.coef_a), _ptr(self.coef_b), scal_a, scal_b, _ptr(self.U),
+                          _ptr(self.coef_c), scal_c)
 
     @property
     def n_cv(self):

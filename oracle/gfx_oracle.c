@@ -666,6 +666,7 @@ static int element_matrix_impl(const orc_problem *p, const orc_tables *t,
   int nd = t->nd, Q = t->qdim, nl = t->nloc;
   double a = coef_of(p->coef_a, p->scal_a, cv);
   double b = coef_of(p->coef_b, p->scal_b, cv);
+  double c = p->term == ORC_TERM_ISOTROPIC ? coef_of(p->coef_c, p->scal_c, cv) : b;
   geo_cache gc;
   gc.cv = -1;
   for (int q = 0; q < t->nq; ++q) {
@@ -679,7 +680,9 @@ static int element_matrix_impl(const orc_problem *p, const orc_tables *t,
             for (int aa = 0; aa < Q; ++aa)
               tq[(i * Q + aa) + nl * (j * Q + bb)] = (aa == bb) ? a * d : 0.0;
         }
-    } else if (p->term == ORC_TERM_ISO_ELAST) {
+    } else if (p->term == ORC_TERM_ISO_ELAST || p->term == ORC_TERM_ISOTROPIC) {
+      /* the merged tree's integrand: the div-div, transposed-gradient and
+       * gradient:gradient contractions summed per point */
       for (int j = 0; j < nd; ++j)
         for (int i = 0; i < nd; ++i) {
           double d = g[3 * i] * g[3 * j] + g[3 * i + 1] * g[3 * j + 1] + g[3 * i + 2] * g[3 * j + 2];
@@ -687,7 +690,7 @@ static int element_matrix_impl(const orc_problem *p, const orc_tables *t,
             for (int aa = 0; aa < 3; ++aa)
               tq[(i * 3 + aa) + nl * (j * 3 + bb)] =
                   a * g[3 * i + aa] * g[3 * j + bb] + b * g[3 * i + bb] * g[3 * j + aa] +
-                  (aa == bb ? b * d : 0.0);
+                  (aa == bb ? c * d : 0.0);
         }
     } else if (p->term == ORC_TERM_NEOHOOKEAN) {
       /* derivative of ((Id+Grad_u)*S(Grad_u)):Grad_Test_u with respect to u:

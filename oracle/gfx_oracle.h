@@ -26,7 +26,12 @@ enum {
   ORC_TERM_LAPLACE = 0,      /* a * Grad_u:Grad_Test_u                    */
   ORC_TERM_ISO_ELAST = 1,    /* lambda*Div_u*Div_Test_u
                                 + 2*mu*Sym(Grad_u):Grad_Test_u             */
-  ORC_TERM_NEOHOOKEAN = 2    /* ((Id+Grad_u)*Bonet_PK2(Grad_u,[l;m])):Grad_Test_u */
+  ORC_TERM_NEOHOOKEAN = 2,   /* ((Id+Grad_u)*Bonet_PK2(Grad_u,[l;m])):Grad_Test_u */
+  ORC_TERM_ISOTROPIC = 3     /* a*Div_u*Div_Test_u + b*(Grad_u)':Grad_Test_u
+                                + c*Grad_u:Grad_Test_u: one tree for the sum of
+                                the two terms above on one mesh_im, as
+                                ga_workspace::add_tree merges them
+                                (getfem_generic_assembly_workspace.cc:474-495) */
 };
 
 /* ---- integration methods (getfem_integration.cc, getfem_im_list.h) ---- */
@@ -65,6 +70,8 @@ typedef struct {
   const double *coef_b;      /* per element (mu) or NULL -> scal_b */
   double scal_a, scal_b;
   const double *U;           /* state vector (ndof), nonlinear terms */
+  const double *coef_c;      /* per element (ORC_TERM_ISOTROPIC c) or NULL -> scal_c */
+  double scal_c;
 } orc_problem;
 
 /* element matrix (column-major, rows = Test_u), accumulated over the

@@ -72,3 +72,48 @@ def test_rejected(expr, msg):
     with pytest.raises(GfxError) as e:
         gwfl_compile(expr, ELAST)
     assert msg in str(e.value)
+
+
+# Sums GetFEM merges into one tree (same mesh_im and test functions,
+# src/getfem_generic_assembly_workspace.cc:474-495) that are neither a plain
+# Laplacian nor isotropic elasticity: the three-coefficient isotropic term
+# alpha*Div.Div + beta*(Grad_u)':Grad_Test_u + gamma*Grad_u:Grad_Test_u.
+def _coefs(desc):
+    out = {}
+    for part in desc.split()[2:]:
+        k, v = part.split("=")
+        out[k] = v
+    return out
+
+
+def test_navier_form_is_isotropic():
+    got = gwfl_compile("mu*Grad_u:Grad_Test_u + (lambda+mu)*Div_u*Div_Test_u", ELAST)
+    assert got.startswith("isotropic unknown=u")
+    c = _coefs(got)
+    assert sorted(c["alpha"].split("+")) == ["1*lambda", "1*mu"] and c["gamma"] == "1*mu" and "beta" not in c
+
+
+def test_laplace_plus_elasticity_merges():
+    got = gwfl_compile(["lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u", "3*Grad_u:Grad_Test_u"], ELAST)
+    c = _coefs(got)
+    assert got.startswith("isotropic unknown=u")
+    assert c["alpha"] == "1*lambda" and c["beta"] == "1*mu" and sorted(c["gamma"].split("+")) == ["1*mu", "3"]
+
+
+def test_transposed_gradient_alone():
+    assert gwfl_compile("mu*(Grad_u'):Grad_Test_u", ELAST) == "isotropic unknown=u beta=1*mu"
+
+
+@pytest.mark.parametrize("expr,order", [
+    ("Grad_Test2_u:Grad_Test_u", 2),
+    ("lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u", 1),
+    ("(lambda*Div_u*Div_u + 2*mu*Sym(Grad_u):Sym(Grad_u))/2", 0),
+    ("((Id(meshdim)+Grad_u)*(Compressible_Neo_Hookean_Bonet_PK2(Grad_u,[lambda;mu]))):Grad_Test_u", 1),
+    ("lambda*Div_Test_u*Div_Test2_u + mu*Grad_u:Grad_Test_u", 2),
+])
+def test_expression_order(expr, order):
+    """The test-function order add_expression computes before deriving
+    (src/getfem_generic_assembly_workspace.cc:545-600): it decides which
+    assemblies an expression takes part in."""
+    from getfem_amd.workspace import gwfl_order
+    assert gwfl_order(expr, ELAST) == order

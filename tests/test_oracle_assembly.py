@@ -357,3 +357,52 @@ def test_compliance_gradients_fd():
                             "IM_TETRAHEDRON(1)", oracle.TERM_ISO_ELAST, scal_a=lh, scal_b=mh, U=u)
         fd = (Pp.compliance(rho, grads=False)[0] - Pm.compliance(rho, grads=False)[0]) / (2 * h)
         assert abs(fd - np.sum(dx * d)) < 1e-6 * max(1.0, abs(fd))
+
+
+@pytest.mark.parametrize("k", [1, 2, 3])
+def test_isotropic_term_is_elasticity_plus_vector_laplacian(k):
+    """The merged tree of 'Laplace + elasticity' (one integrand, one
+    threshold): alpha*Div.Div + beta*Grad':Grad + gamma*Grad:Grad equals the
+    elasticity matrix (alpha, beta), pinned by brute force above, plus
+    (gamma - beta) times the vector Laplacian, i.e. the scalar Laplacian
+    (pinned above) times the 3x3 identity."""
+    X = X0
+    cvx = np.array([[0, 1, 2, 3]], dtype=np.int32)
+    dofs, ndof = oracle.enumerate_dof(oracle.GT_PK31, X, cvx, oracle.FEM_PK, k, 3)
+    al, be, ga = 1.3, 0.7, 2.9
+    P = oracle.Problem(oracle.GT_PK31, X, cvx, oracle.FEM_PK, k, 3, dofs, ndof, RULE[k], oracle.TERM_ISOTROPIC,
+                       scal_a=al, scal_b=be, scal_c=ga)
+    E = P.element_matrix(0)
+    El = single_element_problem(X, k, oracle.TERM_ISO_ELAST, al, be, RULE[k]).element_matrix(0)
+    Ls = single_element_problem(X, k, oracle.TERM_LAPLACE, 1.0, 0.0, RULE[k]).element_matrix(0)
+    ref = El + (ga - be) * np.kron(Ls, np.eye(3))
+    assert np.abs(E - ref).max() <= 1e-13 * np.abs(ref).max()
+    # per-element coefficient arrays give the same matrix as the scalars
+    P2 = oracle.Problem(oracle.GT_PK31, X, cvx, oracle.FEM_PK, k, 3, dofs, ndof, RULE[k], oracle.TERM_ISOTROPIC,
+                        coef_a=np.array([al]), coef_b=np.array([be]), coef_c=np.array([ga]))
+    assert np.array_equal(P2.element_matrix(0), E)
+
+
+@pytest.mark.parametrize("k,n,kat", [(2, 8, "p2_elasticity_8cube_tet5"), (3, 4, None)])
+def test_stored_nnz_does_not_depend_on_the_seeds(k, n, kat):
+    """The stored-nnz fixtures were measured with GetFEM on the survey's inputs
+    (jitter seed 12345, lambda/mu seed 777), not on this repository's seeds.
+    They still pin the repository's meshes because the threshold drops only
+    entries that vanish for geometric reasons (element faces on the flat,
+    unjittered boundary), whatever the jitter inside and the coefficients:
+    the oracle's stored nnz is the same for the survey's seeds, the
+    repository's seeds and a third pair."""
+    im = RULE[k]
+    counts = []
+    for js, cs in ((12345, 777), (meshes.SEED + 1, meshes.SEED + 2), (4242, 99)):
+        pts, cvx = meshes.regular_unit_mesh(n)
+        pts = meshes.jitter(pts, n, seed=js)
+        lam, mu = meshes.lognormal_lame(cvx.shape[0], seed_lambda=cs, seed_mu=cs + 1)
+        dofs, ndof = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, k, 3)
+        P = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, k, 3, dofs, ndof, im, oracle.TERM_ISO_ELAST,
+                           coef_a=lam, coef_b=mu)
+        counts.append(len(P.assemble_csc()[2]))
+    assert counts[0] == counts[1] == counts[2]
+    if kat:
+        g = problems.golden(kat)
+        assert counts[0] == g["topological_nnz"] - g["dropped_by_threshold"]
