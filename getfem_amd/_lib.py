@@ -18,7 +18,7 @@ ERRORS = {
 TERM_LAPLACE, TERM_ISO_ELAST, TERM_NEOHOOKEAN, TERM_ISOTROPIC = 0, 1, 2, 3
 CSR, CSC = 0, 1
 FORM_FACTORIZED, FORM_PERQP = 0, 1
-PREC_FP64, PREC_FP32, PREC_BF16X2 = 0, 1, 2
+PREC_FP64, PREC_FP32, PREC_BF16X2, PREC_FP64_MFMA, PREC_FP32_MFMA = 0, 1, 2, 3, 4
 COEF_CONST, COEF_PER_ELEMENT = 0, 1
 BUILD_MATRIX, BUILD_RHS, MODEL_LINEAR, MODEL_SYMMETRIC = 1, 2, 4, 8
 
@@ -86,6 +86,7 @@ SIGNATURES = {
     "gfx_plan_last_timings": (_i32, [_P, ctypes.POINTER(_f32), ctypes.POINTER(_f32)]),
     "gfx_plan_timing_summary": (_i32, [_P, ctypes.POINTER(_i32), ctypes.POINTER(_f64),
                                        ctypes.POINTER(_f64)]),
+    "gfx_plan_timing_calls": (_i32, [_P, _i32, ctypes.POINTER(_i32), _P, _P]),
     "gfx_plan_destroy": (_i32, [_P]),
     "gfx_ws_create": (_i32, [_P, _pp]),
     "gfx_ws_destroy": (_i32, [_P]),

@@ -197,6 +197,7 @@ class Plan:
         if len(self._views):
             raise RuntimeError("torch views of this plan's CSR are alive: a new pattern "
                                "reallocates the arrays they point to")
+        self._sync_pin()
         nnz = ctypes.c_int64()
         check(lib().gfx_plan_build_pattern(self.h, ctypes.byref(nnz)), self.ctx.h)
         self.nnz = nnz.value
@@ -346,6 +347,15 @@ class Plan:
         check(lib().gfx_plan_timing_summary(self.h, ctypes.byref(n), ctypes.byref(g), ctypes.byref(a)),
               self.ctx.h)
         return n.value, g.value, a.value
+
+    def timing_calls(self, max_calls=100000):
+        """Per-call (geometry ms, contraction + scatter ms) arrays of the
+        recorded calls; resets the record."""
+        g = np.zeros(max_calls, dtype=np.float32)
+        a = np.zeros(max_calls, dtype=np.float32)
+        n = ctypes.c_int()
+        check(lib().gfx_plan_timing_calls(self.h, max_calls, ctypes.byref(n), _ptr(g), _ptr(a)), self.ctx.h)
+        return g[:n.value].astype(np.float64), a[:n.value].astype(np.float64)
 
     def last_timings(self):
         g, a = ctypes.c_float(), ctypes.c_float()

@@ -399,6 +399,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_gref); dfree(p->d_qw); dfree(p->d_R);
   dfree(p->d_rec); dfree(p->d_thr); dfree(p->d_keep);
   dfree(p->d_gref32); dfree(p->d_qw32); dfree(p->d_R32); dfree(p->d_rec32); dfree(p->d_Rq); dfree(p->d_E32);
+  dfree(p->d_E64);
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order); dfree(p->d_desc);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
@@ -562,13 +563,14 @@ int gfx_plan_set_coef(gfx_plan *p, int slot, const gfx_coef *cf) {
 
 int gfx_plan_set_form(gfx_plan *p, int form, int precision) {
   if (!p) return GFX_E_INVALID;
-  if ((form != GFX_FORM_FACTORIZED && form != GFX_FORM_PERQP) ||
-      (precision != GFX_PREC_FP64 && precision != GFX_PREC_FP32 && precision != GFX_PREC_BF16X2))
+  if ((form != GFX_FORM_FACTORIZED && form != GFX_FORM_PERQP) || precision < GFX_PREC_FP64 ||
+      precision > GFX_PREC_FP32_MFMA)
     return fail(p->ctx, GFX_E_INVALID, "unknown form or precision");
-  if (precision == GFX_PREC_BF16X2 &&
+  if (precision >= GFX_PREC_BF16X2 &&
       (p->eb || p->term != GFX_TERM_ISO_ELAST || p->qdim != 3 || (p->nd != 10 && p->nd != 20)))
     return fail(p->ctx, GFX_E_UNSUPPORTED,
-                "the bf16x2 form needs isotropic elasticity on a P2 or P3 vector field of an affine tet mesh");
+                "the element-batched matrix-core forms need isotropic elasticity on a P2 or P3 vector field "
+                "of an affine tet mesh");
   p->form = form;
   p->precision = precision;
   return GFX_OK;
@@ -964,16 +966,29 @@ int gfx_plan_assemble(gfx_plan *p) {
   }
   if (!(eb_fresh && p->precision == GFX_PREC_FP64))
     if ((r = p->eb ? plan_eb_elements(p, p->precision) : plan_records(p, true))) return r;
-  const bool bdb = !p->eb && p->precision == GFX_PREC_BF16X2;
+  const bool bdb = !p->eb && p->precision >= GFX_PREC_BF16X2;
+  const bool bdb64 = bdb && p->precision == GFX_PREC_FP64_MFMA;
   if (bdb) {  // element matrices on the matrix cores, then the node-patch gather below
     const int64_t n_cv = p->mf->mesh->n_cv;
-    // the kernel factors D through its square root: mu > 0 and 3 lambda + 2 mu > 0
+    // the kernels factor D through its square root: mu > 0 and 3 lambda + 2 mu > 0
     // for every element (sufficient: from the smallest lambda and mu)
     const double lmin = p->coef_min[0], mmin = p->coef_min[1];
     if (!(mmin > 0.0 && 3.0 * std::min(lmin, 0.0) + 2.0 * mmin > 0.0))
-      return fail(c, GFX_E_UNSUPPORTED, "the bf16x2 form needs mu > 0 and 3*lambda + 2*mu > 0 on every element");
-    if (!p->d_E32) HIPCHK(c, dalloc(&p->d_E32, (size_t)n_cv * p->nloc * p->nloc));
-    HIPCHK(c, launch_elast_bdb_mfma(c->stream, n_cv, p->nd, p->nq, p->d_rec, p->d_gref, p->d_qw, p->d_E32));
+      return fail(c, GFX_E_UNSUPPORTED,
+                  "the matrix-core forms need mu > 0 and 3*lambda + 2*mu > 0 on every element");
+    const size_t ne = (size_t)n_cv * p->nloc * p->nloc;
+    if (bdb64) {
+      if (!p->d_E64) HIPCHK(c, dalloc(&p->d_E64, ne));
+      HIPCHK(c, launch_elast_bdb_mfma_hp(c->stream, n_cv, p->nd, p->nq, p->d_rec, p->d_gref, p->d_qw, p->d_E64,
+                                         nullptr));
+    } else {
+      if (!p->d_E32) HIPCHK(c, dalloc(&p->d_E32, ne));
+      if (p->precision == GFX_PREC_BF16X2)
+        HIPCHK(c, launch_elast_bdb_mfma(c->stream, n_cv, p->nd, p->nq, p->d_rec, p->d_gref, p->d_qw, p->d_E32));
+      else
+        HIPCHK(c, launch_elast_bdb_mfma_hp(c->stream, n_cv, p->nd, p->nq, p->d_rec, p->d_gref, p->d_qw, nullptr,
+                                           p->d_E32));
+    }
   }
   if (ev) HIPCHK(c, hipEventRecord(ev[1], c->stream));
   PatchArgs a{};
@@ -1015,7 +1030,8 @@ int gfx_plan_assemble(gfx_plan *p) {
       a.n_nodes = 4 * b.count;
       a.desc = p->d_desc + 4 * b.off;
       a.npw = 4;
-      HIPCHK(c, launch_gather_emat(st, a, p->d_E32, true, p->d_thr, b.acc_cap, b.inc_max));
+      HIPCHK(c, launch_gather_emat(st, a, bdb64 ? (const void *)p->d_E64 : (const void *)p->d_E32, !bdb64,
+                                   p->d_thr, b.acc_cap, b.inc_max));
     } else {
       a.n_waves = b.count;
       a.desc = p->d_desc + 4 * b.off;
@@ -1202,6 +1218,25 @@ int gfx_plan_timing_summary(gfx_plan *p, int *n_calls, double *ms_geometry, doub
   if (n_calls) *n_calls = p->n_timed;
   if (ms_geometry) *ms_geometry = g;
   if (ms_assemble) *ms_assemble = a;
+  p->n_timed = 0;
+  return GFX_OK;
+}
+
+int gfx_plan_timing_calls(gfx_plan *p, int max_calls, int *n_calls, float *ms_geometry, float *ms_assemble) {
+  if (!p || !n_calls || max_calls < 0) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (set_device(c)) return GFX_E_HIP;
+  const int n = std::min(p->n_timed, max_calls);
+  for (int k = 0; k < n; ++k) {
+    hipEvent_t *ev = &p->ev[3 * (size_t)k];
+    HIPCHK(c, hipEventSynchronize(ev[2]));
+    float x = 0.f, y = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&x, ev[0], ev[1]));
+    HIPCHK(c, hipEventElapsedTime(&y, ev[1], ev[2]));
+    if (ms_geometry) ms_geometry[k] = x;
+    if (ms_assemble) ms_assemble[k] = y;
+  }
+  *n_calls = n;
   p->n_timed = 0;
   return GFX_OK;
 }

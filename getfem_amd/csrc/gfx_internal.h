@@ -183,6 +183,8 @@ hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, con
                             const double *thr, uint32_t *keep);
 hipError_t launch_elast_bdb_mfma(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,
                                  const double *gref, const double *qw, float *E);
+hipError_t launch_elast_bdb_mfma_hp(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,
+                                    const double *gref, const double *qw, double *E64, float *E32);
 hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a, const void *E, bool e_fp32, const double *thr,
                               int acc_cap, int group_inc);
 hipError_t launch_gather_vec(hipStream_t s, int64_t n_nodes, int nd, int Q, const int32_t *inc_ptr,
@@ -257,7 +259,8 @@ struct gfx_plan {
   double *d_rec = nullptr, *d_thr = nullptr;
   // FP32 contraction form: the tables and the element records in FP32
   float *d_gref32 = nullptr, *d_qw32 = nullptr, *d_R32 = nullptr, *d_rec32 = nullptr;
-  float *d_E32 = nullptr;  // element matrices of the GFX_PREC_BF16X2 form (allocated on first use)
+  float *d_E32 = nullptr;  // element matrices of the bf16x2 / FP32 matrix-core forms (allocated on first use)
+  double *d_E64 = nullptr; // element matrices of the FP64 matrix-core form (allocated on first use)
   // R = N / r_L with small integers N (exact quadrature of polynomials): rk 1
   // int8 / 2 int16 blocks in d_Rq, 0 if R has no such form
   int rk = 0;

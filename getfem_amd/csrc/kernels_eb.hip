@@ -860,6 +860,137 @@ hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision) {
   return hipGetLastError();
 }
 
+// Element-batched isotropic elasticity on the FP64 and FP32 matrix cores
+// (v_mfma_f64_16x16x4_f64, v_mfma_f32_16x16x4_f32): the symmetric root
+// factorization of the bf16 form above, E = sum_q sign(w_q) F_q^T F_q with
+// F_q = sqrt(|w_q| |J|) S B_q, computed in T.  The k axis is the 6 Voigt rows
+// of every point back to back (k = 6 q + s, no padding rows), 4 k per MFMA.
+// Lane layout of the 16x16x4 products: A[i][k] in lane i + 16 k, C[k][j] in
+// lane j + 16 k, D[i][j] in lane j + 16 (i / 4), element i % 4; so a lane
+// holds one F value per dof tile and k-step, and the A fragment of row tile a
+// is the C fragment of column tile a up to sign(w_q).  One wavefront per
+// element (grid stride), only the upper tiles computed.  E (row-major, T) is
+// stored from the accumulators: tile (a, b) as 16-value runs of rows
+// 16 a + 4 g + r (one per lane group g and element r), its mirror (b, a) as
+// 4-value runs per lane.
+template <typename T>
+struct BdbMfma;
+template <>
+struct BdbMfma<double> {
+  typedef __attribute__((__vector_size__(4 * sizeof(double)))) double v4;
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+};
+template <>
+struct BdbMfma<float> {
+  typedef f32x4 v4;
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+};
+
+// F[s][m] of one point: Voigt row s (xx, yy, zz, yz, xz, xy) of S B_q for
+// dof m = 3 i + ax, from the staged (reference gradient of phi_i, signed
+// sqrt(|w_q|)) h; on / off / sh: the normal-diagonal, normal-off-diagonal and
+// shear factors of S (with |J| folded in)
+template <typename T>
+__device__ __forceinline__ T bdb_fvalue(const T (&h)[4], int s, int ax, const T (&Bm)[9], T on, T off, T sh) {
+  const T gx = Bm[0] * h[0] + Bm[1] * h[1] + Bm[2] * h[2];
+  const T gy = Bm[3] * h[0] + Bm[4] * h[1] + Bm[5] * h[2];
+  const T gz = Bm[6] * h[0] + Bm[7] * h[1] + Bm[8] * h[2];
+  const T r = h[3] < T(0) ? -h[3] : h[3];
+  T v;
+  if (s < 3) {
+    v = (s == ax ? on : off) * (ax == 0 ? gx : (ax == 1 ? gy : gz));
+  } else {
+    // shear rows: yz (s 3) couples y and z, xz (4) x and z, xy (5) x and y
+    const int p = s == 3 ? 1 : 0, q = s == 5 ? 1 : 2;
+    const T gp = p == 0 ? gx : gy, gq = q == 1 ? gy : gz;
+    v = ax == p ? sh * gq : (ax == q ? sh * gp : T(0));
+  }
+  return r * v;
+}
+
+template <int ND, typename T>
+__global__ __launch_bounds__(256) void k_elast_bdb_mfma_hp(int64_t n_cv, int nq, const double *__restrict__ rec,
+                                                           const double *__restrict__ gref,
+                                                           const double *__restrict__ qw, T *__restrict__ E) {
+  typedef typename BdbMfma<T>::v4 v4;
+  constexpr int NL = 3 * ND, NT = (NL + 15) / 16, NP = NT * (NT + 1) / 2;
+  extern __shared__ double hs_raw[];  // [nq][ND][4] in T: reference gradient, signed sqrt(|w_q|)
+  T *hs = reinterpret_cast<T *>(hs_raw);
+  const int t = threadIdx.x, w = t / 64, l = t % 64;
+  int neg = 0;
+  for (int x = t; x < nq * ND; x += 256) {
+    const double *g = gref + (int64_t)x * 3;  // [q][i][3]
+    const double wq = qw[x / ND];
+    hs[4 * x] = (T)g[0];
+    hs[4 * x + 1] = (T)g[1];
+    hs[4 * x + 2] = (T)g[2];
+    hs[4 * x + 3] = (T)copysign(sqrt(fabs(wq)), wq);
+    neg |= wq < 0.0;
+  }
+  const bool any_neg = __syncthreads_or(neg);
+  const int lr = l % 16, lg = l / 16;
+  const int K = 6 * nq, steps = (K + 3) / 4;
+  const int64_t stride = 4 * (int64_t)gridDim.x;
+  for (int64_t e = 4 * (int64_t)blockIdx.x + __builtin_amdgcn_readfirstlane(w); e < n_cv; e += stride) {
+    const double *re = rec + kRecStride * e;  // uniform: scalar loads
+    T Bm[9];
+#pragma unroll
+    for (int x = 0; x < 9; ++x) Bm[x] = (T)re[x];
+    // S(|J| lambda, |J| mu) = sqrt(|J|) S(lambda, mu)
+    const T lam = (T)re[12], mu = (T)re[13];
+    const T s2m = sqrt(T(2) * mu), c = lam / (sqrt(T(2) * mu + T(3) * lam) + s2m);
+    const T on = s2m + c, off = c, sh = sqrt(mu);
+    v4 acc[NP];
+#pragma unroll
+    for (int p = 0; p < NP; ++p) acc[p] = v4{T(0), T(0), T(0), T(0)};
+    int q = lg / 6, sv = lg % 6;  // this lane's k = 4 step + lg as (point, Voigt row)
+    for (int st = 0; st < steps; ++st) {
+      const bool kin = q < nq;
+      T f[NT];
+#pragma unroll
+      for (int a = 0; a < NT; ++a) {
+        const int m = 16 * a + lr;
+        T h[4] = {T(0), T(0), T(0), T(0)};
+        if (kin && m < NL) {
+          const T *hp = hs + 4 * (q * ND + m / 3);  // q < nq, m / 3 < ND here
+          h[0] = hp[0]; h[1] = hp[1]; h[2] = hp[2]; h[3] = hp[3];
+        }
+        f[a] = bdb_fvalue<T>(h, sv, m % 3, Bm, on, off, sh);
+      }
+      const T sg = any_neg && hs[4 * (min(q, nq - 1) * ND) + 3] < T(0) ? T(-1) : T(1);
+      int p = 0;
+#pragma unroll
+      for (int b = 0; b < NT; ++b) {
+        const T cb = sg * f[b];
+#pragma unroll
+        for (int a = 0; a <= b; ++a, ++p) acc[p] = BdbMfma<T>::mma(f[a], cb, acc[p]);
+      }
+      sv += 4;
+      if (sv >= 6) { sv -= 6; ++q; }
+    }
+    T *Ee = E + e * (int64_t)(NL * NL);
+    int p = 0;
+#pragma unroll
+    for (int b = 0; b < NT; ++b)
+#pragma unroll
+      for (int a = 0; a <= b; ++a, ++p) {
+        const int col = 16 * b + lr;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = 16 * a + 4 * lg + r;
+          if (row < NL && col < NL) {
+            __builtin_nontemporal_store(acc[p][r], Ee + row * NL + col);
+            if (a != b) __builtin_nontemporal_store(acc[p][r], Ee + col * NL + row);
+          }
+        }
+      }
+  }
+}
+
 hipError_t launch_elast_bdb_mfma(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,
                                  const double *gref, const double *qw, float *E) {
   if (n_cv == 0) return hipSuccess;
@@ -870,6 +1001,25 @@ hipError_t launch_elast_bdb_mfma(hipStream_t s, int64_t n_cv, int nd, int nq, co
   if (nd == 10) k_elast_bdb_mfma<10><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
   else if (nd == 20) k_elast_bdb_mfma<20><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E);
   else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+// the FP64 (E64) or FP32 (E32) matrix-core form; the other pointer is null
+hipError_t launch_elast_bdb_mfma_hp(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,
+                                    const double *gref, const double *qw, double *E64, float *E32) {
+  if (n_cv == 0) return hipSuccess;
+  const size_t lds = (size_t)nq * nd * 4 * (E64 ? sizeof(double) : sizeof(float));
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
+  const unsigned grid = (unsigned)std::min<int64_t>((n_cv + 3) / 4, kBdbMaxGroups);  // 4 wavefronts each
+  if (E64) {
+    if (nd == 10) k_elast_bdb_mfma_hp<10, double><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E64);
+    else if (nd == 20) k_elast_bdb_mfma_hp<20, double><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E64);
+    else return hipErrorInvalidValue;
+  } else {
+    if (nd == 10) k_elast_bdb_mfma_hp<10, float><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E32);
+    else if (nd == 20) k_elast_bdb_mfma_hp<20, float><<<grid, 256, lds, s>>>(n_cv, nq, rec, gref, qw, E32);
+    else return hipErrorInvalidValue;
+  }
   return hipGetLastError();
 }
 
@@ -904,6 +1054,12 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const void *E,
   } else if (a.nd == 8 && a.qdim == 3) {
     if (e_fp32) k_gather_emat<8, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
     else k_gather_emat<8, 3, double><<<grid, kGatherThreads, lds, s>>>(a, E64, thr);
+  } else if (a.nd == 10 && a.qdim == 3 && !e_fp32) {  // element-batched affine P2, FP64 matrix cores
+    if (group_inc <= 32) k_gather_emat<10, 3, double, 64, 32><<<grid, 64, lds, s>>>(a, E64, thr);
+    else k_gather_emat<10, 3, double, 64><<<grid, 64, lds, s>>>(a, E64, thr);
+  } else if (a.nd == 20 && a.qdim == 3 && !e_fp32) {  // element-batched affine P3, FP64 matrix cores
+    if (group_inc <= 32) k_gather_emat<20, 3, double, 64, 32><<<grid, 64, lds, s>>>(a, E64, thr);
+    else k_gather_emat<20, 3, double, 64><<<grid, 64, lds, s>>>(a, E64, thr);
   } else if (a.nd == 10 && a.qdim == 3 && e_fp32) {  // element-batched affine P2 (bf16 split form)
     // groups of at most 32 incidences (the packed waves): half the static
     // LDS per group, so more groups fit on a CU

@@ -427,6 +427,7 @@ hipError_t launch_fill_f64(hipStream_t s, int64_t n, double *p, double v) {
 // preconditioner storage (3 n: room for the 3x3 blocks) + r, z, p, q + partials
 size_t cg_workspace_doubles(int64_t n) { return 7 * (size_t)n + 3 * kRedBlocks + 8; }
 
+
 hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
                     const double *vals, const double *b, double *x, double rtol, int maxit, int qb,
                     double *work, int *iters, double *rel_res) {

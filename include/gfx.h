@@ -65,6 +65,14 @@ extern "C" {
                                  relative error about 1e-5 of the element norm; the
                                  form argument is ignored; needs mu > 0 and
                                  3 lambda + 2 mu > 0 (checked at assembly)          */
+#define GFX_PREC_FP64_MFMA 3  /* the same element-batched B^T D B on the FP64 matrix
+                                 cores (v_mfma_f64_16x16x4_f64), element matrices
+                                 buffered in FP64 and gathered: the FP64 variant of
+                                 the dense element-batched form (same plans and
+                                 material conditions as GFX_PREC_BF16X2)           */
+#define GFX_PREC_FP32_MFMA 4  /* the same on the FP32 matrix cores
+                                 (v_mfma_f32_16x16x4_f32), FP32 accumulation,
+                                 element matrices buffered in FP32                 */
 
 /* ---- coefficient sources ---- */
 #define GFX_COEF_CONST 0       /* one value                                 */
@@ -269,6 +277,10 @@ int gfx_plan_set_timing(gfx_plan *p, int enable);
 int gfx_plan_last_timings(gfx_plan *p, float *ms_geometry, float *ms_assemble);
 int gfx_plan_timing_summary(gfx_plan *p, int *n_calls, double *ms_geometry,
                             double *ms_assemble);
+/* the recorded calls one by one (at most max_calls): durations of each
+ * call's geometry and contraction + scatter phases; resets the record */
+int gfx_plan_timing_calls(gfx_plan *p, int max_calls, int *n_calls, float *ms_geometry,
+                          float *ms_assemble);
 int gfx_plan_destroy(gfx_plan *p);
 
 /* ======================= workspace (ga_workspace) =======

@@ -56,11 +56,20 @@ int main(int argc, char **argv) {
     save(dir + "/rows.u32", K.rows);
     save(dir + "/values.f64", K.values);
     std::cout << workspace.describe() << "\nnnz " << K.values.size() << "\n";
-    // order 1 at u_i = sin(i): the vector K u
+    // order 1 at u_i = sin(i).  The expression above has test-function
+    // order 2 (Test2_u), so it has no order-1 tree: assembly(1) gives zero, as
+    // in GetFEM.  The residual form of the same law (order 1) gives K u.
     for (size_t i = 0; i < U.size(); ++i) U[i] = std::sin((double)i);
     workspace.set_values("u", U);
     workspace.assembly(1);
-    save(dir + "/V.f64", workspace.assembled_vector());
+    save(dir + "/V2.f64", workspace.assembled_vector());
+    getfem_amd::ga_workspace w1(ctx);
+    w1.add_fem_variable("u", mf_u, getfem_amd::sub_interval(0, mf_u.nb_dof()), U);
+    w1.add_fixed_size_constant("lambda", {3.0});
+    w1.add_fixed_size_constant("mu", {2.0});
+    w1.add_expression("lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u", mim);
+    w1.assembly(1);
+    save(dir + "/V.f64", w1.assembled_vector());
     // an unsupported term is reported as an exception, like gmm::gmm_error
     try {
       getfem_amd::ga_workspace w2(ctx);

@@ -41,3 +41,5 @@ def test_cpp_workspace_matches_oracle(tmp_path):
     K = sp.csc_matrix((vr, rr, cpr), shape=(ndof, ndof))
     V = np.fromfile(tmp_path / "V.f64")
     assert problems.rel_frob(V, K @ np.sin(np.arange(ndof, dtype=np.float64))) <= 1e-12
+    # the order-2 spelling has no order-1 tree (GetFEM's assembly(1) adds nothing for it)
+    assert np.abs(np.fromfile(tmp_path / "V2.f64")).max() == 0.0

@@ -87,21 +87,30 @@ BF16X2_CASES["p2_elast_n4_im3"] = lambda: problems.Case(4, 2, 3, "IM_TETRAHEDRON
                                                         mu=np.linspace(1.5, 0.3, 384))
 
 
+# the element-batched B^T D B forms on the matrix cores and their tolerances
+# against the FP64 oracle (relative Frobenius)
+MFMA_FORMS = {"bf16x2": (ga.PREC_BF16X2, TOL_BF16X2), "fp64_mfma": (ga.PREC_FP64_MFMA, TOL_FP64),
+              "fp32_mfma": (ga.PREC_FP32_MFMA, TOL_FP32)}
+
+
+@pytest.mark.parametrize("mf_form", sorted(MFMA_FORMS))
 @pytest.mark.parametrize("name", ["p2_elast_n8", "p2_elast_n3_unjittered", "p3_elast_n3", "p2_elast_n4_im3"])
-def test_bf16x2_element_batched(ctx, name):
-    """Element matrices B^T D B on the matrix cores, gathered over the packed
-    waves and the large-patch launches into GetFEM's pattern."""
+def test_bf16x2_element_batched(ctx, name, mf_form):
+    """Element matrices B^T D B on the matrix cores (bf16 hi + lo split, FP64,
+    FP32), gathered over the packed waves and the large-patch launches into
+    GetFEM's pattern."""
+    prec, tol = MFMA_FORMS[mf_form]
     case = BF16X2_CASES[name]()
-    print("case", name, "ndof", case.ndof, flush=True)
+    print("case", name, mf_form, "ndof", case.ndof, flush=True)
     ip_ref, ix_ref, v_ref = case.oracle_csr()
     plan, keep = gpu_plan(ctx, case)
-    plan.set_form(ga.FORM_FACTORIZED, ga.PREC_BF16X2)
+    plan.set_form(ga.FORM_FACTORIZED, prec)
     plan.assemble()
     ip, ix, v = plan.csr()
     assert np.array_equal(ip, ip_ref) and np.array_equal(ix, ix_ref)
     err = problems.rel_frob(v, v_ref)
-    print("bf16x2 element-batched form: rel frobenius error vs FP64 oracle", err)
-    assert err <= TOL_BF16X2
+    print(mf_form, "element-batched form: rel frobenius error vs FP64 oracle", err)
+    assert err <= tol
     # the same plan returns to the FP64 form
     plan.set_form(ga.FORM_FACTORIZED, ga.PREC_FP64)
     plan.assemble()
@@ -210,11 +219,13 @@ def test_full_size_stored_nnz_equals_reference(ctx, name, n, fem, im, nd):
     assert worst <= TOL_FP64
 
 
-def test_full_size_bf16x2_against_fp64(ctx):
-    """P3 elasticity 32^3x6 (196,608 tets, IM_TETRAHEDRON(8)): the bf16x2
-    element-batched form on the full mesh against the FP64 factorized form of
-    the same plan (itself pinned to the oracle above), relative Frobenius
-    within TOL_BF16X2 and no entry off by more than 1e-4 of the largest."""
+@pytest.mark.parametrize("mf_form", sorted(MFMA_FORMS))
+def test_full_size_bf16x2_against_fp64(ctx, mf_form):
+    """P3 elasticity 32^3x6 (196,608 tets, IM_TETRAHEDRON(8)): the matrix-core
+    element-batched forms on the full mesh against the FP64 factorized form of
+    the same plan (itself pinned to the oracle above): relative Frobenius
+    within the form's tolerance, no entry off by more than that of the largest."""
+    prec, tol = MFMA_FORMS[mf_form]
     import torch
     pts, cvx = meshes.regular_unit_mesh(32)
     pts = meshes.jitter(pts, 32)
@@ -227,10 +238,10 @@ def test_full_size_bf16x2_against_fp64(ctx):
     plan.set_form(ga.FORM_FACTORIZED, ga.PREC_FP64)
     plan.assemble()
     v64 = plan.torch_csr_view()[2].clone()
-    plan.set_form(ga.FORM_FACTORIZED, ga.PREC_BF16X2)
+    plan.set_form(ga.FORM_FACTORIZED, prec)
     plan.assemble()
     v = plan.torch_csr_view()[2]
     err = float(torch.linalg.vector_norm(v - v64) / torch.linalg.vector_norm(v64))
     worst = float((v - v64).abs().max() / v64.abs().max())
-    print("bf16x2 vs fp64 at full size: rel frobenius", err, "max entry", worst, flush=True)
-    assert err <= TOL_BF16X2 and worst <= 1e-4
+    print(mf_form, "vs fp64 at full size: rel frobenius", err, "max entry", worst, flush=True)
+    assert err <= tol and worst <= max(tol, 1e-12)
