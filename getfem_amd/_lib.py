@@ -40,6 +40,7 @@ SIGNATURES = {
     "gfx_enumerate_dofs": (_i32, [_cs, _cs, _i32, _i64, _P, _i64, _P, _P, ctypes.POINTER(_i64)]),
     "gfx_gwfl_compile": (_i32, [_cs, _cs, ctypes.c_char_p, _i64]),
     "gfx_gwfl_order": (_i32, [_cs, _cs, ctypes.POINTER(_i32)]),
+    "gfx_gwfl_compile_scalar": (_i32, [_cs, _cs, _i32, _cs, ctypes.c_char_p, _i64]),
     "gfx_ctx_create": (_i32, [_i32, _pp]),
     "gfx_ctx_destroy": (_i32, [_P]),
     "gfx_ctx_set_stream": (_i32, [_P, _P]),
@@ -107,6 +108,8 @@ SIGNATURES = {
     "gfx_ws_assembly": (_i32, [_P, _i32]),
     "gfx_ws_plan": (_i32, [_P, _pp]),
     "gfx_ws_assembled_vector": (_i32, [_P, _P]),
+    "gfx_ws_vector_size": (_i32, [_P, ctypes.POINTER(_i64)]),
+    "gfx_ws_assembled_potential": (_i32, [_P, ctypes.POINTER(_f64)]),
     "gfx_ws_copy_assembled_vector_device": (_i32, [_P, _P]),
     "gfx_ws_describe": (_i32, [_P, ctypes.c_char_p, _i64]),
     "gfx_compliance_create": (_i32, [_P, _P, _f64, _f64, _f64, _f64, _pp]),

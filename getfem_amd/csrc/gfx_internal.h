@@ -145,6 +145,40 @@ struct EbArgs {
   int *err;
 };
 
+// Order-0 / order-1 polynomial programs (scalar.hip).  The symbol table of a
+// thread holds, for every field f, its values at off..off+q-1 and gradients
+// at off+q+3a+b, then the fixed-size constants from const_off.
+constexpr int kScalarMaxFields = 4;
+constexpr int kScalarMaxSyms = 56;
+constexpr int kScalarMaxOut = 12;
+struct ScalarField {
+  const int32_t *dofs;      // n_cv * nd, dof of component 0 of each element node
+  const double *U;          // the field's values
+  const double *phi;        // nq * nd base values at the rule's points
+  const double *gref;       // nq * nd * 3 reference gradients
+  int nd, qdim, off;
+};
+struct ScalarArgs {
+  int64_t n_cv;
+  const double *pts;
+  const int32_t *cvx;       // affine tetrahedra
+  int nq;
+  const double *qw;
+  int nf;
+  ScalarField f[kScalarMaxFields];
+  int nconst, const_off;
+  const double *consts;
+  int order;                // 0: potential, 1: derivative's element vectors
+  int vf;                   // order 1: the field of the differentiated variable
+  int nout, nm;
+  const double *mcoef;      // program: nm monomials
+  const int32_t *moff;      // nm + 1 offsets into msym
+  const int32_t *msym;      // symbol table indices
+  const int32_t *mout;      // output coefficient of each monomial
+  double *elem;             // order 0: n_cv values; order 1: n_cv * nd * qdim
+  int *err;
+};
+
 // reference counting of the objects above (api.cpp)
 void ref(gfx_ctx *c);
 void unref(gfx_ctx *c);
@@ -189,6 +223,11 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a, const void *E, 
                               int acc_cap, int group_inc);
 hipError_t launch_gather_vec(hipStream_t s, int64_t n_nodes, int nd, int Q, const int32_t *inc_ptr,
                              const uint32_t *inc, const double *res, double *V);
+
+// scalar.hip
+hipError_t launch_scalar_elem(hipStream_t s, const ScalarArgs &a);
+hipError_t launch_scalar_sum(hipStream_t s, int64_t n, const double *x, double *part, double *out);
+int64_t scalar_partials();
 
 // spmv.hip
 hipError_t launch_spmv(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr,

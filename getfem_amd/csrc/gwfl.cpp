@@ -40,6 +40,25 @@ void padd_to(Poly &a, const Poly &b, double s = 1.0) {
     if (v == 0.0) a.erase(kv.first);
   }
 }
+// Truncation of the polynomials at a variable held at zero: the values and
+// gradients of a variable that appears inside Inv(...) are expanded around 0
+// (Inv is a Neumann series there), and an order-0 or order-1 assembly at that
+// variable's zero value needs only the monomials of degree <= 1 in its
+// symbols.  `zsym[s]` marks those symbols; monomials of higher degree in them
+// are dropped by every product.
+struct Trunc {
+  std::vector<char> zsym;
+  int max_deg = -1;  // -1: no truncation
+  bool drops(const Mono &m) const {
+    if (max_deg < 0) return false;
+    int d = 0;
+    for (int s : m)
+      if (s < (int)zsym.size() && zsym[s] && ++d > max_deg) return true;
+    return false;
+  }
+};
+thread_local const Trunc *g_trunc = nullptr;
+
 Poly pmul(const Poly &a, const Poly &b) {
   Poly r;
   for (auto &x : a)
@@ -47,6 +66,7 @@ Poly pmul(const Poly &a, const Poly &b) {
       Mono m = x.first;
       m.insert(m.end(), y.first.begin(), y.first.end());
       std::sort(m.begin(), m.end());
+      if (g_trunc && g_trunc->drops(m)) continue;
       double &v = r[m];
       v += x.second * y.second;
       if (v == 0.0) r.erase(m);
@@ -162,15 +182,30 @@ Tensor ttranspose(const Tensor &a) {
 }
 
 // --------------------------------------------------------------- parser
+// the variable a symbol belongs to and its tag: "u:G_01" -> ("u", "G")
+void split_symbol(const std::string &nm, std::string &var, std::string &tag) {
+  const size_t c = nm.find(':');
+  var = nm.substr(0, c);
+  tag = c == std::string::npos ? "" : nm.substr(c + 1, nm.find('_', c) - c - 1);
+}
+
 struct Ctx {
   const std::map<std::string, GwflVar> *vars;
   std::vector<std::string> symnames;
   std::map<std::string, int> symid;
+  // variables held at zero (they appear inside Inv) and the truncation of
+  // products in their value/gradient symbols
+  std::set<std::string> zero_vars;
+  Trunc trunc;
+  bool collect_inv = false;  // first pass: Inv only records its variables
   int sym(const std::string &n) {
     auto it = symid.find(n);
     if (it != symid.end()) return it->second;
     symid[n] = (int)symnames.size();
     symnames.push_back(n);
+    std::string var, tag;
+    split_symbol(n, var, tag);
+    trunc.zsym.push_back(zero_vars.count(var) && (tag == "V" || tag == "G"));
     return symid[n];
   }
 };
@@ -355,8 +390,91 @@ struct Parser {
   }
   static bool is_function(const std::string &n) {
     static const std::set<std::string> f = {"Sym", "Skew", "Trace", "Id", "sqr", "Norm", "Norm_sqr",
-                                            "qdim", "Deviator"};
+                                            "qdim", "Deviator", "pow", "Det", "Inv"};
     return f.count(n) > 0;
+  }
+  static bool is_number(const Tensor &t, double &v) {
+    if (t.order() != 0 || t.v[0].size() > 1 || (t.v[0].size() == 1 && !t.v[0].begin()->first.empty()))
+      return false;
+    v = t.v[0].empty() ? 0.0 : t.v[0].begin()->second;
+    return true;
+  }
+  // determinant of a 2x2 or 3x3 matrix of polynomials (cofactor expansion)
+  static Poly det(const Tensor &a) {
+    const int d = a.shape[0];
+    auto e = [&](int i, int j) -> const Poly & { return a.v[i * d + j]; };
+    if (d == 1) return e(0, 0);
+    if (d == 2) {
+      Poly r = pmul(e(0, 0), e(1, 1));
+      padd_to(r, pmul(e(0, 1), e(1, 0)), -1.0);
+      return r;
+    }
+    Poly r;
+    for (int j = 0; j < 3; ++j) {
+      Poly minor = pmul(e(1, (j + 1) % 3), e(2, (j + 2) % 3));
+      padd_to(minor, pmul(e(1, (j + 2) % 3), e(2, (j + 1) % 3)), -1.0);
+      padd_to(r, pmul(e(0, j), minor));
+    }
+    return r;
+  }
+  // Inv(M) for M = M0 + N: M0 a numeric invertible matrix, N's monomials in
+  // the value/gradient symbols of fem variables that are held at zero.  Around
+  // N = 0, Inv(M) = A - A N A + (A N)^2 A - ... with A = M0^{-1}; the products
+  // truncate the series at the degree the assembly needs.  This is what the
+  // pulled-back coordinates route uses: F = Id + Grad_w at w = 0.
+  Tensor inverse(const Tensor &a) {
+    if (a.order() != 2 || a.shape[0] != a.shape[1] || a.shape[0] > 3) fail("Inv of a non-square matrix");
+    const int d = a.shape[0];
+    std::vector<double> m0((size_t)d * d, 0.0);
+    Tensor nn = zeros({d, d});
+    for (int k = 0; k < d * d; ++k)
+      for (auto &kv : a.v[k]) {
+        if (kv.first.empty()) { m0[k] = kv.second; continue; }
+        for (int sy : kv.first) {
+          std::string var, tag;
+          split_symbol(C.symnames[sy], var, tag);
+          auto it = C.vars->find(var);
+          if (var == "c" || tag == "T" || tag == "GT" || tag == "S" || tag == "GS" || it == C.vars->end() ||
+              it->second.is_constant)
+            fail("Inv is supported for Id + (gradients of variables held at zero) only");
+          if (C.collect_inv) C.zero_vars.insert(var);
+        }
+        padd_to(nn.v[k], Poly{{kv.first, kv.second}});
+      }
+    // A = M0^{-1} by Gauss-Jordan with partial pivoting
+    std::vector<double> g(m0), inv((size_t)d * d, 0.0);
+    for (int i = 0; i < d; ++i) inv[i * d + i] = 1.0;
+    for (int c = 0; c < d; ++c) {
+      int piv = c;
+      for (int r = c + 1; r < d; ++r)
+        if (std::fabs(g[r * d + c]) > std::fabs(g[piv * d + c])) piv = r;
+      if (std::fabs(g[piv * d + c]) < 1e-300) fail("Inv of a singular matrix");
+      for (int k = 0; k < d; ++k) { std::swap(g[c * d + k], g[piv * d + k]); std::swap(inv[c * d + k], inv[piv * d + k]); }
+      const double pv = g[c * d + c];
+      for (int k = 0; k < d; ++k) { g[c * d + k] /= pv; inv[c * d + k] /= pv; }
+      for (int r = 0; r < d; ++r) {
+        if (r == c) continue;
+        const double f = g[r * d + c];
+        for (int k = 0; k < d; ++k) { g[r * d + k] -= f * g[c * d + k]; inv[r * d + k] -= f * inv[c * d + k]; }
+      }
+    }
+    Tensor A = zeros({d, d});
+    for (int k = 0; k < d * d; ++k) A.v[k] = pconst(inv[k]);
+    if (C.collect_inv) return A;
+    const int deg = C.trunc.max_deg < 0 ? 0 : C.trunc.max_deg;
+    if (C.trunc.max_deg < 0) {
+      bool none = true;
+      for (auto &p : nn.v) none = none && p.empty();
+      if (!none) fail("Inv of a matrix of variables is supported in order-0 and order-1 assemblies only");
+    }
+    // sum_{k <= deg} (-A N)^k A
+    Tensor term = A, total = A;
+    const Tensor mAN = tscale(tcontract(A, nn, 1), pconst(-1.0));
+    for (int k = 1; k <= deg; ++k) {
+      term = tcontract(mAN, term, 1);
+      total = tadd(total, term, 1.0);
+    }
+    return total;
   }
   Tensor function(const std::string &n) {
     expect('(');
@@ -368,7 +486,24 @@ struct Parser {
       return scalar(pconst(it->second.qdim));
     }
     Tensor a = expr();
+    if (n == "pow") {
+      expect(',');
+      Tensor b = expr();
+      expect(')');
+      double k;
+      if (!is_number(b, k) || k < 0 || k != std::floor(k) || k > 16)
+        fail("pow(x, k) is supported for integer exponents 0 <= k <= 16");
+      if (a.order() != 0) fail("pow of a non-scalar");
+      Poly r = pconst(1.0);
+      for (int i = 0; i < (int)k; ++i) r = pmul(r, a.v[0]);
+      return scalar(r);
+    }
     expect(')');
+    if (n == "Det") {
+      if (a.order() != 2 || a.shape[0] != a.shape[1] || a.shape[0] > 3) fail("Det of a non-square matrix");
+      return scalar(det(a));
+    }
+    if (n == "Inv") return inverse(a);
     if (n == "Id") {
       if (a.order() != 0 || a.v[0].size() != 1 || !a.v[0].begin()->first.empty()) fail("Id(n) needs a number");
       const int d = (int)std::lround(a.v[0].begin()->second);
@@ -468,7 +603,7 @@ struct Parser {
     if (it == C.vars->end()) fail("unknown name " + n);
     const GwflVar &v = it->second;
     if (v.is_fem && !v.is_constant) return field(n, "Val");
-    if (v.is_fem && !v.per_element) fail("fem constant " + n + " is not constant per element (only P0 supported)");
+    if (v.is_fem && !v.per_element) return field(n, "Val");  // fem data: values at the point
     if (v.qdim != 1) fail("constant " + n + " is not a scalar");
     return scalar(psym(C.sym("c:" + n)));
   }
@@ -567,7 +702,7 @@ bool gwfl_compile_order2(const std::vector<std::string> &exprs,
           const std::string tag = nm.substr(c + 1, nm.find('_', c) - c - 1);
           if (tag == "S" || tag == "GS") order = std::max(order, 2);
           else if (tag == "T" || tag == "GT") order = std::max(order, 1);
-          else dvars.insert(var);
+          else if (!vars.at(var).is_constant) dvars.insert(var);
         }
       for (const std::string &v : dvars) {
         if (!unknown.empty() && unknown != v) fail("several unknowns in one term are not supported");
@@ -612,6 +747,9 @@ bool gwfl_compile_order2(const std::vector<std::string> &exprs,
           t1 = comp;
         } else if (tag == "T" || tag == "S") {
           fail("value terms (Test_u, mass matrices) are not supported by the kernels");
+        } else if (vars.at(var).is_constant) {
+          fail("the order-2 term depends on the fem data " + var +
+               " (only P0 and fixed-size coefficients are supported by the matrix kernels)");
         } else {
           fail("the order-2 term depends on the unknown (nonlinear form outside the supported subset)");
         }
@@ -690,6 +828,125 @@ bool gwfl_compile_order2(const std::vector<std::string> &exprs,
     err = std::string("parse error: ") + e.what();
     return false;
   }
+}
+
+namespace {
+struct TruncGuard {
+  const Trunc *prev;
+  explicit TruncGuard(const Trunc *t) : prev(g_trunc) { g_trunc = t; }
+  ~TruncGuard() { g_trunc = prev; }
+};
+}  // namespace
+
+bool gwfl_compile_scalar(const std::vector<std::string> &exprs, const std::map<std::string, GwflVar> &vars,
+                         int order, const std::string &var, GwflScalarProgram &out, std::string &err) {
+  out = GwflScalarProgram();
+  out.order = order;
+  out.var = var;
+  try {
+    if (order != 0 && order != 1) fail("the scalar route assembles orders 0 and 1 only");
+    int q = 1;
+    if (order == 1) {
+      auto it = vars.find(var);
+      if (it == vars.end() || !it->second.is_fem || it->second.is_constant)
+        fail("order-1 assembly with respect to " + var + ", which is not a fem variable");
+      q = it->second.qdim;
+    }
+    for (const std::string &ex : exprs) {
+      std::smatch m;
+      std::string sq;
+      if (match_neohookean(ex, m, sq)) fail("the Neo-Hookean term is assembled by its own kernels");
+    }
+    // first pass: the variables Inv(...) expands around zero
+    Ctx C0;
+    C0.vars = &vars;
+    C0.collect_inv = true;
+    for (const std::string &ex : exprs) {
+      Parser P(ex, C0);
+      P.expr();
+    }
+    Ctx C;
+    C.vars = &vars;
+    C.zero_vars = C0.zero_vars;
+    C.trunc.max_deg = C.zero_vars.empty() ? -1 : 1;
+    TruncGuard guard(&C.trunc);
+    Poly total;
+    for (const std::string &ex : exprs) {
+      Parser P(ex, C);
+      Tensor t = P.expr();
+      P.skip();
+      if (P.pos != ex.size()) fail("trailing characters in expression");
+      if (t.order() != 0) fail("the expression is not a scalar");
+      Poly p = t.v[0];
+      const int o = test_order(p, C);
+      if (o > order) continue;  // a tree of a higher-order assembly only
+      if (order == 1) {
+        if (o == 0) {  // the potential's derivative in the direction Test_var
+          std::map<int, int> dir;
+          for (int a = 0; a < q; ++a) {
+            const std::string sa = q == 1 ? "" : std::to_string(a);
+            dir[C.sym(var + ":V_" + sa)] = C.sym(var + ":T_" + sa);
+            for (int b = 0; b < 3; ++b)
+              dir[C.sym(var + ":G_" + sa + std::to_string(b))] = C.sym(var + ":GT_" + sa + std::to_string(b));
+          }
+          p = pderiv(p, dir);
+        } else {  // an order-1 expression joins the block of its own test variable
+          std::string tv;
+          for (auto &kv : p)
+            for (int sy : kv.first) {
+              std::string v, tag;
+              split_symbol(C.symnames[sy], v, tag);
+              if (tag == "T" || tag == "GT") tv = v;
+            }
+          if (tv != var) continue;
+        }
+      } else if (o != 0) {
+        continue;
+      }
+      padd_to(total, p);
+    }
+    // at the zero value of the variables Inv expanded, their symbols vanish
+    for (auto it = total.begin(); it != total.end();) {
+      bool z = false;
+      for (int sy : it->first) z = z || (sy < (int)C.trunc.zsym.size() && C.trunc.zsym[sy]);
+      it = z ? total.erase(it) : std::next(it);
+    }
+    out.zero_vars.assign(C.zero_vars.begin(), C.zero_vars.end());
+    out.nout = order == 0 ? 1 : 4 * q;
+    std::map<int, int> data;  // symbol id -> index in out.syms
+    for (auto &kv : total) {
+      GwflScalarProgram::Mono mo;
+      mo.coef = kv.second;
+      mo.out = order == 0 ? 0 : -1;
+      for (int sy : kv.first) {
+        const std::string &nm = C.symnames[sy];
+        std::string v, tag;
+        split_symbol(nm, v, tag);
+        if (tag == "S" || tag == "GS") fail("a second test function in an order-0/1 expression");
+        if (tag == "T" || tag == "GT") {
+          if (order == 0 || v != var || mo.out >= 0) fail("term not linear in the test function of " + var);
+          const std::string comp = nm.substr(nm.find('_', nm.find(':')) + 1);
+          const int a = q == 1 ? 0 : comp[0] - '0';
+          mo.out = tag == "T" ? a : q + 3 * a + (comp.back() - '0');
+          continue;
+        }
+        auto it = data.find(sy);
+        if (it == data.end()) {
+          it = data.emplace(sy, (int)out.syms.size()).first;
+          out.syms.push_back(nm);
+        }
+        mo.s.push_back(it->second);
+      }
+      if (mo.out < 0) continue;  // order 1: a term without the test function (zero)
+      out.monos.push_back(mo);
+    }
+    return true;
+  } catch (const Error &e) {
+    err = e.msg;
+  } catch (const std::exception &e) {
+    err = std::string("parse error: ") + e.what();
+  }
+  return false;
 }
 
 }  // namespace gfx

@@ -48,4 +48,30 @@ bool gwfl_compile_order2(const std::vector<std::string> &exprs,
                          const std::map<std::string, GwflVar> &vars, GwflKernelParams &out,
                          std::string &err);
 
+// The order-0 / order-1 route (ga_workspace::assembly(0 | 1) of potentials
+// and their derivatives, src/getfem_generic_assembly_workspace.cc:520-541,
+// :832-847): the integrand as a polynomial program over data symbols (values
+// "f:V_a" and gradients "f:G_ab" of fem fields at the point, constants
+// "c:name").  order 0: one polynomial, the potential's integrand.  order 1,
+// variable `var` (qdim q): the derivative's coefficients of Test_var
+// (outputs 0..q-1) and of Grad_Test_var (output q + 3a + b); order-0
+// expressions are differentiated, order-1 expressions with Test_var join as
+// they are.  Variables inside Inv(...) are expanded around zero (zero_vars:
+// their values must be zero at assembly).
+struct GwflScalarProgram {
+  int order = 0;
+  std::string var;
+  int nout = 1;
+  std::vector<std::string> syms;
+  struct Mono {
+    double coef = 0.0;
+    std::vector<int> s;  // indices into syms
+    int out = 0;
+  };
+  std::vector<Mono> monos;
+  std::vector<std::string> zero_vars;
+};
+bool gwfl_compile_scalar(const std::vector<std::string> &exprs, const std::map<std::string, GwflVar> &vars,
+                         int order, const std::string &var, GwflScalarProgram &out, std::string &err);
+
 }  // namespace gfx

@@ -53,6 +53,28 @@ struct WsExpr {
   int order = -1;  // test-function order, set by the compile stage
 };
 
+struct ScProg {
+  GwflScalarProgram prog;
+  int vf = 0;                   // order 1: the field of the variable
+  int nm = 0;
+  double *d_coef = nullptr;
+  int32_t *d_off = nullptr, *d_sym = nullptr, *d_out = nullptr;
+};
+
+struct ScField {
+  std::string name;
+  gfx_mesh_fem *mf = nullptr;
+  int off = 0;
+  double *d_U = nullptr, *d_phi = nullptr, *d_gref = nullptr;
+};
+
+struct ScVar {                  // a fem variable's order-1 block
+  std::string name;
+  ScProg p;
+  int32_t *d_inc_ptr = nullptr;
+  uint32_t *d_inc = nullptr;
+};
+
 struct WsTree {
   gfx_mesh_im *mim = nullptr;
   std::vector<int> e2, e1;      // expressions of the order-2 and order-1 trees
@@ -66,6 +88,22 @@ struct WsTree {
 };
 
 }  // namespace
+
+struct ScalarRoute {
+  uint64_t version = ~0ull;
+  int err_code = GFX_OK;        // the route's build result at `version`
+  std::string err;
+  gfx_mesh_im *mim = nullptr;
+  std::vector<ScField> fields;
+  std::vector<std::string> consts;
+  int const_off = 0, nsym = 0;
+  double *d_consts = nullptr, *d_qw = nullptr;
+  ScProg p0;
+  std::vector<ScVar> vars;
+  double *d_elem = nullptr, *d_part = nullptr, *d_sum = nullptr, *d_V = nullptr;
+  int *d_err = nullptr;
+  int64_t nvec = 0;
+};
 
 struct gfx_workspace {
   gfx_ctx *ctx = nullptr;
@@ -87,6 +125,15 @@ struct gfx_workspace {
   int32_t *d_ix = nullptr;
   double *d_v = nullptr;
   int64_t nnz = -1;
+  // the order-0 / order-1 route (scalar.hip): potentials and the derivatives
+  // the matrix kernels do not cover (data fields, P0 variables, several
+  // variables, pulled-back coordinates)
+  uint64_t version = 0;         // registrations and expressions; the route rebuilds on change
+  bool mat_fail = false;        // the order-2 compile failed at this version
+  ScalarRoute *sc = nullptr;
+  int64_t vec_n = 0;            // size of the last assembled vector
+  double potential = 0.0;       // the last assembly(0)
+  bool has_potential = false;
   // set_assembled_matrix: a caller-owned device matrix the assembly adds into
   bool ext = false;
   int ext_orient = GFX_CSR;
@@ -334,7 +381,7 @@ int compile(gfx_workspace *w) {
       t->mim = e.mim;
     }
     if (e.der_order == 2) t->e2.push_back((int)i);
-    if (e.order <= 1) t->e1.push_back((int)i);
+    if (e.order <= 1 && e.der_order >= 1) t->e1.push_back((int)i);
   }
   std::string unknown;
   bool nonlinear = false;
@@ -389,6 +436,301 @@ int setup(gfx_workspace *w) {
   return GFX_OK;
 }
 
+// ------------------------------------------------ the order-0 / order-1 route
+void sc_free_prog(ScProg &p) {
+  wfree(p.d_coef); wfree(p.d_off); wfree(p.d_sym); wfree(p.d_out);
+  p.nm = 0;
+}
+
+void sc_release(ScalarRoute *sc) {
+  if (!sc) return;
+  for (auto &f : sc->fields) { wfree(f.d_U); wfree(f.d_phi); wfree(f.d_gref); }
+  sc_free_prog(sc->p0);
+  for (auto &v : sc->vars) { sc_free_prog(v.p); wfree(v.d_inc_ptr); wfree(v.d_inc); }
+  wfree(sc->d_consts); wfree(sc->d_qw); wfree(sc->d_elem); wfree(sc->d_part); wfree(sc->d_sum);
+  wfree(sc->d_V); wfree(sc->d_err);
+  sc->fields.clear(); sc->vars.clear(); sc->consts.clear();
+  sc->p0 = ScProg();
+  sc->mim = nullptr;
+}
+
+std::map<std::string, GwflVar> gwfl_vars(const gfx_workspace *w) {
+  std::map<std::string, GwflVar> gv;
+  for (auto &kv : w->vars) {
+    GwflVar g;
+    g.name = kv.first;
+    g.is_fem = kv.second.kind != CONST_FIXED;
+    g.is_constant = kv.second.kind != VAR_FEM;
+    g.qdim = g.is_fem ? kv.second.mf->qdim : (int)kv.second.V.size();
+    g.per_element = g.is_fem && kv.second.mf->fem.kind == FEM_PK_DISC0;
+    gv[kv.first] = g;
+  }
+  return gv;
+}
+
+template <typename T>
+hipError_t upload(T **d, const std::vector<T> &h) {
+  if (hipMalloc((void **)d, sizeof(T) * std::max<size_t>(h.size(), 1)) != hipSuccess) return hipErrorOutOfMemory;
+  if (h.empty()) return hipSuccess;
+  return hipMemcpy(*d, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+}
+
+// Compile the potential and every fem variable's derivative into programs,
+// lay out the symbol table (fields, then fixed-size constants) and upload
+// the programs, the fields' base tables and the incidences of each
+// variable's mesh_fem.  Cached per registration/expression version.
+int sc_build(gfx_workspace *w) {
+  ScalarRoute *sc = w->sc;
+  if (sc->version == w->version)
+    return sc->err_code == GFX_OK ? GFX_OK : wfail(w, sc->err_code, sc->err);
+  sc_release(sc);
+  sc->version = w->version;
+  sc->err_code = GFX_OK;
+  auto bail = [&](int code, const std::string &m) {
+    sc->err_code = code;
+    sc->err = m;
+    return wfail(w, code, m);
+  };
+  if (w->exprs.empty()) return bail(GFX_E_STATE, "no expression");
+  gfx_mesh_im *mim = w->exprs[0].mim;
+  for (auto &e : w->exprs)
+    if (e.mim != mim) return bail(GFX_E_UNSUPPORTED, "the order-0/1 route takes expressions on one mesh_im");
+  gfx_mesh *m = mim->mesh;
+  if (m->gt != GT_TET_P1) return bail(GFX_E_UNSUPPORTED, "the order-0/1 route needs GT_PK(3,1) tetrahedra");
+  sc->mim = mim;
+  const std::map<std::string, GwflVar> gv = gwfl_vars(w);
+  std::vector<std::string> all, deriv;  // deriv: expressions derived at least once (der_order >= 1)
+  for (auto &e : w->exprs) {
+    all.push_back(e.expr);
+    if (e.der_order >= 1) deriv.push_back(e.expr);
+  }
+  std::string err;
+  if (!gwfl_compile_scalar(all, gv, 0, "", sc->p0.prog, err)) return bail(GFX_E_UNSUPPORTED, err);
+  for (auto &kv : w->vars) {
+    if (kv.second.kind != VAR_FEM) continue;
+    ScVar v;
+    v.name = kv.first;
+    if (!gwfl_compile_scalar(deriv, gv, 1, kv.first, v.p.prog, err)) return bail(GFX_E_UNSUPPORTED, err);
+    sc->vars.push_back(v);
+  }
+  // the fields and constants the programs read, and the variables
+  std::vector<std::string> fnames;
+  auto need_field = [&](const std::string &n) {
+    if (std::find(fnames.begin(), fnames.end(), n) == fnames.end()) fnames.push_back(n);
+  };
+  auto scan = [&](const GwflScalarProgram &p) -> bool {
+    for (const std::string &sy : p.syms) {
+      const size_t c = sy.find(':');
+      const std::string var = sy.substr(0, c);
+      if (var == "c") {
+        const std::string n = sy.substr(2);
+        auto it = w->vars.find(n);
+        if (it == w->vars.end()) { err = "unknown name " + n; return false; }
+        if (it->second.kind == CONST_FIXED) {
+          if (std::find(sc->consts.begin(), sc->consts.end(), n) == sc->consts.end()) sc->consts.push_back(n);
+        } else {
+          need_field(n);
+        }
+      } else {
+        need_field(var);
+      }
+    }
+    return true;
+  };
+  if (!scan(sc->p0.prog)) return bail(GFX_E_UNSUPPORTED, err);
+  for (auto &v : sc->vars) {
+    if (!scan(v.p.prog)) return bail(GFX_E_UNSUPPORTED, err);
+    if (!v.p.prog.monos.empty()) need_field(v.name);
+  }
+  if (fnames.empty()) need_field(sc->vars.empty() ? w->vars.begin()->first : sc->vars[0].name);
+  if ((int)fnames.size() > kScalarMaxFields)
+    return bail(GFX_E_UNSUPPORTED, "the order-0/1 route reads at most 4 fem fields");
+  int off = 0;
+  const IntegrationRule &rule = mim->rule;
+  const int nq = rule.n();
+  for (const std::string &n : fnames) {
+    const WsVar &v = w->vars.at(n);
+    if (v.kind == CONST_FIXED || !v.mf) return bail(GFX_E_UNSUPPORTED, n + " is not a fem field");
+    if (v.mf->mesh != m) return bail(GFX_E_UNSUPPORTED, n + " lives on another mesh");
+    if (v.mf->fem.gt != GT_TET_P1) return bail(GFX_E_UNSUPPORTED, n + ": not a tetrahedral element");
+    ScField f;
+    f.name = n;
+    f.mf = v.mf;
+    f.off = off;
+    off += 4 * v.mf->qdim;
+    const int nd = v.mf->fem.nd;
+    std::vector<double> phi((size_t)nq * nd), gref((size_t)nq * nd * 3);
+    for (int q = 0; q < nq; ++q) fem_eval(v.mf->fem, &rule.pts[3 * (size_t)q], &phi[(size_t)q * nd],
+                                          &gref[(size_t)q * nd * 3]);
+    if (upload(&f.d_phi, phi) != hipSuccess || upload(&f.d_gref, gref) != hipSuccess ||
+        hipMalloc((void **)&f.d_U, 8 * (size_t)std::max<int64_t>(v.mf->ndof, 1)) != hipSuccess) {
+      sc->fields.push_back(f);
+      return bail(GFX_E_OOM, "order-0/1 route tables");
+    }
+    sc->fields.push_back(f);
+  }
+  sc->const_off = off;
+  for (const std::string &n : sc->consts) {
+    if (w->vars.at(n).V.size() != 1) return bail(GFX_E_UNSUPPORTED, "constant " + n + " is not a scalar");
+    ++off;
+  }
+  sc->nsym = off;
+  if (off > kScalarMaxSyms) return bail(GFX_E_UNSUPPORTED, "too many symbols for the order-0/1 route");
+  auto field_index = [&](const std::string &n) {
+    return (int)(std::find(fnames.begin(), fnames.end(), n) - fnames.begin());
+  };
+  // the programs in table slots
+  auto map_prog = [&](ScProg &p) -> int {
+    std::vector<int> slot;
+    for (const std::string &sy : p.prog.syms) {
+      const size_t c = sy.find(':');
+      const std::string var = sy.substr(0, c);
+      if (var == "c") {
+        const std::string n = sy.substr(2);
+        auto ci = std::find(sc->consts.begin(), sc->consts.end(), n);
+        slot.push_back(ci != sc->consts.end() ? sc->const_off + (int)(ci - sc->consts.begin())
+                                              : sc->fields[field_index(n)].off);
+        continue;
+      }
+      const ScField &f = sc->fields[field_index(var)];
+      const int q = f.mf->qdim;
+      const size_t us = sy.find('_', c);
+      const std::string tag = sy.substr(c + 1, us - c - 1), comp = sy.substr(us + 1);
+      const int a = q == 1 ? 0 : comp[0] - '0';
+      slot.push_back(tag == "V" ? f.off + a : f.off + q + 3 * a + (comp.back() - '0'));
+    }
+    std::vector<double> coef;
+    std::vector<int32_t> offs{0}, sym, out;
+    for (auto &mo : p.prog.monos) {
+      coef.push_back(mo.coef);
+      for (int x : mo.s) sym.push_back(slot[(size_t)x]);
+      offs.push_back((int32_t)sym.size());
+      out.push_back(mo.out);
+    }
+    p.nm = (int)coef.size();
+    if (upload(&p.d_coef, coef) != hipSuccess || upload(&p.d_off, offs) != hipSuccess ||
+        upload(&p.d_sym, sym) != hipSuccess || upload(&p.d_out, out) != hipSuccess)
+      return bail(GFX_E_OOM, "order-0/1 route programs");
+    return GFX_OK;
+  };
+  int r;
+  if ((r = map_prog(sc->p0))) return r;
+  int64_t elem = m->n_cv, nvec = 0;
+  hipStream_t s = w->ctx->stream;
+  for (auto &v : sc->vars) {
+    if ((r = map_prog(v.p))) return r;
+    const WsVar &wv = w->vars.at(v.name);
+    nvec = std::max(nvec, wv.first + wv.mf->ndof);
+    if (v.p.nm == 0) continue;
+    v.p.vf = field_index(v.name);
+    const int nd = wv.mf->fem.nd, Q = wv.mf->qdim;
+    elem = std::max(elem, m->n_cv * nd * Q);
+    const int64_t nn = wv.mf->ndof / Q;
+    int32_t *tmp = nullptr;
+    if (hipMalloc((void **)&v.d_inc_ptr, 4 * (size_t)(nn + 1)) != hipSuccess ||
+        hipMalloc((void **)&v.d_inc, 4 * (size_t)std::max<int64_t>(m->n_cv * nd, 1)) != hipSuccess ||
+        hipMalloc((void **)&tmp, 4 * (size_t)(2 * nn + scan_tmp_elems(nn))) != hipSuccess) {
+      wfree(tmp);
+      return bail(GFX_E_OOM, "order-0/1 route incidences");
+    }
+    hipError_t e = launch_incidences(s, m->n_cv, nd, Q, wv.mf->d_dofs, nn, v.d_inc_ptr, v.d_inc, tmp);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    wfree(tmp);
+    if (e != hipSuccess) return bail(GFX_E_HIP, std::string("incidences: ") + hipGetErrorString(e));
+  }
+  sc->nvec = nvec;
+  if (upload(&sc->d_qw, rule.w) != hipSuccess ||
+      hipMalloc((void **)&sc->d_elem, 8 * (size_t)std::max<int64_t>(elem, 1)) != hipSuccess ||
+      hipMalloc((void **)&sc->d_part, 8 * (size_t)scalar_partials()) != hipSuccess ||
+      hipMalloc((void **)&sc->d_sum, 8) != hipSuccess ||
+      hipMalloc((void **)&sc->d_V, 8 * (size_t)std::max<int64_t>(nvec, 1)) != hipSuccess ||
+      hipMalloc((void **)&sc->d_err, 4) != hipSuccess ||
+      hipMalloc((void **)&sc->d_consts, 8 * std::max<size_t>(sc->consts.size(), 1)) != hipSuccess)
+    return bail(GFX_E_OOM, "order-0/1 route buffers");
+  return GFX_OK;
+}
+
+// assembly(0): the potential; assembly(1): the vector of every fem variable's
+// derivative, at the variable's interval (first .. first + nb_dof)
+int sc_run(gfx_workspace *w, int order) {
+  if (!w->sc) w->sc = new (std::nothrow) ScalarRoute();
+  if (!w->sc) return wfail(w, GFX_E_OOM, "workspace");
+  int r;
+  if ((r = sc_build(w))) return r;
+  ScalarRoute *sc = w->sc;
+  gfx_mesh *m = sc->mim->mesh;
+  hipStream_t s = w->ctx->stream;
+  // variables that Inv(...) expanded around zero must be zero
+  auto zero_ok = [&](const GwflScalarProgram &p) -> bool {
+    for (const std::string &z : p.zero_vars)
+      for (double x : w->vars.at(z).V)
+        if (x != 0.0) {
+          wfail(w, GFX_E_UNSUPPORTED, "Inv(...) of " + z + " is expanded around " + z + " = 0, but " + z +
+                                          " is not zero");
+          return false;
+        }
+    return true;
+  };
+  if (order == 0 && !zero_ok(sc->p0.prog)) return GFX_E_UNSUPPORTED;
+  if (order == 1)
+    for (auto &v : sc->vars)
+      if (!zero_ok(v.p.prog)) return GFX_E_UNSUPPORTED;
+  for (auto &f : sc->fields) {
+    const WsVar &v = w->vars.at(f.name);
+    WHIP(w, hipMemcpyAsync(f.d_U, v.V.data(), 8 * v.V.size(), hipMemcpyHostToDevice, s));
+  }
+  std::vector<double> cv;
+  for (const std::string &n : sc->consts) cv.push_back(w->vars.at(n).V[0]);
+  if (!cv.empty()) WHIP(w, hipMemcpyAsync(sc->d_consts, cv.data(), 8 * cv.size(), hipMemcpyHostToDevice, s));
+  WHIP(w, hipMemsetAsync(sc->d_err, 0, 4, s));
+  ScalarArgs a{};
+  a.n_cv = m->n_cv; a.pts = m->d_pts; a.cvx = m->d_cvx;
+  a.nq = sc->mim->rule.n(); a.qw = sc->d_qw;
+  a.nf = (int)sc->fields.size();
+  for (int i = 0; i < a.nf; ++i) {
+    const ScField &f = sc->fields[i];
+    a.f[i] = ScalarField{f.mf->d_dofs, f.d_U, f.d_phi, f.d_gref, f.mf->fem.nd, f.mf->qdim, f.off};
+  }
+  a.nconst = (int)sc->consts.size(); a.const_off = sc->const_off; a.consts = sc->d_consts;
+  a.elem = sc->d_elem; a.err = sc->d_err;
+  auto set_prog = [&](const ScProg &p) {
+    a.nout = p.prog.nout; a.nm = p.nm; a.vf = p.vf;
+    a.mcoef = p.d_coef; a.moff = p.d_off; a.msym = p.d_sym; a.mout = p.d_out;
+  };
+  int flag = 0;
+  if (order == 0) {
+    set_prog(sc->p0);
+    a.order = 0;
+    a.vf = 0;
+    WHIP(w, launch_scalar_elem(s, a));
+    WHIP(w, launch_scalar_sum(s, m->n_cv, sc->d_elem, sc->d_part, sc->d_sum));
+    double J = 0.0;
+    WHIP(w, hipMemcpyAsync(&J, sc->d_sum, 8, hipMemcpyDeviceToHost, s));
+    WHIP(w, hipMemcpyAsync(&flag, sc->d_err, 4, hipMemcpyDeviceToHost, s));
+    WHIP(w, hipStreamSynchronize(s));
+    if (flag & 1) return wfail(w, GFX_E_DEGENERATE_ELEMENT, "Non invertible matrix (degenerate element)");
+    w->potential = J;
+    w->has_potential = true;
+    return GFX_OK;
+  }
+  WHIP(w, launch_fill_f64(s, sc->nvec, sc->d_V, 0.0));
+  for (auto &v : sc->vars) {
+    if (v.p.nm == 0) continue;
+    set_prog(v.p);
+    a.order = 1;
+    const WsVar &wv = w->vars.at(v.name);
+    WHIP(w, launch_scalar_elem(s, a));
+    WHIP(w, launch_gather_vec(s, wv.mf->ndof / wv.mf->qdim, wv.mf->fem.nd, wv.mf->qdim, v.d_inc_ptr, v.d_inc,
+                              sc->d_elem, sc->d_V + wv.first));
+  }
+  WHIP(w, hipMemcpyAsync(&flag, sc->d_err, 4, hipMemcpyDeviceToHost, s));
+  WHIP(w, hipStreamSynchronize(s));
+  if (flag & 1) return wfail(w, GFX_E_DEGENERATE_ELEMENT, "Non invertible matrix (degenerate element)");
+  w->V_out = sc->d_V;
+  w->vec_n = sc->nvec;
+  return GFX_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -408,6 +750,8 @@ int gfx_ws_destroy(gfx_workspace *w) {
   (void)hipSetDevice(w->ctx->device);
   drop_plans(w);
   wfree(w->d_u); wfree(w->d_V);
+  sc_release(w->sc);
+  delete w->sc;
   for (auto &kv : w->vars)
     if (kv.second.mf) unref(kv.second.mf);
   for (auto &e : w->exprs) unref(e.mim);
@@ -433,6 +777,8 @@ static int add_var(gfx_workspace *w, const char *name, int kind, gfx_mesh_fem *m
   if (old != w->vars.end() && old->second.mf) unref(old->second.mf);
   w->vars[name] = v;
   w->dirty = true;
+  w->mat_fail = false;
+  ++w->version;
   return GFX_OK;
 }
 
@@ -479,12 +825,14 @@ int gfx_ws_set_values_device(gfx_workspace *w, const char *name, const double *d
 
 int gfx_ws_add_expression(gfx_workspace *w, const char *expr, gfx_mesh_im *mim, int der_order) {
   if (!w || !expr || !mim) return GFX_E_INVALID;
-  if (der_order != 1 && der_order != 2)
-    return wfail(w, GFX_E_UNSUPPORTED, "derivation order must be 1 or 2");
+  if (der_order < 0 || der_order > 2)
+    return wfail(w, GFX_E_UNSUPPORTED, "derivation order must be 0, 1 or 2");
   if (mim->mesh->ctx != w->ctx) return wfail(w, GFX_E_INVALID, "mesh_im of another context");
   ref(mim);
   w->exprs.push_back({expr, mim, der_order, -1});
   w->dirty = true;
+  w->mat_fail = false;
+  ++w->version;
   return GFX_OK;
 }
 
@@ -536,12 +884,25 @@ int gfx_ws_set_assembled_matrix(gfx_workspace *w, int orientation, int64_t n, co
 // of the order-1 tree.
 int gfx_ws_assembly(gfx_workspace *w, int order) {
   if (!w) return GFX_E_INVALID;
-  if (order != 1 && order != 2)
-    return wfail(w, GFX_E_UNSUPPORTED, "only order-1 (vector) and order-2 (matrix) assembly are supported");
+  if (order < 0 || order > 2) return wfail(w, GFX_E_UNSUPPORTED, "assembly order must be 0, 1 or 2");
   if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
+  if (order == 0) return sc_run(w, 0);
   int r;
+  // order 1 outside the matrix kernels' subset (data fields, P0 or several
+  // variables, pulled-back coordinates): the polynomial route
+  if (order == 1 && w->mat_fail) return sc_run(w, 1);
   if (w->dirty) {
-    if ((r = setup(w))) return r;
+    if ((r = setup(w))) {
+      if (order != 1 || (r != GFX_E_UNSUPPORTED && r != GFX_E_STATE && r != GFX_E_INVALID)) return r;
+      const std::string merr = w->ctx->err;
+      const int r1 = sc_run(w, 1);
+      if (r1 == GFX_OK) {
+        w->mat_fail = true;
+        return GFX_OK;
+      }
+      w->ctx->err = merr + " (order-0/1 route: " + w->ctx->err + ")";
+      return r;
+    }
   } else if (w->coefs_dirty) {
     // new coefficient values: the plans re-derive their thresholds and
     // patterns at their next assembly
@@ -566,6 +927,7 @@ int gfx_ws_assembly(gfx_workspace *w, int order) {
       w->res_valid = false;
     }
     w->V_out = p->d_rhs;
+    w->vec_n = w->ndof;
     if (order == 1) {
       if (w->res_valid) return GFX_OK;
       if ((r = gfx_plan_assemble_residual(p))) return r;
@@ -612,6 +974,7 @@ int gfx_ws_assembly(gfx_workspace *w, int order) {
     if ((r = gfx_plan_spmv_add(p, w->d_u, w->d_V))) return r;
   }
   w->V_out = w->d_V;
+  w->vec_n = w->ndof;
   return GFX_OK;
 }
 
@@ -619,9 +982,23 @@ int gfx_ws_copy_assembled_vector_device(gfx_workspace *w, double *d_V) {
   if (!w || !d_V) return GFX_E_INVALID;
   if (!w->V_out) return wfail(w, GFX_E_STATE, "no order-1 assembly done yet");
   if (hipSetDevice(w->ctx->device) != hipSuccess ||
-      hipMemcpyAsync(d_V, w->V_out, 8 * (size_t)w->ndof, hipMemcpyDeviceToDevice, w->ctx->stream) !=
+      hipMemcpyAsync(d_V, w->V_out, 8 * (size_t)w->vec_n, hipMemcpyDeviceToDevice, w->ctx->stream) !=
           hipSuccess)
     return wfail(w, GFX_E_HIP, "copy of the assembled vector");
+  return GFX_OK;
+}
+
+int gfx_ws_assembled_potential(gfx_workspace *w, double *J) {
+  if (!w || !J) return GFX_E_INVALID;
+  if (!w->has_potential) return wfail(w, GFX_E_STATE, "no order-0 assembly done yet");
+  *J = w->potential;
+  return GFX_OK;
+}
+
+int gfx_ws_vector_size(gfx_workspace *w, int64_t *n) {
+  if (!w || !n) return GFX_E_INVALID;
+  if (!w->V_out) return wfail(w, GFX_E_STATE, "no order-1 assembly done yet");
+  *n = w->vec_n;
   return GFX_OK;
 }
 
@@ -630,7 +1007,7 @@ int gfx_ws_assembled_vector(gfx_workspace *w, double *h_V) {
   if (!w->V_out) return wfail(w, GFX_E_STATE, "no order-1 assembly done yet");
   if (hipSetDevice(w->ctx->device) != hipSuccess ||
       hipStreamSynchronize(w->ctx->stream) != hipSuccess ||
-      hipMemcpy(h_V, w->V_out, 8 * (size_t)w->ndof, hipMemcpyDeviceToHost) != hipSuccess)
+      hipMemcpy(h_V, w->V_out, 8 * (size_t)w->vec_n, hipMemcpyDeviceToHost) != hipSuccess)
     return wfail(w, GFX_E_HIP, "copy of the assembled vector");
   return GFX_OK;
 }
@@ -734,9 +1111,11 @@ static bool parse_decls(const char *decls, std::map<std::string, GwflVar> &gv) {
     g.name = item.substr(0, c1);
     const std::string kind = item.substr(c1 + 1, c2 - c1 - 1);
     g.qdim = std::atoi(item.c_str() + c2 + 1);
-    g.is_fem = kind == "var" || kind == "p0";
-    g.is_constant = kind != "var";
-    g.per_element = kind == "p0";
+    // var: fem variable; p0var: fem variable on a P0 mesh_fem; p0: P0 fem
+    // constant; data: fem constant of another fem; anything else: fixed size
+    g.is_fem = kind == "var" || kind == "p0" || kind == "p0var" || kind == "data";
+    g.is_constant = kind != "var" && kind != "p0var";
+    g.per_element = kind == "p0" || kind == "p0var";
     gv[g.name] = g;
   }
   return true;
@@ -769,6 +1148,47 @@ int gfx_gwfl_compile(const char *exprs, const char *decls, char *out, int64_t le
   const bool ok = gwfl_compile_order2(ex, gv, p, err);
   if (ok) describe_params(p, d);
   else d = err;
+  std::strncpy(out, d.c_str(), (size_t)len - 1);
+  out[len - 1] = 0;
+  return ok ? GFX_OK : GFX_E_UNSUPPORTED;
+}
+
+int gfx_gwfl_compile_scalar(const char *exprs, const char *decls, int order, const char *var, char *out,
+                            int64_t len) {
+  if (!exprs || !decls || !out || len <= 0 || (order == 1 && !var)) return GFX_E_INVALID;
+  std::map<std::string, GwflVar> gv;
+  if (!parse_decls(decls, gv)) return GFX_E_INVALID;
+  std::vector<std::string> ex;
+  std::string es(exprs);
+  size_t b = 0;
+  while (b <= es.size()) {
+    size_t e = es.find('\n', b);
+    if (e == std::string::npos) e = es.size();
+    if (e > b) ex.push_back(es.substr(b, e - b));
+    b = e + 1;
+  }
+  GwflScalarProgram prog;
+  std::string err, d;
+  const bool ok = gwfl_compile_scalar(ex, gv, order, var ? var : "", prog, err);
+  if (ok) {
+    d = "nout=" + std::to_string(prog.nout) + " zero=";
+    for (size_t k = 0; k < prog.zero_vars.size(); ++k) d += (k ? "," : "") + prog.zero_vars[k];
+    d += "\nsyms=";
+    for (size_t k = 0; k < prog.syms.size(); ++k) d += (k ? " " : "") + prog.syms[k];
+    char num[40];
+    for (const auto &m : prog.monos) {
+      std::snprintf(num, sizeof num, "%.17g", m.coef);
+      d += "\n" + std::to_string(m.out) + " " + num;
+      for (int x : m.s) d += " " + std::to_string(x);
+    }
+  } else {
+    d = err;
+  }
+  if ((int64_t)d.size() >= len) {
+    std::strncpy(out, "output buffer too small", (size_t)len - 1);
+    out[len - 1] = 0;
+    return GFX_E_INVALID;
+  }
   std::strncpy(out, d.c_str(), (size_t)len - 1);
   out[len - 1] = 0;
   return ok ? GFX_OK : GFX_E_UNSUPPORTED;

@@ -23,7 +23,7 @@ def _ptr(a):
 
 def gwfl_compile(exprs, decls):
     """Run the compile stage alone (no device): returns the normal form.
-    decls: {name: ("var"|"const"|"p0", n)}."""
+    decls: {name: ("var"|"const"|"p0"|"p0var"|"data", n)}."""
     d = ";".join("%s:%s:%d" % (k, v[0], v[1]) for k, v in decls.items())
     e = "\n".join([exprs] if isinstance(exprs, str) else exprs)
     buf = ctypes.create_string_buffer(1024)
@@ -89,11 +89,20 @@ class GaWorkspace:
         check(lib().gfx_ws_assembly(self.h, order), self.ctx.h)
 
     def assembled_vector(self):
-        """The order-1 result (nb_dof values of the unknown's mesh_fem)."""
-        n = self._nbdof()
-        V = np.zeros(n)
+        """The order-1 result: the unknown's nb_dof values on the matrix
+        kernels' route, every fem variable's block at its interval on the
+        order-0/1 route."""
+        n = ctypes.c_int64()
+        check(lib().gfx_ws_vector_size(self.h, ctypes.byref(n)), self.ctx.h)
+        V = np.zeros(n.value)
         check(lib().gfx_ws_assembled_vector(self.h, _ptr(V)), self.ctx.h)
         return V
+
+    def assembled_potential(self):
+        """The order-0 result (ga_workspace::assembled_potential)."""
+        J = ctypes.c_double()
+        check(lib().gfx_ws_assembled_potential(self.h, ctypes.byref(J)), self.ctx.h)
+        return J.value
 
     def set_values_device(self, name, d_V):
         """New values of the fem variable from a device vector (raw address,
@@ -191,4 +200,6 @@ def asm_generic(mim, order, expression, region, *args, ctx=None):
         return ws.assembled_matrix()
     if order == 1:
         return ws.assembled_vector()
-    raise ValueError("only orders 1 and 2 are supported")
+    if order == 0:
+        return ws.assembled_potential()
+    raise ValueError("order must be 0, 1 or 2")

@@ -116,6 +116,14 @@ int gfx_gwfl_compile(const char *exprs, const char *decls, char *out, int64_t le
  * its derivation order */
 int gfx_gwfl_order(const char *expr, const char *decls, int *order);
 
+/* The order-0 / order-1 compile stage alone (the potential, or the
+ * derivative with respect to the fem variable `var`), as text: a first line
+ * "nout=<outputs> zero=<variables held at zero>", a second "syms=<data
+ * symbols>", then one line per monomial "<output> <coefficient> <symbol
+ * indices>".  decls as gfx_gwfl_compile, plus the kinds "p0var" (a fem
+ * variable on a P0 mesh_fem) and "data" (a fem constant of another fem). */
+int gfx_gwfl_compile_scalar(const char *exprs, const char *decls, int order, const char *var, char *out,
+                            int64_t len);
 /* ======================= context ======================= */
 int gfx_ctx_create(int device, gfx_ctx **out);
 int gfx_ctx_destroy(gfx_ctx *ctx);
@@ -317,7 +325,16 @@ int gfx_ws_add_expression(gfx_workspace *w, const char *expr, gfx_mesh_im *mim, 
 int gfx_ws_set_form(gfx_workspace *w, int form, int precision);
 /* the compile stage alone (GFX_E_UNSUPPORTED with a message outside the subset) */
 int gfx_ws_compile(gfx_workspace *w);
-/* ga_workspace::assembly(order) (.h:564).  order 2: the matrix (for the
+/* ga_workspace::assembly(order) (.h:564).  order 0: the potential, the sum
+ * of the expressions of test-function order 0 (read it with
+ * gfx_ws_assembled_potential).  Orders 0 and 1 outside the matrix kernels'
+ * subset take the polynomial route: every integrand that is a polynomial in
+ * the values and gradients of fem fields (variables and fem constants on
+ * affine tetrahedra, P0 included), fixed-size constants, pow(x, k), Det and
+ * Inv(Id + Grad_w) at w = 0 (the pulled-back coordinates), compiled by
+ * symbolic derivation into per-point programs run by one kernel; the order-1
+ * vector then holds the derivative with respect to every fem variable at its
+ * interval.  order 2: the matrix (for the
  * Neo-Hookean term the pattern is rebuilt when the state or the coefficients
  * changed, since GetFEM's stored pattern depends on the values).  order 1: the
  * vector of the expression's first derivative at the unknown's current value
@@ -325,8 +342,16 @@ int gfx_ws_compile(gfx_workspace *w);
  * gfx_ws_assembled_vector.  Expressions must have been added with a
  * derivation order >= order. */
 int gfx_ws_assembly(gfx_workspace *w, int order);
-/* ga_workspace::assembled_vector() (.h:389-390): nb_dof doubles to the host */
+/* ga_workspace::assembled_vector() (.h:389-390): gfx_ws_vector_size doubles
+ * to the host */
 int gfx_ws_assembled_vector(gfx_workspace *w, double *h_V);
+/* size of the last assembled vector: the unknown's nb_dof on the matrix
+ * kernels' route, max(first + nb_dof) over the fem variables on the order-0/1
+ * route (GetFEM's nb_prim_dof) */
+int gfx_ws_vector_size(gfx_workspace *w, int64_t *n);
+/* ga_workspace::assembled_potential() (.h:394-401): the value of the last
+ * assembly(0) */
+int gfx_ws_assembled_potential(gfx_workspace *w, double *J);
 /* the same vector into a caller-owned device buffer (stream-ordered) */
 int gfx_ws_copy_assembled_vector_device(gfx_workspace *w, double *d_V);
 /* the plan of a workspace with one tree (GFX_E_STATE with several) */
