@@ -866,13 +866,14 @@ hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision) {
 // F_q = sqrt(|w_q| |J|) S B_q, computed in T.  The k axis is the 6 Voigt rows
 // of every point back to back (k = 6 q + s, no padding rows), 4 k per MFMA.
 // Lane layout of the 16x16x4 products: A[i][k] in lane i + 16 k, C[k][j] in
-// lane j + 16 k, D[i][j] in lane j + 16 (i / 4), element i % 4; so a lane
-// holds one F value per dof tile and k-step, and the A fragment of row tile a
-// is the C fragment of column tile a up to sign(w_q).  One wavefront per
-// element (grid stride), only the upper tiles computed.  E (row-major, T) is
-// stored from the accumulators: tile (a, b) as 16-value runs of rows
-// 16 a + 4 g + r (one per lane group g and element r), its mirror (b, a) as
-// 4-value runs per lane.
+// lane j + 16 k (both instructions); D[i][j] in lane j + 16 (i / 4), element
+// i % 4 for v_mfma_f32_16x16x4_f32, but in lane j + 16 (i % 4), element i / 4
+// for v_mfma_f64_16x16x4_f64 (BdbMfma<T>::row gives the row of a lane group g
+// and element r).  So a lane holds one F value per dof tile and k-step, and
+// the A fragment of row tile a is the C fragment of column tile a up to
+// sign(w_q).  One wavefront per element (grid stride), only the upper tiles
+// computed.  E (row-major, T) is stored from the accumulators: tile (a, b) as
+// 16-value runs of its rows, its mirror (b, a) as 4-value runs per lane.
 template <typename T>
 struct BdbMfma;
 template <>
@@ -881,6 +882,7 @@ struct BdbMfma<double> {
   static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
     return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * r + g; }
 };
 template <>
 struct BdbMfma<float> {
@@ -888,6 +890,7 @@ struct BdbMfma<float> {
   static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
     return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
   }
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
 };
 
 // F[s][m] of one point: Voigt row s (xx, yy, zz, yz, xz, xy) of S B_q for
@@ -981,7 +984,7 @@ __global__ __launch_bounds__(256) void k_elast_bdb_mfma_hp(int64_t n_cv, int nq,
         const int col = 16 * b + lr;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int row = 16 * a + 4 * lg + r;
+          const int row = 16 * a + BdbMfma<T>::row(lg, r);
           if (row < NL && col < NL) {
             __builtin_nontemporal_store(acc[p][r], Ee + row * NL + col);
             if (a != b) __builtin_nontemporal_store(acc[p][r], Ee + col * NL + row);
