@@ -114,6 +114,7 @@ SIGNATURES = {
     "gfx_ws_describe": (_i32, [_P, ctypes.c_char_p, _i64]),
     "gfx_compliance_create": (_i32, [_P, _P, _f64, _f64, _f64, _f64, _pp]),
     "gfx_compliance_eval": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),
+    "gfx_compliance_eval_noalloc": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "gfx_compliance_status": (_i32, [_P]),
     "gfx_compliance_destroy": (_i32, [_P]),
 }

@@ -12,6 +12,8 @@
 // floating-point atomics), and J is a fixed-order two-stage reduction.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "gfx_internal.h"
 
 namespace gfx {
@@ -179,6 +181,38 @@ __global__ __launch_bounds__(kRedThreads) void k_compliance_finish(
 }
 
 }  // namespace
+
+// The allocation-free entry point's scratch: the block partials of J and the
+// per-(element, vertex) contributions to dJ/dX and dJ/du, in one file-scope
+// device array of kComplianceScratchBytes.  Every GPU the library runs on
+// holds its own copy until the process ends; host code reaches a device's
+// copy only through the address hipGetSymbolAddress returns with that device
+// current (the array's name is a host-side stand-in, not a device address).
+static __device__ __align__(256) double scratch[kComplianceScratchBytes / 8];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch[kMaxDevices];
+}  // namespace
+
+hipError_t compliance_scratch(int device, double **ptr) {
+  *ptr = nullptr;
+  if (device < 0 || device >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch[device]) {
+    int cur = -1;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return e;
+    if (cur != device) return hipErrorInvalidDevice;  // the caller makes the device current
+    void *p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch[device] = static_cast<double *>(p);
+  }
+  *ptr = g_scratch[device];
+  return hipSuccess;
+}
 
 int64_t compliance_nblocks(int64_t n_cv) { return (n_cv + 255) / 256; }
 

@@ -154,6 +154,11 @@ int gfx_ctx_create(int device, gfx_ctx **out) {
     return GFX_E_HIP;
   }
   c->own_stream = true;
+  // set-up of the allocation-free compliance entry point: the address of this
+  // device's copy of its static scratch (kept per device; a failure here is
+  // reported by that entry point, which retries the lookup)
+  double *scr = nullptr;
+  (void)compliance_scratch(device, &scr);
   *out = c;
   return GFX_OK;
 }
@@ -1413,6 +1418,35 @@ int gfx_compliance_eval(gfx_compliance *c, const double *d_X, const double *d_rh
   HIPCHK(ctx, launch_compliance_finish(s, c->n_cv, c->n_pts, c->d_vinc_ptr, c->d_vinc, c->d_cX, d_dJ_dX,
                                        c->ndof / 3, c->d_dinc_ptr, c->d_dinc, c->d_cU, d_dJ_du,
                                        c->d_Jpart, d_J));
+  return GFX_OK;
+}
+
+int gfx_compliance_eval_noalloc(gfx_compliance *c, const double *d_X, const double *d_rho, const double *d_u,
+                                double *d_J, double *d_dJ_drho, double *d_dJ_dX, double *d_dJ_du) {
+  if (!c || !d_rho || !d_u || !d_J) return GFX_E_INVALID;
+  gfx_ctx *ctx = c->ctx;
+  if (set_device(ctx)) return GFX_E_HIP;
+  double *scr = nullptr;
+  if (compliance_scratch(ctx->device, &scr) != hipSuccess || !scr)
+    return fail(ctx, GFX_E_HIP, "hipGetSymbolAddress of the compliance scratch failed");
+  // [J block partials, padded to 256 bytes][dJ/dX contributions][dJ/du contributions]
+  const int64_t jp = (2 * compliance_nblocks(c->n_cv) + 31) & ~int64_t(31);
+  const int64_t per = 12 * c->n_cv;
+  const int64_t need = jp + (d_dJ_dX ? per : 0) + (d_dJ_du ? per : 0);
+  if (8 * need > (int64_t)kComplianceScratchBytes)
+    return fail(ctx, GFX_E_INVALID,
+                "gfx_compliance_eval_noalloc: " + std::to_string(c->n_cv) + " elements need " +
+                    std::to_string(8 * need) + " bytes of scratch, the static scratch holds " +
+                    std::to_string(kComplianceScratchBytes) + " (at most about " +
+                    std::to_string(kComplianceScratchBytes / 192) + " elements with both node gradients)");
+  double *Jpart = scr, *cX = d_dJ_dX ? scr + jp : nullptr;
+  double *cU = d_dJ_du ? scr + jp + (d_dJ_dX ? per : 0) : nullptr;
+  hipStream_t s = ctx->stream;
+  const gfx_mesh *m = c->mf->mesh;
+  HIPCHK(ctx, launch_compliance(s, c->n_cv, d_X ? d_X : m->d_pts, m->d_cvx, c->mf->d_dofs, d_u, d_rho, c->lam,
+                                c->mu, c->emin, c->penal, c->wsum, Jpart, d_J, d_dJ_drho, cX, cU, c->d_err));
+  HIPCHK(ctx, launch_compliance_finish(s, c->n_cv, c->n_pts, c->d_vinc_ptr, c->d_vinc, cX, d_dJ_dX, c->ndof / 3,
+                                       c->d_dinc_ptr, c->d_dinc, cU, d_dJ_du, Jpart, d_J));
   return GFX_OK;
 }
 

@@ -253,6 +253,11 @@ hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr
 
 // adjoint.hip
 int64_t compliance_nblocks(int64_t n_cv);
+// the allocation-free compliance entry point's static scratch (adjoint.hip):
+// its size, and the address of the given device's copy (looked up once per
+// device with that device current; later calls return the kept address)
+constexpr size_t kComplianceScratchBytes = size_t(64) << 20;
+hipError_t compliance_scratch(int device, double **ptr);
 hipError_t launch_compliance(hipStream_t s, int64_t n_cv, const double *X, const int32_t *cvx,
                              const int32_t *dofs, const double *u, const double *rho, double lam,
                              double mu, double emin, double penal, double wsum, double *Jpart,

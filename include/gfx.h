@@ -399,6 +399,22 @@ int gfx_compliance_create(gfx_mesh_fem *mf, gfx_mesh_im *mim, double lambda, dou
 int gfx_compliance_eval(gfx_compliance *c, const double *d_X, const double *d_rho,
                         const double *d_u, double *d_J, double *d_dJ_drho,
                         double *d_dJ_dX, double *d_dJ_du);
+/* gfx_compliance_eval without the object's scratch buffers: no counterpart in
+ * the reference.  For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * the J partials and the per-element node contributions live in one static
+ * device array of 64 MiB per GPU, allocated when the library's code is
+ * loaded on that device (its address is looked up by gfx_ctx_create).  The
+ * call allocates nothing; inputs and outputs are the caller's, as for
+ * gfx_compliance_eval, with the same results.  Limit: 8 * (2 * ceil(n_cv /
+ * 256) rounded up to 32) + 96 * n_cv bytes for each requested node gradient
+ * (d_dJ_dX, d_dJ_du) must fit in 64 MiB (about 349,000 elements with both);
+ * larger meshes return GFX_E_INVALID.  Every call on one GPU shares that
+ * GPU's array: calls that overlap on one GPU (two streams, two threads) race
+ * and must be ordered by the caller. */
+int gfx_compliance_eval_noalloc(gfx_compliance *c, const double *d_X, const double *d_rho,
+                                const double *d_u, double *d_J, double *d_dJ_drho,
+                                double *d_dJ_dX, double *d_dJ_du);
 /* GFX_E_DEGENERATE_ELEMENT if an evaluation since the last call met an
  * element with det K = 0 (GetFEM: "Non invertible matrix",
  * bgeot_geometric_trans.cc:380); synchronises the context's stream and clears
