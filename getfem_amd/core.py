@@ -271,6 +271,31 @@ class Plan:
                                       ctypes.byref(it), ctypes.byref(res)), self.ctx.h)
         return it.value, res.value
 
+    def solve_pcg(self, d_b, d_x, precond=_lib.PRECOND_ILDLT, rtol=1e-10, maxit=10000):
+        """CG on the device with PRECOND_ILDLT (gmm::ildlt_precond, rebuilt
+        from the current values) or PRECOND_BLOCK_JACOBI.  Returns
+        (iterations, relative residual)."""
+        it, res = ctypes.c_int(), ctypes.c_double()
+        check(lib().gfx_plan_solve_pcg(self.h, precond, ctypes.c_void_p(d_b), ctypes.c_void_p(d_x), rtol, maxit,
+                                       ctypes.byref(it), ctypes.byref(res)), self.ctx.h)
+        return it.value, res.value
+
+    def ildlt_factor(self):
+        """Factor the current values; returns the (forward, backward) level counts."""
+        f, b = ctypes.c_int64(), ctypes.c_int64()
+        check(lib().gfx_plan_ildlt_factor(self.h, ctypes.byref(f), ctypes.byref(b)), self.ctx.h)
+        return f.value, b.value
+
+    def ildlt_values(self):
+        """The factor aligned with the CSR: D on the diagonal, U_ij / D_i at
+        (i, j) and (j, i)."""
+        F = np.zeros(self.dims()[1])
+        check(lib().gfx_plan_ildlt_values(self.h, _ptr(F)), self.ctx.h)
+        return F
+
+    def ildlt_apply(self, d_r, d_z):
+        check(lib().gfx_plan_ildlt_apply(self.h, ctypes.c_void_p(d_r), ctypes.c_void_p(d_z)), self.ctx.h)
+
     def residual(self):
         n, _ = self.dims()
         V = np.zeros(n)

@@ -45,6 +45,12 @@ void dfree(T *&p) {
   p = nullptr;
 }
 
+void il_free(gfx::IldltData &L) {
+  dfree(L.d_dpos); dfree(L.d_fdeg); dfree(L.d_bdeg); dfree(L.d_flist); dfree(L.d_blist); dfree(L.d_counter);
+  dfree(L.d_F); dfree(L.d_work); dfree(L.d_cg);
+  L = gfx::IldltData();
+}
+
 int set_device(gfx_ctx *c) {
   hipError_t e = hipSetDevice(c->device);
   if (e != hipSuccess) return hipfail(c, e, "hipSetDevice");
@@ -410,6 +416,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
   dfree(p->d_gN); dfree(p->d_E); dfree(p->d_res); dfree(p->d_U); dfree(p->d_rhs); dfree(p->d_cg);
   dfree(p->d_fixed); dfree(p->d_go);
+  il_free(p->il);
   for (hipEvent_t e : p->ev) (void)hipEventDestroy(e);
   gfx_ctx *c = p->ctx;
   gfx_mesh_fem *mf = p->mf;
@@ -705,7 +712,7 @@ static int plan_waves(gfx_plan *p, const std::vector<uint16_t> &ns, const std::v
   gfx_ctx *c = p->ctx;
   const int64_t nn = p->n_nodes;
   const int Q = p->qdim, half = p->nd / 2;
-  constexpr int kWaveLds = 6144;
+  constexpr int kWaveLds = 5120;  // measured: 5 KB beats 4, 6 and 8 KB (P2, 64^3 x 6)
   const int inc_cap = 3 * (64 / (p->nd >= 20 ? p->nd / 2 : p->nd));  // three passes (kernel: lanes per incidence)
   auto need = [&](int64_t n, int &acc, int &flat, int &inc) {
     const int s = ns[n] & 0x7fff;
@@ -1136,6 +1143,106 @@ int gfx_plan_spmv(gfx_plan *p, const double *d_x, double *d_y) {
   if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern first");
   if (set_device(c)) return GFX_E_HIP;
   HIPCHK(c, launch_spmv(c->stream, p->ndof, p->nnz, p->d_indptr, p->d_indices, p->d_vals, d_x, d_y));
+  return GFX_OK;
+}
+
+// ---- incomplete LDL^T (gmm::ildlt_precond) of the plan's matrix
+static int il_prepare(gfx_plan *p) {
+  gfx_ctx *c = p->ctx;
+  if (p->nnz < 0) return fail(c, GFX_E_STATE, "build the pattern and assemble first");
+  IldltData &L = p->il;
+  const int64_t n = p->ndof;
+  const int qb = (p->qdim == 3 && n % 3 == 0) ? 3 : 1;
+  hipStream_t s = c->stream;
+  int herr = 0;
+  if (L.pattern_id != p->pattern_id || L.n != n || L.qb != qb) {
+    il_free(L);
+    const int64_t nu = n / qb;
+    HIPCHK(c, dalloc(&L.d_dpos, (size_t)std::max<int64_t>(n, 1)));
+    HIPCHK(c, dalloc(&L.d_fdeg, (size_t)std::max<int64_t>(nu, 1)));
+    HIPCHK(c, dalloc(&L.d_bdeg, (size_t)std::max<int64_t>(nu, 1)));
+    HIPCHK(c, dalloc(&L.d_flist, (size_t)std::max<int64_t>(nu, 1)));
+    HIPCHK(c, dalloc(&L.d_blist, (size_t)std::max<int64_t>(nu, 1)));
+    HIPCHK(c, dalloc(&L.d_counter, 1));
+    HIPCHK(c, dalloc(&L.d_F, (size_t)std::max<int64_t>(p->nnz, 1)));
+    HIPCHK(c, dalloc(&L.d_work, (size_t)(n / 1024 + 2)));
+    // the longest row sizes the factorization's LDS
+    std::vector<int64_t> ip((size_t)n + 1);
+    HIPCHK(c, hipMemcpyAsync(ip.data(), p->d_indptr, 8 * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    int64_t cap = 0;
+    for (int64_t i = 0; i < n; ++i) cap = std::max(cap, ip[(size_t)i + 1] - ip[(size_t)i]);
+    if (20 * cap > 64 * 1024) {
+      il_free(L);
+      return fail(c, GFX_E_UNSUPPORTED, "ILDLT: a row of " + std::to_string(cap) + " entries exceeds the "
+                                        "factorization's LDS (3276 entries)");
+    }
+    L.cap = (int)cap;
+    HIPCHK(c, ildlt_analyse(s, L, n, qb, p->d_indptr, p->d_indices, p->d_err, &herr));
+    if (herr) {
+      il_free(L);
+      return fail(c, GFX_E_UNSUPPORTED, (herr & 1) ? "ILDLT: a row has no diagonal entry"
+                                                   : "ILDLT: the matrix pattern is not symmetric");
+    }
+    L.pattern_id = p->pattern_id;
+  }
+  HIPCHK(c, ildlt_factor(s, L, p->d_indptr, p->d_indices, p->d_vals, L.cap, p->d_err, &herr));
+  if (herr) {
+    L.factored = false;
+    return fail(c, GFX_E_UNSUPPORTED, "ILDLT: a pivot that GetFEM replaces by 1 (|D_k| <= the running pivot "
+                                      "bound, gmm_precond_ildlt.h:171-176); the device factorization does not");
+  }
+  L.factored = true;
+  return GFX_OK;
+}
+
+int gfx_plan_ildlt_factor(gfx_plan *p, int64_t *n_levels_forward, int64_t *n_levels_backward) {
+  if (!p) return GFX_E_INVALID;
+  if (set_device(p->ctx)) return GFX_E_HIP;
+  int r = il_prepare(p);
+  if (r) return r;
+  if (n_levels_forward) *n_levels_forward = (int64_t)p->il.flev.size() - 1;
+  if (n_levels_backward) *n_levels_backward = (int64_t)p->il.blev.size() - 1;
+  return GFX_OK;
+}
+
+int gfx_plan_ildlt_values(gfx_plan *p, double *h_F) {
+  if (!p || !h_F) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (!p->il.factored || p->il.pattern_id != p->pattern_id) return fail(c, GFX_E_STATE, "no ILDLT factor");
+  if (set_device(c)) return GFX_E_HIP;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipMemcpy(h_F, p->il.d_F, 8 * (size_t)p->nnz, hipMemcpyDeviceToHost));
+  return GFX_OK;
+}
+
+int gfx_plan_ildlt_apply(gfx_plan *p, const double *d_r, double *d_z) {
+  if (!p || !d_r || !d_z) return GFX_E_INVALID;
+  gfx_ctx *c = p->ctx;
+  if (!p->il.factored || p->il.pattern_id != p->pattern_id) return fail(c, GFX_E_STATE, "no ILDLT factor");
+  if (set_device(c)) return GFX_E_HIP;
+  IldltData &L = p->il;
+  if (!L.d_cg) HIPCHK(c, dalloc(&L.d_cg, ildlt_cg_doubles(p->ndof)));
+  HIPCHK(c, ildlt_apply(c->stream, L, p->d_indptr, p->d_indices, d_r, d_z, L.d_cg + 4 * p->ndof,
+                        L.d_cg + 5 * p->ndof));
+  return GFX_OK;
+}
+
+int gfx_plan_solve_pcg(gfx_plan *p, int precond, const double *d_b, double *d_x, double rtol, int maxit,
+                       int *iters, double *rel_res) {
+  if (!p || !d_b || !d_x || !iters || !rel_res || !(rtol > 0) || maxit < 0) return GFX_E_INVALID;
+  if (precond == GFX_PRECOND_BLOCK_JACOBI) return gfx_plan_solve_cg(p, d_b, d_x, rtol, maxit, iters, rel_res);
+  gfx_ctx *c = p->ctx;
+  if (precond != GFX_PRECOND_ILDLT) return fail(c, GFX_E_INVALID, "unknown preconditioner");
+  if (set_device(c)) return GFX_E_HIP;
+  // the preconditioner is built from the current values at every solve, as
+  // linear_solver_cg_preconditioned_ildlt does
+  int r = il_prepare(p);
+  if (r) return r;
+  IldltData &L = p->il;
+  if (!L.d_cg) HIPCHK(c, dalloc(&L.d_cg, ildlt_cg_doubles(p->ndof)));
+  HIPCHK(c, ildlt_pcg(c->stream, L, p->nnz, p->d_indptr, p->d_indices, p->d_vals, d_b, d_x, rtol, maxit, L.d_cg,
+                      iters, rel_res));
   return GFX_OK;
 }
 

@@ -280,6 +280,35 @@ struct gfx_compliance {
   int *d_err = nullptr;
 };
 
+namespace gfx {
+// Incomplete LDL^T of a plan's matrix (ildlt.hip): diagonal positions, level
+// sets of units of qb rows (forward: host offsets flev into d_flist;
+// backward: blev into d_blist), and the factor F aligned with the CSR.
+struct IldltData {
+  int64_t n = 0;
+  int qb = 1;
+  int cap = 0;                    // longest row
+  int64_t pattern_id = -1;        // pattern the analysis belongs to
+  bool factored = false;
+  int64_t *d_dpos = nullptr;
+  int32_t *d_fdeg = nullptr, *d_bdeg = nullptr, *d_flist = nullptr, *d_blist = nullptr, *d_counter = nullptr;
+  double *d_F = nullptr;          // nnz
+  double *d_work = nullptr;       // pivot-check block maxima
+  double *d_cg = nullptr;         // conjugate gradient vectors
+  std::vector<int64_t> flev, blev;
+};
+hipError_t ildlt_analyse(hipStream_t s, IldltData &L, int64_t n, int qb, const int64_t *ip, const int32_t *ix,
+                         int *d_err, int *h_err);
+hipError_t ildlt_factor(hipStream_t s, IldltData &L, const int64_t *ip, const int32_t *ix, const double *av,
+                        int cap, int *d_err, int *h_err);
+hipError_t ildlt_apply(hipStream_t s, const IldltData &L, const int64_t *ip, const int32_t *ix, const double *r,
+                       double *z, double *y, double *w);
+size_t ildlt_cg_doubles(int64_t n);
+hipError_t ildlt_pcg(hipStream_t s, const IldltData &L, int64_t nnz, const int64_t *ip, const int32_t *ix,
+                     const double *av, const double *b, double *x, double rtol, int maxit, double *work,
+                     int *iters, double *rel_res);
+}  // namespace gfx
+
 struct gfx_plan {
   gfx_ctx *ctx = nullptr;
   gfx_mesh_fem *mf = nullptr;
@@ -340,6 +369,7 @@ struct gfx_plan {
   bool eb = false;
   double *d_gN = nullptr, *d_E = nullptr, *d_res = nullptr, *d_U = nullptr, *d_rhs = nullptr;
   double *d_cg = nullptr;  // conjugate gradient work vectors (allocated on first solve)
+  gfx::IldltData il;       // incomplete LDL^T preconditioner (built on first use)
   uint8_t *d_fixed = nullptr;  // dof constraints: mask and prescribed values
   double *d_go = nullptr;
   std::vector<hipStream_t> aux;  // streams the patch-size bins run on beside ctx->stream

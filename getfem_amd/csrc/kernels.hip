@@ -639,7 +639,10 @@ __global__ __launch_bounds__(kWave) void k_assemble_patch(PatchArgs a) {
         for (int x = 0; x < Q; ++x)
           for (int y = 0; y < Q; ++y) {
             const double val = (double)v[x][y];
-            if (fabs(val) > thr) atomicAdd(&ak[(x * nsk + s) * Q + y], val);
+            // a dropped entry adds +0.0 (the sum is unchanged: the
+            // accumulator starts at +0.0 and is never -0.0), so the adds
+            // need no branch
+            atomicAdd(&ak[(x * nsk + s) * Q + y], fabs(val) > thr ? val : 0.0);
           }
       }
     }

@@ -260,6 +260,29 @@ int gfx_plan_add_to_matrix(gfx_plan *p, int orientation, int64_t n, const int64_
  * with rtol, as gmm::iteration::converged). */
 int gfx_plan_solve_cg(gfx_plan *p, const double *d_b, double *d_x, double rtol, int maxit,
                       int *iters, double *rel_res);
+/* The same solve with a choice of preconditioner: GFX_PRECOND_BLOCK_JACOBI
+ * (gfx_plan_solve_cg) or GFX_PRECOND_ILDLT, GetFEM's default for coercive
+ * problems: gmm::ildlt_precond (src/gmm/gmm_precond_ildlt.h:136-206), the
+ * level-0 incomplete LDL^T on the pattern of K's upper triangle, rebuilt from
+ * the current values at every solve as linear_solver_cg_preconditioned_ildlt
+ * does (src/getfem/getfem_model_solvers.h:90-98), inside gmm::cg
+ * (src/gmm/gmm_solver_cg.h:84-118).  The factor and its triangular solves
+ * equal the reference's sequential algorithm bit for bit (same operations,
+ * same order, no fused multiply-add); rows are scheduled in level sets of
+ * node-sized units.  Needs a symmetric pattern with every diagonal entry
+ * (GFX_E_UNSUPPORTED otherwise) and rows of at most 3276 entries; a pivot the
+ * reference would replace by 1 ("pivot too small") returns GFX_E_UNSUPPORTED. */
+#define GFX_PRECOND_BLOCK_JACOBI 0
+#define GFX_PRECOND_ILDLT 1
+int gfx_plan_solve_pcg(gfx_plan *p, int precond, const double *d_b, double *d_x, double rtol,
+                       int maxit, int *iters, double *rel_res);
+/* ILDLT pieces (for inspection and tests): factor the current values (level
+ * counts out, either may be NULL); the factor aligned with the CSR (nnz
+ * doubles: D_i on the diagonal, U_ij / D_i at (i, j) and at (j, i)); and
+ * z = (L D L^T)^{-1} r on the device (stream-ordered). */
+int gfx_plan_ildlt_factor(gfx_plan *p, int64_t *n_levels_forward, int64_t *n_levels_backward);
+int gfx_plan_ildlt_values(gfx_plan *p, double *h_F);
+int gfx_plan_ildlt_apply(gfx_plan *p, const double *d_r, double *d_z);
 /* Dof constraints as model::assembly applies them after the bricks
  * ("Post simplification for dof constraints", src/getfem_models.cc:
  * 2806-2870; added by add_Dirichlet_condition_with_simplification, :4864-4940

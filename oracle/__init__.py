@@ -54,6 +54,11 @@ def lib():
         L.orc_assemble_vector.argtypes = [P, P]
         L.orc_compliance.argtypes = [P, P, d, d, P, P]
         L.orc_compliance.restype = d
+        L.orc_ildlt_pattern.argtypes = [i64, P, P, P, P]
+        L.orc_ildlt_pattern.restype = i64
+        L.orc_ildlt.argtypes = [i64, P, P, P, P, P, P]
+        L.orc_ildlt.restype = i64
+        L.orc_ildlt_apply.argtypes = [i64, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -204,3 +209,70 @@ def csc_to_csr(colptr, rows, vals, n):
     r = m.tocsr()
     r.sort_indices()
     return r.indptr.astype(np.int64), r.indices.astype(np.int32), r.data
+
+
+class Ildlt:
+    """gmm::ildlt_precond of a CSR matrix (sorted columns): U's slots per row
+    (diagonal first, then the columns above it), D on the diagonal slots,
+    U_ij / D_i elsewhere; ``replaced`` counts pivots replaced by 1."""
+
+    def __init__(self, indptr, indices, vals):
+        ip = np.ascontiguousarray(indptr, dtype=np.int64)
+        ix = np.ascontiguousarray(indices, dtype=np.int32)
+        av = np.ascontiguousarray(vals, dtype=np.float64)
+        self.n = n = ip.size - 1
+        self.uptr = np.zeros(n + 1, np.int64)
+        m = lib().orc_ildlt_pattern(n, _ptr(ip), _ptr(ix), _ptr(self.uptr), None)
+        self.uind = np.zeros(m, np.int32)
+        lib().orc_ildlt_pattern(n, _ptr(ip), _ptr(ix), _ptr(self.uptr), _ptr(self.uind))
+        self.uval = np.zeros(m)
+        self.replaced = lib().orc_ildlt(n, _ptr(ip), _ptr(ix), _ptr(av), _ptr(self.uptr), _ptr(self.uind),
+                                        _ptr(self.uval))
+
+    def diag(self):
+        return self.uval[self.uptr[:-1]].copy()
+
+    def factors(self):
+        """(L unit lower as scipy CSR, D) with L = U^T."""
+        import scipy.sparse as sp
+        n = self.n
+        rows = np.repeat(np.arange(n), np.diff(self.uptr))
+        off = self.uind != rows
+        U = sp.csr_matrix((self.uval[off], (rows[off], self.uind[off])), shape=(n, n)) + sp.identity(n)
+        return U.T.tocsr(), self.diag()
+
+    def apply(self, r):
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        z = np.empty_like(r)
+        lib().orc_ildlt_apply(self.n, _ptr(self.uptr), _ptr(self.uind), _ptr(self.uval), _ptr(r), _ptr(z))
+        return z
+
+
+def pcg(indptr, indices, vals, b, rtol, maxit, precond):
+    """gmm::cg (src/gmm/gmm_solver_cg.h:84-118) from x = 0 with the stopping
+    test of gmm::iteration (|r| <= rtol |b|, checked before each step).
+    precond(r) -> z.  Returns (x, iterations, |r| / |b|)."""
+    import scipy.sparse as sp
+    n = len(indptr) - 1
+    A = sp.csr_matrix((vals, indices, indptr), shape=(n, n))
+    x = np.zeros(n)
+    bn = np.linalg.norm(b)
+    if bn == 0.0:
+        return x, 0, 0.0
+    r = b - A @ x
+    z = precond(r)
+    rho = z @ r
+    p = z.copy()
+    it = 0
+    while np.linalg.norm(r) > rtol * bn and it < maxit:
+        if it > 0:
+            z = precond(r)
+            rho_new = z @ r
+            p = z + (rho_new / rho) * p
+            rho = rho_new
+        q = A @ p
+        a = rho / (q @ p)
+        x += a * p
+        r -= a * q
+        it += 1
+    return x, it, np.linalg.norm(r) / bn

@@ -29,6 +29,12 @@
  *  - orc_assemble_vector (Neo-Hookean residual): finite differences of the
  *    strain energy; tangent = finite differences of the residual.
  *  - orc_compliance: central finite differences on J.
+ *  - orc_ildlt / orc_ildlt_apply: on a pattern without dropped fill
+ *    (tridiagonal, dense) the factor is the exact LDL^T (numpy Cholesky);
+ *    on any pattern (L D L^T)_ij = A_ij for every (i, j) of A's pattern,
+ *    the defining property of a level-0 incomplete factorization; apply =
+ *    the solve with L D L^T (dense numpy); pivot replacement on a matrix
+ *    with a zero pivot.
  */
 #include "gfx_oracle.h"
 
@@ -1019,4 +1025,101 @@ double orc_compliance(const orc_problem *p, const double *rho, double emin,
   }
   free_tables(&t);
   return Jtot;
+}
+
+/* ======================================================================
+ * Incomplete LDL^T preconditioner (gmm::ildlt_precond)
+ * ====================================================================== */
+
+/* U's slots: the diagonal of row i first, then the columns j > i of A's row
+ * i (gmm_precond_ildlt.h:146-166 builds the same layout in two passes). */
+int64_t orc_ildlt_pattern(int64_t n, const int64_t *ip, const int32_t *ix, int64_t *uptr, int32_t *uind) {
+  int64_t s = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    uptr[i] = s;
+    if (uind) uind[s] = (int32_t)i;
+    ++s;
+    for (int64_t k = ip[i]; k < ip[i + 1]; ++k)
+      if (ix[k] > i) {
+        if (uind) uind[s] = ix[k];
+        ++s;
+      }
+  }
+  uptr[n] = s;
+  return s;
+}
+
+/* Right-looking level-0 LDL^T in U's slots (:168-196): for k = 0, 1, ...
+ * the pivot z = U_kk (replaced by 1 when |z| <= maxp, with maxp starting at
+ * eps |A_00| and raised to min(eps |z|, 1) after each pivot), row k scaled
+ * by 1/z, then for every slot (k, h) of row k the entries (h, j) that row k
+ * also holds receive -= (U_kh z) U_kj, in column order.  No fill: entries
+ * outside U's slots are dropped.  Products and differences are rounded one
+ * at a time (no fused multiply-add), as the reference's x86-64 build does. */
+int64_t orc_ildlt(int64_t n, const int64_t *ip, const int32_t *ix, const double *av,
+                  const int64_t *uptr, const int32_t *uind, double *uval) {
+  if (n <= 0) return 0;
+  const double eps = 2.220446049250313e-16; /* numeric_limits<double>::epsilon() */
+  double a00 = 0.0;
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t s = uptr[i];
+    uval[s++] = 0.0;
+    for (int64_t k = ip[i]; k < ip[i + 1]; ++k) {
+      if (ix[k] == i) uval[uptr[i]] = av[k];
+      else if (ix[k] > i) uval[s++] = av[k];
+    }
+    if (i == 0) a00 = uval[uptr[0]];
+  }
+  if (a00 == 0.0) uval[uptr[0]] = 1.0;
+  double maxp = fabs(a00) * eps;
+  int64_t replaced = 0;
+  for (int64_t k = 0; k < n; ++k) {
+    const int64_t d = uptr[k], e = uptr[k + 1];
+    double z = uval[d];
+    if (fabs(z) <= maxp) {
+      uval[d] = z = 1.0;
+      ++replaced;
+    }
+    const double cap = fabs(z) * eps < 1.0 ? fabs(z) * eps : 1.0;
+    if (cap > maxp) maxp = cap;
+    for (int64_t t = d + 1; t < e; ++t) uval[t] /= z;
+    for (int64_t t = d + 1; t < e; ++t) {
+      volatile double zz = uval[t] * z; /* rounded before use */
+      const int32_t h = uind[t];
+      int64_t g = t; /* row k's slots from column h on */
+      for (int64_t j = uptr[h]; j < uptr[h + 1] && g < e; ++j) {
+        while (g < e && uind[g] < uind[j]) ++g;
+        if (g < e && uind[g] == uind[j]) {
+          volatile double prod = zz * uval[g];
+          uval[j] = uval[j] - prod;
+        }
+      }
+    }
+  }
+  return replaced;
+}
+
+/* z = (L D L^T)^{-1} r with L = U^T unit lower (:200-206): the column-wise
+ * forward substitution of gmm::lower_tri_solve on conjugated(U)
+ * (gmm_tri_solve.h:75-89), the division by D, then the row-wise backward
+ * substitution of gmm::upper_tri_solve (:108-123), unit diagonals. */
+void orc_ildlt_apply(int64_t n, const int64_t *uptr, const int32_t *uind, const double *uval,
+                     const double *r, double *z) {
+  for (int64_t i = 0; i < n; ++i) z[i] = r[i];
+  for (int64_t j = 0; j < n; ++j) {
+    const double xj = z[j];
+    for (int64_t t = uptr[j] + 1; t < uptr[j + 1]; ++t) {
+      volatile double prod = xj * uval[t];
+      z[uind[t]] = z[uind[t]] - prod;
+    }
+  }
+  for (int64_t i = 0; i < n; ++i) z[i] /= uval[uptr[i]];
+  for (int64_t i = n - 1; i >= 0; --i) {
+    double t = z[i];
+    for (int64_t s = uptr[i] + 1; s < uptr[i + 1]; ++s) {
+      volatile double prod = uval[s] * z[uind[s]];
+      t = t - prod;
+    }
+    z[i] = t;
+  }
 }

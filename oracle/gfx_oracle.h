@@ -103,6 +103,20 @@ int orc_assemble_vector(const orc_problem *p, double *V);
 double orc_compliance(const orc_problem *p, const double *rho, double emin,
                       double penal, double *dJ_drho, double *dJ_dX);
 
+/* ---- gmm::ildlt_precond (src/gmm/gmm_precond_ildlt.h:136-196) and its
+ * application gmm::mult(P, v1, v2) (:200-206).  A: n x n CSR with sorted
+ * columns (only its upper triangle is read).  U: CSR, row i = slot uptr[i]
+ * for the diagonal (D_i once factored), then the columns j > i of row i of
+ * A in order (values U_ij / D_i once factored).  orc_ildlt_pattern fills
+ * uptr (n + 1) and uind and returns the slot count; orc_ildlt factors in
+ * place and returns the number of pivots it replaced by 1 (GetFEM's
+ * "pivot k is too small" warning). */
+int64_t orc_ildlt_pattern(int64_t n, const int64_t *ip, const int32_t *ix, int64_t *uptr, int32_t *uind);
+int64_t orc_ildlt(int64_t n, const int64_t *ip, const int32_t *ix, const double *av,
+                  const int64_t *uptr, const int32_t *uind, double *uval);
+void orc_ildlt_apply(int64_t n, const int64_t *uptr, const int32_t *uind, const double *uval,
+                     const double *r, double *z);
+
 #ifdef __cplusplus
 }
 #endif

@@ -145,14 +145,25 @@ def test_csc_export_is_getfem_orientation(ctx):
     assert problems.rel_frob(vals, v) <= TOL_FP64
 
 
-def test_repeated_assembly_is_stable(ctx):
-    case = CASES["p2_elast_n4"]()
+@pytest.mark.parametrize("name", ["p2_elast_n8", "p3_elast_n3"])
+@pytest.mark.parametrize("prec", [ga.PREC_FP64, ga.PREC_FP32])
+def test_repeated_assembly_is_bitwise_identical(ctx, name, prec):
+    """The summation order of every CSR value is fixed (incidence lists
+    sorted at pattern build, passes in list order, each wave's rows private
+    to it): repeated assemblies, and a second plan built on the same inputs,
+    give bit-identical values."""
+    case = CASES[name]()
     plan, keep = gpu_plan(ctx, case)
+    plan.set_form(ga.FORM_FACTORIZED, prec)
     plan.assemble()
     a = plan.csr()[2]
-    plan.assemble()
-    b = plan.csr()[2]
-    assert problems.rel_frob(a, b) <= 1e-14
+    for _ in range(4):
+        plan.assemble()
+        assert np.array_equal(plan.csr()[2].view(np.int64), a.view(np.int64))
+    plan2, keep2 = gpu_plan(ctx, case)
+    plan2.set_form(ga.FORM_FACTORIZED, prec)
+    plan2.assemble()
+    assert np.array_equal(plan2.csr()[2].view(np.int64), a.view(np.int64))
 
 
 @pytest.mark.parametrize("name,n,fem,im,nd", [
