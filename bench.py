@@ -71,6 +71,32 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def stream_rates(nbytes=4 << 30):
+    """Plain-device reference rates on this GPU, measured in the same run:
+    torch's fill_ (a write stream) and copy_ (read + write) of nbytes, median
+    of 10 after 3 warm-ups.  They say what HBM delivers to a simple streaming
+    kernel here, beside the vendor peak."""
+    import torch
+    x = torch.empty(nbytes // 8, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    out = {}
+    for name, fn, moved in (("fill", lambda: x.fill_(1.0), nbytes), ("copy", lambda: y.copy_(x), 2 * nbytes)):
+        for _ in range(3):
+            fn()
+        ts = []
+        for _ in range(10):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            ts.append(e0.elapsed_time(e1))
+        out[name + "_gbs"] = moved / (statistics.median(ts) * 1e-3) / 1e9
+    del x, y
+    torch.cuda.empty_cache()
+    return out
+
+
 def run_gpu(args):
     import torch
 
@@ -246,6 +272,14 @@ def run_gpu(args):
         "forms": forms,
         "pattern": pattern,
     }
+    if not args.pmc_child:
+        sr = stream_rates()
+        r = out["roofline"]
+        r["stream_reference"] = dict(sr, note="torch fill_ / copy_ of 4 GiB on this GPU in this run: what a plain "
+                                              "streaming kernel reaches here (the frac above is against the 8 TB/s "
+                                              "vendor peak)")
+        r["frac_of_fill_rate"] = r["achieved"] / sr["fill_gbs"]
+        log("stream reference", json.dumps(sr))
     if not hexes:
         # The element-batched B^T D B forms on the matrix cores: E = sum_q sign(w_q) F_q^T F_q
         # (F = sqrt(|w_q||J|) S B_q, S S = D), upper 16 x 16 tiles only (nt = n_loc/16 rounded
@@ -544,6 +578,72 @@ def run_c4(args):
     return out, (pts, cvx, mf.basic_dof_table(), mf.nbdof(), lam, mu, rho.cpu().numpy(), u.cpu().numpy())
 
 
+def run_c2solve(args):
+    """The C2 system clamped on the face x = 0 (Dirichlet by simplification on
+    the device CSR), right-hand side K applied to a smooth displacement (so
+    the solution is known), solved by the device CG with GetFEM's default
+    preconditioner for coercive problems (ILDLT, rebuilt at every solve) and
+    with block Jacobi: iterations, times and the error against the known
+    solution."""
+    import torch
+
+    import getfem_amd as ga
+    from getfem_amd import _lib, meshes
+
+    n = args.n or 64
+    pts, cvx = meshes.regular_unit_mesh(n)
+    pts = meshes.jitter(pts, n)
+    lam, mu = meshes.lognormal_lame(cvx.shape[0])
+    torch.cuda.init()
+    dev = torch.device("cuda:0")
+    ctx = ga.Context(0)
+    ctx.set_stream(torch.cuda.current_stream(dev).cuda_stream)
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    mf = ga.MeshFem(mesh, "FEM_PK(3,2)", 3)
+    mim = ga.MeshIm(mesh, "IM_TETRAHEDRON(5)")
+    plan = ga.Plan(mf, mim, ga.TERM_ISO_ELAST, [lam, mu])
+    nnz = plan.build_pattern()
+    plan.assemble()
+    nodes = mf.basic_dof_nodes()  # one row per node; dof = 3 node + component
+    ndof = mf.nbdof()
+    on_face = np.flatnonzero(np.abs(nodes[:, 0]) < 1e-12)
+    fixed = (3 * on_face[:, None] + np.arange(3)[None, :]).ravel().astype(np.int32)
+    x_true = meshes.sine_modes()(nodes, 0.05).ravel()
+    x_true[fixed] = 0.0
+    dxt = torch.tensor(x_true, device=dev)
+    db = torch.empty_like(dxt)
+    plan.dof_constraints(fixed, np.zeros(fixed.size),
+                         flags=_lib.BUILD_MATRIX | _lib.MODEL_LINEAR | _lib.MODEL_SYMMETRIC)
+    plan.spmv(dxt.data_ptr(), db.data_ptr())
+    torch.cuda.synchronize()
+    out = {"metric": METRIC, "unit": "iterations", "n_gpus": 1, "higher_is_better": False,
+           "dtype": "f64", "data": "synthetic (seeded jittered mesh, lognormal lambda/mu)",
+           "config": {"workload": "P2-tet isotropic elasticity %d^3x6 = %d tets, %d dofs, %d stored entries, "
+                                  "clamped on x = 0 (%d dofs), b = K x_true, x_true = sum of 8 sine modes; "
+                                  "device CG to |r| <= %g |b| from x = 0" % (n, cvx.shape[0], ndof, nnz,
+                                                                             fixed.size, args.rtol),
+                      "name": "c2solve"},
+           "solves": {}}
+    t0 = time.perf_counter()
+    nf, nb = plan.ildlt_factor()
+    torch.cuda.synchronize()
+    out["ildlt_levels"] = {"forward": nf, "backward": nb, "units": ndof // 3,
+                           "first_factorization_s": time.perf_counter() - t0}
+    for name, pc in (("ildlt", ga.PRECOND_ILDLT), ("block_jacobi", ga.PRECOND_BLOCK_JACOBI)):
+        dx = torch.zeros_like(dxt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        it, res = plan.solve_pcg(db.data_ptr(), dx.data_ptr(), pc, rtol=args.rtol, maxit=args.maxit)
+        torch.cuda.synchronize()
+        t = time.perf_counter() - t0
+        err = float(torch.linalg.vector_norm(dx - dxt) / torch.linalg.vector_norm(dxt))
+        out["solves"][name] = {"iterations": it, "rel_residual": res, "seconds": t,
+                               "ms_per_iteration": 1e3 * t / max(it, 1), "rel_error_vs_x_true": err}
+        log(name, json.dumps(out["solves"][name]))
+    out["value"] = out["solves"]["ildlt"]["iterations"]
+    return out
+
+
 def cpu_baseline_c4(inputs):
     """The oracle's compliance with its analytic gradients over the whole
     mesh: 1 warm-up + median of 5 (one thread)."""
@@ -562,7 +662,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4"])
+    ap.add_argument("--config", default="c2", choices=sorted(CONFIGS) + ["c4", "c2solve"])
+    ap.add_argument("--rtol", type=float, default=1e-8, help="c2solve: CG tolerance")
+    ap.add_argument("--maxit", type=int, default=20000, help="c2solve: CG iteration cap")
     ap.add_argument("--n", type=int, default=0, help="mesh subdivisions (default: the config's)")
     ap.add_argument("--form", default="all")
     ap.add_argument("--no-cpu", action="store_true")
@@ -584,6 +686,9 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+    if args.config == "c2solve":
+        print(json.dumps(run_c2solve(args)), flush=True)
+        return
     if args.config == "c4":
         out, inputs = run_c4(args)
         if not args.no_cpu:

@@ -1172,10 +1172,10 @@ static int il_prepare(gfx_plan *p) {
     HIPCHK(c, hipStreamSynchronize(s));
     int64_t cap = 0;
     for (int64_t i = 0; i < n; ++i) cap = std::max(cap, ip[(size_t)i + 1] - ip[(size_t)i]);
-    if (20 * cap > 64 * 1024) {
+    if (44 * cap > 64 * 1024) {
       il_free(L);
       return fail(c, GFX_E_UNSUPPORTED, "ILDLT: a row of " + std::to_string(cap) + " entries exceeds the "
-                                        "factorization's LDS (3276 entries)");
+                                        "factorization's LDS (1489 entries)");
     }
     L.cap = (int)cap;
     HIPCHK(c, ildlt_analyse(s, L, n, qb, p->d_indptr, p->d_indices, p->d_err, &herr));

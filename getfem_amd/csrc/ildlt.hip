@@ -24,6 +24,7 @@
 //   backward: z_r = w_r - sum over upper (r, j) in column order of F_rj z_j.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <vector>
 
 #include "gfx_internal.h"
@@ -33,6 +34,7 @@ namespace gfx {
 namespace {
 
 constexpr int kIlThreads = 64;
+constexpr int kIlFactorBytes = 44;  // LDS bytes per row entry of the factorization kernel
 
 // one rounding per product and per difference: no contraction into fma
 __device__ __forceinline__ double mul_rn(double a, double b) {
@@ -124,13 +126,14 @@ __global__ void k_il_frontier(int32_t m, const int32_t *__restrict__ cur, int qb
 }
 
 // Factorization of one level: one wavefront per unit, its rows in order.
-// Row r: acc[t] = a at the t-th entry of r from the diagonal on (LDS); for
-// each lower entry (r, k) in column order, zz = L_rk D_k and the part of row
-// k from column r on is staged in LDS (columns and values), each lane finds
-// its column there and subtracts zz U_kj / D_k.  Then D_r = acc[0], the
-// entries right of the diagonal are divided by D_r and written at (r, j) and
-// at their mirrors (j, r).  Dynamic LDS: 2 x cap doubles + cap ints (cap: the
-// longest row).  err bit 4: a row longer than cap (checked by the host).
+// Row r: acc[t] = a at the t-th entry of r from the diagonal on (LDS).  The
+// lanes first look up, for every lower entry (r, k) at once, zz = L_rk D_k and
+// where row k's part from column r on starts and ends; then, in column order
+// of k, that part of row k is staged in LDS (columns and values) and each lane
+// finds its column there and subtracts zz U_kj / D_k.  Then D_r = acc[0], and
+// the entries right of the diagonal are divided by D_r and written at (r, j)
+// and at their mirrors (j, r).  Dynamic LDS: kIlFactorBytes per entry of the
+// longest row (cap).
 __global__ __launch_bounds__(kIlThreads) void k_il_factor(int32_t m, const int32_t *__restrict__ units, int qb,
                                                          const int64_t *__restrict__ ip,
                                                          const int32_t *__restrict__ ix,
@@ -138,22 +141,29 @@ __global__ __launch_bounds__(kIlThreads) void k_il_factor(int32_t m, const int32
                                                          const int64_t *__restrict__ dpos, int cap,
                                                          double *__restrict__ F) {
   extern __shared__ double il_lds[];
-  double *acc = il_lds, *sval = il_lds + cap;
-  int32_t *scol = reinterpret_cast<int32_t *>(il_lds + 2 * cap);
+  double *acc = il_lds, *sval = il_lds + cap, *szz = il_lds + 2 * cap;
+  int64_t *sb = reinterpret_cast<int64_t *>(il_lds + 3 * cap), *se = sb + cap;
+  int32_t *scol = reinterpret_cast<int32_t *>(se + cap);
   if ((int32_t)blockIdx.x >= m) return;
   const int lane = threadIdx.x;
   const int64_t u = units[blockIdx.x];
   for (int64_t r = qb * u; r < qb * u + qb; ++r) {
-    const int64_t d = dpos[r], e = ip[r + 1];
+    const int64_t d = dpos[r], e = ip[r + 1], s0r = ip[r];
     const int nup = (int)(e - d);  // diagonal + entries right of it
+    const int nl = (int)(d - s0r);  // entries left of the diagonal
     for (int t = lane; t < nup; t += kIlThreads) acc[t] = av[d + t];
-    __syncthreads();
-    for (int64_t q = ip[r]; q < d; ++q) {
-      const int32_t k = ix[q];
-      const double zz = mul_rn(F[q], F[dpos[k]]);  // L_rk D_k
+    for (int t = lane; t < nl; t += kIlThreads) {
+      const int32_t k = ix[s0r + t];
+      szz[t] = mul_rn(F[s0r + t], F[dpos[k]]);  // L_rk D_k
       // row k from column r on: r is in row k (symmetric pattern)
-      const int64_t s0 = lower_bound_col(ix, dpos[k], ip[k + 1], (int32_t)r), s1 = ip[k + 1];
-      const int ns = (int)(s1 - s0);
+      sb[t] = lower_bound_col(ix, dpos[k], ip[k + 1], (int32_t)r);
+      se[t] = ip[k + 1];
+    }
+    __syncthreads();
+    for (int q = 0; q < nl; ++q) {
+      const double zz = szz[q];
+      const int64_t s0 = sb[q];
+      const int ns = (int)(se[q] - s0);
       for (int t = lane; t < ns; t += kIlThreads) {
         scol[t] = ix[s0 + t];
         sval[t] = F[s0 + t];
@@ -186,34 +196,61 @@ __global__ __launch_bounds__(kIlThreads) void k_il_factor(int32_t m, const int32
   }
 }
 
-// forward solve of one level: y_r = b_r - sum_{(r,k), k < r} y_k L_rk, w_r = y_r / D_r
-__global__ void k_il_forward(int32_t m, const int32_t *__restrict__ units, int qb, const int64_t *__restrict__ ip,
-                             const int32_t *__restrict__ ix, const int64_t *__restrict__ dpos,
-                             const double *__restrict__ F, const double *__restrict__ b, double *__restrict__ y,
-                             double *__restrict__ w) {
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  const int64_t u = units[t];
+// Triangular solves of one level: one wavefront per unit, its rows in order
+// (forward) or in reverse (backward).  The lanes form the row's products in
+// parallel (one rounding each) into LDS, then lane 0 subtracts them in column
+// order, so the chain of differences is the reference's; the row's result is
+// in global memory before the next row of the unit reads it.  Dynamic LDS:
+// cap doubles.
+//   forward:  y_r = b_r - sum_{(r,k), k < r} y_k L_rk,  w_r = y_r / D_r
+//   backward: z_r = w_r - sum_{(r,j), j > r} F_rj z_j
+__global__ __launch_bounds__(kIlThreads) void k_il_forward(int32_t m, const int32_t *__restrict__ units, int qb,
+                                                          const int64_t *__restrict__ ip,
+                                                          const int32_t *__restrict__ ix,
+                                                          const int64_t *__restrict__ dpos,
+                                                          const double *__restrict__ F,
+                                                          const double *__restrict__ b, double *__restrict__ y,
+                                                          double *__restrict__ w) {
+  extern __shared__ double il_lds[];
+  if ((int32_t)blockIdx.x >= m) return;
+  const int lane = threadIdx.x;
+  const int64_t u = units[blockIdx.x];
   for (int64_t r = qb * u; r < qb * u + qb; ++r) {
-    double s = b[r];
-    const int64_t d = dpos[r];
-    for (int64_t q = ip[r]; q < d; ++q) s = sub_rn(s, mul_rn(y[ix[q]], F[q]));
-    y[r] = s;
-    w[r] = s / F[d];
+    const int64_t s0 = ip[r], d = dpos[r];
+    const int nl = (int)(d - s0);
+    for (int t = lane; t < nl; t += kIlThreads) il_lds[t] = mul_rn(y[ix[s0 + t]], F[s0 + t]);
+    __syncthreads();
+    if (lane == 0) {
+      double s = b[r];
+      for (int t = 0; t < nl; ++t) s = sub_rn(s, il_lds[t]);
+      y[r] = s;
+      w[r] = s / F[d];
+    }
+    __syncthreads();  // y_r before the unit's next row reads it
   }
 }
 
-// backward solve of one level: z_r = w_r - sum_{(r,j), j > r} F_rj z_j, rows downwards
-__global__ void k_il_backward(int32_t m, const int32_t *__restrict__ units, int qb, const int64_t *__restrict__ ip,
-                              const int32_t *__restrict__ ix, const int64_t *__restrict__ dpos,
-                              const double *__restrict__ F, const double *__restrict__ w, double *__restrict__ z) {
-  const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= m) return;
-  const int64_t u = units[t];
+__global__ __launch_bounds__(kIlThreads) void k_il_backward(int32_t m, const int32_t *__restrict__ units, int qb,
+                                                           const int64_t *__restrict__ ip,
+                                                           const int32_t *__restrict__ ix,
+                                                           const int64_t *__restrict__ dpos,
+                                                           const double *__restrict__ F,
+                                                           const double *__restrict__ w, double *__restrict__ z) {
+  extern __shared__ double il_lds[];
+  if ((int32_t)blockIdx.x >= m) return;
+  const int lane = threadIdx.x;
+  const int64_t u = units[blockIdx.x];
   for (int64_t r = qb * u + qb - 1; r >= qb * u; --r) {
-    double s = w[r];
-    for (int64_t p = dpos[r] + 1; p < ip[r + 1]; ++p) s = sub_rn(s, mul_rn(F[p], z[ix[p]]));
-    z[r] = s;
+    const int64_t d = dpos[r];
+    const int nu = (int)(ip[r + 1] - d - 1);
+    for (int t = lane; t < nu; t += kIlThreads) il_lds[t] = mul_rn(F[d + 1 + t], z[ix[d + 1 + t]]);
+    __syncthreads();
+    if (lane == 0) {
+      double s = w[r];
+      for (int t = 0; t < nu; ++t) s = sub_rn(s, il_lds[t]);
+      z[r] = s;
+    }
+    __syncthreads();
   }
 }
 
@@ -392,7 +429,7 @@ hipError_t ildlt_factor(hipStream_t s, IldltData &L, const int64_t *ip, const in
                         int cap, int *d_err, int *h_err) {
   hipError_t e;
   *h_err = 0;
-  const size_t lds = (size_t)cap * 20;
+  const size_t lds = (size_t)cap * kIlFactorBytes;
   if (lds > 64 * 1024) return hipErrorInvalidValue;
   for (size_t l = 0; l + 1 < L.flev.size(); ++l) {
     const int32_t m = (int32_t)(L.flev[l + 1] - L.flev[l]);
@@ -427,13 +464,17 @@ hipError_t ildlt_factor(hipStream_t s, IldltData &L, const int64_t *ip, const in
 
 hipError_t ildlt_apply(hipStream_t s, const IldltData &L, const int64_t *ip, const int32_t *ix, const double *r,
                        double *z, double *y, double *w) {
+  const size_t lds = 8 * (size_t)std::max(L.cap, 1);
+  if (lds > 64 * 1024) return hipErrorInvalidValue;
   for (size_t l = 0; l + 1 < L.flev.size(); ++l) {
     const int32_t m = (int32_t)(L.flev[l + 1] - L.flev[l]);
-    k_il_forward<<<il_blocks(m, 64), 64, 0, s>>>(m, L.d_flist + L.flev[l], L.qb, ip, ix, L.d_dpos, L.d_F, r, y, w);
+    k_il_forward<<<(unsigned)m, kIlThreads, lds, s>>>(m, L.d_flist + L.flev[l], L.qb, ip, ix, L.d_dpos, L.d_F, r,
+                                                       y, w);
   }
   for (size_t l = 0; l + 1 < L.blev.size(); ++l) {
     const int32_t m = (int32_t)(L.blev[l + 1] - L.blev[l]);
-    k_il_backward<<<il_blocks(m, 64), 64, 0, s>>>(m, L.d_blist + L.blev[l], L.qb, ip, ix, L.d_dpos, L.d_F, w, z);
+    k_il_backward<<<(unsigned)m, kIlThreads, lds, s>>>(m, L.d_blist + L.blev[l], L.qb, ip, ix, L.d_dpos, L.d_F, w,
+                                                        z);
   }
   return hipGetLastError();
 }

@@ -270,7 +270,7 @@ int gfx_plan_solve_cg(gfx_plan *p, const double *d_b, double *d_x, double rtol, 
  * equal the reference's sequential algorithm bit for bit (same operations,
  * same order, no fused multiply-add); rows are scheduled in level sets of
  * node-sized units.  Needs a symmetric pattern with every diagonal entry
- * (GFX_E_UNSUPPORTED otherwise) and rows of at most 3276 entries; a pivot the
+ * (GFX_E_UNSUPPORTED otherwise) and rows of at most 1489 entries; a pivot the
  * reference would replace by 1 ("pivot too small") returns GFX_E_UNSUPPORTED. */
 #define GFX_PRECOND_BLOCK_JACOBI 0
 #define GFX_PRECOND_ILDLT 1

@@ -24,8 +24,9 @@ def constrained(ctx, case, fem, qdim):
     plan = ga.Plan(mf, mim, term, case.coefs())
     plan.build_pattern()
     plan.assemble()
-    nodes = mf.basic_dof_nodes()
-    fixed = np.flatnonzero(np.abs(nodes[:, 0]) < 1e-12).astype(np.int32)
+    nodes = mf.basic_dof_nodes()  # one row per node; dof = qdim node + component
+    on_face = np.flatnonzero(np.abs(nodes[:, 0]) < 1e-12)
+    fixed = (qdim * on_face[:, None] + np.arange(qdim)[None, :]).ravel().astype(np.int32)
     plan.dof_constraints(fixed, np.zeros(fixed.size),
                          flags=_lib.BUILD_MATRIX | _lib.MODEL_LINEAR | _lib.MODEL_SYMMETRIC)
     ip, ix, v = plan.csr()
