@@ -26,14 +26,17 @@ constexpr int kNd = 27, kNl = 81, kNv = 8;
 //   g[q][i] = B_q grad(phi_i)(q);  H_q = sum_i U_i (x) g[q][i];  F_q = I + H_q
 //   S_q = mu (I - C^-1) + lambda/2 ln(I3) C^-1,  A = dS/dE  (Bonet law)
 //   re[(i,a)] = sum_q wJ_q (F_q S_q g[q][i])_a
-// Phase B, over chunks of kQc points:
-//   B0  D_q[(a,c),(b,n)] = wJ_q (delta_ab S[n][c] + sum_km F[a][k] A[k c m n] F[b][m])
-//   B1  T_q[(a,c),(j,b)] = sum_n D_q[(a,c),(b,n)] g[q][j][n]
-//   B2  K[(i,a),(j,b)] += sum_c g[q][i][c] T_q[(a,c),(j,b)]
-// B2 is a register-blocked product: thread (a, i-block of 3, jb-block of 9)
-// keeps 27 sums, and each (q, c) step reads 3 + 9 values for 27 FMAs.
+// Phase B, the tangent K[(i,a),(j,b)] = sum_q g[q][i][c] D_q[(a,c),(b,n)]
+// g[q][j][n] with D_q[(a,c),(b,n)] = wJ_q (delta_ab S[n][c] + F[a][k]
+// A[k c m n] F[b][m]) (the GWFL derivative of the residual):
+//   both forms use the spatial form (h = F^{-T} g, see k_nh_element): per
+//     node pair and point, two outer products of spatial gradients and a dot
+//     product of material ones;
+//   FP64 form: the blocks on and above the diagonal on the vector ALUs,
+//     thread (i-group of 3, j) keeping 27 sums, mirrored when written;
+//   FP32 form: the same sums as three products on the matrix cores in bf16x3
+//     splits (nh_phase_b_spatial_mfma).
 constexpr int kNq = 27;
-constexpr int kQc = 3;  // points per phase-B chunk (FP64 form)
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
@@ -73,150 +76,145 @@ __device__ __forceinline__ double block_max(double v, double *red) {
   return m;
 }
 
-// Phase B of the FP32 form on the matrix cores.  For each a:
-//   K_a[i][(j,b)] = sum_k G[i][k] T_a[k][(j,b)],  k = (q, c),
-//   G[i][k] = g[q][i][c],  T_a[k][(j,b)] = sum_n D_q[(a,c),(b,n)] g[q][j][n],
-// as 16x16 output tiles (i: 2 tiles of 27 rows, (j,b): 6 tiles of 81
-// columns, 3 values of a: 36 tiles; wavefront w owns the (a, column tile)
-// pairs w, w+4, ...).  The k axis is k = 4 q + c with c = 3 a zero column,
-// so a 32-wide k-step covers 8 points and a lane's 8 k-values are 2 points
-// x 4 components.  G and T are split into bf16 triples in registers; T is
-// computed straight into the B fragments (no LDS round trip).  MFMA
-// 16x16x32 bf16 lane layout: A[i][k]: lane i + 16 (k / 8), element k % 8;
-// B[k][j]: lane j + 16 (k / 8); D[i][j]: lane j + 16 (i / 4), element i % 4.
-constexpr int kQm = 8;  // points per k-step
+// Phase B of the FP32 form in the spatial form (see k_nh_element), as
+// products on the matrix cores (bf16x3 splits, FP32 accumulation):
+//   W_alpha = X diag(alpha) X^T,  W_beta = X diag(beta) X^T   (81 x 81, k = q)
+//   K3 = G M^T                                                  (27 x 27, k = (q, c))
+// with X[(i,a)][q] = h_qia, G[i][(q,c)] = g_qic, M[j][(q,c)] = (gamma_q g_qj +
+// pen_q C_q g_qj)_c, and K[(i,a),(j,b)] = W_alpha[(i,a),(j,b)] +
+// W_beta[(i,b),(j,a)] + delta_ab K3[i][j].  The W tiles are node-aligned: a
+// 16-wide tile holds the 15 rows (or columns) of 5 nodes and one zero pad,
+// so the (a, b) swap of the W_beta term stays inside a tile pair and is one
+// round trip through a per-wave 16 x 16 LDS scratch.  K3 is formed first
+// (one 16 x 16 node tile per wavefront) and kept in LDS.  A unit of work is
+// (row tile, half of the column tiles); in each of three rounds the four
+// wavefronts take the two halves of two row tiles, combine them in two
+// shared 15-row bands (whole rows of E), and then stream the 30 rows, which
+// are consecutive in E.
+// LDS: K3 (27 x 28 floats, its own array), and in the caller's buffer a
+// 16 x 17 scratch per wave and two 15-row bands of E (a row tile each).
+constexpr int kNhK3 = 27 * 28, kNhScr = 16 * 17, kNhBand = 15 * kNl;
+static_assert(4 * kNhScr + 2 * kNhBand <= kNq * kNd * 3 * 2, "phase B buffers fit in gq");
 
-__device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, double lam, double mu,
-                                                const double (*Fq)[9], const double (*Sq)[9],
-                                                const double (*Cq)[9], const double *l3, const double *wJ,
-                                                const float4 (*g4)[kNd], double (*Ac)[81], float4 (*Dc)[27],
-                                                double *slab, double *red, const EbArgs &a) {
-  const int w = t / 64, l = t % 64;
-  const int lr = l % 16, lq = 2 * (l / 16);  // the lane's first point within a k-step
-  f32x4 acc[5][2];
+__device__ __forceinline__ void nh_bf16x3_frag(const float (&v)[8], bf16x8 &h, bf16x8 &m, bf16x8 &lo) {
 #pragma unroll
-  for (int r = 0; r < 5; ++r)
-    for (int m = 0; m < 2; ++m) acc[r][m] = f32x4{0.f, 0.f, 0.f, 0.f};
-  for (int q0 = 0; q0 < kNq; q0 += kQm) {
-    const int nq = min(kQm, kNq - q0);
-    // B0a: material tangent A[k c m n] of the step's points
-    for (int x = t; x < nq * 81; x += kEbThreads) {
-      const int qq = x / 81, q = q0 + qq, r = x % 81;
-      const int k = r / 27, c = (r / 9) % 3, m = (r / 3) % 3, n = r % 3;
-      const double *Ci = Cq[q];
-      const double dCi = -0.5 * (Ci[3 * k + m] * Ci[3 * n + c] + Ci[3 * k + n] * Ci[3 * m + c]);
-      Ac[qq][r] = 2.0 * (-mu * dCi + 0.5 * lam * (Ci[3 * m + n] * Ci[3 * k + c] + l3[q] * dCi));
-    }
-    __syncthreads();
-    // B0b: D_q[(a,c),(b,n)] = wJ_q (delta_ab S_nc + sum_km F_ak A[k c m n] F_bm), stored as
-    // float4 over n: Dc[qq][(a c) * 3 + b]
-    for (int x = t; x < nq * 27; x += kEbThreads) {
-      const int qq = x / 27, q = q0 + qq, acb = x % 27, ac = acb / 3, bb = acb % 3;
-      const int aa = ac / 3, c = ac % 3;
-      const double *F = Fq[q], *A = Ac[qq];
-      double d[3];
-      for (int n = 0; n < 3; ++n) {
-        double sv = (aa == bb) ? Sq[q][3 * n + c] : 0.0;
-        for (int k = 0; k < 3; ++k) {
-          const double *Ak = A + 27 * k + 9 * c + n;
-          sv += F[3 * aa + k] * (Ak[0] * F[3 * bb] + Ak[3] * F[3 * bb + 1] + Ak[6] * F[3 * bb + 2]);
-        }
-        d[n] = wJ[q] * sv;
-      }
-      Dc[qq][acb] = make_float4((float)d[0], (float)d[1], (float)d[2], 0.f);
-    }
-    __syncthreads();
-    // the lane's two points (zero beyond the last point)
-    const int qa = q0 + lq, qb = q0 + lq + 1;
-    const bool va = qa < kNq, vb = qb < kNq;
-    // A fragments of the two row tiles: element x = 4 (point) + c
-    bf16x8 ah[2], am[2], al[2];
-#pragma unroll
-    for (int mt = 0; mt < 2; ++mt) {
-      const int i = 16 * mt + lr;
-      const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
-      const float4 ga = (va && i < kNd) ? g4[qa][i] : z, gb = (vb && i < kNd) ? g4[qb][i] : z;
-      const float v[8] = {ga.x, ga.y, ga.z, 0.f, gb.x, gb.y, gb.z, 0.f};
+  for (int x = 0; x < 8; ++x) {
+    __bf16 a_, b_, c_;
+    split3(v[x], a_, b_, c_);
+    h[x] = a_; m[x] = b_; lo[x] = c_;
+  }
+}
+
+__device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const double (*Fq)[9],
+                                                        const double (*coef)[9], const double (*gq)[kNd][3],
+                                                        const float4 (*h4)[kNd], float *K3s, float *buf,
+                                                        double *red, const EbArgs &a) {
+  const int w = t / 64, l = t % 64, lr = l % 16, lg = l / 16;
+  float *scr = buf + w * kNhScr, *bands = buf + 4 * kNhScr;
+  // K3 node tile (w / 2, w % 2), k = (q, c): three 32-wide steps
+  {
+    const int i = 16 * (w / 2) + lr, j = 16 * (w % 2) + lr;
+    f32x4 c3 = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int st = 0; st < 3; ++st) {
+      float va[8], vb[8];
 #pragma unroll
       for (int x = 0; x < 8; ++x) {
-        __bf16 h, m_, lo;
-        split3(v[x], h, m_, lo);
-        ah[mt][x] = h; am[mt][x] = m_; al[mt][x] = lo;
+        const int k = 32 * st + 8 * lg + x, q = k / 3, c = k % 3;
+        const bool kv = k < 3 * kNq;
+        const int qc = kv ? q : 0;
+        const double *gi = gq[qc][i < kNd ? i : 0], *gj = gq[qc][j < kNd ? j : 0];
+        double m = coef[qc][2] * gj[c];
+        const double pen = coef[qc][3];
+        if (pen != 0.0) {  // det F <= 0 at this point: + pen (C g_j)_c, C = F^T F
+          const double *F = Fq[qc];
+          double fg[3];
+          for (int r = 0; r < 3; ++r) fg[r] = F[3 * r] * gj[0] + F[3 * r + 1] * gj[1] + F[3 * r + 2] * gj[2];
+          m += pen * (F[c] * fg[0] + F[3 + c] * fg[1] + F[6 + c] * fg[2]);
+        }
+        va[x] = (kv && i < kNd) ? (float)gi[c] : 0.f;
+        vb[x] = (kv && j < kNd) ? (float)m : 0.f;
       }
+      bf16x8 ah, am, al, bh, bm, bl;
+      nh_bf16x3_frag(va, ah, am, al);
+      nh_bf16x3_frag(vb, bh, bm, bl);
+      c3 = mfma_bf16x3(ah, am, al, bh, bm, bl, c3);
     }
 #pragma unroll
-    for (int r = 0; r < 5; ++r) {
-      const int p = w + 4 * r;  // (a, column tile) pair, uniform across the wavefront
-      if (p < 18) {
-        const int aa = p / 6, jb = 16 * (p % 6) + lr, j = jb / 3, bb = jb % 3;
-        const bool vj = jb < kNl;
-        float v[8];
-#pragma unroll
-        for (int h2 = 0; h2 < 2; ++h2) {
-          const int q = h2 ? qb : qa;
-          const bool ok = (h2 ? vb : va) && vj;
-          const float4 gj = ok ? g4[q][j] : make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-          for (int c = 0; c < 3; ++c) {
-            const float4 d = ok ? Dc[q - q0][(3 * aa + c) * 3 + bb] : make_float4(0.f, 0.f, 0.f, 0.f);
-            v[4 * h2 + c] = d.x * gj.x + d.y * gj.y + d.z * gj.z;
-          }
-          v[4 * h2 + 3] = 0.f;
-        }
-        bf16x8 bh, bm, bl;
-#pragma unroll
-        for (int x = 0; x < 8; ++x) {
-          __bf16 h, m_, lo;
-          split3(v[x], h, m_, lo);
-          bh[x] = h; bm[x] = m_; bl[x] = lo;
-        }
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt) acc[r][mt] = mfma_bf16x3(ah[mt], am[mt], al[mt], bh, bm, bl, acc[r][mt]);
-      }
+    for (int x = 0; x < 4; ++x) {  // D layout: row 4 lg + x, column lr
+      const int ii = 16 * (w / 2) + 4 * lg + x;
+      if (ii < kNd && j < kNd) K3s[ii * 28 + j] = c3[x];
     }
-    __syncthreads();  // the next step overwrites Ac and Dc
+  }
+  __syncthreads();  // gq is the scratch and bands from here on
+  // fragment value of X (row or column rr of node tile nt) at point q
+  auto xval = [&](int nt, int rr, int q) -> float {
+    const int node = 5 * nt + rr / 3;
+    if (rr >= 15 || node >= kNd || q >= kNq) return 0.f;
+    const float4 h = h4[q][node];
+    const int c = rr % 3;
+    return c == 0 ? h.x : (c == 1 ? h.y : h.z);
+  };
+  float *E = reinterpret_cast<float *>(a.E) + e * kNl * kNl;
+  double mx = 0.0;
+  // rounds rd = 0, 1, 2: wave w takes row tile mt = 2 rd + w / 2, column half ch = w % 2
+  for (int rd = 0; rd < 3; ++rd) {
+    const int mt = 2 * rd + w / 2, ch = w % 2;
+    float *band = bands + (w / 2) * kNhBand;
+    float va[8], vb[8];
+    bf16x8 aah, aam, aal, abh, abm, abl;
+#pragma unroll
+    for (int x = 0; x < 8; ++x) {
+      const int q = 8 * lg + x;
+      const float xv = xval(mt, lr, q);
+      const int qc = q < kNq ? q : 0;
+      va[x] = (float)coef[qc][0] * xv;
+      vb[x] = (float)coef[qc][1] * xv;
+    }
+    nh_bf16x3_frag(va, aah, aam, aal);
+    nh_bf16x3_frag(vb, abh, abm, abl);
+    for (int n3 = 0; n3 < 3; ++n3) {
+      const int nt = 3 * ch + n3;
+#pragma unroll
+      for (int x = 0; x < 8; ++x) vb[x] = xval(nt, lr, 8 * lg + x);
+      bf16x8 bh, bm, bl;
+      nh_bf16x3_frag(vb, bh, bm, bl);
+      const f32x4 z = f32x4{0.f, 0.f, 0.f, 0.f};
+      const f32x4 wa = mfma_bf16x3(aah, aam, aal, bh, bm, bl, z);
+      const f32x4 wb = mfma_bf16x3(abh, abm, abl, bh, bm, bl, z);
+      // W_beta through the scratch: (r, c) -> read back at (r with c's
+      // component, c with r's component)
+#pragma unroll
+      for (int x = 0; x < 4; ++x) scr[(4 * lg + x) * 17 + lr] = wb[x];
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+      const int cc = lr, jn = 5 * nt + cc / 3, b = cc % 3;
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        const int rr = 4 * lg + x, in = 5 * mt + rr / 3, ar = rr % 3;
+        if (rr < 15 && cc < 15 && in < kNd && jn < kNd) {
+          float v = wa[x] + scr[(rr - ar + b) * 17 + (cc - b + ar)];
+          if (ar == b) v += K3s[in * 28 + jn];
+          band[rr * kNl + 15 * nt + cc] = v;
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    }
+    __syncthreads();
+    // the two bands: rows 15 (2 rd) .. 15 (2 rd + 2) - 1 of E, contiguous
+    const int r0 = 30 * rd, nrow = min(30, kNl - r0);
+    for (int x = t; x < nrow * kNl; x += kEbThreads) {
+      const float v = bands[x];
+      mx = fmax(mx, fabs((double)v));
+      __builtin_nontemporal_store(v, E + r0 * kNl + x);
+    }
+    __syncthreads();
   }
   // threshold 1e-14 max|K| (all-zero element skipped)
-  double mx = 0.0;
-#pragma unroll
-  for (int r = 0; r < 5; ++r) {
-    const int p = w + 4 * r;
-    if (p < 18) {
-      const int jb = 16 * (p % 6) + lr;
-#pragma unroll
-      for (int mt = 0; mt < 2; ++mt)
-        for (int x = 0; x < 4; ++x) {
-          const int i = 16 * mt + 4 * (l / 16) + x;
-          if (i < kNd && jb < kNl) mx = fmax(mx, fabs((double)acc[r][mt][x]));
-        }
-    }
-  }
   const double mxe = block_max(mx, red);
   if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
-  // E (FP32: the form's values are FP32 already) in three slabs of 27
-  // contiguous rows (row 3 i + a, i in 9 p..9 p+8), staged through LDS and
-  // streamed with consecutive addresses across lanes
-  float *E = reinterpret_cast<float *>(a.E) + e * kNl * kNl;
-  float *slab32 = reinterpret_cast<float *>(slab);
-  for (int ps = 0; ps < 3; ++ps) {
-#pragma unroll
-    for (int r = 0; r < 5; ++r) {
-      const int p = w + 4 * r;
-      if (p < 18) {
-        const int aa = p / 6, jb = 16 * (p % 6) + lr;
-#pragma unroll
-        for (int mt = 0; mt < 2; ++mt)
-          for (int x = 0; x < 4; ++x) {
-            const int i = 16 * mt + 4 * (l / 16) + x;
-            if (i < kNd && jb < kNl && i / 9 == ps) slab32[(3 * (i - 9 * ps) + aa) * kNl + jb] = acc[r][mt][x];
-          }
-      }
-    }
-    __syncthreads();
-    for (int x = t; x < 27 * kNl; x += kEbThreads)
-      __builtin_nontemporal_store(slab32[x], E + (int64_t)ps * 27 * kNl + x);
-    __syncthreads();
-  }
 }
 
 template <typename Real>
@@ -225,17 +223,13 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   __shared__ double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], l3[kNq];
   __shared__ double gq[kNq][kNd][3];
   constexpr bool kLow = !std::is_same<Real, double>::value;
-  __shared__ float4 g4[kLow ? kNq : 1][kNd];  // FP32 form: the gradients as float4 (w = 0)
-  Real (*gr)[kNd][3] = reinterpret_cast<Real (*)[kNd][3]>(gq);  // FP64 form
-  __shared__ Real Dc[kLow ? 1 : kQc][81];
-  __shared__ double Ac32[kLow ? kQm : 1][81];  // FP32 form: a k-step's material tangents
-  __shared__ float4 Dc32[kLow ? kQm : 1][27];
-  // T of a chunk and, at the end, a 27-row slab of E.  The FP32 form reads the
-  // gradients from its own copy in phase B, so its buffer reuses gq.
-  __shared__ double Tbuf_d[kLow ? 1 : kQc * 9 * kNl];
+  __shared__ float4 h4[kLow ? kNq : 1][kNd];  // FP32 form: the spatial gradients as float4 (w = 0)
+  __shared__ float K3s[kLow ? kNhK3 : 1];      // FP32 form: the 27 x 27 node term of phase B
+  // FP64 form: the spatial gradients in phase B.  FP32 form: the per-wave
+  // scratch and bands of phase B, in gq once the node term K3 has read it.
+  __shared__ double Tbuf_d[kLow ? 1 : kNq * kNd * 3];
   double *Tbuf = kLow ? &gq[0][0][0] : Tbuf_d;
-  static_assert(sizeof(gq) >= kQc * 9 * kNl * sizeof(double), "the FP32 form's buffer fits in gq");
-  Real (*Tc)[9][kNl] = reinterpret_cast<Real (*)[9][kNl]>(Tbuf);
+  static_assert(sizeof(gq) >= 27 * kNl * sizeof(float), "the FP32 form's E slab fits in gq");
   __shared__ double red[kEbThreads];
   const int64_t e = blockIdx.x;
   if (e >= a.n_cv || a.nq != kNq) return;
@@ -277,11 +271,6 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     const double *B = Bq[q];
     const double v = B[3 * c] * h[0] + B[3 * c + 1] * h[1] + B[3 * c + 2] * h[2];
     gq[q][i][c] = v;
-    if (kLow) {
-      float *g4f = reinterpret_cast<float *>(&g4[q][i]);
-      g4f[c] = (float)v;
-      if (c == 2) g4f[3] = 0.f;
-    }
   }
   __syncthreads();
   // A3: deformation gradient
@@ -340,93 +329,148 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   __syncthreads();
   if (t < kNl) a.res[(int64_t)e * kNl + t] = red[t] + red[kNl + t] + red[2 * kNl + t];
   if constexpr (!kLow) {
-    // Phase B: thread (a, ib, jbb) for t < 243
-    const bool owner = t < 3 * kNl;
-    const int ar = owner ? t / kNl : 0, rr = owner ? t % kNl : 0;
-    const int ib = rr / 9, jbb = rr % 9;  // rows i = 3 ib + r (r < 3), columns jb = 9 jbb + m (m < 9)
-    // the sums stay FP64 in both forms (FP32 accumulators measured slower here)
-    double K[3][9];
-    for (int r = 0; r < 3; ++r)
-      for (int m = 0; m < 9; ++m) K[r][m] = 0.0;
-    for (int q0 = 0; q0 < kNq; q0 += kQc) {
-      // B0: D_q[(a,c),(b,n)] = wJ_q (delta_ab S_nc + sum_km F_ak A[k c m n] F_bm), with the
-      // material tangent A[k c m n] = dS_kc/dE_mn evaluated in place (no LDS round trip):
-      //   A = 2 (-mu dCi + lambda/2 (Ci_mn Ci_kc + ln(I3) dCi)),
-      //   dCi = dCi_kc/dC_mn = -(Ci_km Ci_nc + Ci_kn Ci_mc) / 2
-      for (int x = t; x < kQc * 81; x += kEbThreads) {
-        const int qq = x / 81, q = q0 + qq, ac = (x % 81) / 9, bn = x % 9;
-        const int aa = ac / 3, c = ac % 3, bb = bn / 3, n = bn % 3;
-        const double *F = Fq[q], *Ci = Cq[q];
-        const double lq = l3[q];
-        double s = (aa == bb) ? Sq[q][3 * n + c] : 0.0;
-        for (int k = 0; k < 3; ++k) {
-          double sk = 0.0;
-          for (int m = 0; m < 3; ++m) {
-            const double dCi = -0.5 * (Ci[3 * k + m] * Ci[3 * n + c] + Ci[3 * k + n] * Ci[3 * m + c]);
-            const double A = 2.0 * (-mu * dCi + 0.5 * lam * (Ci[3 * m + n] * Ci[3 * k + c] + lq * dCi));
-            sk += A * F[3 * bb + m];
-          }
-          s += F[3 * aa + k] * sk;
+    // Phase B in the spatial form.  With h_qi = F_q^{-T} g_qi (spatial
+    // gradients), the push-forward of this law's material tangent is
+    // J c = lambda 1 x 1 + (mu - lambda ln J)(two symmetric identities), and
+    // F^{-1} b F^{-T} = I folds the stress term into mu g_qi . g_qj, so the
+    // (i, j) block of D's contraction is, per point,
+    //   wJ_q [lambda h_qi (x) h_qj + (mu - lambda ln J_q) h_qj (x) h_qi
+    //         + (mu g_qi . g_qj + pen_q (F g_qi) . (F g_qj)) I]
+    // (pen_q = 1e200 where det F <= 0: the derivative of the law's penalty
+    // term 1e200 C added to S there).  Same matrix as the material form
+    // D[(a,c),(b,n)] = wJ (delta_ab S_nc + F_ak A_kcmn F_bm) up to roundoff,
+    // without forming D or D g: 24 FMA per block and point.
+    // B0: F^{-1} (into Cq, free now) and the point's coefficients (into Bq)
+    if (t < kNq) {
+      const double *F = Fq[t];
+      const double f0 = F[4] * F[8] - F[5] * F[7], f1 = F[5] * F[6] - F[3] * F[8], f2 = F[3] * F[7] - F[4] * F[6];
+      const double dF = F[0] * f0 + F[1] * f1 + F[2] * f2;
+      double *Fi = Cq[t];  // row-major F^{-1}
+      Fi[0] = f0 / dF; Fi[3] = f1 / dF; Fi[6] = f2 / dF;
+      Fi[1] = (F[2] * F[7] - F[1] * F[8]) / dF;
+      Fi[4] = (F[0] * F[8] - F[2] * F[6]) / dF;
+      Fi[7] = (F[1] * F[6] - F[0] * F[7]) / dF;
+      Fi[2] = (F[1] * F[5] - F[2] * F[4]) / dF;
+      Fi[5] = (F[2] * F[3] - F[0] * F[5]) / dF;
+      Fi[8] = (F[0] * F[4] - F[1] * F[3]) / dF;
+      const double w = wJ[t];
+      Bq[t][0] = w * lam;
+      Bq[t][1] = w * (mu - 0.5 * lam * l3[t]);
+      Bq[t][2] = w * mu;
+      Bq[t][3] = dF <= 0.0 ? w * 1e200 : 0.0;
+    }
+    __syncthreads();
+    // B1: spatial gradients h[q][i][a] = sum_c (F^{-1})_ca g[q][i][c]
+    double (*hq)[kNd][3] = reinterpret_cast<double (*)[kNd][3]>(Tbuf);
+    static_assert(sizeof(Tbuf_d) >= sizeof(double) * kNq * kNd * 3, "h fits in the T buffer");
+    for (int x = t; x < kNq * kNd * 3; x += kEbThreads) {
+      const int q = x / (kNd * 3), i = (x / 3) % kNd, aa = x % 3;
+      const double *Fi = Cq[q], *g = gq[q][i];
+      hq[q][i][aa] = Fi[aa] * g[0] + Fi[3 + aa] * g[1] + Fi[6 + aa] * g[2];
+    }
+    __syncthreads();
+    // B2: K is symmetric, so only the blocks (i, j) with i <= j are formed,
+    // by 135 threads (i-group ig of 3 nodes, node j >= 3 ig), 27 sums each;
+    // a block below the diagonal that a thread also forms (j inside its own
+    // group) is not written, its mirror is
+    int ig = 0, jn = 0;
+    const bool owner = t < 135;
+    if (owner) {
+      int off = t;
+      while (off >= kNd - 3 * ig) off -= kNd - 3 * ig++;
+      jn = 3 * ig + off;
+    }
+    double K[3][3][3];
+    for (int ii = 0; ii < 3; ++ii)
+      for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y) K[ii][x][y] = 0.0;
+    if (owner) {
+      for (int q = 0; q < kNq; ++q) {
+        const double al = Bq[q][0], be = Bq[q][1], ga = Bq[q][2], pen = Bq[q][3];
+        const double *hj = hq[q][jn], *gj = gq[q][jn];
+        double ha[3], hb[3], m[3];
+        for (int x = 0; x < 3; ++x) {
+          ha[x] = al * hj[x];
+          hb[x] = be * hj[x];
+          m[x] = ga * gj[x];
         }
-        Dc[qq][x % 81] = (Real)(wJ[q] * s);
-      }
-      __syncthreads();
-      // B1: T_q[(a,c),(j,b)] = sum_n D_q[(a,c),(b,n)] g[q][j][n]; thread (a, jb)
-      if (owner) {
-        const int jb = rr, j = jb / 3, bb = jb % 3;
-        for (int qq = 0; qq < kQc; ++qq) {
-          const Real *g = gr[q0 + qq][j];
-          const Real g0 = g[0], g1 = g[1], g2 = g[2];
-          for (int c = 0; c < 3; ++c) {
-            const Real *d = &Dc[qq][9 * (3 * ar + c) + 3 * bb];
-            Tc[qq][3 * ar + c][jb] = d[0] * g0 + d[1] * g1 + d[2] * g2;
-          }
+        if (pen != 0.0) {  // det F <= 0 at this point: + pen C g_j, C = F^T F
+          const double *F = Fq[q];
+          double fg[3];
+          for (int r = 0; r < 3; ++r) fg[r] = F[3 * r] * gj[0] + F[3 * r + 1] * gj[1] + F[3 * r + 2] * gj[2];
+          for (int x = 0; x < 3; ++x) m[x] += pen * (F[x] * fg[0] + F[3 + x] * fg[1] + F[6 + x] * fg[2]);
+        }
+        for (int ii = 0; ii < 3; ++ii) {
+          const double *hi = hq[q][3 * ig + ii], *gi = gq[q][3 * ig + ii];
+          const double hh[3] = {hi[0], hi[1], hi[2]};
+          const double d = gi[0] * m[0] + gi[1] * m[1] + gi[2] * m[2];
+          for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) K[ii][x][y] += hh[x] * ha[y] + hh[y] * hb[x];
+          for (int x = 0; x < 3; ++x) K[ii][x][x] += d;
         }
       }
-      __syncthreads();
-      // B2: K += G^T T over the chunk's (q, c)
-      if (owner) {
-        for (int qq = 0; qq < kQc; ++qq)
-          for (int c = 0; c < 3; ++c) {
-            const double g0 = gr[q0 + qq][3 * ib][c], g1 = gr[q0 + qq][3 * ib + 1][c],
-                         g2 = gr[q0 + qq][3 * ib + 2][c];
-            const Real *tv = &Tc[qq][3 * ar + c][9 * jbb];
-            for (int m = 0; m < 9; ++m) {
-              const double v = tv[m];
-              K[0][m] += g0 * v;
-              K[1][m] += g1 * v;
-              K[2][m] += g2 * v;
-            }
-          }
-      }
-      __syncthreads();
     }
     // threshold 1e-14 max|K| (all-zero element skipped)
     double mx = 0.0;
-    for (int r = 0; r < 3; ++r)
-      for (int m = 0; m < 9; ++m) mx = fmax(mx, fabs((double)K[r][m]));
+    for (int ii = 0; ii < 3; ++ii)
+      for (int x = 0; x < 3; ++x)
+        for (int y = 0; y < 3; ++y) mx = fmax(mx, fabs(K[ii][x][y]));
     const double mxe = block_max(owner ? mx : 0.0, red);
     if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
-    // E is streamed to HBM (GBs, read back once by the gather) in three slabs of
-    // 27 contiguous rows (rows 27p..27p+26 = i in 9p..9p+8, owned by ib in
-    // 3p..3p+2), staged through LDS so that each store instruction writes
-    // consecutive addresses across the lanes
-    double *slab = Tbuf;
-    static_assert(kQc * 9 >= 27, "a 27-row slab fits in the T buffer");
-    double *E = a.E + (int64_t)e * kNl * kNl;
-    for (int p = 0; p < 3; ++p) {
-      if (owner && ib / 3 == p)
-        for (int r = 0; r < 3; ++r) {
-          double *row = slab + (3 * (3 * (ib - 3 * p) + r) + ar) * kNl + 9 * jbb;
-          for (int m = 0; m < 9; ++m) row[m] = K[r][m];
+    // E row-major in three slabs of 27 rows (nodes 9 p .. 9 p + 8) staged in
+    // LDS (the spatial gradients' buffer, free now): block (i, j) at rows
+    // 3 i + x, columns 3 j + y, and for i < j its transpose at rows 3 j + y,
+    // columns 3 i + x; each slab then goes out as one contiguous run
+    double *slab = Tbuf, *E = a.E + (int64_t)e * kNl * kNl;
+    static_assert(sizeof(Tbuf_d) >= sizeof(double) * 27 * kNl, "a 27-row slab fits in the T buffer");
+    for (int ps = 0; ps < 3; ++ps) {
+      __syncthreads();  // h is read until B2 ends; the previous slab is streamed
+      if (owner)
+        for (int ii = 0; ii < 3; ++ii) {
+          const int i = 3 * ig + ii;
+          if (i > jn) continue;
+          const bool dir = i / 9 == ps, mir = i < jn && jn / 9 == ps;
+          for (int x = 0; x < 3; ++x)
+            for (int y = 0; y < 3; ++y) {
+              if (dir) slab[(3 * (i - 9 * ps) + x) * kNl + 3 * jn + y] = K[ii][x][y];
+              if (mir) slab[(3 * (jn - 9 * ps) + y) * kNl + 3 * i + x] = K[ii][x][y];
+            }
         }
       __syncthreads();
       for (int x = t; x < 27 * kNl; x += kEbThreads)
-        __builtin_nontemporal_store(slab[x], E + (int64_t)p * 27 * kNl + x);
-      __syncthreads();
+        __builtin_nontemporal_store(slab[x], E + (int64_t)ps * 27 * kNl + x);
     }
   } else {
-    nh_phase_b_mfma(t, e, lam, mu, Fq, Sq, Cq, l3, wJ, g4, Ac32, Dc32, Tbuf, red, a);
+    // the spatial form's inputs (as in the FP64 form): F^{-1} into Cq, the
+    // point's coefficients into Bq, h = F^{-T} g in FP32 into h4
+    if (t < kNq) {
+      const double *F = Fq[t];
+      const double f0 = F[4] * F[8] - F[5] * F[7], f1 = F[5] * F[6] - F[3] * F[8], f2 = F[3] * F[7] - F[4] * F[6];
+      const double dF = F[0] * f0 + F[1] * f1 + F[2] * f2;
+      double *Fi = Cq[t];
+      Fi[0] = f0 / dF; Fi[3] = f1 / dF; Fi[6] = f2 / dF;
+      Fi[1] = (F[2] * F[7] - F[1] * F[8]) / dF;
+      Fi[4] = (F[0] * F[8] - F[2] * F[6]) / dF;
+      Fi[7] = (F[1] * F[6] - F[0] * F[7]) / dF;
+      Fi[2] = (F[1] * F[5] - F[2] * F[4]) / dF;
+      Fi[5] = (F[2] * F[3] - F[0] * F[5]) / dF;
+      Fi[8] = (F[0] * F[4] - F[1] * F[3]) / dF;
+      const double w = wJ[t];
+      Bq[t][0] = w * lam;
+      Bq[t][1] = w * (mu - 0.5 * lam * l3[t]);
+      Bq[t][2] = w * mu;
+      Bq[t][3] = dF <= 0.0 ? w * 1e200 : 0.0;
+    }
+    __syncthreads();
+    for (int x = t; x < kNq * kNd; x += kEbThreads) {
+      const int q = x / kNd, i = x % kNd;
+      const double *Fi = Cq[q], *g = gq[q][i];
+      h4[q][i] = make_float4((float)(Fi[0] * g[0] + Fi[3] * g[1] + Fi[6] * g[2]),
+                             (float)(Fi[1] * g[0] + Fi[4] * g[1] + Fi[7] * g[2]),
+                             (float)(Fi[2] * g[0] + Fi[5] * g[1] + Fi[8] * g[2]), 0.f);
+    }
+    __syncthreads();
+    nh_phase_b_spatial_mfma(t, e, Fq, Bq, gq, h4, K3s, reinterpret_cast<float *>(Tbuf), red, a);
   }
 }
 
