@@ -34,35 +34,12 @@ constexpr int kNd = 27, kNl = 81, kNv = 8;
 //     product of material ones;
 //   FP64 form: the blocks on and above the diagonal on the vector ALUs,
 //     thread (i-group of 3, j) keeping 27 sums, mirrored when written;
-//   FP32 form: the same sums as three products on the matrix cores in bf16x3
-//     splits (nh_phase_b_spatial_mfma).
+//   FP32 form: the same sums as FP32 products on the matrix cores
+//     (nh_phase_b_spatial_mfma).
 constexpr int kNq = 27;
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
-
-// x = hi + mid + lo with three bf16 numbers (24 of FP32's bits; the rest is
-// the rounding of lo)
-__device__ __forceinline__ void split3(float x, __bf16 &hi, __bf16 &mid, __bf16 &lo) {
-  hi = (__bf16)x;
-  const float r1 = x - (float)hi;
-  mid = (__bf16)r1;
-  lo = (__bf16)(r1 - (float)mid);
-}
-
-// acc += A B in about FP32 accuracy from bf16 MFMAs: the six products of the
-// three-way splits whose orders of magnitude reach 2^-24 of the leading one.
-__device__ __forceinline__ f32x4 mfma_bf16x3(const bf16x8 &ah, const bf16x8 &am, const bf16x8 &al,
-                                             const bf16x8 &bh, const bf16x8 &bm, const bf16x8 &bl,
-                                             f32x4 acc) {
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bm, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bl, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al, bh, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bm, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(am, bh, acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah, bh, acc, 0, 0, 0);
-  return acc;
-}
 
 // maximum over the workgroup's threads: shuffles within each wavefront, then
 // the wavefronts' maxima through LDS (one barrier instead of a tree)
@@ -76,111 +53,103 @@ __device__ __forceinline__ double block_max(double v, double *red) {
   return m;
 }
 
-// Phase B of the FP32 form in the spatial form (see k_nh_element), as
-// products on the matrix cores (bf16x3 splits, FP32 accumulation):
+// Phase B of the FP32 form in the spatial form (see k_nh_element), as FP32
+// products on the matrix cores (v_mfma_f32_16x16x4_f32: exact products, FP32
+// sums, so no operand splitting on the vector ALUs):
 //   W_alpha = X diag(alpha) X^T,  W_beta = X diag(beta) X^T   (81 x 81, k = q)
-//   K3 = G M^T                                                  (27 x 27, k = (q, c))
-// with X[(i,a)][q] = h_qia, G[i][(q,c)] = g_qic, M[j][(q,c)] = (gamma_q g_qj +
+//   K3 = sum_c G_c M_c^T                                        (27 x 27, k = q)
+// with X[(i,a)][q] = h_qia, G_c[i][q] = g_qic, M_c[j][q] = (gamma_q g_qj +
 // pen_q C_q g_qj)_c, and K[(i,a),(j,b)] = W_alpha[(i,a),(j,b)] +
-// W_beta[(i,b),(j,a)] + delta_ab K3[i][j].  The W tiles are node-aligned: a
-// 16-wide tile holds the 15 rows (or columns) of 5 nodes and one zero pad,
-// so the (a, b) swap of the W_beta term stays inside a tile pair and is one
-// round trip through a per-wave 16 x 16 LDS scratch.  K3 is formed first
-// (one 16 x 16 node tile per wavefront) and kept in LDS.  A unit of work is
-// (row tile, half of the column tiles); in each of three rounds the four
-// wavefronts take the two halves of two row tiles, combine them in two
-// shared 15-row bands (whole rows of E), and then stream the 30 rows, which
-// are consecutive in E.
-// LDS: K3 (27 x 28 floats, its own array), and in the caller's buffer a
-// 16 x 17 scratch per wave and two 15-row bands of E (a row tile each).
-constexpr int kNhK3 = 27 * 28, kNhScr = 16 * 17, kNhBand = 15 * kNl;
+// W_beta[(i,b),(j,a)] + delta_ab K3[i][j].  Lane layout of the 16x16x4
+// product: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane
+// j + 16 (i / 4), element i % 4; the k axis is the 27 points and one zero
+// (7 steps of 4).  The W tiles are node-aligned: a 16-wide tile holds the 15
+// rows (or columns) of 5 nodes and one zero pad, which hT (h transposed,
+// [q][16 tile + row in tile], zero pads) stores as laid out, so a fragment
+// value is one LDS read with no index arithmetic; the (a, b) swap of the
+// W_beta term stays inside a tile pair and is one round trip through a
+// per-wave 16 x 16 LDS scratch.  K3 is formed first (one 16 x 16 node tile
+// per wavefront) and kept in LDS.  A unit of work is (row tile, half of the
+// column tiles); in each of three rounds the four wavefronts take the two
+// halves of two row tiles, combine them in two shared 15-row bands (whole
+// rows of E), and then stream the 30 rows, which are consecutive in E.
+// LDS: K3 (27 x 28 floats, its own array), hT (28 x 96 floats), and in the
+// caller's buffer a 16 x 17 scratch per wave and two 15-row bands of E.
+constexpr int kNhK3 = 27 * 28, kNhScr = 16 * 17, kNhBand = 15 * kNl, kNhQ = 28, kNhRows = 96;
 static_assert(4 * kNhScr + 2 * kNhBand <= kNq * kNd * 3 * 2, "phase B buffers fit in gq");
-
-__device__ __forceinline__ void nh_bf16x3_frag(const float (&v)[8], bf16x8 &h, bf16x8 &m, bf16x8 &lo) {
-#pragma unroll
-  for (int x = 0; x < 8; ++x) {
-    __bf16 a_, b_, c_;
-    split3(v[x], a_, b_, c_);
-    h[x] = a_; m[x] = b_; lo[x] = c_;
-  }
-}
 
 __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const double (*Fq)[9],
                                                         const double (*coef)[9], const double (*gq)[kNd][3],
-                                                        const float4 (*h4)[kNd], float *K3s, float *buf,
+                                                        const float (*hT)[kNhRows], float *K3s, float *buf,
                                                         double *red, const EbArgs &a) {
   const int w = t / 64, l = t % 64, lr = l % 16, lg = l / 16;
   float *scr = buf + w * kNhScr, *bands = buf + 4 * kNhScr;
-  // K3 node tile (w / 2, w % 2), k = (q, c): three 32-wide steps
+  // K3 node tile (w / 2, w % 2): for each component c, 7 steps over the points
   {
     const int i = 16 * (w / 2) + lr, j = 16 * (w % 2) + lr;
+    const bool iv = i < kNd, jv = j < kNd;
+    const int ic = iv ? i : 0, jc = jv ? j : 0;
     f32x4 c3 = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int st = 0; st < 3; ++st) {
-      float va[8], vb[8];
 #pragma unroll
-      for (int x = 0; x < 8; ++x) {
-        const int k = 32 * st + 8 * lg + x, q = k / 3, c = k % 3;
-        const bool kv = k < 3 * kNq;
-        const int qc = kv ? q : 0;
-        const double *gi = gq[qc][i < kNd ? i : 0], *gj = gq[qc][j < kNd ? j : 0];
-        double m = coef[qc][2] * gj[c];
-        const double pen = coef[qc][3];
-        if (pen != 0.0) {  // det F <= 0 at this point: + pen (C g_j)_c, C = F^T F
-          const double *F = Fq[qc];
-          double fg[3];
-          for (int r = 0; r < 3; ++r) fg[r] = F[3 * r] * gj[0] + F[3 * r + 1] * gj[1] + F[3 * r + 2] * gj[2];
-          m += pen * (F[c] * fg[0] + F[3 + c] * fg[1] + F[6 + c] * fg[2]);
-        }
-        va[x] = (kv && i < kNd) ? (float)gi[c] : 0.f;
-        vb[x] = (kv && j < kNd) ? (float)m : 0.f;
+    for (int st = 0; st < kNhQ / 4; ++st) {
+      const int q = 4 * st + lg;
+      const bool kv = q < kNq;
+      const int qc = kv ? q : 0;
+      const double *gi = gq[qc][ic], *gj = gq[qc][jc];
+      const double ga = coef[qc][2], pen = coef[qc][3];
+      double m[3] = {ga * gj[0], ga * gj[1], ga * gj[2]};
+      if (pen != 0.0) {  // det F <= 0 at this point: + pen C g_j, C = F^T F
+        const double *F = Fq[qc];
+        double fg[3];
+        for (int r = 0; r < 3; ++r) fg[r] = F[3 * r] * gj[0] + F[3 * r + 1] * gj[1] + F[3 * r + 2] * gj[2];
+        for (int c = 0; c < 3; ++c) m[c] += pen * (F[c] * fg[0] + F[3 + c] * fg[1] + F[6 + c] * fg[2]);
       }
-      bf16x8 ah, am, al, bh, bm, bl;
-      nh_bf16x3_frag(va, ah, am, al);
-      nh_bf16x3_frag(vb, bh, bm, bl);
-      c3 = mfma_bf16x3(ah, am, al, bh, bm, bl, c3);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const float va = (kv && iv) ? (float)gi[c] : 0.f;
+        const float vb = (kv && jv) ? (float)m[c] : 0.f;
+        c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(va, vb, c3, 0, 0, 0);
+      }
     }
 #pragma unroll
     for (int x = 0; x < 4; ++x) {  // D layout: row 4 lg + x, column lr
       const int ii = 16 * (w / 2) + 4 * lg + x;
-      if (ii < kNd && j < kNd) K3s[ii * 28 + j] = c3[x];
+      if (ii < kNd && jv) K3s[ii * 28 + j] = c3[x];
     }
   }
+  // this lane's alpha_q and beta_q at q = 4 st + lg (hT is zero at q = 27)
+  float al[kNhQ / 4], be[kNhQ / 4];
+#pragma unroll
+  for (int st = 0; st < kNhQ / 4; ++st) {
+    const int qc = min(4 * st + lg, kNq - 1);
+    al[st] = (float)coef[qc][0];
+    be[st] = (float)coef[qc][1];
+  }
   __syncthreads();  // gq is the scratch and bands from here on
-  // fragment value of X (row or column rr of node tile nt) at point q
-  auto xval = [&](int nt, int rr, int q) -> float {
-    const int node = 5 * nt + rr / 3;
-    if (rr >= 15 || node >= kNd || q >= kNq) return 0.f;
-    const float4 h = h4[q][node];
-    const int c = rr % 3;
-    return c == 0 ? h.x : (c == 1 ? h.y : h.z);
-  };
+  // lane constants of the band write: row rr = 4 lg + x, column cc = lr of a tile
+  const int cc = lr, cn = cc / 3, b = cc % 3;
   float *E = reinterpret_cast<float *>(a.E) + e * kNl * kNl;
-  double mx = 0.0;
+  float mx = 0.f;
   // rounds rd = 0, 1, 2: wave w takes row tile mt = 2 rd + w / 2, column half ch = w % 2
   for (int rd = 0; rd < 3; ++rd) {
     const int mt = 2 * rd + w / 2, ch = w % 2;
     float *band = bands + (w / 2) * kNhBand;
-    float va[8], vb[8];
-    bf16x8 aah, aam, aal, abh, abm, abl;
+    float xa[kNhQ / 4], xb[kNhQ / 4];
 #pragma unroll
-    for (int x = 0; x < 8; ++x) {
-      const int q = 8 * lg + x;
-      const float xv = xval(mt, lr, q);
-      const int qc = q < kNq ? q : 0;
-      va[x] = (float)coef[qc][0] * xv;
-      vb[x] = (float)coef[qc][1] * xv;
+    for (int st = 0; st < kNhQ / 4; ++st) {
+      const float h = hT[4 * st + lg][16 * mt + lr];
+      xa[st] = al[st] * h;
+      xb[st] = be[st] * h;
     }
-    nh_bf16x3_frag(va, aah, aam, aal);
-    nh_bf16x3_frag(vb, abh, abm, abl);
     for (int n3 = 0; n3 < 3; ++n3) {
       const int nt = 3 * ch + n3;
+      f32x4 wa = f32x4{0.f, 0.f, 0.f, 0.f}, wb = wa;
 #pragma unroll
-      for (int x = 0; x < 8; ++x) vb[x] = xval(nt, lr, 8 * lg + x);
-      bf16x8 bh, bm, bl;
-      nh_bf16x3_frag(vb, bh, bm, bl);
-      const f32x4 z = f32x4{0.f, 0.f, 0.f, 0.f};
-      const f32x4 wa = mfma_bf16x3(aah, aam, aal, bh, bm, bl, z);
-      const f32x4 wb = mfma_bf16x3(abh, abm, abl, bh, bm, bl, z);
+      for (int st = 0; st < kNhQ / 4; ++st) {
+        const float h = hT[4 * st + lg][16 * nt + lr];
+        wa = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[st], h, wa, 0, 0, 0);
+        wb = __builtin_amdgcn_mfma_f32_16x16x4f32(xb[st], h, wb, 0, 0, 0);
+      }
       // W_beta through the scratch: (r, c) -> read back at (r with c's
       // component, c with r's component)
 #pragma unroll
@@ -188,7 +157,7 @@ __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const 
       __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
       __builtin_amdgcn_wave_barrier();
       __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int cc = lr, jn = 5 * nt + cc / 3, b = cc % 3;
+      const int jn = 5 * nt + cn;
 #pragma unroll
       for (int x = 0; x < 4; ++x) {
         const int rr = 4 * lg + x, in = 5 * mt + rr / 3, ar = rr % 3;
@@ -207,13 +176,13 @@ __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const 
     const int r0 = 30 * rd, nrow = min(30, kNl - r0);
     for (int x = t; x < nrow * kNl; x += kEbThreads) {
       const float v = bands[x];
-      mx = fmax(mx, fabs((double)v));
+      mx = fmaxf(mx, fabsf(v));
       __builtin_nontemporal_store(v, E + r0 * kNl + x);
     }
     __syncthreads();
   }
   // threshold 1e-14 max|K| (all-zero element skipped)
-  const double mxe = block_max(mx, red);
+  const double mxe = block_max((double)mx, red);
   if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
 }
 
@@ -223,7 +192,7 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   __shared__ double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], l3[kNq];
   __shared__ double gq[kNq][kNd][3];
   constexpr bool kLow = !std::is_same<Real, double>::value;
-  __shared__ float4 h4[kLow ? kNq : 1][kNd];  // FP32 form: the spatial gradients as float4 (w = 0)
+  __shared__ float hT[kLow ? kNhQ : 1][kNhRows];  // FP32 form: the spatial gradients, tile layout
   __shared__ float K3s[kLow ? kNhK3 : 1];      // FP32 form: the 27 x 27 node term of phase B
   // FP64 form: the spatial gradients in phase B.  FP32 form: the per-wave
   // scratch and bands of phase B, in gq once the node term K3 has read it.
@@ -442,7 +411,7 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     }
   } else {
     // the spatial form's inputs (as in the FP64 form): F^{-1} into Cq, the
-    // point's coefficients into Bq, h = F^{-T} g in FP32 into h4
+    // point's coefficients into Bq, h = F^{-T} g in FP32 into hT
     if (t < kNq) {
       const double *F = Fq[t];
       const double f0 = F[4] * F[8] - F[5] * F[7], f1 = F[5] * F[6] - F[3] * F[8], f2 = F[3] * F[7] - F[4] * F[6];
@@ -462,15 +431,19 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
       Bq[t][3] = dF <= 0.0 ? w * 1e200 : 0.0;
     }
     __syncthreads();
-    for (int x = t; x < kNq * kNd; x += kEbThreads) {
-      const int q = x / kNd, i = x % kNd;
-      const double *Fi = Cq[q], *g = gq[q][i];
-      h4[q][i] = make_float4((float)(Fi[0] * g[0] + Fi[3] * g[1] + Fi[6] * g[2]),
-                             (float)(Fi[1] * g[0] + Fi[4] * g[1] + Fi[7] * g[2]),
-                             (float)(Fi[2] * g[0] + Fi[5] * g[1] + Fi[8] * g[2]), 0.f);
+    // hT[q][16 nt + r]: component r % 3 of h at node 5 nt + r / 3 (r < 15),
+    // zero in the pads (r = 15, node >= 27, q = 27)
+    for (int x = t; x < kNhQ * kNhRows; x += kEbThreads) {
+      const int q = x / kNhRows, nt = (x % kNhRows) / 16, r = x % 16, i = 5 * nt + r / 3, c = r % 3;
+      float v = 0.f;
+      if (q < kNq && r < 15 && i < kNd) {
+        const double *Fi = Cq[q], *g = gq[q][i];
+        v = (float)(Fi[c] * g[0] + Fi[3 + c] * g[1] + Fi[6 + c] * g[2]);
+      }
+      hT[q][x % kNhRows] = v;
     }
     __syncthreads();
-    nh_phase_b_spatial_mfma(t, e, Fq, Bq, gq, h4, K3s, reinterpret_cast<float *>(Tbuf), red, a);
+    nh_phase_b_spatial_mfma(t, e, Fq, Bq, gq, hT, K3s, reinterpret_cast<float *>(Tbuf), red, a);
   }
 }
 
