@@ -55,35 +55,85 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 
 // Phase B of the FP32 form in the spatial form (see k_nh_element), as FP32
 // products on the matrix cores (v_mfma_f32_16x16x4_f32: exact products, FP32
-// sums, so no operand splitting on the vector ALUs):
-//   W_alpha = X diag(alpha) X^T,  W_beta = X diag(beta) X^T   (81 x 81, k = q)
-//   K3 = sum_c G_c M_c^T                                        (27 x 27, k = q)
-// with X[(i,a)][q] = h_qia, G_c[i][q] = g_qic, M_c[j][q] = (gamma_q g_qj +
-// pen_q C_q g_qj)_c, and K[(i,a),(j,b)] = W_alpha[(i,a),(j,b)] +
-// W_beta[(i,b),(j,a)] + delta_ab K3[i][j].  Lane layout of the 16x16x4
-// product: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in lane
-// j + 16 (i / 4), element i % 4; the k axis is the 27 points and one zero
-// (7 steps of 4).  The W tiles are node-aligned: a 16-wide tile holds the 15
-// rows (or columns) of 5 nodes and one zero pad, which hT (h transposed,
-// [q][16 tile + row in tile], zero pads) stores as laid out, so a fragment
-// value is one LDS read with no index arithmetic; the (a, b) swap of the
-// W_beta term stays inside a tile pair and is one round trip through a
-// per-wave 16 x 16 LDS scratch.  K3 is formed first (one 16 x 16 node tile
-// per wavefront) and kept in LDS.  A unit of work is (row tile, half of the
-// column tiles); in each of three rounds the four wavefronts take the two
-// halves of two row tiles, combine them in two shared 15-row bands (whole
-// rows of E), and then stream the 30 rows, which are consecutive in E.
-// LDS: K3 (27 x 28 floats, its own array), hT (28 x 96 floats), and in the
-// caller's buffer a 16 x 17 scratch per wave and two 15-row bands of E.
-constexpr int kNhK3 = 27 * 28, kNhScr = 16 * 17, kNhBand = 15 * kNl, kNhQ = 28, kNhRows = 96;
-static_assert(4 * kNhScr + 2 * kNhBand <= kNq * kNd * 3 * 2, "phase B buffers fit in gq");
+// sums, no operand splitting on the vector ALUs).  With H_c[i][q] = h_qic
+// (node i, point q), the (a, b) component block of K over the node pairs is
+//   K_ab = H_a diag(alpha) H_b^T + H_b diag(beta) H_a^T + delta_ab K3,
+//   K3 = sum_c G_c M_c^T,  G_c[i][q] = g_qic,  M_c[j][q] = (gamma_q g_qj + pen_q C_q g_qj)_c,
+// so both outer-product terms of a block go into one accumulator and no
+// value moves between lanes.  A tile is a 16 x 16 node tile (I, J) in
+// {0, 1}^2 (27 nodes and 5 zero pads) of one component pair; the k axis is
+// the 27 points and one zero (7 steps of 4).  Lane layout of the 16x16x4
+// product: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in
+// lane j + 16 (i / 4), element i % 4.  hC ([q][c][32 nodes], zero pads) holds
+// h as the fragments read it: one LDS read per value, no index arithmetic.
+// K3 is formed first (node tile (w / 2, w % 2) per wavefront) and kept in
+// LDS.  Then two slabs, node rows I = 0 (rows 0-47 of E) and I = 1 (rows
+// 48-80): wavefront w takes column tile J = w % 2 and five or four of the
+// nine component pairs (the sets alternate between the slabs, so every
+// wavefront issues 21 + 126 products), stages its tiles in the slab's rows
+// of E in LDS, and the workgroup streams the slab, which is contiguous in E.
+constexpr int kNhQ = 28, kNhN = 32;
+constexpr int kNhK3 = kNhN * 28;   // K3, rows padded to 32 (rows 27-31 are never read for stored rows)
+constexpr int kNhSlab = 48 * kNl;  // 16 node rows of E in FP32
+static_assert(kNhSlab * sizeof(float) <= kNq * kNd * 3 * sizeof(double), "the E slab fits in gq");
+
+// the component pairs (a, b) of set H: set 0 holds the three diagonal ones
+template <int H>
+struct NhPairs {
+  static constexpr int n = H == 0 ? 5 : 4;
+  static __device__ __forceinline__ constexpr int a(int t) { return H == 0 ? (t < 3 ? t : t - 3) : (t < 2 ? 2 * t : 1 + t / 3); }
+  static __device__ __forceinline__ constexpr int b(int t) { return H == 0 ? (t < 3 ? t : 4 - t) : (t < 2 ? 2 - 2 * t : 2 - (t - 2)); }
+};
+static_assert(NhPairs<0>::a(3) == 0 && NhPairs<0>::b(3) == 1 && NhPairs<0>::a(4) == 1 && NhPairs<0>::b(4) == 0, "");
+static_assert(NhPairs<1>::a(0) == 0 && NhPairs<1>::b(0) == 2 && NhPairs<1>::a(1) == 2 && NhPairs<1>::b(1) == 0, "");
+static_assert(NhPairs<1>::a(2) == 1 && NhPairs<1>::b(2) == 2 && NhPairs<1>::a(3) == 2 && NhPairs<1>::b(3) == 1, "");
+
+// one wavefront's tiles of slab s: node rows 16 s + lane row, node columns 16 J + lane column
+template <int H>
+__device__ __forceinline__ void nh_slab_tiles(int s, int J, int lr, int lg, const float (*hC)[3][kNhN],
+                                              const float (&al)[kNhQ / 4], const float (&be)[kNhQ / 4],
+                                              const float *K3s, float *stage) {
+  typedef NhPairs<H> P;
+  constexpr int ns = kNhQ / 4;
+  float R[3][ns], C[3][ns];
+#pragma unroll
+  for (int st = 0; st < ns; ++st)
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      R[c][st] = hC[4 * st + lg][c][16 * s + lr];
+      C[c][st] = hC[4 * st + lg][c][16 * J + lr];
+    }
+  f32x4 acc[P::n];
+#pragma unroll
+  for (int u = 0; u < P::n; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+  for (int st = 0; st < ns; ++st)
+#pragma unroll
+    for (int u = 0; u < P::n; ++u) {
+      const int ca = P::a(u), cb = P::b(u);
+      acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[st] * R[ca][st], C[cb][st], acc[u], 0, 0, 0);
+      acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(be[st] * R[cb][st], C[ca][st], acc[u], 0, 0, 0);
+    }
+  const int j = 16 * J + lr;
+  if (j >= kNd) return;
+#pragma unroll
+  for (int u = 0; u < P::n; ++u) {
+    const int ca = P::a(u), cb = P::b(u);
+#pragma unroll
+    for (int x = 0; x < 4; ++x) {  // D layout: node row 4 lg + x of the tile, column lr
+      const int il = 4 * lg + x;
+      float v = acc[u][x];
+      if (ca == cb) v += K3s[(16 * s + il) * 28 + j];
+      stage[(3 * il + ca) * kNl + 3 * j + cb] = v;
+    }
+  }
+}
 
 __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const double (*Fq)[9],
                                                         const double (*coef)[9], const double (*gq)[kNd][3],
-                                                        const float (*hT)[kNhRows], float *K3s, float *buf,
+                                                        const float (*hC)[3][kNhN], float *K3s, float *stage,
                                                         double *red, const EbArgs &a) {
   const int w = t / 64, l = t % 64, lr = l % 16, lg = l / 16;
-  float *scr = buf + w * kNhScr, *bands = buf + 4 * kNhScr;
   // K3 node tile (w / 2, w % 2): for each component c, 7 steps over the points
   {
     const int i = 16 * (w / 2) + lr, j = 16 * (w % 2) + lr;
@@ -117,7 +167,7 @@ __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const 
       if (ii < kNd && jv) K3s[ii * 28 + j] = c3[x];
     }
   }
-  // this lane's alpha_q and beta_q at q = 4 st + lg (hT is zero at q = 27)
+  // this lane's alpha_q and beta_q at q = 4 st + lg (hC is zero at q = 27)
   float al[kNhQ / 4], be[kNhQ / 4];
 #pragma unroll
   for (int st = 0; st < kNhQ / 4; ++st) {
@@ -125,59 +175,20 @@ __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const 
     al[st] = (float)coef[qc][0];
     be[st] = (float)coef[qc][1];
   }
-  __syncthreads();  // gq is the scratch and bands from here on
-  // lane constants of the band write: row rr = 4 lg + x, column cc = lr of a tile
-  const int cc = lr, cn = cc / 3, b = cc % 3;
+  __syncthreads();  // gq is the slab buffer from here on; K3 is complete
   float *E = reinterpret_cast<float *>(a.E) + e * kNl * kNl;
   float mx = 0.f;
-  // rounds rd = 0, 1, 2: wave w takes row tile mt = 2 rd + w / 2, column half ch = w % 2
-  for (int rd = 0; rd < 3; ++rd) {
-    const int mt = 2 * rd + w / 2, ch = w % 2;
-    float *band = bands + (w / 2) * kNhBand;
-    float xa[kNhQ / 4], xb[kNhQ / 4];
-#pragma unroll
-    for (int st = 0; st < kNhQ / 4; ++st) {
-      const float h = hT[4 * st + lg][16 * mt + lr];
-      xa[st] = al[st] * h;
-      xb[st] = be[st] * h;
-    }
-    for (int n3 = 0; n3 < 3; ++n3) {
-      const int nt = 3 * ch + n3;
-      f32x4 wa = f32x4{0.f, 0.f, 0.f, 0.f}, wb = wa;
-#pragma unroll
-      for (int st = 0; st < kNhQ / 4; ++st) {
-        const float h = hT[4 * st + lg][16 * nt + lr];
-        wa = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[st], h, wa, 0, 0, 0);
-        wb = __builtin_amdgcn_mfma_f32_16x16x4f32(xb[st], h, wb, 0, 0, 0);
-      }
-      // W_beta through the scratch: (r, c) -> read back at (r with c's
-      // component, c with r's component)
-#pragma unroll
-      for (int x = 0; x < 4; ++x) scr[(4 * lg + x) * 17 + lr] = wb[x];
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-      const int jn = 5 * nt + cn;
-#pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        const int rr = 4 * lg + x, in = 5 * mt + rr / 3, ar = rr % 3;
-        if (rr < 15 && cc < 15 && in < kNd && jn < kNd) {
-          float v = wa[x] + scr[(rr - ar + b) * 17 + (cc - b + ar)];
-          if (ar == b) v += K3s[in * 28 + jn];
-          band[rr * kNl + 15 * nt + cc] = v;
-        }
-      }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-      __builtin_amdgcn_wave_barrier();
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    }
+  for (int s = 0; s < 2; ++s) {
+    if (((w / 2) ^ s) == 0)
+      nh_slab_tiles<0>(s, w % 2, lr, lg, hC, al, be, K3s, stage);
+    else
+      nh_slab_tiles<1>(s, w % 2, lr, lg, hC, al, be, K3s, stage);
     __syncthreads();
-    // the two bands: rows 15 (2 rd) .. 15 (2 rd + 2) - 1 of E, contiguous
-    const int r0 = 30 * rd, nrow = min(30, kNl - r0);
+    const int nrow = s == 0 ? 48 : kNl - 48;
     for (int x = t; x < nrow * kNl; x += kEbThreads) {
-      const float v = bands[x];
+      const float v = stage[x];
       mx = fmaxf(mx, fabsf(v));
-      __builtin_nontemporal_store(v, E + r0 * kNl + x);
+      __builtin_nontemporal_store(v, E + 48 * s * kNl + x);
     }
     __syncthreads();
   }
@@ -192,8 +203,8 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   __shared__ double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], l3[kNq];
   __shared__ double gq[kNq][kNd][3];
   constexpr bool kLow = !std::is_same<Real, double>::value;
-  __shared__ float hT[kLow ? kNhQ : 1][kNhRows];  // FP32 form: the spatial gradients, tile layout
-  __shared__ float K3s[kLow ? kNhK3 : 1];      // FP32 form: the 27 x 27 node term of phase B
+  __shared__ float hC[kLow ? kNhQ : 1][3][kNhN];  // FP32 form: the spatial gradients, fragment layout
+  __shared__ float K3s[kLow ? kNhK3 : 1];      // FP32 form: the node term K3 of phase B
   // FP64 form: the spatial gradients in phase B.  FP32 form: the per-wave
   // scratch and bands of phase B, in gq once the node term K3 has read it.
   __shared__ double Tbuf_d[kLow ? 1 : kNq * kNd * 3];
@@ -411,7 +422,7 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     }
   } else {
     // the spatial form's inputs (as in the FP64 form): F^{-1} into Cq, the
-    // point's coefficients into Bq, h = F^{-T} g in FP32 into hT
+    // point's coefficients into Bq, h = F^{-T} g in FP32 into hC
     if (t < kNq) {
       const double *F = Fq[t];
       const double f0 = F[4] * F[8] - F[5] * F[7], f1 = F[5] * F[6] - F[3] * F[8], f2 = F[3] * F[7] - F[4] * F[6];
@@ -431,19 +442,18 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
       Bq[t][3] = dF <= 0.0 ? w * 1e200 : 0.0;
     }
     __syncthreads();
-    // hT[q][16 nt + r]: component r % 3 of h at node 5 nt + r / 3 (r < 15),
-    // zero in the pads (r = 15, node >= 27, q = 27)
-    for (int x = t; x < kNhQ * kNhRows; x += kEbThreads) {
-      const int q = x / kNhRows, nt = (x % kNhRows) / 16, r = x % 16, i = 5 * nt + r / 3, c = r % 3;
+    // hC[q][c][i] = h_qic (zero for q = 27 and nodes i >= 27)
+    for (int x = t; x < kNhQ * 3 * kNhN; x += kEbThreads) {
+      const int q = x / (3 * kNhN), c = (x / kNhN) % 3, i = x % kNhN;
       float v = 0.f;
-      if (q < kNq && r < 15 && i < kNd) {
+      if (q < kNq && i < kNd) {
         const double *Fi = Cq[q], *g = gq[q][i];
         v = (float)(Fi[c] * g[0] + Fi[3 + c] * g[1] + Fi[6 + c] * g[2]);
       }
-      hT[q][x % kNhRows] = v;
+      hC[q][c][i] = v;
     }
     __syncthreads();
-    nh_phase_b_spatial_mfma(t, e, Fq, Bq, gq, hT, K3s, reinterpret_cast<float *>(Tbuf), red, a);
+    nh_phase_b_spatial_mfma(t, e, Fq, Bq, gq, hC, K3s, reinterpret_cast<float *>(Tbuf), red, a);
   }
 }
 
