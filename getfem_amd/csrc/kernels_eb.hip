@@ -10,6 +10,7 @@
 #include <hip/hip_runtime.h>
 
 #include <type_traits>
+#include <utility>
 
 #include "gfx_internal.h"
 
@@ -61,82 +62,108 @@ __device__ __forceinline__ double block_max(double v, double *red) {
 //   K3 = sum_c G_c M_c^T,  G_c[i][q] = g_qic,  M_c[j][q] = (gamma_q g_qj + pen_q C_q g_qj)_c,
 // so both outer-product terms of a block go into one accumulator and no
 // value moves between lanes.  A tile is a 16 x 16 node tile (I, J) in
-// {0, 1}^2 (27 nodes and 5 zero pads) of one component pair; the k axis is
-// the 27 points and one zero (7 steps of 4).  Lane layout of the 16x16x4
-// product: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j] in
-// lane j + 16 (i / 4), element i % 4.  hC ([q][c][32 nodes], zero pads) holds
-// h as the fragments read it: one LDS read per value, no index arithmetic.
-// K3 is formed first (node tile (w / 2, w % 2) per wavefront) and kept in
-// LDS.  Then two slabs, node rows I = 0 (rows 0-47 of E) and I = 1 (rows
-// 48-80): wavefront w takes column tile J = w % 2 and five or four of the
-// nine component pairs (the sets alternate between the slabs, so every
-// wavefront issues 21 + 126 products), stages its tiles in the slab's rows
-// of E in LDS, and the workgroup streams the slab, which is contiguous in E.
-constexpr int kNhQ = 28, kNhN = 32;
-constexpr int kNhK3 = kNhN * 28;   // K3, rows padded to 32 (rows 27-31 are never read for stored rows)
-constexpr int kNhSlab = 48 * kNl;  // 16 node rows of E in FP32
-static_assert(kNhSlab * sizeof(float) <= kNq * kNd * 3 * sizeof(double), "the E slab fits in gq");
+// {0, 1}^2 (27 nodes and 5 zero pads) of one component pair (a, b); the k
+// axis is the 27 points and one zero (7 steps of 4).  Lane layout of the
+// 16x16x4 product: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j]
+// in lane j + 16 (i / 4), element i % 4.  hC ([q][c][32 nodes], zero pads)
+// holds h as the fragments read it: one LDS read per value.
+// K is symmetric: with P = 2 a + I and Q = 2 b + J only the 21 tiles P <= Q
+// are formed (the others are their transposes), and of K3 the node tiles
+// (0, 0), (0, 1), (1, 1).  Wavefronts 0-2 form one K3 tile and five W tiles,
+// wavefront 3 six W tiles (91, 91, 91, 84 products).  The tiles stay in the
+// accumulators until K3 is complete and phase A's arrays are free; then each
+// is written with K3 added (a = b) into the whole of E staged in LDS, with
+// its transpose, and the workgroup streams E as one contiguous run.
+constexpr int kNhQ = 28, kNhN = 32, kNhK3 = 27 * 28, kNhTiles = 21;
 
-// the component pairs (a, b) of set H: set 0 holds the three diagonal ones
-template <int H>
-struct NhPairs {
-  static constexpr int n = H == 0 ? 5 : 4;
-  static __device__ __forceinline__ constexpr int a(int t) { return H == 0 ? (t < 3 ? t : t - 3) : (t < 2 ? 2 * t : 1 + t / 3); }
-  static __device__ __forceinline__ constexpr int b(int t) { return H == 0 ? (t < 3 ? t : 4 - t) : (t < 2 ? 2 - 2 * t : 2 - (t - 2)); }
+// upper tile n (0..20) in the order (P, Q) = (0,0) (0,1) .. (0,5) (1,1) .. (5,5)
+__host__ __device__ constexpr int nh_tile_p(int n) {
+  int p = 0;
+  while (n >= 6 - p) n -= 6 - p++;
+  return p;
+}
+__host__ __device__ constexpr int nh_tile_q(int n) {
+  int p = 0;
+  while (n >= 6 - p) n -= 6 - p++;
+  return p + n;
+}
+static_assert(nh_tile_p(5) == 0 && nh_tile_q(5) == 5 && nh_tile_p(6) == 1 && nh_tile_q(6) == 1 &&
+              nh_tile_p(20) == 5 && nh_tile_q(20) == 5, "upper tile order");
+
+// the component pair and node tiles of upper tile N, as constants
+template <int N>
+struct NhTile {
+  static constexpr int P = nh_tile_p(N), Q = nh_tile_q(N), a = P / 2, I = P % 2, b = Q / 2, J = Q % 2;
 };
-static_assert(NhPairs<0>::a(3) == 0 && NhPairs<0>::b(3) == 1 && NhPairs<0>::a(4) == 1 && NhPairs<0>::b(4) == 0, "");
-static_assert(NhPairs<1>::a(0) == 0 && NhPairs<1>::b(0) == 2 && NhPairs<1>::a(1) == 2 && NhPairs<1>::b(1) == 0, "");
-static_assert(NhPairs<1>::a(2) == 1 && NhPairs<1>::b(2) == 2 && NhPairs<1>::a(3) == 2 && NhPairs<1>::b(3) == 1, "");
 
-// one wavefront's tiles of slab s: node rows 16 s + lane row, node columns 16 J + lane column
-template <int H>
-__device__ __forceinline__ void nh_slab_tiles(int s, int J, int lr, int lg, const float (*hC)[3][kNhN],
-                                              const float (&al)[kNhQ / 4], const float (&be)[kNhQ / 4],
-                                              const float *K3s, float *stage) {
-  typedef NhPairs<H> P;
-  constexpr int ns = kNhQ / 4;
-  float R[3][ns], C[3][ns];
+// one k-step of tile N: f[c][I] = h at this lane's point, component c, node 16 I + lr
+template <int N>
+__device__ __forceinline__ void nh_w_mma(f32x4 &acc, float al, float be, const float (&f)[3][2]) {
+  typedef NhTile<N> T;
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(al * f[T::a][T::I], f[T::b][T::J], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(be * f[T::b][T::I], f[T::a][T::J], acc, 0, 0, 0);
+}
+template <int T0, int... U>
+__device__ __forceinline__ void nh_w_step(std::integer_sequence<int, U...>, f32x4 (&acc)[6], float al, float be,
+                                          const float (&f)[3][2]) {
+  (nh_w_mma<T0 + U>(acc[U], al, be, f), ...);
+}
+
+// the products of tiles T0 .. T0 + NT - 1 (accumulators acc[0 .. NT - 1])
+template <int T0, int NT>
+__device__ __forceinline__ void nh_w_tiles(int lr, int lg, const float (*hC)[3][kNhN], const float (&al)[kNhQ / 4],
+                                           const float (&be)[kNhQ / 4], f32x4 (&acc)[6]) {
 #pragma unroll
-  for (int st = 0; st < ns; ++st)
+  for (int u = 0; u < 6; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      R[c][st] = hC[4 * st + lg][c][16 * s + lr];
-      C[c][st] = hC[4 * st + lg][c][16 * J + lr];
-    }
-  f32x4 acc[P::n];
+  for (int st = 0; st < kNhQ / 4; ++st) {
+    float f[3][2];
 #pragma unroll
-  for (int u = 0; u < P::n; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int c = 0; c < 3; ++c)
 #pragma unroll
-  for (int st = 0; st < ns; ++st)
-#pragma unroll
-    for (int u = 0; u < P::n; ++u) {
-      const int ca = P::a(u), cb = P::b(u);
-      acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(al[st] * R[ca][st], C[cb][st], acc[u], 0, 0, 0);
-      acc[u] = __builtin_amdgcn_mfma_f32_16x16x4f32(be[st] * R[cb][st], C[ca][st], acc[u], 0, 0, 0);
-    }
-  const int j = 16 * J + lr;
-  if (j >= kNd) return;
-#pragma unroll
-  for (int u = 0; u < P::n; ++u) {
-    const int ca = P::a(u), cb = P::b(u);
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {  // D layout: node row 4 lg + x of the tile, column lr
-      const int il = 4 * lg + x;
-      float v = acc[u][x];
-      if (ca == cb) v += K3s[(16 * s + il) * 28 + j];
-      stage[(3 * il + ca) * kNl + 3 * j + cb] = v;
-    }
+      for (int I = 0; I < 2; ++I) f[c][I] = hC[4 * st + lg][c][16 * I + lr];
+    nh_w_step<T0>(std::make_integer_sequence<int, NT>{}, acc, al[st], be[st], f);
   }
 }
 
+// tile N into E staged in LDS (row-major 81 x 81), K3 added on the diagonal
+// component blocks, the transpose written for P < Q
+template <int N>
+__device__ __forceinline__ void nh_w_store1(int lr, int lg, const f32x4 &acc, const float *K3s, float *stage) {
+  typedef NhTile<N> T;
+  const int j = 16 * T::J + lr;
+  if (j >= kNd) return;
+#pragma unroll
+  for (int x = 0; x < 4; ++x) {  // D layout: node row 4 lg + x of the tile, column lr
+    const int i = 16 * T::I + 4 * lg + x;
+    if (i < kNd) {
+      float v = acc[x];
+      if (T::a == T::b) v += K3s[i * 28 + j];
+      stage[(3 * i + T::a) * kNl + 3 * j + T::b] = v;
+      if (T::P != T::Q) stage[(3 * j + T::b) * kNl + 3 * i + T::a] = v;
+    }
+  }
+}
+template <int T0, int... U>
+__device__ __forceinline__ void nh_w_store_seq(std::integer_sequence<int, U...>, int lr, int lg, const f32x4 (&acc)[6],
+                                               const float *K3s, float *stage) {
+  (nh_w_store1<T0 + U>(lr, lg, acc[U], K3s, stage), ...);
+}
+template <int T0, int NT>
+__device__ __forceinline__ void nh_w_store(int lr, int lg, const f32x4 (&acc)[6], const float *K3s, float *stage) {
+  nh_w_store_seq<T0>(std::make_integer_sequence<int, NT>{}, lr, lg, acc, K3s, stage);
+}
+
+// stage aliases phase A's arrays (gq, coef, Fq): written only after the
+// workgroup barrier that follows the last read of them
 __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const double (*Fq)[9],
                                                         const double (*coef)[9], const double (*gq)[kNd][3],
                                                         const float (*hC)[3][kNhN], float *K3s, float *stage,
                                                         double *red, const EbArgs &a) {
   const int w = t / 64, l = t % 64, lr = l % 16, lg = l / 16;
-  // K3 node tile (w / 2, w % 2): for each component c, 7 steps over the points
-  {
-    const int i = 16 * (w / 2) + lr, j = 16 * (w % 2) + lr;
+  // K3 node tile (w / 2, (w + 1) / 2) for wavefronts 0-2: for each component c, 7 steps over the points
+  if (w < 3) {
+    const int i = 16 * (w / 2) + lr, j = 16 * ((w + 1) / 2) + lr;
     const bool iv = i < kNd, jv = j < kNd;
     const int ic = iv ? i : 0, jc = jv ? j : 0;
     f32x4 c3 = f32x4{0.f, 0.f, 0.f, 0.f};
@@ -147,19 +174,20 @@ __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const 
       const int qc = kv ? q : 0;
       const double *gi = gq[qc][ic], *gj = gq[qc][jc];
       const double ga = coef[qc][2], pen = coef[qc][3];
-      double m[3] = {ga * gj[0], ga * gj[1], ga * gj[2]};
+      const double g0 = gj[0], g1 = gj[1], g2 = gj[2];
+      double m0 = ga * g0, m1 = ga * g1, m2 = ga * g2;
       if (pen != 0.0) {  // det F <= 0 at this point: + pen C g_j, C = F^T F
         const double *F = Fq[qc];
-        double fg[3];
-        for (int r = 0; r < 3; ++r) fg[r] = F[3 * r] * gj[0] + F[3 * r + 1] * gj[1] + F[3 * r + 2] * gj[2];
-        for (int c = 0; c < 3; ++c) m[c] += pen * (F[c] * fg[0] + F[3 + c] * fg[1] + F[6 + c] * fg[2]);
+        const double f0 = F[0] * g0 + F[1] * g1 + F[2] * g2, f1 = F[3] * g0 + F[4] * g1 + F[5] * g2,
+                     f2 = F[6] * g0 + F[7] * g1 + F[8] * g2;
+        m0 += pen * (F[0] * f0 + F[3] * f1 + F[6] * f2);
+        m1 += pen * (F[1] * f0 + F[4] * f1 + F[7] * f2);
+        m2 += pen * (F[2] * f0 + F[5] * f1 + F[8] * f2);
       }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const float va = (kv && iv) ? (float)gi[c] : 0.f;
-        const float vb = (kv && jv) ? (float)m[c] : 0.f;
-        c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(va, vb, c3, 0, 0, 0);
-      }
+      const bool ok_a = kv && iv, ok_b = kv && jv;
+      c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok_a ? (float)gi[0] : 0.f, ok_b ? (float)m0 : 0.f, c3, 0, 0, 0);
+      c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok_a ? (float)gi[1] : 0.f, ok_b ? (float)m1 : 0.f, c3, 0, 0, 0);
+      c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok_a ? (float)gi[2] : 0.f, ok_b ? (float)m2 : 0.f, c3, 0, 0, 0);
     }
 #pragma unroll
     for (int x = 0; x < 4; ++x) {  // D layout: row 4 lg + x, column lr
@@ -175,41 +203,58 @@ __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const 
     al[st] = (float)coef[qc][0];
     be[st] = (float)coef[qc][1];
   }
-  __syncthreads();  // gq is the slab buffer from here on; K3 is complete
+  f32x4 acc[6];
+  if (w == 0) nh_w_tiles<0, 5>(lr, lg, hC, al, be, acc);
+  else if (w == 1) nh_w_tiles<5, 5>(lr, lg, hC, al, be, acc);
+  else if (w == 2) nh_w_tiles<10, 5>(lr, lg, hC, al, be, acc);
+  else nh_w_tiles<15, 6>(lr, lg, hC, al, be, acc);
+  __syncthreads();  // K3 complete, gq / coef / Fq read: the stage from here on
+  if (w == 0) nh_w_store<0, 5>(lr, lg, acc, K3s, stage);
+  else if (w == 1) nh_w_store<5, 5>(lr, lg, acc, K3s, stage);
+  else if (w == 2) nh_w_store<10, 5>(lr, lg, acc, K3s, stage);
+  else nh_w_store<15, 6>(lr, lg, acc, K3s, stage);
+  __syncthreads();
   float *E = reinterpret_cast<float *>(a.E) + e * kNl * kNl;
   float mx = 0.f;
-  for (int s = 0; s < 2; ++s) {
-    if (((w / 2) ^ s) == 0)
-      nh_slab_tiles<0>(s, w % 2, lr, lg, hC, al, be, K3s, stage);
-    else
-      nh_slab_tiles<1>(s, w % 2, lr, lg, hC, al, be, K3s, stage);
-    __syncthreads();
-    const int nrow = s == 0 ? 48 : kNl - 48;
-    for (int x = t; x < nrow * kNl; x += kEbThreads) {
-      const float v = stage[x];
-      mx = fmaxf(mx, fabsf(v));
-      __builtin_nontemporal_store(v, E + 48 * s * kNl + x);
-    }
-    __syncthreads();
+  for (int x = t; x < kNl * kNl; x += kEbThreads) {
+    const float v = stage[x];
+    mx = fmaxf(mx, fabsf(v));
+    __builtin_nontemporal_store(v, E + x);
   }
   // threshold 1e-14 max|K| (all-zero element skipped)
   const double mxe = block_max((double)mx, red);
   if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
 }
 
+// phase A's per-element arrays; in the FP32 form phase B stages E over them
+struct NhPhaseA {
+  double X[kNv][3], Ue[kNd][3];
+  double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], l3[kNq];
+  double gq[kNq][kNd][3];
+};
+template <bool kLow>
+union NhLds {
+  NhPhaseA A;
+  float stage[kLow ? kNl * kNl : 1];
+};
+
 template <typename Real>
 __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
-  __shared__ double X[kNv][3], Ue[kNd][3];
-  __shared__ double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], l3[kNq];
-  __shared__ double gq[kNq][kNd][3];
   constexpr bool kLow = !std::is_same<Real, double>::value;
+  __shared__ NhLds<kLow> L;
+  double(&X)[kNv][3] = L.A.X;
+  double(&Ue)[kNd][3] = L.A.Ue;
+  double(&Bq)[kNq][9] = L.A.Bq;
+  double(&wJ)[kNq] = L.A.wJ;
+  double(&Fq)[kNq][9] = L.A.Fq;
+  double(&Sq)[kNq][9] = L.A.Sq;
+  double(&Cq)[kNq][9] = L.A.Cq;
+  double(&l3)[kNq] = L.A.l3;
+  double(&gq)[kNq][kNd][3] = L.A.gq;
   __shared__ float hC[kLow ? kNhQ : 1][3][kNhN];  // FP32 form: the spatial gradients, fragment layout
   __shared__ float K3s[kLow ? kNhK3 : 1];      // FP32 form: the node term K3 of phase B
-  // FP64 form: the spatial gradients in phase B.  FP32 form: the per-wave
-  // scratch and bands of phase B, in gq once the node term K3 has read it.
-  __shared__ double Tbuf_d[kLow ? 1 : kNq * kNd * 3];
-  double *Tbuf = kLow ? &gq[0][0][0] : Tbuf_d;
-  static_assert(sizeof(gq) >= 27 * kNl * sizeof(float), "the FP32 form's E slab fits in gq");
+  // FP64 form: the spatial gradients and E's slabs in phase B
+  __shared__ double Tbuf[kLow ? 1 : kNq * kNd * 3];
   __shared__ double red[kEbThreads];
   const int64_t e = blockIdx.x;
   if (e >= a.n_cv || a.nq != kNq) return;
@@ -342,7 +387,7 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     __syncthreads();
     // B1: spatial gradients h[q][i][a] = sum_c (F^{-1})_ca g[q][i][c]
     double (*hq)[kNd][3] = reinterpret_cast<double (*)[kNd][3]>(Tbuf);
-    static_assert(sizeof(Tbuf_d) >= sizeof(double) * kNq * kNd * 3, "h fits in the T buffer");
+    static_assert(sizeof(Tbuf) >= sizeof(double) * kNq * kNd * 3, "h fits in the T buffer");
     for (int x = t; x < kNq * kNd * 3; x += kEbThreads) {
       const int q = x / (kNd * 3), i = (x / 3) % kNd, aa = x % 3;
       const double *Fi = Cq[q], *g = gq[q][i];
@@ -402,7 +447,7 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     // 3 i + x, columns 3 j + y, and for i < j its transpose at rows 3 j + y,
     // columns 3 i + x; each slab then goes out as one contiguous run
     double *slab = Tbuf, *E = a.E + (int64_t)e * kNl * kNl;
-    static_assert(sizeof(Tbuf_d) >= sizeof(double) * 27 * kNl, "a 27-row slab fits in the T buffer");
+    static_assert(sizeof(Tbuf) >= sizeof(double) * 27 * kNl, "a 27-row slab fits in the T buffer");
     for (int ps = 0; ps < 3; ++ps) {
       __syncthreads();  // h is read until B2 ends; the previous slab is streamed
       if (owner)
@@ -453,7 +498,7 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
       hC[q][c][i] = v;
     }
     __syncthreads();
-    nh_phase_b_spatial_mfma(t, e, Fq, Bq, gq, hC, K3s, reinterpret_cast<float *>(Tbuf), red, a);
+    nh_phase_b_spatial_mfma(t, e, Fq, Bq, gq, hC, K3s, L.stage, red, a);
   }
 }
 
