@@ -157,7 +157,7 @@ __device__ __forceinline__ void nh_w_store(int lr, int lg, const f32x4 (&acc)[6]
 // stage aliases phase A's arrays (gq, coef, Fq): written only after the
 // workgroup barrier that follows the last read of them
 __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const double (*Fq)[9],
-                                                        const double (*coef)[9], const double (*gq)[kNd][3],
+                                                        const double (*coef)[4], const double (*gq)[kNd][3],
                                                         const float (*hC)[3][kNhN], float *K3s, float *stage,
                                                         double *red, const EbArgs &a) {
   const int w = t / 64, l = t % 64, lr = l % 16, lg = l / 16;
@@ -229,7 +229,7 @@ __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const 
 // phase A's per-element arrays; in the FP32 form phase B stages E over them
 struct NhPhaseA {
   double X[kNv][3], Ue[kNd][3];
-  double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], l3[kNq];
+  double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], cf[kNq][4];
   double gq[kNq][kNd][3];
 };
 template <bool kLow>
@@ -249,7 +249,7 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   double(&Fq)[kNq][9] = L.A.Fq;
   double(&Sq)[kNq][9] = L.A.Sq;
   double(&Cq)[kNq][9] = L.A.Cq;
-  double(&l3)[kNq] = L.A.l3;
+  double(&cf)[kNq][4] = L.A.cf;
   double(&gq)[kNq][kNd][3] = L.A.gq;
   __shared__ float hC[kLow ? kNhQ : 1][3][kNhN];  // FP32 form: the spatial gradients, fragment layout
   __shared__ float K3s[kLow ? kNhK3 : 1];      // FP32 form: the node term K3 of phase B
@@ -278,24 +278,26 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     const double a2 = k[3] * k[7] - k[4] * k[6];
     const double J = k[0] * a0 + k[1] * a1 + k[2] * a2;
     if (J == 0.0) atomicOr(a.err, 1);
+    const double rJ = 1.0 / J;
     double *B = Bq[t];  // row-major B(r,c)
-    B[0] = a0 / J; B[3] = a1 / J; B[6] = a2 / J;
-    B[1] = (k[2] * k[7] - k[1] * k[8]) / J;
-    B[4] = (k[0] * k[8] - k[2] * k[6]) / J;
-    B[7] = (k[1] * k[6] - k[0] * k[7]) / J;
-    B[2] = (k[1] * k[5] - k[2] * k[4]) / J;
-    B[5] = (k[2] * k[3] - k[0] * k[5]) / J;
-    B[8] = (k[0] * k[4] - k[1] * k[3]) / J;
+    B[0] = a0 * rJ; B[3] = a1 * rJ; B[6] = a2 * rJ;
+    B[1] = (k[2] * k[7] - k[1] * k[8]) * rJ;
+    B[4] = (k[0] * k[8] - k[2] * k[6]) * rJ;
+    B[7] = (k[1] * k[6] - k[0] * k[7]) * rJ;
+    B[2] = (k[1] * k[5] - k[2] * k[4]) * rJ;
+    B[5] = (k[2] * k[3] - k[0] * k[5]) * rJ;
+    B[8] = (k[0] * k[4] - k[1] * k[3]) * rJ;
     wJ[t] = fabs(J) * a.qw[t];
   }
   __syncthreads();
   // A2: real gradients
-  for (int x = t; x < kNq * kNd * 3; x += kEbThreads) {
-    const int q = x / (kNd * 3), i = (x / 3) % kNd, c = x % 3;
+  for (int x = t; x < kNq * kNd; x += kEbThreads) {
+    const int q = x / kNd, i = x % kNd;
     const double *h = a.gref + ((int64_t)q * kNd + i) * 3;
+    const double h0 = h[0], h1 = h[1], h2 = h[2];
     const double *B = Bq[q];
-    const double v = B[3 * c] * h[0] + B[3 * c + 1] * h[1] + B[3 * c + 2] * h[2];
-    gq[q][i][c] = v;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) gq[q][i][c] = B[3 * c] * h0 + B[3 * c + 1] * h1 + B[3 * c + 2] * h2;
   }
   __syncthreads();
   // A3: deformation gradient
@@ -316,23 +318,39 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     const double c1 = C[5] * C[6] - C[3] * C[8];
     const double c2 = C[3] * C[7] - C[4] * C[6];
     const double I3 = C[0] * c0 + C[1] * c1 + C[2] * c2;
-    Ci[0] = c0 / I3; Ci[3] = c1 / I3; Ci[6] = c2 / I3;
-    Ci[1] = (C[2] * C[7] - C[1] * C[8]) / I3;
-    Ci[4] = (C[0] * C[8] - C[2] * C[6]) / I3;
-    Ci[7] = (C[1] * C[6] - C[0] * C[7]) / I3;
-    Ci[2] = (C[1] * C[5] - C[2] * C[4]) / I3;
-    Ci[5] = (C[2] * C[3] - C[0] * C[5]) / I3;
-    Ci[8] = (C[0] * C[4] - C[1] * C[3]) / I3;
-    const double detF = F[0] * (F[4] * F[8] - F[5] * F[7]) - F[1] * (F[3] * F[8] - F[5] * F[6]) +
-                        F[2] * (F[3] * F[7] - F[4] * F[6]);
+    const double rI3 = 1.0 / I3;
+    Ci[0] = c0 * rI3; Ci[3] = c1 * rI3; Ci[6] = c2 * rI3;
+    Ci[1] = (C[2] * C[7] - C[1] * C[8]) * rI3;
+    Ci[4] = (C[0] * C[8] - C[2] * C[6]) * rI3;
+    Ci[7] = (C[1] * C[6] - C[0] * C[7]) * rI3;
+    Ci[2] = (C[1] * C[5] - C[2] * C[4]) * rI3;
+    Ci[5] = (C[2] * C[3] - C[0] * C[5]) * rI3;
+    Ci[8] = (C[0] * C[4] - C[1] * C[3]) * rI3;
+    const double f0 = F[4] * F[8] - F[5] * F[7], f1 = F[5] * F[6] - F[3] * F[8], f2 = F[3] * F[7] - F[4] * F[6];
+    const double detF = F[0] * f0 + F[1] * f1 + F[2] * f2;
     const double ln3 = log(I3);
-    l3[t] = ln3;
     for (int i = 0; i < 9; ++i) {
       double sv = mu * ((i % 4 == 0 ? 1.0 : 0.0) - Ci[i]) + 0.5 * lam * ln3 * Ci[i];
       if (detF <= 0.0) sv += 1e200 * C[i];
       Sq[t][i] = sv;
-      Cq[t][i] = Ci[i];
     }
+    // the spatial form's inputs of phase B: F^{-1} (row-major, into Cq) and
+    // the point's coefficients alpha = wJ lambda, beta = wJ (mu - lambda ln J),
+    // gamma = wJ mu and the penalty pen = 1e200 wJ where det F <= 0
+    const double rF = 1.0 / detF;
+    double *Fi = Cq[t];
+    Fi[0] = f0 * rF; Fi[3] = f1 * rF; Fi[6] = f2 * rF;
+    Fi[1] = (F[2] * F[7] - F[1] * F[8]) * rF;
+    Fi[4] = (F[0] * F[8] - F[2] * F[6]) * rF;
+    Fi[7] = (F[1] * F[6] - F[0] * F[7]) * rF;
+    Fi[2] = (F[1] * F[5] - F[2] * F[4]) * rF;
+    Fi[5] = (F[2] * F[3] - F[0] * F[5]) * rF;
+    Fi[8] = (F[0] * F[4] - F[1] * F[3]) * rF;
+    const double w = wJ[t];
+    cf[t][0] = w * lam;
+    cf[t][1] = w * (mu - 0.5 * lam * ln3);
+    cf[t][2] = w * mu;
+    cf[t][3] = detF <= 0.0 ? w * 1e200 : 0.0;
   }
   __syncthreads();
   // A5: the residual re[(i,a)] = sum_q sum_c P_q[a][c] g[q][i][c] with
@@ -365,33 +383,15 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
     // term 1e200 C added to S there).  Same matrix as the material form
     // D[(a,c),(b,n)] = wJ (delta_ab S_nc + F_ak A_kcmn F_bm) up to roundoff,
     // without forming D or D g: 24 FMA per block and point.
-    // B0: F^{-1} (into Cq, free now) and the point's coefficients (into Bq)
-    if (t < kNq) {
-      const double *F = Fq[t];
-      const double f0 = F[4] * F[8] - F[5] * F[7], f1 = F[5] * F[6] - F[3] * F[8], f2 = F[3] * F[7] - F[4] * F[6];
-      const double dF = F[0] * f0 + F[1] * f1 + F[2] * f2;
-      double *Fi = Cq[t];  // row-major F^{-1}
-      Fi[0] = f0 / dF; Fi[3] = f1 / dF; Fi[6] = f2 / dF;
-      Fi[1] = (F[2] * F[7] - F[1] * F[8]) / dF;
-      Fi[4] = (F[0] * F[8] - F[2] * F[6]) / dF;
-      Fi[7] = (F[1] * F[6] - F[0] * F[7]) / dF;
-      Fi[2] = (F[1] * F[5] - F[2] * F[4]) / dF;
-      Fi[5] = (F[2] * F[3] - F[0] * F[5]) / dF;
-      Fi[8] = (F[0] * F[4] - F[1] * F[3]) / dF;
-      const double w = wJ[t];
-      Bq[t][0] = w * lam;
-      Bq[t][1] = w * (mu - 0.5 * lam * l3[t]);
-      Bq[t][2] = w * mu;
-      Bq[t][3] = dF <= 0.0 ? w * 1e200 : 0.0;
-    }
-    __syncthreads();
     // B1: spatial gradients h[q][i][a] = sum_c (F^{-1})_ca g[q][i][c]
     double (*hq)[kNd][3] = reinterpret_cast<double (*)[kNd][3]>(Tbuf);
     static_assert(sizeof(Tbuf) >= sizeof(double) * kNq * kNd * 3, "h fits in the T buffer");
-    for (int x = t; x < kNq * kNd * 3; x += kEbThreads) {
-      const int q = x / (kNd * 3), i = (x / 3) % kNd, aa = x % 3;
+    for (int x = t; x < kNq * kNd; x += kEbThreads) {
+      const int q = x / kNd, i = x % kNd;
       const double *Fi = Cq[q], *g = gq[q][i];
-      hq[q][i][aa] = Fi[aa] * g[0] + Fi[3 + aa] * g[1] + Fi[6 + aa] * g[2];
+      const double g0 = g[0], g1 = g[1], g2 = g[2];
+#pragma unroll
+      for (int aa = 0; aa < 3; ++aa) hq[q][i][aa] = Fi[aa] * g0 + Fi[3 + aa] * g1 + Fi[6 + aa] * g2;
     }
     __syncthreads();
     // B2: K is symmetric, so only the blocks (i, j) with i <= j are formed,
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
         for (int y = 0; y < 3; ++y) K[ii][x][y] = 0.0;
     if (owner) {
       for (int q = 0; q < kNq; ++q) {
-        const double al = Bq[q][0], be = Bq[q][1], ga = Bq[q][2], pen = Bq[q][3];
+        const double al = cf[q][0], be = cf[q][1], ga = cf[q][2], pen = cf[q][3];
         const double *hj = hq[q][jn], *gj = gq[q][jn];
         double ha[3], hb[3], m[3];
         for (int x = 0; x < 3; ++x) {
@@ -466,39 +466,21 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
         __builtin_nontemporal_store(slab[x], E + (int64_t)ps * 27 * kNl + x);
     }
   } else {
-    // the spatial form's inputs (as in the FP64 form): F^{-1} into Cq, the
-    // point's coefficients into Bq, h = F^{-T} g in FP32 into hC
-    if (t < kNq) {
-      const double *F = Fq[t];
-      const double f0 = F[4] * F[8] - F[5] * F[7], f1 = F[5] * F[6] - F[3] * F[8], f2 = F[3] * F[7] - F[4] * F[6];
-      const double dF = F[0] * f0 + F[1] * f1 + F[2] * f2;
-      double *Fi = Cq[t];
-      Fi[0] = f0 / dF; Fi[3] = f1 / dF; Fi[6] = f2 / dF;
-      Fi[1] = (F[2] * F[7] - F[1] * F[8]) / dF;
-      Fi[4] = (F[0] * F[8] - F[2] * F[6]) / dF;
-      Fi[7] = (F[1] * F[6] - F[0] * F[7]) / dF;
-      Fi[2] = (F[1] * F[5] - F[2] * F[4]) / dF;
-      Fi[5] = (F[2] * F[3] - F[0] * F[5]) / dF;
-      Fi[8] = (F[0] * F[4] - F[1] * F[3]) / dF;
-      const double w = wJ[t];
-      Bq[t][0] = w * lam;
-      Bq[t][1] = w * (mu - 0.5 * lam * l3[t]);
-      Bq[t][2] = w * mu;
-      Bq[t][3] = dF <= 0.0 ? w * 1e200 : 0.0;
-    }
-    __syncthreads();
     // hC[q][c][i] = h_qic (zero for q = 27 and nodes i >= 27)
-    for (int x = t; x < kNhQ * 3 * kNhN; x += kEbThreads) {
-      const int q = x / (3 * kNhN), c = (x / kNhN) % 3, i = x % kNhN;
-      float v = 0.f;
+    for (int x = t; x < kNhQ * kNhN; x += kEbThreads) {
+      const int q = x / kNhN, i = x % kNhN;
+      float v[3] = {0.f, 0.f, 0.f};
       if (q < kNq && i < kNd) {
         const double *Fi = Cq[q], *g = gq[q][i];
-        v = (float)(Fi[c] * g[0] + Fi[3 + c] * g[1] + Fi[6 + c] * g[2]);
+        const double g0 = g[0], g1 = g[1], g2 = g[2];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[c] = (float)(Fi[c] * g0 + Fi[3 + c] * g1 + Fi[6 + c] * g2);
       }
-      hC[q][c][i] = v;
+#pragma unroll
+      for (int c = 0; c < 3; ++c) hC[q][c][i] = v[c];
     }
     __syncthreads();
-    nh_phase_b_spatial_mfma(t, e, Fq, Bq, gq, hC, K3s, L.stage, red, a);
+    nh_phase_b_spatial_mfma(t, e, Fq, cf, gq, hC, K3s, L.stage, red, a);
   }
 }
 
