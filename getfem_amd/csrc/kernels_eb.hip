@@ -30,13 +30,10 @@ constexpr int kNd = 27, kNl = 81, kNv = 8;
 // Phase B, the tangent K[(i,a),(j,b)] = sum_q g[q][i][c] D_q[(a,c),(b,n)]
 // g[q][j][n] with D_q[(a,c),(b,n)] = wJ_q (delta_ab S[n][c] + F[a][k]
 // A[k c m n] F[b][m]) (the GWFL derivative of the residual):
-//   both forms use the spatial form (h = F^{-T} g, see k_nh_element): per
-//     node pair and point, two outer products of spatial gradients and a dot
-//     product of material ones;
-//   FP64 form: the blocks on and above the diagonal on the vector ALUs,
-//     thread (i-group of 3, j) keeping 27 sums, mirrored when written;
-//   FP32 form: the same sums as FP32 products on the matrix cores
-//     (nh_phase_b_spatial_mfma).
+//   in the spatial form (h = F^{-T} g, see k_nh_element): per node pair and
+//   point, two outer products of spatial gradients and a dot product of
+//   material ones, as products on the matrix cores in FP64 (FP64 form) or
+//   FP32 (FP32 form), upper tiles only (nh_phase_b_mfma).
 constexpr int kNq = 27;
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
@@ -54,119 +51,151 @@ __device__ __forceinline__ double block_max(double v, double *red) {
   return m;
 }
 
-// Phase B of the FP32 form in the spatial form (see k_nh_element), as FP32
-// products on the matrix cores (v_mfma_f32_16x16x4_f32: exact products, FP32
-// sums, no operand splitting on the vector ALUs).  With H_c[i][q] = h_qic
-// (node i, point q), the (a, b) component block of K over the node pairs is
+// Lane layout of the 16x16x4 matrix-core products (v_mfma_f64_16x16x4_f64,
+// v_mfma_f32_16x16x4_f32): A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k
+// (both instructions); D[i][j] in lane j + 16 (i / 4), element i % 4 for the
+// FP32 one, but in lane j + 16 (i % 4), element i / 4 for the FP64 one
+// (BdbMfma<T>::row gives the row of lane group g and element r).
+template <typename T>
+struct BdbMfma;
+template <>
+struct BdbMfma<double> {
+  typedef __attribute__((__vector_size__(4 * sizeof(double)))) double v4;
+  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * r + g; }
+};
+template <>
+struct BdbMfma<float> {
+  typedef f32x4 v4;
+  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
+};
+
+// Phase B in the spatial form (see k_nh_element) on the matrix cores, in T
+// (FP64: v_mfma_f64_16x16x4_f64; FP32: v_mfma_f32_16x16x4_f32, exact
+// products and FP32 sums).  With H_c[i][q] = h_qic (node i, point q), the
+// (a, b) component block of K over the node pairs is
 //   K_ab = H_a diag(alpha) H_b^T + H_b diag(beta) H_a^T + delta_ab K3,
 //   K3 = sum_c G_c M_c^T,  G_c[i][q] = g_qic,  M_c[j][q] = (gamma_q g_qj + pen_q C_q g_qj)_c,
 // so both outer-product terms of a block go into one accumulator and no
 // value moves between lanes.  A tile is a 16 x 16 node tile (I, J) in
 // {0, 1}^2 (27 nodes and 5 zero pads) of one component pair (a, b); the k
-// axis is the 27 points and one zero (7 steps of 4).  Lane layout of the
-// 16x16x4 product: A[i][k] in lane i + 16 k, B[k][j] in lane j + 16 k, D[i][j]
-// in lane j + 16 (i / 4), element i % 4.  hC ([q][c][32 nodes], zero pads)
-// holds h as the fragments read it: one LDS read per value.
+// axis is the 27 points and one zero (7 steps of 4).  hC ([q][c][32 nodes],
+// zero pads) holds h as the fragments read it: one LDS read per value.
 // K is symmetric: with P = 2 a + I and Q = 2 b + J only the 21 tiles P <= Q
 // are formed (the others are their transposes), and of K3 the node tiles
-// (0, 0), (0, 1), (1, 1).  Wavefronts 0-2 form one K3 tile and five W tiles,
-// wavefront 3 six W tiles (91, 91, 91, 84 products).  The tiles stay in the
-// accumulators until K3 is complete and phase A's arrays are free; then each
-// is written with K3 added (a = b) into the whole of E staged in LDS, with
-// its transpose, and the workgroup streams E as one contiguous run.
-constexpr int kNhQ = 28, kNhN = 32, kNhK3 = 27 * 28, kNhTiles = 21;
+// (0, 0), (0, 1), (1, 1).  Wavefront w < 3 forms K3 tile w and keeps it in
+// its accumulators beside the three tiles (a, a) of the same node tiles, to
+// which it is added, and two of the twelve tiles with a < b; wavefront 3
+// forms the other six (91, 91, 91, 84 products).  The tiles stay in the
+// accumulators until phase A's arrays and hC are free; then E is staged in
+// LDS over them (FP32: whole; FP64: node rows 0-15, then 16-26), each tile
+// written with its transpose, and each staged run, contiguous in E, is
+// streamed by the workgroup.
+constexpr int kNhQ = 28, kNhN = 32, kNhTiles = 21;
 
-// upper tile n (0..20) in the order (P, Q) = (0,0) (0,1) .. (0,5) (1,1) .. (5,5)
-__host__ __device__ constexpr int nh_tile_p(int n) {
-  int p = 0;
-  while (n >= 6 - p) n -= 6 - p++;
-  return p;
-}
-__host__ __device__ constexpr int nh_tile_q(int n) {
-  int p = 0;
-  while (n >= 6 - p) n -= 6 - p++;
-  return p + n;
-}
-static_assert(nh_tile_p(5) == 0 && nh_tile_q(5) == 5 && nh_tile_p(6) == 1 && nh_tile_q(6) == 1 &&
-              nh_tile_p(20) == 5 && nh_tile_q(20) == 5, "upper tile order");
-
-// the component pair and node tiles of upper tile N, as constants
+// tile N (0..20): wavefront N / 5 (N < 15) takes N % 5 = 0..2 as the (a, a)
+// tiles of its K3 node tile and 3..4 as with-a<b tiles 2 (N / 5) + (N % 5 - 3);
+// N >= 15 are the with-a<b tiles 6..11 of wavefront 3.  With-a<b tile n:
+// (a, b) = (0,1), (0,2), (1,2) for n / 4 = 0, 1, 2, (I, J) = (n % 4 / 2, n % 2).
 template <int N>
 struct NhTile {
-  static constexpr int P = nh_tile_p(N), Q = nh_tile_q(N), a = P / 2, I = P % 2, b = Q / 2, J = Q % 2;
+  static constexpr bool diag = N < 15 && N % 5 < 3;  // an (a, a) tile: K3 added
+  static constexpr int w = N < 15 ? N / 5 : 3;
+  static constexpr int n = N < 15 ? 2 * w + (N % 5 - 3) : 6 + (N - 15);  // with-a<b index (when !diag)
+  static constexpr int a = diag ? N % 5 : (n / 4 == 2 ? 1 : 0);
+  static constexpr int b = diag ? N % 5 : (n / 4 == 0 ? 1 : 2);
+  static constexpr int I = diag ? w / 2 : n % 4 / 2;
+  static constexpr int J = diag ? (w + 1) / 2 : n % 2;
+  static constexpr int P = 2 * a + I, Q = 2 * b + J;
+  static_assert(P <= Q, "upper tiles only");
 };
+static_assert(NhTile<0>::P == 0 && NhTile<0>::Q == 0 && NhTile<5>::P == 0 && NhTile<5>::Q == 1 &&
+              NhTile<14>::a == 0 && NhTile<14>::b == 2 && NhTile<14>::I == 0 && NhTile<14>::J == 1 &&
+              NhTile<20>::a == 1 && NhTile<20>::b == 2 && NhTile<20>::I == 1 && NhTile<20>::J == 1,
+              "tile table");
 
 // one k-step of tile N: f[c][I] = h at this lane's point, component c, node 16 I + lr
-template <int N>
-__device__ __forceinline__ void nh_w_mma(f32x4 &acc, float al, float be, const float (&f)[3][2]) {
-  typedef NhTile<N> T;
-  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(al * f[T::a][T::I], f[T::b][T::J], acc, 0, 0, 0);
-  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(be * f[T::b][T::I], f[T::a][T::J], acc, 0, 0, 0);
+template <typename T, int N>
+__device__ __forceinline__ void nh_w_mma(typename BdbMfma<T>::v4 &acc, T al, T be, const T (&f)[3][2]) {
+  typedef NhTile<N> D;
+  acc = BdbMfma<T>::mma(al * f[D::a][D::I], f[D::b][D::J], acc);
+  acc = BdbMfma<T>::mma(be * f[D::b][D::I], f[D::a][D::J], acc);
 }
-template <int T0, int... U>
-__device__ __forceinline__ void nh_w_step(std::integer_sequence<int, U...>, f32x4 (&acc)[6], float al, float be,
-                                          const float (&f)[3][2]) {
-  (nh_w_mma<T0 + U>(acc[U], al, be, f), ...);
+template <typename T, int T0, int... U>
+__device__ __forceinline__ void nh_w_step(std::integer_sequence<int, U...>, typename BdbMfma<T>::v4 (&acc)[6], T al,
+                                          T be, const T (&f)[3][2]) {
+  (nh_w_mma<T, T0 + U>(acc[U], al, be, f), ...);
 }
 
 // the products of tiles T0 .. T0 + NT - 1 (accumulators acc[0 .. NT - 1])
-template <int T0, int NT>
-__device__ __forceinline__ void nh_w_tiles(int lr, int lg, const float (*hC)[3][kNhN], const float (&al)[kNhQ / 4],
-                                           const float (&be)[kNhQ / 4], f32x4 (&acc)[6]) {
+template <typename T, int T0, int NT>
+__device__ __forceinline__ void nh_w_tiles(int lr, int lg, const T (*hC)[3][kNhN], const T (&al)[kNhQ / 4],
+                                           const T (&be)[kNhQ / 4], typename BdbMfma<T>::v4 (&acc)[6]) {
 #pragma unroll
-  for (int u = 0; u < 6; ++u) acc[u] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int u = 0; u < 6; ++u) acc[u] = typename BdbMfma<T>::v4{T(0), T(0), T(0), T(0)};
 #pragma unroll
   for (int st = 0; st < kNhQ / 4; ++st) {
-    float f[3][2];
+    T f[3][2];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
 #pragma unroll
       for (int I = 0; I < 2; ++I) f[c][I] = hC[4 * st + lg][c][16 * I + lr];
-    nh_w_step<T0>(std::make_integer_sequence<int, NT>{}, acc, al[st], be[st], f);
+    nh_w_step<T, T0>(std::make_integer_sequence<int, NT>{}, acc, al[st], be[st], f);
   }
 }
 
-// tile N into E staged in LDS (row-major 81 x 81), K3 added on the diagonal
-// component blocks, the transpose written for P < Q
-template <int N>
-__device__ __forceinline__ void nh_w_store1(int lr, int lg, const f32x4 &acc, const float *K3s, float *stage) {
-  typedef NhTile<N> T;
-  const int j = 16 * T::J + lr;
+// tile N's entries in node rows [n0, n1) of E into the stage (row-major,
+// row 3 (i - n0) + component), K3 added to (a, a) tiles, the transpose
+// written for P < Q
+template <typename T, int N>
+__device__ __forceinline__ void nh_w_store1(int lr, int lg, int n0, int n1, const typename BdbMfma<T>::v4 &acc,
+                                            const typename BdbMfma<T>::v4 &k3, T *stage) {
+  typedef NhTile<N> D;
+  const int j = 16 * D::J + lr;
   if (j >= kNd) return;
+  const bool jin = j >= n0 && j < n1;
 #pragma unroll
-  for (int x = 0; x < 4; ++x) {  // D layout: node row 4 lg + x of the tile, column lr
-    const int i = 16 * T::I + 4 * lg + x;
+  for (int x = 0; x < 4; ++x) {
+    const int i = 16 * D::I + BdbMfma<T>::row(lg, x);
     if (i < kNd) {
-      float v = acc[x];
-      if (T::a == T::b) v += K3s[i * 28 + j];
-      stage[(3 * i + T::a) * kNl + 3 * j + T::b] = v;
-      if (T::P != T::Q) stage[(3 * j + T::b) * kNl + 3 * i + T::a] = v;
+      const T v = D::diag ? acc[x] + k3[x] : acc[x];
+      if (i >= n0 && i < n1) stage[(3 * (i - n0) + D::a) * kNl + 3 * j + D::b] = v;
+      if (D::P != D::Q && jin) stage[(3 * (j - n0) + D::b) * kNl + 3 * i + D::a] = v;
     }
   }
 }
-template <int T0, int... U>
-__device__ __forceinline__ void nh_w_store_seq(std::integer_sequence<int, U...>, int lr, int lg, const f32x4 (&acc)[6],
-                                               const float *K3s, float *stage) {
-  (nh_w_store1<T0 + U>(lr, lg, acc[U], K3s, stage), ...);
+template <typename T, int T0, int... U>
+__device__ __forceinline__ void nh_w_store_seq(std::integer_sequence<int, U...>, int lr, int lg, int n0, int n1,
+                                               const typename BdbMfma<T>::v4 (&acc)[6],
+                                               const typename BdbMfma<T>::v4 &k3, T *stage) {
+  (nh_w_store1<T, T0 + U>(lr, lg, n0, n1, acc[U], k3, stage), ...);
 }
-template <int T0, int NT>
-__device__ __forceinline__ void nh_w_store(int lr, int lg, const f32x4 (&acc)[6], const float *K3s, float *stage) {
-  nh_w_store_seq<T0>(std::make_integer_sequence<int, NT>{}, lr, lg, acc, K3s, stage);
+template <typename T, int T0, int NT>
+__device__ __forceinline__ void nh_w_store(int lr, int lg, int n0, int n1, const typename BdbMfma<T>::v4 (&acc)[6],
+                                           const typename BdbMfma<T>::v4 &k3, T *stage) {
+  nh_w_store_seq<T, T0>(std::make_integer_sequence<int, NT>{}, lr, lg, n0, n1, acc, k3, stage);
 }
 
-// stage aliases phase A's arrays (gq, coef, Fq): written only after the
+// the stage aliases phase A's arrays and hC: written only after the
 // workgroup barrier that follows the last read of them
-__device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const double (*Fq)[9],
-                                                        const double (*coef)[4], const double (*gq)[kNd][3],
-                                                        const float (*hC)[3][kNhN], float *K3s, float *stage,
-                                                        double *red, const EbArgs &a) {
+template <typename T>
+__device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (*Fq)[9], const double (*coef)[4],
+                                                const double (*gq)[kNd][3], const T (*hC)[3][kNhN], T *stage,
+                                                double *red, const EbArgs &a) {
+  typedef typename BdbMfma<T>::v4 v4;
   const int w = t / 64, l = t % 64, lr = l % 16, lg = l / 16;
-  // K3 node tile (w / 2, (w + 1) / 2) for wavefronts 0-2: for each component c, 7 steps over the points
+  // K3 node tile (w / 2, (w + 1) / 2) for wavefronts 0-2: for each point
+  // step, the three components
+  v4 k3 = v4{T(0), T(0), T(0), T(0)};
   if (w < 3) {
     const int i = 16 * (w / 2) + lr, j = 16 * ((w + 1) / 2) + lr;
     const bool iv = i < kNd, jv = j < kNd;
     const int ic = iv ? i : 0, jc = jv ? j : 0;
-    f32x4 c3 = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int st = 0; st < kNhQ / 4; ++st) {
       const int q = 4 * st + lg;
@@ -185,76 +214,78 @@ __device__ __forceinline__ void nh_phase_b_spatial_mfma(int t, int64_t e, const 
         m2 += pen * (F[2] * f0 + F[5] * f1 + F[8] * f2);
       }
       const bool ok_a = kv && iv, ok_b = kv && jv;
-      c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok_a ? (float)gi[0] : 0.f, ok_b ? (float)m0 : 0.f, c3, 0, 0, 0);
-      c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok_a ? (float)gi[1] : 0.f, ok_b ? (float)m1 : 0.f, c3, 0, 0, 0);
-      c3 = __builtin_amdgcn_mfma_f32_16x16x4f32(ok_a ? (float)gi[2] : 0.f, ok_b ? (float)m2 : 0.f, c3, 0, 0, 0);
-    }
-#pragma unroll
-    for (int x = 0; x < 4; ++x) {  // D layout: row 4 lg + x, column lr
-      const int ii = 16 * (w / 2) + 4 * lg + x;
-      if (ii < kNd && jv) K3s[ii * 28 + j] = c3[x];
+      k3 = BdbMfma<T>::mma(ok_a ? (T)gi[0] : T(0), ok_b ? (T)m0 : T(0), k3);
+      k3 = BdbMfma<T>::mma(ok_a ? (T)gi[1] : T(0), ok_b ? (T)m1 : T(0), k3);
+      k3 = BdbMfma<T>::mma(ok_a ? (T)gi[2] : T(0), ok_b ? (T)m2 : T(0), k3);
     }
   }
   // this lane's alpha_q and beta_q at q = 4 st + lg (hC is zero at q = 27)
-  float al[kNhQ / 4], be[kNhQ / 4];
+  T al[kNhQ / 4], be[kNhQ / 4];
 #pragma unroll
   for (int st = 0; st < kNhQ / 4; ++st) {
     const int qc = min(4 * st + lg, kNq - 1);
-    al[st] = (float)coef[qc][0];
-    be[st] = (float)coef[qc][1];
+    al[st] = (T)coef[qc][0];
+    be[st] = (T)coef[qc][1];
   }
-  f32x4 acc[6];
-  if (w == 0) nh_w_tiles<0, 5>(lr, lg, hC, al, be, acc);
-  else if (w == 1) nh_w_tiles<5, 5>(lr, lg, hC, al, be, acc);
-  else if (w == 2) nh_w_tiles<10, 5>(lr, lg, hC, al, be, acc);
-  else nh_w_tiles<15, 6>(lr, lg, hC, al, be, acc);
-  __syncthreads();  // K3 complete, gq / coef / Fq read: the stage from here on
-  if (w == 0) nh_w_store<0, 5>(lr, lg, acc, K3s, stage);
-  else if (w == 1) nh_w_store<5, 5>(lr, lg, acc, K3s, stage);
-  else if (w == 2) nh_w_store<10, 5>(lr, lg, acc, K3s, stage);
-  else nh_w_store<15, 6>(lr, lg, acc, K3s, stage);
-  __syncthreads();
-  float *E = reinterpret_cast<float *>(a.E) + e * kNl * kNl;
-  float mx = 0.f;
-  for (int x = t; x < kNl * kNl; x += kEbThreads) {
-    const float v = stage[x];
-    mx = fmaxf(mx, fabsf(v));
-    __builtin_nontemporal_store(v, E + x);
+  v4 acc[6];
+  if (w == 0) nh_w_tiles<T, 0, 5>(lr, lg, hC, al, be, acc);
+  else if (w == 1) nh_w_tiles<T, 5, 5>(lr, lg, hC, al, be, acc);
+  else if (w == 2) nh_w_tiles<T, 10, 5>(lr, lg, hC, al, be, acc);
+  else nh_w_tiles<T, 15, 6>(lr, lg, hC, al, be, acc);
+  T *E = reinterpret_cast<T *>(a.E) + e * kNl * kNl;
+  double mx = 0.0;
+  // FP32: one stage of all 27 node rows; FP64: node rows 0-15, then 16-26
+  constexpr int kSlabs = sizeof(T) == 4 ? 1 : 2;
+#pragma unroll
+  for (int sl = 0; sl < kSlabs; ++sl) {
+    const int n0 = kSlabs == 1 ? 0 : 16 * sl, n1 = kSlabs == 1 || sl == 1 ? kNd : 16;
+    __syncthreads();  // the last reads of what the stage overlays (or of the previous slab) are done
+    if (w == 0) nh_w_store<T, 0, 5>(lr, lg, n0, n1, acc, k3, stage);
+    else if (w == 1) nh_w_store<T, 5, 5>(lr, lg, n0, n1, acc, k3, stage);
+    else if (w == 2) nh_w_store<T, 10, 5>(lr, lg, n0, n1, acc, k3, stage);
+    else nh_w_store<T, 15, 6>(lr, lg, n0, n1, acc, k3, stage);
+    __syncthreads();
+    const int len = 3 * (n1 - n0) * kNl;
+    T *dst = E + 3 * n0 * kNl;
+    for (int x = t; x < len; x += kEbThreads) {
+      const T v = stage[x];
+      mx = fmax(mx, (double)fabs(v));
+      __builtin_nontemporal_store(v, dst + x);
+    }
   }
   // threshold 1e-14 max|K| (all-zero element skipped)
-  const double mxe = block_max((double)mx, red);
+  const double mxe = block_max(mx, red);
   if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
 }
 
-// phase A's per-element arrays; in the FP32 form phase B stages E over them
+// phase A's per-element arrays; phase B stages E over them and hC
 struct NhPhaseA {
   double X[kNv][3], Ue[kNd][3];
   double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], cf[kNq][4];
   double gq[kNq][kNd][3];
 };
-template <bool kLow>
+template <typename T>
 union NhLds {
-  NhPhaseA A;
-  float stage[kLow ? kNl * kNl : 1];
+  struct {
+    NhPhaseA A;
+    T hC[kNhQ][3][kNhN];
+  } P;
+  T stage[3 * (sizeof(T) == 4 ? kNd : 16) * kNl];
 };
 
 template <typename Real>
 __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
-  constexpr bool kLow = !std::is_same<Real, double>::value;
-  __shared__ NhLds<kLow> L;
-  double(&X)[kNv][3] = L.A.X;
-  double(&Ue)[kNd][3] = L.A.Ue;
-  double(&Bq)[kNq][9] = L.A.Bq;
-  double(&wJ)[kNq] = L.A.wJ;
-  double(&Fq)[kNq][9] = L.A.Fq;
-  double(&Sq)[kNq][9] = L.A.Sq;
-  double(&Cq)[kNq][9] = L.A.Cq;
-  double(&cf)[kNq][4] = L.A.cf;
-  double(&gq)[kNq][kNd][3] = L.A.gq;
-  __shared__ float hC[kLow ? kNhQ : 1][3][kNhN];  // FP32 form: the spatial gradients, fragment layout
-  __shared__ float K3s[kLow ? kNhK3 : 1];      // FP32 form: the node term K3 of phase B
-  // FP64 form: the spatial gradients and E's slabs in phase B
-  __shared__ double Tbuf[kLow ? 1 : kNq * kNd * 3];
+  __shared__ NhLds<Real> L;
+  double(&X)[kNv][3] = L.P.A.X;
+  double(&Ue)[kNd][3] = L.P.A.Ue;
+  double(&Bq)[kNq][9] = L.P.A.Bq;
+  double(&wJ)[kNq] = L.P.A.wJ;
+  double(&Fq)[kNq][9] = L.P.A.Fq;
+  double(&Sq)[kNq][9] = L.P.A.Sq;
+  double(&Cq)[kNq][9] = L.P.A.Cq;
+  double(&cf)[kNq][4] = L.P.A.cf;
+  double(&gq)[kNq][kNd][3] = L.P.A.gq;
+  Real(&hC)[kNhQ][3][kNhN] = L.P.hC;  // phase B: the spatial gradients in the fragments' layout
   __shared__ double red[kEbThreads];
   const int64_t e = blockIdx.x;
   if (e >= a.n_cv || a.nq != kNq) return;
@@ -371,117 +402,32 @@ __global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
   }
   __syncthreads();
   if (t < kNl) a.res[(int64_t)e * kNl + t] = red[t] + red[kNl + t] + red[2 * kNl + t];
-  if constexpr (!kLow) {
-    // Phase B in the spatial form.  With h_qi = F_q^{-T} g_qi (spatial
-    // gradients), the push-forward of this law's material tangent is
-    // J c = lambda 1 x 1 + (mu - lambda ln J)(two symmetric identities), and
-    // F^{-1} b F^{-T} = I folds the stress term into mu g_qi . g_qj, so the
-    // (i, j) block of D's contraction is, per point,
-    //   wJ_q [lambda h_qi (x) h_qj + (mu - lambda ln J_q) h_qj (x) h_qi
-    //         + (mu g_qi . g_qj + pen_q (F g_qi) . (F g_qj)) I]
-    // (pen_q = 1e200 where det F <= 0: the derivative of the law's penalty
-    // term 1e200 C added to S there).  Same matrix as the material form
-    // D[(a,c),(b,n)] = wJ (delta_ab S_nc + F_ak A_kcmn F_bm) up to roundoff,
-    // without forming D or D g: 24 FMA per block and point.
-    // B1: spatial gradients h[q][i][a] = sum_c (F^{-1})_ca g[q][i][c]
-    double (*hq)[kNd][3] = reinterpret_cast<double (*)[kNd][3]>(Tbuf);
-    static_assert(sizeof(Tbuf) >= sizeof(double) * kNq * kNd * 3, "h fits in the T buffer");
-    for (int x = t; x < kNq * kNd; x += kEbThreads) {
-      const int q = x / kNd, i = x % kNd;
+  // Phase B in the spatial form.  With h_qi = F_q^{-T} g_qi (spatial
+  // gradients), the push-forward of this law's material tangent is
+  // J c = lambda 1 x 1 + (mu - lambda ln J)(two symmetric identities), and
+  // F^{-1} b F^{-T} = I folds the stress term into mu g_qi . g_qj, so the
+  // (i, j) block of D's contraction is, per point,
+  //   wJ_q [lambda h_qi (x) h_qj + (mu - lambda ln J_q) h_qj (x) h_qi
+  //         + (mu g_qi . g_qj + pen_q (F g_qi) . (F g_qj)) I]
+  // (pen_q = 1e200 where det F <= 0: the derivative of the law's penalty
+  // term 1e200 C added to S there).  Same matrix as the material form
+  // D[(a,c),(b,n)] = wJ (delta_ab S_nc + F_ak A_kcmn F_bm) up to roundoff,
+  // without forming D or D g (nh_phase_b_mfma).
+  // hC[q][c][i] = h_qic in Real (zero for q = 27 and nodes i >= 27)
+  for (int x = t; x < kNhQ * kNhN; x += kEbThreads) {
+    const int q = x / kNhN, i = x % kNhN;
+    Real v[3] = {Real(0), Real(0), Real(0)};
+    if (q < kNq && i < kNd) {
       const double *Fi = Cq[q], *g = gq[q][i];
       const double g0 = g[0], g1 = g[1], g2 = g[2];
 #pragma unroll
-      for (int aa = 0; aa < 3; ++aa) hq[q][i][aa] = Fi[aa] * g0 + Fi[3 + aa] * g1 + Fi[6 + aa] * g2;
+      for (int c = 0; c < 3; ++c) v[c] = (Real)(Fi[c] * g0 + Fi[3 + c] * g1 + Fi[6 + c] * g2);
     }
-    __syncthreads();
-    // B2: K is symmetric, so only the blocks (i, j) with i <= j are formed,
-    // by 135 threads (i-group ig of 3 nodes, node j >= 3 ig), 27 sums each;
-    // a block below the diagonal that a thread also forms (j inside its own
-    // group) is not written, its mirror is
-    int ig = 0, jn = 0;
-    const bool owner = t < 135;
-    if (owner) {
-      int off = t;
-      while (off >= kNd - 3 * ig) off -= kNd - 3 * ig++;
-      jn = 3 * ig + off;
-    }
-    double K[3][3][3];
-    for (int ii = 0; ii < 3; ++ii)
-      for (int x = 0; x < 3; ++x)
-        for (int y = 0; y < 3; ++y) K[ii][x][y] = 0.0;
-    if (owner) {
-      for (int q = 0; q < kNq; ++q) {
-        const double al = cf[q][0], be = cf[q][1], ga = cf[q][2], pen = cf[q][3];
-        const double *hj = hq[q][jn], *gj = gq[q][jn];
-        double ha[3], hb[3], m[3];
-        for (int x = 0; x < 3; ++x) {
-          ha[x] = al * hj[x];
-          hb[x] = be * hj[x];
-          m[x] = ga * gj[x];
-        }
-        if (pen != 0.0) {  // det F <= 0 at this point: + pen C g_j, C = F^T F
-          const double *F = Fq[q];
-          double fg[3];
-          for (int r = 0; r < 3; ++r) fg[r] = F[3 * r] * gj[0] + F[3 * r + 1] * gj[1] + F[3 * r + 2] * gj[2];
-          for (int x = 0; x < 3; ++x) m[x] += pen * (F[x] * fg[0] + F[3 + x] * fg[1] + F[6 + x] * fg[2]);
-        }
-        for (int ii = 0; ii < 3; ++ii) {
-          const double *hi = hq[q][3 * ig + ii], *gi = gq[q][3 * ig + ii];
-          const double hh[3] = {hi[0], hi[1], hi[2]};
-          const double d = gi[0] * m[0] + gi[1] * m[1] + gi[2] * m[2];
-          for (int x = 0; x < 3; ++x)
-            for (int y = 0; y < 3; ++y) K[ii][x][y] += hh[x] * ha[y] + hh[y] * hb[x];
-          for (int x = 0; x < 3; ++x) K[ii][x][x] += d;
-        }
-      }
-    }
-    // threshold 1e-14 max|K| (all-zero element skipped)
-    double mx = 0.0;
-    for (int ii = 0; ii < 3; ++ii)
-      for (int x = 0; x < 3; ++x)
-        for (int y = 0; y < 3; ++y) mx = fmax(mx, fabs(K[ii][x][y]));
-    const double mxe = block_max(owner ? mx : 0.0, red);
-    if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
-    // E row-major in three slabs of 27 rows (nodes 9 p .. 9 p + 8) staged in
-    // LDS (the spatial gradients' buffer, free now): block (i, j) at rows
-    // 3 i + x, columns 3 j + y, and for i < j its transpose at rows 3 j + y,
-    // columns 3 i + x; each slab then goes out as one contiguous run
-    double *slab = Tbuf, *E = a.E + (int64_t)e * kNl * kNl;
-    static_assert(sizeof(Tbuf) >= sizeof(double) * 27 * kNl, "a 27-row slab fits in the T buffer");
-    for (int ps = 0; ps < 3; ++ps) {
-      __syncthreads();  // h is read until B2 ends; the previous slab is streamed
-      if (owner)
-        for (int ii = 0; ii < 3; ++ii) {
-          const int i = 3 * ig + ii;
-          if (i > jn) continue;
-          const bool dir = i / 9 == ps, mir = i < jn && jn / 9 == ps;
-          for (int x = 0; x < 3; ++x)
-            for (int y = 0; y < 3; ++y) {
-              if (dir) slab[(3 * (i - 9 * ps) + x) * kNl + 3 * jn + y] = K[ii][x][y];
-              if (mir) slab[(3 * (jn - 9 * ps) + y) * kNl + 3 * i + x] = K[ii][x][y];
-            }
-        }
-      __syncthreads();
-      for (int x = t; x < 27 * kNl; x += kEbThreads)
-        __builtin_nontemporal_store(slab[x], E + (int64_t)ps * 27 * kNl + x);
-    }
-  } else {
-    // hC[q][c][i] = h_qic (zero for q = 27 and nodes i >= 27)
-    for (int x = t; x < kNhQ * kNhN; x += kEbThreads) {
-      const int q = x / kNhN, i = x % kNhN;
-      float v[3] = {0.f, 0.f, 0.f};
-      if (q < kNq && i < kNd) {
-        const double *Fi = Cq[q], *g = gq[q][i];
-        const double g0 = g[0], g1 = g[1], g2 = g[2];
 #pragma unroll
-        for (int c = 0; c < 3; ++c) v[c] = (float)(Fi[c] * g0 + Fi[3 + c] * g1 + Fi[6 + c] * g2);
-      }
-#pragma unroll
-      for (int c = 0; c < 3; ++c) hC[q][c][i] = v[c];
-    }
-    __syncthreads();
-    nh_phase_b_spatial_mfma(t, e, Fq, cf, gq, hC, K3s, L.stage, red, a);
+    for (int c = 0; c < 3; ++c) hC[q][c][i] = v[c];
   }
+  __syncthreads();
+  nh_phase_b_mfma<Real>(t, e, Fq, cf, gq, hC, L.stage, red, a);
 }
 
 // keep bits of element matrices held in the buffer: bit r*nl + c
@@ -928,25 +874,6 @@ hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision) {
 // sign(w_q).  One wavefront per element (grid stride), only the upper tiles
 // computed.  E (row-major, T) is stored from the accumulators: tile (a, b) as
 // 16-value runs of its rows, its mirror (b, a) as 4-value runs per lane.
-template <typename T>
-struct BdbMfma;
-template <>
-struct BdbMfma<double> {
-  typedef __attribute__((__vector_size__(4 * sizeof(double)))) double v4;
-  static __device__ __forceinline__ v4 mma(double a, double b, v4 c) {
-    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int g, int r) { return 4 * r + g; }
-};
-template <>
-struct BdbMfma<float> {
-  typedef f32x4 v4;
-  static __device__ __forceinline__ v4 mma(float a, float b, v4 c) {
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-  }
-  static __device__ __forceinline__ int row(int g, int r) { return 4 * g + r; }
-};
-
 // F[s][m] of one point: Voigt row s (xx, yy, zz, yz, xz, xy) of S B_q for
 // dof m = 3 i + ax, from the staged (reference gradient of phi_i, signed
 // sqrt(|w_q|)) h; on / off / sh: the normal-diagonal, normal-off-diagonal and
