@@ -139,7 +139,8 @@ struct EbArgs {
   const double *gref;       // nq*nd*3 base gradients
   const double *qw;
   int nq;
-  double *E;                // n_cv * nl * nl element matrices (row-major; FP32 values in the FP32 form)
+  double *E;                // n_cv * nl * nl element matrices, node-blocked: K[(i,a),(j,b)] at
+                            // ((3 i + a) 3 + b) nd + j (FP32 values in the FP32 form)
   double *res;              // n_cv * nl element residuals
   double *thr;              // n_cv thresholds
   int *err;

@@ -93,10 +93,10 @@ struct BdbMfma<float> {
 // which it is added, and two of the twelve tiles with a < b; wavefront 3
 // forms the other six (91, 91, 91, 84 products).  The tiles stay in the
 // accumulators until phase A's arrays and hC are free; then E is staged in
-// LDS over them (FP32: whole; FP64: node rows 0-15, then 16-26), each tile
-// written with its transpose, and each staged run, contiguous in E, is
-// streamed by the workgroup.
-constexpr int kNhQ = 28, kNhN = 32, kNhTiles = 21;
+// LDS over them (node rows 0-15, then 16-26), each tile written with its
+// transpose, and each staged run, contiguous in E, is streamed by the
+// workgroup.
+constexpr int kNhQ = 28, kNhN = 32;
 
 // tile N (0..20): wavefront N / 5 (N < 15) takes N % 5 = 0..2 as the (a, a)
 // tiles of its K3 node tile and 3..4 as with-a<b tiles 2 (N / 5) + (N % 5 - 3);
@@ -149,36 +149,38 @@ __device__ __forceinline__ void nh_w_tiles(int lr, int lg, const T (*hC)[3][kNhN
   }
 }
 
-// tile N's entries in node rows [n0, n1) of E into the stage (row-major,
-// row 3 (i - n0) + component), K3 added to (a, a) tiles, the transpose
-// written for P < Q
+// tile N's entries in node rows [n0, n1) of E into the stage (E's
+// node-blocked layout, entry ((3 (i - n0) + a) 3 + b) 27 + j for K[(i,a),(j,b)]),
+// K3 added to (a, a) tiles, the transpose written for P < Q.  Branch-free:
+// a lane whose entry is outside the slab (or a zero pad) writes its own slot
+// of the trash row behind the stage instead.
 template <typename T, int N>
-__device__ __forceinline__ void nh_w_store1(int lr, int lg, int n0, int n1, const typename BdbMfma<T>::v4 &acc,
+__device__ __forceinline__ void nh_w_store1(int lr, int lg, int n0, int n1, int trash,
+                                            const typename BdbMfma<T>::v4 &acc,
                                             const typename BdbMfma<T>::v4 &k3, T *stage) {
   typedef NhTile<N> D;
   const int j = 16 * D::J + lr;
-  if (j >= kNd) return;
-  const bool jin = j >= n0 && j < n1;
+  const bool jv = j < kNd, jin = jv && j >= n0 && j < n1;
 #pragma unroll
   for (int x = 0; x < 4; ++x) {
     const int i = 16 * D::I + BdbMfma<T>::row(lg, x);
-    if (i < kNd) {
-      const T v = D::diag ? acc[x] + k3[x] : acc[x];
-      if (i >= n0 && i < n1) stage[(3 * (i - n0) + D::a) * kNl + 3 * j + D::b] = v;
-      if (D::P != D::Q && jin) stage[(3 * (j - n0) + D::b) * kNl + 3 * i + D::a] = v;
-    }
+    const bool iv = i < kNd, iin = iv && i >= n0 && i < n1;
+    const T v = D::diag ? acc[x] + k3[x] : acc[x];
+    stage[iin && jv ? ((3 * (i - n0) + D::a) * 3 + D::b) * kNd + j : trash] = v;
+    if (D::P != D::Q) stage[jin && iv ? ((3 * (j - n0) + D::b) * 3 + D::a) * kNd + i : trash] = v;
   }
 }
 template <typename T, int T0, int... U>
 __device__ __forceinline__ void nh_w_store_seq(std::integer_sequence<int, U...>, int lr, int lg, int n0, int n1,
-                                               const typename BdbMfma<T>::v4 (&acc)[6],
+                                               int trash, const typename BdbMfma<T>::v4 (&acc)[6],
                                                const typename BdbMfma<T>::v4 &k3, T *stage) {
-  (nh_w_store1<T, T0 + U>(lr, lg, n0, n1, acc[U], k3, stage), ...);
+  (nh_w_store1<T, T0 + U>(lr, lg, n0, n1, trash, acc[U], k3, stage), ...);
 }
 template <typename T, int T0, int NT>
-__device__ __forceinline__ void nh_w_store(int lr, int lg, int n0, int n1, const typename BdbMfma<T>::v4 (&acc)[6],
+__device__ __forceinline__ void nh_w_store(int lr, int lg, int n0, int n1, int trash,
+                                           const typename BdbMfma<T>::v4 (&acc)[6],
                                            const typename BdbMfma<T>::v4 &k3, T *stage) {
-  nh_w_store_seq<T, T0>(std::make_integer_sequence<int, NT>{}, lr, lg, n0, n1, acc, k3, stage);
+  nh_w_store_seq<T, T0>(std::make_integer_sequence<int, NT>{}, lr, lg, n0, n1, trash, acc, k3, stage);
 }
 
 // the stage aliases phase A's arrays and hC: written only after the
@@ -188,7 +190,8 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (
                                                 const double (*gq)[kNd][3], const T (*hC)[3][kNhN], T *stage,
                                                 double *red, const EbArgs &a) {
   typedef typename BdbMfma<T>::v4 v4;
-  const int w = t / 64, l = t % 64, lr = l % 16, lg = l / 16;
+  // the wavefront index as a scalar: the branches on it are uniform
+  const int w = __builtin_amdgcn_readfirstlane(t / 64), l = t % 64, lr = l % 16, lg = l / 16;
   // K3 node tile (w / 2, (w + 1) / 2) for wavefronts 0-2: for each point
   // step, the three components
   v4 k3 = v4{T(0), T(0), T(0), T(0)};
@@ -234,16 +237,21 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (
   else nh_w_tiles<T, 15, 6>(lr, lg, hC, al, be, acc);
   T *E = reinterpret_cast<T *>(a.E) + e * kNl * kNl;
   double mx = 0.0;
-  // FP32: one stage of all 27 node rows; FP64: node rows 0-15, then 16-26
-  constexpr int kSlabs = sizeof(T) == 4 ? 1 : 2;
+  // two stages: node rows 0-15, then 16-26; the trash row (a slot per lane)
+  // behind the larger one.  (FP32 would fit one stage of all 27 node rows;
+  // that variant gave wrong values with this compiler at -O3, with a fill of
+  // the stage before the tile stores or at -O1 it did not, and the cause was
+  // not found: both precisions use the two stages, which are checked.)
+  constexpr int kSlabs = 2;
+  const int trash = 3 * 16 * kNl + l;
 #pragma unroll
   for (int sl = 0; sl < kSlabs; ++sl) {
-    const int n0 = kSlabs == 1 ? 0 : 16 * sl, n1 = kSlabs == 1 || sl == 1 ? kNd : 16;
+    const int n0 = 16 * sl, n1 = sl == 1 ? kNd : 16;
     __syncthreads();  // the last reads of what the stage overlays (or of the previous slab) are done
-    if (w == 0) nh_w_store<T, 0, 5>(lr, lg, n0, n1, acc, k3, stage);
-    else if (w == 1) nh_w_store<T, 5, 5>(lr, lg, n0, n1, acc, k3, stage);
-    else if (w == 2) nh_w_store<T, 10, 5>(lr, lg, n0, n1, acc, k3, stage);
-    else nh_w_store<T, 15, 6>(lr, lg, n0, n1, acc, k3, stage);
+    if (w == 0) nh_w_store<T, 0, 5>(lr, lg, n0, n1, trash, acc, k3, stage);
+    else if (w == 1) nh_w_store<T, 5, 5>(lr, lg, n0, n1, trash, acc, k3, stage);
+    else if (w == 2) nh_w_store<T, 10, 5>(lr, lg, n0, n1, trash, acc, k3, stage);
+    else nh_w_store<T, 15, 6>(lr, lg, n0, n1, trash, acc, k3, stage);
     __syncthreads();
     const int len = 3 * (n1 - n0) * kNl;
     T *dst = E + 3 * n0 * kNl;
@@ -270,7 +278,7 @@ union NhLds {
     NhPhaseA A;
     T hC[kNhQ][3][kNhN];
   } P;
-  T stage[3 * (sizeof(T) == 4 ? kNd : 16) * kNl];
+  T stage[3 * 16 * kNl + 64];  // the slab and the trash row
 };
 
 template <typename Real>
@@ -441,10 +449,13 @@ __global__ __launch_bounds__(256) void k_keep_emat(int64_t n_cv, int nl, int kwo
   const int word = (int)(w - e * kwords);
   const double th = thr[e];
   const double *Ee = E + e * (int64_t)nl * nl;
+  const int nd = nl / 3;
   uint32_t m = 0;
   for (int b = 0; b < 32; ++b) {
-    const int bit = word * 32 + b;
-    if (bit < nl * nl && fabs(Ee[bit]) > th) m |= 1u << b;
+    const int bit = word * 32 + b;  // row-major entry (r, c) = (3 i + x, 3 j + y)
+    if (bit >= nl * nl) break;
+    const int r = bit / nl, c = bit - r * nl;
+    if (fabs(Ee[(3 * r + c % 3) * nd + c / 3]) > th) m |= 1u << b;
   }
   keep[w] = m;
 }
@@ -453,17 +464,19 @@ __global__ __launch_bounds__(256) void k_keep_emat(int64_t n_cv, int nl, int kwo
 // an incidence are one contiguous run of Q*NL values (FP64 or FP32); the runs
 // of all the node's incidences are streamed by the whole workgroup at once
 // (independent loads, LDS atomics), then the node's CSR rows are written once.
-// TH threads per workgroup: 256 for the Q2 hexahedra (one node, long runs),
-// 64 for the affine tets' packed wave groups (short runs, more groups in
-// flight); wavefront w writes rows w, w + TH/64, ... of each node.
-constexpr int kGatherThreads = 256;
+// TH threads per workgroup: 256 for the Q2 hexahedra (one node, long runs;
+// four wavefronts owning two, two, two and three (row component, column
+// component) pairs), 64 for the affine tets' packed wave groups (short
+// runs, more groups in flight); wavefront w writes rows w, w + TH/64, ... of
+// each node.
+constexpr int kGatherThreadsQ2 = 256;
 
 // npw (1..4) small patches of one bin share a workgroup: their incidence
 // lists are concatenated and streamed together; patch k's accumulator starts
 // at the prefix sum of Q*Q*ns over the patches before it.
 constexpr int kGatherNpw = 4;
 
-template <int ND, int Q, typename TE, int TH = kGatherThreads, int MI = 64>
+template <int ND, int Q, typename TE, int TH, int MI = 64>
 __global__ __launch_bounds__(TH) void k_gather_emat(PatchArgs a, const TE *__restrict__ E,
                                                     const double *__restrict__ thr) {
   extern __shared__ double acc[];  // the npw accumulators, acc_cap doubles in all
@@ -549,19 +562,42 @@ __global__ __launch_bounds__(TH) void k_gather_emat(PatchArgs a, const TE *__res
   // group) loads kI incidences with clamped indices; 256 threads (Q2
   // hexahedra, runs of 243, often fewer incidences than kI) skip the loads
   // past the group's last incidence.
-  constexpr int NV = (RUN + TH - 1) / TH;
+  // Which entries (row component x, column node j, column component y) of
+  // a run a thread owns.  One wavefront (E row-major, run offset
+  // x NL + j Q + y): t, t + 64, ...  Several wavefronts (E node-blocked, run
+  // offset (x Q + y) ND + j, the Neo-Hookean element kernel's layout): each
+  // owns whole (x, y) pairs, each ND consecutive values: 64 / ND pairs for
+  // all but the last wavefront, which owns the rest in NV passes.  Every
+  // accumulator entry (x, slot, y) is then added to by one wavefront only,
+  // in incidence order, and no two lanes of one ds_add_f64 share an entry
+  // (the nodes of an incidence have distinct slots): the sums do not depend
+  // on timing.
+  constexpr bool kPairs = TH > 64;
+  constexpr int kW = TH / 64, kPpw = 64 / ND, kLast = Q * Q - kPpw * (kW - 1);
+  static_assert(!kPairs || (kLast > 0 && kLast * ND <= 2 * 64), "the last wavefront's pairs fit two passes");
+  constexpr int NV = kPairs ? (kLast * ND + 63) / 64 : (RUN + TH - 1) / TH;
   constexpr int kI = TH == 64 ? (NV >= 16 ? 1 : 16 / NV) : 8;
   int ov[NV], xv[NV], jv[NV], yv[NV];
   bool okv[NV];
 #pragma unroll
   for (int v = 0; v < NV; ++v) {
-    const int o = t + v * TH;
-    okv[v] = o < RUN;
-    ov[v] = okv[v] ? o : 0;
-    xv[v] = ov[v] / NL;
-    const int c = ov[v] - xv[v] * NL;
-    jv[v] = c / Q;
-    yv[v] = c - jv[v] * Q;
+    if (kPairs) {  // E node-blocked: pair p's ND values are consecutive
+      const int w = t / 64, L = t % 64 + 64 * v;  // this wavefront's value index
+      const int p = kPpw * w + L / ND;
+      okv[v] = w < kW - 1 ? L < kPpw * ND : L < kLast * ND;
+      xv[v] = okv[v] ? p / Q : 0;
+      yv[v] = okv[v] ? p % Q : 0;
+      jv[v] = okv[v] ? L % ND : 0;
+      ov[v] = okv[v] ? p * ND + jv[v] : 0;
+    } else {
+      const int o = t + v * TH;
+      okv[v] = o < RUN;
+      ov[v] = okv[v] ? o : 0;
+      xv[v] = ov[v] / NL;
+      const int c = ov[v] - xv[v] * NL;
+      jv[v] = c / Q;
+      yv[v] = c - jv[v] * Q;
+    }
   }
   for (int c0 = 0; c0 < total_inc; c0 += kI) {
     TE val[kI][NV];
@@ -1033,11 +1069,8 @@ hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const void *E,
   const double *E64 = static_cast<const double *>(E);
   const float *E32 = static_cast<const float *>(E);
   if (a.nd == 27 && a.qdim == 3) {
-    if (e_fp32) k_gather_emat<27, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
-    else k_gather_emat<27, 3, double><<<grid, kGatherThreads, lds, s>>>(a, E64, thr);
-  } else if (a.nd == 8 && a.qdim == 3) {
-    if (e_fp32) k_gather_emat<8, 3, float><<<grid, kGatherThreads, lds, s>>>(a, E32, thr);
-    else k_gather_emat<8, 3, double><<<grid, kGatherThreads, lds, s>>>(a, E64, thr);
+    if (e_fp32) k_gather_emat<27, 3, float, kGatherThreadsQ2><<<grid, kGatherThreadsQ2, lds, s>>>(a, E32, thr);
+    else k_gather_emat<27, 3, double, kGatherThreadsQ2><<<grid, kGatherThreadsQ2, lds, s>>>(a, E64, thr);
   } else if (a.nd == 10 && a.qdim == 3 && !e_fp32) {  // element-batched affine P2, FP64 matrix cores
     if (group_inc <= 32) k_gather_emat<10, 3, double, 64, 32><<<grid, 64, lds, s>>>(a, E64, thr);
     else k_gather_emat<10, 3, double, 64><<<grid, 64, lds, s>>>(a, E64, thr);
