@@ -282,7 +282,7 @@ union NhLds {
 };
 
 template <typename Real>
-__global__ __launch_bounds__(kEbThreads) void k_nh_element(EbArgs a) {
+__global__ __launch_bounds__(kEbThreads, sizeof(Real) == 4 ? 4 : 3) void k_nh_element(EbArgs a) {
   __shared__ NhLds<Real> L;
   double(&X)[kNv][3] = L.P.A.X;
   double(&Ue)[kNd][3] = L.P.A.Ue;
