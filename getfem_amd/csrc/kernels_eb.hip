@@ -186,40 +186,27 @@ __device__ __forceinline__ void nh_w_store(int lr, int lg, int n0, int n1, int t
 // the stage aliases phase A's arrays and hC: written only after the
 // workgroup barrier that follows the last read of them
 template <typename T>
-__device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (*Fq)[9], const double (*coef)[4],
-                                                const double (*gq)[kNd][3], const T (*hC)[3][kNhN], T *stage,
-                                                double *red, const EbArgs &a) {
+__device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (*coef)[2], const double (*Mq)[9],
+                                                const T (*hC)[3][kNhN], T *stage, double *red, const EbArgs &a) {
   typedef typename BdbMfma<T>::v4 v4;
   // the wavefront index as a scalar: the branches on it are uniform
   const int w = __builtin_amdgcn_readfirstlane(t / 64), l = t % 64, lr = l % 16, lg = l / 16;
-  // K3 node tile (w / 2, (w + 1) / 2) for wavefronts 0-2: for each point
-  // step, the three components
+  // K3 node tile (w / 2, (w + 1) / 2) for wavefronts 0-2, in the spatial
+  // gradients: K3[i][j] = sum_q h_qi^T M_q h_qj, k = (c, q): A = h_qic,
+  // B = (M_q h_qj)_c (zero pads in hC make the pads' products zero)
   v4 k3 = v4{T(0), T(0), T(0), T(0)};
   if (w < 3) {
     const int i = 16 * (w / 2) + lr, j = 16 * ((w + 1) / 2) + lr;
-    const bool iv = i < kNd, jv = j < kNd;
-    const int ic = iv ? i : 0, jc = jv ? j : 0;
 #pragma unroll
     for (int st = 0; st < kNhQ / 4; ++st) {
       const int q = 4 * st + lg;
-      const bool kv = q < kNq;
-      const int qc = kv ? q : 0;
-      const double *gi = gq[qc][ic], *gj = gq[qc][jc];
-      const double ga = coef[qc][2], pen = coef[qc][3];
-      const double g0 = gj[0], g1 = gj[1], g2 = gj[2];
-      double m0 = ga * g0, m1 = ga * g1, m2 = ga * g2;
-      if (pen != 0.0) {  // det F <= 0 at this point: + pen C g_j, C = F^T F
-        const double *F = Fq[qc];
-        const double f0 = F[0] * g0 + F[1] * g1 + F[2] * g2, f1 = F[3] * g0 + F[4] * g1 + F[5] * g2,
-                     f2 = F[6] * g0 + F[7] * g1 + F[8] * g2;
-        m0 += pen * (F[0] * f0 + F[3] * f1 + F[6] * f2);
-        m1 += pen * (F[1] * f0 + F[4] * f1 + F[7] * f2);
-        m2 += pen * (F[2] * f0 + F[5] * f1 + F[8] * f2);
+      const double *M = Mq[min(q, kNq - 1)];
+      const T h0 = hC[q][0][j], h1 = hC[q][1][j], h2 = hC[q][2][j];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T m = (T)M[3 * c] * h0 + (T)M[3 * c + 1] * h1 + (T)M[3 * c + 2] * h2;
+        k3 = BdbMfma<T>::mma(hC[q][c][i], m, k3);
       }
-      const bool ok_a = kv && iv, ok_b = kv && jv;
-      k3 = BdbMfma<T>::mma(ok_a ? (T)gi[0] : T(0), ok_b ? (T)m0 : T(0), k3);
-      k3 = BdbMfma<T>::mma(ok_a ? (T)gi[1] : T(0), ok_b ? (T)m1 : T(0), k3);
-      k3 = BdbMfma<T>::mma(ok_a ? (T)gi[2] : T(0), ok_b ? (T)m2 : T(0), k3);
     }
   }
   // this lane's alpha_q and beta_q at q = 4 st + lg (hC is zero at q = 27)
@@ -266,23 +253,29 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (
   if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
 }
 
-// phase A's per-element arrays; phase B stages E over them and hC
+// phase A's per-element arrays; the material gradients gq (phase A) and
+// then the spatial ones hC (phase B) share their space; phase B stages E
+// over all of it
 struct NhPhaseA {
   double X[kNv][3], Ue[kNd][3];
-  double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], cf[kNq][4];
+  double Bq[kNq][9], wJ[kNq], Fq[kNq][9], Sq[kNq][9], Cq[kNq][9], cf[kNq][2], Mq[kNq][9];
+};
+template <typename T>
+union NhGrad {
   double gq[kNq][kNd][3];
+  T hC[kNhQ][3][kNhN];
 };
 template <typename T>
 union NhLds {
   struct {
     NhPhaseA A;
-    T hC[kNhQ][3][kNhN];
+    NhGrad<T> G;
   } P;
   T stage[3 * 16 * kNl + 64];  // the slab and the trash row
 };
 
 template <typename Real>
-__global__ __launch_bounds__(kEbThreads, sizeof(Real) == 4 ? 4 : 3) void k_nh_element(EbArgs a) {
+__global__ __launch_bounds__(kEbThreads, 4) void k_nh_element(EbArgs a) {
   __shared__ NhLds<Real> L;
   double(&X)[kNv][3] = L.P.A.X;
   double(&Ue)[kNd][3] = L.P.A.Ue;
@@ -291,9 +284,10 @@ __global__ __launch_bounds__(kEbThreads, sizeof(Real) == 4 ? 4 : 3) void k_nh_el
   double(&Fq)[kNq][9] = L.P.A.Fq;
   double(&Sq)[kNq][9] = L.P.A.Sq;
   double(&Cq)[kNq][9] = L.P.A.Cq;
-  double(&cf)[kNq][4] = L.P.A.cf;
-  double(&gq)[kNq][kNd][3] = L.P.A.gq;
-  Real(&hC)[kNhQ][3][kNhN] = L.P.hC;  // phase B: the spatial gradients in the fragments' layout
+  double(&cf)[kNq][2] = L.P.A.cf;
+  double(&Mq)[kNq][9] = L.P.A.Mq;
+  double(&gq)[kNq][kNd][3] = L.P.G.gq;
+  Real(&hC)[kNhQ][3][kNhN] = L.P.G.hC;  // phase B: the spatial gradients in the fragments' layout
   __shared__ double red[kEbThreads];
   const int64_t e = blockIdx.x;
   if (e >= a.n_cv || a.nq != kNq) return;
@@ -388,8 +382,23 @@ __global__ __launch_bounds__(kEbThreads, sizeof(Real) == 4 ? 4 : 3) void k_nh_el
     const double w = wJ[t];
     cf[t][0] = w * lam;
     cf[t][1] = w * (mu - 0.5 * lam * ln3);
-    cf[t][2] = w * mu;
-    cf[t][3] = detF <= 0.0 ? w * 1e200 : 0.0;
+    // the node term's kernel in the spatial gradients: with g = F^T h,
+    // gamma g_i . g_j + pen (F g_i) . (F g_j) = h_i^T M h_j, M = gamma b + pen b^2,
+    // b = F F^T (symmetric)
+    const double ga = w * mu, pen = detF <= 0.0 ? w * 1e200 : 0.0;
+    double b[9];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) b[3 * r + c] = F[3 * r] * F[3 * c] + F[3 * r + 1] * F[3 * c + 1] + F[3 * r + 2] * F[3 * c + 2];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        double m = ga * b[3 * r + c];
+        if (pen != 0.0) m += pen * (b[3 * r] * b[c] + b[3 * r + 1] * b[3 + c] + b[3 * r + 2] * b[6 + c]);
+        Mq[t][3 * r + c] = m;
+      }
   }
   __syncthreads();
   // A5: the residual re[(i,a)] = sum_q sum_c P_q[a][c] g[q][i][c] with
@@ -421,21 +430,31 @@ __global__ __launch_bounds__(kEbThreads, sizeof(Real) == 4 ? 4 : 3) void k_nh_el
   // term 1e200 C added to S there).  Same matrix as the material form
   // D[(a,c),(b,n)] = wJ (delta_ab S_nc + F_ak A_kcmn F_bm) up to roundoff,
   // without forming D or D g (nh_phase_b_mfma).
-  // hC[q][c][i] = h_qic in Real (zero for q = 27 and nodes i >= 27)
-  for (int x = t; x < kNhQ * kNhN; x += kEbThreads) {
-    const int q = x / kNhN, i = x % kNhN;
-    Real v[3] = {Real(0), Real(0), Real(0)};
+  // hC[q][c][i] = h_qic in Real (zero for q = 27 and nodes i >= 27), over
+  // gq: formed in registers, written once every thread has read gq
+  constexpr int kHc = (kNhQ * kNhN + kEbThreads - 1) / kEbThreads;
+  Real hv[kHc][3];
+#pragma unroll
+  for (int k = 0; k < kHc; ++k) {
+    const int x = t + k * kEbThreads, q = x / kNhN, i = x % kNhN;
+    hv[k][0] = hv[k][1] = hv[k][2] = Real(0);
     if (q < kNq && i < kNd) {
       const double *Fi = Cq[q], *g = gq[q][i];
       const double g0 = g[0], g1 = g[1], g2 = g[2];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) v[c] = (Real)(Fi[c] * g0 + Fi[3 + c] * g1 + Fi[6 + c] * g2);
+      for (int c = 0; c < 3; ++c) hv[k][c] = (Real)(Fi[c] * g0 + Fi[3 + c] * g1 + Fi[6 + c] * g2);
     }
-#pragma unroll
-    for (int c = 0; c < 3; ++c) hC[q][c][i] = v[c];
   }
   __syncthreads();
-  nh_phase_b_mfma<Real>(t, e, Fq, cf, gq, hC, L.stage, red, a);
+#pragma unroll
+  for (int k = 0; k < kHc; ++k) {
+    const int x = t + k * kEbThreads, q = x / kNhN, i = x % kNhN;
+    if (x < kNhQ * kNhN)
+#pragma unroll
+      for (int c = 0; c < 3; ++c) hC[q][c][i] = hv[k][c];
+  }
+  __syncthreads();
+  nh_phase_b_mfma<Real>(t, e, cf, Mq, hC, L.stage, red, a);
 }
 
 // keep bits of element matrices held in the buffer: bit r*nl + c
