@@ -409,13 +409,16 @@ def _pmc_pass(counter, regex, child_args, timeout=180):
 
 def measure_traffic(args):
     """HBM bytes per assembly of the dominant kernel(s): FETCH_SIZE and
-    WRITE_SIZE (KiB per dispatch) summed over one assembly's launches, each
+    WRITE_SIZE (KiB per dispatch) of the listed kernels, per assembly as the
+    difference between a child run of 3 assemblies and one of 1, halved (so
+    launches of the same kernels during the pattern build cancel), each
     scaled by what the counter reads for a plain 1 GiB device copy."""
     regex = PMC_KERNELS[args.config]
-    reps = 2
-    child = ["--pmc-child", "--config", args.config, "--form", args.form if args.form != "all" else
-             {"c3": "fp64_element_batched"}.get(args.config, "fp64_factorized"),
-             "--steps", str(reps), "--warmup", "0", "--no-cpu"]
+    form = args.form if args.form != "all" else {"c3": "fp64_element_batched"}.get(args.config, "fp64_factorized")
+
+    def child(steps):
+        return ["--pmc-child", "--config", args.config, "--form", form, "--steps", str(steps), "--warmup", "0",
+                "--no-cpu"]
     copy_bytes = float(8 << 27)
     res = {}
     for counter in ("FETCH_SIZE", "WRITE_SIZE"):
@@ -423,10 +426,12 @@ def measure_traffic(args):
         if not cal:
             raise RuntimeError("no copy kernel seen in the calibration pass")
         factor = copy_bytes / (statistics.median(cal) * 1024.0)
-        vals = _pmc_pass(counter, regex, child)
-        if not vals:
+        v1 = _pmc_pass(counter, regex, child(1))
+        v3 = _pmc_pass(counter, regex, child(3))
+        if not v1 or not v3:
             raise RuntimeError("no %s dispatch seen" % regex)
-        res[counter] = {"kib_per_assembly": sum(vals) / reps, "dispatches": len(vals),
+        res[counter] = {"kib_per_assembly": (sum(v3) - sum(v1)) / 2.0,
+                        "dispatches_per_assembly": (len(v3) - len(v1)) / 2.0,
                         "copy_kib": statistics.median(cal), "factor": factor}
     traffic = sum(v["kib_per_assembly"] * 1024.0 * v["factor"] for v in res.values())
     return traffic, res
@@ -731,9 +736,10 @@ def main():
             r["traffic_frac"] = r["traffic_gbs"] / HBM_PEAK_GBS
             r["traffic_pmc"] = detail
             r["traffic_kernels"] = PMC_KERNELS[args.config]
-            r["traffic_note"] = ("measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over a "
-                                 "child run of the same workload and form, summed over one assembly's launches of "
-                                 "the listed kernels, each scaled by the factor a plain 1 GiB device copy gives")
+            r["traffic_note"] = ("measured in this run: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE passes over "
+                                 "child runs of the same workload and form (3 and 1 assemblies, difference "
+                                 "halved), summed over one assembly's launches of the listed kernels, each "
+                                 "scaled by the factor a plain 1 GiB device copy gives")
         except Exception as exc:  # report, never hide
             out["roofline"]["traffic_error"] = repr(exc)
     if not args.no_cpu:
