@@ -145,3 +145,40 @@ def test_neohookean_repeated_assembly_is_bitwise_identical(ctx, prec):
     plan2.set_form(ga.FORM_PERQP, prec)
     plan2.assemble()
     assert np.array_equal(plan2.csr()[2].view(np.int64), v0.view(np.int64))
+
+
+def test_neohookean_inverted_points_follow_the_penalty(ctx):
+    """u_x = -1.6 x^2 inverts every point with x > 0.3125 (det F = 1 - 3.2 x, not
+    zero at any Gauss point of the 2^3 mesh): there the
+    law adds 1e200 C to S (getfem_nonlinear_elasticity.cc:655-656), which
+    enters the tangent through the stress term; the FP64 device matrix (the
+    node term as h^T (gamma b + pen b^2) h) and residual against the oracle
+    (material form), pattern included: the 1e200 terms set those elements'
+    thresholds."""
+    n = 2
+    pts, cvx = meshes.regular_unit_mesh(n, "GT_QK(3,1)")
+    mesh = ga.Mesh(ctx, "GT_QK(3,1)", pts, cvx)
+    mf = ga.MeshFem(mesh, "FEM_QK(3,2)", 3)
+    mim = ga.MeshIm(mesh, "IM_GAUSS_PARALLELEPIPED(3,5)")
+    nodes = mf.basic_dof_nodes()
+    U = np.zeros((nodes.shape[0], 3))
+    U[:, 0] = -1.6 * nodes[:, 0] ** 2
+    U = U.ravel()
+    plan = ga.Plan(mf, mim, ga.TERM_NEOHOOKEAN, [10.0, 1.0])
+    plan.set_state(U)
+    plan.build_pattern()
+    plan.assemble()
+    ip, ix, v = plan.csr()
+    R = plan.residual()
+    P = oracle.Problem(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, 2, 3, mf.basic_dof_table(), mf.nbdof(),
+                       "IM_GAUSS_PARALLELEPIPED(3,5)", oracle.TERM_NEOHOOKEAN, scal_a=10.0, scal_b=1.0, U=U)
+    ipr, ixr, vr = oracle.csc_to_csr(*P.assemble_csc(), mf.nbdof())
+    assert np.abs(vr).max() > 1e190  # the penalty is active
+    assert np.array_equal(ip, ipr) and np.array_equal(ix, ixr)
+    assert np.all(np.isfinite(v)) and np.all(np.isfinite(R))
+    Rr = P.assemble_vector()
+    # norms of values near 1e200 overflow: scale by the largest entry first
+    err = problems.rel_frob(v / np.abs(vr).max(), vr / np.abs(vr).max())
+    rerr = problems.rel_frob(R / np.abs(Rr).max(), Rr / np.abs(Rr).max())
+    print("inverted points: tangent rel err", err, "residual rel err", rerr)
+    assert err <= 1e-12 and rerr <= 1e-12
