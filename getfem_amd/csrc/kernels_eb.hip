@@ -457,26 +457,34 @@ __global__ __launch_bounds__(kEbThreads, 4) void k_nh_element(EbArgs a) {
   nh_phase_b_mfma<Real>(t, e, cf, Mq, hC, L.stage, red, a);
 }
 
-// keep bits of element matrices held in the buffer: bit r*nl + c
+// keep bits of element matrices held in the buffer: bit r*nl + c (row-major
+// entry (r, c) = (3 i + x, 3 j + y)) set where |K| > thr_e.  E is node-blocked
+// (K[(i,x),(j,y)] at ((3 i + x) 3 + y) nd + j): one workgroup per element
+// reads it in order (coalesced) and sets the row-major bits in an LDS copy of
+// the element's words, written out whole.
+constexpr int kKeepMaxWords = 256;  // nl^2 / 32 rounded up: 206 for Q2 (nl = 81)
 __global__ __launch_bounds__(256) void k_keep_emat(int64_t n_cv, int nl, int kwords,
                                                    const double *__restrict__ E,
                                                    const double *__restrict__ thr,
                                                    uint32_t *__restrict__ keep) {
-  const int64_t w = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (w >= n_cv * kwords) return;
-  const int64_t e = w / kwords;
-  const int word = (int)(w - e * kwords);
+  __shared__ uint32_t sk[kKeepMaxWords];
+  const int64_t e = blockIdx.x;
+  if (e >= n_cv) return;
+  const int t = threadIdx.x, nd = nl / 3;
+  for (int w = t; w < kwords; w += 256) sk[w] = 0u;
+  __syncthreads();
   const double th = thr[e];
   const double *Ee = E + e * (int64_t)nl * nl;
-  const int nd = nl / 3;
-  uint32_t m = 0;
-  for (int b = 0; b < 32; ++b) {
-    const int bit = word * 32 + b;  // row-major entry (r, c) = (3 i + x, 3 j + y)
-    if (bit >= nl * nl) break;
-    const int r = bit / nl, c = bit - r * nl;
-    if (fabs(Ee[(3 * r + c % 3) * nd + c / 3]) > th) m |= 1u << b;
+  for (int x = t; x < nl * nl; x += 256) {
+    if (fabs(Ee[x]) > th) {
+      const int row3 = x / nd, j = x - row3 * nd;  // row3 = (3 i + a) 3 + b
+      const int r = row3 / 3, y = row3 - 3 * r;
+      const int bit = r * nl + 3 * j + y;
+      atomicOr(&sk[bit >> 5], 1u << (bit & 31));
+    }
   }
-  keep[w] = m;
+  __syncthreads();
+  for (int w = t; w < kwords; w += 256) keep[e * kwords + w] = sk[w];
 }
 
 // node-patch gather of the rows of buffered element matrices: the Q rows of
@@ -1064,9 +1072,9 @@ hipError_t launch_elast_bdb_mfma_hp(hipStream_t s, int64_t n_cv, int nd, int nq,
 
 hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, const double *E,
                             const double *thr, uint32_t *keep) {
-  const int64_t n = n_cv * kwords;
-  if (n == 0) return hipSuccess;
-  k_keep_emat<<<(unsigned)((n + 255) / 256), 256, 0, s>>>(n_cv, nl, kwords, E, thr, keep);
+  if (n_cv == 0) return hipSuccess;
+  if (kwords > kKeepMaxWords || nl % 3 != 0 || (int64_t)kwords * 32 < (int64_t)nl * nl) return hipErrorInvalidValue;
+  k_keep_emat<<<(unsigned)n_cv, 256, 0, s>>>(n_cv, nl, kwords, E, thr, keep);
   return hipGetLastError();
 }
 
