@@ -608,7 +608,8 @@ static int plan_records(gfx_plan *p, bool with_thr) {
   return GFX_OK;
 }
 
-static int plan_eb_elements(gfx_plan *p, int precision) {
+// the element kernel; with_keep (FP64): it also writes the keep bits
+static int plan_eb_elements(gfx_plan *p, int precision, bool with_keep = false) {
   const gfx_mesh *m = p->mf->mesh;
   gfx_ctx *c = p->ctx;
   EbArgs a{};
@@ -618,6 +619,7 @@ static int plan_eb_elements(gfx_plan *p, int precision) {
   a.ca_c = p->coef[0].value; a.cb_c = p->coef[1].value;
   a.gN = p->d_gN; a.gref = p->d_gref; a.qw = p->d_qw; a.nq = p->nq;
   a.E = p->d_E; a.res = p->d_res; a.thr = p->d_thr; a.err = p->d_err;
+  a.keep = with_keep ? p->d_keep : nullptr; a.kwords = p->kwords;
   HIPCHK(c, launch_nh_element(c->stream, a, precision));
   return GFX_OK;
 }
@@ -800,7 +802,7 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   hipStream_t s = c->stream;
   int r;
   HIPCHK(c, hipMemsetAsync(p->d_err, 0, 4, s));
-  if ((r = p->eb ? plan_eb_elements(p, GFX_PREC_FP64) : plan_records(p, false))) return r;
+  if (!p->eb && (r = plan_records(p, false))) return r;  // element-batched: with the keep bits below
   // incidences
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order); dfree(p->d_desc);
   dfree(p->d_nbr); dfree(p->d_mask); dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals);
@@ -848,8 +850,11 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   a.kwords = p->kwords; a.mwpi = p->mwpi; a.dofs = p->mf->d_dofs; a.rec = p->d_rec;
   a.gref = p->d_gref; a.qw = p->d_qw; a.nq = p->nq; a.thr = p->d_thr; a.keep = p->d_keep;
   a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.err = p->d_err;
-  if (p->eb) HIPCHK(c, launch_keep_emat(s, n_cv, p->nloc, p->kwords, p->d_E, p->d_thr, p->d_keep));
-  else HIPCHK(c, launch_keep(s, p->term, a));
+  if (p->eb) {
+    if ((r = plan_eb_elements(p, GFX_PREC_FP64, true))) return r;
+  } else {
+    HIPCHK(c, launch_keep(s, p->term, a));
+  }
   // patches (the patch kernel addresses presence words and slot pairs with
   // 32-bit word offsets)
   if ((int64_t)p->n_inc * std::max(p->mwpi, nd / 2) >= (int64_t(1) << 32))
@@ -929,8 +934,7 @@ static int plan_validate(gfx_plan *p, bool *eb_fresh) {
   a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot; a.nslots = p->d_nslots;
   a.mask = p->d_mask; a.err = p->d_err;
   if (p->eb) {
-    if ((r = plan_eb_elements(p, GFX_PREC_FP64))) return r;
-    HIPCHK(c, launch_keep_emat(s, m->n_cv, p->nloc, p->kwords, p->d_E, p->d_thr, p->d_keep));
+    if ((r = plan_eb_elements(p, GFX_PREC_FP64, true))) return r;
   } else {
     if ((r = plan_records(p, false))) return r;
     HIPCHK(c, launch_keep(s, p->term, a));

@@ -144,6 +144,8 @@ struct EbArgs {
   double *res;              // n_cv * nl element residuals
   double *thr;              // n_cv thresholds
   int *err;
+  uint32_t *keep;           // FP64 form: when set, the keep bits (row-major, kwords words per element)
+  int kwords;
 };
 
 // Order-0 / order-1 polynomial programs (scalar.hip).  The symbol table of a
@@ -214,8 +216,6 @@ int64_t scan_tmp_elems(int64_t n);
 
 // kernels_eb.hip
 hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision);
-hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, const double *E,
-                            const double *thr, uint32_t *keep);
 hipError_t launch_elast_bdb_mfma(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,
                                  const double *gref, const double *qw, float *E);
 hipError_t launch_elast_bdb_mfma_hp(hipStream_t s, int64_t n_cv, int nd, int nq, const double *rec,

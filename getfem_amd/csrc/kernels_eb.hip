@@ -35,6 +35,7 @@ constexpr int kNd = 27, kNl = 81, kNv = 8;
 //   material ones, as products on the matrix cores in FP64 (FP64 form) or
 //   FP32 (FP32 form), upper tiles only (nh_phase_b_mfma).
 constexpr int kNq = 27;
+constexpr int kKeepWords = (kNl * kNl + 31) / 32;  // keep-bit words per element (206)
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(4 * sizeof(float)))) float f32x4;
@@ -185,9 +186,33 @@ __device__ __forceinline__ void nh_w_store(int lr, int lg, int n0, int n1, int t
 
 // the stage aliases phase A's arrays and hC: written only after the
 // workgroup barrier that follows the last read of them
-template <typename T>
+// this lane's largest |entry| of tiles T0 .. T0 + NT - 1 (K3 added to the
+// (a, a) tiles; the zero pads add nothing)
+template <typename T, int N>
+__device__ __forceinline__ double nh_w_max1(const typename BdbMfma<T>::v4 &acc, const typename BdbMfma<T>::v4 &k3) {
+  double m = 0.0;
+#pragma unroll
+  for (int x = 0; x < 4; ++x) m = fmax(m, (double)fabs(NhTile<N>::diag ? acc[x] + k3[x] : acc[x]));
+  return m;
+}
+template <typename T, int T0, int... U>
+__device__ __forceinline__ double nh_w_max_seq(std::integer_sequence<int, U...>,
+                                               const typename BdbMfma<T>::v4 (&acc)[6],
+                                               const typename BdbMfma<T>::v4 &k3) {
+  double m = 0.0;
+  ((m = fmax(m, nh_w_max1<T, T0 + U>(acc[U], k3))), ...);
+  return m;
+}
+template <typename T, int T0, int NT>
+__device__ __forceinline__ double nh_w_max(const typename BdbMfma<T>::v4 (&acc)[6],
+                                           const typename BdbMfma<T>::v4 &k3) {
+  return nh_w_max_seq<T, T0>(std::make_integer_sequence<int, NT>{}, acc, k3);
+}
+
+template <typename T, bool KEEP>
 __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (*coef)[2], const double (*Mq)[9],
-                                                const T (*hC)[3][kNhN], T *stage, double *red, const EbArgs &a) {
+                                                const T (*hC)[3][kNhN], T *stage, uint32_t *skeep, double *red,
+                                                const EbArgs &a) {
   typedef typename BdbMfma<T>::v4 v4;
   // the wavefront index as a scalar: the branches on it are uniform
   const int w = __builtin_amdgcn_readfirstlane(t / 64), l = t % 64, lr = l % 16, lg = l / 16;
@@ -222,8 +247,23 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (
   else if (w == 1) nh_w_tiles<T, 5, 5>(lr, lg, hC, al, be, acc);
   else if (w == 2) nh_w_tiles<T, 10, 5>(lr, lg, hC, al, be, acc);
   else nh_w_tiles<T, 15, 6>(lr, lg, hC, al, be, acc);
+  // threshold 1e-14 max|K| (all-zero element skipped): over the streamed
+  // entries, or, when the FP64 pass also sets the keep bits while it streams
+  // E (pattern build, re-validation), from the accumulators before any entry
+  // is written
+  constexpr bool kb = KEEP;
+  double mx = 0.0, th = 0.0;
+  if constexpr (kb) {
+    if (w == 0) mx = nh_w_max<T, 0, 5>(acc, k3);
+    else if (w == 1) mx = nh_w_max<T, 5, 5>(acc, k3);
+    else if (w == 2) mx = nh_w_max<T, 10, 5>(acc, k3);
+    else mx = nh_w_max<T, 15, 6>(acc, k3);
+    const double mxe = block_max(mx, red);
+    th = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
+    if (t == 0) a.thr[e] = th;
+    for (int x = t; x < a.kwords; x += kEbThreads) skeep[x] = 0u;
+  }
   T *E = reinterpret_cast<T *>(a.E) + e * kNl * kNl;
-  double mx = 0.0;
   // two stages: node rows 0-15, then 16-26; the trash row (a slot per lane)
   // behind the larger one.  (FP32 would fit one stage of all 27 node rows;
   // that variant gave wrong values with this compiler at -O3, with a fill of
@@ -244,13 +284,23 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (
     T *dst = E + 3 * n0 * kNl;
     for (int x = t; x < len; x += kEbThreads) {
       const T v = stage[x];
-      mx = fmax(mx, (double)fabs(v));
       __builtin_nontemporal_store(v, dst + x);
+      if constexpr (!kb) {
+        mx = fmax(mx, (double)fabs(v));
+      } else if (fabs((double)v) > th) {  // row-major bit (r, c) = (3 i + a, 3 j + b)
+        const int X = 3 * n0 * kNl + x, row3 = X / kNd, j = X - row3 * kNd, r = row3 / 3;
+        const int bit = r * kNl + 3 * j + (row3 - 3 * r);
+        atomicOr(&skeep[bit >> 5], 1u << (bit & 31));
+      }
     }
   }
-  // threshold 1e-14 max|K| (all-zero element skipped)
-  const double mxe = block_max(mx, red);
-  if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
+  if constexpr (!kb) {
+    const double mxe = block_max(mx, red);
+    if (t == 0) a.thr[e] = mxe > 0.0 ? 1e-14 * mxe : INFINITY;
+  } else {
+    __syncthreads();
+    for (int x = t; x < a.kwords; x += kEbThreads) a.keep[e * a.kwords + x] = skeep[x];
+  }
 }
 
 // phase A's per-element arrays; the material gradients gq (phase A) and
@@ -274,7 +324,7 @@ union NhLds {
   T stage[3 * 16 * kNl + 64];  // the slab and the trash row
 };
 
-template <typename Real>
+template <typename Real, bool KEEP>
 __global__ __launch_bounds__(kEbThreads, 4) void k_nh_element(EbArgs a) {
   __shared__ NhLds<Real> L;
   double(&X)[kNv][3] = L.P.A.X;
@@ -289,6 +339,7 @@ __global__ __launch_bounds__(kEbThreads, 4) void k_nh_element(EbArgs a) {
   double(&gq)[kNq][kNd][3] = L.P.G.gq;
   Real(&hC)[kNhQ][3][kNhN] = L.P.G.hC;  // phase B: the spatial gradients in the fragments' layout
   __shared__ double red[kEbThreads];
+  __shared__ uint32_t skeep[KEEP ? kKeepWords : 1];  // the element's keep bits
   const int64_t e = blockIdx.x;
   if (e >= a.n_cv || a.nq != kNq) return;
   const int t = threadIdx.x;
@@ -454,37 +505,7 @@ __global__ __launch_bounds__(kEbThreads, 4) void k_nh_element(EbArgs a) {
       for (int c = 0; c < 3; ++c) hC[q][c][i] = hv[k][c];
   }
   __syncthreads();
-  nh_phase_b_mfma<Real>(t, e, cf, Mq, hC, L.stage, red, a);
-}
-
-// keep bits of element matrices held in the buffer: bit r*nl + c (row-major
-// entry (r, c) = (3 i + x, 3 j + y)) set where |K| > thr_e.  E is node-blocked
-// (K[(i,x),(j,y)] at ((3 i + x) 3 + y) nd + j): one workgroup per element
-// reads it in order (coalesced) and sets the row-major bits in an LDS copy of
-// the element's words, written out whole.
-constexpr int kKeepMaxWords = 256;  // nl^2 / 32 rounded up: 206 for Q2 (nl = 81)
-__global__ __launch_bounds__(256) void k_keep_emat(int64_t n_cv, int nl, int kwords,
-                                                   const double *__restrict__ E,
-                                                   const double *__restrict__ thr,
-                                                   uint32_t *__restrict__ keep) {
-  __shared__ uint32_t sk[kKeepMaxWords];
-  const int64_t e = blockIdx.x;
-  if (e >= n_cv) return;
-  const int t = threadIdx.x, nd = nl / 3;
-  for (int w = t; w < kwords; w += 256) sk[w] = 0u;
-  __syncthreads();
-  const double th = thr[e];
-  const double *Ee = E + e * (int64_t)nl * nl;
-  for (int x = t; x < nl * nl; x += 256) {
-    if (fabs(Ee[x]) > th) {
-      const int row3 = x / nd, j = x - row3 * nd;  // row3 = (3 i + a) 3 + b
-      const int r = row3 / 3, y = row3 - 3 * r;
-      const int bit = r * nl + 3 * j + y;
-      atomicOr(&sk[bit >> 5], 1u << (bit & 31));
-    }
-  }
-  __syncthreads();
-  for (int w = t; w < kwords; w += 256) keep[e * kwords + w] = sk[w];
+  nh_phase_b_mfma<Real, KEEP>(t, e, cf, Mq, hC, L.stage, skeep, red, a);
 }
 
 // node-patch gather of the rows of buffered element matrices: the Q rows of
@@ -916,10 +937,15 @@ __global__ __launch_bounds__(256) void k_gather_vec(int64_t n_nodes, int nd, int
 hipError_t launch_nh_element(hipStream_t s, const EbArgs &a, int precision) {
   if (a.n_cv == 0) return hipSuccess;
   if (a.nq != kNq) return hipErrorInvalidValue;
-  if (precision == GFX_PREC_FP32)
-    k_nh_element<float><<<(unsigned)a.n_cv, kEbThreads, 0, s>>>(a);
-  else
-    k_nh_element<double><<<(unsigned)a.n_cv, kEbThreads, 0, s>>>(a);
+  if (precision == GFX_PREC_FP32) {
+    if (a.keep) return hipErrorInvalidValue;  // keep bits come from the FP64 pass
+    k_nh_element<float, false><<<(unsigned)a.n_cv, kEbThreads, 0, s>>>(a);
+  } else if (a.keep) {
+    if (a.kwords != kKeepWords) return hipErrorInvalidValue;
+    k_nh_element<double, true><<<(unsigned)a.n_cv, kEbThreads, 0, s>>>(a);
+  } else {
+    k_nh_element<double, false><<<(unsigned)a.n_cv, kEbThreads, 0, s>>>(a);
+  }
   return hipGetLastError();
 }
 
@@ -1070,13 +1096,6 @@ hipError_t launch_elast_bdb_mfma_hp(hipStream_t s, int64_t n_cv, int nd, int nq,
   return hipGetLastError();
 }
 
-hipError_t launch_keep_emat(hipStream_t s, int64_t n_cv, int nl, int kwords, const double *E,
-                            const double *thr, uint32_t *keep) {
-  if (n_cv == 0) return hipSuccess;
-  if (kwords > kKeepMaxWords || nl % 3 != 0 || (int64_t)kwords * 32 < (int64_t)nl * nl) return hipErrorInvalidValue;
-  k_keep_emat<<<(unsigned)n_cv, 256, 0, s>>>(n_cv, nl, kwords, E, thr, keep);
-  return hipGetLastError();
-}
 
 hipError_t launch_gather_emat(hipStream_t s, const PatchArgs &a0, const void *E, bool e_fp32, const double *thr,
                               int acc_cap, int group_inc) {
