@@ -182,6 +182,56 @@ struct ScalarArgs {
   int *err;
 };
 
+// Order-2 polynomial programs (polymat.hip): the element matrices of one
+// block (test variable x trial variable), thresholded per element as
+// ga_instruction_matrix_assembly_mf_mf does, then the kept entries of every
+// block as (row, column) keys, sorted and summed in element order into a CSR.
+constexpr int kPolyMaxOut = 144;       // 16 q q2 for q, q2 <= 3
+constexpr int kPolyMaxNd = 20;         // base functions of one field (P3)
+constexpr int kPolyMaxEntries = 4096;  // entries of one block's element matrix
+struct PolyMatArgs {
+  int64_t n_cv;
+  const double *pts;
+  const int32_t *cvx;        // affine tetrahedra
+  int nq;
+  const double *qw;
+  int nf;
+  ScalarField f[kScalarMaxFields];
+  int nconst, const_off;
+  const double *consts;
+  int vt, vs;                // fields of the test and of the trial variable
+  int nout;
+  const double *mcoef;       // monomials, sorted by output
+  const int32_t *mptr;       // nout + 1: the monomials of output o
+  const int32_t *moff;       // offsets into msym
+  const int32_t *msym;
+  double *elem;              // n_cv * nr * nc, row r = i qt + a, column c = j qs + b
+  double *thr;               // n_cv: 1e-14 max|elem| (INFINITY: a zero element)
+  int64_t *cnt;              // n_cv: entries above the threshold
+  int *err;
+};
+struct PolyEmitArgs {
+  int64_t n_cv;
+  int ndt, nds, qt, qs;
+  const int32_t *dt, *ds;    // dof tables of the two fields (component 0 of each node)
+  int64_t first_t, first_s;  // intervals of the two variables
+  int64_t ncols;             // key = row * ncols + column
+  const double *elem, *thr;
+  const int64_t *off;        // n_cv: first slot of each element's entries
+  uint64_t *keys;
+  double *vals;
+};
+hipError_t launch_poly_mat(hipStream_t s, const PolyMatArgs &a);
+hipError_t launch_poly_emit(hipStream_t s, const PolyEmitArgs &a);
+// temporary storage of the stable key sort of n entries (bytes)
+size_t poly_sort_bytes(int64_t n, int end_bit);
+// sort (stable, by key) then sum equal keys in their order: n entries ->
+// CSR with nrows rows (ip, ix, v: at least n entries); *d_nnz receives the
+// entry count on the device.  work: 3 n + 1 + scan_tmp_elems(n) int64.
+hipError_t poly_to_csr(hipStream_t s, void *tmp, size_t tmp_bytes, uint64_t *keys, double *vals, uint64_t *keys2,
+                       double *vals2, int64_t n, int end_bit, int64_t nrows, int64_t ncols, int64_t *work,
+                       int64_t *ip, int32_t *ix, double *v, int64_t *d_nnz);
+
 // reference counting of the objects above (api.cpp)
 void ref(gfx_ctx *c);
 void unref(gfx_ctx *c);

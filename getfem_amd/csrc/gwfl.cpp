@@ -42,9 +42,9 @@ void padd_to(Poly &a, const Poly &b, double s = 1.0) {
 }
 // Truncation of the polynomials at a variable held at zero: the values and
 // gradients of a variable that appears inside Inv(...) are expanded around 0
-// (Inv is a Neumann series there), and an order-0 or order-1 assembly at that
-// variable's zero value needs only the monomials of degree <= 1 in its
-// symbols.  `zsym[s]` marks those symbols; monomials of higher degree in them
+// (Inv is a Neumann series there), and an order-k assembly (k = 0, 1, 2) at
+// that variable's zero value needs only the monomials of degree <= max(k, 1)
+// in its symbols.  `zsym[s]` marks those symbols; monomials of higher degree in them
 // are dropped by every product.
 struct Trunc {
   std::vector<char> zsym;
@@ -465,7 +465,7 @@ struct Parser {
     if (C.trunc.max_deg < 0) {
       bool none = true;
       for (auto &p : nn.v) none = none && p.empty();
-      if (!none) fail("Inv of a matrix of variables is supported in order-0 and order-1 assemblies only");
+      if (!none) fail("Inv of a matrix of variables is supported by the polynomial route only");
     }
     // sum_{k <= deg} (-A N)^k A
     Tensor term = A, total = A;
@@ -839,19 +839,23 @@ struct TruncGuard {
 }  // namespace
 
 bool gwfl_compile_scalar(const std::vector<std::string> &exprs, const std::map<std::string, GwflVar> &vars,
-                         int order, const std::string &var, GwflScalarProgram &out, std::string &err) {
+                         int order, const std::string &var, GwflScalarProgram &out, std::string &err,
+                         const std::string &var2) {
   out = GwflScalarProgram();
   out.order = order;
   out.var = var;
+  out.var2 = var2;
   try {
-    if (order != 0 && order != 1) fail("the scalar route assembles orders 0 and 1 only");
-    int q = 1;
-    if (order == 1) {
-      auto it = vars.find(var);
+    if (order < 0 || order > 2) fail("the polynomial route assembles orders 0, 1 and 2");
+    int q = 1, q2 = 1;
+    auto fem_var = [&](const std::string &n) {
+      auto it = vars.find(n);
       if (it == vars.end() || !it->second.is_fem || it->second.is_constant)
-        fail("order-1 assembly with respect to " + var + ", which is not a fem variable");
-      q = it->second.qdim;
-    }
+        fail("order-" + std::to_string(order) + " assembly with respect to " + n + ", which is not a fem variable");
+      return it->second.qdim;
+    };
+    if (order >= 1) q = fem_var(var);
+    if (order == 2) q2 = fem_var(var2);
     for (const std::string &ex : exprs) {
       std::smatch m;
       std::string sq;
@@ -868,8 +872,20 @@ bool gwfl_compile_scalar(const std::vector<std::string> &exprs, const std::map<s
     Ctx C;
     C.vars = &vars;
     C.zero_vars = C0.zero_vars;
-    C.trunc.max_deg = C.zero_vars.empty() ? -1 : 1;
+    C.trunc.max_deg = C.zero_vars.empty() ? -1 : std::max(order, 1);
     TruncGuard guard(&C.trunc);
+    // the direction of a derivative with respect to variable v: its value and
+    // gradient symbols to the test (T, GT) or second test (S, GS) symbols
+    auto direction = [&](const std::string &v, int qv, const char *tv, const char *tg) {
+      std::map<int, int> dir;
+      for (int a = 0; a < qv; ++a) {
+        const std::string sa = qv == 1 ? "" : std::to_string(a);
+        dir[C.sym(v + ":V_" + sa)] = C.sym(v + ":" + tv + "_" + sa);
+        for (int b = 0; b < 3; ++b)
+          dir[C.sym(v + ":G_" + sa + std::to_string(b))] = C.sym(v + ":" + tg + "_" + sa + std::to_string(b));
+      }
+      return dir;
+    };
     Poly total;
     for (const std::string &ex : exprs) {
       Parser P(ex, C);
@@ -880,29 +896,11 @@ bool gwfl_compile_scalar(const std::vector<std::string> &exprs, const std::map<s
       Poly p = t.v[0];
       const int o = test_order(p, C);
       if (o > order) continue;  // a tree of a higher-order assembly only
-      if (order == 1) {
-        if (o == 0) {  // the potential's derivative in the direction Test_var
-          std::map<int, int> dir;
-          for (int a = 0; a < q; ++a) {
-            const std::string sa = q == 1 ? "" : std::to_string(a);
-            dir[C.sym(var + ":V_" + sa)] = C.sym(var + ":T_" + sa);
-            for (int b = 0; b < 3; ++b)
-              dir[C.sym(var + ":G_" + sa + std::to_string(b))] = C.sym(var + ":GT_" + sa + std::to_string(b));
-          }
-          p = pderiv(p, dir);
-        } else {  // an order-1 expression joins the block of its own test variable
-          std::string tv;
-          for (auto &kv : p)
-            for (int sy : kv.first) {
-              std::string v, tag;
-              split_symbol(C.symnames[sy], v, tag);
-              if (tag == "T" || tag == "GT") tv = v;
-            }
-          if (tv != var) continue;
-        }
-      } else if (o != 0) {
-        continue;
-      }
+      // the potential's derivative in the direction Test_var, then (order 2)
+      // the derivative in the direction Test2_var2; the monomials of other
+      // test variables belong to other blocks and are dropped below
+      if (order >= 1 && o == 0) p = pderiv(p, direction(var, q, "T", "GT"));
+      if (order == 2 && o <= 1) p = pderiv(p, direction(var2, q2, "S", "GS"));
       padd_to(total, p);
     }
     // at the zero value of the variables Inv expanded, their symbols vanish
@@ -912,22 +910,31 @@ bool gwfl_compile_scalar(const std::vector<std::string> &exprs, const std::map<s
       it = z ? total.erase(it) : std::next(it);
     }
     out.zero_vars.assign(C.zero_vars.begin(), C.zero_vars.end());
-    out.nout = order == 0 ? 1 : 4 * q;
+    out.nout = order == 0 ? 1 : (order == 1 ? 4 * q : 16 * q * q2);
     std::map<int, int> data;  // symbol id -> index in out.syms
     for (auto &kv : total) {
       GwflScalarProgram::Mono mo;
       mo.coef = kv.second;
-      mo.out = order == 0 ? 0 : -1;
+      int s1 = order == 0 ? 0 : -1, s2 = order == 2 ? -1 : 0;  // test and second test slots
+      bool other = false;  // a test function of another variable: another block
       for (int sy : kv.first) {
         const std::string &nm = C.symnames[sy];
         std::string v, tag;
         split_symbol(nm, v, tag);
-        if (tag == "S" || tag == "GS") fail("a second test function in an order-0/1 expression");
-        if (tag == "T" || tag == "GT") {
-          if (order == 0 || v != var || mo.out >= 0) fail("term not linear in the test function of " + var);
+        if ((tag == "S" || tag == "GS") && order < 2) fail("a second test function in an order-0/1 expression");
+        if (tag == "T" || tag == "GT" || tag == "S" || tag == "GS") {
+          const bool second = tag == "S" || tag == "GS";
+          const int qv = second ? q2 : q;
+          int &slot = second ? s2 : s1;
+          if (order == 0) fail("a test function in a potential");
+          if (v != (second ? var2 : var)) {
+            other = true;
+            continue;
+          }
+          if (slot >= 0) fail("term not linear in the test function of " + v);
           const std::string comp = nm.substr(nm.find('_', nm.find(':')) + 1);
-          const int a = q == 1 ? 0 : comp[0] - '0';
-          mo.out = tag == "T" ? a : q + 3 * a + (comp.back() - '0');
+          const int a = qv == 1 ? 0 : comp[0] - '0';
+          slot = (tag == "T" || tag == "S") ? a : qv + 3 * a + (comp.back() - '0');
           continue;
         }
         auto it = data.find(sy);
@@ -937,7 +944,8 @@ bool gwfl_compile_scalar(const std::vector<std::string> &exprs, const std::map<s
         }
         mo.s.push_back(it->second);
       }
-      if (mo.out < 0) continue;  // order 1: a term without the test function (zero)
+      if (other || s1 < 0 || s2 < 0) continue;  // another block's term, or one without a test function (zero)
+      mo.out = order == 2 ? s1 * 4 * q2 + s2 : s1;
       out.monos.push_back(mo);
     }
     return true;

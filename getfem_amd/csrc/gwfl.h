@@ -56,11 +56,18 @@ bool gwfl_compile_order2(const std::vector<std::string> &exprs,
 // variable `var` (qdim q): the derivative's coefficients of Test_var
 // (outputs 0..q-1) and of Grad_Test_var (output q + 3a + b); order-0
 // expressions are differentiated, order-1 expressions with Test_var join as
-// they are.  Variables inside Inv(...) are expanded around zero (zero_vars:
-// their values must be zero at assembly).
+// they are.  order 2, the block of test variable `var` (qdim q) and trial
+// variable `var2` (qdim q2): the coefficients of the products of a Test_var
+// slot (0..q-1 the value, q + 3a + b the gradient) and a Test2_var2 slot
+// (the same numbering with q2), output slot1 * 4 q2 + slot2; order-0
+// expressions are differentiated twice, order-1 expressions in Test_var once
+// along Test2_var2, and order-2 expressions in Test_var and Test2_var2 join
+// as they are (ga_workspace::add_tree, src/getfem_generic_assembly_
+// workspace.cc:439-543).  Variables inside Inv(...) are expanded around zero
+// (zero_vars: their values must be zero at assembly).
 struct GwflScalarProgram {
   int order = 0;
-  std::string var;
+  std::string var, var2;
   int nout = 1;
   std::vector<std::string> syms;
   struct Mono {
@@ -72,6 +79,7 @@ struct GwflScalarProgram {
   std::vector<std::string> zero_vars;
 };
 bool gwfl_compile_scalar(const std::vector<std::string> &exprs, const std::map<std::string, GwflVar> &vars,
-                         int order, const std::string &var, GwflScalarProgram &out, std::string &err);
+                         int order, const std::string &var, GwflScalarProgram &out, std::string &err,
+                         const std::string &var2 = "");
 
 }  // namespace gfx

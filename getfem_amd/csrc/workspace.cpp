@@ -44,6 +44,8 @@ struct WsVar {
   gfx_mesh_fem *mf = nullptr;
   int64_t first = 0;
   std::vector<double> V;
+  double *d_val = nullptr;      // values given on the device (polynomial route)
+  bool dev = false;             // d_val, not V, holds the current values
 };
 
 struct WsExpr {
@@ -59,6 +61,13 @@ struct ScProg {
   int nm = 0;
   double *d_coef = nullptr;
   int32_t *d_off = nullptr, *d_sym = nullptr, *d_out = nullptr;
+  int32_t *d_mptr = nullptr;    // order 2: monomials of each output (sorted by output)
+};
+
+struct ScBlock {                // an order-2 block: test variable x trial variable
+  std::string tv, sv;
+  ScProg p;
+  int vt = 0, vs = 0;           // their fields
 };
 
 struct ScField {
@@ -103,6 +112,23 @@ struct ScalarRoute {
   double *d_elem = nullptr, *d_part = nullptr, *d_sum = nullptr, *d_V = nullptr;
   int *d_err = nullptr;
   int64_t nvec = 0;
+  // order 2 (polymat.hip): the blocks, or why they do not compile
+  std::vector<ScBlock> blocks;
+  std::string err2;
+  double *d_pelem = nullptr, *d_thr = nullptr;
+  int64_t *d_cnt = nullptr, *d_off = nullptr, *d_scan = nullptr, *d_nnz = nullptr;
+  // kept entries (keys, values and their sorted copies), sort and run work
+  int64_t cap = 0;
+  uint64_t *d_keys = nullptr, *d_keys2 = nullptr;
+  double *d_vals = nullptr, *d_vals2 = nullptr;
+  int64_t *d_work = nullptr;
+  void *d_sort = nullptr;
+  size_t sort_cap = 0;
+  // the assembled matrix: nvec x nvec CSR
+  int64_t *d_ip = nullptr;
+  int32_t *d_ix = nullptr;
+  double *d_v = nullptr;
+  int64_t nnz = -1;
 };
 
 struct gfx_workspace {
@@ -141,6 +167,11 @@ struct gfx_workspace {
   const int64_t *ext_ptr = nullptr;
   const int32_t *ext_ind = nullptr;
   double *ext_vals = nullptr;
+  // solves and dof constraints on the workspace's matrix
+  double *d_cg = nullptr, *d_go = nullptr;
+  uint8_t *d_fixed = nullptr;
+  int *d_cerr = nullptr;
+  int64_t cg_n = 0, con_n = 0;
 };
 
 namespace {
@@ -438,7 +469,7 @@ int setup(gfx_workspace *w) {
 
 // ------------------------------------------------ the order-0 / order-1 route
 void sc_free_prog(ScProg &p) {
-  wfree(p.d_coef); wfree(p.d_off); wfree(p.d_sym); wfree(p.d_out);
+  wfree(p.d_coef); wfree(p.d_off); wfree(p.d_sym); wfree(p.d_out); wfree(p.d_mptr);
   p.nm = 0;
 }
 
@@ -449,6 +480,12 @@ void sc_release(ScalarRoute *sc) {
   for (auto &v : sc->vars) { sc_free_prog(v.p); wfree(v.d_inc_ptr); wfree(v.d_inc); }
   wfree(sc->d_consts); wfree(sc->d_qw); wfree(sc->d_elem); wfree(sc->d_part); wfree(sc->d_sum);
   wfree(sc->d_V); wfree(sc->d_err);
+  for (auto &b : sc->blocks) sc_free_prog(b.p);
+  wfree(sc->d_pelem); wfree(sc->d_thr); wfree(sc->d_cnt); wfree(sc->d_off); wfree(sc->d_scan);
+  wfree(sc->d_nnz); wfree(sc->d_keys); wfree(sc->d_keys2); wfree(sc->d_vals); wfree(sc->d_vals2);
+  wfree(sc->d_work); wfree(sc->d_sort); wfree(sc->d_ip); wfree(sc->d_ix); wfree(sc->d_v);
+  sc->blocks.clear(); sc->err2.clear();
+  sc->cap = 0; sc->sort_cap = 0; sc->nnz = -1;
   sc->fields.clear(); sc->vars.clear(); sc->consts.clear();
   sc->p0 = ScProg();
   sc->mim = nullptr;
@@ -499,10 +536,11 @@ int sc_build(gfx_workspace *w) {
   if (m->gt != GT_TET_P1) return bail(GFX_E_UNSUPPORTED, "the order-0/1 route needs GT_PK(3,1) tetrahedra");
   sc->mim = mim;
   const std::map<std::string, GwflVar> gv = gwfl_vars(w);
-  std::vector<std::string> all, deriv;  // deriv: expressions derived at least once (der_order >= 1)
+  std::vector<std::string> all, deriv, deriv2;  // derived at least once (der_order >= 1), twice (2)
   for (auto &e : w->exprs) {
     all.push_back(e.expr);
     if (e.der_order >= 1) deriv.push_back(e.expr);
+    if (e.der_order == 2) deriv2.push_back(e.expr);
   }
   std::string err;
   if (!gwfl_compile_scalar(all, gv, 0, "", sc->p0.prog, err)) return bail(GFX_E_UNSUPPORTED, err);
@@ -513,6 +551,29 @@ int sc_build(gfx_workspace *w) {
     if (!gwfl_compile_scalar(deriv, gv, 1, kv.first, v.p.prog, err)) return bail(GFX_E_UNSUPPORTED, err);
     sc->vars.push_back(v);
   }
+  // order 2: one block per (test variable, trial variable) with terms; a
+  // block that does not compile leaves orders 0 and 1 usable
+  if (!deriv2.empty())
+    for (auto &kt : w->vars) {
+      if (kt.second.kind != VAR_FEM || !sc->err2.empty()) continue;
+      for (auto &ks : w->vars) {
+        if (ks.second.kind != VAR_FEM) continue;
+        ScBlock b;
+        b.tv = kt.first;
+        b.sv = ks.first;
+        if (!gwfl_compile_scalar(deriv2, gv, 2, b.tv, b.p.prog, err, b.sv)) {
+          sc->err2 = err;
+          sc->blocks.clear();
+          break;
+        }
+        if (b.p.prog.monos.empty()) continue;
+        std::stable_sort(b.p.prog.monos.begin(), b.p.prog.monos.end(),
+                         [](const GwflScalarProgram::Mono &x, const GwflScalarProgram::Mono &y) {
+                           return x.out < y.out;
+                         });
+        sc->blocks.push_back(b);
+      }
+    }
   // the fields and constants the programs read, and the variables
   std::vector<std::string> fnames;
   auto need_field = [&](const std::string &n) {
@@ -541,6 +602,11 @@ int sc_build(gfx_workspace *w) {
   for (auto &v : sc->vars) {
     if (!scan(v.p.prog)) return bail(GFX_E_UNSUPPORTED, err);
     if (!v.p.prog.monos.empty()) need_field(v.name);
+  }
+  for (auto &b : sc->blocks) {
+    if (!scan(b.p.prog)) return bail(GFX_E_UNSUPPORTED, err);
+    need_field(b.tv);
+    need_field(b.sv);
   }
   if (fnames.empty()) need_field(sc->vars.empty() ? w->vars.begin()->first : sc->vars[0].name);
   if ((int)fnames.size() > kScalarMaxFields)
@@ -615,6 +681,29 @@ int sc_build(gfx_workspace *w) {
   };
   int r;
   if ((r = map_prog(sc->p0))) return r;
+  int64_t pelem = 0;
+  for (auto &b : sc->blocks) {
+    if ((r = map_prog(b.p))) return r;
+    b.vt = field_index(b.tv);
+    b.vs = field_index(b.sv);
+    const gfx_mesh_fem *ft = sc->fields[b.vt].mf, *fs = sc->fields[b.vs].mf;
+    if (ft->fem.nd > kPolyMaxNd || fs->fem.nd > kPolyMaxNd ||
+        (int64_t)ft->fem.nd * ft->qdim * fs->fem.nd * fs->qdim > kPolyMaxEntries || b.p.prog.nout > kPolyMaxOut)
+      return bail(GFX_E_UNSUPPORTED, "the order-2 block " + b.tv + " x " + b.sv +
+                                         " is larger than the polynomial route's element matrices");
+    std::vector<int32_t> mptr((size_t)b.p.prog.nout + 1, 0);
+    for (auto &mo : b.p.prog.monos) mptr[(size_t)mo.out + 1]++;
+    for (size_t o = 0; o + 1 < mptr.size(); ++o) mptr[o + 1] += mptr[o];
+    if (upload(&b.p.d_mptr, mptr) != hipSuccess) return bail(GFX_E_OOM, "order-2 route programs");
+    pelem = std::max(pelem, m->n_cv * (int64_t)ft->fem.nd * ft->qdim * fs->fem.nd * fs->qdim);
+  }
+  if (!sc->blocks.empty() &&
+      (hipMalloc((void **)&sc->d_pelem, 8 * (size_t)std::max<int64_t>(pelem, 1)) != hipSuccess ||
+       hipMalloc((void **)&sc->d_thr, 8 * (size_t)std::max<int64_t>(m->n_cv, 1)) != hipSuccess ||
+       hipMalloc((void **)&sc->d_cnt, 8 * (size_t)std::max<int64_t>(m->n_cv, 1)) != hipSuccess ||
+       hipMalloc((void **)&sc->d_off, 8 * (size_t)std::max<int64_t>(m->n_cv, 1)) != hipSuccess ||
+       hipMalloc((void **)&sc->d_scan, 8 * (size_t)std::max<int64_t>(scan_tmp_elems(m->n_cv), 1)) != hipSuccess))
+    return bail(GFX_E_OOM, "order-2 route buffers");
   int64_t elem = m->n_cv, nvec = 0;
   hipStream_t s = w->ctx->stream;
   for (auto &v : sc->vars) {
@@ -650,6 +739,176 @@ int sc_build(gfx_workspace *w) {
   return GFX_OK;
 }
 
+// the current values of a variable on the host (a device copy is read back)
+int host_values(gfx_workspace *w, const std::string &n, std::vector<double> &out) {
+  const WsVar &v = w->vars.at(n);
+  out = v.V;
+  if (v.dev) {
+    WHIP(w, hipStreamSynchronize(w->ctx->stream));
+    WHIP(w, hipMemcpy(out.data(), v.d_val, 8 * out.size(), hipMemcpyDeviceToHost));
+  }
+  return GFX_OK;
+}
+
+// variables that Inv(...) expanded around zero must be zero
+bool sc_zero_ok(gfx_workspace *w, const GwflScalarProgram &p) {
+  for (const std::string &z : p.zero_vars) {
+    std::vector<double> V;
+    if (host_values(w, z, V)) return false;
+    for (double x : V)
+      if (x != 0.0) {
+        wfail(w, GFX_E_UNSUPPORTED, "Inv(...) of " + z + " is expanded around " + z + " = 0, but " + z +
+                                        " is not zero");
+        return false;
+      }
+  }
+  return true;
+}
+
+// the fields' values (from the host, or device to device), the constants,
+// the error flag cleared
+int sc_upload(gfx_workspace *w) {
+  ScalarRoute *sc = w->sc;
+  hipStream_t s = w->ctx->stream;
+  for (auto &f : sc->fields) {
+    const WsVar &v = w->vars.at(f.name);
+    if (v.dev)
+      WHIP(w, hipMemcpyAsync(f.d_U, v.d_val, 8 * v.V.size(), hipMemcpyDeviceToDevice, s));
+    else
+      WHIP(w, hipMemcpyAsync(f.d_U, v.V.data(), 8 * v.V.size(), hipMemcpyHostToDevice, s));
+  }
+  std::vector<double> cv;
+  for (const std::string &n : sc->consts) cv.push_back(w->vars.at(n).V[0]);
+  if (!cv.empty()) WHIP(w, hipMemcpyAsync(sc->d_consts, cv.data(), 8 * cv.size(), hipMemcpyHostToDevice, s));
+  WHIP(w, hipMemsetAsync(sc->d_err, 0, 4, s));
+  // the host buffers above are read before they can change
+  WHIP(w, hipStreamSynchronize(s));
+  return GFX_OK;
+}
+
+int sc_grow(gfx_workspace *w, int64_t need) {
+  ScalarRoute *sc = w->sc;
+  if (need <= sc->cap) return GFX_OK;
+  const int64_t cap = std::max<int64_t>(need, 2 * sc->cap);
+  hipStream_t s = w->ctx->stream;
+  WHIP(w, hipStreamSynchronize(s));
+  uint64_t *k = nullptr, *k2 = nullptr;
+  double *v = nullptr, *v2 = nullptr, *vv = nullptr;
+  int64_t *wk = nullptr;
+  int32_t *ix = nullptr;
+  const size_t n8 = 8 * (size_t)cap;
+  if (hipMalloc((void **)&k, n8) != hipSuccess || hipMalloc((void **)&k2, n8) != hipSuccess ||
+      hipMalloc((void **)&v, n8) != hipSuccess || hipMalloc((void **)&v2, n8) != hipSuccess ||
+      hipMalloc((void **)&vv, n8) != hipSuccess || hipMalloc((void **)&ix, 4 * (size_t)cap) != hipSuccess ||
+      hipMalloc((void **)&wk, 8 * (size_t)(3 * cap + 1 + scan_tmp_elems(cap))) != hipSuccess) {
+    wfree(k); wfree(k2); wfree(v); wfree(v2); wfree(vv); wfree(ix); wfree(wk);
+    return wfail(w, GFX_E_OOM, "order-2 route entries");
+  }
+  // the entries emitted so far move with the buffers
+  if (sc->cap > 0) {
+    WHIP(w, hipMemcpy(k, sc->d_keys, 8 * (size_t)sc->cap, hipMemcpyDeviceToDevice));
+    WHIP(w, hipMemcpy(v, sc->d_vals, 8 * (size_t)sc->cap, hipMemcpyDeviceToDevice));
+  }
+  wfree(sc->d_keys); wfree(sc->d_keys2); wfree(sc->d_vals); wfree(sc->d_vals2); wfree(sc->d_v);
+  wfree(sc->d_ix); wfree(sc->d_work);
+  sc->d_keys = k; sc->d_keys2 = k2; sc->d_vals = v; sc->d_vals2 = v2; sc->d_v = vv; sc->d_ix = ix;
+  sc->d_work = wk;
+  sc->cap = cap;
+  sc->nnz = -1;
+  return GFX_OK;
+}
+
+// assembly(2) on the polynomial route: every block's element matrices, their
+// entries above the element threshold as keys in element order, then one
+// sort and sum into the nvec x nvec matrix (tested dofs x trial dofs at the
+// variables' intervals)
+int sc_run2(gfx_workspace *w) {
+  if (!w->sc) w->sc = new (std::nothrow) ScalarRoute();
+  if (!w->sc) return wfail(w, GFX_E_OOM, "workspace");
+  int r;
+  if ((r = sc_build(w))) return r;
+  ScalarRoute *sc = w->sc;
+  if (!sc->err2.empty()) return wfail(w, GFX_E_UNSUPPORTED, sc->err2);
+  gfx_mesh *m = sc->mim->mesh;
+  hipStream_t s = w->ctx->stream;
+  for (auto &b : sc->blocks)
+    if (!sc_zero_ok(w, b.p.prog)) return GFX_E_UNSUPPORTED;
+  if ((r = sc_upload(w))) return r;
+  const int64_t n = sc->nvec;
+  if (n > INT32_MAX) return wfail(w, GFX_E_UNSUPPORTED, "more than 2^31 dofs");
+  if (!sc->d_ip) {
+    if (hipMalloc((void **)&sc->d_ip, 8 * (size_t)(n + 1)) != hipSuccess ||
+        hipMalloc((void **)&sc->d_nnz, 8) != hipSuccess)
+      return wfail(w, GFX_E_OOM, "order-2 route matrix");
+  }
+  PolyMatArgs a{};
+  a.n_cv = m->n_cv; a.pts = m->d_pts; a.cvx = m->d_cvx;
+  a.nq = sc->mim->rule.n(); a.qw = sc->d_qw;
+  a.nf = (int)sc->fields.size();
+  for (int i = 0; i < a.nf; ++i) {
+    const ScField &f = sc->fields[i];
+    a.f[i] = ScalarField{f.mf->d_dofs, f.d_U, f.d_phi, f.d_gref, f.mf->fem.nd, f.mf->qdim, f.off};
+  }
+  a.nconst = (int)sc->consts.size(); a.const_off = sc->const_off; a.consts = sc->d_consts;
+  a.elem = sc->d_pelem; a.thr = sc->d_thr; a.cnt = sc->d_cnt; a.err = sc->d_err;
+  int64_t total = 0;
+  for (auto &b : sc->blocks) {
+    a.vt = b.vt; a.vs = b.vs;
+    a.nout = b.p.prog.nout;
+    a.mcoef = b.p.d_coef; a.mptr = b.p.d_mptr; a.moff = b.p.d_off; a.msym = b.p.d_sym;
+    WHIP(w, launch_poly_mat(s, a));
+    WHIP(w, scan_exclusive_i64(s, sc->d_cnt, sc->d_off, m->n_cv, sc->d_scan));
+    int64_t tail[2] = {0, 0};
+    if (m->n_cv > 0) {
+      WHIP(w, hipMemcpyAsync(&tail[0], sc->d_off + m->n_cv - 1, 8, hipMemcpyDeviceToHost, s));
+      WHIP(w, hipMemcpyAsync(&tail[1], sc->d_cnt + m->n_cv - 1, 8, hipMemcpyDeviceToHost, s));
+    }
+    WHIP(w, hipStreamSynchronize(s));
+    const int64_t nb = tail[0] + tail[1];
+    if (total + nb > INT32_MAX) return wfail(w, GFX_E_UNSUPPORTED, "more than 2^31 kept entries");
+    if ((r = sc_grow(w, total + nb))) return r;
+    const WsVar &vt = w->vars.at(b.tv), &vs = w->vars.at(b.sv);
+    PolyEmitArgs pe{};
+    pe.n_cv = m->n_cv;
+    pe.ndt = vt.mf->fem.nd; pe.nds = vs.mf->fem.nd; pe.qt = vt.mf->qdim; pe.qs = vs.mf->qdim;
+    pe.dt = vt.mf->d_dofs; pe.ds = vs.mf->d_dofs;
+    pe.first_t = vt.first; pe.first_s = vs.first; pe.ncols = n;
+    pe.elem = sc->d_pelem; pe.thr = sc->d_thr; pe.off = sc->d_off;
+    pe.keys = sc->d_keys + total; pe.vals = sc->d_vals + total;
+    WHIP(w, launch_poly_emit(s, pe));
+    total += nb;
+  }
+  if ((r = sc_grow(w, std::max<int64_t>(total, 1)))) return r;
+  int end_bit = 1;
+  while (end_bit < 64 && ((uint64_t)1 << end_bit) < (uint64_t)n * (uint64_t)n) ++end_bit;
+  const size_t sb = poly_sort_bytes(total, end_bit);
+  if (sb > sc->sort_cap) {
+    WHIP(w, hipStreamSynchronize(s));
+    wfree(sc->d_sort);
+    if (hipMalloc(&sc->d_sort, sb) != hipSuccess) return wfail(w, GFX_E_OOM, "order-2 route sort");
+    sc->sort_cap = sb;
+  }
+  WHIP(w, poly_to_csr(s, sc->d_sort, sc->sort_cap, sc->d_keys, sc->d_vals, sc->d_keys2, sc->d_vals2, total,
+                      end_bit, n, n, sc->d_work, sc->d_ip, sc->d_ix, sc->d_v, sc->d_nnz));
+  int flag = 0;
+  int64_t nnz = 0;
+  WHIP(w, hipMemcpyAsync(&nnz, sc->d_nnz, 8, hipMemcpyDeviceToHost, s));
+  WHIP(w, hipMemcpyAsync(&flag, sc->d_err, 4, hipMemcpyDeviceToHost, s));
+  WHIP(w, hipStreamSynchronize(s));
+  if (flag & 1) return wfail(w, GFX_E_DEGENERATE_ELEMENT, "Non invertible matrix (degenerate element)");
+  sc->nnz = nnz;
+  if (w->ext) {
+    if (w->ext_n != n) return wfail(w, GFX_E_INVALID, "target matrix of another size");
+    WHIP(w, hipMemsetAsync(sc->d_err, 0, 4, s));
+    WHIP(w, launch_add_into(s, w->ext_orient == GFX_CSC, n, sc->d_ip, sc->d_ix, sc->d_v, w->ext_ptr, w->ext_ind,
+                            w->ext_vals, sc->d_err));
+    WHIP(w, hipMemcpyAsync(&flag, sc->d_err, 4, hipMemcpyDeviceToHost, s));
+    WHIP(w, hipStreamSynchronize(s));
+    if (flag & 8) return wfail(w, GFX_E_PATTERN, "the target matrix lacks entries of the assembled pattern");
+  }
+  return GFX_OK;
+}
+
 // assembly(0): the potential; assembly(1): the vector of every fem variable's
 // derivative, at the variable's interval (first .. first + nb_dof)
 int sc_run(gfx_workspace *w, int order) {
@@ -660,29 +919,11 @@ int sc_run(gfx_workspace *w, int order) {
   ScalarRoute *sc = w->sc;
   gfx_mesh *m = sc->mim->mesh;
   hipStream_t s = w->ctx->stream;
-  // variables that Inv(...) expanded around zero must be zero
-  auto zero_ok = [&](const GwflScalarProgram &p) -> bool {
-    for (const std::string &z : p.zero_vars)
-      for (double x : w->vars.at(z).V)
-        if (x != 0.0) {
-          wfail(w, GFX_E_UNSUPPORTED, "Inv(...) of " + z + " is expanded around " + z + " = 0, but " + z +
-                                          " is not zero");
-          return false;
-        }
-    return true;
-  };
-  if (order == 0 && !zero_ok(sc->p0.prog)) return GFX_E_UNSUPPORTED;
+  if (order == 0 && !sc_zero_ok(w, sc->p0.prog)) return GFX_E_UNSUPPORTED;
   if (order == 1)
     for (auto &v : sc->vars)
-      if (!zero_ok(v.p.prog)) return GFX_E_UNSUPPORTED;
-  for (auto &f : sc->fields) {
-    const WsVar &v = w->vars.at(f.name);
-    WHIP(w, hipMemcpyAsync(f.d_U, v.V.data(), 8 * v.V.size(), hipMemcpyHostToDevice, s));
-  }
-  std::vector<double> cv;
-  for (const std::string &n : sc->consts) cv.push_back(w->vars.at(n).V[0]);
-  if (!cv.empty()) WHIP(w, hipMemcpyAsync(sc->d_consts, cv.data(), 8 * cv.size(), hipMemcpyHostToDevice, s));
-  WHIP(w, hipMemsetAsync(sc->d_err, 0, 4, s));
+      if (!sc_zero_ok(w, v.p.prog)) return GFX_E_UNSUPPORTED;
+  if ((r = sc_upload(w))) return r;
   ScalarArgs a{};
   a.n_cv = m->n_cv; a.pts = m->d_pts; a.cvx = m->d_cvx;
   a.nq = sc->mim->rule.n(); a.qw = sc->d_qw;
@@ -750,10 +991,13 @@ int gfx_ws_destroy(gfx_workspace *w) {
   (void)hipSetDevice(w->ctx->device);
   drop_plans(w);
   wfree(w->d_u); wfree(w->d_V);
+  wfree(w->d_cg); wfree(w->d_go); wfree(w->d_fixed); wfree(w->d_cerr);
   sc_release(w->sc);
   delete w->sc;
-  for (auto &kv : w->vars)
+  for (auto &kv : w->vars) {
+    wfree(kv.second.d_val);
     if (kv.second.mf) unref(kv.second.mf);
+  }
   for (auto &e : w->exprs) unref(e.mim);
   gfx_ctx *c = w->ctx;
   delete w;
@@ -774,7 +1018,10 @@ static int add_var(gfx_workspace *w, const char *name, int kind, gfx_mesh_fem *m
   if (V) std::memcpy(v.V.data(), V, 8 * (size_t)size);
   if (mf) ref(mf);  // the workspace keeps the mesh_fem alive while it names it
   auto old = w->vars.find(name);
-  if (old != w->vars.end() && old->second.mf) unref(old->second.mf);
+  if (old != w->vars.end()) {
+    wfree(old->second.d_val);
+    if (old->second.mf) unref(old->second.mf);
+  }
   w->vars[name] = v;
   w->dirty = true;
   w->mat_fail = false;
@@ -801,6 +1048,7 @@ int gfx_ws_set_values(gfx_workspace *w, const char *name, const double *h_V) {
   auto it = w->vars.find(name);
   if (it == w->vars.end()) return wfail(w, GFX_E_INVALID, std::string("unknown name ") + name);
   std::memcpy(it->second.V.data(), h_V, 8 * it->second.V.size());
+  it->second.dev = false;
   if (it->second.kind == VAR_FEM) w->state_dirty = true;
   else w->coefs_dirty = true;
   return GFX_OK;
@@ -810,11 +1058,22 @@ int gfx_ws_set_values_device(gfx_workspace *w, const char *name, const double *d
   if (!w || !name || !d_V) return GFX_E_INVALID;
   auto it = w->vars.find(name);
   if (it == w->vars.end()) return wfail(w, GFX_E_INVALID, std::string("unknown name ") + name);
-  if (it->second.kind != VAR_FEM) return wfail(w, GFX_E_INVALID, "device values are for the fem variable");
-  int r;
-  if (w->dirty && (r = setup(w))) return r;
-  if (name != w->unknown) return wfail(w, GFX_E_INVALID, std::string(name) + " is not the unknown");
+  if (it->second.kind != VAR_FEM) return wfail(w, GFX_E_INVALID, "device values are for fem variables");
   if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
+  int r;
+  bool poly = w->mat_fail;
+  if (!poly && w->dirty && (r = setup(w))) {
+    if (r != GFX_E_UNSUPPORTED && r != GFX_E_STATE && r != GFX_E_INVALID) return r;
+    poly = true;  // the polynomial route reads the variables' device copies
+  }
+  if (poly || name != w->unknown) {
+    WsVar &v = it->second;
+    if (!v.d_val && hipMalloc((void **)&v.d_val, 8 * std::max<size_t>(v.V.size(), 1)) != hipSuccess)
+      return wfail(w, GFX_E_OOM, "device values");
+    WHIP(w, hipMemcpyAsync(v.d_val, d_V, 8 * v.V.size(), hipMemcpyDeviceToDevice, w->ctx->stream));
+    v.dev = true;
+    return GFX_OK;
+  }
   // stream-ordered: the next assembly reads it on the same stream
   WHIP(w, hipMemcpyAsync(w->d_u, d_V, 8 * (size_t)w->ndof, hipMemcpyDeviceToDevice, w->ctx->stream));
   if (w->nonlinear && (r = gfx_plan_set_state_device(w->trees[0].plan, w->d_u))) return r;
@@ -888,19 +1147,20 @@ int gfx_ws_assembly(gfx_workspace *w, int order) {
   if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
   if (order == 0) return sc_run(w, 0);
   int r;
-  // order 1 outside the matrix kernels' subset (data fields, P0 or several
-  // variables, pulled-back coordinates): the polynomial route
-  if (order == 1 && w->mat_fail) return sc_run(w, 1);
+  // outside the matrix kernels' subset (data fields, P0 or several
+  // variables, blocks between mesh_fems, pulled-back coordinates,
+  // polynomial nonlinearities): the polynomial route
+  if (w->mat_fail) return order == 1 ? sc_run(w, 1) : sc_run2(w);
   if (w->dirty) {
     if ((r = setup(w))) {
-      if (order != 1 || (r != GFX_E_UNSUPPORTED && r != GFX_E_STATE && r != GFX_E_INVALID)) return r;
+      if (r != GFX_E_UNSUPPORTED && r != GFX_E_STATE && r != GFX_E_INVALID) return r;
       const std::string merr = w->ctx->err;
-      const int r1 = sc_run(w, 1);
+      const int r1 = order == 1 ? sc_run(w, 1) : sc_run2(w);
       if (r1 == GFX_OK) {
         w->mat_fail = true;
         return GFX_OK;
       }
-      w->ctx->err = merr + " (order-0/1 route: " + w->ctx->err + ")";
+      w->ctx->err = merr + " (polynomial route: " + w->ctx->err + ")";
       return r;
     }
   } else if (w->coefs_dirty) {
@@ -1017,6 +1277,16 @@ int gfx_ws_assembled_vector(gfx_workspace *w, double *h_V) {
 int gfx_ws_device_csr(gfx_workspace *w, int64_t *nrows, int64_t *nnz, const int64_t **d_indptr,
                       const int32_t **d_indices, double **d_vals) {
   if (!w) return GFX_E_INVALID;
+  if (w->mat_fail) {  // the polynomial route's matrix
+    if (w->ext) return wfail(w, GFX_E_STATE, "the assembled matrix is the caller's (set_assembled_matrix)");
+    if (!w->sc || w->sc->nnz < 0) return wfail(w, GFX_E_STATE, "no matrix assembled yet");
+    if (nrows) *nrows = w->sc->nvec;
+    if (nnz) *nnz = w->sc->nnz;
+    if (d_indptr) *d_indptr = w->sc->d_ip;
+    if (d_indices) *d_indices = w->sc->d_ix;
+    if (d_vals) *d_vals = w->sc->d_v;
+    return GFX_OK;
+  }
   std::vector<WsTree *> mt = matrix_trees(w);
   if (mt.empty()) return wfail(w, GFX_E_STATE, "no matrix assembled yet");
   if (w->ext) return wfail(w, GFX_E_STATE, "the assembled matrix is the caller's (set_assembled_matrix)");
@@ -1074,6 +1344,70 @@ int gfx_ws_export_csc(gfx_workspace *w, int64_t *h_colptr, uint32_t *h_rows, dou
       if (h_rows) h_rows[d] = (uint32_t)row;
       if (h_vals) h_vals[d] = v[k];
     }
+  return GFX_OK;
+}
+
+// CG on the workspace's assembled matrix (diagonal preconditioner; the
+// 3x3 block Jacobi of gfx_plan_solve_cg needs one vector field)
+int gfx_ws_solve_cg(gfx_workspace *w, const double *d_b, double *d_x, double rtol, int maxit, int *iters,
+                    double *rel_res) {
+  if (!w || !d_b || !d_x || !iters || !rel_res || !(rtol > 0) || maxit < 0) return GFX_E_INVALID;
+  int64_t n = 0, nnz = 0;
+  const int64_t *ip = nullptr;
+  const int32_t *ix = nullptr;
+  double *v = nullptr;
+  int r = gfx_ws_device_csr(w, &n, &nnz, &ip, &ix, &v);
+  if (r) return r;
+  if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
+  if (w->cg_n != n) {
+    WHIP(w, hipStreamSynchronize(w->ctx->stream));
+    wfree(w->d_cg);
+    if (hipMalloc((void **)&w->d_cg, 8 * cg_workspace_doubles(n)) != hipSuccess)
+      return wfail(w, GFX_E_OOM, "CG vectors");
+    w->cg_n = n;
+  }
+  WHIP(w, cg_solve(w->ctx->stream, n, nnz, ip, ix, v, d_b, d_x, rtol, maxit, 1, w->d_cg, iters, rel_res));
+  return GFX_OK;
+}
+
+// model::assembly's Dirichlet conditions by simplification
+// (src/getfem_models.cc:2806-2870) on the workspace's assembled matrix, as
+// gfx_plan_dof_constraints does on a plan's
+int gfx_ws_dof_constraints(gfx_workspace *w, int64_t n_fixed, const int32_t *h_dofs, const double *h_go,
+                           const double *d_pr, double *d_rhs, int flags) {
+  if (!w || n_fixed < 0 || (n_fixed && (!h_dofs || !h_go))) return GFX_E_INVALID;
+  int64_t n = 0, nnz = 0;
+  const int64_t *ip = nullptr;
+  const int32_t *ix = nullptr;
+  double *v = nullptr;
+  int r = gfx_ws_device_csr(w, &n, &nnz, &ip, &ix, &v);
+  if (r) return r;
+  if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
+  hipStream_t s = w->ctx->stream;
+  std::vector<uint8_t> fixed((size_t)n, 0);
+  std::vector<double> go((size_t)n, 0.0);
+  for (int64_t k = 0; k < n_fixed; ++k) {
+    if (h_dofs[k] < 0 || h_dofs[k] >= n) return wfail(w, GFX_E_INVALID, "constrained dof out of range");
+    fixed[h_dofs[k]] = 1;
+    go[h_dofs[k]] = h_go[k];
+  }
+  if (w->con_n != n) {
+    WHIP(w, hipStreamSynchronize(s));
+    wfree(w->d_fixed); wfree(w->d_go); wfree(w->d_cerr);
+    if (hipMalloc((void **)&w->d_fixed, std::max<size_t>((size_t)n, 1)) != hipSuccess ||
+        hipMalloc((void **)&w->d_go, 8 * std::max<size_t>((size_t)n, 1)) != hipSuccess ||
+        hipMalloc((void **)&w->d_cerr, 4) != hipSuccess)
+      return wfail(w, GFX_E_OOM, "constraint buffers");
+    w->con_n = n;
+  }
+  WHIP(w, hipMemcpyAsync(w->d_fixed, fixed.data(), (size_t)n, hipMemcpyHostToDevice, s));
+  WHIP(w, hipMemcpyAsync(w->d_go, go.data(), 8 * (size_t)n, hipMemcpyHostToDevice, s));
+  WHIP(w, hipMemsetAsync(w->d_cerr, 0, 4, s));
+  WHIP(w, launch_dof_constraints(s, n, ip, ix, v, w->d_fixed, w->d_go, d_pr, d_rhs, flags, w->d_cerr));
+  int flag = 0;
+  WHIP(w, hipMemcpyAsync(&flag, w->d_cerr, 4, hipMemcpyDeviceToHost, s));
+  WHIP(w, hipStreamSynchronize(s));
+  if (flag & 4) return wfail(w, GFX_E_PATTERN, "a constrained dof has no diagonal entry in the pattern");
   return GFX_OK;
 }
 
@@ -1155,7 +1489,7 @@ int gfx_gwfl_compile(const char *exprs, const char *decls, char *out, int64_t le
 
 int gfx_gwfl_compile_scalar(const char *exprs, const char *decls, int order, const char *var, char *out,
                             int64_t len) {
-  if (!exprs || !decls || !out || len <= 0 || (order == 1 && !var)) return GFX_E_INVALID;
+  if (!exprs || !decls || !out || len <= 0 || (order >= 1 && !var)) return GFX_E_INVALID;
   std::map<std::string, GwflVar> gv;
   if (!parse_decls(decls, gv)) return GFX_E_INVALID;
   std::vector<std::string> ex;
@@ -1169,7 +1503,15 @@ int gfx_gwfl_compile_scalar(const char *exprs, const char *decls, int order, con
   }
   GwflScalarProgram prog;
   std::string err, d;
-  const bool ok = gwfl_compile_scalar(ex, gv, order, var ? var : "", prog, err);
+  // order 2: var names the test and the trial variable, "u,p"
+  std::string v1 = var ? var : "", v2;
+  if (order == 2) {
+    const size_t c = v1.find(',');
+    if (c == std::string::npos) return GFX_E_INVALID;
+    v2 = v1.substr(c + 1);
+    v1 = v1.substr(0, c);
+  }
+  const bool ok = gwfl_compile_scalar(ex, gv, order, v1, prog, err, v2);
   if (ok) {
     d = "nout=" + std::to_string(prog.nout) + " zero=";
     for (size_t k = 0; k < prog.zero_vars.size(); ++k) d += (k ? "," : "") + prog.zero_vars[k];

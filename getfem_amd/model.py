@@ -1,6 +1,15 @@
 """A model over the device assembly, after getfem::model (src/getfem_models.cc)
-and the Python interface's gf.Model('real'), for one fem variable, generic
+and the Python interface's gf.Model('real'): fem variables, generic
 assembly terms and Dirichlet conditions by simplification.
+
+Variables are laid out one after the other in the global vector in the
+order of their names, as model::resize_global_system gives each its
+interval from the name-ordered variable map (src/getfem_models.cc:305-316).  One variable whose terms the matrix kernels cover assembles and
+solves through that tree's plan (3x3 block Jacobi CG for a vector field);
+otherwise (several variables, coupling blocks, polynomial nonlinearities)
+the workspace takes the polynomial route, the tangent is the matrix over
+all intervals and every solve is a Newton iteration on the residual form
+(a linear model converges in one).
 
 What runs where: the tangent matrix and the residual are assembled on the
 device by the workspace (ga_workspace inside model::assembly,
@@ -31,8 +40,10 @@ class Model:
     def __init__(self, ctx):
         self.ctx = ctx
         self.ws = GaWorkspace(ctx)
-        self.var = None
+        self.var = None     # the first variable (single-variable models)
         self.mf = None
+        self.vars = {}      # name -> (mesh_fem, first)
+        self.nvec = 0
         self.data = {}
         self.terms = []  # (expr, mim, linear, symmetric, coercive)
         self.dirichlet = []  # (dofs, dataname or None)
@@ -42,13 +53,34 @@ class Model:
 
     # -------------------------------------------------------------- variables
     def add_fem_variable(self, name, mf):
-        """model::add_fem_variable (one fem variable per model here)."""
+        """model::add_fem_variable.  The intervals are re-laid out in name
+        order; the variables' values move with them."""
         import torch
-        if self.var is not None:
-            raise ValueError("this model holds one fem variable")
-        self.var, self.mf = name, mf
-        self.U = torch.zeros(mf.nbdof(), dtype=torch.float64, device="cuda")
-        self.ws.add_fem_variable(name, mf, np.zeros(mf.nbdof()))
+        if name in self.vars or name in self.data:
+            raise ValueError("name already used: " + name)
+        old = {k: self._slice(k).clone() for k in self.vars}
+        self.vars[name] = (mf, 0)
+        first = 0
+        for k in sorted(self.vars):
+            self.vars[k] = (self.vars[k][0], first)
+            first += self.vars[k][0].nbdof()
+        self.nvec = first
+        if self.var is None:
+            self.var, self.mf = name, mf
+        self.U = torch.zeros(self.nvec, dtype=torch.float64, device="cuda")
+        for k, v in old.items():
+            self._slice(k).copy_(v)
+        for k in sorted(self.vars):
+            m, f = self.vars[k]
+            self.ws.add_fem_variable(k, m, np.zeros(m.nbdof()), first=f)
+        self._built = False
+
+    def _multi(self):
+        return len(self.vars) > 1
+
+    def _slice(self, name):
+        mf, first = self.vars[name]
+        return self.U[first:first + mf.nbdof()]
 
     def add_initialized_data(self, name, value):
         """model::add_initialized_fixed_size_data."""
@@ -65,16 +97,16 @@ class Model:
 
     def set_variable(self, name, V):
         import torch
-        if name == self.var:
-            self.U.copy_(torch.as_tensor(np.asarray(V, dtype=np.float64)))
+        if name in self.vars:
+            self._slice(name).copy_(torch.as_tensor(np.asarray(V, dtype=np.float64)))
         else:
             V = np.atleast_1d(np.asarray(V, dtype=np.float64))
             self.data[name] = V
             self.ws.set_values(name, V)
 
     def variable(self, name):
-        if name == self.var:
-            return self.U.cpu().numpy()
+        if name in self.vars:
+            return self._slice(name).cpu().numpy()
         return self.data[name]
 
     # ------------------------------------------------------------------ bricks
@@ -100,9 +132,9 @@ class Model:
         mf.dof_on_region(rg)).  dataname: fixed-size data of qdim values
         (value of dof i = data[i % qdim]) or fem data on the same mesh_fem;
         None = homogeneous."""
-        if varname != self.var:
+        if varname not in self.vars:
             raise ValueError("unknown variable " + varname)
-        self.dirichlet.append((np.asarray(dofs, dtype=np.int64), dataname))
+        self.dirichlet.append((varname, np.asarray(dofs, dtype=np.int64), dataname))
 
     def is_linear(self):
         return all(t[2] for t in self.terms)
@@ -115,15 +147,17 @@ class Model:
 
     # ---------------------------------------------------------------- assembly
     def _constraints(self):
+        """(global dofs, values): each variable's dofs at its interval."""
         dofs, vals = [], []
-        q = self.mf.qdim
-        for d, name in self.dirichlet:
+        for var, d, name in self.dirichlet:
+            mf, first = self.vars[var]
+            q = mf.qdim
             if name is None:
                 v = np.zeros(d.size)
             else:
                 A = self.data[name]
                 v = A[d % q] if A.size == q else A[d]
-            dofs.append(d)
+            dofs.append(d + first)
             vals.append(v)
         if not dofs:
             return np.zeros(0, np.int32), np.zeros(0)
@@ -152,13 +186,14 @@ class Model:
         # the state goes to the workspace device to device (no host copy);
         # torch's writes to U are complete before the context's stream reads it
         torch.cuda.current_stream().synchronize()
-        self.ws.set_values_device(self.var, self.U.data_ptr())
-        n = self.mf.nbdof()
+        for name in self.vars:
+            self.ws.set_values_device(name, self._slice(name).data_ptr())
+        n = self.nvec
         rhs = torch.zeros(n, dtype=torch.float64, device="cuda")
         if build_matrix or not self._built:
             self.ws.assembly(2)
             self._built = True
-        if build_rhs and not self.is_linear():
+        if build_rhs and (self._multi() or not self.is_linear()):
             self.ws.assembly(1)
             check(lib().gfx_ws_copy_assembled_vector_device(self.ws.h, ctypes.c_void_p(rhs.data_ptr())),
                   self.ctx.h)
@@ -166,13 +201,15 @@ class Model:
             rhs.neg_()
         dofs, vals = self._constraints()
         flags = (_lib.BUILD_MATRIX if build_matrix else 0) | (_lib.BUILD_RHS if build_rhs else 0)
-        flags |= _lib.MODEL_LINEAR if self.is_linear() else 0
+        flags |= _lib.MODEL_LINEAR if (self.is_linear() and not self._multi()) else 0
         flags |= _lib.MODEL_SYMMETRIC if self.is_symmetric() else 0
         if dofs.size:
-            check(lib().gfx_plan_dof_constraints(self._plan(), dofs.size, ctypes.c_void_p(dofs.ctypes.data),
-                                                 ctypes.c_void_p(vals.ctypes.data),
-                                                 ctypes.c_void_p(self.U.data_ptr()),
-                                                 ctypes.c_void_p(rhs.data_ptr()), flags), self.ctx.h)
+            args = (dofs.size, ctypes.c_void_p(dofs.ctypes.data), ctypes.c_void_p(vals.ctypes.data),
+                    ctypes.c_void_p(self.U.data_ptr()), ctypes.c_void_p(rhs.data_ptr()), flags)
+            if self._multi():
+                check(lib().gfx_ws_dof_constraints(self.ws.h, *args), self.ctx.h)
+            else:
+                check(lib().gfx_plan_dof_constraints(self._plan(), *args), self.ctx.h)
         check(lib().gfx_ctx_synchronize(self.ctx.h), self.ctx.h)
         self._rhs = rhs
         return rhs
@@ -185,9 +222,12 @@ class Model:
 
     def _cg(self, rhs, dx, rtol, maxit):
         it, res = ctypes.c_int(), ctypes.c_double()
-        check(lib().gfx_plan_solve_cg(self._plan(), ctypes.c_void_p(rhs.data_ptr()),
-                                      ctypes.c_void_p(dx.data_ptr()), rtol, maxit, ctypes.byref(it),
-                                      ctypes.byref(res)), self.ctx.h)
+        args = (ctypes.c_void_p(rhs.data_ptr()), ctypes.c_void_p(dx.data_ptr()), rtol, maxit, ctypes.byref(it),
+                ctypes.byref(res))
+        if self._multi():
+            check(lib().gfx_ws_solve_cg(self.ws.h, *args), self.ctx.h)
+        else:
+            check(lib().gfx_plan_solve_cg(self._plan(), *args), self.ctx.h)
         return it.value, res.value
 
     # ------------------------------------------------------------------- solve
@@ -201,7 +241,7 @@ class Model:
             raise ValueError("the device solver is CG: declare the terms symmetric (is_symmetric=True)")
         cg_rtol = max_res / 20.0  # iter_linsolv0.set_resmax(iter.get_resmax()/20)
         log = []
-        if self.is_linear():
+        if self.is_linear() and not self._multi():
             rhs = self.assembly("build_all")
             it, res = self._cg(rhs, self.U, cg_rtol, cg_maxit)
             log.append((it, res))

@@ -116,8 +116,11 @@ int gfx_gwfl_compile(const char *exprs, const char *decls, char *out, int64_t le
  * its derivation order */
 int gfx_gwfl_order(const char *expr, const char *decls, int *order);
 
-/* The order-0 / order-1 compile stage alone (the potential, or the
- * derivative with respect to the fem variable `var`), as text: a first line
+/* The polynomial route's compile stage alone (order 0: the potential;
+ * order 1: the derivative with respect to the fem variable `var`; order 2:
+ * the block of test variable and trial variable named by `var` as "u,p",
+ * output slot_u * 4 qdim_p + slot_p with slot 0..qdim-1 the value and
+ * qdim + 3a + b the gradient of a test function), as text: a first line
  * "nout=<outputs> zero=<variables held at zero>", a second "syms=<data
  * symbols>", then one line per monomial "<output> <coefficient> <symbol
  * indices>".  decls as gfx_gwfl_compile, plus the kinds "p0var" (a fem
@@ -350,14 +353,19 @@ int gfx_ws_set_form(gfx_workspace *w, int form, int precision);
 int gfx_ws_compile(gfx_workspace *w);
 /* ga_workspace::assembly(order) (.h:564).  order 0: the potential, the sum
  * of the expressions of test-function order 0 (read it with
- * gfx_ws_assembled_potential).  Orders 0 and 1 outside the matrix kernels'
+ * gfx_ws_assembled_potential).  Orders 0, 1 and 2 outside the matrix kernels'
  * subset take the polynomial route: every integrand that is a polynomial in
  * the values and gradients of fem fields (variables and fem constants on
  * affine tetrahedra, P0 included), fixed-size constants, pow(x, k), Det and
  * Inv(Id + Grad_w) at w = 0 (the pulled-back coordinates), compiled by
  * symbolic derivation into per-point programs run by one kernel; the order-1
  * vector then holds the derivative with respect to every fem variable at its
- * interval.  order 2: the matrix (for the
+ * interval, and the order-2 matrix (nvec x nvec, nvec = the end of the last
+ * variable's interval) every block test variable x trial variable at their
+ * intervals, each element matrix thresholded at 1e-14 max|elem| and the
+ * entries summed in element order (ga_instruction_matrix_assembly_mf_mf and
+ * add_elem_matrix, src/getfem_generic_assembly_compile_and_exec.cc:
+ * 5074-5189, :4853-4936).  order 2: the matrix (for the
  * Neo-Hookean term the pattern is rebuilt when the state or the coefficients
  * changed, since GetFEM's stored pattern depends on the values).  order 1: the
  * vector of the expression's first derivative at the unknown's current value
@@ -404,6 +412,18 @@ int gfx_ws_set_assembled_matrix(gfx_workspace *w, int orientation, int64_t n, co
                                 const int32_t *d_ind, double *d_vals);
 /* normal form of the compiled term, e.g. "isotropic elasticity unknown=u ..." */
 int gfx_ws_describe(gfx_workspace *w, char *buf, int64_t len);
+/* CG on the workspace's assembled matrix (the single tree's plan CSR, the
+ * union of several trees, or the polynomial route's matrix over all fem
+ * variables), diagonal preconditioner, stopping test and arguments as
+ * gfx_plan_solve_cg; what model::solve's default_linear_solver hands a
+ * coercive system of several variables (src/getfem/getfem_model_solvers.h:
+ * 652-655) */
+int gfx_ws_solve_cg(gfx_workspace *w, const double *d_b, double *d_x, double rtol, int maxit, int *iters,
+                    double *rel_res);
+/* gfx_plan_dof_constraints on the workspace's assembled matrix (dofs are
+ * global: the variables' intervals) */
+int gfx_ws_dof_constraints(gfx_workspace *w, int64_t n_fixed, const int32_t *h_dofs, const double *h_go,
+                           const double *d_pr, double *d_rhs, int flags);
 
 /* ======================= adjoint (SIMP compliance) =====
  * J = u^T K(rho, X) u with K the isotropic elasticity matrix of a P1 vector

@@ -59,6 +59,8 @@ def lib():
         L.orc_ildlt.argtypes = [i64, P, P, P, P, P, P]
         L.orc_ildlt.restype = i64
         L.orc_ildlt_apply.argtypes = [i64, P, P, P, P, P]
+        L.orc_assemble_blocks.argtypes = [i64, i64, i32, P]
+        L.orc_assemble_blocks.restype = P
         _lib = L
     return _lib
 
@@ -200,6 +202,101 @@ class Problem:
         dx = np.zeros_like(self.pts) if grads else None
         J = lib().orc_compliance(ctypes.byref(self.s), _ptr(rho), emin, penal, _ptr(dr), _ptr(dx))
         return J, dr, dx
+
+
+class _Block(ctypes.Structure):
+    _fields_ = [("s1", ctypes.c_int), ("s2", ctypes.c_int), ("rows", ctypes.c_void_p),
+                ("cols", ctypes.c_void_p), ("elem", ctypes.c_void_p)]
+
+
+def assemble_blocks(n_cv, ndof, blocks):
+    """A matrix over several variables from its blocks' element matrices, as
+    ga_instruction_matrix_assembly_mf_mf::exec adds them
+    (src/getfem_generic_assembly_compile_and_exec.cc:5074-5189): element by
+    element, block by block, zero element matrices skipped, the others added
+    by add_elem_matrix (:4853-4936) with the threshold 1e-14 max|elem|.
+    blocks: (rows (n_cv, s1) global row dofs, cols (n_cv, s2) global column
+    dofs, elem (n_cv, s1, s2) with rows = test dofs).  Returns GetFEM's CSC
+    (colptr, rows, vals) of the ndof x ndof matrix."""
+    keep = []
+    arr = (_Block * max(len(blocks), 1))()
+    for b, (r, c, e) in enumerate(blocks):
+        r = np.ascontiguousarray(r, dtype=np.int32)
+        c = np.ascontiguousarray(c, dtype=np.int32)
+        ecm = np.ascontiguousarray(np.transpose(np.asarray(e, dtype=np.float64), (0, 2, 1)))  # column-major
+        keep += [r, c, ecm]
+        arr[b] = _Block(r.shape[1], c.shape[1], r.ctypes.data, c.ctypes.data, ecm.ctypes.data)
+    m = lib().orc_assemble_blocks(n_cv, ndof, len(blocks), arr)
+    try:
+        nnz = lib().orc_mat_nnz(m)
+        colptr = np.zeros(ndof + 1, dtype=np.int64)
+        rows = np.zeros(nnz, dtype=np.int32)
+        vals = np.zeros(nnz)
+        lib().orc_mat_export_csc(m, _ptr(colptr), _ptr(rows), _ptr(vals))
+    finally:
+        lib().orc_mat_free(m)
+    return colptr, rows, vals
+
+
+def tet_geometry(pts, cv):
+    """Affine tetrahedron: K = [x1 - x0, x2 - x0, x3 - x0], J = det K and
+    B = K^{-T} (src/bgeot_geometric_trans.cc:357-413)."""
+    X = pts[cv]
+    K = (X[1:] - X[0]).T
+    return np.linalg.det(K), np.linalg.inv(K).T
+
+
+def block_element_matrices(pts, cvx, im_name, test, trial, fields, integrand2):
+    """Element matrices of one block (test variable x trial variable) of a
+    polynomial integrand on affine tetrahedra, accumulated over the
+    integration points as ga_exec runs the element loop
+    (src/getfem_generic_assembly_compile_and_exec.cc:8750-8869) and the
+    matrix instruction adds coeff * t to elem (:5074-5100).
+    test, trial: (fem_kind, k, qdim, dofs (n_cv, nd)).  fields: name ->
+    (fem_kind, k, qdim, dofs, U), the fields the integrand reads.
+    integrand2(vals) -> D (4 qt, 4 qs): the integrand's second derivative with
+    respect to a test slot and a trial slot (slot a < q the value of component
+    a, q + 3a + b its gradient along b), with vals[name] = (value (q,),
+    gradient (q, 3)) at the point.  Returns (n_cv, nd_t qt, nd_s qs), rows and
+    columns ordered node-major (node i, component a) -> i q + a."""
+    qp, qw = im_rule(im_name)
+    tabs = {}
+
+    def table(kind, k):
+        if (kind, k) not in tabs:
+            ev = [fem_eval(kind, k, x) for x in qp]
+            tabs[(kind, k)] = (np.array([v for v, _ in ev]), np.array([g for _, g in ev]))
+        return tabs[(kind, k)]
+
+    def slots(kind, k, q, cv_dofs, B, iq):
+        phi, gref = table(kind, k)
+        g = gref[iq] @ B.T  # real gradients (nd, 3)
+        nd = phi.shape[1]
+        S = np.zeros((nd * q, 4 * q))
+        for i in range(nd):
+            for a in range(q):
+                S[i * q + a, a] = phi[iq, i]
+                S[i * q + a, q + 3 * a:q + 3 * a + 3] = g[i]
+        return S
+
+    n_cv = cvx.shape[0]
+    tk, tkk, tq, tdofs = test
+    sk, skk, sq, sdofs = trial
+    out = np.zeros((n_cv, tdofs.shape[1] * tq, sdofs.shape[1] * sq))
+    for e in range(n_cv):
+        J, B = tet_geometry(pts, cvx[e])
+        for iq in range(len(qw)):
+            vals = {}
+            for name, (kind, k, q, dofs, U) in fields.items():
+                phi, gref = table(kind, k)
+                g = gref[iq] @ B.T
+                loc = np.array([[U[d + c] for c in range(q)] for d in dofs[e]])  # (nd, q)
+                vals[name] = (phi[iq] @ loc, loc.T @ g)
+            D = integrand2(vals)
+            St = slots(tk, tkk, tq, tdofs[e], B, iq)
+            Ss = slots(sk, skk, sq, sdofs[e], B, iq)
+            out[e] += qw[iq] * abs(J) * (St @ D @ Ss.T)
+    return out
 
 
 def csc_to_csr(colptr, rows, vals, n):

@@ -28,6 +28,14 @@
  *    P2 elasticity 8^3x6: 1,096,305 of 1,141,065).
  *  - orc_assemble_vector (Neo-Hookean residual): finite differences of the
  *    strain energy; tangent = finite differences of the residual.
+ *  - orc_assemble_blocks (matrices over several variables): one square
+ *    block equals orc_assemble_matrix of the same element matrices; a
+ *    rectangular block equals a dense brute force (per-element threshold,
+ *    zero elements skipped, sorted rows per column).  The numpy element
+ *    matrices that feed it (oracle.block_element_matrices) equal
+ *    orc_element_matrix's Laplacian, the closed-form P1 mass
+ *    |T|/20 (1 + delta_ij), and for the P1 x P2 coupling the same values
+ *    under two exact rules and the exact integral of div u of a linear u.
  *  - orc_compliance: central finite differences on J.
  *  - orc_ildlt / orc_ildlt_apply: on a pattern without dropped fill
  *    (tridiagonal, dense) the factor is the exact LDL^T (numpy Cholesky);
@@ -817,25 +825,26 @@ static void col_reserve(orc_col *c, int32_t need) {
   c->cap = nc;
 }
 
-/* add_elem_matrix (compile_and_exec.cc:4853-4936): rows insertion-sorted,
- * per column a merge by binary search into the sorted column */
-static void add_elem_matrix(orc_mat *K, const int32_t *dofs, int s1,
-                            const double *elem, double thr, int *sortv) {
+/* add_elem_matrix (compile_and_exec.cc:4853-4936): rows (dofs1, s1 of them)
+ * insertion-sorted, then for each column dofs2[jc] a merge by binary search
+ * into the sorted column; elem is column-major s1 x s2 */
+static void add_elem_matrix_rect(orc_mat *K, const int32_t *dofs1, int s1, const int32_t *dofs2, int s2,
+                                 const double *elem, double thr, int *sortv) {
   for (int i = 0; i < s1; ++i) {
     int j = i;
-    while (j > 0 && dofs[i] < dofs[sortv[j - 1]]) { sortv[j] = sortv[j - 1]; --j; }
+    while (j > 0 && dofs1[i] < dofs1[sortv[j - 1]]) { sortv[j] = sortv[j - 1]; --j; }
     sortv[j] = i;
   }
-  for (int jc = 0; jc < s1; ++jc) {
+  for (int jc = 0; jc < s2; ++jc) {
     const double *it = elem + (size_t)s1 * jc;
-    orc_col *col = &K->cols[dofs[jc]];
+    orc_col *col = &K->cols[dofs2[jc]];
     int32_t nb = col->n, ind = 0;
     col_reserve(col, nb + s1);
     for (int kk = 0; kk < s1; ++kk) {
       int k = sortv[kk];
       double e = it[k];
       if (!(fabs(e) > thr)) continue;
-      int32_t c = dofs[k];
+      int32_t c = dofs1[k];
       int32_t count = nb - ind;
       while (count > 0) {
         int32_t step = count / 2, l = ind + step;
@@ -854,6 +863,34 @@ static void add_elem_matrix(orc_mat *K, const int32_t *dofs, int s1,
     }
     col->n = nb;
   }
+}
+
+static void add_elem_matrix(orc_mat *K, const int32_t *dofs, int s1, const double *elem, double thr,
+                            int *sortv) {
+  add_elem_matrix_rect(K, dofs, s1, dofs, s1, elem, thr, sortv);
+}
+
+/* ga_instruction_matrix_assembly_mf_mf::exec (compile_and_exec.cc:5074-5189)
+ * for given element matrices: element by element, every block's matrix is
+ * skipped when it is zero, else added with the threshold 1e-14 max|elem|. */
+void *orc_assemble_blocks(int64_t n_cv, int64_t ndof, int nblk, const orc_block *blk) {
+  orc_mat *K = mat_new(ndof);
+  int smax = 1;
+  for (int b = 0; b < nblk; ++b) if (blk[b].s1 > smax) smax = blk[b].s1;
+  int *sortv = (int *)malloc(sizeof(int) * smax);
+  for (int64_t cv = 0; cv < n_cv; ++cv)
+    for (int b = 0; b < nblk; ++b) {
+      const orc_block *B = &blk[b];
+      const size_t ne = (size_t)B->s1 * B->s2;
+      const double *elem = B->elem + ne * cv;
+      double ninf = 0.0;
+      for (size_t k = 0; k < ne; ++k) if (fabs(elem[k]) > ninf) ninf = fabs(elem[k]);
+      if (ninf == 0.0) continue;
+      add_elem_matrix_rect(K, B->rows + (int64_t)B->s1 * cv, B->s1, B->cols + (int64_t)B->s2 * cv, B->s2,
+                           elem, ninf * 1e-14, sortv);
+    }
+  free(sortv);
+  return K;
 }
 
 static void assemble_range(const orc_problem *p, orc_mat *K, int64_t b, int64_t e) {

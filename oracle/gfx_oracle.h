@@ -94,6 +94,19 @@ void orc_mat_export_csc(const void *m, int64_t *colptr, int32_t *rows,
                         double *vals);
 void orc_mat_free(void *m);
 
+/* Blocks of a matrix over several variables, given their element
+ * matrices: block b has s1 rows (global dofs rows[cv*s1 + i], the test
+ * variable's interval included) and s2 columns (cols[cv*s2 + j]); elem is
+ * n_cv column-major s1 x s2 matrices.  Assembled element by element, block
+ * by block, as ga_instruction_matrix_assembly_mf_mf::exec
+ * (compile_and_exec.cc:5074-5189) with add_elem_matrix (:4853-4936). */
+typedef struct {
+  int s1, s2;
+  const int32_t *rows, *cols;
+  const double *elem;
+} orc_block;
+void *orc_assemble_blocks(int64_t n_cv, int64_t ndof, int nblk, const orc_block *blk);
+
 /* residual vector assembly, ga_instruction_vector_assembly_mf
  * (compile_and_exec.cc:4669-4735): no threshold */
 int orc_assemble_vector(const orc_problem *p, double *V);

@@ -74,7 +74,7 @@ int main(int argc, char **argv) {
     try {
       getfem_amd::ga_workspace w2(ctx);
       w2.add_fem_variable("u", mf_u, getfem_amd::sub_interval(0, mf_u.nb_dof()), U);
-      w2.add_expression("Test_u.Test2_u", mim);
+      w2.add_expression("Heaviside(Norm(u))*Test_u.Test2_u", mim);
       w2.assembly(2);
       std::cout << "no exception\n";
       return 1;

@@ -113,3 +113,113 @@ def test_neohookean_newton(ctx):
     print("newton", its, "cg", log, "rel err vs host Newton", err)
     assert err <= 1e-8
     assert np.abs(U[top] - g[bottom.size:]).max() < 1e-10
+
+
+
+IM5 = "IM_TETRAHEDRON(5)"
+
+
+def _host_pieces(pts, cvx, du, dv, kappa):
+    """The linear tangent of  Grad_u.Grad_Test_u + Grad_v.Grad_Test_v
+    + kappa (u - v)(Test_u - Test_v)  and u's mass matrix, from the oracle's
+    block element matrices, over the global vector (u's interval first)."""
+    nu = int(du.max()) + 1
+    n = nu + int(dv.max()) + 1
+    fu, fv = (oracle.FEM_PK, 2, 1, du), (oracle.FEM_PK, 1, 1, dv)
+
+    def diag(k):
+        def fn(vals):
+            D = np.zeros((4, 4))
+            D[0, 0] = k
+            D[1:, 1:] = np.eye(3)
+            return D
+        return fn
+
+    def mass(k):
+        def fn(vals):
+            D = np.zeros((4, 4))
+            D[0, 0] = k
+            return D
+        return fn
+
+    rv = nu + dv
+    Euu = oracle.block_element_matrices(pts, cvx, IM5, fu, fu, {}, diag(kappa))
+    Evv = oracle.block_element_matrices(pts, cvx, IM5, fv, fv, {}, diag(kappa))
+    Euv = oracle.block_element_matrices(pts, cvx, IM5, fu, fv, {}, mass(-kappa))
+    Mu = oracle.block_element_matrices(pts, cvx, IM5, fu, fu, {}, mass(1.0))
+    K = oracle.assemble_blocks(cvx.shape[0], n, [(du, du, Euu), (du, rv, Euv),
+                                                 (rv, du, np.transpose(Euv, (0, 2, 1))), (rv, rv, Evv)])
+    M = oracle.assemble_blocks(cvx.shape[0], n, [(du, du, Mu)])
+    return (sp.csc_matrix((K[2], K[1], K[0]), shape=(n, n)),
+            sp.csc_matrix((M[2], M[1], M[0]), shape=(n, n)))
+
+
+def _u2_mass(pts, cvx, du, n, Uu):
+    """int u^2 phi_i phi_j over u's block (the cubic term's weight)"""
+    def fn(vals):
+        D = np.zeros((4, 4))
+        D[0, 0] = vals["u"][0][0] ** 2
+        return D
+    f = (oracle.FEM_PK, 2, 1, du)
+    E = oracle.block_element_matrices(pts, cvx, IM5, f, f, {"u": (oracle.FEM_PK, 2, 1, du, Uu)}, fn)
+    cp, r, v = oracle.assemble_blocks(cvx.shape[0], n, [(du, du, E)])
+    return sp.csc_matrix((v, r, cp), shape=(n, n))
+
+
+@pytest.mark.parametrize("c", [0.0, 2.0])
+def test_two_variable_model_matches_host_newton(ctx, c):
+    """Two scalar fields (u P2, v P1) coupled by kappa (u - v), a source on u,
+    u = v = 0 on the boundary, and for c > 0 the nonlinear term c u^3: the
+    device model (polynomial route, Newton with device CG) against Newton on
+    the host with the oracle's matrices and a direct solver.  v is added
+    before u: the intervals follow the names (u first)."""
+    print("case: two-variable model c=%g" % c, flush=True)
+    n, kappa, f = 3, 5.0, 10.0
+    pts, cvx = meshes.regular_unit_mesh(n)
+    pts = meshes.jitter(pts, n)
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    mfu = ga.MeshFem(mesh, "FEM_PK(3,2)", 1)
+    mfv = ga.MeshFem(mesh, "FEM_PK(3,1)", 1)
+    mim = ga.MeshIm(mesh, IM5)
+    md = Model(ctx)
+    md.add_fem_variable("v", mfv)
+    md.add_fem_variable("u", mfu)
+    md.add_initialized_data("kappa", kappa)
+    md.add_initialized_data("f", f)
+    md.add_initialized_data("c", c)
+    md.add_linear_term(mim, "Grad_u.Grad_Test_u + Grad_v.Grad_Test_v + kappa*(u-v)*(Test_u-Test_v)",
+                       is_symmetric=True, is_coercive=True)
+    md.add_linear_term(mim, "-f*Test_u", is_symmetric=True, is_coercive=True)
+    if c:
+        md.add_nonlinear_term(mim, "c*u*u*u*Test_u", is_symmetric=True, is_coercive=True)
+
+    def boundary(mf):
+        x = mf.basic_dof_nodes()
+        return np.nonzero(np.any((np.abs(x) < 1e-10) | (np.abs(x - 1) < 1e-10), axis=1))[0]
+
+    bu, bv = boundary(mfu), boundary(mfv)
+    md.add_Dirichlet_condition_with_simplification("u", bu)
+    md.add_Dirichlet_condition_with_simplification("v", bv)
+    its, ok, log = md.solve(max_res=1e-11)
+    nu = mfu.nbdof()
+    U = np.concatenate([md.variable("u"), md.variable("v")])
+    # host Newton on the same system
+    du, dv = mfu.basic_dof_table(), mfv.basic_dof_table()
+    dofs = np.concatenate([bu, nu + bv])
+    Kl, Mu = _host_pieces(pts, cvx, du, dv, kappa)
+    F = f * np.asarray(Mu.sum(axis=1)).ravel()  # f int phi_i on u's interval
+    X = np.zeros_like(F)
+    for _ in range(30):
+        Kt, R = Kl, Kl @ X - F
+        if c:
+            M2 = _u2_mass(pts, cvx, du, Kl.shape[0], X[:nu])
+            Kt, R = Kl + 3 * c * M2, R + c * (M2 @ X)
+        Ks, rhs = simplify(Kt, -R, dofs, np.zeros(dofs.size), False, X)
+        dX = spla.spsolve(Ks.tocsc(), rhs)
+        X += dX
+        if np.abs(dX).max() <= 1e-14 * np.abs(X).max():
+            break
+    err = np.linalg.norm(U - X) / np.linalg.norm(X)
+    print("newton", its, "cg", log, "rel err vs host", err)
+    assert ok and err <= 1e-8
+    assert np.abs(U[dofs]).max() == 0.0

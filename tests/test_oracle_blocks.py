@@ -1,0 +1,137 @@
+"""The oracle's matrices over several variables: the rectangular
+add_elem_matrix and the block assembly of
+ga_instruction_matrix_assembly_mf_mf (src/getfem_generic_assembly_compile_
+and_exec.cc:5074-5189, :4853-4936), and the numpy element matrices of
+polynomial integrands.
+
+Pins: the block assembly of one square block equals the oracle's pinned
+single-variable assembly; a rectangular block equals a dense brute force
+(threshold per element, sum over elements); the numpy element matrices
+equal the C oracle's Laplacian, the closed-form P1 mass matrix
+|T|/20 (1 + delta_ij), and, for the P1 x P2 coupling phi_i div(psi_j), the
+same matrices under a second integration rule (the integrand has degree 2,
+so both rules are exact) and the exact integral of div u for a linear u."""
+import numpy as np
+import pytest
+
+import oracle
+from getfem_amd import meshes
+
+
+def mesh(n, seed=3):
+    pts, cvx = meshes.regular_unit_mesh(n)
+    return meshes.jitter(pts, n, seed=seed), cvx
+
+
+def dense(colptr, rows, vals, n):
+    A = np.zeros((n, n))
+    for c in range(n):
+        for k in range(colptr[c], colptr[c + 1]):
+            A[rows[k], c] = vals[k]
+    return A
+
+
+def lap(vals):
+    D = np.zeros((4, 4))
+    D[1:, 1:] = np.eye(3)
+    return D
+
+
+def test_square_block_equals_single_variable_assembly():
+    pts, cvx = mesh(2)
+    dofs, ndof = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 2, 1)
+    qp, qw = oracle.im_rule("IM_TETRAHEDRON(5)")
+    prob = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 2, 1, dofs, ndof, "IM_TETRAHEDRON(5)",
+                          oracle.TERM_LAPLACE, scal_a=1.0)
+    E = np.array([prob.element_matrix(e) for e in range(cvx.shape[0])])
+    ref = prob.assemble_csc()
+    got = oracle.assemble_blocks(cvx.shape[0], ndof, [(dofs, dofs, E)])
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+def test_rectangular_block_against_brute_force():
+    rng = np.random.default_rng(5)
+    n_cv, s1, s2, ndof = 40, 4, 6, 30
+    rows = np.array([rng.choice(15, s1, replace=False) for _ in range(n_cv)], dtype=np.int32)
+    cols = np.array([15 + rng.choice(15, s2, replace=False) for _ in range(n_cv)], dtype=np.int32)
+    E = rng.standard_normal((n_cv, s1, s2))
+    E[::7, 0, 0] *= 1e-16  # under the element threshold: dropped
+    E[3] = 0.0             # a zero element adds nothing
+    colptr, r, v = oracle.assemble_blocks(n_cv, ndof, [(rows, cols, E)])
+    A = np.zeros((ndof, ndof))
+    present = np.zeros((ndof, ndof), dtype=bool)
+    for e in range(n_cv):
+        ninf = np.abs(E[e]).max()
+        if ninf == 0.0:
+            continue
+        keep = np.abs(E[e]) > 1e-14 * ninf
+        for i in range(s1):
+            for j in range(s2):
+                if keep[i, j]:
+                    A[rows[e, i], cols[e, j]] += E[e, i, j]
+                    present[rows[e, i], cols[e, j]] = True
+    assert colptr[-1] == present.sum()
+    assert np.array_equal(dense(colptr, r, np.ones_like(v), ndof) != 0, present)
+    assert np.abs(dense(colptr, r, v, ndof) - A).max() <= 1e-14 * np.abs(A).max()
+    for c in range(ndof):  # GetFEM's rsvector: sorted rows in every column
+        assert np.all(np.diff(r[colptr[c]:colptr[c + 1]]) > 0)
+
+
+def test_numpy_laplacian_equals_c_oracle():
+    pts, cvx = mesh(2)
+    dofs, ndof = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 2, 1)
+    prob = oracle.Problem(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 2, 1, dofs, ndof, "IM_TETRAHEDRON(5)",
+                          oracle.TERM_LAPLACE, scal_a=1.0)
+    f = (oracle.FEM_PK, 2, 1, dofs)
+    E = oracle.block_element_matrices(pts, cvx[:12], "IM_TETRAHEDRON(5)", f, f, {}, lap)
+    for e in range(12):
+        C = prob.element_matrix(e)
+        assert np.abs(E[e] - C).max() <= 1e-13 * np.abs(C).max()
+
+
+def test_p1_mass_closed_form():
+    pts, cvx = mesh(2)
+    dofs, _ = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 1, 1)
+    f = (oracle.FEM_PK, 1, 1, dofs)
+
+    def mass(vals):
+        D = np.zeros((4, 4))
+        D[0, 0] = 1.0
+        return D
+
+    E = oracle.block_element_matrices(pts, cvx, "IM_TETRAHEDRON(2)", f, f, {}, mass)
+    for e in range(cvx.shape[0]):
+        J, _ = oracle.tet_geometry(pts, cvx[e])
+        vol = abs(J) / 6.0
+        assert np.abs(E[e] - vol / 20.0 * (np.ones((4, 4)) + np.eye(4))).max() <= 1e-15
+
+
+@pytest.mark.parametrize("rule", ["IM_TETRAHEDRON(2)", "IM_TETRAHEDRON(5)"])
+def test_p1_p2_coupling_exact(rule):
+    """Test_p * Div_Test2_u, p P1 scalar, u P2 vector: degree 2, exact under
+    both rules; with u = (x, 2y, -z) (div u = 2) the block applied to u's
+    nodal values gives 2 int phi_i = 2 |T| / 4 per vertex."""
+    pts, cvx = mesh(2)
+    dp, _ = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 1, 1)
+    du, _ = oracle.enumerate_dof(oracle.GT_PK31, pts, cvx, oracle.FEM_PK, 2, 3)
+
+    def coupling(vals):
+        D = np.zeros((4, 12))
+        for a in range(3):
+            D[0, 3 + 3 * a + a] = 1.0
+        return D
+
+    E = oracle.block_element_matrices(pts, cvx, rule, (oracle.FEM_PK, 1, 1, dp), (oracle.FEM_PK, 2, 3, du),
+                                      {}, coupling)
+    nodes = oracle.fem_nodes(oracle.FEM_PK, 2)
+    for e in range(cvx.shape[0]):
+        J, B = oracle.tet_geometry(pts, cvx[e])
+        X = pts[cvx[e]]
+        x = X[0] + nodes @ (X[1:] - X[0])  # P2 node positions
+        u = np.stack([x[:, 0], 2 * x[:, 1], -x[:, 2]], axis=1).reshape(-1)
+        assert np.abs(E[e] @ u - 2 * abs(J) / 6.0 / 4.0).max() <= 1e-14
+    if rule == "IM_TETRAHEDRON(5)":
+        E2 = oracle.block_element_matrices(pts, cvx, "IM_TETRAHEDRON(2)", (oracle.FEM_PK, 1, 1, dp),
+                                           (oracle.FEM_PK, 2, 3, du), {}, coupling)
+        assert np.abs(E - E2).max() <= 1e-15
