@@ -115,6 +115,7 @@ SIGNATURES = {
     "gfx_ws_describe": (_i32, [_P, ctypes.c_char_p, _i64]),
     "gfx_ws_solve_cg": (_i32, [_P, _P, _P, _f64, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_f64)]),
     "gfx_ws_dof_constraints": (_i32, [_P, _i64, _P, _P, _P, _P, _i32]),
+    "gfx_ws_solve_gmres": (_i32, [_P, _P, _P, _f64, _i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_f64)]),
     "gfx_compliance_create": (_i32, [_P, _P, _f64, _f64, _f64, _f64, _pp]),
     "gfx_compliance_eval": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "gfx_compliance_eval_noalloc": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),

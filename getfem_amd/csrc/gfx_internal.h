@@ -298,6 +298,10 @@ hipError_t launch_dof_constraints(hipStream_t s, int64_t n, const int64_t *indpt
                                   double *vals, const uint8_t *fixed, const double *go, const double *pr,
                                   double *rhs, int flags, int *err);
 size_t cg_workspace_doubles(int64_t n);
+size_t gmres_workspace_doubles(int64_t n, int m);
+hipError_t gmres_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
+                       const double *vals, const double *b, double *x, double rtol, int maxit, int m,
+                       double *work, int *iters, double *rel_res);
 hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
                     const double *vals, const double *b, double *x, double rtol, int maxit, int qb, double *work,
                     int *iters, double *rel_res);

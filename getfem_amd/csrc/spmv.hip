@@ -13,6 +13,10 @@
 // reads r.r once per iteration for the stopping test.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
 #include "gfx_internal.h"
 
 namespace gfx {
@@ -426,6 +430,235 @@ hipError_t launch_fill_f64(hipStream_t s, int64_t n, double *p, double v) {
 
 // preconditioner storage (3 n: room for the 3x3 blocks) + r, z, p, q + partials
 size_t cg_workspace_doubles(int64_t n) { return 7 * (size_t)n + 3 * kRedBlocks + 8; }
+
+// ------------------------------------------------------------------ GMRES
+namespace {
+
+constexpr int kGmBlocks = 256;  // partial sums per dot product (fixed order)
+constexpr int kGmRows = 8;      // basis vectors per block of the multi-dot
+
+// w = dinv (b - w): the preconditioned residual (w holds A x on entry)
+__global__ void k_gm_residual(int64_t n, const double *__restrict__ b, const double *__restrict__ dinv,
+                              double *__restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = dinv[i] * (b[i] - w[i]);
+}
+
+// a row without diagonal (or a zero one) is left unscaled
+__global__ void k_gm_fix_dinv(int64_t n, double *__restrict__ dinv) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && dinv[i] == 0.0) dinv[i] = 1.0;
+}
+
+// y = dinv y
+__global__ void k_gm_scale_diag(int64_t n, const double *__restrict__ dinv, double *__restrict__ y) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] *= dinv[i];
+}
+
+// partials of V_k . w for the basis vectors k0 .. k0 + kGmRows - 1 (k < nk)
+__global__ __launch_bounds__(kRedThreads) void k_gm_multidot(int64_t n, int nk, const double *__restrict__ V,
+                                                             const double *__restrict__ w,
+                                                             double *__restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  const int k0 = blockIdx.y * kGmRows;
+  double acc[kGmRows];
+  for (int r = 0; r < kGmRows; ++r) acc[r] = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (int64_t)kGmBlocks * kRedThreads) {
+    const double wi = w[i];
+    for (int r = 0; r < kGmRows; ++r)
+      if (k0 + r < nk) acc[r] += V[(int64_t)(k0 + r) * n + i] * wi;
+  }
+  for (int r = 0; r < kGmRows; ++r) {
+    const double t = block_sum(acc[r], sh);  // uniform: every thread takes part
+    if (threadIdx.x == 0 && k0 + r < nk) part[(int64_t)(k0 + r) * kGmBlocks + blockIdx.x] = t;
+  }
+}
+
+// h_k (+)= sum of the partials of k, in a fixed order
+__global__ __launch_bounds__(kRedThreads) void k_gm_sum(int nk, const double *__restrict__ part,
+                                                        double *__restrict__ h, int accumulate) {
+  __shared__ double sh[kRedThreads];
+  const int k = blockIdx.x;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < kGmBlocks; i += kRedThreads) s += part[(int64_t)k * kGmBlocks + i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0 && k < nk) h[k] = accumulate ? h[k] + s : s;
+}
+
+// w -= sum_k h_k V_k
+__global__ void k_gm_project(int64_t n, int nk, const double *__restrict__ V, const double *__restrict__ h,
+                             double *__restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double t = w[i];
+  for (int k = 0; k < nk; ++k) t -= h[k] * V[(int64_t)k * n + i];
+  w[i] = t;
+}
+
+// out = w * (1 / a)  with a = *anorm (0 leaves zeros)
+__global__ void k_gm_normalize(int64_t n, const double *__restrict__ anorm, const double *__restrict__ w,
+                               double *__restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const double a = *anorm;
+  if (i < n) out[i] = a != 0.0 ? w[i] * (1.0 / a) : 0.0;
+}
+
+// x += sum_k y_k V_k
+__global__ void k_gm_combine(int64_t n, int nk, const double *__restrict__ V, const double *__restrict__ y,
+                             double *__restrict__ x) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double t = x[i];
+  for (int k = 0; k < nk; ++k) t += y[k] * V[(int64_t)k * n + i];
+  x[i] = t;
+}
+
+// ||w||^2 partials then the norm into *out
+__global__ __launch_bounds__(kRedThreads) void k_gm_norm(int64_t n, const double *__restrict__ w,
+                                                         double *__restrict__ part) {
+  __shared__ double sh[kRedThreads];
+  double s = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * kRedThreads + threadIdx.x; i < n; i += (int64_t)kGmBlocks * kRedThreads)
+    s += w[i] * w[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) part[blockIdx.x] = s;
+}
+__global__ __launch_bounds__(kRedThreads) void k_gm_norm_final(const double *__restrict__ part,
+                                                               double *__restrict__ out) {
+  __shared__ double sh[kRedThreads];
+  double s = 0.0;
+  for (int i = threadIdx.x; i < kGmBlocks; i += kRedThreads) s += part[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) *out = sqrt(s);
+}
+
+// Givens rotation zeroing b in (a, b), as gmm::Givens_rotation
+void givens(double a, double b, double &c, double &s) {
+  if (b == 0.0) { c = 1.0; s = 0.0; return; }
+  if (std::fabs(b) > std::fabs(a)) {
+    const double t = a / b;
+    s = 1.0 / std::sqrt(1.0 + t * t);
+    c = s * t;
+  } else {
+    const double t = b / a;
+    c = 1.0 / std::sqrt(1.0 + t * t);
+    s = c * t;
+  }
+}
+void rotate(double &x, double &y, double c, double s) {
+  const double t = c * x + s * y;
+  y = c * y - s * x;
+  x = t;
+}
+
+}  // namespace
+
+size_t gmres_workspace_doubles(int64_t n, int m) {
+  return (size_t)(m + 1) * (size_t)n + 3 * (size_t)n + (size_t)(m + 2) * kGmBlocks + 2 * (size_t)(m + 2) + 8;
+}
+
+// gmm::gmres (src/gmm/gmm_solver_gmres.h:85-157): restarted, left
+// preconditioned (here by the inverse diagonal), the stopping test on the
+// preconditioned residual relative to ||M b||, a restart that makes no
+// progress ten times in a row ends the solve.  The basis is orthogonalised by
+// classical Gram-Schmidt applied twice (two batched passes per step, where
+// gmm runs modified Gram-Schmidt one vector at a time): the same subspace,
+// with the same orthogonality to working precision.  The Hessenberg matrix,
+// its Givens rotations and the triangular solve live on the host.
+hipError_t gmres_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
+                       const double *vals, const double *b, double *x, double rtol, int maxit, int m,
+                       double *work, int *iters, double *rel_res) {
+  *iters = 0;
+  *rel_res = 0.0;
+  if (n == 0) return hipSuccess;
+  if (m < 1) return hipErrorInvalidValue;
+  double *V = work, *dinv = V + (size_t)(m + 1) * n, *w = dinv + n, *u = w + n;
+  double *part = u + n, *hd = part + (size_t)(m + 2) * kGmBlocks, *yd = hd + (m + 2);
+  double *anorm = yd + (m + 2);
+  hipError_t e;
+  const unsigned nb = nb256(n);
+  auto norm = [&](const double *v, double &out) -> hipError_t {
+    k_gm_norm<<<kGmBlocks, kRedThreads, 0, s>>>(n, v, part);
+    k_gm_norm_final<<<1, kRedThreads, 0, s>>>(part, anorm);
+    hipError_t er = hipMemcpyAsync(&out, anorm, 8, hipMemcpyDeviceToHost, s);
+    if (er == hipSuccess) er = hipStreamSynchronize(s);
+    return er;
+  };
+  k_inv_diag<<<nb, 256, 0, s>>>(n, indptr, indices, vals, dinv);
+  k_gm_fix_dinv<<<nb, 256, 0, s>>>(n, dinv);
+  // ||M b||
+  if ((e = hipMemcpyAsync(w, b, 8 * (size_t)n, hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
+  k_gm_scale_diag<<<nb, 256, 0, s>>>(n, dinv, w);
+  double bn = 0.0;
+  if ((e = norm(w, bn)) != hipSuccess) return e;
+  if (bn == 0.0) {
+    if ((e = hipMemsetAsync(x, 0, 8 * (size_t)n, s)) != hipSuccess) return e;
+    return hipStreamSynchronize(s);
+  }
+  auto residual = [&](double &beta) -> hipError_t {  // w = M (b - A x), beta = ||w||
+    hipError_t er = launch_spmv(s, n, nnz, indptr, indices, vals, x, w, false);
+    if (er != hipSuccess) return er;
+    k_gm_residual<<<nb, 256, 0, s>>>(n, b, dinv, w);
+    return norm(w, beta);
+  };
+  double beta = 0.0;
+  if ((e = residual(beta)) != hipSuccess) return e;
+  double beta_old = beta;
+  int blocked = 0, it = 0;
+  std::vector<double> H((size_t)(m + 1) * m), cr(m + 1), sr(m + 1), g(m + 1), h(m + 2);
+  while (beta / bn > rtol && it < maxit) {
+    k_gm_normalize<<<nb, 256, 0, s>>>(n, anorm, w, V);  // anorm holds beta
+    std::fill(g.begin(), g.end(), 0.0);
+    g[0] = beta;
+    int i = 0;
+    do {
+      // u = M A V_i, orthogonalised against V_0..V_i twice
+      if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, V + (size_t)i * n, u, false)) != hipSuccess) return e;
+      k_gm_scale_diag<<<nb, 256, 0, s>>>(n, dinv, u);
+      const int nk = i + 1;
+      const dim3 grid(kGmBlocks, (unsigned)((nk + kGmRows - 1) / kGmRows));
+      for (int pass = 0; pass < 2; ++pass) {
+        k_gm_multidot<<<grid, kRedThreads, 0, s>>>(n, nk, V, u, part);
+        k_gm_sum<<<nk, kRedThreads, 0, s>>>(nk, part, pass == 0 ? hd : yd, 0);
+        k_gm_project<<<nb, 256, 0, s>>>(n, nk, V, pass == 0 ? hd : yd, u);
+      }
+      if ((e = hipGetLastError()) != hipSuccess) return e;
+      double a = 0.0;
+      if ((e = norm(u, a)) != hipSuccess) return e;
+      if ((e = hipMemcpy(h.data(), hd, 8 * (size_t)nk, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+      std::vector<double> h2((size_t)nk);
+      if ((e = hipMemcpy(h2.data(), yd, 8 * (size_t)nk, hipMemcpyDeviceToHost)) != hipSuccess) return e;
+      double *col = &H[(size_t)i * (m + 1)];  // column i of H (column-major)
+      for (int k = 0; k < nk; ++k) col[k] = h[k] + h2[k];
+      col[i + 1] = a;
+      k_gm_normalize<<<nb, 256, 0, s>>>(n, anorm, u, V + (size_t)(i + 1) * n);
+      for (int k = 0; k < i; ++k) rotate(col[k], col[k + 1], cr[k], sr[k]);
+      givens(col[i], col[i + 1], cr[i], sr[i]);
+      rotate(col[i], col[i + 1], cr[i], sr[i]);
+      rotate(g[i], g[i + 1], cr[i], sr[i]);
+      ++i;
+      ++it;
+    } while (std::fabs(g[i]) / bn > rtol && i < m && it < maxit);
+    // y = H(0:i, 0:i)^{-1} g, x += V y
+    std::vector<double> y(g.begin(), g.begin() + i);
+    for (int r = i - 1; r >= 0; --r) {
+      for (int c = r + 1; c < i; ++c) y[r] -= H[(size_t)c * (m + 1) + r] * y[c];
+      y[r] /= H[(size_t)r * (m + 1) + r];
+    }
+    if ((e = hipMemcpyAsync(yd, y.data(), 8 * (size_t)i, hipMemcpyHostToDevice, s)) != hipSuccess) return e;
+    k_gm_combine<<<nb, 256, 0, s>>>(n, i, V, yd, x);
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;  // y is read before it changes
+    beta_old = std::min(beta, beta_old);
+    if ((e = residual(beta)) != hipSuccess) return e;
+    if (i < m - 1 || beta_old <= beta) ++blocked;
+    else blocked = 0;
+    if (blocked > 10) break;
+  }
+  *iters = it;
+  *rel_res = beta / bn;
+  return hipSuccess;
+}
 
 
 hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,

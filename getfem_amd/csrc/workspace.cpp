@@ -168,10 +168,11 @@ struct gfx_workspace {
   const int32_t *ext_ind = nullptr;
   double *ext_vals = nullptr;
   // solves and dof constraints on the workspace's matrix
-  double *d_cg = nullptr, *d_go = nullptr;
+  double *d_cg = nullptr, *d_go = nullptr, *d_gm = nullptr;
   uint8_t *d_fixed = nullptr;
   int *d_cerr = nullptr;
   int64_t cg_n = 0, con_n = 0;
+  size_t gm_size = 0;
 };
 
 namespace {
@@ -991,7 +992,7 @@ int gfx_ws_destroy(gfx_workspace *w) {
   (void)hipSetDevice(w->ctx->device);
   drop_plans(w);
   wfree(w->d_u); wfree(w->d_V);
-  wfree(w->d_cg); wfree(w->d_go); wfree(w->d_fixed); wfree(w->d_cerr);
+  wfree(w->d_cg); wfree(w->d_go); wfree(w->d_fixed); wfree(w->d_cerr); wfree(w->d_gm);
   sc_release(w->sc);
   delete w->sc;
   for (auto &kv : w->vars) {
@@ -1367,6 +1368,36 @@ int gfx_ws_solve_cg(gfx_workspace *w, const double *d_b, double *d_x, double rto
     w->cg_n = n;
   }
   WHIP(w, cg_solve(w->ctx->stream, n, nnz, ip, ix, v, d_b, d_x, rtol, maxit, 1, w->d_cg, iters, rel_res));
+  return GFX_OK;
+}
+
+// restarted GMRES on the workspace's assembled matrix: what
+// default_linear_solver gives a model that is not coercive
+// (linear_solver_gmres_preconditioned_ilu, src/getfem/getfem_model_solvers.h:
+// 101-109, restart 500); the preconditioner here is the inverse diagonal.
+// The Krylov basis takes (restart + 1) n doubles: the restart is lowered to
+// keep it within 4 GiB.
+int gfx_ws_solve_gmres(gfx_workspace *w, const double *d_b, double *d_x, double rtol, int maxit, int restart,
+                       int *iters, double *rel_res) {
+  if (!w || !d_b || !d_x || !iters || !rel_res || !(rtol > 0) || maxit < 0 || restart < 1) return GFX_E_INVALID;
+  int64_t n = 0, nnz = 0;
+  const int64_t *ip = nullptr;
+  const int32_t *ix = nullptr;
+  double *v = nullptr;
+  int r = gfx_ws_device_csr(w, &n, &nnz, &ip, &ix, &v);
+  if (r) return r;
+  if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
+  int m = std::max(1, std::min(restart, std::max(maxit, 1)));
+  const int64_t cap = (int64_t(4) << 30) / 8 / std::max<int64_t>(n, 1) - 1;
+  if (m > cap) m = (int)std::max<int64_t>(cap, 1);
+  const size_t need = gmres_workspace_doubles(n, m);
+  if (need > w->gm_size) {
+    WHIP(w, hipStreamSynchronize(w->ctx->stream));
+    wfree(w->d_gm);
+    if (hipMalloc((void **)&w->d_gm, 8 * need) != hipSuccess) return wfail(w, GFX_E_OOM, "GMRES basis");
+    w->gm_size = need;
+  }
+  WHIP(w, gmres_solve(w->ctx->stream, n, nnz, ip, ix, v, d_b, d_x, rtol, maxit, m, w->d_gm, iters, rel_res));
   return GFX_OK;
 }
 

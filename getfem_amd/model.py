@@ -4,8 +4,9 @@ assembly terms and Dirichlet conditions by simplification.
 
 Variables are laid out one after the other in the global vector in the
 order of their names, as model::resize_global_system gives each its
-interval from the name-ordered variable map (src/getfem_models.cc:305-316).  One variable whose terms the matrix kernels cover assembles and
-solves through that tree's plan (3x3 block Jacobi CG for a vector field);
+interval from the name-ordered variable map (src/getfem_models.cc:305-316).
+One variable whose terms the matrix kernels cover assembles and solves
+through that tree's plan (3x3 block Jacobi CG for a vector field);
 otherwise (several variables, coupling blocks, polynomial nonlinearities)
 the workspace takes the polynomial route, the tangent is the matrix over
 all intervals and every solve is a Newton iteration on the residual form
@@ -16,7 +17,8 @@ device by the workspace (ga_workspace inside model::assembly,
 src/getfem_models.cc:2646-2736, one device call instead of one per OpenMP
 partition); the dof constraints are applied to the device CSR
 (src/getfem_models.cc:2806-2870); the linear systems are solved on the device
-by block-Jacobi-preconditioned CG; the Newton loop follows classical_Newton
+by preconditioned CG (coercive models) or restarted GMRES (the others); the
+Newton loop follows classical_Newton
 (src/getfem/getfem_model_solvers.h:548-605) with basic_newton_line_search
 (:336-360).  The state vector stays on the device between iterations; the
 host sees scalars only.  A Newton iteration's tangent and residual come from
@@ -46,7 +48,7 @@ class Model:
         self.nvec = 0
         self.data = {}
         self.terms = []  # (expr, mim, linear, symmetric, coercive)
-        self.dirichlet = []  # (dofs, dataname or None)
+        self.dirichlet = []  # (variable, dofs, dataname or None)
         self.U = None  # device state (torch)
         self._rhs = None
         self._built = False
@@ -220,30 +222,36 @@ class Model:
     def rhs(self):
         return self._rhs.cpu().numpy()
 
-    def _cg(self, rhs, dx, rtol, maxit):
+    def _linear_solve(self, rhs, dx, rtol, maxit):
+        """default_linear_solver's iterative branch
+        (src/getfem/getfem_model_solvers.h:652-660): CG for a coercive model
+        (block Jacobi on a single plan, the diagonal on the workspace's
+        matrix), restarted GMRES(500) otherwise (diagonal preconditioner)."""
         it, res = ctypes.c_int(), ctypes.c_double()
-        args = (ctypes.c_void_p(rhs.data_ptr()), ctypes.c_void_p(dx.data_ptr()), rtol, maxit, ctypes.byref(it),
-                ctypes.byref(res))
-        if self._multi():
-            check(lib().gfx_ws_solve_cg(self.ws.h, *args), self.ctx.h)
+        b, x = ctypes.c_void_p(rhs.data_ptr()), ctypes.c_void_p(dx.data_ptr())
+        if not (self.is_symmetric() and self.is_coercive()):
+            check(lib().gfx_ws_solve_gmres(self.ws.h, b, x, rtol, maxit, 500, ctypes.byref(it), ctypes.byref(res)),
+                  self.ctx.h)
+        elif self._multi():
+            check(lib().gfx_ws_solve_cg(self.ws.h, b, x, rtol, maxit, ctypes.byref(it), ctypes.byref(res)), self.ctx.h)
         else:
-            check(lib().gfx_plan_solve_cg(self._plan(), *args), self.ctx.h)
+            check(lib().gfx_plan_solve_cg(self._plan(), b, x, rtol, maxit, ctypes.byref(it), ctypes.byref(res)),
+                  self.ctx.h)
         return it.value, res.value
 
     # ------------------------------------------------------------------- solve
     def solve(self, max_iter=100, max_res=1e-6, cg_maxit=20000, noisy=False):
-        """getfem::standard_solve for this model: a linear model is one CG solve
-        of K U = rhs; otherwise classical_Newton with basic_newton_line_search.
-        The systems must be symmetric positive definite (CG).  Returns
-        (newton iterations, converged, list of (cg iterations, cg residual))."""
+        """getfem::standard_solve for this model: a linear model of one
+        variable is one linear solve of K U = rhs; otherwise classical_Newton
+        with basic_newton_line_search.  Linear solves: CG for symmetric
+        coercive models, GMRES otherwise.  Returns (newton iterations,
+        converged, list of (linear solver iterations, residual))."""
         import torch
-        if not self.is_symmetric():
-            raise ValueError("the device solver is CG: declare the terms symmetric (is_symmetric=True)")
         cg_rtol = max_res / 20.0  # iter_linsolv0.set_resmax(iter.get_resmax()/20)
         log = []
         if self.is_linear() and not self._multi():
             rhs = self.assembly("build_all")
-            it, res = self._cg(rhs, self.U, cg_rtol, cg_maxit)
+            it, res = self._linear_solve(rhs, self.U, cg_rtol, cg_maxit)
             log.append((it, res))
             return 1, res <= cg_rtol, log
         eln = self.approx_external_load() or 1.0
@@ -254,7 +262,7 @@ class Model:
         while crit > max_res and k < max_iter:
             rhs = self.assembly("build_all")
             dU = torch.zeros_like(self.U)
-            it, cres = self._cg(rhs, dU, cg_rtol, cg_maxit)
+            it, cres = self._linear_solve(rhs, dU, cg_rtol, cg_maxit)
             log.append((it, cres))
             if cres > cg_rtol:
                 break

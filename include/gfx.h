@@ -420,6 +420,17 @@ int gfx_ws_describe(gfx_workspace *w, char *buf, int64_t len);
  * 652-655) */
 int gfx_ws_solve_cg(gfx_workspace *w, const double *d_b, double *d_x, double rtol, int maxit, int *iters,
                     double *rel_res);
+/* Restarted GMRES on the workspace's assembled matrix, after gmm::gmres
+ * (src/gmm/gmm_solver_gmres.h:85-157), the solver default_linear_solver gives
+ * a model that is not coercive (linear_solver_gmres_preconditioned_ilu,
+ * src/getfem/getfem_model_solvers.h:101-109, :652-660; restart 500 there).
+ * Left preconditioned by the inverse diagonal (GetFEM: ILU); the stopping
+ * test is ||M (b - K x)|| <= rtol ||M b||; ten restarts without progress end
+ * it; maxit bounds the total iterations.  The Krylov basis holds (restart +
+ * 1) nb_dof doubles, the restart lowered to keep it within 4 GiB.  d_x:
+ * initial guess in, solution out; iters and rel_res as gfx_plan_solve_cg. */
+int gfx_ws_solve_gmres(gfx_workspace *w, const double *d_b, double *d_x, double rtol, int maxit, int restart,
+                       int *iters, double *rel_res);
 /* gfx_plan_dof_constraints on the workspace's assembled matrix (dofs are
  * global: the variables' intervals) */
 int gfx_ws_dof_constraints(gfx_workspace *w, int64_t n_fixed, const int32_t *h_dofs, const double *h_go,

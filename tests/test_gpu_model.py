@@ -223,3 +223,72 @@ def test_two_variable_model_matches_host_newton(ctx, c):
     print("newton", its, "cg", log, "rel err vs host", err)
     assert ok and err <= 1e-8
     assert np.abs(U[dofs]).max() == 0.0
+
+
+def test_mixed_pressure_model_gmres(ctx):
+    """Nearly incompressible elasticity in mixed form, u P2 vector and p P1:
+    2 mu Sym(Grad_u):Grad_Test_u - p Div_Test_u - Test_p Div_u - p Test_p / lambda,
+    symmetric but indefinite (not coercive), so the model solves it with
+    GMRES; u clamped at x = 0 and pulled at x = 1.  Against the host direct
+    solve of the oracle's block matrices.  (lambda = 100: with the diagonal
+    preconditioner the preconditioned residual bounds the error less tightly
+    as lambda grows.)"""
+    print("case: mixed u-p model, GMRES", flush=True)
+    n, mu, lam = 3, 1.0, 100.0
+    pts, cvx = meshes.regular_unit_mesh(n)
+    pts = meshes.jitter(pts, n)
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    mfu = ga.MeshFem(mesh, "FEM_PK(3,2)", 3)
+    mfp = ga.MeshFem(mesh, "FEM_PK(3,1)", 1)
+    mim = ga.MeshIm(mesh, IM5)
+    md = Model(ctx)
+    md.add_fem_variable("u", mfu)
+    md.add_fem_variable("p", mfp)
+    md.add_initialized_data("mu", mu)
+    md.add_initialized_data("ilam", 1.0 / lam)
+    md.add_initialized_data("pull", [0.02, 0.0, 0.0])
+    md.add_linear_term(mim, "2*mu*Sym(Grad_u):Grad_Test_u - p*Div_Test_u - Test_p*Div_u - ilam*p*Test_p",
+                       is_symmetric=True, is_coercive=False)
+    left, right = face_dofs(mfu, 0, 0.0), face_dofs(mfu, 0, 1.0)
+    md.add_Dirichlet_condition_with_simplification("u", left)
+    md.add_Dirichlet_condition_with_simplification("u", right, "pull")
+    its, ok, log = md.solve(max_res=1e-10)
+    npp, nu = mfp.nbdof(), mfu.nbdof()
+    X = np.concatenate([md.variable("p"), md.variable("u")])  # name order: p, then u
+    dp, du = mfp.basic_dof_table(), mfu.basic_dof_table()
+    rows_u = npp + (du[:, :, None] + np.arange(3)).reshape(cvx.shape[0], -1)
+    rows_p = dp
+    fu, fp = (oracle.FEM_PK, 2, 3, du), (oracle.FEM_PK, 1, 1, dp)
+
+    def uu(vals):
+        d = np.eye(3)
+        D = np.zeros((12, 12))
+        D[3:, 3:] = (mu * (np.einsum("ac,bd->abcd", d, d) + np.einsum("ad,bc->abcd", d, d))).reshape(9, 9)
+        return D
+
+    def up(vals):
+        D = np.zeros((12, 4))
+        for a in range(3):
+            D[3 + 4 * a, 0] = -1.0
+        return D
+
+    def pp(vals):
+        D = np.zeros((4, 4))
+        D[0, 0] = -1.0 / lam
+        return D
+
+    Euu = oracle.block_element_matrices(pts, cvx, IM5, fu, fu, {}, uu)
+    Eup = oracle.block_element_matrices(pts, cvx, IM5, fu, fp, {}, up)
+    Epp = oracle.block_element_matrices(pts, cvx, IM5, fp, fp, {}, pp)
+    N = npp + nu
+    cp, r, v = oracle.assemble_blocks(cvx.shape[0], N, [(rows_u, rows_u, Euu), (rows_u, rows_p, Eup),
+                                                        (rows_p, rows_u, np.transpose(Eup, (0, 2, 1))),
+                                                        (rows_p, rows_p, Epp)])
+    K = sp.csc_matrix((v, r, cp), shape=(N, N))
+    dofs = npp + np.concatenate([left, right])
+    g = np.concatenate([np.zeros(left.size), np.array([0.02, 0, 0])[right % 3]])
+    Ks, rhs = simplify(K, np.zeros(N), dofs, g, True)
+    Xref = spla.spsolve(Ks.tocsc(), rhs)
+    err = np.linalg.norm(X - Xref) / np.linalg.norm(Xref)
+    print("newton", its, "gmres", log, "rel err vs direct", err)
+    assert ok and err <= 1e-8
