@@ -415,6 +415,31 @@ __global__ __launch_bounds__(128) void k_mask_check(PatternArgs a) {
   const int i0 = a.inc_ptr[node], ninc = a.inc_ptr[node + 1] - i0;
   const int ns = a.nslots[node] & (kFullRows - 1);
   const int mw = (Q * Q * ns + 31) / 32;
+  if (a.nslots[node] & kFullRows) {
+    // a node stored as full: every keep bit of its rows set means every
+    // presence bit is set again (the converse needs the exact check below)
+    __shared__ int allset;
+    if (threadIdx.x == 0) allset = 1;
+    __syncthreads();
+    for (int t = threadIdx.x; t < ninc * Q; t += blockDim.x) {
+      const int k = t / Q, x = t - k * Q;
+      const uint32_t inc = a.inc[i0 + k];
+      const int64_t e = inc >> 5;
+      const int i = inc & 31;
+      const uint32_t *kw = a.keep + e * a.kwords;
+      const int b0 = (i * Q + x) * nl, b1 = b0 + nl;  // the row's bits [b0, b1)
+      bool ok = true;
+      for (int b = b0; b < b1;) {
+        const int w = b >> 5, lo = b & 31, hi = min(32, b1 - (w << 5));
+        const uint32_t m = (hi - lo == 32 ? 0xffffffffu : ((1u << (hi - lo)) - 1u)) << lo;
+        ok = ok && (kw[w] & m) == m;
+        b = (w << 5) + hi;
+      }
+      if (!ok) allset = 0;
+    }
+    __syncthreads();
+    if (allset) return;  // uniform
+  }
   for (int t = threadIdx.x; t < mw; t += blockDim.x) maskC[t] = 0u;
   __syncthreads();
   const int per_inc = Q * nd * Q;
