@@ -1569,6 +1569,7 @@ int gfx_gwfl_compile_scalar(const char *exprs, const char *decls, int order, con
 
 int gfx_ws_plan(gfx_workspace *w, gfx_plan **out) {
   if (!w || !out) return GFX_E_INVALID;
+  if (w->mat_fail) return wfail(w, GFX_E_STATE, "the polynomial route holds the workspace's matrix: no plan");
   std::vector<WsTree *> mt = matrix_trees(w);
   if (mt.empty()) return wfail(w, GFX_E_STATE, "no assembly done yet");
   if (mt.size() > 1)

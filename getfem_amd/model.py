@@ -52,6 +52,7 @@ class Model:
         self.U = None  # device state (torch)
         self._rhs = None
         self._built = False
+        self._route = None  # the workspace's route, known after its first assembly
 
     # -------------------------------------------------------------- variables
     def add_fem_variable(self, name, mf):
@@ -76,9 +77,19 @@ class Model:
             m, f = self.vars[k]
             self.ws.add_fem_variable(k, m, np.zeros(m.nbdof()), first=f)
         self._built = False
+        self._route = None
 
-    def _multi(self):
-        return len(self.vars) > 1
+    def _ws_route(self):
+        """True when the workspace assembles through the polynomial route
+        (several variables, or terms outside the matrix kernels): constraints
+        and solves then work on the workspace's matrix, and the right-hand
+        side is always the residual form.  Decided by the compile stage."""
+        if len(self.vars) > 1:
+            return True
+        if self._route is None:
+            h = ctypes.c_void_p()
+            self._route = lib().gfx_ws_plan(self.ws.h, ctypes.byref(h)) != 0
+        return self._route
 
     def _slice(self, name):
         mf, first = self.vars[name]
@@ -126,6 +137,7 @@ class Model:
             raise ValueError("only region -1 (all elements) is supported")
         self.terms.append((expr, mim, linear, sym, coercive))
         self.ws.add_expression(expr, mim, 2)
+        self._route = None
 
     def add_Dirichlet_condition_with_simplification(self, varname, dofs, dataname=None):
         """getfem::add_Dirichlet_condition_with_simplification
@@ -195,7 +207,7 @@ class Model:
         if build_matrix or not self._built:
             self.ws.assembly(2)
             self._built = True
-        if build_rhs and (self._multi() or not self.is_linear()):
+        if build_rhs and (self._ws_route() or not self.is_linear()):
             self.ws.assembly(1)
             check(lib().gfx_ws_copy_assembled_vector_device(self.ws.h, ctypes.c_void_p(rhs.data_ptr())),
                   self.ctx.h)
@@ -203,12 +215,12 @@ class Model:
             rhs.neg_()
         dofs, vals = self._constraints()
         flags = (_lib.BUILD_MATRIX if build_matrix else 0) | (_lib.BUILD_RHS if build_rhs else 0)
-        flags |= _lib.MODEL_LINEAR if (self.is_linear() and not self._multi()) else 0
+        flags |= _lib.MODEL_LINEAR if (self.is_linear() and not self._ws_route()) else 0
         flags |= _lib.MODEL_SYMMETRIC if self.is_symmetric() else 0
         if dofs.size:
             args = (dofs.size, ctypes.c_void_p(dofs.ctypes.data), ctypes.c_void_p(vals.ctypes.data),
                     ctypes.c_void_p(self.U.data_ptr()), ctypes.c_void_p(rhs.data_ptr()), flags)
-            if self._multi():
+            if self._ws_route():
                 check(lib().gfx_ws_dof_constraints(self.ws.h, *args), self.ctx.h)
             else:
                 check(lib().gfx_plan_dof_constraints(self._plan(), *args), self.ctx.h)
@@ -232,7 +244,7 @@ class Model:
         if not (self.is_symmetric() and self.is_coercive()):
             check(lib().gfx_ws_solve_gmres(self.ws.h, b, x, rtol, maxit, 500, ctypes.byref(it), ctypes.byref(res)),
                   self.ctx.h)
-        elif self._multi():
+        elif self._ws_route():
             check(lib().gfx_ws_solve_cg(self.ws.h, b, x, rtol, maxit, ctypes.byref(it), ctypes.byref(res)), self.ctx.h)
         else:
             check(lib().gfx_plan_solve_cg(self._plan(), b, x, rtol, maxit, ctypes.byref(it), ctypes.byref(res)),
@@ -249,11 +261,14 @@ class Model:
         import torch
         cg_rtol = max_res / 20.0  # iter_linsolv0.set_resmax(iter.get_resmax()/20)
         log = []
-        if self.is_linear() and not self._multi():
+        if self.is_linear() and len(self.vars) == 1:
             rhs = self.assembly("build_all")
-            it, res = self._linear_solve(rhs, self.U, cg_rtol, cg_maxit)
-            log.append((it, res))
-            return 1, res <= cg_rtol, log
+            if not self._ws_route():
+                it, res = self._linear_solve(rhs, self.U, cg_rtol, cg_maxit)
+                log.append((it, res))
+                return 1, res <= cg_rtol, log
+            # the polynomial route's rhs is the residual form: Newton below
+            # (one iteration for a linear model)
         eln = self.approx_external_load() or 1.0
         rhs = self.assembly("build_rhs")
         res = float(rhs.abs().sum())

@@ -292,3 +292,57 @@ def test_mixed_pressure_model_gmres(ctx):
     err = np.linalg.norm(X - Xref) / np.linalg.norm(Xref)
     print("newton", its, "gmres", log, "rel err vs direct", err)
     assert ok and err <= 1e-8
+
+
+def test_convection_diffusion_model_gmres(ctx):
+    """One scalar P2 variable with a convection term (not symmetric, outside
+    the matrix kernels: the polynomial route) and a source, u = 0 on the
+    boundary: GMRES through Newton's residual form, against the host direct
+    solve with the oracle's block matrix."""
+    print("case: convection-diffusion model, GMRES", flush=True)
+    n, beta, f = 3, np.array([3.0, -2.0, 1.0]), 5.0
+    pts, cvx = meshes.regular_unit_mesh(n)
+    pts = meshes.jitter(pts, n)
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    mf = ga.MeshFem(mesh, "FEM_PK(3,2)", 1)
+    mim = ga.MeshIm(mesh, IM5)
+    md = Model(ctx)
+    md.add_fem_variable("u", mf)
+    for name, val in zip(("bx", "by", "bz"), beta):
+        md.add_initialized_data(name, val)
+    md.add_initialized_data("f", f)
+    md.add_linear_term(mim, "Grad_u.Grad_Test_u + (bx*Grad_u(1) + by*Grad_u(2) + bz*Grad_u(3))*Test_u"
+                       " - f*Test_u")
+    x = mf.basic_dof_nodes()
+    bd = np.nonzero(np.any((np.abs(x) < 1e-10) | (np.abs(x - 1) < 1e-10), axis=1))[0]
+    md.add_Dirichlet_condition_with_simplification("u", bd)
+    its, ok, log = md.solve(max_res=1e-10)
+    U = md.variable("u")
+    dofs = mf.basic_dof_table()
+    fu = (oracle.FEM_PK, 2, 1, dofs)
+
+    def conv(vals):
+        D = np.zeros((4, 4))
+        D[1:, 1:] = np.eye(3)
+        D[0, 1:] = beta
+        return D
+
+    def mass(vals):
+        D = np.zeros((4, 4))
+        D[0, 0] = 1.0
+        return D
+
+    nd = mf.nbdof()
+    E = oracle.block_element_matrices(pts, cvx, IM5, fu, fu, {}, conv)
+    M = oracle.block_element_matrices(pts, cvx, IM5, fu, fu, {}, mass)
+    cp, r, v = oracle.assemble_blocks(cvx.shape[0], nd, [(dofs, dofs, E)])
+    cm, rm, vm = oracle.assemble_blocks(cvx.shape[0], nd, [(dofs, dofs, M)])
+    K = sp.csc_matrix((v, r, cp), shape=(nd, nd)).tolil()
+    F = f * np.asarray(sp.csc_matrix((vm, rm, cm), shape=(nd, nd)).sum(axis=1)).ravel()
+    K[bd, :] = 0
+    K[bd, bd] = 1.0
+    F[bd] = 0.0
+    Uref = spla.spsolve(K.tocsc(), F)
+    err = np.linalg.norm(U - Uref) / np.linalg.norm(Uref)
+    print("newton", its, "gmres", log, "rel err vs direct", err)
+    assert ok and err <= 1e-8
