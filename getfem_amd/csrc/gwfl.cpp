@@ -369,14 +369,26 @@ struct Parser {
       expect(']');
       break;
     }
+    // GWFL's explicit tensors: [a,b,c] and [a;b;c] are vectors, [a,b;c,d] a
+    // matrix by lines, [[a,c],[b,d]] the same matrix by columns
+    if (rows.size() == 1 && !rows[0].empty() && rows[0][0].order() == 1) {
+      const int m = rows[0][0].shape[0], n = (int)rows[0].size();
+      Tensor r = zeros({m, n});
+      for (int j = 0; j < n; ++j) {
+        const Tensor &col = rows[0][j];
+        if (col.order() != 1 || col.shape[0] != m) fail("ragged [[...],[...]]");
+        for (int i = 0; i < m; ++i) r.v[i * n + j] = col.v[i];
+      }
+      return r;
+    }
     for (auto &r : rows)
       for (auto &t : r)
-        if (t.order() != 0) fail("only scalar entries in [...]");
-    if (rows[0].size() == 1) {  // column vector
-      Tensor r = zeros({(int)rows.size()});
+        if (t.order() != 0) fail("only scalar entries or column vectors in [...]");
+    if (rows.size() == 1 || rows[0].size() == 1) {  // a vector
+      Tensor r = zeros({(int)(rows.size() == 1 ? rows[0].size() : rows.size())});
       for (size_t i = 0; i < rows.size(); ++i) {
-        if (rows[i].size() != 1) fail("ragged [...]");
-        r.v[i] = rows[i][0].v[0];
+        if (rows.size() > 1 && rows[i].size() != 1) fail("ragged [...]");
+        for (size_t j = 0; j < rows[i].size(); ++j) r.v[i + j] = rows[i][j].v[0];
       }
       return r;
     }

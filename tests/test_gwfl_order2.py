@@ -133,3 +133,18 @@ def test_expression_with_two_test_variables_splits_by_block():
         expect = np.zeros((4, 4))
         expect[0, 0] = sign * k
         assert np.abs(D - expect).max() <= 1e-15
+
+
+def test_explicit_vectors_and_matrices():
+    """GWFL's explicit tensors (doc/sphinx/source/userdoc/gasm_high.rst:
+    511-520): [a,b,c] and [a;b;c] are vectors, [a,b;c,d] a matrix by lines
+    and [[a,c],[b,d]] the same matrix by columns."""
+    decls = "u:var:1;bx:const:1;by:const:1;bz:const:1"
+    v = random_values(np.random.default_rng(2), decls)
+    expect = block(["(bx*Grad_u(1)+by*Grad_u(2)+bz*Grad_u(3))*Test_u"], decls, "u", 1, "u", 1, v)
+    assert np.any(expect != 0)
+    for e in ["([bx,by,bz].Grad_u)*Test_u", "([bx;by;bz].Grad_u)*Test_u"]:
+        assert np.array_equal(block([e], decls, "u", 1, "u", 1, v), expect)
+    cols = block(["([[1,2],[3,4]]:[1,3;2,4])*u*Test_u"], "u:var:1", "u", 1, "u", 1, {"u:V_": 0.0})
+    rows = block(["([[1,2],[3,4]]:[1,2;3,4])*u*Test_u"], "u:var:1", "u", 1, "u", 1, {"u:V_": 0.0})
+    assert cols[0, 0] == 30.0 and rows[0, 0] == 29.0
