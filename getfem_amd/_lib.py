@@ -21,7 +21,7 @@ FORM_FACTORIZED, FORM_PERQP = 0, 1
 PREC_FP64, PREC_FP32, PREC_BF16X2, PREC_FP64_MFMA, PREC_FP32_MFMA = 0, 1, 2, 3, 4
 COEF_CONST, COEF_PER_ELEMENT = 0, 1
 BUILD_MATRIX, BUILD_RHS, MODEL_LINEAR, MODEL_SYMMETRIC = 1, 2, 4, 8
-PRECOND_BLOCK_JACOBI, PRECOND_ILDLT = 0, 1
+PRECOND_BLOCK_JACOBI, PRECOND_ILDLT, PRECOND_ILU, PRECOND_DIAGONAL = 0, 1, 2, 3
 
 # every symbol include/gfx.h declares, with (restype, argtypes)
 _P = ctypes.c_void_p
@@ -115,7 +115,10 @@ SIGNATURES = {
     "gfx_ws_describe": (_i32, [_P, ctypes.c_char_p, _i64]),
     "gfx_ws_solve_cg": (_i32, [_P, _P, _P, _f64, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_f64)]),
     "gfx_ws_dof_constraints": (_i32, [_P, _i64, _P, _P, _P, _P, _i32]),
-    "gfx_ws_solve_gmres": (_i32, [_P, _P, _P, _f64, _i32, _i32, ctypes.POINTER(_i32), ctypes.POINTER(_f64)]),
+    "gfx_ws_ilu_factor": (_i32, [_P, ctypes.POINTER(_i64)]),
+    "gfx_ws_ilu_apply": (_i32, [_P, _P, _P]),
+    "gfx_ws_solve_gmres": (_i32, [_P, _i32, _P, _P, _f64, _i32, _i32, ctypes.POINTER(_i32),
+                                  ctypes.POINTER(_f64)]),
     "gfx_compliance_create": (_i32, [_P, _P, _f64, _f64, _f64, _f64, _pp]),
     "gfx_compliance_eval": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),
     "gfx_compliance_eval_noalloc": (_i32, [_P, _P, _P, _P, _P, _P, _P, _P]),

@@ -299,9 +299,10 @@ hipError_t launch_dof_constraints(hipStream_t s, int64_t n, const int64_t *indpt
                                   double *rhs, int flags, int *err);
 size_t cg_workspace_doubles(int64_t n);
 size_t gmres_workspace_doubles(int64_t n, int m);
+struct IluData;
 hipError_t gmres_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
                        const double *vals, const double *b, double *x, double rtol, int maxit, int m,
-                       double *work, int *iters, double *rel_res);
+                       double *work, const IluData *ilu, int *iters, double *rel_res);
 hipError_t cg_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
                     const double *vals, const double *b, double *x, double rtol, int maxit, int qb, double *work,
                     int *iters, double *rel_res);
@@ -362,6 +363,21 @@ size_t ildlt_cg_doubles(int64_t n);
 hipError_t ildlt_pcg(hipStream_t s, const IldltData &L, int64_t nnz, const int64_t *ip, const int32_t *ix,
                      const double *av, const double *b, double *x, double rtol, int maxit, double *work,
                      int *iters, double *rel_res);
+// gmm::ilu_precond of GetFEM's column-major matrix (ildlt.hip): the
+// analysis of the ILDLT (qb = 1: diagonal positions, level sets) plus the
+// transpose positions, the factor and a work vector.
+struct IluData {
+  IldltData lv;
+  int64_t *d_T = nullptr;         // nnz: position of (j, i) for the entry (i, j)
+  int64_t *d_rows = nullptr;      // n: rows whose pivot gmm replaces by 1
+  double *d_F = nullptr;          // nnz
+  double *d_y = nullptr;          // n
+  int64_t pattern_nnz = -1;
+};
+hipError_t ilu_factor(hipStream_t s, IluData &U, const int64_t *ip, const int32_t *ix, const double *av,
+                      int *d_err, int *h_err, int64_t *replaced);
+hipError_t ilu_apply(hipStream_t s, const IluData &U, const int64_t *ip, const int32_t *ix, const double *r,
+                     double *z);
 }  // namespace gfx
 
 struct gfx_plan {

@@ -481,6 +481,156 @@ hipError_t ildlt_apply(hipStream_t s, const IldltData &L, const int64_t *ip, con
 
 size_t ildlt_cg_doubles(int64_t n) { return 6 * (size_t)n + 2 * kIlRedBlocks + 8; }
 
+// ------------------------------------------------------------------- ILU(0)
+// gmm::ilu_precond (src/gmm/gmm_precond_ilu.h:104-219) of GetFEM's matrix,
+// which is column-major: the preconditioner factors A^T by rows (invert =
+// true) and gmm::mult (:221-231) applies it as the lower solve with U^T, then
+// the unit upper solve with L^T (src/gmm/gmm_tri_solve.h:75-89, :45-57).
+// With a symmetric pattern, A^T's row i has A's row i's columns: F is aligned
+// with A's CSR, F at (i, j) = L_ij (j < i) or U_ij (j >= i) of A^T, and T at
+// (i, j) is the position of (j, i).  One thread per row, rows of one level
+// set at a time; every value is formed with the reference's operations in
+// its order (no fused multiply-add), so F and the solves equal the
+// sequential algorithm bit for bit.
+namespace {
+
+__global__ void k_ilu_tpos(int64_t n, const int64_t *__restrict__ ip, const int32_t *__restrict__ ix,
+                           int64_t *__restrict__ T, const double *__restrict__ av, double *__restrict__ F,
+                           int *__restrict__ err) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int64_t p = ip[i]; p < ip[i + 1]; ++p) {
+    const int32_t j = ix[p];
+    const int64_t q = lower_bound_col(ix, ip[j], ip[j + 1], (int32_t)i);
+    if (q == ip[j + 1] || ix[q] != i) {
+      atomicOr(err, 2);
+      T[p] = p;
+      F[p] = 0.0;
+      continue;
+    }
+    T[p] = q;
+    F[p] = av[q];  // A^T
+  }
+}
+
+__global__ void k_ilu_diag(int64_t n, const int64_t *__restrict__ dpos, const double *__restrict__ F,
+                           double *__restrict__ d) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = F[dpos[i]];
+}
+
+// the diagonals gmm replaces by 1 before their row is eliminated
+__global__ void k_ilu_unit(int64_t m, const int64_t *__restrict__ rows, const int64_t *__restrict__ dpos,
+                           double *__restrict__ F) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < m) F[dpos[rows[k]]] = 1.0;
+}
+
+// row i: for its lower entries k in column order, the multiplier L_ik /= U_kk,
+// then every later entry (i, c) of the row present in row k: -= L_ik U_kc
+__global__ void k_ilu_factor(int32_t m, const int32_t *__restrict__ list, const int64_t *__restrict__ ip,
+                             const int32_t *__restrict__ ix, const int64_t *__restrict__ dpos,
+                             double *__restrict__ F) {
+  const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= m) return;
+  const int64_t i = list[u], e = ip[i + 1];
+  for (int64_t p = ip[i]; p < e && ix[p] < i; ++p) {
+    const int32_t k = ix[p];
+    const double mult = F[p] / F[dpos[k]];
+    F[p] = mult;
+    int64_t r = p + 1;
+    for (int64_t q = dpos[k] + 1; q < ip[k + 1]; ++q) {
+      const int32_t c = ix[q];
+      while (r < e && ix[r] < c) ++r;
+      if (r < e && ix[r] == c) F[r] = sub_rn(F[r], mul_rn(mult, F[q]));
+    }
+  }
+}
+
+// y_c = (r_c - sum over j < c, ascending, of y_j U_jc) / U_cc
+__global__ void k_ilu_forward(int32_t m, const int32_t *__restrict__ list, const int64_t *__restrict__ ip,
+                              const int32_t *__restrict__ ix, const int64_t *__restrict__ dpos,
+                              const int64_t *__restrict__ T, const double *__restrict__ F,
+                              const double *__restrict__ r, double *__restrict__ y) {
+  const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= m) return;
+  const int64_t c = list[u];
+  double x = r[c];
+  for (int64_t p = ip[c]; p < ip[c + 1] && ix[p] < c; ++p) x = sub_rn(x, mul_rn(y[ix[p]], F[T[p]]));
+  y[c] = x / F[dpos[c]];
+}
+
+// z_i = y_i - sum over j > i, descending, of z_j L_ji (unit diagonal)
+__global__ void k_ilu_backward(int32_t m, const int32_t *__restrict__ list, const int64_t *__restrict__ ip,
+                               const int32_t *__restrict__ ix, const int64_t *__restrict__ T,
+                               const double *__restrict__ F, const double *__restrict__ y, double *__restrict__ z) {
+  const int32_t u = blockIdx.x * blockDim.x + threadIdx.x;
+  if (u >= m) return;
+  const int64_t i = list[u];
+  double x = y[i];
+  for (int64_t p = ip[i + 1] - 1; p >= ip[i] && ix[p] > i; --p) x = sub_rn(x, mul_rn(z[ix[p]], F[T[p]]));
+  z[i] = x;
+}
+
+}  // namespace
+
+hipError_t ilu_factor(hipStream_t s, IluData &U, const int64_t *ip, const int32_t *ix, const double *av,
+                      int *d_err, int *h_err, int64_t *replaced) {
+  hipError_t e;
+  *h_err = 0;
+  *replaced = 0;
+  const int64_t n = U.lv.n;
+  if ((e = hipMemsetAsync(d_err, 0, 4, s)) != hipSuccess) return e;
+  k_ilu_tpos<<<il_blocks(n, 256), 256, 0, s>>>(n, ip, ix, U.d_T, av, U.d_F, d_err);
+  // the pivots gmm replaces by 1: decided on the raw diagonal of each row in
+  // order, against the running bound max_k min(|U_kk| eps, 1) (:181-196)
+  std::vector<double> diag((size_t)n);
+  k_ilu_diag<<<il_blocks(n, 256), 256, 0, s>>>(n, U.lv.d_dpos, U.d_F, U.d_y);
+  if ((e = hipMemcpyAsync(diag.data(), U.d_y, 8 * (size_t)n, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipMemcpyAsync(h_err, d_err, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+  if (*h_err) return hipSuccess;
+  const double eps = 2.220446049250313e-16;
+  std::vector<int64_t> unit;
+  double maxp = std::fabs(diag[0]) * eps;
+  if (diag[0] == 0.0) unit.push_back(0);
+  for (int64_t i = 1; i < n; ++i) {
+    double d = diag[(size_t)i];
+    if (std::fabs(d) <= maxp) {
+      d = 1.0;
+      unit.push_back(i);
+    }
+    maxp = std::max(maxp, std::min(std::fabs(d) * eps, 1.0));
+  }
+  *replaced = (int64_t)unit.size();
+  if (!unit.empty()) {
+    if ((e = hipMemcpyAsync(U.d_rows, unit.data(), 8 * unit.size(), hipMemcpyHostToDevice, s)) != hipSuccess)
+      return e;
+    k_ilu_unit<<<il_blocks((int64_t)unit.size(), 256), 256, 0, s>>>((int64_t)unit.size(), U.d_rows, U.lv.d_dpos,
+                                                                      U.d_F);
+  }
+  for (size_t l = 0; l + 1 < U.lv.flev.size(); ++l) {
+    const int32_t m = (int32_t)(U.lv.flev[l + 1] - U.lv.flev[l]);
+    k_ilu_factor<<<il_blocks(m, 64), 64, 0, s>>>(m, U.lv.d_flist + U.lv.flev[l], ip, ix, U.lv.d_dpos, U.d_F);
+  }
+  if ((e = hipGetLastError()) != hipSuccess) return e;
+  return hipStreamSynchronize(s);  // the host vectors above stay alive until here
+}
+
+hipError_t ilu_apply(hipStream_t s, const IluData &U, const int64_t *ip, const int32_t *ix, const double *r,
+                     double *z) {
+  for (size_t l = 0; l + 1 < U.lv.flev.size(); ++l) {
+    const int32_t m = (int32_t)(U.lv.flev[l + 1] - U.lv.flev[l]);
+    k_ilu_forward<<<il_blocks(m, 64), 64, 0, s>>>(m, U.lv.d_flist + U.lv.flev[l], ip, ix, U.lv.d_dpos, U.d_T,
+                                                   U.d_F, r, U.d_y);
+  }
+  for (size_t l = 0; l + 1 < U.lv.blev.size(); ++l) {
+    const int32_t m = (int32_t)(U.lv.blev[l + 1] - U.lv.blev[l]);
+    k_ilu_backward<<<il_blocks(m, 64), 64, 0, s>>>(m, U.lv.d_blist + U.lv.blev[l], ip, ix, U.d_T, U.d_F, U.d_y, z);
+  }
+  return hipGetLastError();
+}
+
 // gmm::cg with the ILDLT preconditioner: r = b - K x, z = P r, rho = z.r,
 // p = z; while |r| > rtol |b|: (after the first step: z = P r, rho = z.r,
 // p = z + rho / rho_1 p), q = K p, a = rho / q.p, x += a p, r -= a q.

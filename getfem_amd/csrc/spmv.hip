@@ -437,23 +437,22 @@ namespace {
 constexpr int kGmBlocks = 256;  // partial sums per dot product (fixed order)
 constexpr int kGmRows = 8;      // basis vectors per block of the multi-dot
 
-// w = dinv (b - w): the preconditioned residual (w holds A x on entry)
-__global__ void k_gm_residual(int64_t n, const double *__restrict__ b, const double *__restrict__ dinv,
-                              double *__restrict__ w) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) w[i] = dinv[i] * (b[i] - w[i]);
-}
-
 // a row without diagonal (or a zero one) is left unscaled
 __global__ void k_gm_fix_dinv(int64_t n, double *__restrict__ dinv) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n && dinv[i] == 0.0) dinv[i] = 1.0;
 }
 
-// y = dinv y
-__global__ void k_gm_scale_diag(int64_t n, const double *__restrict__ dinv, double *__restrict__ y) {
+// y = dinv x
+__global__ void k_gm_scale_diag(int64_t n, const double *__restrict__ dinv, const double *x, double *y) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) y[i] *= dinv[i];
+  if (i < n) y[i] = dinv[i] * x[i];
+}
+
+// w = b - w
+__global__ void k_gm_sub(int64_t n, const double *__restrict__ b, double *__restrict__ w) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) w[i] = b[i] - w[i];
 }
 
 // partials of V_k . w for the basis vectors k0 .. k0 + kGmRows - 1 (k < nk)
@@ -559,7 +558,7 @@ size_t gmres_workspace_doubles(int64_t n, int m) {
 }
 
 // gmm::gmres (src/gmm/gmm_solver_gmres.h:85-157): restarted, left
-// preconditioned (here by the inverse diagonal), the stopping test on the
+// preconditioned (gmm::ilu_precond, or the inverse diagonal), the stopping test on the
 // preconditioned residual relative to ||M b||, a restart that makes no
 // progress ten times in a row ends the solve.  The basis is orthogonalised by
 // classical Gram-Schmidt applied twice (two batched passes per step, where
@@ -568,7 +567,7 @@ size_t gmres_workspace_doubles(int64_t n, int m) {
 // its Givens rotations and the triangular solve live on the host.
 hipError_t gmres_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *indptr, const int32_t *indices,
                        const double *vals, const double *b, double *x, double rtol, int maxit, int m,
-                       double *work, int *iters, double *rel_res) {
+                       double *work, const IluData *ilu, int *iters, double *rel_res) {
   *iters = 0;
   *rel_res = 0.0;
   if (n == 0) return hipSuccess;
@@ -587,9 +586,14 @@ hipError_t gmres_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *ind
   };
   k_inv_diag<<<nb, 256, 0, s>>>(n, indptr, indices, vals, dinv);
   k_gm_fix_dinv<<<nb, 256, 0, s>>>(n, dinv);
+  // out = M^-1 in (in == out allowed): the ILU solves or the inverse diagonal
+  auto prec = [&](const double *in, double *out) -> hipError_t {
+    if (ilu) return ilu_apply(s, *ilu, indptr, indices, in, out);
+    k_gm_scale_diag<<<nb, 256, 0, s>>>(n, dinv, in, out);
+    return hipGetLastError();
+  };
   // ||M b||
-  if ((e = hipMemcpyAsync(w, b, 8 * (size_t)n, hipMemcpyDeviceToDevice, s)) != hipSuccess) return e;
-  k_gm_scale_diag<<<nb, 256, 0, s>>>(n, dinv, w);
+  if ((e = prec(b, w)) != hipSuccess) return e;
   double bn = 0.0;
   if ((e = norm(w, bn)) != hipSuccess) return e;
   if (bn == 0.0) {
@@ -599,7 +603,8 @@ hipError_t gmres_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *ind
   auto residual = [&](double &beta) -> hipError_t {  // w = M (b - A x), beta = ||w||
     hipError_t er = launch_spmv(s, n, nnz, indptr, indices, vals, x, w, false);
     if (er != hipSuccess) return er;
-    k_gm_residual<<<nb, 256, 0, s>>>(n, b, dinv, w);
+    k_gm_sub<<<nb, 256, 0, s>>>(n, b, w);
+    if ((er = prec(w, w)) != hipSuccess) return er;
     return norm(w, beta);
   };
   double beta = 0.0;
@@ -615,7 +620,7 @@ hipError_t gmres_solve(hipStream_t s, int64_t n, int64_t nnz, const int64_t *ind
     do {
       // u = M A V_i, orthogonalised against V_0..V_i twice
       if ((e = launch_spmv(s, n, nnz, indptr, indices, vals, V + (size_t)i * n, u, false)) != hipSuccess) return e;
-      k_gm_scale_diag<<<nb, 256, 0, s>>>(n, dinv, u);
+      if ((e = prec(u, u)) != hipSuccess) return e;
       const int nk = i + 1;
       const dim3 grid(kGmBlocks, (unsigned)((nk + kGmRows - 1) / kGmRows));
       for (int pass = 0; pass < 2; ++pass) {

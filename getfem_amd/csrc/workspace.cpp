@@ -173,6 +173,9 @@ struct gfx_workspace {
   int *d_cerr = nullptr;
   int64_t cg_n = 0, con_n = 0;
   size_t gm_size = 0;
+  IluData ilu;                  // GMRES's ILU preconditioner
+  int64_t ilu_n = -1, ilu_nnz = -1;
+  bool ilu_ready = false;       // the factor belongs to the current matrix (cleared by every assembly)
 };
 
 namespace {
@@ -977,6 +980,8 @@ int sc_run(gfx_workspace *w, int order) {
 
 extern "C" {
 
+static void ilu_release(gfx_workspace *w);
+
 int gfx_ws_create(gfx_ctx *ctx, gfx_workspace **out) {
   if (!ctx || !out) return GFX_E_INVALID;
   gfx_workspace *w = new (std::nothrow) gfx_workspace();
@@ -993,6 +998,7 @@ int gfx_ws_destroy(gfx_workspace *w) {
   drop_plans(w);
   wfree(w->d_u); wfree(w->d_V);
   wfree(w->d_cg); wfree(w->d_go); wfree(w->d_fixed); wfree(w->d_cerr); wfree(w->d_gm);
+  ilu_release(w);
   sc_release(w->sc);
   delete w->sc;
   for (auto &kv : w->vars) {
@@ -1144,6 +1150,7 @@ int gfx_ws_set_assembled_matrix(gfx_workspace *w, int orientation, int64_t n, co
 // of the order-1 tree.
 int gfx_ws_assembly(gfx_workspace *w, int order) {
   if (!w) return GFX_E_INVALID;
+  if (order == 2) w->ilu_ready = false;
   if (order < 0 || order > 2) return wfail(w, GFX_E_UNSUPPORTED, "assembly order must be 0, 1 or 2");
   if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
   if (order == 0) return sc_run(w, 0);
@@ -1371,15 +1378,69 @@ int gfx_ws_solve_cg(gfx_workspace *w, const double *d_b, double *d_x, double rto
   return GFX_OK;
 }
 
+static void ilu_release(gfx_workspace *w) {
+  IluData &U = w->ilu;
+  wfree(U.lv.d_dpos); wfree(U.lv.d_fdeg); wfree(U.lv.d_bdeg); wfree(U.lv.d_flist); wfree(U.lv.d_blist);
+  wfree(U.lv.d_counter); wfree(U.d_T); wfree(U.d_rows); wfree(U.d_F); wfree(U.d_y);
+  U.lv.flev.clear(); U.lv.blev.clear();
+  w->ilu_n = w->ilu_nnz = -1;
+}
+
+// gmm::ilu_precond of the workspace's matrix: the analysis (diagonal
+// positions, level sets) is redone at every solve, since the polynomial
+// route rebuilds its pattern with every assembly
+static int ilu_prepare(gfx_workspace *w, int64_t n, int64_t nnz, const int64_t *ip, const int32_t *ix,
+                       const double *v, int64_t *replaced_out = nullptr) {
+  hipStream_t s = w->ctx->stream;
+  IluData &U = w->ilu;
+  if (w->ilu_n != n || w->ilu_nnz < nnz) {
+    WHIP(w, hipStreamSynchronize(s));
+    ilu_release(w);
+    const size_t n1 = (size_t)std::max<int64_t>(n, 1), z1 = (size_t)std::max<int64_t>(nnz, 1);
+    if (hipMalloc((void **)&U.lv.d_dpos, 8 * n1) != hipSuccess ||
+        hipMalloc((void **)&U.lv.d_fdeg, 4 * n1) != hipSuccess ||
+        hipMalloc((void **)&U.lv.d_bdeg, 4 * n1) != hipSuccess ||
+        hipMalloc((void **)&U.lv.d_flist, 4 * n1) != hipSuccess ||
+        hipMalloc((void **)&U.lv.d_blist, 4 * n1) != hipSuccess || hipMalloc((void **)&U.lv.d_counter, 4) != hipSuccess ||
+        hipMalloc((void **)&U.d_T, 8 * z1) != hipSuccess || hipMalloc((void **)&U.d_rows, 8 * n1) != hipSuccess ||
+        hipMalloc((void **)&U.d_F, 8 * z1) != hipSuccess || hipMalloc((void **)&U.d_y, 8 * n1) != hipSuccess) {
+      ilu_release(w);
+      return wfail(w, GFX_E_OOM, "ILU buffers");
+    }
+    w->ilu_n = n;
+    w->ilu_nnz = nnz;
+  }
+  // gmm's ILU keeps the first 1000 entries of a row (gmm_precond_ilu.h:147)
+  std::vector<int64_t> hp((size_t)n + 1);
+  WHIP(w, hipMemcpyAsync(hp.data(), ip, 8 * ((size_t)n + 1), hipMemcpyDeviceToHost, s));
+  WHIP(w, hipStreamSynchronize(s));
+  for (int64_t i = 0; i < n; ++i)
+    if (hp[(size_t)i + 1] - hp[(size_t)i] > 1000)
+      return wfail(w, GFX_E_UNSUPPORTED, "ILU: a row of more than 1000 entries (GetFEM truncates it)");
+  if (!w->d_cerr && hipMalloc((void **)&w->d_cerr, 4) != hipSuccess) return wfail(w, GFX_E_OOM, "flag");
+  int herr = 0;
+  WHIP(w, ildlt_analyse(s, U.lv, n, 1, ip, ix, w->d_cerr, &herr));
+  if (herr)
+    return wfail(w, GFX_E_UNSUPPORTED, (herr & 1) ? "ILU: a row has no diagonal entry"
+                                                  : "ILU: the matrix pattern is not symmetric");
+  int64_t replaced = 0;
+  WHIP(w, ilu_factor(s, U, ip, ix, v, w->d_cerr, &herr, &replaced));
+  if (herr) return wfail(w, GFX_E_UNSUPPORTED, "ILU: the matrix pattern is not symmetric");
+  if (replaced_out) *replaced_out = replaced;
+  return GFX_OK;
+}
+
 // restarted GMRES on the workspace's assembled matrix: what
 // default_linear_solver gives a model that is not coercive
 // (linear_solver_gmres_preconditioned_ilu, src/getfem/getfem_model_solvers.h:
-// 101-109, restart 500); the preconditioner here is the inverse diagonal.
-// The Krylov basis takes (restart + 1) n doubles: the restart is lowered to
-// keep it within 4 GiB.
-int gfx_ws_solve_gmres(gfx_workspace *w, const double *d_b, double *d_x, double rtol, int maxit, int restart,
-                       int *iters, double *rel_res) {
+// 101-109, restart 500), with its ILU or the inverse diagonal.  The Krylov
+// basis takes (restart + 1) n doubles: the restart is lowered to keep it
+// within 4 GiB.
+int gfx_ws_solve_gmres(gfx_workspace *w, int precond, const double *d_b, double *d_x, double rtol, int maxit,
+                       int restart, int *iters, double *rel_res) {
   if (!w || !d_b || !d_x || !iters || !rel_res || !(rtol > 0) || maxit < 0 || restart < 1) return GFX_E_INVALID;
+  if (precond != GFX_PRECOND_ILU && precond != GFX_PRECOND_DIAGONAL)
+    return wfail(w, GFX_E_INVALID, "GMRES takes GFX_PRECOND_ILU or GFX_PRECOND_DIAGONAL");
   int64_t n = 0, nnz = 0;
   const int64_t *ip = nullptr;
   const int32_t *ix = nullptr;
@@ -1397,7 +1458,40 @@ int gfx_ws_solve_gmres(gfx_workspace *w, const double *d_b, double *d_x, double 
     if (hipMalloc((void **)&w->d_gm, 8 * need) != hipSuccess) return wfail(w, GFX_E_OOM, "GMRES basis");
     w->gm_size = need;
   }
-  WHIP(w, gmres_solve(w->ctx->stream, n, nnz, ip, ix, v, d_b, d_x, rtol, maxit, m, w->d_gm, iters, rel_res));
+  if (precond == GFX_PRECOND_ILU) {
+    if ((r = ilu_prepare(w, n, nnz, ip, ix, v))) return r;
+    w->ilu_ready = true;
+  }
+  WHIP(w, gmres_solve(w->ctx->stream, n, nnz, ip, ix, v, d_b, d_x, rtol, maxit, m, w->d_gm,
+                      precond == GFX_PRECOND_ILU ? &w->ilu : nullptr, iters, rel_res));
+  return GFX_OK;
+}
+
+// GMRES's ILU on its own: factor the workspace's current matrix, then apply
+int gfx_ws_ilu_factor(gfx_workspace *w, int64_t *replaced) {
+  if (!w) return GFX_E_INVALID;
+  int64_t n = 0, nnz = 0;
+  const int64_t *ip = nullptr;
+  const int32_t *ix = nullptr;
+  double *v = nullptr;
+  int r = gfx_ws_device_csr(w, &n, &nnz, &ip, &ix, &v);
+  if (r) return r;
+  if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
+  if ((r = ilu_prepare(w, n, nnz, ip, ix, v, replaced))) return r;
+  w->ilu_ready = true;
+  return GFX_OK;
+}
+
+int gfx_ws_ilu_apply(gfx_workspace *w, const double *d_r, double *d_z) {
+  if (!w || !d_r || !d_z) return GFX_E_INVALID;
+  if (!w->ilu_ready) return wfail(w, GFX_E_STATE, "no ILU factor (gfx_ws_ilu_factor)");
+  int64_t n = 0, nnz = 0;
+  const int64_t *ip = nullptr;
+  const int32_t *ix = nullptr;
+  int r = gfx_ws_device_csr(w, &n, &nnz, &ip, &ix, nullptr);
+  if (r) return r;
+  if (hipSetDevice(w->ctx->device) != hipSuccess) return wfail(w, GFX_E_HIP, "hipSetDevice");
+  WHIP(w, ilu_apply(w->ctx->stream, w->ilu, ip, ix, d_r, d_z));
   return GFX_OK;
 }
 

@@ -234,15 +234,19 @@ class Model:
     def rhs(self):
         return self._rhs.cpu().numpy()
 
-    def _linear_solve(self, rhs, dx, rtol, maxit):
+    def _linear_solve(self, rhs, dx, rtol, maxit, precond="ilu"):
         """default_linear_solver's iterative branch
         (src/getfem/getfem_model_solvers.h:652-660): CG for a coercive model
         (block Jacobi on a single plan, the diagonal on the workspace's
-        matrix), restarted GMRES(500) otherwise (diagonal preconditioner)."""
+        matrix), restarted GMRES(500) otherwise, preconditioned by GetFEM's
+        ILU ("ilu") or the inverse diagonal ("diagonal").  ILU(0) breaks down
+        on saddle points whose multiplier or pressure rows come first (their
+        tiny pivots), as GetFEM's does; there "diagonal" converges."""
         it, res = ctypes.c_int(), ctypes.c_double()
         b, x = ctypes.c_void_p(rhs.data_ptr()), ctypes.c_void_p(dx.data_ptr())
         if not (self.is_symmetric() and self.is_coercive()):
-            check(lib().gfx_ws_solve_gmres(self.ws.h, b, x, rtol, maxit, 500, ctypes.byref(it), ctypes.byref(res)),
+            pc = _lib.PRECOND_ILU if precond == "ilu" else _lib.PRECOND_DIAGONAL
+            check(lib().gfx_ws_solve_gmres(self.ws.h, pc, b, x, rtol, maxit, 500, ctypes.byref(it), ctypes.byref(res)),
                   self.ctx.h)
         elif self._ws_route():
             check(lib().gfx_ws_solve_cg(self.ws.h, b, x, rtol, maxit, ctypes.byref(it), ctypes.byref(res)), self.ctx.h)
@@ -252,11 +256,12 @@ class Model:
         return it.value, res.value
 
     # ------------------------------------------------------------------- solve
-    def solve(self, max_iter=100, max_res=1e-6, cg_maxit=20000, noisy=False):
+    def solve(self, max_iter=100, max_res=1e-6, cg_maxit=20000, noisy=False, precond="ilu"):
         """getfem::standard_solve for this model: a linear model of one
         variable is one linear solve of K U = rhs; otherwise classical_Newton
         with basic_newton_line_search.  Linear solves: CG for symmetric
-        coercive models, GMRES otherwise.  Returns (newton iterations,
+        coercive models, GMRES otherwise (precond: "ilu" or "diagonal", see
+        _linear_solve).  Returns (newton iterations,
         converged, list of (linear solver iterations, residual))."""
         import torch
         cg_rtol = max_res / 20.0  # iter_linsolv0.set_resmax(iter.get_resmax()/20)
@@ -264,7 +269,7 @@ class Model:
         if self.is_linear() and len(self.vars) == 1:
             rhs = self.assembly("build_all")
             if not self._ws_route():
-                it, res = self._linear_solve(rhs, self.U, cg_rtol, cg_maxit)
+                it, res = self._linear_solve(rhs, self.U, cg_rtol, cg_maxit, precond)
                 log.append((it, res))
                 return 1, res <= cg_rtol, log
             # the polynomial route's rhs is the residual form: Newton below
@@ -277,7 +282,7 @@ class Model:
         while crit > max_res and k < max_iter:
             rhs = self.assembly("build_all")
             dU = torch.zeros_like(self.U)
-            it, cres = self._linear_solve(rhs, dU, cg_rtol, cg_maxit)
+            it, cres = self._linear_solve(rhs, dU, cg_rtol, cg_maxit, precond)
             log.append((it, cres))
             if cres > cg_rtol:
                 break

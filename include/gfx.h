@@ -277,6 +277,8 @@ int gfx_plan_solve_cg(gfx_plan *p, const double *d_b, double *d_x, double rtol, 
  * reference would replace by 1 ("pivot too small") returns GFX_E_UNSUPPORTED. */
 #define GFX_PRECOND_BLOCK_JACOBI 0
 #define GFX_PRECOND_ILDLT 1
+#define GFX_PRECOND_ILU 2        /* gmm::ilu_precond (GMRES) */
+#define GFX_PRECOND_DIAGONAL 3   /* the inverse diagonal (GMRES) */
 int gfx_plan_solve_pcg(gfx_plan *p, int precond, const double *d_b, double *d_x, double rtol,
                        int maxit, int *iters, double *rel_res);
 /* ILDLT pieces (for inspection and tests): factor the current values (level
@@ -424,13 +426,24 @@ int gfx_ws_solve_cg(gfx_workspace *w, const double *d_b, double *d_x, double rto
  * (src/gmm/gmm_solver_gmres.h:85-157), the solver default_linear_solver gives
  * a model that is not coercive (linear_solver_gmres_preconditioned_ilu,
  * src/getfem/getfem_model_solvers.h:101-109, :652-660; restart 500 there).
- * Left preconditioned by the inverse diagonal (GetFEM: ILU); the stopping
- * test is ||M (b - K x)|| <= rtol ||M b||; ten restarts without progress end
- * it; maxit bounds the total iterations.  The Krylov basis holds (restart +
- * 1) nb_dof doubles, the restart lowered to keep it within 4 GiB.  d_x:
- * initial guess in, solution out; iters and rel_res as gfx_plan_solve_cg. */
-int gfx_ws_solve_gmres(gfx_workspace *w, const double *d_b, double *d_x, double rtol, int maxit, int restart,
-                       int *iters, double *rel_res);
+ * Left preconditioned by GFX_PRECOND_ILU, GetFEM's gmm::ilu_precond of its
+ * column-major matrix (src/gmm/gmm_precond_ilu.h: the rows of K^T
+ * factored, bit for bit, pivots GetFEM replaces by 1 replaced alike; the
+ * pattern must be symmetric with every diagonal entry and rows of at most
+ * 1000 entries), or GFX_PRECOND_DIAGONAL, the inverse diagonal.  The
+ * stopping test is ||M (b - K x)|| <= rtol ||M b||; ten restarts without
+ * progress end it; maxit bounds the total iterations.  The Krylov basis holds
+ * (restart + 1) nb_dof doubles, the restart lowered to keep it within 4 GiB.
+ * d_x: initial guess in, solution out; iters and rel_res as
+ * gfx_plan_solve_cg. */
+int gfx_ws_solve_gmres(gfx_workspace *w, int precond, const double *d_b, double *d_x, double rtol, int maxit,
+                       int restart, int *iters, double *rel_res);
+/* The ILU of gfx_ws_solve_gmres on its own: factor the workspace's current
+ * matrix (replaced: pivots GetFEM replaces by 1, may be NULL), then z =
+ * M^-1 r on the device (stream-ordered); an order-2 assembly invalidates the
+ * factor */
+int gfx_ws_ilu_factor(gfx_workspace *w, int64_t *replaced);
+int gfx_ws_ilu_apply(gfx_workspace *w, const double *d_r, double *d_z);
 /* gfx_plan_dof_constraints on the workspace's assembled matrix (dofs are
  * global: the variables' intervals) */
 int gfx_ws_dof_constraints(gfx_workspace *w, int64_t n_fixed, const int32_t *h_dofs, const double *h_go,

@@ -61,6 +61,9 @@ def lib():
         L.orc_ildlt_apply.argtypes = [i64, P, P, P, P, P]
         L.orc_assemble_blocks.argtypes = [i64, i64, i32, P]
         L.orc_assemble_blocks.restype = P
+        L.orc_ilu.argtypes = [i64, P, P, P, P]
+        L.orc_ilu.restype = i64
+        L.orc_ilu_apply.argtypes = [i64, P, P, P, P, P]
         _lib = L
     return _lib
 
@@ -297,6 +300,27 @@ def block_element_matrices(pts, cvx, im_name, test, trial, fields, integrand2):
             Ss = slots(sk, skk, sq, sdofs[e], B, iq)
             out[e] += qw[iq] * abs(J) * (St @ D @ Ss.T)
     return out
+
+
+def ilu(indptr, indices, vals):
+    """gmm::ilu_precond of GetFEM's column-major matrix (the rows of A^T
+    factored): F aligned with A's CSR and the count of replaced pivots."""
+    ip = np.ascontiguousarray(indptr, dtype=np.int64)
+    ix = np.ascontiguousarray(indices, dtype=np.int32)
+    av = np.ascontiguousarray(vals, dtype=np.float64)
+    F = np.zeros(av.size)
+    rep = lib().orc_ilu(ip.size - 1, _ptr(ip), _ptr(ix), _ptr(av), _ptr(F))
+    return F, int(rep)
+
+
+def ilu_apply(indptr, indices, F, r):
+    ip = np.ascontiguousarray(indptr, dtype=np.int64)
+    ix = np.ascontiguousarray(indices, dtype=np.int32)
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    r = np.ascontiguousarray(r, dtype=np.float64)
+    z = np.zeros_like(r)
+    lib().orc_ilu_apply(ip.size - 1, _ptr(ip), _ptr(ix), _ptr(F), _ptr(r), _ptr(z))
+    return z
 
 
 def csc_to_csr(colptr, rows, vals, n):

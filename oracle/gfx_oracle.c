@@ -36,6 +36,10 @@
  *    orc_element_matrix's Laplacian, the closed-form P1 mass
  *    |T|/20 (1 + delta_ij), and for the P1 x P2 coupling the same values
  *    under two exact rules and the exact integral of div u of a linear u.
+ *  - orc_ilu / orc_ilu_apply: where ILU(0) drops no fill (tridiagonal,
+ *    dense) the apply is the exact solve (numpy); on any symmetric pattern
+ *    (L U)_ij = (A^T)_ij on the pattern; the apply equals the dense
+ *    triangular solves with those factors; a zero first pivot is replaced.
  *  - orc_compliance: central finite differences on J.
  *  - orc_ildlt / orc_ildlt_apply: on a pattern without dropped fill
  *    (tridiagonal, dense) the factor is the exact LDL^T (numpy Cholesky);
@@ -1160,3 +1164,103 @@ void orc_ildlt_apply(int64_t n, const int64_t *uptr, const int32_t *uind, const 
     z[i] = t;
   }
 }
+
+/* ======================================================================
+ * gmm::ilu_precond of GetFEM's column-major matrix (gmm_precond_ilu.h:
+ * 104-219 on transposed(A), invert = true) and gmm::mult (:221-231)
+ * ====================================================================== */
+
+/* B = A^T by rows (CSR of B = CSC of A), sorted columns */
+static void csr_transpose(int64_t n, const int64_t *ip, const int32_t *ix, const double *av,
+                          int64_t *bp, int32_t *bx, double *bv) {
+  for (int64_t i = 0; i <= n; ++i) bp[i] = 0;
+  for (int64_t p = 0; p < ip[n]; ++p) bp[ix[p] + 1]++;
+  for (int64_t i = 0; i < n; ++i) bp[i + 1] += bp[i];
+  int64_t *fill = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+  memcpy(fill, bp, sizeof(int64_t) * (n + 1));
+  for (int64_t i = 0; i < n; ++i)
+    for (int64_t p = ip[i]; p < ip[i + 1]; ++p) {
+      const int64_t d = fill[ix[p]]++;
+      bx[d] = (int32_t)i;
+      bv[d] = av[p];
+    }
+  free(fill);
+}
+
+int64_t orc_ilu(int64_t n, const int64_t *ip, const int32_t *ix, const double *av, double *F) {
+  const int64_t nnz = ip[n];
+  int64_t *bp = (int64_t *)malloc(sizeof(int64_t) * (n + 1));
+  int32_t *bx = (int32_t *)malloc(sizeof(int32_t) * (nnz ? nnz : 1));
+  double *bv = (double *)malloc(sizeof(double) * (nnz ? nnz : 1));
+  csr_transpose(n, ip, ix, av, bp, bx, bv);
+  /* L (strictly lower) and U (diagonal first, then the upper entries) of B */
+  int64_t *Lp = (int64_t *)calloc(n + 1, sizeof(int64_t)), *Up = (int64_t *)calloc(n + 1, sizeof(int64_t));
+  int32_t *Li = (int32_t *)malloc(sizeof(int32_t) * (nnz + 1)), *Ui = (int32_t *)malloc(sizeof(int32_t) * (nnz + n + 1));
+  double *Lv = (double *)malloc(sizeof(double) * (nnz + 1)), *Uv = (double *)malloc(sizeof(double) * (nnz + n + 1));
+  int64_t lo = 0, uo = 0;
+  for (int64_t i = 0; i < n; ++i) {
+    Ui[uo] = (int32_t)i; Uv[uo] = 0.0; ++uo;  /* the diagonal slot */
+    for (int64_t p = bp[i]; p < bp[i + 1] && p - bp[i] < 1000; ++p) {
+      const int32_t j = bx[p];
+      if (j < i) { Li[lo] = j; Lv[lo] = bv[p]; ++lo; }
+      else if (j == i) Uv[Up[i]] = bv[p];
+      else { Ui[uo] = j; Uv[uo] = bv[p]; ++uo; }
+    }
+    Lp[i + 1] = lo;
+    Up[i + 1] = uo;
+  }
+  const double prec = 2.220446049250313e-16;
+  double max_pivot = fabs(Uv[0]) * prec;
+  int64_t replaced = 0;
+  if (n > 0 && Uv[0] == 0.0) { Uv[0] = 1.0; ++replaced; }
+  for (int64_t i = 1; i < n; ++i) {
+    int64_t pn = Up[i];
+    if (fabs(Uv[pn]) <= max_pivot) { Uv[pn] = 1.0; ++replaced; }
+    { double t = fabs(Uv[pn]) * prec; if (t > 1.0) t = 1.0; if (t > max_pivot) max_pivot = t; }
+    for (int64_t j = Lp[i]; j < Lp[i + 1]; ++j) {
+      pn = Up[Li[j]];
+      const double multiplier = (Lv[j] /= Uv[pn]);
+      int64_t qn = j + 1, rn = Up[i];
+      for (pn++; pn < Up[Li[j] + 1] && Ui[pn] < i; pn++) {
+        while (qn < Lp[i + 1] && Li[qn] < Ui[pn]) qn++;
+        if (qn < Lp[i + 1] && Ui[pn] == Li[qn]) Lv[qn] -= multiplier * Uv[pn];
+      }
+      for (; pn < Up[Li[j] + 1]; pn++) {
+        while (rn < Up[i + 1] && Ui[rn] < Ui[pn]) rn++;
+        if (rn < Up[i + 1] && Ui[pn] == Ui[rn]) Uv[rn] -= multiplier * Uv[pn];
+      }
+    }
+  }
+  /* F aligned with A's CSR: the entry (i, j) of row i of B has A's pattern */
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t l = Lp[i], u = Up[i] + 1;
+    for (int64_t p = ip[i]; p < ip[i + 1]; ++p) {
+      const int32_t j = ix[p];
+      if (j < i) { F[p] = (l < Lp[i + 1] && Li[l] == j) ? Lv[l] : 0.0; ++l; }
+      else if (j == i) F[p] = Uv[Up[i]];
+      else { F[p] = (u < Up[i + 1] && Ui[u] == j) ? Uv[u] : 0.0; ++u; }
+    }
+  }
+  free(bp); free(bx); free(bv); free(Lp); free(Up); free(Li); free(Ui); free(Lv); free(Uv);
+  return replaced;
+}
+
+/* z = M^-1 r: lower_tri_solve(transposed(U), z, false) then
+ * upper_tri_solve(transposed(L), z, true) (gmm_tri_solve.h:75-89, :45-57),
+ * column-oriented on the transposes, i.e. over the rows of U and L; F as
+ * orc_ilu returns it (A's pattern, which must be symmetric) */
+void orc_ilu_apply(int64_t n, const int64_t *ip, const int32_t *ix, const double *F, const double *r, double *z) {
+  for (int64_t i = 0; i < n; ++i) z[i] = r[i];
+  for (int64_t j = 0; j < n; ++j) {  /* column j of U^T = row j of U */
+    double dj = 0.0;
+    for (int64_t p = ip[j]; p < ip[j + 1]; ++p) if (ix[p] == j) dj = F[p];
+    z[j] /= dj;
+    const double xj = z[j];
+    for (int64_t p = ip[j]; p < ip[j + 1]; ++p) if (ix[p] > j) z[ix[p]] -= xj * F[p];
+  }
+  for (int64_t j = n - 1; j >= 0; --j) {  /* column j of L^T = row j of L, unit diagonal */
+    const double xj = z[j];
+    for (int64_t p = ip[j]; p < ip[j + 1]; ++p) if (ix[p] < j) z[ix[p]] -= xj * F[p];
+  }
+}
+

@@ -130,6 +130,15 @@ int64_t orc_ildlt(int64_t n, const int64_t *ip, const int32_t *ix, const double 
 void orc_ildlt_apply(int64_t n, const int64_t *uptr, const int32_t *uind, const double *uval,
                      const double *r, double *z);
 
+/* ---- gmm::ilu_precond of GetFEM's column-major matrix: the rows of A^T
+ * factored (gmm_precond_ilu.h:104-219, invert = true), and its application
+ * gmm::mult (:221-231).  A: n x n CSR with sorted columns and a symmetric
+ * pattern.  F (aligned with A's CSR): at (i, j) of row i of A^T, L_ij
+ * (j < i, the multipliers) or U_ij (j >= i).  Returns the number of pivots
+ * replaced by 1. */
+int64_t orc_ilu(int64_t n, const int64_t *ip, const int32_t *ix, const double *av, double *F);
+void orc_ilu_apply(int64_t n, const int64_t *ip, const int32_t *ix, const double *F, const double *r, double *z);
+
 #ifdef __cplusplus
 }
 #endif
