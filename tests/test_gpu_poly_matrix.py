@@ -160,3 +160,46 @@ def test_nonlinear_scalar_tangent_and_device_values(ctx):
     ws.assembly(2)
     c = ws.assembled_matrix_csr()
     assert all(np.array_equal(x, y) for x, y in zip(a, c))
+
+
+def test_p3_vector_block_and_size_limit(ctx):
+    """The largest element matrices of the route: P3 vector x P3 vector
+    (3,600 entries, IM_TETRAHEDRON(8)) beside a P1 variable, against the
+    matrix kernels' plan of the same elasticity term; a P4 vector block
+    (11,025 entries) is refused with a message, before any launch."""
+    print("case: polynomial route P3 vector block, P4 refusal", flush=True)
+    n = 2
+    pts, cvx = meshes.regular_unit_mesh(n)
+    pts = meshes.jitter(pts, n)
+    mesh = ga.Mesh(ctx, "GT_PK(3,1)", pts, cvx)
+    mfu = ga.MeshFem(mesh, "FEM_PK(3,3)", 3)
+    mfp = ga.MeshFem(mesh, "FEM_PK(3,1)", 1)
+    mim = ga.MeshIm(mesh, "IM_TETRAHEDRON(8)")
+    nu = mfu.nbdof()
+    elast = "lambda*Div_u*Div_Test_u + 2*mu*Sym(Grad_u):Grad_Test_u"
+    ws = GaWorkspace(ctx)
+    ws.add_fem_variable("u", mfu, np.zeros(nu), first=0)
+    ws.add_fem_variable("p", mfp, np.zeros(mfp.nbdof()), first=nu)
+    ws.add_fixed_size_constant("lambda", LAM)
+    ws.add_fixed_size_constant("mu", MU)
+    ws.add_expression(elast, mim)
+    ws.add_expression("p*Test_p", mim)
+    ws.assembly(2)
+    ip, ix, v = ws.assembled_matrix_csr()
+    ws1 = GaWorkspace(ctx)
+    ws1.add_fem_variable("u", mfu, np.zeros(nu))
+    ws1.add_fixed_size_constant("lambda", LAM)
+    ws1.add_fixed_size_constant("mu", MU)
+    ws1.add_expression(elast, mim)
+    ws1.assembly(2)
+    kip, kix, kv = ws1.assembled_matrix_csr()
+    assert np.array_equal(ip[:nu + 1], kip) and np.array_equal(ix[:kip[-1]], kix)
+    assert np.abs(v[:kip[-1]] - kv).max() <= 1e-12 * np.abs(kv).max()
+    mf4 = ga.MeshFem(mesh, "FEM_PK(3,4)", 3)
+    ws2 = GaWorkspace(ctx)
+    ws2.add_fem_variable("u", mf4, np.zeros(mf4.nbdof()), first=0)
+    ws2.add_fem_variable("p", mfp, np.zeros(mfp.nbdof()), first=mf4.nbdof())
+    ws2.add_expression("Grad_u:Grad_Test_u + p*Test_p", mim)
+    with pytest.raises(ga.GfxError) as e:
+        ws2.assembly(2)
+    assert "larger than the polynomial route's element matrices" in str(e.value)
