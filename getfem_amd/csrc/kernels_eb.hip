@@ -285,12 +285,25 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (
     for (int x = t; x < len; x += kEbThreads) {
       const T v = stage[x];
       __builtin_nontemporal_store(v, dst + x);
-      if constexpr (!kb) {
-        mx = fmax(mx, (double)fabs(v));
-      } else if (fabs((double)v) > th) {  // row-major bit (r, c) = (3 i + a, 3 j + b)
-        const int X = 3 * n0 * kNl + x, row3 = X / kNd, j = X - row3 * kNd, r = row3 / 3;
-        const int bit = r * kNl + 3 * j + (row3 - 3 * r);
-        atomicOr(&skeep[bit >> 5], 1u << (bit & 31));
+      if constexpr (!kb) mx = fmax(mx, (double)fabs(v));
+    }
+    if constexpr (kb) {
+      // keep bits, row-major (r, c) = (3 i + a, 3 j + b): one thread per
+      // 32-bit word of this slab's rows reads its entries from the stage
+      // (staged at ((3 i + a) 3 + b) 27 + j); the word shared with the other
+      // slab gets its two halves from the same thread, a barrier apart
+      const int b0 = 3 * n0 * kNl, b1 = 3 * n1 * kNl;
+      const int w0 = b0 >> 5, w1 = (b1 + 31) >> 5;
+      for (int wd = w0 + t; wd < w1; wd += kEbThreads) {
+        uint32_t bits = 0u;
+        for (int k = 0; k < 32; ++k) {
+          const int bit = 32 * wd + k;
+          if (bit < b0 || bit >= b1) continue;
+          const int r = bit / kNl, c = bit - r * kNl, j = c / 3;
+          const int x = (3 * r + (c - 3 * j)) * kNd + j - b0;  // the slab starts at entry b0
+          if (fabs((double)stage[x]) > th) bits |= 1u << k;
+        }
+        skeep[wd] |= bits;
       }
     }
   }
