@@ -164,7 +164,9 @@ struct ScalarField {
 struct ScalarArgs {
   int64_t n_cv;
   const double *pts;
-  const int32_t *cvx;       // affine tetrahedra
+  const int32_t *cvx;       // n_cv * nv vertices
+  int nv;                   // 4: affine tetrahedra; 8: Q1 hexahedra
+  const double *gN;         // hexahedra: nq * nv * 3 shape gradients at the rule's points
   int nq;
   const double *qw;
   int nf;
@@ -187,12 +189,14 @@ struct ScalarArgs {
 // ga_instruction_matrix_assembly_mf_mf does, then the kept entries of every
 // block as (row, column) keys, sorted and summed in element order into a CSR.
 constexpr int kPolyMaxOut = 144;       // 16 q q2 for q, q2 <= 3
-constexpr int kPolyMaxNd = 20;         // base functions of one field (P3)
+constexpr int kPolyMaxNd = 32;         // base functions of one field (P3: 20, Q2: 27)
 constexpr int kPolyMaxEntries = 4096;  // entries of one block's element matrix
 struct PolyMatArgs {
   int64_t n_cv;
   const double *pts;
-  const int32_t *cvx;        // affine tetrahedra
+  const int32_t *cvx;        // n_cv * nv vertices
+  int nv;                    // 4: affine tetrahedra; 8: Q1 hexahedra
+  const double *gN;          // hexahedra: nq * nv * 3 shape gradients at the rule's points
   int nq;
   const double *qw;
   int nf;
@@ -221,6 +225,32 @@ struct PolyEmitArgs {
   uint64_t *keys;
   double *vals;
 };
+// K = sum_v X_v grad(N_v) at point q (hexahedra) or the tetrahedron's edge
+// matrix; J = det K and B = K^{-T} row-major; false where J = 0
+__device__ __forceinline__ bool poly_geometry(const double *pts, const int32_t *cv, int nv, const double *gNq,
+                                              double (&B)[9], double &aJ) {
+  double k[9];
+  if (!gNq) {
+    const double *p0 = pts + 3 * (int64_t)cv[0];
+    for (int r = 0; r < 3; ++r)
+      for (int c = 0; c < 3; ++c) k[r + 3 * c] = pts[3 * (int64_t)cv[c + 1] + r] - p0[r];
+  } else {
+    for (int t = 0; t < 9; ++t) k[t] = 0.0;
+    for (int v = 0; v < nv; ++v) {
+      const double *x = pts + 3 * (int64_t)cv[v];
+      for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) k[r + 3 * c] += x[r] * gNq[3 * v + c];
+    }
+  }
+  const double c0 = k[4] * k[8] - k[5] * k[7], c1 = k[5] * k[6] - k[3] * k[8], c2 = k[3] * k[7] - k[4] * k[6];
+  const double J = k[0] * c0 + k[1] * c1 + k[2] * c2;
+  if (J == 0.0) return false;
+  B[0] = c0 / J; B[1] = (k[2] * k[7] - k[1] * k[8]) / J; B[2] = (k[1] * k[5] - k[2] * k[4]) / J;
+  B[3] = c1 / J; B[4] = (k[0] * k[8] - k[2] * k[6]) / J; B[5] = (k[2] * k[3] - k[0] * k[5]) / J;
+  B[6] = c2 / J; B[7] = (k[1] * k[6] - k[0] * k[7]) / J; B[8] = (k[0] * k[4] - k[1] * k[3]) / J;
+  aJ = fabs(J);
+  return true;
+}
 hipError_t launch_poly_mat(hipStream_t s, const PolyMatArgs &a);
 hipError_t launch_poly_emit(hipStream_t s, const PolyEmitArgs &a);
 // temporary storage of the stable key sort of n entries (bytes)

@@ -50,16 +50,17 @@ __global__ __launch_bounds__(kPm) void k_poly_mat(PolyMatArgs a) {
   __shared__ double E[kPolyMaxEntries];
   const int64_t e = blockIdx.x;
   const int t = threadIdx.x;
-  // affine geometry, the same on every lane: K = [x1 - x0, x2 - x0, x3 - x0],
-  // B = K^{-T} (row-major)
-  const int32_t *cv = a.cvx + 4 * e;
-  const double *p0 = a.pts + 3 * (int64_t)cv[0];
-  double k[9];
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) k[r + 3 * c] = a.pts[3 * (int64_t)cv[c + 1] + r] - p0[r];
-  const double c0 = k[4] * k[8] - k[5] * k[7], c1 = k[5] * k[6] - k[3] * k[8], c2 = k[3] * k[7] - k[4] * k[6];
-  const double J = k[0] * c0 + k[1] * c1 + k[2] * c2;
-  if (J == 0.0) {  // uniform: every lane leaves
+  // geometry, the same on every lane: once for an affine tetrahedron, at
+  // every point for a hexahedron (a degenerate point leaves the element,
+  // uniformly)
+  const int32_t *cv = a.cvx + (int64_t)a.nv * e;
+  double B[9], aJ = 0.0;
+  bool ok = poly_geometry(a.pts, cv, a.nv, a.gN, B, aJ);
+  for (int q = 1; ok && a.gN && q < a.nq; ++q) {  // every hexahedron point checked first
+    double Bq[9], aq;
+    ok = poly_geometry(a.pts, cv, a.nv, a.gN + (int64_t)q * a.nv * 3, Bq, aq);
+  }
+  if (!ok) {
     if (t == 0) {
       atomicOr(a.err, 1);
       a.cnt[e] = 0;
@@ -67,16 +68,12 @@ __global__ __launch_bounds__(kPm) void k_poly_mat(PolyMatArgs a) {
     }
     return;
   }
-  double B[9];
-  B[0] = c0 / J; B[1] = (k[2] * k[7] - k[1] * k[8]) / J; B[2] = (k[1] * k[5] - k[2] * k[4]) / J;
-  B[3] = c1 / J; B[4] = (k[0] * k[8] - k[2] * k[6]) / J; B[5] = (k[2] * k[3] - k[0] * k[5]) / J;
-  B[6] = c2 / J; B[7] = (k[1] * k[6] - k[0] * k[7]) / J; B[8] = (k[0] * k[4] - k[1] * k[3]) / J;
-  const double aJ = fabs(J);
   const ScalarField &ft = a.f[a.vt], &fs = a.f[a.vs];
   const int qt = ft.qdim, qs = fs.qdim, nc = fs.nd * qs, ne = ft.nd * qt * nc, w4 = 4 * qs;
   for (int c = t; c < a.nconst; c += kPm) T[a.const_off + c] = a.consts[c];
   for (int x = t; x < ne; x += kPm) E[x] = 0.0;
   for (int q = 0; q < a.nq; ++q) {
+    if (a.gN && q > 0) poly_geometry(a.pts, cv, a.nv, a.gN + (int64_t)q * a.nv * 3, B, aJ);
     // symbol table: lane -> (field, component)
     int kc = t;
     for (int fi = 0; fi < a.nf; ++fi) {
@@ -96,7 +93,7 @@ __global__ __launch_bounds__(kPm) void k_poly_mat(PolyMatArgs a) {
       }
       kc -= f.qdim;
     }
-    // base functions of the two fields at the point (nd <= 20 < 32)
+    // base functions of the two fields at the point (nd <= 32)
     if (t < ft.nd) {
       const double *h = ft.gref + ((int64_t)q * ft.nd + t) * 3;
       bt[t][0] = ft.phi[q * ft.nd + t];

@@ -5,7 +5,8 @@
 // src/getfem_generic_assembly_compile_and_exec.cc:4669-4735) for integrands
 // the compile stage turned into polynomial programs.
 //
-// k_scalar_elem: one thread per element of an affine tetrahedral mesh.  At
+// k_scalar_elem: one thread per element (affine tetrahedra, or Q1 hexahedra
+// with the geometry at every point).  At
 // each integration point the thread fills its row of a symbol table in LDS
 // (values and gradients of the fem fields the program reads, then the
 // fixed-size constants), evaluates the program's monomials into its output
@@ -34,23 +35,18 @@ __global__ __launch_bounds__(kScThreads) void k_scalar_elem(ScalarArgs a) {
   if (e >= a.n_cv) return;  // no barrier below
   double *T = tab[threadIdx.x];
   double *O = oc[threadIdx.x];
-  // affine geometry: K = [x1 - x0, x2 - x0, x3 - x0], B = K^{-T} (row-major)
-  const int32_t *cv = a.cvx + 4 * e;
-  const double *p0 = a.pts + 3 * (int64_t)cv[0];
-  double k[9];
-  for (int r = 0; r < 3; ++r)
-    for (int c = 0; c < 3; ++c) k[r + 3 * c] = a.pts[3 * (int64_t)cv[c + 1] + r] - p0[r];
-  const double c0 = k[4] * k[8] - k[5] * k[7], c1 = k[5] * k[6] - k[3] * k[8], c2 = k[3] * k[7] - k[4] * k[6];
-  const double J = k[0] * c0 + k[1] * c1 + k[2] * c2;
-  if (J == 0.0) {
+  // geometry: once for an affine tetrahedron, at every point for a hexahedron
+  const int32_t *cv = a.cvx + (int64_t)a.nv * e;
+  double B[9], aJ = 0.0;
+  bool ok = poly_geometry(a.pts, cv, a.nv, a.gN, B, aJ);
+  for (int q = 1; ok && a.gN && q < a.nq; ++q) {
+    double Bq[9], aq;
+    ok = poly_geometry(a.pts, cv, a.nv, a.gN + (int64_t)q * a.nv * 3, Bq, aq);
+  }
+  if (!ok) {
     atomicOr(a.err, 1);
     return;
   }
-  double B[9];
-  B[0] = c0 / J; B[1] = (k[2] * k[7] - k[1] * k[8]) / J; B[2] = (k[1] * k[5] - k[2] * k[4]) / J;
-  B[3] = c1 / J; B[4] = (k[0] * k[8] - k[2] * k[6]) / J; B[5] = (k[2] * k[3] - k[0] * k[5]) / J;
-  B[6] = c2 / J; B[7] = (k[1] * k[6] - k[0] * k[7]) / J; B[8] = (k[0] * k[4] - k[1] * k[3]) / J;
-  const double aJ = fabs(J);
   for (int c = 0; c < a.nconst; ++c) T[a.const_off + c] = a.consts[c];
   const ScalarField &vf = a.f[a.vf];
   const int nl = a.order == 1 ? vf.nd * vf.qdim : 0;
@@ -58,6 +54,7 @@ __global__ __launch_bounds__(kScThreads) void k_scalar_elem(ScalarArgs a) {
   for (int t = 0; t < nl; ++t) el[t] = 0.0;
   double acc = 0.0;
   for (int q = 0; q < a.nq; ++q) {
+    if (a.gN && q > 0) poly_geometry(a.pts, cv, a.nv, a.gN + (int64_t)q * a.nv * 3, B, aJ);
     const double wJ = a.qw[q] * aJ;
     for (int fi = 0; fi < a.nf; ++fi) {
       const ScalarField &f = a.f[fi];

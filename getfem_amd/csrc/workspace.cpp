@@ -107,6 +107,7 @@ struct ScalarRoute {
   std::vector<std::string> consts;
   int const_off = 0, nsym = 0;
   double *d_consts = nullptr, *d_qw = nullptr;
+  double *d_gN = nullptr;       // hexahedra: the geometric transformation's gradients at the points
   ScProg p0;
   std::vector<ScVar> vars;
   double *d_elem = nullptr, *d_part = nullptr, *d_sum = nullptr, *d_V = nullptr;
@@ -482,7 +483,7 @@ void sc_release(ScalarRoute *sc) {
   for (auto &f : sc->fields) { wfree(f.d_U); wfree(f.d_phi); wfree(f.d_gref); }
   sc_free_prog(sc->p0);
   for (auto &v : sc->vars) { sc_free_prog(v.p); wfree(v.d_inc_ptr); wfree(v.d_inc); }
-  wfree(sc->d_consts); wfree(sc->d_qw); wfree(sc->d_elem); wfree(sc->d_part); wfree(sc->d_sum);
+  wfree(sc->d_consts); wfree(sc->d_qw); wfree(sc->d_gN); wfree(sc->d_elem); wfree(sc->d_part); wfree(sc->d_sum);
   wfree(sc->d_V); wfree(sc->d_err);
   for (auto &b : sc->blocks) sc_free_prog(b.p);
   wfree(sc->d_pelem); wfree(sc->d_thr); wfree(sc->d_cnt); wfree(sc->d_off); wfree(sc->d_scan);
@@ -537,7 +538,8 @@ int sc_build(gfx_workspace *w) {
   for (auto &e : w->exprs)
     if (e.mim != mim) return bail(GFX_E_UNSUPPORTED, "the order-0/1 route takes expressions on one mesh_im");
   gfx_mesh *m = mim->mesh;
-  if (m->gt != GT_TET_P1) return bail(GFX_E_UNSUPPORTED, "the order-0/1 route needs GT_PK(3,1) tetrahedra");
+  if (m->gt != GT_TET_P1 && m->gt != GT_HEX_Q1)
+    return bail(GFX_E_UNSUPPORTED, "the polynomial route needs GT_PK(3,1) or GT_QK(3,1) elements");
   sc->mim = mim;
   const std::map<std::string, GwflVar> gv = gwfl_vars(w);
   std::vector<std::string> all, deriv, deriv2;  // derived at least once (der_order >= 1), twice (2)
@@ -622,7 +624,7 @@ int sc_build(gfx_workspace *w) {
     const WsVar &v = w->vars.at(n);
     if (v.kind == CONST_FIXED || !v.mf) return bail(GFX_E_UNSUPPORTED, n + " is not a fem field");
     if (v.mf->mesh != m) return bail(GFX_E_UNSUPPORTED, n + " lives on another mesh");
-    if (v.mf->fem.gt != GT_TET_P1) return bail(GFX_E_UNSUPPORTED, n + ": not a tetrahedral element");
+    if (v.mf->fem.gt != m->gt) return bail(GFX_E_UNSUPPORTED, n + ": an element of another shape than the mesh's");
     ScField f;
     f.name = n;
     f.mf = v.mf;
@@ -732,6 +734,11 @@ int sc_build(gfx_workspace *w) {
     if (e != hipSuccess) return bail(GFX_E_HIP, std::string("incidences: ") + hipGetErrorString(e));
   }
   sc->nvec = nvec;
+  if (m->gt == GT_HEX_Q1) {
+    std::vector<double> gN((size_t)nq * m->nv * 3), Nv((size_t)m->nv);
+    for (int q = 0; q < nq; ++q) geotrans_eval(m->gt, &rule.pts[3 * (size_t)q], Nv.data(), &gN[(size_t)q * m->nv * 3]);
+    if (upload(&sc->d_gN, gN) != hipSuccess) return bail(GFX_E_OOM, "polynomial route geometry");
+  }
   if (upload(&sc->d_qw, rule.w) != hipSuccess ||
       hipMalloc((void **)&sc->d_elem, 8 * (size_t)std::max<int64_t>(elem, 1)) != hipSuccess ||
       hipMalloc((void **)&sc->d_part, 8 * (size_t)scalar_partials()) != hipSuccess ||
@@ -846,7 +853,7 @@ int sc_run2(gfx_workspace *w) {
       return wfail(w, GFX_E_OOM, "order-2 route matrix");
   }
   PolyMatArgs a{};
-  a.n_cv = m->n_cv; a.pts = m->d_pts; a.cvx = m->d_cvx;
+  a.n_cv = m->n_cv; a.pts = m->d_pts; a.cvx = m->d_cvx; a.nv = m->nv; a.gN = sc->d_gN;
   a.nq = sc->mim->rule.n(); a.qw = sc->d_qw;
   a.nf = (int)sc->fields.size();
   for (int i = 0; i < a.nf; ++i) {
@@ -929,7 +936,7 @@ int sc_run(gfx_workspace *w, int order) {
       if (!sc_zero_ok(w, v.p.prog)) return GFX_E_UNSUPPORTED;
   if ((r = sc_upload(w))) return r;
   ScalarArgs a{};
-  a.n_cv = m->n_cv; a.pts = m->d_pts; a.cvx = m->d_cvx;
+  a.n_cv = m->n_cv; a.pts = m->d_pts; a.cvx = m->d_cvx; a.nv = m->nv; a.gN = sc->d_gN;
   a.nq = sc->mim->rule.n(); a.qw = sc->d_qw;
   a.nf = (int)sc->fields.size();
   for (int i = 0; i < a.nf; ++i) {

@@ -358,7 +358,7 @@ int gfx_ws_compile(gfx_workspace *w);
  * gfx_ws_assembled_potential).  Orders 0, 1 and 2 outside the matrix kernels'
  * subset take the polynomial route: every integrand that is a polynomial in
  * the values and gradients of fem fields (variables and fem constants on
- * affine tetrahedra, P0 included), fixed-size constants, pow(x, k), Det and
+ * affine tetrahedra or Q1 hexahedra, P0 included), fixed-size constants, pow(x, k), Det and
  * Inv(Id + Grad_w) at w = 0 (the pulled-back coordinates), compiled by
  * symbolic derivation into per-point programs run by one kernel; the order-1
  * vector then holds the derivative with respect to every fem variable at its

@@ -251,7 +251,8 @@ def tet_geometry(pts, cv):
 
 def block_element_matrices(pts, cvx, im_name, test, trial, fields, integrand2):
     """Element matrices of one block (test variable x trial variable) of a
-    polynomial integrand on affine tetrahedra, accumulated over the
+    polynomial integrand on affine tetrahedra or Q1 hexahedra (cvx with 8
+    vertices: the geometry at every point), accumulated over the
     integration points as ga_exec runs the element loop
     (src/getfem_generic_assembly_compile_and_exec.cc:8750-8869) and the
     matrix instruction adds coeff * t to elem (:5074-5100).
@@ -283,12 +284,18 @@ def block_element_matrices(pts, cvx, im_name, test, trial, fields, integrand2):
         return S
 
     n_cv = cvx.shape[0]
+    hexa = cvx.shape[1] == 8  # Q1 hexahedra: the geometry at every point
+    gq = [fem_eval(FEM_QK, 1, x)[1] for x in qp] if hexa else None
     tk, tkk, tq, tdofs = test
     sk, skk, sq, sdofs = trial
     out = np.zeros((n_cv, tdofs.shape[1] * tq, sdofs.shape[1] * sq))
     for e in range(n_cv):
-        J, B = tet_geometry(pts, cvx[e])
+        if not hexa:
+            J, B = tet_geometry(pts, cvx[e])
         for iq in range(len(qw)):
+            if hexa:  # K = sum_v X_v grad(N_v) (bgeot_geometric_trans.cc:270-287)
+                K = pts[cvx[e]].T @ gq[iq]
+                J, B = np.linalg.det(K), np.linalg.inv(K).T
             vals = {}
             for name, (kind, k, q, dofs, U) in fields.items():
                 phi, gref = table(kind, k)

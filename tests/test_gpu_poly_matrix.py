@@ -203,3 +203,57 @@ def test_p3_vector_block_and_size_limit(ctx):
     with pytest.raises(ga.GfxError) as e:
         ws2.assembly(2)
     assert "larger than the polynomial route's element matrices" in str(e.value)
+
+
+def test_hexahedra_two_variables(ctx):
+    """Q1 hexahedra (the geometry at every point): u Q2 and v Q1 coupled by
+    kappa (u - v) with k u^3 on u, tangent against the oracle's blocks."""
+    print("case: polynomial route on hexahedra, u Q2 + v Q1", flush=True)
+    n, kappa = 2, 3.0
+    pts, cvx = meshes.regular_unit_mesh(n, "GT_QK(3,1)")
+    pts = meshes.jitter(pts, n)
+    mesh = ga.Mesh(ctx, "GT_QK(3,1)", pts, cvx)
+    mfu = ga.MeshFem(mesh, "FEM_QK(3,2)", 1)
+    mfv = ga.MeshFem(mesh, "FEM_QK(3,1)", 1)
+    im = "IM_GAUSS_PARALLELEPIPED(3,5)"
+    mim = ga.MeshIm(mesh, im)
+    nu, nv = mfu.nbdof(), mfv.nbdof()
+    rng = np.random.default_rng(8)
+    U = rng.uniform(-1.0, 1.0, nu)
+    ws = GaWorkspace(ctx)
+    ws.add_fem_variable("u", mfu, U, first=0)
+    ws.add_fem_variable("v", mfv, np.zeros(nv), first=nu)
+    ws.add_fixed_size_constant("kappa", kappa)
+    ws.add_fixed_size_constant("k", K3)
+    ws.add_expression("Grad_u.Grad_Test_u + Grad_v.Grad_Test_v + kappa*(u-v)*(Test_u-Test_v)"
+                      " + k*u*u*u*Test_u", mim)
+    ws.assembly(2)
+    ip, ix, v = ws.assembled_matrix_csr()
+    du, dv = mfu.basic_dof_table(), mfv.basic_dof_table()
+    fu, fv = (oracle.FEM_QK, 2, 1, du), (oracle.FEM_QK, 1, 1, dv)
+    fields = {"u": (oracle.FEM_QK, 2, 1, du, U)}
+
+    def uu(vals):
+        D = np.zeros((4, 4))
+        D[0, 0] = kappa + 3 * K3 * vals["u"][0][0] ** 2
+        D[1:, 1:] = np.eye(3)
+        return D
+
+    def vv(vals):
+        D = np.zeros((4, 4))
+        D[0, 0] = kappa
+        D[1:, 1:] = np.eye(3)
+        return D
+
+    def uv(vals):
+        D = np.zeros((4, 4))
+        D[0, 0] = -kappa
+        return D
+
+    Euu = oracle.block_element_matrices(pts, cvx, im, fu, fu, fields, uu)
+    Evv = oracle.block_element_matrices(pts, cvx, im, fv, fv, fields, vv)
+    Euv = oracle.block_element_matrices(pts, cvx, im, fu, fv, fields, uv)
+    rv = nu + dv
+    ref = oracle.assemble_blocks(cvx.shape[0], nu + nv, [(du, du, Euu), (du, rv, Euv),
+                                                          (rv, du, np.transpose(Euv, (0, 2, 1))), (rv, rv, Evv)])
+    check(ip, ix, v, ref, nu + nv)

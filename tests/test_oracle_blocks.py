@@ -10,7 +10,8 @@ single-variable assembly; a rectangular block equals a dense brute force
 equal the C oracle's Laplacian, the closed-form P1 mass matrix
 |T|/20 (1 + delta_ij), and, for the P1 x P2 coupling phi_i div(psi_j), the
 same matrices under a second integration rule (the integrand has degree 2,
-so both rules are exact) and the exact integral of div u for a linear u."""
+so both rules are exact) and the exact integral of div u for a linear u; on
+Q1 hexahedra the C oracle's Laplacian and the mass summing to the volume."""
 import numpy as np
 import pytest
 
@@ -135,3 +136,31 @@ def test_p1_p2_coupling_exact(rule):
         E2 = oracle.block_element_matrices(pts, cvx, "IM_TETRAHEDRON(2)", (oracle.FEM_PK, 1, 1, dp),
                                            (oracle.FEM_PK, 2, 3, du), {}, coupling)
         assert np.abs(E - E2).max() <= 1e-15
+
+
+def test_hexahedra_equal_c_oracle():
+    """Q1 hexahedra (the geometry at every point): the numpy Laplacian of a
+    Q2 field equals the C oracle's, whose hexahedral geometry the
+    Neo-Hookean checks pin; the Q1 mass sums to the element's volume."""
+    n = 2
+    pts, cvx = meshes.regular_unit_mesh(n, "GT_QK(3,1)")
+    pts = meshes.jitter(pts, n, seed=4)
+    dofs, ndof = oracle.enumerate_dof(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, 2, 1)
+    im = "IM_GAUSS_PARALLELEPIPED(3,5)"
+    prob = oracle.Problem(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, 2, 1, dofs, ndof, im, oracle.TERM_LAPLACE,
+                          scal_a=1.0)
+    f = (oracle.FEM_QK, 2, 1, dofs)
+    E = oracle.block_element_matrices(pts, cvx, im, f, f, {}, lap)
+    for e in range(cvx.shape[0]):
+        C = prob.element_matrix(e)
+        assert np.abs(E[e] - C).max() <= 1e-13 * np.abs(C).max()
+    d1, _ = oracle.enumerate_dof(oracle.GT_QK31, pts, cvx, oracle.FEM_QK, 1, 1)
+    f1 = (oracle.FEM_QK, 1, 1, d1)
+
+    def mass(vals):
+        D = np.zeros((4, 4))
+        D[0, 0] = 1.0
+        return D
+
+    M = oracle.block_element_matrices(pts, cvx, im, f1, f1, {}, mass)
+    assert abs(M.sum() - 1.0) <= 1e-13  # the unit cube's volume
