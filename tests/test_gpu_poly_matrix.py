@@ -257,3 +257,31 @@ def test_hexahedra_two_variables(ctx):
     ref = oracle.assemble_blocks(cvx.shape[0], nu + nv, [(du, du, Euu), (du, rv, Euv),
                                                           (rv, du, np.transpose(Euv, (0, 2, 1))), (rv, rv, Evv)])
     check(ip, ix, v, ref, nu + nv)
+
+
+def test_hexahedra_residual_is_matrix_times_state(ctx):
+    """Order 1 on Q1 hexahedra: for an expression linear in the variables
+    the residual is the tangent times the state (up to the entries the
+    threshold drops, none here)."""
+    print("case: polynomial route on hexahedra, order 1 = K U", flush=True)
+    n = 2
+    pts, cvx = meshes.regular_unit_mesh(n, "GT_QK(3,1)")
+    pts = meshes.jitter(pts, n)
+    mesh = ga.Mesh(ctx, "GT_QK(3,1)", pts, cvx)
+    mfu = ga.MeshFem(mesh, "FEM_QK(3,2)", 1)
+    mfv = ga.MeshFem(mesh, "FEM_QK(3,1)", 1)
+    mim = ga.MeshIm(mesh, "IM_GAUSS_PARALLELEPIPED(3,5)")
+    nu, nv = mfu.nbdof(), mfv.nbdof()
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal(nu + nv)
+    ws = GaWorkspace(ctx)
+    ws.add_fem_variable("u", mfu, X[:nu], first=0)
+    ws.add_fem_variable("v", mfv, X[nu:], first=nu)
+    ws.add_fixed_size_constant("kappa", 3.0)
+    ws.add_expression("Grad_u.Grad_Test_u + Grad_v.Grad_Test_v + kappa*(u-v)*(Test_u-Test_v)", mim)
+    ws.assembly(2)
+    K = ws.assembled_matrix()
+    ws.assembly(1)
+    V = ws.assembled_vector()
+    KX = K @ X  # rows: test dofs, columns: trial dofs
+    assert np.linalg.norm(V - KX) <= 1e-12 * np.linalg.norm(KX)
