@@ -172,20 +172,28 @@ class GaWorkspace:
         self.h = None
 
 
-def asm_generic(mim, order, expression, region, *args, ctx=None):
+def asm_generic(mim, order, expression, region, *args, ctx=None, select_output=None):
     """gf.asm('generic', mim, order, expression, region, [name, is_variable,
-    [mf,] value]...).  region must be -1 (all elements)."""
+    [mf,] value]..., ['select_output', var1[, var2]]) (interface/src/gf_asm.cc:
+    421-548): the variables take consecutive intervals in argument order
+    (:486-488); select_output=(var1,) or (var1, var2) returns the part of
+    the vector or matrix at those variables' intervals (:519-541).  region
+    must be -1 (all elements)."""
     if region not in (-1, None):
         raise ValueError("only region -1 (all elements) is supported")
     ctx = ctx or mim.mesh.ctx
     ws = GaWorkspace(ctx)
+    intervals = {}
+    nbdof = 0
     i = 0
     while i < len(args):
         name, is_var = args[i], args[i + 1]
         if i + 2 < len(args) and isinstance(args[i + 2], MeshFem):
             mf, val = args[i + 2], args[i + 3]
             if is_var:
-                ws.add_fem_variable(name, mf, val)
+                ws.add_fem_variable(name, mf, val, first=nbdof)
+                intervals[name] = (nbdof, nbdof + mf.nbdof())
+                nbdof += mf.nbdof()
             else:
                 ws.add_fem_constant(name, mf, val)
             i += 4
@@ -196,10 +204,16 @@ def asm_generic(mim, order, expression, region, *args, ctx=None):
             i += 3
     ws.add_expression(expression, mim, 2)
     ws.assembly(order)
+    sel = [intervals[v] for v in (select_output or ())]
     if order == 2:
-        return ws.assembled_matrix()
+        K = ws.assembled_matrix()
+        if sel:
+            (a, b), (c, d) = sel[0], sel[-1]
+            K = K[a:b, c:d]
+        return K
     if order == 1:
-        return ws.assembled_vector()
+        V = ws.assembled_vector()
+        return V[sel[0][0]:sel[0][1]] if sel else V
     if order == 0:
         return ws.assembled_potential()
     raise ValueError("order must be 0, 1 or 2")

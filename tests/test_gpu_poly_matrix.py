@@ -285,3 +285,26 @@ def test_hexahedra_residual_is_matrix_times_state(ctx):
     V = ws.assembled_vector()
     KX = K @ X  # rows: test dofs, columns: trial dofs
     assert np.linalg.norm(V - KX) <= 1e-12 * np.linalg.norm(KX)
+
+
+def test_asm_generic_two_variables_select_output(ctx):
+    """gf.asm('generic') with two variables: consecutive intervals in argument
+    order, select_output the (u, p) block and u's part of the vector."""
+    from getfem_amd.workspace import asm_generic
+    print("case: asm_generic, two variables, select_output", flush=True)
+    pts, cvx, mesh, mf_u, mf_p, mf_r, mim = setup(ctx, 2)
+    nu, npp = mf_u.nbdof(), mf_p.nbdof()
+    rng = np.random.default_rng(6)
+    U, P = rng.standard_normal(nu), rng.standard_normal(npp)
+    expr = "2*Sym(Grad_u):Grad_Test_u - p*Div_Test_u - Test_p*Div_u"
+    args = ("u", 1, mf_u, U, "p", 1, mf_p, P)
+    K = asm_generic(mim, 2, expr, -1, *args)
+    Kup = asm_generic(mim, 2, expr, -1, *args, select_output=("u", "p"))
+    assert K.shape == (nu + npp, nu + npp) and Kup.shape == (nu, npp)
+    assert (abs(K[:nu, nu:] - Kup)).max() == 0.0
+    assert abs(K[:nu, nu:] - K[nu:, :nu].T).max() <= 1e-14 * abs(K[:nu, nu:]).max()
+    V = asm_generic(mim, 1, expr, -1, *args)
+    Vu = asm_generic(mim, 1, expr, -1, *args, select_output=("u",))
+    assert np.array_equal(V[:nu], Vu)
+    X = np.concatenate([U, P])
+    assert np.linalg.norm(V - K @ X) <= 1e-12 * np.linalg.norm(K @ X)
