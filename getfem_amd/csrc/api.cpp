@@ -412,6 +412,7 @@ int gfx_plan_destroy(gfx_plan *p) {
   dfree(p->d_gref32); dfree(p->d_qw32); dfree(p->d_R32); dfree(p->d_rec32); dfree(p->d_Rq); dfree(p->d_E32);
   dfree(p->d_E64);
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order); dfree(p->d_desc);
+  dfree(p->d_node_ok);
   dfree(p->d_nbr); dfree(p->d_mask);
   dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals); dfree(p->d_err);
   dfree(p->d_gN); dfree(p->d_E); dfree(p->d_res); dfree(p->d_U); dfree(p->d_rhs); dfree(p->d_cg);
@@ -805,6 +806,7 @@ int gfx_plan_build_pattern(gfx_plan *p, int64_t *nnz_out) {
   if (!p->eb && (r = plan_records(p, false))) return r;  // element-batched: with the keep bits below
   // incidences
   dfree(p->d_inc_ptr); dfree(p->d_inc); dfree(p->d_slot); dfree(p->d_nslots); dfree(p->d_order); dfree(p->d_desc);
+  dfree(p->d_node_ok);
   dfree(p->d_nbr); dfree(p->d_mask); dfree(p->d_indptr); dfree(p->d_indices); dfree(p->d_vals);
   dfree(p->d_keep);
   p->nnz = -1;
@@ -933,6 +935,8 @@ static int plan_validate(gfx_plan *p, bool *eb_fresh) {
   a.gref = p->d_gref; a.qw = p->d_qw; a.nq = p->nq; a.thr = p->d_thr; a.keep = p->d_keep;
   a.inc_ptr = p->d_inc_ptr; a.inc = p->d_inc; a.slot = p->d_slot; a.nslots = p->d_nslots;
   a.mask = p->d_mask; a.err = p->d_err;
+  if (!p->d_node_ok) HIPCHK(c, dalloc(&p->d_node_ok, (size_t)std::max<int64_t>(p->n_nodes, 1)));
+  a.node_ok = p->d_node_ok;
   if (p->eb) {
     if ((r = plan_eb_elements(p, GFX_PREC_FP64, true))) return r;
   } else {

@@ -121,6 +121,7 @@ struct PatternArgs {
   uint16_t *nslots;
   int32_t *nbr;
   uint32_t *mask;
+  uint8_t *node_ok;         // pattern check: nodes confirmed full by their keep bits
   int64_t *rowcnt;          // per dof
   int64_t *indptr;
   int32_t *indices;
@@ -445,6 +446,7 @@ struct gfx_plan {
   int32_t *d_inc_ptr = nullptr;
   uint32_t *d_inc = nullptr;
   uint16_t *d_slot = nullptr, *d_nslots = nullptr;
+  uint8_t *d_node_ok = nullptr;   // pattern check scratch (n_nodes)
   int32_t *d_nbr = nullptr;
   uint32_t *d_mask = nullptr;
   int max_nslots = 0, max_cand = 0, max_inc = 0;

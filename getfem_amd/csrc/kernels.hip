@@ -407,39 +407,40 @@ __global__ __launch_bounds__(256) void k_patch_build(PatternArgs a, int pmax) {
 // with the stored ones.  Any difference sets bit 4 of the error flag: the
 // stored pattern is no longer GetFEM's and must be rebuilt.  Dynamic LDS:
 // mwords u32 (the largest node's presence words).
+// One thread per node: a node stored as full whose incidence rows have every
+// keep bit set has every presence bit set again (the converse needs the exact
+// check of k_mask_check, which the other nodes take).
+__global__ __launch_bounds__(256) void k_mask_fast(PatternArgs a) {
+  const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= a.n_nodes) return;
+  const int Q = a.qdim, nl = a.nloc;
+  bool ok = (a.nslots[node] & kFullRows) != 0;
+  for (int k = a.inc_ptr[node]; ok && k < a.inc_ptr[node + 1]; ++k) {
+    const uint32_t inc = a.inc[k];
+    const uint32_t *kw = a.keep + (int64_t)(inc >> 5) * a.kwords;
+    const int i = inc & 31;
+    for (int x = 0; ok && x < Q; ++x) {
+      const int b0 = (i * Q + x) * nl, b1 = b0 + nl;  // the row's bits [b0, b1)
+      for (int b = b0; ok && b < b1;) {
+        const int w = b >> 5, lo = b & 31, hi = min(32, b1 - (w << 5));
+        const uint32_t m = (hi - lo == 32 ? 0xffffffffu : ((1u << (hi - lo)) - 1u)) << lo;
+        ok = (kw[w] & m) == m;
+        b = (w << 5) + hi;
+      }
+    }
+  }
+  a.node_ok[node] = ok ? 1 : 0;
+}
+
 __global__ __launch_bounds__(128) void k_mask_check(PatternArgs a) {
   extern __shared__ uint32_t maskC[];
   const int64_t node = blockIdx.x;
   if (node >= a.n_nodes) return;
   const int nd = a.nd, Q = a.qdim, nl = a.nloc;
   const int i0 = a.inc_ptr[node], ninc = a.inc_ptr[node + 1] - i0;
+  if (a.node_ok[node]) return;  // confirmed by k_mask_fast (uniform)
   const int ns = a.nslots[node] & (kFullRows - 1);
   const int mw = (Q * Q * ns + 31) / 32;
-  if (a.nslots[node] & kFullRows) {
-    // a node stored as full: every keep bit of its rows set means every
-    // presence bit is set again (the converse needs the exact check below)
-    __shared__ int allset;
-    if (threadIdx.x == 0) allset = 1;
-    __syncthreads();
-    for (int t = threadIdx.x; t < ninc * Q; t += blockDim.x) {
-      const int k = t / Q, x = t - k * Q;
-      const uint32_t inc = a.inc[i0 + k];
-      const int64_t e = inc >> 5;
-      const int i = inc & 31;
-      const uint32_t *kw = a.keep + e * a.kwords;
-      const int b0 = (i * Q + x) * nl, b1 = b0 + nl;  // the row's bits [b0, b1)
-      bool ok = true;
-      for (int b = b0; b < b1;) {
-        const int w = b >> 5, lo = b & 31, hi = min(32, b1 - (w << 5));
-        const uint32_t m = (hi - lo == 32 ? 0xffffffffu : ((1u << (hi - lo)) - 1u)) << lo;
-        ok = ok && (kw[w] & m) == m;
-        b = (w << 5) + hi;
-      }
-      if (!ok) allset = 0;
-    }
-    __syncthreads();
-    if (allset) return;  // uniform
-  }
   for (int t = threadIdx.x; t < mw; t += blockDim.x) maskC[t] = 0u;
   __syncthreads();
   const int per_inc = Q * nd * Q;
@@ -899,6 +900,7 @@ hipError_t launch_mask_check(hipStream_t s, const PatternArgs &a, int max_nslots
   const size_t mwords = ((size_t)a.qdim * a.qdim * max_nslots + 31) / 32;
   const size_t lds = 4 * (mwords > 0 ? mwords : 1);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
+  k_mask_fast<<<nblocks(a.n_nodes, 256), 256, 0, s>>>(a);
   k_mask_check<<<(unsigned)a.n_nodes, 128, lds, s>>>(a);
   return hipGetLastError();
 }
