@@ -943,10 +943,17 @@ static int plan_validate(gfx_plan *p, bool *eb_fresh) {
     if ((r = plan_records(p, false))) return r;
     HIPCHK(c, launch_keep(s, p->term, a));
   }
-  HIPCHK(c, launch_mask_check(s, a, p->max_nslots));
+  // nodes whose rows keep every entry are confirmed by one pass; the exact
+  // check runs only when some node is not
+  HIPCHK(c, launch_mask_fast(s, a));
   int flag = 0;
   HIPCHK(c, hipMemcpyAsync(&flag, p->d_err, 4, hipMemcpyDeviceToHost, s));
   HIPCHK(c, hipStreamSynchronize(s));
+  if (flag & 16) {
+    HIPCHK(c, launch_mask_exact(s, a, p->max_nslots));
+    HIPCHK(c, hipMemcpyAsync(&flag, p->d_err, 4, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+  }
   if (flag & 1) return fail(c, GFX_E_DEGENERATE_ELEMENT, "Non invertible matrix (degenerate element)");
   if (flag & 4) {
     if (p->pinned)

@@ -285,7 +285,11 @@ hipError_t launch_incidences(hipStream_t s, int64_t n_cv, int nd, int qdim,
 hipError_t launch_keep(hipStream_t s, int term, const PatternArgs &a);
 hipError_t launch_patch_build(hipStream_t s, const PatternArgs &a, int max_cand);
 hipError_t launch_fill_columns(hipStream_t s, const PatternArgs &a);
-hipError_t launch_mask_check(hipStream_t s, const PatternArgs &a, int max_nslots);
+// the pattern check against new keep bits: the thread-per-node pass (sets bit
+// 16 of the error flag when some node needs the exact check), then the exact
+// check (bit 4: the pattern changed)
+hipError_t launch_mask_fast(hipStream_t s, const PatternArgs &a);
+hipError_t launch_mask_exact(hipStream_t s, const PatternArgs &a, int max_nslots);
 hipError_t launch_fill_desc(hipStream_t s, int64_t n, int qdim, const int32_t *order, const int32_t *inc_ptr,
                             const uint16_t *nslots, const int64_t *indptr, PatchDesc *desc);
 hipError_t launch_assemble_patch(hipStream_t s, int term, int form, int prec, const PatchArgs &a);

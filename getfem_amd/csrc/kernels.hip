@@ -430,6 +430,7 @@ __global__ __launch_bounds__(256) void k_mask_fast(PatternArgs a) {
     }
   }
   a.node_ok[node] = ok ? 1 : 0;
+  if (!ok) atomicOr(a.err, 16);  // some node needs the exact check
 }
 
 __global__ __launch_bounds__(128) void k_mask_check(PatternArgs a) {
@@ -895,12 +896,17 @@ hipError_t launch_patch_build(hipStream_t s, const PatternArgs &a, int max_cand)
   return hipGetLastError();
 }
 
-hipError_t launch_mask_check(hipStream_t s, const PatternArgs &a, int max_nslots) {
+hipError_t launch_mask_fast(hipStream_t s, const PatternArgs &a) {
+  if (a.n_nodes == 0) return hipSuccess;
+  k_mask_fast<<<nblocks(a.n_nodes, 256), 256, 0, s>>>(a);
+  return hipGetLastError();
+}
+
+hipError_t launch_mask_exact(hipStream_t s, const PatternArgs &a, int max_nslots) {
   if (a.n_nodes == 0) return hipSuccess;
   const size_t mwords = ((size_t)a.qdim * a.qdim * max_nslots + 31) / 32;
   const size_t lds = 4 * (mwords > 0 ? mwords : 1);
   if (lds > 64 * 1024) return hipErrorInvalidValue;
-  k_mask_fast<<<nblocks(a.n_nodes, 256), 256, 0, s>>>(a);
   k_mask_check<<<(unsigned)a.n_nodes, 128, lds, s>>>(a);
   return hipGetLastError();
 }
