@@ -152,9 +152,9 @@ struct gfx_workspace {
   int32_t *d_ix = nullptr;
   double *d_v = nullptr;
   int64_t nnz = -1;
-  // the order-0 / order-1 route (scalar.hip): potentials and the derivatives
-  // the matrix kernels do not cover (data fields, P0 variables, several
-  // variables, pulled-back coordinates)
+  // the polynomial route (scalar.hip, polymat.hip): potentials, derivatives
+  // and tangents the matrix kernels do not cover (data fields, P0 or several
+  // variables, blocks between mesh_fems, pulled-back coordinates, hexahedra)
   uint64_t version = 0;         // registrations and expressions; the route rebuilds on change
   bool mat_fail = false;        // the order-2 compile failed at this version
   ScalarRoute *sc = nullptr;
@@ -472,7 +472,7 @@ int setup(gfx_workspace *w) {
   return GFX_OK;
 }
 
-// ------------------------------------------------ the order-0 / order-1 route
+// ------------------------------------------------ the polynomial route (orders 0, 1, 2)
 void sc_free_prog(ScProg &p) {
   wfree(p.d_coef); wfree(p.d_off); wfree(p.d_sym); wfree(p.d_out); wfree(p.d_mptr);
   p.nm = 0;
@@ -536,7 +536,7 @@ int sc_build(gfx_workspace *w) {
   if (w->exprs.empty()) return bail(GFX_E_STATE, "no expression");
   gfx_mesh_im *mim = w->exprs[0].mim;
   for (auto &e : w->exprs)
-    if (e.mim != mim) return bail(GFX_E_UNSUPPORTED, "the order-0/1 route takes expressions on one mesh_im");
+    if (e.mim != mim) return bail(GFX_E_UNSUPPORTED, "the polynomial route takes expressions on one mesh_im");
   gfx_mesh *m = mim->mesh;
   if (m->gt != GT_TET_P1 && m->gt != GT_HEX_Q1)
     return bail(GFX_E_UNSUPPORTED, "the polynomial route needs GT_PK(3,1) or GT_QK(3,1) elements");
@@ -616,7 +616,7 @@ int sc_build(gfx_workspace *w) {
   }
   if (fnames.empty()) need_field(sc->vars.empty() ? w->vars.begin()->first : sc->vars[0].name);
   if ((int)fnames.size() > kScalarMaxFields)
-    return bail(GFX_E_UNSUPPORTED, "the order-0/1 route reads at most 4 fem fields");
+    return bail(GFX_E_UNSUPPORTED, "the polynomial route reads at most 4 fem fields");
   int off = 0;
   const IntegrationRule &rule = mim->rule;
   const int nq = rule.n();
@@ -637,7 +637,7 @@ int sc_build(gfx_workspace *w) {
     if (upload(&f.d_phi, phi) != hipSuccess || upload(&f.d_gref, gref) != hipSuccess ||
         hipMalloc((void **)&f.d_U, 8 * (size_t)std::max<int64_t>(v.mf->ndof, 1)) != hipSuccess) {
       sc->fields.push_back(f);
-      return bail(GFX_E_OOM, "order-0/1 route tables");
+      return bail(GFX_E_OOM, "polynomial route tables");
     }
     sc->fields.push_back(f);
   }
@@ -647,7 +647,7 @@ int sc_build(gfx_workspace *w) {
     ++off;
   }
   sc->nsym = off;
-  if (off > kScalarMaxSyms) return bail(GFX_E_UNSUPPORTED, "too many symbols for the order-0/1 route");
+  if (off > kScalarMaxSyms) return bail(GFX_E_UNSUPPORTED, "too many symbols for the polynomial route");
   auto field_index = [&](const std::string &n) {
     return (int)(std::find(fnames.begin(), fnames.end(), n) - fnames.begin());
   };
@@ -682,7 +682,7 @@ int sc_build(gfx_workspace *w) {
     p.nm = (int)coef.size();
     if (upload(&p.d_coef, coef) != hipSuccess || upload(&p.d_off, offs) != hipSuccess ||
         upload(&p.d_sym, sym) != hipSuccess || upload(&p.d_out, out) != hipSuccess)
-      return bail(GFX_E_OOM, "order-0/1 route programs");
+      return bail(GFX_E_OOM, "polynomial route programs");
     return GFX_OK;
   };
   int r;
@@ -726,7 +726,7 @@ int sc_build(gfx_workspace *w) {
         hipMalloc((void **)&v.d_inc, 4 * (size_t)std::max<int64_t>(m->n_cv * nd, 1)) != hipSuccess ||
         hipMalloc((void **)&tmp, 4 * (size_t)(2 * nn + scan_tmp_elems(nn))) != hipSuccess) {
       wfree(tmp);
-      return bail(GFX_E_OOM, "order-0/1 route incidences");
+      return bail(GFX_E_OOM, "polynomial route incidences");
     }
     hipError_t e = launch_incidences(s, m->n_cv, nd, Q, wv.mf->d_dofs, nn, v.d_inc_ptr, v.d_inc, tmp);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -746,7 +746,7 @@ int sc_build(gfx_workspace *w) {
       hipMalloc((void **)&sc->d_V, 8 * (size_t)std::max<int64_t>(nvec, 1)) != hipSuccess ||
       hipMalloc((void **)&sc->d_err, 4) != hipSuccess ||
       hipMalloc((void **)&sc->d_consts, 8 * std::max<size_t>(sc->consts.size(), 1)) != hipSuccess)
-    return bail(GFX_E_OOM, "order-0/1 route buffers");
+    return bail(GFX_E_OOM, "polynomial route buffers");
   return GFX_OK;
 }
 
