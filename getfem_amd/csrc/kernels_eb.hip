@@ -294,16 +294,20 @@ __device__ __forceinline__ void nh_phase_b_mfma(int t, int64_t e, const double (
       // slab gets its two halves from the same thread, a barrier apart
       const int b0 = 3 * n0 * kNl, b1 = 3 * n1 * kNl;
       const int w0 = b0 >> 5, w1 = (b1 + 31) >> 5;
-      for (int wd = w0 + t; wd < w1; wd += kEbThreads) {
+      // two lanes per word (16 bits each, lanes 2m and 2m + 1: both in the
+      // loop, since the bound is even), combined by a lane exchange
+      for (int id = t; id < 2 * (w1 - w0); id += kEbThreads) {
+        const int wd = w0 + (id >> 1), k0 = 16 * (id & 1);
         uint32_t bits = 0u;
-        for (int k = 0; k < 32; ++k) {
+        for (int k = k0; k < k0 + 16; ++k) {
           const int bit = 32 * wd + k;
           if (bit < b0 || bit >= b1) continue;
           const int r = bit / kNl, c = bit - r * kNl, j = c / 3;
           const int x = (3 * r + (c - 3 * j)) * kNd + j - b0;  // the slab starts at entry b0
           if (fabs((double)stage[x]) > th) bits |= 1u << k;
         }
-        skeep[wd] |= bits;
+        bits |= __shfl_xor(bits, 1);
+        if ((id & 1) == 0) skeep[wd] |= bits;
       }
     }
   }
